@@ -1,0 +1,119 @@
+/*
+ * sunhip.h -- C ABI of the MI355X-native DQMC sweep (libsunhip.so).
+ *
+ * Drop-in boundary for the reference's sweep routines (YuanyueChen/sunhubbardmy):
+ * a Fortran control layer keeps dqmc.in, the driver and the measurement output, and calls
+ * these entry points through ISO_C_BINDING in place of the module procedures of
+ * src/dqmc_ft_core.f90.  Plain pointers and sizes only; every host buffer is owned by the
+ * caller and read or written only during the call.  Complex numbers are complex(dp) /
+ * double _Complex: two doubles (re, im); matrices are column-major.
+ *
+ * Return codes: 0 = success; > 0 = numerical breakdown (zero pivot in the LU of
+ * green_equaltime, index + 1); < 0 = usage or runtime error (SH_E_*).  The library never
+ * aborts; sh_last_error() gives a message.
+ */
+#ifndef SUNHIP_H
+#define SUNHIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SH_E_ARG (-1)      /* bad argument */
+#define SH_E_HIP (-2)      /* HIP runtime failure (allocation, launch) */
+#define SH_E_STATE (-3)    /* call out of order (e.g. sweep before sweep_start) */
+#define SH_E_NONFINITE (-4) /* non-finite Green's function detected */
+
+#define SH_K_CHECKERBOARD 0 /* BREAKUP_T: 2x2 bond exponentials, nn_nf families */
+#define SH_K_DENSE 1        /* dense exp(-dtau K/2) (also the FFT build's operator) */
+
+#define SH_U_DELAY 0        /* dqmc_update_delay.f90 (nublock = 1 is the fast update) */
+#define SH_U_SUBMATRIX 1    /* dqmc_update_submatrix.f90 */
+
+/*
+ * One chain's model tables: what model_para%make_tables, h0c%set_h0conf and
+ * u0conf%set_conf leave in the reference's module state
+ * (src/model_para.f90:421-453, src/dqmc_config_h0/dqmc_set_h0conf.f90,
+ *  src/dqmc_config_u/dqmc_set_conf.f90:122-175).  All index tables are 0-based.
+ */
+typedef struct sh_model {
+    int ndim;               /* matrix dimension N (= nsites) */
+    int lq;                 /* number of unit cells (normalises S_stag, density) */
+    int ltrot, nwrap, nst;  /* slices, stabilization interval, number of stabilization points */
+    const int32_t *iwrap_nt;   /* [ltrot+1] iwrap_nt(0:ltrot) */
+    const int32_t *wrap_step;  /* [2*nst] wrap_step(1:2, 1:nst), 1-based slice numbers */
+    int lwrapT, lwrapu;     /* kinetic / Hubbard terms present (rt != 0, rhub != 0) */
+    int kmode;              /* SH_K_* */
+    int nn_nf, nn_lf;       /* bond families, bonds per family */
+    const int32_t *bonds;   /* [nn_nf*nn_lf*2] (A site, B site) per bond, family-major */
+    const double *urt;      /* checkerboard: [nn_nf*nn_lf] 2x2 complex col-major; dense: N*N complex */
+    const double *urtm1;    /* inverse propagator, same layout */
+    int lcomp, nflr;
+    const double *bmat;        /* [lcomp] complex  exp(i alpha eta_l) */
+    const double *bmat_inv;    /* [lcomp] complex */
+    const double *phase_arg;   /* [lcomp] real: arg of u0conf%phase(l) (set_phase product) */
+    const double *delta_bmat;  /* [(lcomp-1)*lcomp] complex, (iflip, is) column-major */
+    const double *phase_ratio; /* [(lcomp-1)*lcomp] complex */
+    const double *gaml;        /* [lcomp] */
+    int phase_from_conf;    /* 1 when set_phase multiplies u0conf%phase (rhub > 0) */
+    const double *stag;     /* [ndim] +-1 staggering used for S_stag (S(pi,pi) on the square lattice) */
+    int update;             /* SH_U_* */
+    int nublock;            /* delay depth / submatrix block */
+    int32_t seed;           /* SFMT19937 seed of this chain (spring_sfmt_init) */
+} sh_model;
+
+typedef struct sh_ctx sh_ctx;
+
+/* allocate_core for a batch of independent chains (uneven ndim allowed; all chains share
+ * ltrot, nwrap and nst).  Replaces allocate_core (src/dqmc_ft_core.f90:22-50) per MPI rank. */
+int sh_create(sh_ctx **ctx, int nchains, const sh_model *models, int device);
+/* deallocate_core (src/dqmc_ft_core.f90:52-58) */
+int sh_destroy(sh_ctx *ctx);
+const char *sh_last_error(const sh_ctx *ctx);
+
+/* u0conf%input_conf, fresh start: each chain draws conf(i,nt) = ceiling(u*lcomp) from its own
+ * SFMT stream, nt outer, i inner (src/dqmc_config_u/dqmc_input_conf.f90:55-63) */
+int sh_init_conf(sh_ctx *ctx);
+/* conf(1:nsites, 1:ltrot) as int32 [ltrot][ndim], values 1..lcomp (conf_u.in restart path) */
+int sh_set_conf(sh_ctx *ctx, int chain, const int32_t *conf);
+int sh_get_conf(sh_ctx *ctx, int chain, int32_t *conf);
+
+/* dqmc_ft_sweep_start_0b (src/dqmc_ft_core/dqmc_ft_sweep_start_0b.f90:1-71) */
+int sh_sweep_start_0b(sh_ctx *ctx);
+/* dqmc_ft_sweep_b0(lupdate, lmeasure_equaltime) (src/dqmc_ft_core/dqmc_ft_sweep_b0.f90:1-216) */
+int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure);
+/* dqmc_ft_sweep_0b(lupdate, lmeasure_equaltime, lmeasure_dyn)
+ * (src/dqmc_ft_core/dqmc_ft_sweep_0b.f90:1-288); lmeasure_dyn must be 0 */
+int sh_sweep_0b(sh_ctx *ctx, int lupdate, int lmeasure, int lmeasure_dyn);
+/* push_stage / pop_stage (src/dqmc_ft_core.f90:290-312) */
+int sh_push_stage(sh_ctx *ctx);
+int sh_pop_stage(sh_ctx *ctx);
+
+/* gf%blk1 of one chain, N*N complex column-major */
+int sh_get_g(sh_ctx *ctx, int chain, double *g);
+/* diag[10]: max_wrap_error, max_phasediff, phase(re,im), logweightf(re,im),
+ * accepted, tried (main_obs(3)), number of wrap checks, sum log10 wrap error */
+int sh_get_diag(sh_ctx *ctx, int chain, double *diag);
+int sh_reset_diag(sh_ctx *ctx);
+/* obs[11]: nobs, then complex bin sums of sign, density/lq (energy_bin(1:2)), zkint
+ * (energy_bin(3)), zint_u (energy_bin(4)), S_stag (zspsm Fourier component at the
+ * staggering wave vector), all weighted by zphi as in equaltime_measure */
+int sh_get_obs(sh_ctx *ctx, int chain, double *obs);
+int sh_reset_obs(sh_ctx *ctx);
+
+/* Record G after every stabilization (nwrap) of chain `chain`, up to cap snapshots; buf is
+ * device-owned and read back with sh_get_trace (N*N complex per snapshot). cap = 0 stops. */
+int sh_set_trace(sh_ctx *ctx, int chain, int cap);
+int sh_get_trace(sh_ctx *ctx, int chain, double *buf, int *count);
+
+/* per-kernel-class device time in ms since the last reset (hipEvent pairs around classes):
+ * t[0] wrap (K and V), t[1] update, t[2] stabilization B-products, t[3] QR, t[4] green
+ * (GEMM + LU + solve), t[5] measure, t[6] other; enable = 0 turns the events off */
+int sh_timers(sh_ctx *ctx, int enable, double *t);
+int sh_synchronize(sh_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

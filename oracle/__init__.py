@@ -1,0 +1,216 @@
+"""ctypes front-end of the CPU oracle (oracle/dqmc_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` as the
+checker, and by ``bench.py``'s ``cpu_baseline`` leg as the timed CPU restatement.  The
+product package ``sunhubbardmy_amd`` never imports it.
+
+Parity status of each oracle function (what pins it) is listed in ``tests/test_oracle.py``.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-B", "-C", _HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int32)
+        L.orc_create.restype = P
+        L.orc_create.argtypes = [ctypes.POINTER(OrcParams)]
+        for name in ("orc_destroy", "orc_init_conf", "orc_reset_diag", "orc_reset_obs"):
+            getattr(L, name).argtypes = [P]
+            getattr(L, name).restype = None
+        L.orc_sizes.argtypes = [P, ip]
+        L.orc_get_conf.argtypes = [P, ip]
+        L.orc_set_conf.argtypes = [P, ip]
+        L.orc_sweep_start.argtypes = [P]
+        L.orc_sweep_b0.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.orc_sweep_0b.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.orc_get_g.argtypes = [P, dp]
+        L.orc_get_diag.argtypes = [P, dp]
+        L.orc_get_obs.argtypes = [P, dp]
+        L.orc_set_trace.argtypes = [P, dp, ctypes.c_int]
+        L.orc_trace_count.argtypes = [P]
+        L.orc_get_table.argtypes = [P, ctypes.c_int, dp]
+        L.orc_sfmt_u32.argtypes = [ctypes.c_int32, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
+        L.orc_sfmt_stream.argtypes = [ctypes.c_int32, ctypes.c_int, dp]
+        L.orc_chain_seed.argtypes = [ctypes.c_int32]
+        L.orc_chain_seed.restype = ctypes.c_int32
+        L.orc_qr.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, ip, dp, ctypes.c_int]
+        L.orc_inv_logdet.argtypes = [ctypes.c_int, dp, dp]
+        _LIB = L
+    return _LIB
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("lattice", ctypes.c_int), ("la", ctypes.c_int), ("lb", ctypes.c_int),
+        ("beta", ctypes.c_double), ("dtau", ctypes.c_double), ("rt", ctypes.c_double),
+        ("mu", ctypes.c_double), ("rhub", ctypes.c_double), ("nu", ctypes.c_double),
+        ("nflr", ctypes.c_int), ("exact_hs", ctypes.c_int), ("kmode", ctypes.c_int),
+        ("update", ctypes.c_int), ("nwrap", ctypes.c_int), ("nublock", ctypes.c_int),
+        ("seed", ctypes.c_int32),
+    ]
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+LATTICES = {"square": 0, "honeycomb": 1}
+KMODES = {"checkerboard": 0, "dense": 1, "fft": 1}
+UPDATES = {"fast": 0, "delay": 1, "submatrix": 2}
+
+
+class OracleChain:
+    """One Markov chain of the CPU restatement, driven like the reference's dqmc_ctrl."""
+
+    def __init__(self, lattice="square", la=4, lb=4, beta=4.0, dtau=0.1, rt=1.0, mu=0.0,
+                 rhub=4.0, nu=0.0, nflr=2, exact_hs=True, kmode="checkerboard",
+                 update="delay", nwrap=10, nublock=8, seed=None):
+        L = lib()
+        if seed is None:
+            seed = L.orc_chain_seed(0)
+        self.params = OrcParams(LATTICES[lattice], la, lb, beta, dtau, rt, mu, rhub, nu, nflr,
+                                int(bool(exact_hs)), KMODES[kmode], UPDATES[update], nwrap,
+                                nublock, int(np.int32(seed)))
+        self._c = L.orc_create(ctypes.byref(self.params))
+        s = np.zeros(8, np.int32)
+        L.orc_sizes(self._c, _iptr(s))
+        (self.ndim, self.lq, self.ltrot, self.nst, self.lcomp, self.nn_nf, self.nn_lf,
+         self.nublock) = [int(v) for v in s]
+        self._trace = None
+
+    def __del__(self):
+        if getattr(self, "_c", None):
+            lib().orc_destroy(self._c)
+            self._c = None
+
+    def init_conf(self):
+        lib().orc_init_conf(self._c)
+
+    @property
+    def conf(self):
+        c = np.zeros((self.ltrot, self.ndim), np.int32)
+        lib().orc_get_conf(self._c, _iptr(c))
+        return c
+
+    @conf.setter
+    def conf(self, value):
+        c = np.ascontiguousarray(value, dtype=np.int32).reshape(self.ltrot, self.ndim)
+        lib().orc_set_conf(self._c, _iptr(c))
+
+    def sweep_start(self):
+        rc = lib().orc_sweep_start(self._c)
+        if rc:
+            raise RuntimeError("oracle sweep_start failed: %d" % rc)
+
+    def sweep_b0(self, lupdate=True, lmeasure=True):
+        rc = lib().orc_sweep_b0(self._c, int(lupdate), int(lmeasure))
+        if rc:
+            raise RuntimeError("oracle sweep_b0 failed: %d" % rc)
+
+    def sweep_0b(self, lupdate=True, lmeasure=True):
+        rc = lib().orc_sweep_0b(self._c, int(lupdate), int(lmeasure))
+        if rc:
+            raise RuntimeError("oracle sweep_0b failed: %d" % rc)
+
+    def sweep(self, lupdate=True, lmeasure=True):
+        self.sweep_b0(lupdate, lmeasure)
+        self.sweep_0b(lupdate, lmeasure)
+
+    @property
+    def G(self):
+        g = np.zeros((self.ndim, self.ndim), np.complex128, order="F")
+        lib().orc_get_g(self._c, _dptr(g))
+        return g
+
+    def diag(self):
+        d = np.zeros(10)
+        lib().orc_get_diag(self._c, _dptr(d))
+        return {"max_wrap_error": d[0], "max_phasediff": d[1], "phase": complex(d[2], d[3]),
+                "logweightf": complex(d[4], d[5]), "accepted": d[6], "tried": d[7],
+                "nchecks": d[8], "sumlog10err": d[9]}
+
+    def reset_diag(self):
+        lib().orc_reset_diag(self._c)
+
+    def obs(self):
+        o = np.zeros(11)
+        lib().orc_get_obs(self._c, _dptr(o))
+        z = o[1::2] + 1j * o[2::2]
+        return {"nobs": o[0], "sign": z[0], "density": z[1], "ekin": z[2], "dblocc": z[3],
+                "s_stag": z[4]}
+
+    def reset_obs(self):
+        lib().orc_reset_obs(self._c)
+
+    def set_trace(self, cap):
+        self._trace = np.zeros((cap, self.ndim, self.ndim), np.complex128)
+        lib().orc_set_trace(self._c, _dptr(self._trace), cap)
+
+    def trace(self):
+        n = lib().orc_trace_count(self._c)
+        # snapshots are stored column-major per matrix
+        return np.array([t.reshape(self.ndim, self.ndim).T for t in self._trace[:n]])
+
+    def table(self, kind):
+        buf = np.zeros(2 * max(self.ndim * self.ndim, 4 * self.nn_nf * self.nn_lf, 64) + 64)
+        n = lib().orc_get_table(self._c, kind, _dptr(buf))
+        if kind in (3, 6, 7, 8):
+            return buf[:n].copy()
+        return (buf[0:2 * n:2] + 1j * buf[1:2 * n:2]).copy()
+
+
+def sfmt_u32(seed, n):
+    out = np.zeros(n, np.uint32)
+    lib().orc_sfmt_u32(int(np.int32(seed)), n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    return out
+
+
+def sfmt_stream(seed, n):
+    out = np.zeros(n)
+    lib().orc_sfmt_stream(int(np.int32(seed)), n, _dptr(out))
+    return out
+
+
+def chain_seed(c):
+    return int(lib().orc_chain_seed(int(c)))
+
+
+def qr(a, pivot):
+    a = np.asfortranarray(a, dtype=np.complex128)
+    m, n = a.shape
+    k = min(m, n)
+    q = np.zeros((m, k), np.complex128, order="F")
+    r = np.zeros((k, n), np.complex128, order="F")
+    jp = np.zeros(n, np.int32)
+    ld = np.zeros(2)
+    lib().orc_qr(m, n, _dptr(a), _dptr(q), _dptr(r), _iptr(jp), _dptr(ld), int(pivot))
+    return q, r, jp, complex(ld[0], ld[1])
+
+
+def inv_logdet(a):
+    a = np.array(a, dtype=np.complex128, order="F")
+    ld = np.zeros(2)
+    info = lib().orc_inv_logdet(a.shape[0], _dptr(a), _dptr(ld))
+    return a, complex(ld[0], ld[1]), info

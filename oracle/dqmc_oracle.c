@@ -1,0 +1,1418 @@
+/*
+ * dqmc_oracle.c -- CPU restatement of the reference's finite-temperature DQMC sweep.
+ *
+ * TEST INFRASTRUCTURE ONLY (see dqmc_oracle.h).  Plain C99, single thread, complex128.
+ * Each routine names the reference file:line it restates; paths are relative to the
+ * YuanyueChen/sunhubbardmy source tree.  LAPACK calls of the reference (zgeqp3, zgeqrf,
+ * zungqr, zgetrf, ztrtri, dznrm2, idamax, izamax) are restated from their published
+ * algorithms (LAPACK 3.x reference implementation) without the blocking, which changes
+ * rounding only.
+ */
+#include "dqmc_oracle.h"
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double complex zc;
+
+#define IDX(i, j, ld) ((size_t)(i) + (size_t)(j) * (size_t)(ld))
+
+/* ------------------------------------------------------------------------------------ */
+/* SFMT19937: lib/m_spring.f90:187-379 (Saito & Matsumoto's SIMD-oriented Fast MT).       */
+/* 156 128-bit words = 624 uint32 lanes; lane 0 is the least significant word.           */
+/* ------------------------------------------------------------------------------------ */
+#define SFMT_N32 624
+#define SFMT_NW 156
+#define SFMT_POS1 122
+
+typedef struct { uint32_t s[SFMT_N32]; int idx; } sfmt_t;
+
+static const uint32_t sfmt_parity[4] = {0x00000001u, 0x00000000u, 0x00000000u, 0x13c9e684u};
+static const uint32_t sfmt_msk[4] = {0xdfffffefu, 0xddfecb7fu, 0xbffaffffu, 0xbffffff6u};
+
+/* m_spring.f90:187-243: Knuth-style linear init, then period certification */
+static void sfmt_init(sfmt_t *g, int32_t seed)
+{
+    g->s[0] = (uint32_t)seed;
+    for (int i = 1; i < SFMT_N32; i++)
+        g->s[i] = 1812433253u * (g->s[i - 1] ^ (g->s[i - 1] >> 30)) + (uint32_t)i;
+    uint32_t inner = 0;
+    for (int i = 0; i < 4; i++) inner ^= g->s[i] & sfmt_parity[i];
+    for (int sh = 16; sh > 0; sh >>= 1) inner ^= inner >> sh;
+    if ((inner & 1u) != 1u) {
+        int done = 0;
+        for (int i = 0; i < 4 && !done; i++) {
+            uint32_t work = 1;
+            for (int j = 0; j < 32; j++) {
+                if (work & sfmt_parity[i]) { g->s[i] ^= work; done = 1; break; }
+                work <<= 1;
+            }
+        }
+    }
+    g->idx = SFMT_N32;
+}
+
+/* 128-bit shifts by whole bytes, lanes little-endian */
+static void sfmt_lsh8(uint32_t o[4], const uint32_t a[4])
+{
+    uint64_t hi = ((uint64_t)a[3] << 32) | a[2], lo = ((uint64_t)a[1] << 32) | a[0];
+    uint64_t oh = (hi << 8) | (lo >> 56), ol = lo << 8;
+    o[0] = (uint32_t)ol; o[1] = (uint32_t)(ol >> 32); o[2] = (uint32_t)oh; o[3] = (uint32_t)(oh >> 32);
+}
+static void sfmt_rsh8(uint32_t o[4], const uint32_t a[4])
+{
+    uint64_t hi = ((uint64_t)a[3] << 32) | a[2], lo = ((uint64_t)a[1] << 32) | a[0];
+    uint64_t ol = (lo >> 8) | (hi << 56), oh = hi >> 8;
+    o[0] = (uint32_t)ol; o[1] = (uint32_t)(ol >> 32); o[2] = (uint32_t)oh; o[3] = (uint32_t)(oh >> 32);
+}
+
+/* m_spring.f90:290-379: one refill of the whole state (recursion r = a ^ a<<8 ^ (b>>11 & msk) ^ c>>8 ^ d<<18) */
+static void sfmt_refill(sfmt_t *g)
+{
+    uint32_t *w = g->s;
+    const uint32_t *r1 = w + 4 * (SFMT_NW - 2), *r2 = w + 4 * (SFMT_NW - 1);
+    for (int i = 0; i < SFMT_NW; i++) {
+        uint32_t *a = w + 4 * i;
+        const uint32_t *b = w + 4 * ((i + SFMT_POS1) % SFMT_NW);
+        uint32_t x[4], y[4];
+        sfmt_lsh8(x, a);
+        sfmt_rsh8(y, r1);
+        for (int l = 0; l < 4; l++)
+            a[l] = a[l] ^ x[l] ^ ((b[l] >> 11) & sfmt_msk[l]) ^ y[l] ^ (r2[l] << 18);
+        r1 = r2;
+        r2 = a;
+    }
+}
+
+static uint32_t sfmt_u32(sfmt_t *g)
+{
+    if (g->idx >= SFMT_N32) { sfmt_refill(g); g->idx = 0; }
+    return g->s[g->idx++];
+}
+
+/* m_spring.f90:245-254: (u + 0.5) / 2^32, in (0,1) */
+static double sfmt_stream(sfmt_t *g) { return ((double)sfmt_u32(g) + 0.5) * (1.0 / 4294967296.0); }
+
+void orc_sfmt_u32(int32_t seed, int n, uint32_t *out)
+{
+    sfmt_t g; sfmt_init(&g, seed);
+    for (int i = 0; i < n; i++) out[i] = sfmt_u32(&g);
+}
+void orc_sfmt_stream(int32_t seed, int n, double *out)
+{
+    sfmt_t g; sfmt_init(&g, seed);
+    for (int i = 0; i < n; i++) out[i] = sfmt_stream(&g);
+}
+
+/* src/dqmc_util.f90:70-72: abs(0-(irank*1981+2008)*951049) in default (32-bit) integer;
+ * the product wraps in two's complement before abs */
+int32_t orc_chain_seed(int32_t chain)
+{
+    uint32_t a = (uint32_t)chain * 1981u + 2008u;
+    uint32_t p = a * 951049u;
+    int32_t v = (int32_t)p;
+    if (v < 0) v = (int32_t)(0u - (uint32_t)v);
+    return v;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* dense complex helpers                                                                 */
+/* ------------------------------------------------------------------------------------ */
+static zc *zalloc(size_t n) { return (zc *)calloc(n ? n : 1, sizeof(zc)); }
+
+/* C(m x n) = op(A) * op(B) (+ beta*C), op: 'N', 'T', 'C'; A, B, C column-major */
+static void zgemm(char ta, char tb, int m, int n, int k, const zc *A, int lda, const zc *B, int ldb,
+                  zc beta, zc *C, int ldc)
+{
+    zc *Ae = NULL, *Be = NULL;
+    const zc *Ap = A, *Bp = B;
+    int la = lda, lb = ldb;
+    if (ta != 'N') {
+        Ae = zalloc((size_t)m * k);
+        for (int j = 0; j < k; j++)
+            for (int i = 0; i < m; i++) {
+                zc v = A[IDX(j, i, lda)];
+                Ae[IDX(i, j, m)] = (ta == 'C') ? conj(v) : v;
+            }
+        Ap = Ae; la = m;
+    }
+    if (tb != 'N') {
+        Be = zalloc((size_t)k * n);
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < k; i++) {
+                zc v = B[IDX(j, i, ldb)];
+                Be[IDX(i, j, k)] = (tb == 'C') ? conj(v) : v;
+            }
+        Bp = Be; lb = k;
+    }
+    for (int j = 0; j < n; j++) {
+        zc *c = C + (size_t)j * ldc;
+        if (beta == 0) for (int i = 0; i < m; i++) c[i] = 0;
+        else if (beta != 1) for (int i = 0; i < m; i++) c[i] *= beta;
+        for (int l = 0; l < k; l++) {
+            zc bl = Bp[IDX(l, j, lb)];
+            if (bl == 0) continue;
+            const zc *a = Ap + (size_t)l * la;
+            for (int i = 0; i < m; i++) c[i] += a[i] * bl;
+        }
+    }
+    free(Ae);
+    free(Be);
+}
+
+/* BLAS dznrm2 (scaled sum of squares) */
+static double dznrm2(int n, const zc *x, int incx)
+{
+    double scale = 0.0, ssq = 1.0;
+    for (int i = 0; i < n; i++) {
+        double v[2] = {creal(x[(size_t)i * incx]), cimag(x[(size_t)i * incx])};
+        for (int t = 0; t < 2; t++) {
+            if (v[t] != 0.0) {
+                double a = fabs(v[t]);
+                if (scale < a) { ssq = 1.0 + ssq * (scale / a) * (scale / a); scale = a; }
+                else ssq += (a / scale) * (a / scale);
+            }
+        }
+    }
+    return scale * sqrt(ssq);
+}
+
+static double sgn(double a, double b) { return b >= 0 ? fabs(a) : -fabs(a); }
+
+/* LAPACK zlarfg: alpha <- beta (real), x <- v(2:n), returns tau; H = I - tau v v^H,
+ * H^H [alpha; x] = [beta; 0] */
+static zc zlarfg(int n, zc *alpha, zc *x)
+{
+    if (n <= 0) return 0;
+    double xnorm = dznrm2(n - 1, x, 1);
+    double ar = creal(*alpha), ai = cimag(*alpha);
+    if (xnorm == 0.0 && ai == 0.0) return 0;
+    double beta = -sgn(hypot(hypot(ar, ai), xnorm), ar);
+    zc tau = CMPLX((beta - ar) / beta, -ai / beta);
+    zc scal = 1.0 / (*alpha - beta);
+    for (int i = 0; i < n - 1; i++) x[i] *= scal;
+    *alpha = beta;
+    return tau;
+}
+
+/* apply H^H = I - conj(tau) v v^H from the left to A(m x n) (zlarf 'Left' with conj(tau)) */
+static void apply_hh_left(int m, int n, const zc *v, zc tauc, zc *A, int lda)
+{
+    if (tauc == 0) return;
+    for (int j = 0; j < n; j++) {
+        zc *a = A + (size_t)j * lda;
+        zc w = 0;
+        for (int i = 0; i < m; i++) w += conj(v[i]) * a[i];
+        w *= tauc;
+        for (int i = 0; i < m; i++) a[i] -= v[i] * w;
+    }
+}
+
+/*
+ * Householder QR of A (m x n, m >= n), optionally with column pivoting.
+ *   pivot = 1: LAPACK zgeqp3 / zlaqp2 (max partial column norm, downdated with the
+ *              tol3z = sqrt(eps) recompute rule); as called by s_zgeQRPT_logdetQ
+ *              (lib/s_matrix.f90:1222-1341).
+ *   pivot = 0: zgeqrf as called by s_zgeQR_logdetQ (lib/s_matrix.f90:1998-2109).
+ * Returns the reflectors in A, tau[n], jpvt (1-based).  logdetQ = sum_i log(1 - tau_i 2Re(tau_i)/|tau_i|^2)
+ * for tau_i != 0 (s_matrix.f90:1279-1289).
+ */
+static void zgeqr_core(int m, int n, zc *A, int lda, zc *tau, int32_t *jpvt, int pivot)
+{
+    double *vn1 = NULL, *vn2 = NULL;
+    for (int j = 0; j < n; j++) jpvt[j] = j + 1;
+    if (pivot) {
+        vn1 = (double *)malloc(sizeof(double) * n);
+        vn2 = (double *)malloc(sizeof(double) * n);
+        for (int j = 0; j < n; j++) vn1[j] = vn2[j] = dznrm2(m, A + (size_t)j * lda, 1);
+    }
+    const double tol3z = sqrt(2.220446049250313e-16);
+    int kmax = m < n ? m : n;
+    zc *v = zalloc(m);
+    for (int i = 0; i < kmax; i++) {
+        if (pivot) {
+            int p = i;
+            double best = fabs(vn1[i]);
+            for (int j = i + 1; j < n; j++)
+                if (fabs(vn1[j]) > best) { best = fabs(vn1[j]); p = j; }
+            if (p != i) {
+                for (int r = 0; r < m; r++) {
+                    zc t = A[IDX(r, p, lda)]; A[IDX(r, p, lda)] = A[IDX(r, i, lda)]; A[IDX(r, i, lda)] = t;
+                }
+                int32_t it = jpvt[p]; jpvt[p] = jpvt[i]; jpvt[i] = it;
+                vn1[p] = vn1[i];
+                vn2[p] = vn2[i];
+            }
+        }
+        zc *col = A + IDX(i, i, lda);
+        tau[i] = zlarfg(m - i, col, col + 1);
+        if (i < n - 1) {
+            v[0] = 1;
+            for (int r = 1; r < m - i; r++) v[r] = col[r];
+            apply_hh_left(m - i, n - i - 1, v, conj(tau[i]), A + IDX(i, i + 1, lda), lda);
+        }
+        if (pivot) {
+            for (int j = i + 1; j < n; j++) {
+                if (vn1[j] != 0.0) {
+                    double t = cabs(A[IDX(i, j, lda)]) / vn1[j];
+                    t = 1.0 - t * t;
+                    if (t < 0) t = 0;
+                    double t2 = t * (vn1[j] / vn2[j]) * (vn1[j] / vn2[j]);
+                    if (t2 <= tol3z) {
+                        if (i < m - 1) {
+                            vn1[j] = dznrm2(m - i - 1, A + IDX(i + 1, j, lda), 1);
+                            vn2[j] = vn1[j];
+                        } else {
+                            vn1[j] = 0.0;
+                            vn2[j] = 0.0;
+                        }
+                    } else {
+                        vn1[j] *= sqrt(t);
+                    }
+                }
+            }
+        }
+    }
+    free(v);
+    free(vn1);
+    free(vn2);
+}
+
+static zc logdet_q(int k, const zc *tau)
+{
+    zc s = 0;
+    for (int i = 0; i < k; i++)
+        if (cabs(tau[i]) != 0.0)
+            s += clog(1.0 - tau[i] * (2.0 * creal(tau[i])) / (tau[i] * conj(tau[i])));
+    return s;
+}
+
+/* zung2r: Q = H_1 ... H_k formed in place from the reflectors in A (m x k) */
+static void zungqr(int m, int k, zc *A, int lda, const zc *tau)
+{
+    zc *v = zalloc(m);
+    for (int i = k - 1; i >= 0; i--) {
+        /* apply H_i to A(i:m, i+1:k) from the left */
+        if (i < k - 1) {
+            v[0] = 1;
+            for (int r = 1; r < m - i; r++) v[r] = A[IDX(i + r, i, lda)];
+            /* H_i = I - tau v v^H : zlarf with tau (not conj) */
+            for (int j = i + 1; j < k; j++) {
+                zc *a = A + IDX(i, j, lda);
+                zc w = 0;
+                for (int r = 0; r < m - i; r++) w += conj(v[r]) * a[r];
+                w *= tau[i];
+                for (int r = 0; r < m - i; r++) a[r] -= v[r] * w;
+            }
+        }
+        if (i < m - 1)
+            for (int r = i + 1; r < m; r++) A[IDX(r, i, lda)] *= -tau[i];
+        A[IDX(i, i, lda)] = 1.0 - tau[i];
+        for (int r = 0; r < i; r++) A[IDX(r, i, lda)] = 0;
+    }
+    free(v);
+}
+
+/* QR of a square or tall matrix: Q (m x kmax), R (kmax x n) upper triangular */
+static void qr_full(int m, int n, const zc *a, zc *q, zc *r, int32_t *jpvt, zc *ldq, int pivot)
+{
+    int k = m < n ? m : n;
+    zc *w = zalloc((size_t)m * n), *tau = zalloc(n);
+    memcpy(w, a, sizeof(zc) * (size_t)m * n);
+    zgeqr_core(m, n, w, m, tau, jpvt, pivot);
+    *ldq = logdet_q(k, tau);
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < k; i++) r[IDX(i, j, k)] = (i <= j) ? w[IDX(i, j, m)] : 0;
+    for (int j = 0; j < k; j++)
+        for (int i = 0; i < m; i++) q[IDX(i, j, m)] = (i > j) ? w[IDX(i, j, m)] : (i == j ? 1 : 0);
+    zungqr(m, k, q, m, tau);
+    free(w);
+    free(tau);
+}
+
+void orc_qr(int m, int n, const double *a, double *q, double *r, int32_t *jpvt, double *logdetq, int pivot)
+{
+    zc ldq;
+    qr_full(m, n, (const zc *)a, (zc *)q, (zc *)r, jpvt, &ldq, pivot);
+    logdetq[0] = creal(ldq);
+    logdetq[1] = cimag(ldq);
+}
+
+/*
+ * s_inv_logdet_lu_z (lib/s_matrix.f90:2689-2820): zgetrf with partial pivoting (pivot =
+ * izamax, i.e. the largest |Re|+|Im|), log det = sum log(+-U_ii) with the imaginary part
+ * folded by nint(Im/2pi), then A^-1.  The inverse is formed by triangular solves, which is
+ * the same matrix as U^-1 L^-1 P^T up to rounding.
+ */
+static int zinv_logdet(int n, zc *a, zc *logdet)
+{
+    int32_t *ipiv = (int32_t *)malloc(sizeof(int32_t) * n);
+    int info = 0;
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double best = -1;
+        for (int i = k; i < n; i++) {
+            double v = fabs(creal(a[IDX(i, k, n)])) + fabs(cimag(a[IDX(i, k, n)]));
+            if (v > best) { best = v; p = i; }
+        }
+        ipiv[k] = p;
+        if (a[IDX(p, k, n)] == 0) { if (!info) info = k + 1; continue; }
+        if (p != k)
+            for (int j = 0; j < n; j++) {
+                zc t = a[IDX(p, j, n)]; a[IDX(p, j, n)] = a[IDX(k, j, n)]; a[IDX(k, j, n)] = t;
+            }
+        zc piv = 1.0 / a[IDX(k, k, n)];
+        for (int i = k + 1; i < n; i++) a[IDX(i, k, n)] *= piv;
+        for (int j = k + 1; j < n; j++) {
+            zc akj = a[IDX(k, j, n)];
+            if (akj == 0) continue;
+            zc *cj = a + (size_t)j * n;
+            const zc *ck = a + (size_t)k * n;
+            for (int i = k + 1; i < n; i++) cj[i] -= ck[i] * akj;
+        }
+    }
+    zc ld = 0;
+    for (int i = 0; i < n; i++) ld += (ipiv[i] == i) ? clog(a[IDX(i, i, n)]) : clog(-a[IDX(i, i, n)]);
+    double im = cimag(ld);
+    double k2 = round(im / 2.0 / M_PI);
+    ld = CMPLX(creal(ld), im - 2.0 * M_PI * k2);
+    *logdet = ld;
+    if (info) { free(ipiv); return info; }
+    /* inverse: solve A X = I with P A = L U */
+    zc *x = zalloc((size_t)n * n);
+    for (int i = 0; i < n; i++) x[IDX(i, i, n)] = 1;
+    for (int k = 0; k < n; k++)
+        if (ipiv[k] != k)
+            for (int j = 0; j < n; j++) {
+                zc t = x[IDX(ipiv[k], j, n)]; x[IDX(ipiv[k], j, n)] = x[IDX(k, j, n)]; x[IDX(k, j, n)] = t;
+            }
+    for (int j = 0; j < n; j++) {
+        zc *xj = x + (size_t)j * n;
+        for (int k = 0; k < n; k++) {          /* L y = b, unit lower */
+            zc v = xj[k];
+            if (v == 0) continue;
+            const zc *ck = a + (size_t)k * n;
+            for (int i = k + 1; i < n; i++) xj[i] -= ck[i] * v;
+        }
+        for (int k = n - 1; k >= 0; k--) {     /* U x = y */
+            xj[k] /= a[IDX(k, k, n)];
+            zc v = xj[k];
+            const zc *ck = a + (size_t)k * n;
+            for (int i = 0; i < k; i++) xj[i] -= ck[i] * v;
+        }
+    }
+    memcpy(a, x, sizeof(zc) * (size_t)n * n);
+    free(x);
+    free(ipiv);
+    return 0;
+}
+
+int orc_inv_logdet(int n, double *a, double *logdet)
+{
+    zc ld;
+    int info = zinv_logdet(n, (zc *)a, &ld);
+    logdet[0] = creal(ld);
+    logdet[1] = cimag(ld);
+    return info;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* context                                                                               */
+/* ------------------------------------------------------------------------------------ */
+struct orc_ctx {
+    orc_params p;
+    int ndim, lq, nsub, ltrot, nst, nwrap, nublock;
+    int nn_nf, nn_lf;
+    int *nnlf;          /* [nn_lf] A sites, 0-based */
+    int *nnlist;        /* [nsites*nn_nf] */
+    int *cx, *cy, *sub; /* unit-cell coordinates and sublattice of each site */
+    int *iwrap_nt;      /* [ltrot+1] */
+    int *wrap_step;     /* [2*nst], 1-based slice numbers */
+    /* kinetic */
+    int lwrapT, lwrapu;
+    zc *urt, *urtm1;    /* checkerboard: [ist*4 + (i-1) + 2*(j-1)]; dense: N*N */
+    /* HS */
+    int lcomp;
+    double alpha, etal[4], gaml[4];
+    zc bmat[4], bmat_inv[4], phase_tab[4], delta_bmat[3 * 4], phase_ratio[3 * 4];
+    int32_t *conf;      /* [ltrot][nsites] */
+    sfmt_t rng;
+    /* sweep state */
+    zc *G, *Gtmp;
+    zc **Ust, **Vst;
+    double **Dst;
+    zc *logdetQst;
+    zc phase, logweightf;
+    double max_wrap_error, max_phasediff, nchecks, sumlog10err, accepted, tried;
+    double nobs;
+    zc obs[5];
+    double *trace;
+    int trace_cap, trace_n;
+};
+
+/* src/lattice.f90:792-961 (square) and :611-776 (honeycomb); 0-based site indices */
+static void setup_lattice(orc_ctx *c)
+{
+    int l1 = c->p.la, l2 = c->p.lb;
+#define NPBC(v, l) ((v) > (l) ? (v) - (l) : ((v) < 1 ? (v) + (l) : (v)))
+    if (c->p.lattice == 0) {
+        c->nsub = 1; c->lq = l1 * l2; c->ndim = c->lq;
+        c->nn_nf = 4; c->nn_lf = l1 * l2 / 2;
+        c->nnlf = (int *)calloc(c->nn_lf > 0 ? c->nn_lf : 1, sizeof(int));
+        c->nnlist = (int *)calloc((size_t)c->ndim * 4, sizeof(int));
+        c->cx = (int *)calloc(c->ndim, sizeof(int));
+        c->cy = (int *)calloc(c->ndim, sizeof(int));
+        c->sub = (int *)calloc(c->ndim, sizeof(int));
+        int ind = 0;
+        for (int i1 = 1; i1 <= l1; i1++)
+            for (int i2 = 1; i2 <= l2; i2++) {
+                int s = i2 + (i1 - 1) * l2 - 1;
+                c->cx[s] = i1; c->cy[s] = i2; c->sub[s] = 0;
+                if ((i1 + i2) % 2 == 0 && ind < c->nn_lf) c->nnlf[ind++] = s;
+            }
+        for (int s = 0; s < c->ndim; s++) {
+            int x = c->cx[s], y = c->cy[s];
+            c->nnlist[s * 4 + 0] = (NPBC(x + 1, l1) - 1) * l2 + NPBC(y, l2) - 1;
+            c->nnlist[s * 4 + 1] = (NPBC(x, l1) - 1) * l2 + NPBC(y + 1, l2) - 1;
+            c->nnlist[s * 4 + 2] = (NPBC(x - 1, l1) - 1) * l2 + NPBC(y, l2) - 1;
+            c->nnlist[s * 4 + 3] = (NPBC(x, l1) - 1) * l2 + NPBC(y - 1, l2) - 1;
+        }
+    } else {
+        c->nsub = 2; c->lq = l1 * l2; c->ndim = 2 * c->lq;
+        c->nn_nf = 3; c->nn_lf = l1 * l2;
+        c->nnlf = (int *)calloc(c->nn_lf, sizeof(int));
+        c->nnlist = (int *)calloc((size_t)c->ndim * 3, sizeof(int));
+        c->cx = (int *)calloc(c->ndim, sizeof(int));
+        c->cy = (int *)calloc(c->ndim, sizeof(int));
+        c->sub = (int *)calloc(c->ndim, sizeof(int));
+        for (int i = 0; i < c->nn_lf; i++) c->nnlf[i] = 2 * i;
+        int ind = 0;
+        for (int i1 = 1; i1 <= l1; i1++)
+            for (int i2 = 1; i2 <= l2; i2++)
+                for (int o = 0; o < 2; o++) { c->cx[ind] = i1; c->cy[ind] = i2; c->sub[ind] = o; ind++; }
+        for (int s = 0; s < c->ndim; s++) {
+            int x = c->cx[s], y = c->cy[s];
+            if (c->sub[s] == 0) {
+                c->nnlist[s * 3 + 0] = s + 1;
+                c->nnlist[s * 3 + 1] = ((NPBC(x + 1, l1) - 1) * l2 + NPBC(y, l2)) * 2 - 1;
+                c->nnlist[s * 3 + 2] = ((NPBC(x, l1) - 1) * l2 + NPBC(y - 1, l2)) * 2 - 1;
+            } else {
+                c->nnlist[s * 3 + 0] = s - 1;
+                c->nnlist[s * 3 + 1] = ((NPBC(x - 1, l1) - 1) * l2 + NPBC(y, l2)) * 2 - 2;
+                c->nnlist[s * 3 + 2] = ((NPBC(x, l1) - 1) * l2 + NPBC(y + 1, l2)) * 2 - 2;
+            }
+        }
+    }
+#undef NPBC
+}
+
+/* src/model_para.f90:421-453: nst, iwrap_nt(0:ltrot), wrap_step(2,nst) */
+static void setup_schedule(orc_ctx *c)
+{
+    int lt = c->ltrot, nw = c->p.nwrap;
+    c->iwrap_nt = (int *)calloc(lt + 1, sizeof(int));
+    if (lt < nw) nw = lt;
+    c->nwrap = nw;
+    if (lt % nw == 0) {
+        c->nst = lt / nw;
+        c->wrap_step = (int *)calloc(2 * c->nst, sizeof(int));
+        for (int i = 1; i <= c->nst; i++) {
+            c->iwrap_nt[i * nw] = i;
+            c->wrap_step[2 * (i - 1)] = 1 + (i - 1) * nw;
+            c->wrap_step[2 * (i - 1) + 1] = i * nw;
+        }
+    } else {
+        c->nst = lt / nw + 1;
+        c->wrap_step = (int *)calloc(2 * c->nst, sizeof(int));
+        for (int i = 1; i <= c->nst - 1; i++) {
+            c->iwrap_nt[i * nw] = i;
+            c->wrap_step[2 * (i - 1)] = 1 + (i - 1) * nw;
+            c->wrap_step[2 * (i - 1) + 1] = i * nw;
+        }
+        int i = c->nst;
+        c->iwrap_nt[lt] = i;
+        c->wrap_step[2 * (i - 1)] = (i - 1) * nw + 1;
+        c->wrap_step[2 * (i - 1) + 1] = lt;
+    }
+}
+
+/* src/dqmc_config_u/dqmc_set_conf.f90:1-176 (repulsive and attractive U, no projection) */
+static void setup_hs(orc_ctx *c)
+{
+    double u = c->p.rhub, dtau = c->p.dtau, nu = c->p.nu;
+    int nflr = c->p.nflr;
+    int exact = c->p.exact_hs;
+    int general = 1;
+    if (exact && nflr == 2) {
+        c->lcomp = 2;
+        c->etal[0] = -1.0; c->etal[1] = 1.0;
+        c->alpha = (u >= 0) ? acos(exp(-dtau * fabs(u) * 0.5)) : acosh(exp(dtau * fabs(u) * 0.5));
+        c->gaml[0] = c->gaml[1] = 1.0;
+        general = 0;
+    } else if (exact && (nflr == 4 || nflr == 6)) {
+        c->lcomp = 4;
+        double a = exp(-0.5 * dtau * u);
+        double d = sqrt(8.0 + a * a * (3.0 + a * a) * (3.0 + a * a));
+        double xp = (a + 2.0 * a * a * a + pow(a, 5) + (a * a - 1.0) * d) / 4.0;
+        double xm = (a + 2.0 * a * a * a + pow(a, 5) - (a * a - 1.0) * d) / 4.0;
+        if (u >= 0) {
+            c->etal[0] = -acos(xm); c->etal[1] = -acos(xp); c->etal[2] = acos(xp); c->etal[3] = acos(xm);
+        } else {
+            c->etal[0] = -acosh(xm); c->etal[1] = -acosh(xp); c->etal[2] = acosh(xp); c->etal[3] = acosh(xm);
+        }
+        c->alpha = 1.0;
+        double t = a * (3.0 + a * a);
+        c->gaml[0] = (t + d) / d; c->gaml[1] = (-t + d) / d; c->gaml[2] = (-t + d) / d; c->gaml[3] = (t + d) / d;
+        general = 0;
+    }
+    if (general) {
+        c->lcomp = 4;
+        double s6 = sqrt(6.0);
+        c->etal[0] = -sqrt(2.0 * (3.0 + s6));
+        c->etal[1] = -sqrt(2.0 * (3.0 - s6));
+        c->etal[2] = sqrt(2.0 * (3.0 - s6));
+        c->etal[3] = sqrt(2.0 * (3.0 + s6));
+        c->alpha = sqrt(dtau * fabs(u) * 0.5);
+        c->gaml[0] = 1.0 - s6 / 3.0; c->gaml[1] = 1.0 + s6 / 3.0;
+        c->gaml[2] = 1.0 + s6 / 3.0; c->gaml[3] = 1.0 - s6 / 3.0;
+    }
+    int L = c->lcomp;
+    double al = c->alpha;
+    for (int is = 0; is < L; is++) {
+        if (u >= 0) {
+            c->phase_tab[is] = cexp(CMPLX(0.0, al * (-c->etal[is]) * 0.5 * ((double)nflr + nu)));
+            c->bmat[is] = cexp(CMPLX(0.0, al * c->etal[is]));
+            c->bmat_inv[is] = cexp(CMPLX(0.0, -al * c->etal[is]));
+        } else {
+            c->phase_tab[is] = cexp(CMPLX(al * (-c->etal[is]) * 0.5 * ((double)nflr + nu), 0.0));
+            c->bmat[is] = cexp(CMPLX(al * c->etal[is], 0.0));
+            c->bmat_inv[is] = cexp(CMPLX(-al * c->etal[is], 0.0));
+        }
+    }
+    for (int is = 0; is < L; is++)
+        for (int fl = 1; fl <= L - 1; fl++) {
+            int isp = (is + fl) % L; /* mod(is+iflip-1,lcomp)+1 with 1-based is */
+            int k = (fl - 1) + (L - 1) * is;
+            if (u >= 0) {
+                c->phase_ratio[k] = cexp(CMPLX(0.0, al * (c->etal[is] - c->etal[isp]) * 0.5 * ((double)nflr + nu)));
+                c->delta_bmat[k] = cexp(CMPLX(0.0, al * (c->etal[isp] - c->etal[is]))) - 1.0;
+            } else {
+                c->phase_ratio[k] = cexp(CMPLX(al * (c->etal[is] - c->etal[isp]) * 0.5 * ((double)nflr + nu), 0.0));
+                c->delta_bmat[k] = cexp(CMPLX(al * (c->etal[isp] - c->etal[is]), 0.0)) - 1.0;
+            }
+        }
+}
+
+/* exp(-s H) for H = [[d, z],[conj z, d]] (2x2 bond block of dqmc_set_h0conf.f90:77-123) */
+static void bond_exp(zc z, double d, double s, zc out[4])
+{
+    double r = cabs(z);
+    double ch = cosh(s * r), sh_r = (r > 0) ? sinh(s * r) / r : s;
+    double e = exp(-s * d);
+    out[0] = e * ch;                /* (1,1) */
+    out[1] = -e * sh_r * conj(z);   /* (2,1) */
+    out[2] = -e * sh_r * z;         /* (1,2) */
+    out[3] = e * ch;                /* (2,2) */
+}
+
+/* src/dqmc_config_h0/seth0.f90:1-28 with expar = 1 (no field, flux or dimerisation) */
+static void build_h0(const orc_ctx *c, zc *H)
+{
+    int N = c->ndim;
+    memset(H, 0, sizeof(zc) * (size_t)N * N);
+    for (int b = 0; b < c->nn_lf; b++) {
+        int i0 = c->nnlf[b];
+        for (int nf = 0; nf < c->nn_nf; nf++) {
+            int in = c->nnlist[i0 * c->nn_nf + nf];
+            H[IDX(i0, in, N)] += -c->p.rt;
+            H[IDX(in, i0, N)] += -c->p.rt;
+        }
+    }
+    for (int i = 0; i < N; i++) H[IDX(i, i, N)] -= c->p.mu;
+}
+
+/* exp(s H) by a Taylor series with scaling and squaring (same matrix as the reference's
+ * s_eig_he route in dqmc_set_h0conf.f90:484-510, up to rounding) */
+static void expm_herm(int N, const zc *H, double s, zc *out)
+{
+    double nrm = 0;
+    for (int j = 0; j < N; j++) {
+        double cs = 0;
+        for (int i = 0; i < N; i++) cs += cabs(H[IDX(i, j, N)]);
+        if (cs > nrm) nrm = cs;
+    }
+    int sq = 0;
+    double sc = fabs(s) * nrm;
+    while (sc > 0.25) { sc *= 0.5; sq++; }
+    double f = s / (double)(1 << sq);
+    zc *A = zalloc((size_t)N * N), *T = zalloc((size_t)N * N), *T2 = zalloc((size_t)N * N);
+    for (size_t k = 0; k < (size_t)N * N; k++) A[k] = f * H[k];
+    memset(out, 0, sizeof(zc) * (size_t)N * N);
+    for (int i = 0; i < N; i++) { out[IDX(i, i, N)] = 1; T[IDX(i, i, N)] = 1; }
+    for (int term = 1; term <= 30; term++) {
+        zgemm('N', 'N', N, N, N, A, N, T, N, 0, T2, N);
+        double mx = 0;
+        for (size_t k = 0; k < (size_t)N * N; k++) { T2[k] /= (double)term; if (cabs(T2[k]) > mx) mx = cabs(T2[k]); out[k] += T2[k]; }
+        zc *sw = T; T = T2; T2 = sw;
+        if (mx < 1e-18) break;
+    }
+    for (int q = 0; q < sq; q++) {
+        zgemm('N', 'N', N, N, N, out, N, out, N, 0, T2, N);
+        memcpy(out, T2, sizeof(zc) * (size_t)N * N);
+    }
+    free(A); free(T); free(T2);
+}
+
+static void setup_kinetic(orc_ctx *c)
+{
+    int N = c->ndim;
+    c->lwrapT = (c->p.rt != 0.0);
+    if (c->p.kmode == 0) {
+        int nb = c->nn_nf * c->nn_lf;
+        c->urt = zalloc((size_t)nb * 4);
+        c->urtm1 = zalloc((size_t)nb * 4);
+        for (int nf = 0; nf < c->nn_nf; nf++)
+            for (int b = 0; b < c->nn_lf; b++) {
+                int ist = b + nf * c->nn_lf;
+                zc z = -c->p.rt;
+                double d = -c->p.mu / (double)c->nn_nf;
+                bond_exp(z, d, 0.5 * c->p.dtau, c->urt + 4 * ist);
+                bond_exp(z, d, -0.5 * c->p.dtau, c->urtm1 + 4 * ist);
+            }
+    } else {
+        zc *H = zalloc((size_t)N * N);
+        build_h0(c, H);
+        c->urt = zalloc((size_t)N * N);
+        c->urtm1 = zalloc((size_t)N * N);
+        expm_herm(N, H, -0.5 * c->p.dtau, c->urt);
+        expm_herm(N, H, 0.5 * c->p.dtau, c->urtm1);
+        free(H);
+    }
+}
+
+orc_ctx *orc_create(const orc_params *p)
+{
+    orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+    c->p = *p;
+    setup_lattice(c);
+    c->ltrot = (int)lround(p->beta / p->dtau);
+    setup_schedule(c);
+    c->nublock = p->nublock > 0 ? p->nublock : 1;
+    c->lwrapu = (p->rhub != 0.0);
+    setup_hs(c);
+    setup_kinetic(c);
+    sfmt_init(&c->rng, p->seed);
+    int N = c->ndim;
+    c->conf = (int32_t *)calloc((size_t)N * c->ltrot, sizeof(int32_t));
+    for (size_t k = 0; k < (size_t)N * c->ltrot; k++) c->conf[k] = 1;
+    c->G = zalloc((size_t)N * N);
+    c->Gtmp = zalloc((size_t)N * N);
+    c->Ust = (zc **)calloc(c->nst + 1, sizeof(zc *));
+    c->Vst = (zc **)calloc(c->nst + 1, sizeof(zc *));
+    c->Dst = (double **)calloc(c->nst + 1, sizeof(double *));
+    c->logdetQst = zalloc(c->nst + 1);
+    for (int n = 0; n <= c->nst; n++) {
+        c->Ust[n] = zalloc((size_t)N * N);
+        c->Vst[n] = zalloc((size_t)N * N);
+        c->Dst[n] = (double *)calloc(N, sizeof(double));
+    }
+    c->phase = 1.0;
+    return c;
+}
+
+void orc_destroy(orc_ctx *c)
+{
+    if (!c) return;
+    for (int n = 0; n <= c->nst; n++) { free(c->Ust[n]); free(c->Vst[n]); free(c->Dst[n]); }
+    free(c->Ust); free(c->Vst); free(c->Dst); free(c->logdetQst);
+    free(c->G); free(c->Gtmp); free(c->conf);
+    free(c->urt); free(c->urtm1);
+    free(c->nnlf); free(c->nnlist); free(c->cx); free(c->cy); free(c->sub);
+    free(c->iwrap_nt); free(c->wrap_step);
+    free(c);
+}
+
+void orc_sizes(const orc_ctx *c, int *out)
+{
+    out[0] = c->ndim; out[1] = c->lq; out[2] = c->ltrot; out[3] = c->nst;
+    out[4] = c->lcomp; out[5] = c->nn_nf; out[6] = c->nn_lf; out[7] = c->nublock;
+}
+
+/* dqmc_input_conf.f90:55-63 (single rank): conf(i,nt) = ceiling(u*lcomp), nt outer, i inner */
+void orc_init_conf(orc_ctx *c)
+{
+    if (!c->lwrapu) return;
+    for (int nt = 0; nt < c->ltrot; nt++)
+        for (int i = 0; i < c->ndim; i++)
+            c->conf[(size_t)nt * c->ndim + i] = (int32_t)ceil(sfmt_stream(&c->rng) * c->lcomp);
+}
+void orc_get_conf(const orc_ctx *c, int32_t *conf) { memcpy(conf, c->conf, sizeof(int32_t) * (size_t)c->ndim * c->ltrot); }
+void orc_set_conf(orc_ctx *c, const int32_t *conf) { memcpy(c->conf, conf, sizeof(int32_t) * (size_t)c->ndim * c->ltrot); }
+
+/* ------------------------------------------------------------------------------------ */
+/* operator plugins (h0conf, uconf) acting on an N x N matrix                            */
+/* ------------------------------------------------------------------------------------ */
+/* kinetic, from the left: rows <- T * rows (dqmc_config_h0/dqmc_left_forward_prop.f90:18-44, :243) */
+static void k_left(orc_ctx *c, int nf, const zc *T, zc *M)
+{
+    int N = c->ndim;
+    if (!c->lwrapT || !(c->p.rt > 0)) return;
+    if (c->p.kmode == 1) {
+        zgemm('N', 'N', N, N, N, T, N, M, N, 0, c->Gtmp, N);
+        memcpy(M, c->Gtmp, sizeof(zc) * (size_t)N * N);
+        return;
+    }
+    for (int b = 0; b < c->nn_lf; b++) {
+        int ist = b + nf * c->nn_lf;
+        int i1 = c->nnlf[b], i2 = c->nnlist[i1 * c->nn_nf + nf];
+        const zc *u = T + 4 * ist;
+        for (int j = 0; j < N; j++) {
+            zc a = M[IDX(i1, j, N)], bb = M[IDX(i2, j, N)];
+            M[IDX(i1, j, N)] = u[0] * a + u[2] * bb;
+            M[IDX(i2, j, N)] = u[1] * a + u[3] * bb;
+        }
+    }
+}
+/* kinetic, from the right: cols <- cols * T (dqmc_right_backward_prop.f90:18-44, :268) */
+static void k_right(orc_ctx *c, int nf, const zc *T, zc *M)
+{
+    int N = c->ndim;
+    if (!c->lwrapT || !(c->p.rt > 0)) return;
+    if (c->p.kmode == 1) {
+        zgemm('N', 'N', N, N, N, M, N, T, N, 0, c->Gtmp, N);
+        memcpy(M, c->Gtmp, sizeof(zc) * (size_t)N * N);
+        return;
+    }
+    for (int b = 0; b < c->nn_lf; b++) {
+        int ist = b + nf * c->nn_lf;
+        int i1 = c->nnlf[b], i2 = c->nnlist[i1 * c->nn_nf + nf];
+        const zc *u = T + 4 * ist;
+        zc *ca = M + (size_t)i1 * N, *cb = M + (size_t)i2 * N;
+        for (int j = 0; j < N; j++) {
+            zc a = ca[j], bb = cb[j];
+            ca[j] = a * u[0] + bb * u[1];
+            cb[j] = a * u[2] + bb * u[3];
+        }
+    }
+}
+/* the kinetic half-step of one slice, from the left, families in the given order */
+static void khalf_left(orc_ctx *c, const zc *T, zc *M, int reverse)
+{
+    int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
+    for (int q = 0; q < nfam; q++) k_left(c, reverse ? nfam - 1 - q : q, T, M);
+}
+
+/* uconf row/column scalings (dqmc_config_u/dqmc_left_forward_prop.f90:27-32 and siblings) */
+static void v_rows(orc_ctx *c, int nt, const zc *tab, int conjugate, zc *M)
+{
+    int N = c->ndim;
+    if (!c->lwrapu) return;
+    const int32_t *cf = c->conf + (size_t)(nt - 1) * N;
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+            zc b = tab[cf[i] - 1];
+            M[IDX(i, j, N)] *= conjugate ? conj(b) : b;
+        }
+}
+static void v_cols(orc_ctx *c, int nt, const zc *tab, zc *M)
+{
+    int N = c->ndim;
+    if (!c->lwrapu) return;
+    const int32_t *cf = c->conf + (size_t)(nt - 1) * N;
+    for (int j = 0; j < N; j++) {
+        zc b = tab[cf[j] - 1];
+        for (int i = 0; i < N; i++) M[IDX(i, j, N)] *= b;
+    }
+}
+
+/* Bmat_left_forward (src/dqmc_ft_core.f90:70-119): M <- B(nt1..nt2) M */
+static void bmat_left_forward(orc_ctx *c, int nt1, int nt2, zc *M)
+{
+    for (int nt = nt2; nt <= nt1; nt++) {
+        khalf_left(c, c->urt, M, 0);
+        v_rows(c, nt, c->bmat, 0, M);
+        khalf_left(c, c->urt, M, 1);
+    }
+}
+/* Bmat_left_forward_hc (src/dqmc_ft_core.f90:121-171): M <- B(nt1..nt2)^H M */
+static void bmat_left_forward_hc(orc_ctx *c, int nt1, int nt2, zc *M)
+{
+    for (int nt = nt1; nt >= nt2; nt--) {
+        khalf_left(c, c->urt, M, 0);
+        v_rows(c, nt, c->bmat, 1, M);
+        khalf_left(c, c->urt, M, 1);
+    }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* local updates                                                                          */
+/* ------------------------------------------------------------------------------------ */
+static zc zpow(zc z, int n) { zc r = 1; for (int i = 0; i < n; i++) r *= z; return r; }
+
+/* the Metropolis decision shared by all update flavours (dqmc_update_delay.f90:55-76) */
+static int accept_site(orc_ctx *c, zc ratio1, int is, int iflip, int isp, double random, zc *ratiotot_out)
+{
+    zc ratiotot = zpow(ratio1, c->p.nflr);
+    ratiotot = ratiotot * (c->gaml[isp - 1] / c->gaml[is - 1]) * c->phase_ratio[(iflip - 1) + (c->lcomp - 1) * (is - 1)];
+    double ratio_re = creal(ratiotot * c->phase) / creal(c->phase);
+    double ratio_re_abs = ratio_re < 0 ? -ratio_re : ratio_re;
+    *ratiotot_out = ratiotot;
+    return ratio_re_abs > random;
+}
+static void accept_phase(orc_ctx *c, zc ratiotot)
+{
+    double weight = sqrt(creal(ratiotot * conj(ratiotot)));
+    c->phase = c->phase * ratiotot / weight;
+}
+
+/* fast rank-1 update: src/dqmc_config_u/dqmc_update.f90:33-86 */
+static void update_fast(orc_ctx *c, int nt)
+{
+    int N = c->ndim, L = c->lcomp;
+    zc *G = c->G;
+    zc *u = zalloc(N), *v = zalloc(N);
+    double accm = 0;
+    for (int s = 0; s < N; s++) {
+        int32_t *cf = c->conf + (size_t)(nt - 1) * N;
+        int is = cf[s];
+        int iflip = (int)ceil(sfmt_stream(&c->rng) * (L - 1));
+        int isp = (is + iflip - 1) % L + 1;
+        for (int j = 0; j < N; j++) v[j] = -G[IDX(s, j, N)];
+        v[s] = 1.0 - G[IDX(s, s, N)];
+        zc db = c->delta_bmat[(iflip - 1) + (L - 1) * (is - 1)];
+        zc ratio1 = v[s] * db + 1.0;
+        zc sscl = db / ratio1;
+        zc rt;
+        double random;
+        int acc = accept_site(c, ratio1, is, iflip, isp, (random = sfmt_stream(&c->rng)), &rt);
+        if (acc) {
+            accm += 1;
+            accept_phase(c, rt);
+            for (int i = 0; i < N; i++) u[i] = G[IDX(i, s, N)];
+            for (int j = 0; j < N; j++) {
+                zc f = sscl * v[j];
+                for (int i = 0; i < N; i++) G[IDX(i, j, N)] -= u[i] * f;
+            }
+            cf[s] = isp;
+        }
+    }
+    c->accepted += accm;
+    c->tried += N;
+    free(u);
+    free(v);
+}
+
+/* delay update: src/dqmc_config_u/dqmc_update_delay.f90:39-128 */
+static void update_delay(orc_ctx *c, int nt)
+{
+    int N = c->ndim, L = c->lcomp, K = c->nublock;
+    zc *G = c->G;
+    zc *diagg = zalloc(N), *avec = zalloc((size_t)N * K), *bvec = zalloc((size_t)N * K);
+    double accm = 0;
+    int ik = 0;
+    for (int i = 0; i < N; i++) diagg[i] = G[IDX(i, i, N)];
+    int32_t *cf = c->conf + (size_t)(nt - 1) * N;
+    for (int s = 0; s < N; s++) {
+        int is = cf[s];
+        int iflip = (int)ceil(sfmt_stream(&c->rng) * (L - 1));
+        int isp = (is + iflip - 1) % L + 1;
+        zc db = c->delta_bmat[(iflip - 1) + (L - 1) * (is - 1)];
+        zc ratio1 = (1.0 - diagg[s]) * db + 1.0;
+        zc sscl = db / ratio1;
+        zc rt;
+        double random = sfmt_stream(&c->rng);
+        if (accept_site(c, ratio1, is, iflip, isp, random, &rt)) {
+            accm += 1;
+            accept_phase(c, rt);
+            zc *a = avec + (size_t)ik * N, *b = bvec + (size_t)ik * N;
+            for (int i = 0; i < N; i++) { a[i] = G[IDX(i, s, N)]; b[i] = G[IDX(s, i, N)]; }
+            for (int m = 0; m < ik; m++) {
+                zc bs = bvec[IDX(s, m, N)], as = avec[IDX(s, m, N)];
+                const zc *am = avec + (size_t)m * N, *bm = bvec + (size_t)m * N;
+                for (int i = 0; i < N; i++) { a[i] += bs * am[i]; b[i] += as * bm[i]; }
+            }
+            for (int i = 0; i < N; i++) a[i] *= sscl;
+            b[s] -= 1.0;
+            for (int i = 0; i < N; i++) diagg[i] += a[i] * b[i];
+            ik++;
+            cf[s] = isp;
+        }
+        if (ik == K || s == N - 1) {
+            /* G += avec * bvec^T (zgemm 'N','T', always nublock columns, zero padded) */
+            zgemm('N', 'T', N, N, K, avec, N, bvec, N, 1.0, G, N);
+            ik = 0;
+            if (s < N - 1) {
+                for (int i = 0; i < N; i++) diagg[i] = G[IDX(i, i, N)];
+                memset(avec, 0, sizeof(zc) * (size_t)N * K);
+                memset(bvec, 0, sizeof(zc) * (size_t)N * K);
+            }
+        }
+    }
+    c->accepted += accm;
+    c->tried += N;
+    free(diagg); free(avec); free(bvec);
+}
+
+/* submatrix update: src/dqmc_config_u/dqmc_update_submatrix.f90:55-162 */
+static void update_submatrix(orc_ctx *c, int nt)
+{
+    int N = c->ndim, L = c->lcomp, K = c->nublock;
+    zc *G = c->G;
+    int *pvec = (int *)calloc(K, sizeof(int));
+    zc *gi = zalloc((size_t)K * K), *Gcut = zalloc((size_t)N * N);
+    zc *cvec = zalloc(K), *bv = zalloc(K), *v1 = zalloc(K), *v4 = zalloc(K);
+    zc *gm1 = zalloc((size_t)N * K), *gm2 = zalloc((size_t)K * N), *Gmat1 = zalloc((size_t)N * K);
+    memcpy(Gcut, G, sizeof(zc) * (size_t)N * N);
+    double accm = 0;
+    int ik = 0;
+    int32_t *cf = c->conf + (size_t)(nt - 1) * N;
+#define GI(i, m) gi[IDX(i, m, K)]
+    for (int s = 0; s < N; s++) {
+        int is = cf[s];
+        int iflip = (int)ceil(sfmt_stream(&c->rng) * (L - 1));
+        int isp = (is + iflip - 1) % L + 1;
+        for (int i = 0; i < ik; i++) { cvec[i] = G[IDX(s, pvec[i], N)]; bv[i] = G[IDX(pvec[i], s, N)]; }
+        for (int i = 0; i < ik; i++) {
+            zc acc = 0;
+            for (int m = 0; m < ik; m++) acc += cvec[m] * GI(m, i);
+            v1[i] = acc;
+        }
+        zc v = 0;
+        for (int i = 0; i < ik; i++) v += v1[i] * bv[i];
+        zc v2 = v + G[IDX(s, s, N)] - 1.0;
+        zc db = c->delta_bmat[(iflip - 1) + (L - 1) * (is - 1)];
+        zc ratio1 = -v2 * db + 1.0;
+        zc rt;
+        double random = sfmt_stream(&c->rng);
+        if (accept_site(c, ratio1, is, iflip, isp, random, &rt)) {
+            accm += 1;
+            accept_phase(c, rt);
+            pvec[ik] = s;
+            zc v3 = 1.0 / (-v + 1.0 + 1.0 / db - G[IDX(s, s, N)]);
+            GI(ik, ik) = v3;
+            for (int i = 0; i < ik; i++) {
+                zc acc = 0;
+                for (int m = 0; m < ik; m++) acc += GI(i, m) * bv[m];
+                v4[i] = acc;
+            }
+            for (int i = 0; i < ik; i++) { GI(ik, i) = v3 * v1[i]; GI(i, ik) = v4[i] * v3; }
+            for (int m = 0; m < ik; m++)
+                for (int i = 0; i < ik; i++) GI(i, m) += v4[i] * v3 * v1[m];
+            zc f = db / (1.0 + db);
+            for (int i = 0; i < N; i++) Gcut[IDX(i, s, N)] -= f * Gcut[IDX(i, s, N)];
+            for (int i = 0; i < N; i++) gm1[IDX(i, ik, N)] = G[IDX(i, s, N)];
+            ik++;
+            cf[s] = isp;
+        }
+        if (ik == K || s == N - 1) {
+            for (int i = 0; i < ik; i++)
+                for (int j = 0; j < N; j++) gm2[IDX(i, j, K)] = Gcut[IDX(pvec[i], j, N)];
+            if (ik > 0) {
+                zgemm('N', 'N', N, ik, ik, gm1, N, gi, K, 0, Gmat1, N);
+                zgemm('N', 'N', N, N, ik, Gmat1, N, gm2, K, 1.0, Gcut, N);
+            }
+            memcpy(G, Gcut, sizeof(zc) * (size_t)N * N);
+            ik = 0;
+        }
+    }
+#undef GI
+    c->accepted += accm;
+    c->tried += N;
+    free(pvec); free(gi); free(Gcut); free(cvec); free(bv); free(v1); free(v4);
+    free(gm1); free(gm2); free(Gmat1);
+}
+
+static void do_update(orc_ctx *c, int nt)
+{
+    if (c->p.update == 0) update_fast(c, nt);
+    else if (c->p.update == 1) update_delay(c, nt);
+    else update_submatrix(c, nt);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* stabilization and Green's functions                                                    */
+/* ------------------------------------------------------------------------------------ */
+/* s_zmcpt (lib/s_matrix.f90:785-823): columns sorted by decreasing 2-norm (norms computed
+ * once, first maximum wins); jpvt 1-based, A <- A P */
+static void zmcpt(int m, int n, zc *A, int32_t *jpvt)
+{
+    double *vn = (double *)malloc(sizeof(double) * n);
+    for (int k = 0; k < n; k++) { vn[k] = dznrm2(m, A + (size_t)k * m, 1); jpvt[k] = k + 1; }
+    zc *tmp = zalloc(m);
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double best = fabs(vn[k]);
+        for (int j = k + 1; j < n; j++) if (fabs(vn[j]) > best) { best = fabs(vn[j]); p = j; }
+        if (p != k) {
+            memcpy(tmp, A + (size_t)p * m, sizeof(zc) * m);
+            memcpy(A + (size_t)p * m, A + (size_t)k * m, sizeof(zc) * m);
+            memcpy(A + (size_t)k * m, tmp, sizeof(zc) * m);
+            int32_t it = jpvt[p]; jpvt[p] = jpvt[k]; jpvt[k] = it;
+            vn[p] = vn[k];
+        }
+    }
+    free(tmp);
+    free(vn);
+}
+
+/*
+ * first point of the right (0b, n==1) or left (b0, n==nst) UDV chain, column-pivoted QR:
+ *   dqmc_ft_stablize_0b_qr.f90:25-32 and dqmc_ft_stablize_b0_qr.f90:25-32.
+ * Bd holds B*U_prev; outputs U = Q, D = Re diag(R), V = D^-1 triu(R) P^T, logdetQ.
+ */
+static void stabilize_first(orc_ctx *c, zc *Bd, zc *Uo, double *Do, zc *Vo, zc *ldq)
+{
+    int N = c->ndim;
+    int32_t *jp = (int32_t *)malloc(sizeof(int32_t) * N);
+    zc *R = zalloc((size_t)N * N);
+    qr_full(N, N, Bd, Uo, R, jp, ldq, 1);
+    for (int i = 0; i < N; i++) Do[i] = creal(R[IDX(i, i, N)]);
+    /* s_invdiag_d_x_zr then s_zmp: column k of D^-1 R lands in column jpvt(k) */
+    memset(Vo, 0, sizeof(zc) * (size_t)N * N);
+    for (int k = 0; k < N; k++)
+        for (int i = 0; i <= k; i++) Vo[IDX(i, jp[k] - 1, N)] = R[IDX(i, k, N)] / Do[i];
+    free(R);
+    free(jp);
+}
+
+/* the prepivoted branch needs the pivot order, so it is written out separately */
+static void stabilize_prepivot(orc_ctx *c, zc *Bd, const double *Dprev, const zc *Vprev,
+                               zc *Uo, double *Do, zc *Vo, zc *ldq)
+{
+    int N = c->ndim;
+    int32_t *jp = (int32_t *)malloc(sizeof(int32_t) * N), *jd = (int32_t *)malloc(sizeof(int32_t) * N);
+    zc *R = zalloc((size_t)N * N), *PV = zalloc((size_t)N * N);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) Bd[IDX(i, j, N)] *= Dprev[j];           /* s_z_x_diag_d */
+    zmcpt(N, N, Bd, jp);                                                    /* s_zmcpt */
+    qr_full(N, N, Bd, Uo, R, jd, ldq, 0);                                   /* s_zgeQR_logdetQ */
+    for (int i = 0; i < N; i++) Do[i] = creal(R[IDX(i, i, N)]);
+    for (int j = 0; j < N; j++)                                             /* s_invdiag_d_x_zr */
+        for (int i = 0; i < N; i++) R[IDX(i, j, N)] = (i <= j) ? R[IDX(i, j, N)] / Do[i] : 0;
+    for (int k = 0; k < N; k++)                                             /* s_zpm: P^T V_prev */
+        for (int j = 0; j < N; j++) PV[IDX(k, j, N)] = Vprev[IDX(jp[k] - 1, j, N)];
+    zgemm('N', 'N', N, N, N, R, N, PV, N, 0, Vo, N);
+    free(jp); free(jd); free(R); free(PV);
+}
+
+/* s_dvec_min_max (lib/s_matrix.f90:752-783) */
+static void dvec_min_max(int n, const double *d, double *dmax, double *dmin)
+{
+    for (int i = 0; i < n; i++) {
+        if (fabs(d[i]) > 1.0) { dmax[i] = fabs(d[i]); dmin[i] = d[i] >= 0 ? 1.0 : -1.0; }
+        else { dmax[i] = 1.0; dmin[i] = d[i]; }
+    }
+}
+
+/* green_equaltime (src/dqmc_ft_core/green_equaltime.f90:1-107) */
+static int green_equaltime(orc_ctx *c, const zc *ur, const double *dr, const zc *vr, const zc *vl,
+                           const double *dl, const zc *ul, zc *gtt, zc ldqr, zc ldql, zc *logw)
+{
+    int N = c->ndim;
+    double *drmax = malloc(sizeof(double) * N), *drmin = malloc(sizeof(double) * N);
+    double *dlmax = malloc(sizeof(double) * N), *dlmin = malloc(sizeof(double) * N);
+    dvec_min_max(N, dr, drmax, drmin);
+    dvec_min_max(N, dl, dlmax, dlmin);
+    zc *uu = zalloc((size_t)N * N), *vv = zalloc((size_t)N * N), *M = zalloc((size_t)N * N), *B = zalloc((size_t)N * N);
+    zgemm('C', 'N', N, N, N, ur, N, ul, N, 0, uu, N);
+    zgemm('N', 'C', N, N, N, vr, N, vl, N, 0, vv, N);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++)
+            M[IDX(i, j, N)] = uu[IDX(i, j, N)] / (drmax[i] * dlmax[j]) + vv[IDX(i, j, N)] * drmin[i] * dlmin[j];
+    zc zld;
+    int info = zinv_logdet(N, M, &zld);
+    /* s_v_invd_u: B = (ul / dlmax) M^-1 ; s_v_invd_uH: G = (B / drmax) ur^H */
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) uu[IDX(i, j, N)] = ul[IDX(i, j, N)] / dlmax[j];
+    zgemm('N', 'N', N, N, N, uu, N, M, N, 0, B, N);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) B[IDX(i, j, N)] /= drmax[j];
+    zgemm('N', 'C', N, N, N, B, N, ur, N, 0, gtt, N);
+    double r1 = 0, r2 = 0;
+    for (int i = 0; i < N; i++) r1 += log(dlmax[i]);
+    for (int i = 0; i < N; i++) r2 += log(drmax[i]);
+    *logw = -ldql + ldqr + zld + (r1 + r2);
+    free(drmax); free(drmin); free(dlmax); free(dlmin); free(uu); free(vv); free(M); free(B);
+    return info;
+}
+
+/* green_equaltimebb (src/dqmc_ft_core/green_equaltimebb.f90:1-84) */
+static int green_equaltimebb(orc_ctx *c, const zc *ur, const double *dr, const zc *vr, zc *gtt, zc ldqr, zc *logw)
+{
+    int N = c->ndim;
+    double *drmax = malloc(sizeof(double) * N), *drmin = malloc(sizeof(double) * N);
+    dvec_min_max(N, dr, drmax, drmin);
+    zc *M = zalloc((size_t)N * N);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++)
+            M[IDX(i, j, N)] = conj(ur[IDX(j, i, N)]) / drmax[i] + vr[IDX(i, j, N)] * drmin[i];
+    zc zld;
+    int info = zinv_logdet(N, M, &zld);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) M[IDX(i, j, N)] /= drmax[j];
+    zgemm('N', 'C', N, N, N, M, N, ur, N, 0, gtt, N);
+    double r = 0;
+    for (int i = 0; i < N; i++) r += log(drmax[i]);
+    *logw = ldqr + zld + r;
+    free(drmax); free(drmin); free(M);
+    return info;
+}
+
+/* set_phase (src/dqmc_ft_core/set_phase.f90:1-24) */
+static zc set_phase(orc_ctx *c, zc logw)
+{
+    zc w = cexp(CMPLX(0.0, (double)c->p.nflr * cimag(logw)));
+    if (c->lwrapu && c->p.rhub > 0.0)
+        for (size_t k = 0; k < (size_t)c->ltrot * c->ndim; k++) w *= c->phase_tab[c->conf[k] - 1];
+    return w;
+}
+
+/* dqmc_ft_stablize_0b_qr(n) */
+static void stablize_0b(orc_ctx *c, int n)
+{
+    int N = c->ndim;
+    zc *Bd = zalloc((size_t)N * N);
+    memcpy(Bd, c->Ust[n - 1], sizeof(zc) * (size_t)N * N);
+    bmat_left_forward(c, c->wrap_step[2 * (n - 1) + 1], c->wrap_step[2 * (n - 1)], Bd);
+    if (n == 1) stabilize_first(c, Bd, c->Ust[n], c->Dst[n], c->Vst[n], &c->logdetQst[n]);
+    else stabilize_prepivot(c, Bd, c->Dst[n - 1], c->Vst[n - 1], c->Ust[n], c->Dst[n], c->Vst[n], &c->logdetQst[n]);
+    free(Bd);
+}
+
+/* dqmc_ft_stablize_b0_qr(n): writes point n-1 of the left chain */
+static void stablize_b0(orc_ctx *c, int n)
+{
+    int N = c->ndim;
+    zc *Bd = zalloc((size_t)N * N), *U = zalloc((size_t)N * N), *V = zalloc((size_t)N * N);
+    double *D = (double *)calloc(N, sizeof(double));
+    zc ldq;
+    memcpy(Bd, c->Ust[n], sizeof(zc) * (size_t)N * N);
+    bmat_left_forward_hc(c, c->wrap_step[2 * (n - 1) + 1], c->wrap_step[2 * (n - 1)], Bd);
+    if (n == c->nst) stabilize_first(c, Bd, U, D, V, &ldq);
+    else stabilize_prepivot(c, Bd, c->Dst[n], c->Vst[n], U, D, V, &ldq);
+    memcpy(c->Ust[n - 1], U, sizeof(zc) * (size_t)N * N);
+    memcpy(c->Vst[n - 1], V, sizeof(zc) * (size_t)N * N);
+    memcpy(c->Dst[n - 1], D, sizeof(double) * N);
+    c->logdetQst[n - 1] = ldq;
+    free(Bd); free(U); free(V); free(D);
+}
+
+static void set_identity_point(orc_ctx *c, int n)
+{
+    int N = c->ndim;
+    memset(c->Ust[n], 0, sizeof(zc) * (size_t)N * N);
+    memset(c->Vst[n], 0, sizeof(zc) * (size_t)N * N);
+    for (int i = 0; i < N; i++) { c->Ust[n][IDX(i, i, N)] = 1; c->Vst[n][IDX(i, i, N)] = 1; c->Dst[n][i] = 1; }
+    c->logdetQst[n] = 0;
+}
+
+int orc_sweep_start(orc_ctx *c)
+{
+    if (c->nst <= 0) return -1;
+    set_identity_point(c, 0);
+    for (int n = 1; n <= c->nst; n++) stablize_0b(c, n);
+    int info = green_equaltimebb(c, c->Ust[c->nst], c->Dst[c->nst], c->Vst[c->nst], c->G, c->logdetQst[c->nst], &c->logweightf);
+    c->phase = set_phase(c, c->logweightf);
+    return info;
+}
+
+/* equaltime_measure (src/dqmc_measure/equaltime_measure.f90:1-69): sign, density, zkint,
+ * zint_u and the staggered spin structure factor built from zspsm (:283-299) */
+static void measure(orc_ctx *c)
+{
+    int N = c->ndim;
+    const zc *G = c->G;
+    c->nobs += 1;
+    double sg = creal(c->phase) < 0 ? -1.0 : 1.0;
+    zc zphi = (sg / creal(c->phase)) * c->phase;
+    zc zne = 0, zk = 0, zu = 0, ss = 0;
+    for (int i = 0; i < N; i++) {
+        zc gc = 1.0 - G[IDX(i, i, N)];
+        zne += gc;
+        zu += gc * gc;
+    }
+    for (int b = 0; b < c->nn_lf; b++) {
+        int i0 = c->nnlf[b];
+        for (int nf = 0; nf < c->nn_nf; nf++) {
+            int in = c->nnlist[i0 * c->nn_nf + nf];
+            zc gc = -G[IDX(in, i0, N)] + (in == i0 ? 1.0 : 0.0);
+            zk += -2.0 * creal(gc);
+        }
+    }
+    double fac = 1.0 - 1.0 / (double)(c->p.nflr * c->p.nflr);
+    for (int j = 0; j < N; j++) {
+        double sj = (c->p.lattice == 0) ? (((c->cx[j] + c->cy[j]) & 1) ? -1.0 : 1.0) : (c->sub[j] ? -1.0 : 1.0);
+        for (int i = 0; i < N; i++) {
+            double si = (c->p.lattice == 0) ? (((c->cx[i] + c->cy[i]) & 1) ? -1.0 : 1.0) : (c->sub[i] ? -1.0 : 1.0);
+            zc gc = (i == j ? 1.0 : 0.0) - G[IDX(j, i, N)];
+            ss += si * sj * gc * G[IDX(i, j, N)];
+        }
+    }
+    ss *= fac / (double)c->lq;
+    c->obs[0] += sg;
+    c->obs[1] += zne * zphi / (double)c->lq;
+    c->obs[2] += zk * zphi;
+    c->obs[3] += zu * zphi;
+    c->obs[4] += ss * zphi;
+}
+
+static void record_wrap(orc_ctx *c, const zc *gscr, zc logw_tmp, int info)
+{
+    int N = c->ndim;
+    double mx = 0;
+    for (size_t k = 0; k < (size_t)N * N; k++) { double d = cabs(gscr[k] - c->G[k]); if (d > mx) mx = d; }
+    if (mx > c->max_wrap_error) c->max_wrap_error = mx;
+    zc ph = set_phase(c, logw_tmp);
+    double pd = cabs(c->phase - ph);
+    if (pd > c->max_phasediff) c->max_phasediff = pd;
+    c->nchecks += 1;
+    c->sumlog10err += log10(mx > 0 ? mx : 1e-300);
+    if (info == 0) {
+        memcpy(c->G, gscr, sizeof(zc) * (size_t)N * N);
+        c->logweightf = logw_tmp;
+        c->phase = ph;
+    }
+    if (c->trace && c->trace_n < c->trace_cap) {
+        memcpy(c->trace + (size_t)2 * N * N * c->trace_n, c->G, sizeof(zc) * (size_t)N * N);
+        c->trace_n++;
+    }
+}
+
+/* dqmc_ft_sweep_b0 (src/dqmc_ft_core/dqmc_ft_sweep_b0.f90:1-216) */
+int orc_sweep_b0(orc_ctx *c, int lupdate, int lmeasure)
+{
+    int N = c->ndim, nst = c->nst;
+    if (nst <= 0) return -1;
+    zc *gs = zalloc((size_t)N * N);
+    set_identity_point(c, nst);
+    int rc = 0;
+    for (int nt = c->ltrot; nt >= 1; nt--) {
+        if (lmeasure) measure(c);
+        if (c->lwrapT) {
+            int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
+            for (int q = 0; q < nfam; q++) { k_right(c, q, c->urt, c->G); k_left(c, q, c->urtm1, c->G); }
+        }
+        if (c->lwrapu) {
+            if (lupdate) do_update(c, nt);
+            v_cols(c, nt, c->bmat, c->G);
+            v_rows(c, nt, c->bmat_inv, 0, c->G);
+        }
+        if (c->lwrapT) {
+            int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
+            for (int q = nfam - 1; q >= 0; q--) { k_right(c, q, c->urt, c->G); k_left(c, q, c->urtm1, c->G); }
+        }
+        if ((c->iwrap_nt[nt - 1] > 0 || c->nwrap == 1) || nt == 1) {
+            int n = c->iwrap_nt[nt - 1];
+            zc *UR = zalloc((size_t)N * N), *VR = zalloc((size_t)N * N);
+            double *DR = (double *)malloc(sizeof(double) * N);
+            memcpy(UR, c->Ust[n], sizeof(zc) * (size_t)N * N);
+            memcpy(VR, c->Vst[n], sizeof(zc) * (size_t)N * N);
+            memcpy(DR, c->Dst[n], sizeof(double) * N);
+            zc ldqr = c->logdetQst[n];
+            stablize_b0(c, n + 1);
+            zc lw;
+            int info = green_equaltime(c, UR, DR, VR, c->Vst[n], c->Dst[n], c->Ust[n], gs, ldqr, c->logdetQst[n], &lw);
+            record_wrap(c, gs, lw, info);
+            free(UR); free(VR); free(DR);
+            if (info != 0) { rc = info; break; }
+        }
+    }
+    free(gs);
+    return rc;
+}
+
+/* dqmc_ft_sweep_0b (src/dqmc_ft_core/dqmc_ft_sweep_0b.f90:1-288), without the ltau branch */
+int orc_sweep_0b(orc_ctx *c, int lupdate, int lmeasure)
+{
+    int N = c->ndim;
+    if (c->nst <= 0) return -1;
+    zc *gs = zalloc((size_t)N * N);
+    set_identity_point(c, 0);
+    int rc = 0;
+    for (int nt = 1; nt <= c->ltrot; nt++) {
+        if (c->lwrapT) {
+            int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
+            for (int q = 0; q < nfam; q++) { k_left(c, q, c->urt, c->G); k_right(c, q, c->urtm1, c->G); }
+        }
+        if (c->lwrapu) {
+            v_rows(c, nt, c->bmat, 0, c->G);
+            v_cols(c, nt, c->bmat_inv, c->G);
+            if (lupdate) do_update(c, nt);
+        }
+        if (c->lwrapT) {
+            int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
+            for (int q = nfam - 1; q >= 0; q--) { k_left(c, q, c->urt, c->G); k_right(c, q, c->urtm1, c->G); }
+        }
+        if (lmeasure) measure(c);
+        if (c->iwrap_nt[nt] > 0) {
+            int n = c->iwrap_nt[nt];
+            zc *UL = zalloc((size_t)N * N), *VL = zalloc((size_t)N * N);
+            double *DL = (double *)malloc(sizeof(double) * N);
+            memcpy(UL, c->Ust[n], sizeof(zc) * (size_t)N * N);
+            memcpy(VL, c->Vst[n], sizeof(zc) * (size_t)N * N);
+            memcpy(DL, c->Dst[n], sizeof(double) * N);
+            zc ldql = c->logdetQst[n];
+            stablize_0b(c, n);
+            zc lw;
+            int info = green_equaltime(c, c->Ust[n], c->Dst[n], c->Vst[n], VL, DL, UL, gs, c->logdetQst[n], ldql, &lw);
+            record_wrap(c, gs, lw, info);
+            free(UL); free(VL); free(DL);
+            if (info != 0) { rc = info; break; }
+        }
+    }
+    free(gs);
+    return rc;
+}
+
+void orc_get_g(const orc_ctx *c, double *g) { memcpy(g, c->G, sizeof(zc) * (size_t)c->ndim * c->ndim); }
+
+void orc_get_diag(const orc_ctx *c, double *d)
+{
+    d[0] = c->max_wrap_error; d[1] = c->max_phasediff;
+    d[2] = creal(c->phase); d[3] = cimag(c->phase);
+    d[4] = creal(c->logweightf); d[5] = cimag(c->logweightf);
+    d[6] = c->accepted; d[7] = c->tried; d[8] = c->nchecks; d[9] = c->sumlog10err;
+}
+void orc_reset_diag(orc_ctx *c)
+{
+    c->max_wrap_error = c->max_phasediff = c->nchecks = c->sumlog10err = c->accepted = c->tried = 0;
+}
+void orc_get_obs(const orc_ctx *c, double *o)
+{
+    o[0] = c->nobs;
+    for (int k = 0; k < 5; k++) { o[1 + 2 * k] = creal(c->obs[k]); o[2 + 2 * k] = cimag(c->obs[k]); }
+}
+void orc_reset_obs(orc_ctx *c) { c->nobs = 0; for (int k = 0; k < 5; k++) c->obs[k] = 0; }
+void orc_set_trace(orc_ctx *c, double *buf, int cap) { c->trace = buf; c->trace_cap = cap; c->trace_n = 0; }
+int orc_trace_count(const orc_ctx *c) { return c->trace_n; }
+
+int orc_get_table(const orc_ctx *c, int kind, double *out)
+{
+    int L = c->lcomp, N = c->ndim;
+    zc *z = (zc *)out;
+    switch (kind) {
+    case 0:
+        for (int i = 0; i < L; i++) { z[3 * i] = c->bmat[i]; z[3 * i + 1] = c->bmat_inv[i]; z[3 * i + 2] = c->phase_tab[i]; }
+        return 3 * L;
+    case 1: memcpy(z, c->delta_bmat, sizeof(zc) * (L - 1) * L); return (L - 1) * L;
+    case 2: memcpy(z, c->phase_ratio, sizeof(zc) * (L - 1) * L); return (L - 1) * L;
+    case 3: memcpy(out, c->gaml, sizeof(double) * L); return L;
+    case 4: case 5: {
+        size_t n = (c->p.kmode == 1) ? (size_t)N * N : (size_t)c->nn_nf * c->nn_lf * 4;
+        memcpy(z, kind == 4 ? c->urt : c->urtm1, sizeof(zc) * n);
+        return (int)n;
+    }
+    case 6:
+        for (int nf = 0; nf < c->nn_nf; nf++)
+            for (int b = 0; b < c->nn_lf; b++) {
+                int i0 = c->nnlf[b];
+                out[2 * (nf * c->nn_lf + b)] = i0;
+                out[2 * (nf * c->nn_lf + b) + 1] = c->nnlist[i0 * c->nn_nf + nf];
+            }
+        return 2 * c->nn_nf * c->nn_lf;
+    case 7: for (int i = 0; i <= c->ltrot; i++) out[i] = c->iwrap_nt[i]; return c->ltrot + 1;
+    case 8: for (int i = 0; i < 2 * c->nst; i++) out[i] = c->wrap_step[i]; return 2 * c->nst;
+    default: return -1;
+    }
+}

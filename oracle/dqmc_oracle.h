@@ -1,0 +1,85 @@
+/*
+ * dqmc_oracle.h -- CPU restatement of the reference's finite-temperature DQMC sweep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity oracle for the HIP sweep in
+ * sunhubbardmy_amd/.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it, and only as the checker (or the timed CPU baseline), never as the
+ * thing measured or shipped.  The product path never calls into it.
+ *
+ * Every routine in dqmc_oracle.c cites the reference file:line it restates
+ * (paths relative to the YuanyueChen/sunhubbardmy tree).  Storage is column-major
+ * complex128 (re, im interleaved), the layout of the reference's complex(dp) arrays.
+ */
+#ifndef DQMC_ORACLE_H
+#define DQMC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_params {
+    int lattice;    /* 0 = square (src/lattice.f90:792), 1 = honeycomb (:611) */
+    int la, lb;
+    double beta, dtau, rt, mu, rhub, nu;
+    int nflr;
+    int exact_hs;   /* 1 = EXACTHS tables, 0 = general 4-component (dqmc_set_conf.f90:99-113) */
+    int kmode;      /* 0 = checkerboard (BREAKUP_T), 1 = dense exp(-dtau K/2) (also what FFT computes) */
+    int update;     /* 0 = fast, 1 = delay, 2 = submatrix */
+    int nwrap, nublock;
+    int seed;       /* SFMT19937 seed, int32 */
+} orc_params;
+
+typedef struct orc_ctx orc_ctx;
+
+orc_ctx *orc_create(const orc_params *p);
+void orc_destroy(orc_ctx *c);
+
+/* sizes: out[0]=ndim, [1]=lq, [2]=ltrot, [3]=nst, [4]=lcomp, [5]=nn_nf, [6]=nn_lf, [7]=nublock */
+void orc_sizes(const orc_ctx *c, int *out);
+
+/* random initial HS field from the chain's own SFMT stream (dqmc_input_conf.f90:55-63) */
+void orc_init_conf(orc_ctx *c);
+void orc_get_conf(const orc_ctx *c, int32_t *conf);       /* [ltrot][nsites], values 1..lcomp */
+void orc_set_conf(orc_ctx *c, const int32_t *conf);
+
+int orc_sweep_start(orc_ctx *c);                           /* dqmc_ft_sweep_start_0b */
+int orc_sweep_b0(orc_ctx *c, int lupdate, int lmeasure);   /* dqmc_ft_sweep_b0 */
+int orc_sweep_0b(orc_ctx *c, int lupdate, int lmeasure);   /* dqmc_ft_sweep_0b */
+
+void orc_get_g(const orc_ctx *c, double *g);               /* N*N complex, column-major */
+/* diag: [0]=max_wrap_error [1]=max_phasediff [2,3]=phase [4,5]=logweightf
+ *       [6]=accepted [7]=tried [8]=nwrap_checks [9]=sum log10 wrap error */
+void orc_get_diag(const orc_ctx *c, double *d);
+void orc_reset_diag(orc_ctx *c);
+/* obs: [0]=nobs, then complex sums (re,im) of: sign, density/lq, zkint, zint_u, S_stag */
+void orc_get_obs(const orc_ctx *c, double *o);
+void orc_reset_obs(orc_ctx *c);
+
+/* per-stabilization trace of G (after replacement by the scratch G): cap snapshots */
+void orc_set_trace(orc_ctx *c, double *buf, int cap);
+int orc_trace_count(const orc_ctx *c);
+
+/* tables, for cross-checking the product's host-side setup:
+ * kind 0: hs (lcomp x [bmat, bmat_inv, phase]) complex -> 3*lcomp complex
+ * kind 1: delta_bmat (lcomp-1)*lcomp complex; kind 2: phase_ratio (lcomp-1)*lcomp complex
+ * kind 3: gaml (lcomp doubles);  kind 4: K urt (dense N*N or nn_nf*nn_lf*4 complex)
+ * kind 5: K urtm1; kind 6: bonds (int32 pairs as doubles, nn_nf*nn_lf*2)
+ * kind 7: iwrap_nt (ltrot+1 as doubles); kind 8: wrap_step (2*nst as doubles) */
+int orc_get_table(const orc_ctx *c, int kind, double *out);
+
+/* RNG restatement of lib/m_spring.f90 */
+void orc_sfmt_u32(int32_t seed, int n, uint32_t *out);
+void orc_sfmt_stream(int32_t seed, int n, double *out);
+int32_t orc_chain_seed(int32_t chain);    /* abs((c*1981+2008)*951049) in int32 */
+
+/* linear-algebra primitives, exposed for invariant tests */
+/* A (m x n) -> Q (m x m), R (m x n), jpvt (1-based), logdetQ (re,im); pivot=1: column pivoting */
+void orc_qr(int m, int n, const double *a, double *q, double *r, int32_t *jpvt, double *logdetq, int pivot);
+/* inverse with log det via LU (s_inv_logdet_lu_z); returns 0 or the singular column */
+int orc_inv_logdet(int n, double *a, double *logdet);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

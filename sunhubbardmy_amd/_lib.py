@@ -1,0 +1,84 @@
+"""ctypes binding of libsunhip.so (include/sunhip.h).  Loading fails loudly when the library
+has not been built; there is no CPU fallback."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsunhip.so")
+_LIB = None
+
+ip = ctypes.POINTER(ctypes.c_int32)
+dp = ctypes.POINTER(ctypes.c_double)
+
+
+class ShModel(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int), ("lq", ctypes.c_int),
+        ("ltrot", ctypes.c_int), ("nwrap", ctypes.c_int), ("nst", ctypes.c_int),
+        ("iwrap_nt", ip), ("wrap_step", ip),
+        ("lwrapT", ctypes.c_int), ("lwrapu", ctypes.c_int), ("kmode", ctypes.c_int),
+        ("nn_nf", ctypes.c_int), ("nn_lf", ctypes.c_int), ("bonds", ip),
+        ("urt", dp), ("urtm1", dp),
+        ("lcomp", ctypes.c_int), ("nflr", ctypes.c_int),
+        ("bmat", dp), ("bmat_inv", dp), ("phase_arg", dp), ("delta_bmat", dp), ("phase_ratio", dp),
+        ("gaml", dp), ("phase_from_conf", ctypes.c_int), ("stag", dp),
+        ("update", ctypes.c_int), ("nublock", ctypes.c_int), ("seed", ctypes.c_int32),
+    ]
+
+
+EXPORTS = [
+    "sh_create", "sh_destroy", "sh_last_error", "sh_init_conf", "sh_set_conf", "sh_get_conf",
+    "sh_sweep_start_0b", "sh_sweep_b0", "sh_sweep_0b", "sh_push_stage", "sh_pop_stage",
+    "sh_get_g", "sh_get_diag", "sh_reset_diag", "sh_get_obs", "sh_reset_obs", "sh_set_trace",
+    "sh_get_trace", "sh_timers", "sh_synchronize",
+]
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libsunhip.so is not built: run `python -m sunhubbardmy_amd.build` (hipcc, gfx950)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    L.sh_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(ShModel), ctypes.c_int]
+    L.sh_destroy.argtypes = [P]
+    L.sh_last_error.argtypes = [P]
+    L.sh_last_error.restype = ctypes.c_char_p
+    L.sh_init_conf.argtypes = [P]
+    L.sh_set_conf.argtypes = [P, ctypes.c_int, ip]
+    L.sh_get_conf.argtypes = [P, ctypes.c_int, ip]
+    L.sh_sweep_start_0b.argtypes = [P]
+    L.sh_sweep_b0.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.sh_sweep_0b.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.sh_push_stage.argtypes = [P]
+    L.sh_pop_stage.argtypes = [P]
+    L.sh_get_g.argtypes = [P, ctypes.c_int, dp]
+    L.sh_get_diag.argtypes = [P, ctypes.c_int, dp]
+    L.sh_reset_diag.argtypes = [P]
+    L.sh_get_obs.argtypes = [P, ctypes.c_int, dp]
+    L.sh_reset_obs.argtypes = [P]
+    L.sh_set_trace.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.sh_get_trace.argtypes = [P, ctypes.c_int, dp, ip]
+    L.sh_timers.argtypes = [P, ctypes.c_int, dp]
+    L.sh_synchronize.argtypes = [P]
+    for name in EXPORTS:
+        if name != "sh_last_error":
+            getattr(L, name).restype = ctypes.c_int
+    _LIB = L
+    return L
+
+
+def testing_lib():
+    """libsunhip.so with the test hooks of include/sunhip_testing.h bound."""
+    L = lib()
+    if not hasattr(L, "_dbg_bound"):
+        P = ctypes.c_void_p
+        L.sh_dbg_put.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+        L.sh_dbg_get.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+        L.sh_dbg_op.argtypes = [P, ctypes.c_int, ip]
+        for n in ("sh_dbg_put", "sh_dbg_get", "sh_dbg_op"):
+            getattr(L, n).restype = ctypes.c_int
+        L._dbg_bound = True
+    return L

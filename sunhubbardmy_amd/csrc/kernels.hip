@@ -1,0 +1,1378 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the DQMC sweep.  Every kernel is batched over
+// independent Markov chains: blockIdx.z (GEMM, elementwise) or blockIdx.x (one workgroup per
+// chain) selects the chain, whose matrix dimension may differ from its neighbours'.
+#include "sunhip_kernels.h"
+
+#define FULL_DBL 1
+
+// ------------------------------------------------------------------------------------------
+// SFMT19937 on the device (lib/m_spring.f90:187-379): one thread per chain walks the chain's
+// own generator; state = 624 uint32 lanes + index, kept in global memory.
+// ------------------------------------------------------------------------------------------
+__device__ static void sfmt_refill(uint32_t *w)
+{
+    const uint32_t msk[4] = {0xdfffffefu, 0xddfecb7fu, 0xbffaffffu, 0xbffffff6u};
+    uint32_t r1[4], r2[4];
+    for (int l = 0; l < 4; l++) { r1[l] = w[4 * 154 + l]; r2[l] = w[4 * 155 + l]; }
+    for (int i = 0; i < 156; i++) {
+        uint32_t a[4], b[4];
+        int ib = i + 122; if (ib >= 156) ib -= 156;
+        for (int l = 0; l < 4; l++) { a[l] = w[4 * i + l]; b[l] = w[4 * ib + l]; }
+        uint64_t ahi = ((uint64_t)a[3] << 32) | a[2], alo = ((uint64_t)a[1] << 32) | a[0];
+        uint64_t xhi = (ahi << 8) | (alo >> 56), xlo = alo << 8;
+        uint64_t chi = ((uint64_t)r1[3] << 32) | r1[2], clo = ((uint64_t)r1[1] << 32) | r1[0];
+        uint64_t ylo = (clo >> 8) | (chi << 56), yhi = chi >> 8;
+        uint32_t x[4] = {(uint32_t)xlo, (uint32_t)(xlo >> 32), (uint32_t)xhi, (uint32_t)(xhi >> 32)};
+        uint32_t y[4] = {(uint32_t)ylo, (uint32_t)(ylo >> 32), (uint32_t)yhi, (uint32_t)(yhi >> 32)};
+        uint32_t o[4];
+        for (int l = 0; l < 4; l++) o[l] = a[l] ^ x[l] ^ ((b[l] >> 11) & msk[l]) ^ y[l] ^ (r2[l] << 18);
+        for (int l = 0; l < 4; l++) { w[4 * i + l] = o[l]; r1[l] = r2[l]; r2[l] = o[l]; }
+    }
+}
+
+__device__ static double sfmt_next(uint32_t *w, int &idx)
+{
+    if (idx >= 624) { sfmt_refill(w); idx = 0; }
+    return ((double)w[idx++] + 0.5) * (1.0 / 4294967296.0);
+}
+
+extern "C" __global__ void k_sfmt_seed(const DChain *chains, int nchains, const int32_t *seeds)
+{
+    int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= nchains) return;
+    uint32_t *w = chains[ci].rng;
+    const uint32_t parity[4] = {0x00000001u, 0x00000000u, 0x00000000u, 0x13c9e684u};
+    w[0] = (uint32_t)seeds[ci];
+    for (int i = 1; i < 624; i++) w[i] = 1812433253u * (w[i - 1] ^ (w[i - 1] >> 30)) + (uint32_t)i;
+    uint32_t inner = 0;
+    for (int i = 0; i < 4; i++) inner ^= w[i] & parity[i];
+    for (int sh = 16; sh > 0; sh >>= 1) inner ^= inner >> sh;
+    if ((inner & 1u) != 1u) {
+        bool done = false;
+        for (int i = 0; i < 4 && !done; i++) {
+            uint32_t work = 1;
+            for (int j = 0; j < 32; j++) {
+                if (work & parity[i]) { w[i] ^= work; done = true; break; }
+                work <<= 1;
+            }
+        }
+    }
+    w[624] = 624;
+}
+
+// mode 0: fresh HS field conf(i,nt) = ceiling(u*lcomp); mode 1: the pass's 2*N*ltrot uniforms
+extern "C" __global__ void k_sfmt_draw(const DChain *chains, int nchains, int mode)
+{
+    int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= nchains) return;
+    const DChain &c = chains[ci];
+    uint32_t *w = c.rng;
+    int idx = (int)w[624];
+    if (mode == 0) {
+        if (!c.lwrapu) return;
+        size_t tot = (size_t)c.n * c.ltrot;
+        for (size_t k = 0; k < tot; k++) c.conf[k] = (int32_t)ceil(sfmt_next(w, idx) * c.lcomp);
+    } else {
+        size_t tot = (size_t)2 * c.n * c.ltrot;
+        for (size_t k = 0; k < tot; k++) c.unif[k] = sfmt_next(w, idx);
+    }
+    w[624] = (uint32_t)idx;
+}
+
+// ------------------------------------------------------------------------------------------
+// Kinetic and interaction propagation, checkerboard form (BREAKUP_T).
+//   column pass (left products, rows mixed): a workgroup owns CPB full columns in LDS.
+//   row pass (right products, columns mixed): a workgroup owns RPB full rows in LDS.
+// Program bits: 1 = families 0..nn_nf-1, 2 = V scaling, 4 = families nn_nf-1..0,
+//               8 = V scaling before everything.  Several slices can be chained (B products).
+// Bond 2x2 blocks are column-major [T11, T21, T12, T22] (dqmc_left_forward_prop.f90:18-44).
+// ------------------------------------------------------------------------------------------
+#define CPB 4
+#define RPB 8
+
+__device__ __forceinline__ cplx vtab(const DChain &c, int vsel, int is)
+{
+    // vsel 0: bmat, 1: bmat_inv, 2: conj(bmat)
+    cplx b = (vsel == 1) ? c.bmat_inv[is - 1] : c.bmat[is - 1];
+    if (vsel == 2) b.y = -b.y;
+    return b;
+}
+
+__device__ static void cb_left_fams(const DChain &c, cplx *lds, int ncol, const cplx *T, bool rev)
+{
+    const int nf = c.nn_nf, nl = c.nn_lf, n = c.n;
+    for (int q = 0; q < nf; q++) {
+        int f = rev ? nf - 1 - q : q;
+        for (int e = threadIdx.x; e < nl * ncol; e += blockDim.x) {
+            int b = e % nl, cc = e / nl;
+            int ist = f * nl + b;
+            int i1 = c.bonds[2 * ist], i2 = c.bonds[2 * ist + 1];
+            const cplx *t = T + 4 * ist;
+            cplx x = lds[cc * n + i1], y = lds[cc * n + i2];
+            lds[cc * n + i1] = cadd(cmul(t[0], x), cmul(t[2], y));
+            lds[cc * n + i2] = cadd(cmul(t[1], x), cmul(t[3], y));
+        }
+        __syncthreads();
+    }
+}
+
+__device__ static void cb_left_v(const DChain &c, cplx *lds, int ncol, int nt, int vsel)
+{
+    const int n = c.n;
+    const int32_t *cf = c.conf + (size_t)(nt - 1) * n;
+    for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+        int r = e % n;
+        lds[e] = cmul(vtab(c, vsel, cf[r]), lds[e]);
+    }
+    __syncthreads();
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+k_cb_cols(const DChain *chains, int msel, int pt, int ksel, int prog, int vsel, int nt0, int nt1, int ntstep)
+{
+    extern __shared__ cplx lds[];
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    const int c0 = blockIdx.x * CPB;
+    if (c0 >= n) return;
+    const int ncol = min(CPB, n - c0);
+    cplx *M = chain_mat(c, msel, pt);
+    const cplx *T = ksel ? c.ktm1 : c.kt;
+    for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+        int r = e % n, cc = e / n;
+        lds[e] = M[(size_t)r + (size_t)(c0 + cc) * n];
+    }
+    __syncthreads();
+    // slices nt0, nt0+ntstep, ..., nt1 (ntstep = +-1); the count bounds the loop
+    const int nslice = (ntstep > 0 ? nt1 - nt0 : nt0 - nt1) + 1;
+    for (int q = 0, nt = nt0; q < nslice; q++, nt += ntstep) {
+        if (nt < 1 || nt > c.ltrot) break;
+        if ((prog & 8) && c.lwrapu) cb_left_v(c, lds, ncol, nt, vsel);
+        if ((prog & 1) && c.lwrapT) cb_left_fams(c, lds, ncol, T, false);
+        if ((prog & 2) && c.lwrapu) cb_left_v(c, lds, ncol, nt, vsel);
+        if ((prog & 4) && c.lwrapT) cb_left_fams(c, lds, ncol, T, true);
+    }
+    for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+        int r = e % n, cc = e / n;
+        M[(size_t)r + (size_t)(c0 + cc) * n] = lds[e];
+    }
+}
+
+__device__ static void cb_right_fams(const DChain &c, cplx *lds, int nrow, const cplx *T, bool rev)
+{
+    const int nf = c.nn_nf, nl = c.nn_lf;
+    for (int q = 0; q < nf; q++) {
+        int f = rev ? nf - 1 - q : q;
+        for (int e = threadIdx.x; e < nl * RPB; e += blockDim.x) {
+            int rr = e % RPB, b = e / RPB;
+            if (rr >= nrow) continue;
+            int ist = f * nl + b;
+            int i1 = c.bonds[2 * ist], i2 = c.bonds[2 * ist + 1];
+            const cplx *t = T + 4 * ist;
+            cplx x = lds[i1 * RPB + rr], y = lds[i2 * RPB + rr];
+            lds[i1 * RPB + rr] = cadd(cmul(x, t[0]), cmul(y, t[1]));
+            lds[i2 * RPB + rr] = cadd(cmul(x, t[2]), cmul(y, t[3]));
+        }
+        __syncthreads();
+    }
+}
+
+__device__ static void cb_right_v(const DChain &c, cplx *lds, int nt, int vsel)
+{
+    const int n = c.n;
+    const int32_t *cf = c.conf + (size_t)(nt - 1) * n;
+    for (int e = threadIdx.x; e < n * RPB; e += blockDim.x) {
+        int col = e / RPB;
+        lds[e] = cmul(lds[e], vtab(c, vsel, cf[col]));
+    }
+    __syncthreads();
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+k_cb_rows(const DChain *chains, int msel, int pt, int ksel, int prog, int vsel, int nt)
+{
+    extern __shared__ cplx lds[];
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    const int r0 = blockIdx.x * RPB;
+    if (r0 >= n) return;
+    const int nrow = min(RPB, n - r0);
+    cplx *M = chain_mat(c, msel, pt);
+    const cplx *T = ksel ? c.ktm1 : c.kt;
+    for (int e = threadIdx.x; e < n * RPB; e += blockDim.x) {
+        int rr = e % RPB, col = e / RPB;
+        lds[e] = (rr < nrow) ? M[(size_t)(r0 + rr) + (size_t)col * n] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    if ((prog & 8) && c.lwrapu) cb_right_v(c, lds, nt, vsel);
+    if ((prog & 1) && c.lwrapT) cb_right_fams(c, lds, nrow, T, false);
+    if ((prog & 2) && c.lwrapu) cb_right_v(c, lds, nt, vsel);
+    if ((prog & 4) && c.lwrapT) cb_right_fams(c, lds, nrow, T, true);
+    for (int e = threadIdx.x; e < n * RPB; e += blockDim.x) {
+        int rr = e % RPB, col = e / RPB;
+        if (rr < nrow) M[(size_t)(r0 + rr) + (size_t)col * n] = lds[e];
+    }
+}
+
+// exp(V) scalings for the dense-K form: rows (side 0) or columns (side 1), or both (side 2:
+// rows by vsel, columns by vsel2)
+extern "C" __global__ void __launch_bounds__(256)
+k_vscale(const DChain *chains, int msel, int pt, int side, int vsel, int vsel2, int nt)
+{
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    if (!c.lwrapu) return;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    int r = (int)(e % n), col = (int)(e / n);
+    const int32_t *cf = c.conf + (size_t)(nt - 1) * n;
+    cplx *M = chain_mat(c, msel, pt);
+    cplx v = M[e];
+    if (side == 0 || side == 2) v = cmul(vtab(c, vsel, cf[r]), v);
+    if (side == 1) v = cmul(v, vtab(c, vsel, cf[col]));
+    if (side == 2) v = cmul(v, vtab(c, vsel2, cf[col]));
+    M[e] = v;
+}
+
+// ------------------------------------------------------------------------------------------
+// Batched ZGEMM on FP64 MFMA (v_mfma_f64_16x16x4_f64), C = alpha op(A) op(B) + beta C,
+// all N x N of the chain.  Workgroup tile 64x64, 4 waves of 32x32, K chunk 16 staged in LDS.
+// The accumulator holds the transposed tile so that stores are 256-byte column runs.
+// MFMA operand layout (probed on gfx950): A(16x4): lane l holds A[l%16][l/16]; B(4x16): lane l
+// holds B[l/16][l%16]; D(16x16): lane l, element r holds D[l/16 + 4r][l%16].
+// ------------------------------------------------------------------------------------------
+#define GT 64
+#define GK 16
+
+__device__ __forceinline__ double4_t mfma(double a, double b, double4_t c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <int OPA, int OPB>
+__global__ void __launch_bounds__(256)
+k_zgemm(const DChain *chains, int selA, int ptA, int selB, int ptB, int selC, int ptC,
+        double ar, double ai, double br, double bi)
+{
+    __shared__ double sAr[GT][GK + 1], sAi[GT][GK + 1];
+    __shared__ double sBr[GK][GT + 1], sBi[GK][GT + 1];
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    const int tm = blockIdx.x * GT, tn = blockIdx.y * GT;
+    if (tm >= n || tn >= n) return;
+    const cplx *A = chain_mat(c, selA, ptA);
+    const cplx *B = chain_mat(c, selB, ptB);
+    cplx *C = chain_mat(c, selC, ptC);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w & 1, wn = w >> 1;
+    double4_t accr[2][2], acci[2][2];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
+            for (int r = 0; r < 4; r++) { accr[i][j][r] = 0.0; acci[i][j][r] = 0.0; }
+    for (int k0 = 0; k0 < n; k0 += GK) {
+        for (int q = 0; q < 4; q++) {
+            int idx = t + 256 * q;
+            int i, kk;
+            cplx v = make_double2(0.0, 0.0);
+            if (OPA == 0) { i = idx % GT; kk = idx / GT; if (tm + i < n && k0 + kk < n) v = A[(size_t)(tm + i) + (size_t)(k0 + kk) * n]; }
+            else { kk = idx % GK; i = idx / GK; if (tm + i < n && k0 + kk < n) v = cconj(A[(size_t)(k0 + kk) + (size_t)(tm + i) * n]); }
+            sAr[i][kk] = v.x; sAi[i][kk] = v.y;
+            int j;
+            cplx u = make_double2(0.0, 0.0);
+            if (OPB == 0) { kk = idx % GK; j = idx / GK; if (tn + j < n && k0 + kk < n) u = B[(size_t)(k0 + kk) + (size_t)(tn + j) * n]; }
+            else { j = idx % GT; kk = idx / GT; if (tn + j < n && k0 + kk < n) u = cconj(B[(size_t)(tn + j) + (size_t)(k0 + kk) * n]); }
+            sBr[kk][j] = u.x; sBi[kk][j] = u.y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < GK / 4; ks++) {
+            const int kk = 4 * ks + (lane >> 4), l16 = lane & 15;
+            double xr[2], xi[2], yr[2], yi[2];
+            for (int s = 0; s < 2; s++) {
+                // transposed product: X = op(B)^T rows (output columns), Y = op(A)^T cols (output rows)
+                xr[s] = sBr[kk][32 * wn + 16 * s + l16]; xi[s] = sBi[kk][32 * wn + 16 * s + l16];
+                yr[s] = sAr[32 * wm + 16 * s + l16][kk]; yi[s] = sAi[32 * wm + 16 * s + l16][kk];
+            }
+            for (int sn = 0; sn < 2; sn++)
+                for (int sm = 0; sm < 2; sm++) {
+                    accr[sm][sn] = mfma(xr[sn], yr[sm], accr[sm][sn]);
+                    accr[sm][sn] = mfma(-xi[sn], yi[sm], accr[sm][sn]);
+                    acci[sm][sn] = mfma(xr[sn], yi[sm], acci[sm][sn]);
+                    acci[sm][sn] = mfma(xi[sn], yr[sm], acci[sm][sn]);
+                }
+        }
+        __syncthreads();
+    }
+    const bool zb = (br == 0.0 && bi == 0.0);
+    for (int sm = 0; sm < 2; sm++)
+        for (int sn = 0; sn < 2; sn++)
+            for (int r = 0; r < 4; r++) {
+                int row = tm + 32 * wm + 16 * sm + (lane & 15);
+                int col = tn + 32 * wn + 16 * sn + (lane >> 4) + 4 * r;
+                if (row < n && col < n) {
+                    size_t o = (size_t)row + (size_t)col * n;
+                    double vr = accr[sm][sn][r], vi = acci[sm][sn][r];
+                    double outr = ar * vr - ai * vi, outi = ar * vi + ai * vr;
+                    if (!zb) { cplx cv = C[o]; outr += br * cv.x - bi * cv.y; outi += br * cv.y + bi * cv.x; }
+                    C[o] = make_double2(outr, outi);
+                }
+            }
+}
+
+template __global__ void k_zgemm<0, 0>(const DChain *, int, int, int, int, int, int, double, double, double, double);
+template __global__ void k_zgemm<1, 0>(const DChain *, int, int, int, int, int, int, double, double, double, double);
+template __global__ void k_zgemm<0, 1>(const DChain *, int, int, int, int, int, int, double, double, double, double);
+
+// ------------------------------------------------------------------------------------------
+// Workgroup-level rank-k update on MFMA: G(n x n) = (G + A B^T) diag(cs) with A, B n x k
+// column-major (leading dimension n).  Used by the delay flush (cs = 1) and the submatrix flush.
+// Computed as G^T tiles so that every global access is a 16-element column run.
+// ------------------------------------------------------------------------------------------
+__device__ static void wg_rank_k(cplx *G, int n, const cplx *A, const cplx *B, int k, const double *csr,
+                                 const double *csi)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int nt = (n + 15) >> 4;
+    const int l16 = lane & 15, kq = lane >> 4;
+    for (int tile = w; tile < nt * nt; tile += nw) {
+        const int i0 = (tile % nt) * 16, j0 = (tile / nt) * 16;
+        // accumulator = G^T tile: element (p, q) = G[i0+q][j0+p], lane holds p = l/16 + 4r, q = l%16
+        double4_t accr, acci;
+        const int row = i0 + l16;
+        for (int r = 0; r < 4; r++) {
+            int col = j0 + kq + 4 * r;
+            cplx g = (row < n && col < n) ? G[(size_t)row + (size_t)col * n] : make_double2(0.0, 0.0);
+            accr[r] = g.x; acci[r] = g.y;
+        }
+        for (int m0 = 0; m0 < k; m0 += 4) {
+            const int m = m0 + kq;
+            cplx x = make_double2(0.0, 0.0), y = make_double2(0.0, 0.0);
+            if (m < k) {
+                if (j0 + l16 < n) x = B[(size_t)(j0 + l16) + (size_t)m * n];   // X[p][m] = B[j0+p][m]
+                if (i0 + l16 < n) y = A[(size_t)(i0 + l16) + (size_t)m * n];   // Y[m][q] = A[i0+q][m]
+            }
+            accr = mfma(x.x, y.x, accr);
+            accr = mfma(-x.y, y.y, accr);
+            acci = mfma(x.x, y.y, acci);
+            acci = mfma(x.y, y.x, acci);
+        }
+        for (int r = 0; r < 4; r++) {
+            int col = j0 + kq + 4 * r;
+            if (row < n && col < n) {
+                cplx v = make_double2(accr[r], acci[r]);
+                if (csr) v = cmul(v, make_double2(csr[col], csi[col]));
+                G[(size_t)row + (size_t)col * n] = v;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Metropolis decision of one site (dqmc_update_delay.f90:45-76), evaluated identically by
+// every thread of the chain's workgroup.
+// ------------------------------------------------------------------------------------------
+struct SiteDraw {
+    int is, isp, fl;
+    cplx db;
+};
+
+__device__ __forceinline__ bool site_accept(const DChain &c, cplx ratio1, const SiteDraw &d, double u2,
+                                            cplx phase, cplx *ratiotot)
+{
+    cplx rt = make_double2(1.0, 0.0);
+    for (int f = 0; f < c.nflr; f++) rt = cmul(rt, ratio1);
+    rt = cscale(rt, c.gaml[d.isp - 1] / c.gaml[d.is - 1]);
+    rt = cmul(rt, c.phase_ratio[(d.fl - 1) + (c.lcomp - 1) * (d.is - 1)]);
+    double ratio_re = cmul(rt, phase).x / phase.x;
+    double ratio_abs = ratio_re < 0.0 ? -ratio_re : ratio_re;
+    *ratiotot = rt;
+    return ratio_abs > u2;
+}
+
+__device__ __forceinline__ SiteDraw site_draw(const DChain &c, int is, double u1)
+{
+    SiteDraw d;
+    d.is = is;
+    d.fl = (int)ceil(u1 * (c.lcomp - 1));
+    d.isp = (is + d.fl - 1) % c.lcomp + 1;
+    d.db = c.delta_bmat[(d.fl - 1) + (c.lcomp - 1) * (is - 1)];
+    return d;
+}
+
+__device__ __forceinline__ cplx phase_after(cplx phase, cplx rt)
+{
+    double wgt = sqrt(cmul(rt, cconj(rt)).x);
+    return cscale(cmul(phase, rt), 1.0 / wgt);
+}
+
+// delay update of one slice (src/dqmc_config_u/dqmc_update_delay.f90:39-128); one workgroup per
+// chain; uoff = index of this slice's first uniform in the pass buffer
+extern "C" __global__ void __launch_bounds__(256)
+k_update_delay(const DChain *chains, int nt, int slice_in_pass)
+{
+    extern __shared__ cplx sh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n, K = c.nublock;
+    cplx *diagg = sh;                 // n
+    cplx *rowa = sh + n;              // K: avec(s, 0:ik)
+    cplx *rowb = rowa + K;            // K: bvec(s, 0:ik)
+    cplx *G = c.G, *avec = c.avec, *bvec = c.bvec;
+    int32_t *cf = c.conf + (size_t)(nt - 1) * n;
+    const double *u = c.unif + (size_t)2 * n * slice_in_pass;
+    cplx phase = make_double2(c.sc[SC_PHASE], c.sc[SC_PHASE + 1]);
+    for (int r = threadIdx.x; r < n; r += blockDim.x) diagg[r] = G[(size_t)r + (size_t)r * n];
+    __syncthreads();
+    int ik = 0;
+    double accm = 0.0;
+    for (int s = 0; s < n; s++) {
+        const SiteDraw d = site_draw(c, cf[s], u[2 * s]);
+        cplx dg = diagg[s];
+        cplx ratio1 = cadd(cmul(make_double2(1.0 - dg.x, -dg.y), d.db), make_double2(1.0, 0.0));
+        cplx rt;
+        bool acc = site_accept(c, ratio1, d, u[2 * s + 1], phase, &rt);
+        if (acc) {
+            accm += 1.0;
+            phase = phase_after(phase, rt);
+            cplx sscl = cdiv(d.db, ratio1);
+            for (int m = threadIdx.x; m < ik; m += blockDim.x) {
+                rowa[m] = avec[(size_t)s + (size_t)m * n];
+                rowb[m] = bvec[(size_t)s + (size_t)m * n];
+            }
+            __syncthreads();
+            for (int r = threadIdx.x; r < n; r += blockDim.x) {
+                cplx a = G[(size_t)r + (size_t)s * n];
+                cplx b = G[(size_t)s + (size_t)r * n];
+                for (int m = 0; m < ik; m++) {
+                    a = cadd(a, cmul(rowb[m], avec[(size_t)r + (size_t)m * n]));
+                    b = cadd(b, cmul(rowa[m], bvec[(size_t)r + (size_t)m * n]));
+                }
+                a = cmul(a, sscl);
+                if (r == s) b.x -= 1.0;
+                diagg[r] = cadd(diagg[r], cmul(a, b));
+                avec[(size_t)r + (size_t)ik * n] = a;
+                bvec[(size_t)r + (size_t)ik * n] = b;
+            }
+            ik++;
+            if (threadIdx.x == 0) cf[s] = d.isp;
+            __syncthreads();
+        }
+        if (ik == K || s == n - 1) {
+            if (ik > 0) wg_rank_k(G, n, avec, bvec, ik, nullptr, nullptr);
+            __syncthreads();
+            ik = 0;
+            if (s < n - 1)
+                for (int r = threadIdx.x; r < n; r += blockDim.x) diagg[r] = G[(size_t)r + (size_t)r * n];
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        c.sc[SC_PHASE] = phase.x;
+        c.sc[SC_PHASE + 1] = phase.y;
+        c.sc[SC_ACC] += accm;
+        c.sc[SC_TRIED] += n;
+    }
+}
+
+// submatrix update of one slice (src/dqmc_config_u/dqmc_update_submatrix.f90:55-162).
+// Gcuta is kept implicitly as G diag(cs); the flush writes (G + (G(:,p) Gamma^-1) G(p,:)) diag(cs).
+// Scratch: avec = G(:,p) columns, bvec = Gmat1, W1 = G(p,:)^T, rwork = cs.
+extern "C" __global__ void __launch_bounds__(256)
+k_update_submatrix(const DChain *chains, int nt, int slice_in_pass)
+{
+    extern __shared__ cplx sh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n, K = c.nublock;
+    cplx *gi = sh;                    // K*K Gamma^-1, column-major
+    cplx *cv = gi + K * K;            // K
+    cplx *bv = cv + K;                // K
+    cplx *v1 = bv + K;                // K
+    cplx *v4 = v1 + K;                // K
+    __shared__ int pvec[256];
+    __shared__ cplx red[256];
+    cplx *G = c.G, *gm1 = c.avec, *gmat1 = c.bvec, *gm2t = c.W1;
+    double *csr = c.rwork, *csi = c.rwork + n;
+    int32_t *cf = c.conf + (size_t)(nt - 1) * n;
+    const double *u = c.unif + (size_t)2 * n * slice_in_pass;
+    cplx phase = make_double2(c.sc[SC_PHASE], c.sc[SC_PHASE + 1]);
+    for (int r = threadIdx.x; r < n; r += blockDim.x) { csr[r] = 1.0; csi[r] = 0.0; }
+    __syncthreads();
+    int ik = 0;
+    double accm = 0.0;
+    for (int s = 0; s < n; s++) {
+        const SiteDraw d = site_draw(c, cf[s], u[2 * s]);
+        for (int i = threadIdx.x; i < ik; i += blockDim.x) {
+            cv[i] = G[(size_t)s + (size_t)pvec[i] * n];
+            bv[i] = G[(size_t)pvec[i] + (size_t)s * n];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < ik; i += blockDim.x) {
+            cplx a = make_double2(0.0, 0.0);
+            for (int m = 0; m < ik; m++) a = cadd(a, cmul(cv[m], gi[m + i * K]));
+            v1[i] = a;
+        }
+        __syncthreads();
+        cplx part = make_double2(0.0, 0.0);
+        if (threadIdx.x == 0)
+            for (int i = 0; i < ik; i++) part = cadd(part, cmul(v1[i], bv[i]));
+        if (threadIdx.x == 0) red[0] = part;
+        __syncthreads();
+        cplx v = red[0];
+        cplx gss = G[(size_t)s + (size_t)s * n];
+        cplx v2 = make_double2(v.x + gss.x - 1.0, v.y + gss.y);
+        cplx ratio1 = csub(make_double2(1.0, 0.0), cmul(v2, d.db));
+        cplx rt;
+        bool acc = site_accept(c, ratio1, d, u[2 * s + 1], phase, &rt);
+        __syncthreads();
+        if (acc) {
+            accm += 1.0;
+            phase = phase_after(phase, rt);
+            cplx den = make_double2(-v.x + 1.0 - gss.x, -v.y - gss.y);
+            den = cadd(den, cinv(d.db));
+            cplx v3 = cinv(den);
+            for (int i = threadIdx.x; i < ik; i += blockDim.x) {
+                cplx a = make_double2(0.0, 0.0);
+                for (int m = 0; m < ik; m++) a = cadd(a, cmul(gi[i + m * K], bv[m]));
+                v4[i] = a;
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < ik * ik; e += blockDim.x) {
+                int i = e % ik, m = e / ik;
+                gi[i + m * K] = cadd(gi[i + m * K], cmul(cmul(v4[i], v3), v1[m]));
+            }
+            for (int i = threadIdx.x; i < ik; i += blockDim.x) {
+                gi[ik + i * K] = cmul(v3, v1[i]);
+                gi[i + ik * K] = cmul(v4[i], v3);
+            }
+            if (threadIdx.x == 0) {
+                gi[ik + ik * K] = v3;
+                pvec[ik] = s;
+                cf[s] = d.isp;
+                cplx f = cdiv(make_double2(1.0, 0.0), cadd(make_double2(1.0, 0.0), d.db));
+                cplx cs = cmul(make_double2(csr[s], csi[s]), f);
+                csr[s] = cs.x; csi[s] = cs.y;
+            }
+            for (int r = threadIdx.x; r < n; r += blockDim.x) gm1[(size_t)r + (size_t)ik * n] = G[(size_t)r + (size_t)s * n];
+            ik++;
+            __syncthreads();
+        }
+        if (ik == K || s == n - 1) {
+            if (ik > 0) {
+                // Gmat1 = G(:,p) Gamma^-1 ; gm2t(j, i) = G(p_i, j)
+                for (int r = threadIdx.x; r < n; r += blockDim.x) {
+                    for (int j = 0; j < ik; j++) {
+                        cplx a = make_double2(0.0, 0.0);
+                        for (int m = 0; m < ik; m++) a = cadd(a, cmul(gm1[(size_t)r + (size_t)m * n], gi[m + j * K]));
+                        gmat1[(size_t)r + (size_t)j * n] = a;
+                    }
+                    for (int i = 0; i < ik; i++) gm2t[(size_t)r + (size_t)i * n] = G[(size_t)pvec[i] + (size_t)r * n];
+                }
+                __syncthreads();
+                wg_rank_k(G, n, gmat1, gm2t, ik, csr, csi);
+            } else {
+                for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+                    int col = (int)(e / n);
+                    G[e] = cmul(G[e], make_double2(csr[col], csi[col]));
+                }
+            }
+            __syncthreads();
+            for (int r = threadIdx.x; r < n; r += blockDim.x) { csr[r] = 1.0; csi[r] = 0.0; }
+            ik = 0;
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        c.sc[SC_PHASE] = phase.x;
+        c.sc[SC_PHASE + 1] = phase.y;
+        c.sc[SC_ACC] += accm;
+        c.sc[SC_TRIED] += n;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Equal-time measurement (src/dqmc_measure/equaltime_measure.f90:1-69, zkint :71-114,
+// zint_u :116-125, zspsm :283-299 -> Fourier component at the staggering vector).
+// One workgroup per chain; the G(i,j) G(j,i) sum is taken over 32x32 tile pairs staged in LDS.
+// ------------------------------------------------------------------------------------------
+__device__ static double block_sum(double v, double *red)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < nw; i++) s += red[i];
+    return s;
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+k_measure(const DChain *chains)
+{
+    __shared__ cplx ta[32][33], tb[32][33];
+    __shared__ double red[16];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    const cplx *G = c.G;
+    double zne_r = 0, zne_i = 0, zu_r = 0, zu_i = 0, zk = 0, trr = 0, tri = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        cplx g = G[(size_t)i + (size_t)i * n];
+        cplx gc = make_double2(1.0 - g.x, -g.y);
+        zne_r += gc.x; zne_i += gc.y;
+        cplx g2 = cmul(gc, gc);
+        zu_r += g2.x; zu_i += g2.y;
+        trr += g.x; tri += g.y;
+    }
+    const int nb = c.nn_nf * c.nn_lf;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        int i0 = c.bonds[2 * b], i1 = c.bonds[2 * b + 1];
+        // gfc(i0,i1) = delta - gf(i1,i0); kinetic term -2 Re(gfc(i0,i1))
+        double gcr = (i0 == i1 ? 1.0 : 0.0) - G[(size_t)i1 + (size_t)i0 * n].x;
+        zk += -2.0 * gcr;
+    }
+    // S = sum_ij s_i s_j G(i,j) G(j,i)
+    double ssr = 0, ssi = 0;
+    const int ntile = (n + 31) / 32;
+    for (int bi = 0; bi < ntile; bi++)
+        for (int bj = bi; bj < ntile; bj++) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+                int q = e % 32, p = e / 32;
+                int r1 = bi * 32 + q, c1 = bj * 32 + p;
+                ta[q][p] = (r1 < n && c1 < n) ? G[(size_t)r1 + (size_t)c1 * n] : make_double2(0.0, 0.0);
+                int r2 = bj * 32 + q, c2 = bi * 32 + p;
+                tb[q][p] = (r2 < n && c2 < n) ? G[(size_t)r2 + (size_t)c2 * n] : make_double2(0.0, 0.0);
+            }
+            __syncthreads();
+            double wgt = (bi == bj) ? 1.0 : 2.0;
+            for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+                int q = e % 32, p = e / 32;
+                int i = bi * 32 + q, j = bj * 32 + p;
+                if (i < n && j < n) {
+                    cplx pr = cmul(ta[q][p], tb[p][q]);   // G(i,j) G(j,i)
+                    double s = c.stag[i] * c.stag[j] * wgt;
+                    ssr += s * pr.x; ssi += s * pr.y;
+                }
+            }
+        }
+    zne_r = block_sum(zne_r, red); zne_i = block_sum(zne_i, red);
+    zu_r = block_sum(zu_r, red); zu_i = block_sum(zu_i, red);
+    zk = block_sum(zk, red);
+    trr = block_sum(trr, red); tri = block_sum(tri, red);
+    ssr = block_sum(ssr, red); ssi = block_sum(ssi, red);
+    if (threadIdx.x == 0) {
+        double *sc = c.sc;
+        cplx phase = make_double2(sc[SC_PHASE], sc[SC_PHASE + 1]);
+        double sg = phase.x < 0 ? -1.0 : 1.0;
+        cplx zphi = cscale(phase, sg / phase.x);
+        double fac = 1.0 - 1.0 / (double)(c.nflr * c.nflr);
+        cplx ss = cscale(make_double2(trr - ssr, tri - ssi), fac / (double)c.lq);
+        cplx v[5];
+        v[0] = make_double2(sg, 0.0);
+        v[1] = cmul(cscale(make_double2(zne_r, zne_i), 1.0 / (double)c.lq), zphi);
+        v[2] = cmul(make_double2(zk, 0.0), zphi);
+        v[3] = cmul(make_double2(zu_r, zu_i), zphi);
+        v[4] = cmul(ss, zphi);
+        sc[SC_NOBS] += 1.0;
+        for (int k = 0; k < 5; k++) { sc[SC_OBS + 2 * k] += v[k].x; sc[SC_OBS + 2 * k + 1] += v[k].y; }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Householder QR (zgeqp3 / zgeqrf semantics, LAPACK zlarfg reflectors), one workgroup per
+// chain, in place on matrix `sel`; tau -> c.tau, jpvt -> c.jpvt (1-based), log det Q -> ldq.
+// pivot = 1: column pivoting on downdated partial norms with the tol3z recompute rule.
+// ------------------------------------------------------------------------------------------
+__device__ static void block_argmax(double v, int idx, double *rv, int *ri, double *outv, int *outi)
+{
+    // first index of the largest value
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) {
+        double v2 = __shfl_xor(v, o, 64);
+        int i2 = __shfl_xor(idx, o, 64);
+        if (v2 > v || (v2 == v && i2 < idx)) { v = v2; idx = i2; }
+    }
+    __syncthreads();
+    if (lane == 0) { rv[w] = v; ri[w] = idx; }
+    __syncthreads();
+    double bv = rv[0];
+    int bi = ri[0];
+    for (int i = 1; i < nw; i++)
+        if (rv[i] > bv || (rv[i] == bv && ri[i] < bi)) { bv = rv[i]; bi = ri[i]; }
+    *outv = bv;
+    *outi = bi;
+}
+
+extern "C" __global__ void __launch_bounds__(1024)
+k_qr(const DChain *chains, int sel, int pivot, int out_pt)
+{
+    extern __shared__ double qsh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    double *vn1 = qsh, *vn2 = qsh + n;
+    cplx *v = (cplx *)(qsh + 2 * n);          // n
+    __shared__ double rv[16];
+    __shared__ int ri[16];
+    __shared__ cplx s_tau;
+    cplx *A = chain_mat(c, sel, 0);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    // the unpivoted call keeps the prepivot order that k_prepivot left in jpvt
+    if (pivot)
+        for (int j = threadIdx.x; j < n; j += blockDim.x) c.jpvt[j] = j + 1;
+    if (pivot) {
+        for (int j = w; j < n; j += nw) {
+            double s = 0.0;
+            for (int r = lane; r < n; r += 64) s += cabs2(A[(size_t)r + (size_t)j * n]);
+            s = wave_sum(s);
+            if (lane == 0) { vn1[j] = sqrt(s); vn2[j] = vn1[j]; }
+        }
+    }
+    __syncthreads();
+    const double tol3z = sqrt(2.220446049250313e-16);
+    for (int i = 0; i < n; i++) {
+        if (pivot) {
+            double bv = -1.0;
+            int bi = i;
+            for (int j = i + threadIdx.x; j < n; j += blockDim.x) {
+                double a = fabs(vn1[j]);
+                if (a > bv) { bv = a; bi = j; }
+            }
+            double mv;
+            int p;
+            block_argmax(bv, bi, rv, ri, &mv, &p);
+            if (p != i) {
+                for (int r = threadIdx.x; r < n; r += blockDim.x) {
+                    cplx t = A[(size_t)r + (size_t)p * n];
+                    A[(size_t)r + (size_t)p * n] = A[(size_t)r + (size_t)i * n];
+                    A[(size_t)r + (size_t)i * n] = t;
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    int32_t it = c.jpvt[p]; c.jpvt[p] = c.jpvt[i]; c.jpvt[i] = it;
+                    vn1[p] = vn1[i];
+                    vn2[p] = vn2[i];
+                }
+            }
+            __syncthreads();
+        }
+        // reflector of column i (zlarfg), wave 0
+        if (w == 0) {
+            cplx *col = A + (size_t)i * n;
+            double s = 0.0;
+            for (int r = i + 1 + lane; r < n; r += 64) s += cabs2(col[r]);
+            s = wave_sum(s);
+            cplx alpha = col[i];
+            double xnorm = sqrt(s);
+            cplx tau = make_double2(0.0, 0.0);
+            cplx scal = make_double2(0.0, 0.0);
+            double beta = 0.0;
+            bool triv = (xnorm == 0.0 && alpha.y == 0.0);
+            if (!triv) {
+                double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + s);
+                beta = alpha.x >= 0 ? -nrm : nrm;
+                tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+                scal = cinv(make_double2(alpha.x - beta, alpha.y));
+            }
+            for (int r = i + 1 + lane; r < n; r += 64) {
+                cplx x = triv ? col[r] : cmul(col[r], scal);
+                col[r] = x;
+                v[r] = x;
+            }
+            if (lane == 0) {
+                if (!triv) col[i] = make_double2(beta, 0.0);
+                v[i] = make_double2(1.0, 0.0);
+                s_tau = tau;
+                c.tau[i] = tau;
+            }
+        }
+        __syncthreads();
+        const cplx tc = cconj(s_tau);
+        // apply H^H = I - conj(tau) v v^H to the trailing columns, one wave per column
+        for (int j = i + 1 + w; j < n; j += nw) {
+            cplx *a = A + (size_t)j * n;
+            double sr = 0.0, si = 0.0;
+            for (int r = i + lane; r < n; r += 64) {
+                cplx vv = v[r], aa = a[r];
+                sr += vv.x * aa.x + vv.y * aa.y;     // conj(v) a
+                si += vv.x * aa.y - vv.y * aa.x;
+            }
+            sr = wave_sum(sr);
+            si = wave_sum(si);
+            cplx f = cmul(tc, make_double2(sr, si));
+            double nrm2 = 0.0;
+            cplx ai_new = make_double2(0.0, 0.0);
+            for (int r = i + lane; r < n; r += 64) {
+                cplx aa = csub(a[r], cmul(v[r], f));
+                a[r] = aa;
+                if (r > i) nrm2 += cabs2(aa);
+                else ai_new = aa;   // r == i happens on lane 0 only
+            }
+            if (pivot) {
+                nrm2 = wave_sum(nrm2);
+                if (lane == 0 && vn1[j] != 0.0) {
+                    double t = sqrt(cabs2(ai_new)) / vn1[j];
+                    t = 1.0 - t * t;
+                    if (t < 0.0) t = 0.0;
+                    double t2 = t * (vn1[j] / vn2[j]) * (vn1[j] / vn2[j]);
+                    if (t2 <= tol3z) {
+                        vn1[j] = (i < n - 1) ? sqrt(nrm2) : 0.0;
+                        vn2[j] = vn1[j];
+                    } else {
+                        vn1[j] *= sqrt(t);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cplx s = make_double2(0.0, 0.0);
+        for (int i = 0; i < n; i++) {
+            cplx t = c.tau[i];
+            if (t.x != 0.0 || t.y != 0.0) {
+                cplx q = cdiv(cscale(t, 2.0 * t.x), cmul(t, cconj(t)));
+                s = cadd(s, clog_(make_double2(1.0 - q.x, -q.y)));
+            }
+        }
+        c.ldq[out_pt] = s;
+    }
+}
+
+// Q = H_1 ... H_n from the reflectors in matrix `sel` (zung2r), written to chain point U[pt]
+// (dest 0) or Us (dest 1)
+extern "C" __global__ void __launch_bounds__(1024)
+k_formq(const DChain *chains, int sel, int out_pt)
+{
+    extern __shared__ double qsh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    cplx *v = (cplx *)qsh;
+    const cplx *A = chain_mat(c, sel, 0);
+    cplx *Q = c.U + (size_t)n * n * out_pt;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) Q[e] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int i = n - 1; i >= 0; i--) {
+        const cplx tau = c.tau[i];
+        for (int r = i + threadIdx.x; r < n; r += blockDim.x)
+            v[r] = (r == i) ? make_double2(1.0, 0.0) : A[(size_t)r + (size_t)i * n];
+        __syncthreads();
+        for (int j = i + 1 + w; j < n; j += nw) {
+            cplx *q = Q + (size_t)j * n;
+            double sr = 0.0, si = 0.0;
+            for (int r = i + lane; r < n; r += 64) {
+                cplx vv = v[r], qq = q[r];
+                sr += vv.x * qq.x + vv.y * qq.y;
+                si += vv.x * qq.y - vv.y * qq.x;
+            }
+            sr = wave_sum(sr);
+            si = wave_sum(si);
+            cplx f = cmul(tau, make_double2(sr, si));
+            for (int r = i + lane; r < n; r += 64) q[r] = csub(q[r], cmul(v[r], f));
+        }
+        __syncthreads();
+        for (int r = i + threadIdx.x; r < n; r += blockDim.x) {
+            cplx val = (r == i) ? make_double2(1.0 - tau.x, -tau.y)
+                                : make_double2(-(tau.x * v[r].x - tau.y * v[r].y), -(tau.x * v[r].y + tau.y * v[r].x));
+            Q[(size_t)r + (size_t)i * n] = val;
+        }
+        __syncthreads();
+    }
+}
+
+// D = Re diag(R), V = D^-1 triu(R) P^T (first point, column-pivoted QR) -> D[pt], V[pt]
+extern "C" __global__ void __launch_bounds__(256)
+k_v_from_qrcp(const DChain *chains, int sel, int pt)
+{
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    int i = (int)(e % n), k = (int)(e / n);
+    const cplx *R = chain_mat(c, sel, 0);
+    double di = R[(size_t)i + (size_t)i * n].x;
+    cplx val = (i <= k) ? cscale(R[(size_t)i + (size_t)k * n], 1.0 / di) : make_double2(0.0, 0.0);
+    c.V[(size_t)n * n * pt + (size_t)i + (size_t)(c.jpvt[k] - 1) * n] = val;
+    if (k == 0) c.D[(size_t)n * pt + i] = di;
+}
+
+// prepivot: Btmp = Bd diag(Dprev); columns sorted by decreasing norm (s_zmcpt); the sorted
+// matrix goes to W2, the order to jpvt.  One workgroup per chain.
+extern "C" __global__ void __launch_bounds__(1024)
+k_prepivot(const DChain *chains, int dpt)
+{
+    extern __shared__ double psh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    double *vn = psh;
+    int *jp = (int *)(psh + n);
+    __shared__ double rv[16];
+    __shared__ int ri[16];
+    const double *Dp = c.D + (size_t)n * dpt;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    cplx *B = c.W1;
+    for (int j = w; j < n; j += nw) {
+        double s = 0.0;
+        double dj = Dp[j];
+        for (int r = lane; r < n; r += 64) {
+            cplx b = cscale(B[(size_t)r + (size_t)j * n], dj);
+            B[(size_t)r + (size_t)j * n] = b;
+            s += cabs2(b);
+        }
+        s = wave_sum(s);
+        if (lane == 0) { vn[j] = sqrt(s); jp[j] = j + 1; }
+    }
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+        double bv = -1.0;
+        int bi = k;
+        for (int j = k + threadIdx.x; j < n; j += blockDim.x)
+            if (fabs(vn[j]) > bv) { bv = fabs(vn[j]); bi = j; }
+        double mv;
+        int p;
+        block_argmax(bv, bi, rv, ri, &mv, &p);
+        if (threadIdx.x == 0 && p != k) {
+            int it = jp[p]; jp[p] = jp[k]; jp[k] = it;
+            vn[p] = vn[k];
+        }
+        __syncthreads();
+    }
+    for (int j = threadIdx.x; j < n; j += blockDim.x) c.jpvt[j] = jp[j];
+    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+        int r = (int)(e % n), k = (int)(e / n);
+        c.W2[e] = B[(size_t)r + (size_t)(jp[k] - 1) * n];
+    }
+}
+
+// after the prepivoted QR of W2: D[pt] = Re diag R, W3 = D^-1 triu(R), W4 = P^T V[vpt]
+extern "C" __global__ void __launch_bounds__(256)
+k_after_qr(const DChain *chains, int pt, int vsel, int vpt)
+{
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    int i = (int)(e % n), j = (int)(e / n);
+    const cplx *R = c.W2;
+    double di = R[(size_t)i + (size_t)i * n].x;
+    c.W3[e] = (i <= j) ? cscale(R[e], 1.0 / di) : make_double2(0.0, 0.0);
+    const cplx *Vp = chain_mat(c, vsel, vpt);
+    c.W4[e] = Vp[(size_t)(c.jpvt[i] - 1) + (size_t)j * n];
+    if (j == 0) c.D[(size_t)n * pt + i] = di;
+}
+
+// ------------------------------------------------------------------------------------------
+// green_equaltime / green_equaltimebb pieces (src/dqmc_ft_core/green_equaltime.f90:1-107,
+// green_equaltimebb.f90:1-84).  Right point R = (U[ptR], D[ptR], V[ptR]) or the saved point;
+// left point likewise.  rsel/lsel: 0 = chain point, 1 = saved copy (Us, Ds, Vs).
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ double dmax_of(double d) { return fabs(d) > 1.0 ? fabs(d) : 1.0; }
+__device__ __forceinline__ double dmin_of(double d) { return fabs(d) > 1.0 ? (d >= 0 ? 1.0 : -1.0) : d; }
+
+__device__ __forceinline__ const double *dvec_of(const DChain &c, int saved, int pt)
+{
+    return saved ? c.Ds : c.D + (size_t)c.n * pt;
+}
+
+// M = uu/(drmax dlmax) + vv drmin dlmin into W3 (uu in W1, vv in W2); bb = 1: M = UR^H/drmax + VR drmin
+extern "C" __global__ void __launch_bounds__(256)
+k_green_m(const DChain *chains, int bb, int rsaved, int rpt, int lsaved, int lpt)
+{
+    __shared__ cplx tile[32][33];
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    const int bx = blockIdx.x % ((n + 31) / 32), by = blockIdx.x / ((n + 31) / 32);
+    if (by * 32 >= n) return;
+    const double *dr = dvec_of(c, rsaved, rpt);
+    if (!bb) {
+        const double *dl = dvec_of(c, lsaved, lpt);
+        for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+            int i = bx * 32 + (e % 32), j = by * 32 + (e / 32);
+            if (i < n && j < n) {
+                size_t o = (size_t)i + (size_t)j * n;
+                double a = 1.0 / (dmax_of(dr[i]) * dmax_of(dl[j]));
+                double b = dmin_of(dr[i]) * dmin_of(dl[j]);
+                c.W3[o] = cadd(cscale(c.W1[o], a), cscale(c.W2[o], b));
+            }
+        }
+    } else {
+        const cplx *ur = rsaved ? c.Us : c.U + (size_t)n * n * rpt;
+        const cplx *vr = rsaved ? c.Vs : c.V + (size_t)n * n * rpt;
+        // tile of UR^H: element (i, j) = conj(UR(j, i)); read UR block (by, bx) coalesced
+        for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+            int q = e % 32, p = e / 32;
+            int r = by * 32 + q, cc = bx * 32 + p;
+            tile[q][p] = (r < n && cc < n) ? ur[(size_t)r + (size_t)cc * n] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+            int q = e % 32, p = e / 32;
+            int i = bx * 32 + q, j = by * 32 + p;
+            if (i < n && j < n) {
+                size_t o = (size_t)i + (size_t)j * n;
+                cplx uh = cconj(tile[p][q]);
+                c.W3[o] = cadd(cscale(uh, 1.0 / dmax_of(dr[i])), cscale(vr[o], dmin_of(dr[i])));
+            }
+        }
+    }
+}
+
+// right-hand side X = DRmax^-1 UR^H into W4 (tiled transpose)
+extern "C" __global__ void __launch_bounds__(256)
+k_green_rhs(const DChain *chains, int rsaved, int rpt)
+{
+    __shared__ cplx tile[32][33];
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    const int nb = (n + 31) / 32;
+    const int bx = blockIdx.x % nb, by = blockIdx.x / nb;
+    if (by >= nb) return;
+    const double *dr = dvec_of(c, rsaved, rpt);
+    const cplx *ur = rsaved ? c.Us : c.U + (size_t)n * n * rpt;
+    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+        int q = e % 32, p = e / 32;
+        int r = by * 32 + q, cc = bx * 32 + p;
+        tile[q][p] = (r < n && cc < n) ? ur[(size_t)r + (size_t)cc * n] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+        int q = e % 32, p = e / 32;
+        int i = bx * 32 + q, j = by * 32 + p;
+        if (i < n && j < n) c.W4[(size_t)i + (size_t)j * n] = cscale(cconj(tile[p][q]), 1.0 / dmax_of(dr[i]));
+    }
+}
+
+// W1 = UL diag(1/dlmax)
+extern "C" __global__ void __launch_bounds__(256)
+k_green_ulscale(const DChain *chains, int lsaved, int lpt)
+{
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    int j = (int)(e / n);
+    const double *dl = dvec_of(c, lsaved, lpt);
+    const cplx *ul = lsaved ? c.Us : c.U + (size_t)n * n * lpt;
+    c.W1[e] = cscale(ul[e], 1.0 / dmax_of(dl[j]));
+}
+
+// LU with partial pivoting (zgetrf, pivot = largest |Re|+|Im|) of W3 in place, one workgroup per
+// chain; log det with the imaginary part folded into (-pi, pi] (s_inv_logdet_lu_z) -> SC_ZLD
+extern "C" __global__ void __launch_bounds__(1024)
+k_lu(const DChain *chains)
+{
+    extern __shared__ cplx lsh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    cplx *lcol = lsh;           // n
+    cplx *lrow = lsh + n;       // n
+    __shared__ double rv[16];
+    __shared__ int ri[16];
+    __shared__ int s_piv;
+    __shared__ int s_info;
+    cplx *A = c.W3;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (threadIdx.x == 0) s_info = 0;
+    {
+        double bv = -1.0;
+        int bi = 0;
+        for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            cplx a = A[r];
+            double v = fabs(a.x) + fabs(a.y);
+            if (v > bv) { bv = v; bi = r; }
+        }
+        double mv;
+        int p;
+        block_argmax(bv, bi, rv, ri, &mv, &p);
+        if (threadIdx.x == 0) s_piv = p;
+        __syncthreads();
+    }
+    for (int k = 0; k < n; k++) {
+        const int p = s_piv;
+        if (threadIdx.x == 0) c.piv[k] = p;
+        if (p != k)
+            for (int j = threadIdx.x; j < n; j += blockDim.x) {
+                cplx t = A[(size_t)p + (size_t)j * n];
+                A[(size_t)p + (size_t)j * n] = A[(size_t)k + (size_t)j * n];
+                A[(size_t)k + (size_t)j * n] = t;
+            }
+        __syncthreads();
+        cplx akk = A[(size_t)k + (size_t)k * n];
+        bool zero = (akk.x == 0.0 && akk.y == 0.0);
+        if (zero && threadIdx.x == 0 && s_info == 0) s_info = k + 1;
+        cplx pinv = zero ? make_double2(0.0, 0.0) : cinv(akk);
+        for (int r = k + 1 + threadIdx.x; r < n; r += blockDim.x) {
+            cplx l = zero ? A[(size_t)r + (size_t)k * n] : cmul(A[(size_t)r + (size_t)k * n], pinv);
+            A[(size_t)r + (size_t)k * n] = l;
+            lcol[r] = l;
+        }
+        for (int j = k + 1 + threadIdx.x; j < n; j += blockDim.x) lrow[j] = A[(size_t)k + (size_t)j * n];
+        __syncthreads();
+        double bv = -1.0;
+        int bi = k + 1;
+        for (int j = k + 1 + w; j < n; j += nw) {
+            cplx *a = A + (size_t)j * n;
+            cplx f = lrow[j];
+            bool track = (j == k + 1);
+            for (int r = k + 1 + lane; r < n; r += 64) {
+                cplx v = csub(a[r], cmul(lcol[r], f));
+                a[r] = v;
+                if (track) {
+                    double m = fabs(v.x) + fabs(v.y);
+                    if (m > bv) { bv = m; bi = r; }
+                }
+            }
+        }
+        double mv;
+        int np;
+        block_argmax(bv, bi, rv, ri, &mv, &np);
+        if (threadIdx.x == 0) s_piv = (k + 1 < n) ? np : 0;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cplx ld = make_double2(0.0, 0.0);
+        for (int i = 0; i < n; i++) {
+            cplx d = A[(size_t)i + (size_t)i * n];
+            if (c.piv[i] != i) d = make_double2(-d.x, -d.y);
+            ld = cadd(ld, clog_(d));
+        }
+        double im = ld.y;
+        double k2 = round(im / 2.0 / M_PI);
+        ld.y = im - 2.0 * M_PI * k2;
+        c.sc[SC_ZLD] = ld.x;
+        c.sc[SC_ZLD + 1] = ld.y;
+        c.sc[SC_INFO] = (double)s_info;
+    }
+}
+
+// solve (P L U) Y = X for X = matrix `xsel`, in place; one wave per right-hand-side column,
+// the column held in registers (n <= 1024)
+#define SOLVE_SLOTS 16
+extern "C" __global__ void __launch_bounds__(512)
+k_lu_solve(const DChain *chains, int xsel)
+{
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    const cplx *A = c.W3;
+    cplx *X = chain_mat(c, xsel, 0);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int ns = (n + 63) / 64;
+    for (int j = w; j < n; j += nw) {
+        cplx *x = X + (size_t)j * n;
+        cplx reg[SOLVE_SLOTS];
+#pragma unroll
+        for (int s = 0; s < SOLVE_SLOTS; s++) {
+            int r = lane + 64 * s;
+            reg[s] = (s < ns && r < n) ? x[r] : make_double2(0.0, 0.0);
+        }
+        // row interchanges, in order
+        for (int k = 0; k < n; k++) {
+            int p = c.piv[k];
+            if (p == k) continue;
+            int sk = k >> 6, lk = k & 63, sp = p >> 6, lp = p & 63;
+            double kr = 0, ki = 0, pr = 0, pi = 0;
+#pragma unroll
+            for (int s = 0; s < SOLVE_SLOTS; s++) {
+                if (s == sk) { kr = __shfl(reg[s].x, lk, 64); ki = __shfl(reg[s].y, lk, 64); }
+                if (s == sp) { pr = __shfl(reg[s].x, lp, 64); pi = __shfl(reg[s].y, lp, 64); }
+            }
+#pragma unroll
+            for (int s = 0; s < SOLVE_SLOTS; s++) {
+                if (s == sk && lane == lk) reg[s] = make_double2(pr, pi);
+                if (s == sp && lane == lp) reg[s] = make_double2(kr, ki);
+            }
+        }
+        // L y = b (unit lower)
+        for (int k = 0; k < n - 1; k++) {
+            int sk = k >> 6, lk = k & 63;
+            double yr = 0, yi = 0;
+#pragma unroll
+            for (int s = 0; s < SOLVE_SLOTS; s++)
+                if (s == sk) { yr = __shfl(reg[s].x, lk, 64); yi = __shfl(reg[s].y, lk, 64); }
+            const cplx yk = make_double2(yr, yi);
+            const cplx *lk_col = A + (size_t)k * n;
+#pragma unroll
+            for (int s = 0; s < SOLVE_SLOTS; s++) {
+                int r = lane + 64 * s;
+                if (s >= sk && s < ns && r > k && r < n) reg[s] = csub(reg[s], cmul(lk_col[r], yk));
+            }
+        }
+        // U x = y
+        for (int k = n - 1; k >= 0; k--) {
+            int sk = k >> 6, lk = k & 63;
+            cplx ukk = A[(size_t)k + (size_t)k * n];
+#pragma unroll
+            for (int s = 0; s < SOLVE_SLOTS; s++)
+                if (s == sk && lane == lk) reg[s] = cdiv(reg[s], ukk);
+            double yr = 0, yi = 0;
+#pragma unroll
+            for (int s = 0; s < SOLVE_SLOTS; s++)
+                if (s == sk) { yr = __shfl(reg[s].x, lk, 64); yi = __shfl(reg[s].y, lk, 64); }
+            const cplx yk = make_double2(yr, yi);
+            const cplx *uk_col = A + (size_t)k * n;
+#pragma unroll
+            for (int s = 0; s < SOLVE_SLOTS; s++) {
+                int r = lane + 64 * s;
+                if (s <= sk && r < k) reg[s] = csub(reg[s], cmul(uk_col[r], yk));
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < SOLVE_SLOTS; s++) {
+            int r = lane + 64 * s;
+            if (s < ns && r < n) x[r] = reg[s];
+        }
+    }
+}
+
+// logweightf scratch: general (bb = 0): -ldql + ldqr + zld + sum log dlmax + sum log drmax;
+// bb = 1: ldqr + zld + sum log drmax
+extern "C" __global__ void __launch_bounds__(256)
+k_green_logw(const DChain *chains, int bb, int rsaved, int rpt, int lsaved, int lpt, int dst_logw)
+{
+    __shared__ double red[16];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    const double *dr = dvec_of(c, rsaved, rpt);
+    const double *dl = bb ? nullptr : dvec_of(c, lsaved, lpt);
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        s += log(dmax_of(dr[i]));
+        if (!bb) s += log(dmax_of(dl[i]));
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) {
+        cplx ldqr = rsaved ? make_double2(c.sc[SC_LDQ_SAVE], c.sc[SC_LDQ_SAVE + 1]) : c.ldq[rpt];
+        cplx lw = cadd(ldqr, make_double2(c.sc[SC_ZLD] + s, c.sc[SC_ZLD + 1]));
+        if (!bb) {
+            cplx ldql = lsaved ? make_double2(c.sc[SC_LDQ_SAVE], c.sc[SC_LDQ_SAVE + 1]) : c.ldq[lpt];
+            lw = csub(lw, ldql);
+        }
+        c.sc[dst_logw] = lw.x;
+        c.sc[dst_logw + 1] = lw.y;
+    }
+}
+
+// wrap check and replacement (dqmc_ft_sweep_0b.f90:212-259): max|Gs - G|, set_phase of the
+// scratch weight, G <- Gs when the LU succeeded; compare = 0 at sweep start (plain set)
+extern "C" __global__ void __launch_bounds__(1024)
+k_wrap(const DChain *chains, int compare)
+{
+    __shared__ double red[16];
+    __shared__ double cnt_sh[4][16];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    double *sc = c.sc;
+    double mx = 0.0;
+    bool nonfinite = false;
+    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+        cplx a = c.Gs[e];
+        if (!isfinite(a.x) || !isfinite(a.y)) nonfinite = true;
+        if (compare) {
+            cplx b = c.G[e];
+            double d = hypot(a.x - b.x, a.y - b.y);
+            if (d > mx) mx = d;
+        }
+    }
+    // max reduction
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) red[w] = mx;
+    __syncthreads();
+    mx = 0.0;
+    for (int i = 0; i < nw; i++) mx = fmax(mx, red[i]);
+    // set_phase: exp(i nflr Im logw) * prod_{i,nt} phase(conf) = exp(i (nflr Im logw + sum cnt_l arg_l))
+    double cnt[4] = {0, 0, 0, 0};
+    if (c.phase_from_conf && c.lwrapu) {
+        size_t tot = (size_t)n * c.ltrot;
+        for (size_t e = threadIdx.x; e < tot; e += blockDim.x) {
+            int v = c.conf[e];
+            cnt[0] += (v == 1); cnt[1] += (v == 2); cnt[2] += (v == 3); cnt[3] += (v == 4);
+        }
+    }
+    for (int l = 0; l < 4; l++) {
+        double v = cnt[l];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) cnt_sh[l][w] = v;
+    }
+    int nf_flag = __syncthreads_or(nonfinite ? 1 : 0);
+    if (threadIdx.x == 0) {
+        double ang = (double)c.nflr * sc[SC_LOGW_TMP + 1];
+        for (int l = 0; l < c.lcomp && l < 4; l++) {
+            double s = 0.0;
+            for (int i = 0; i < nw; i++) s += cnt_sh[l][i];
+            ang += s * c.parg[l];
+        }
+        cplx ph = make_double2(cos(ang), sin(ang));
+        int info = (int)sc[SC_INFO];
+        if (nf_flag) sc[SC_NONFINITE] = 1.0;
+        if (compare) {
+            if (mx > sc[SC_MAXWRAP]) sc[SC_MAXWRAP] = mx;
+            cplx ph0 = make_double2(sc[SC_PHASE], sc[SC_PHASE + 1]);
+            double pd = hypot(ph0.x - ph.x, ph0.y - ph.y);
+            if (pd > sc[SC_MAXPH]) sc[SC_MAXPH] = pd;
+            sc[SC_NCHK] += 1.0;
+            sc[SC_SUMLOG] += log10(mx > 0 ? mx : 1e-300);
+        }
+        if (info == 0) {
+            sc[SC_LOGW] = sc[SC_LOGW_TMP];
+            sc[SC_LOGW + 1] = sc[SC_LOGW_TMP + 1];
+            sc[SC_PHASE] = ph.x;
+            sc[SC_PHASE + 1] = ph.y;
+        }
+        red[0] = (double)info;
+    }
+    __syncthreads();
+    if ((int)red[0] == 0) {
+        for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) c.G[e] = c.Gs[e];
+    }
+    __syncthreads();
+    int tn = (int)sc[SC_TRACE_N];
+    if (c.trace && tn < c.trace_cap) {
+        for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) c.trace[(size_t)tn * n * n + e] = c.G[e];
+        __syncthreads();
+        if (threadIdx.x == 0) sc[SC_TRACE_N] = tn + 1;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// small utility kernels
+// ------------------------------------------------------------------------------------------
+// set chain point pt to (I, 1, I, 0)
+extern "C" __global__ void __launch_bounds__(256)
+k_set_identity(const DChain *chains, int pt)
+{
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    int i = (int)(e % n), j = (int)(e / n);
+    cplx v = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    c.U[(size_t)n * n * pt + e] = v;
+    c.V[(size_t)n * n * pt + e] = v;
+    if (j == 0) c.D[(size_t)n * pt + i] = 1.0;
+    if (e == 0) c.ldq[pt] = make_double2(0.0, 0.0);
+}
+
+// copy between matrices; mode 1 additionally saves D[pt] and ldq[pt] of the chain point
+extern "C" __global__ void __launch_bounds__(256)
+k_copy(const DChain *chains, int dsel, int dpt, int ssel, int spt)
+{
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    chain_mat(c, dsel, dpt)[e] = chain_mat(c, ssel, spt)[e];
+}
+
+// save chain point pt (U, D, V, logdetQ) into (Us, Ds, Vs, SC_LDQ_SAVE)
+extern "C" __global__ void __launch_bounds__(256)
+k_save_point(const DChain *chains, int pt)
+{
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    c.Us[e] = c.U[(size_t)n * n * pt + e];
+    c.Vs[e] = c.V[(size_t)n * n * pt + e];
+    if (e < (size_t)n) c.Ds[e] = c.D[(size_t)n * pt + e];
+    if (e == 0) { c.sc[SC_LDQ_SAVE] = c.ldq[pt].x; c.sc[SC_LDQ_SAVE + 1] = c.ldq[pt].y; }
+}

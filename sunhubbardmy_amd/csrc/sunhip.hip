@@ -1,0 +1,845 @@
+// sunhip.hip -- host side of libsunhip.so: the C ABI of include/sunhip.h.
+// The slice loop mirrors src/dqmc_ft_core/dqmc_ft_sweep_b0.f90 and dqmc_ft_sweep_0b.f90; every
+// launch advances all chains of the batch at once.
+#include "kernels.hip"
+#include "../../include/sunhip.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+enum { T_WRAP = 0, T_UPDATE, T_BPROD, T_QR, T_GREEN, T_MEASURE, T_OTHER, T_NCLS };
+
+struct EvRec {
+    int cls;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct sh_ctx {
+    int dev = 0;
+    hipStream_t st = nullptr;
+    int nch = 0, nmax = 0, ltrot = 0, nwrap = 0, nst = 0, kmode = 0, update = 0;
+    std::vector<int> iwrap_nt, wrap_step;
+    std::vector<DChain> hc;
+    DChain *dc = nullptr;
+    std::vector<void *> allocs;
+    int32_t *d_seeds = nullptr;
+    bool started = false;
+    std::string err;
+    // timing
+    bool timing = false;
+    std::vector<EvRec> evs;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    double tms[T_NCLS] = {0};
+    double tcnt[T_NCLS] = {0};
+    // push/pop_stage backups
+    std::vector<void *> stage;
+    std::vector<cplx *> trace_bufs;
+};
+
+#define HIPCHK(x)                                                                  \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                    \
+            ctx->err = std::string(#x) + ": " + hipGetErrorString(e_);            \
+            return SH_E_HIP;                                                       \
+        }                                                                          \
+    } while (0)
+
+static int fail(sh_ctx *ctx, int code, const std::string &m)
+{
+    if (ctx) ctx->err = m;
+    return code;
+}
+
+template <typename T>
+static T *dalloc(sh_ctx *ctx, size_t n)
+{
+    void *p = nullptr;
+    if (hipMalloc(&p, n * sizeof(T) + 256) != hipSuccess) return nullptr;
+    hipMemset(p, 0, n * sizeof(T) + 256);
+    ctx->allocs.push_back(p);
+    return (T *)p;
+}
+
+template <typename T>
+static T *dupload(sh_ctx *ctx, const T *src, size_t n)
+{
+    T *p = dalloc<T>(ctx, n);
+    if (p && src && n) hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice);
+    return p;
+}
+
+// ---- timing ------------------------------------------------------------------------------
+static hipEvent_t ev_get(sh_ctx *ctx)
+{
+    if (ctx->pool_used == ctx->pool.size()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        ctx->pool.push_back(e);
+    }
+    return ctx->pool[ctx->pool_used++];
+}
+
+struct Tic {
+    sh_ctx *ctx;
+    int cls;
+    hipEvent_t a = nullptr;
+    Tic(sh_ctx *c, int k) : ctx(c), cls(k)
+    {
+        if (ctx->timing) {
+            a = ev_get(ctx);
+            hipEventRecord(a, ctx->st);
+        }
+    }
+    ~Tic()
+    {
+        if (ctx->timing) {
+            hipEvent_t b = ev_get(ctx);
+            hipEventRecord(b, ctx->st);
+            ctx->evs.push_back({cls, a, b});
+        }
+    }
+};
+
+static void timers_collect(sh_ctx *ctx)
+{
+    if (ctx->evs.empty()) return;
+    hipStreamSynchronize(ctx->st);
+    for (auto &r : ctx->evs) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, r.a, r.b);
+        ctx->tms[r.cls] += ms;
+        ctx->tcnt[r.cls] += 1.0;
+    }
+    ctx->evs.clear();
+    ctx->pool_used = 0;
+}
+
+// ---- launch helpers ----------------------------------------------------------------------
+static dim3 grid_elem(sh_ctx *ctx) { return dim3((unsigned)(((size_t)ctx->nmax * ctx->nmax + 255) / 256), 1, ctx->nch); }
+static dim3 grid_tiles32(sh_ctx *ctx)
+{
+    unsigned nb = (ctx->nmax + 31) / 32;
+    return dim3(nb * nb, 1, ctx->nch);
+}
+
+template <int OPA, int OPB>
+static void gemm(sh_ctx *ctx, int selA, int ptA, int selB, int ptB, int selC, int ptC, double ar = 1.0,
+                 double br = 0.0)
+{
+    unsigned nb = (ctx->nmax + GT - 1) / GT;
+    hipLaunchKernelGGL((k_zgemm<OPA, OPB>), dim3(nb, nb, ctx->nch), dim3(256), 0, ctx->st, ctx->dc, selA, ptA, selB,
+                       ptB, selC, ptC, ar, 0.0, br, 0.0);
+}
+
+static void copy_mat(sh_ctx *ctx, int dsel, int dpt, int ssel, int spt)
+{
+    hipLaunchKernelGGL(k_copy, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, dsel, dpt, ssel, spt);
+}
+
+static size_t lds_cols(sh_ctx *ctx) { return (size_t)ctx->nmax * CPB * sizeof(cplx); }
+static size_t lds_rows(sh_ctx *ctx) { return (size_t)ctx->nmax * RPB * sizeof(cplx); }
+
+// left product (rows mixed) on matrix sel: program bits as in k_cb_cols
+static void left_pass(sh_ctx *ctx, int sel, int pt, int ksel, int prog, int vsel, int nt0, int nt1, int step)
+{
+    if (ctx->kmode == SH_K_CHECKERBOARD) {
+        hipLaunchKernelGGL(k_cb_cols, dim3((ctx->nmax + CPB - 1) / CPB, 1, ctx->nch), dim3(256), lds_cols(ctx),
+                           ctx->st, ctx->dc, sel, pt, ksel, prog, vsel, nt0, nt1, step);
+        return;
+    }
+    // dense kinetic term: matrices K (ksel 0 -> urt, 1 -> urtm1) as GEMMs through W2
+    const int kmat = ksel ? M_KTM1 : M_KT;
+    const int nslice = (step > 0 ? nt1 - nt0 : nt0 - nt1) + 1;
+    for (int q = 0, nt = nt0; q < nslice; q++, nt += step) {
+        if (prog & 8) hipLaunchKernelGGL(k_vscale, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, sel, pt, 0, vsel, 0, nt);
+        if (prog & 1) { gemm<0, 0>(ctx, kmat, 0, sel, pt, M_W2, 0); copy_mat(ctx, sel, pt, M_W2, 0); }
+        if (prog & 2) hipLaunchKernelGGL(k_vscale, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, sel, pt, 0, vsel, 0, nt);
+        if (prog & 4) { gemm<0, 0>(ctx, kmat, 0, sel, pt, M_W2, 0); copy_mat(ctx, sel, pt, M_W2, 0); }
+    }
+}
+
+// right product (columns mixed) on G
+static void right_pass(sh_ctx *ctx, int ksel, int prog, int vsel, int nt)
+{
+    if (ctx->kmode == SH_K_CHECKERBOARD) {
+        hipLaunchKernelGGL(k_cb_rows, dim3((ctx->nmax + RPB - 1) / RPB, 1, ctx->nch), dim3(256), lds_rows(ctx),
+                           ctx->st, ctx->dc, M_G, 0, ksel, prog, vsel, nt);
+        return;
+    }
+    const int kmat = ksel ? M_KTM1 : M_KT;
+    if (prog & 8) hipLaunchKernelGGL(k_vscale, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, M_G, 0, 1, vsel, 0, nt);
+    if (prog & 1) { gemm<0, 0>(ctx, M_G, 0, kmat, 0, M_W2, 0); copy_mat(ctx, M_G, 0, M_W2, 0); }
+    if (prog & 2) hipLaunchKernelGGL(k_vscale, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, M_G, 0, 1, vsel, 0, nt);
+    if (prog & 4) { gemm<0, 0>(ctx, M_G, 0, kmat, 0, M_W2, 0); copy_mat(ctx, M_G, 0, M_W2, 0); }
+}
+
+// ---- stabilization -------------------------------------------------------------------------
+static size_t lds_qr(sh_ctx *ctx) { return (size_t)ctx->nmax * 32; }
+
+// QR of W1 (first point: column pivoting) or of the prepivoted W1 diag(D[dpt]) -> point `pt`
+static void udv_step(sh_ctx *ctx, bool first, int pt, int dpt)
+{
+    if (first) {
+        {
+            Tic t(ctx, T_QR);
+            hipLaunchKernelGGL(k_qr, dim3(ctx->nch), dim3(1024), lds_qr(ctx), ctx->st, ctx->dc, M_W1, 1, pt);
+            hipLaunchKernelGGL(k_formq, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * sizeof(cplx), ctx->st, ctx->dc,
+                               M_W1, pt);
+        }
+        Tic t(ctx, T_OTHER);
+        hipLaunchKernelGGL(k_v_from_qrcp, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, M_W1, pt);
+        return;
+    }
+    {
+        Tic t(ctx, T_QR);
+        hipLaunchKernelGGL(k_prepivot, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * 12, ctx->st, ctx->dc, dpt);
+        hipLaunchKernelGGL(k_qr, dim3(ctx->nch), dim3(1024), lds_qr(ctx), ctx->st, ctx->dc, M_W2, 0, pt);
+        hipLaunchKernelGGL(k_formq, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * sizeof(cplx), ctx->st, ctx->dc,
+                           M_W2, pt);
+    }
+    Tic t(ctx, T_GREEN);
+    // V_new = D^-1 R P^T V_prev: W3 = D^-1 triu R, W4 = P^T V[dpt]
+    hipLaunchKernelGGL(k_after_qr, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, pt, (int)M_V, dpt);
+    gemm<0, 0>(ctx, M_W3, 0, M_W4, 0, M_V, pt);
+}
+
+// dqmc_ft_stablize_0b_qr(n): point n of the right chain from point n-1
+static void stablize_0b(sh_ctx *ctx, int n)
+{
+    {
+        Tic t(ctx, T_BPROD);
+        copy_mat(ctx, M_W1, 0, M_U, n - 1);
+        int ws1 = ctx->wrap_step[2 * (n - 1)], ws2 = ctx->wrap_step[2 * (n - 1) + 1];
+        // Bmat_left_forward: slices ws1..ws2, each K-half, V, K-half (src/dqmc_ft_core.f90:70-119)
+        left_pass(ctx, M_W1, 0, 0, 1 | 2 | 4, 0, ws1, ws2, +1);
+    }
+    udv_step(ctx, n == 1, n, n - 1);
+}
+
+// dqmc_ft_stablize_b0_qr(n): point n-1 of the left chain from point n
+static void stablize_b0(sh_ctx *ctx, int n)
+{
+    {
+        Tic t(ctx, T_BPROD);
+        copy_mat(ctx, M_W1, 0, M_U, n);
+        int ws1 = ctx->wrap_step[2 * (n - 1)], ws2 = ctx->wrap_step[2 * (n - 1) + 1];
+        // Bmat_left_forward_hc: slices ws2 down to ws1 with conj(bmat) (src/dqmc_ft_core.f90:121-171)
+        left_pass(ctx, M_W1, 0, 0, 1 | 2 | 4, 2, ws2, ws1, -1);
+    }
+    udv_step(ctx, n == ctx->nst, n - 1, n);
+}
+
+// green_equaltime into Gs + scratch logweight (rsaved/lsaved select the saved chain point)
+static void green(sh_ctx *ctx, int rsaved, int rpt, int lsaved, int lpt)
+{
+    Tic t(ctx, T_GREEN);
+    const int URs = rsaved ? M_US : M_U, VRs = rsaved ? M_VS : M_V;
+    const int ULs = lsaved ? M_US : M_U, VLs = lsaved ? M_VS : M_V;
+    gemm<1, 0>(ctx, URs, rsaved ? 0 : rpt, ULs, lsaved ? 0 : lpt, M_W1, 0);   // uu = UR^H UL
+    gemm<0, 1>(ctx, VRs, rsaved ? 0 : rpt, VLs, lsaved ? 0 : lpt, M_W2, 0);   // vv = VR VL^H
+    hipLaunchKernelGGL(k_green_m, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, 0, rsaved, rpt, lsaved, lpt);
+    hipLaunchKernelGGL(k_lu, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * 2 * sizeof(cplx), ctx->st, ctx->dc);
+    hipLaunchKernelGGL(k_green_rhs, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, rsaved, rpt);
+    hipLaunchKernelGGL(k_lu_solve, dim3(ctx->nch), dim3(512), 0, ctx->st, ctx->dc, (int)M_W4);
+    hipLaunchKernelGGL(k_green_ulscale, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, lsaved, lpt);
+    gemm<0, 0>(ctx, M_W1, 0, M_W4, 0, M_GS, 0);
+    hipLaunchKernelGGL(k_green_logw, dim3(ctx->nch), dim3(256), 0, ctx->st, ctx->dc, 0, rsaved, rpt, lsaved, lpt,
+                       (int)SC_LOGW_TMP);
+}
+
+// green_equaltimebb: G(beta,beta) from the right chain point nst
+static void green_bb(sh_ctx *ctx, int rpt)
+{
+    Tic t(ctx, T_GREEN);
+    hipLaunchKernelGGL(k_green_m, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, 1, 0, rpt, 0, 0);
+    hipLaunchKernelGGL(k_lu, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * 2 * sizeof(cplx), ctx->st, ctx->dc);
+    hipLaunchKernelGGL(k_green_rhs, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, 0, rpt);
+    hipLaunchKernelGGL(k_lu_solve, dim3(ctx->nch), dim3(512), 0, ctx->st, ctx->dc, (int)M_W4);
+    copy_mat(ctx, M_GS, 0, M_W4, 0);
+    hipLaunchKernelGGL(k_green_logw, dim3(ctx->nch), dim3(256), 0, ctx->st, ctx->dc, 1, 0, rpt, 0, 0, (int)SC_LOGW_TMP);
+}
+
+static void wrap_replace(sh_ctx *ctx, int compare)
+{
+    Tic t(ctx, T_OTHER);
+    hipLaunchKernelGGL(k_wrap, dim3(ctx->nch), dim3(1024), 0, ctx->st, ctx->dc, compare);
+}
+
+static void set_identity(sh_ctx *ctx, int pt)
+{
+    hipLaunchKernelGGL(k_set_identity, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, pt);
+}
+
+static void update_slice(sh_ctx *ctx, int nt, int slice_in_pass)
+{
+    Tic t(ctx, T_UPDATE);
+    if (ctx->update == SH_U_DELAY) {
+        size_t lds = (size_t)(ctx->nmax + 2 * 64) * sizeof(cplx);
+        hipLaunchKernelGGL(k_update_delay, dim3(ctx->nch), dim3(256), lds, ctx->st, ctx->dc, nt, slice_in_pass);
+    } else {
+        int K = 0;
+        for (auto &c : ctx->hc) K = std::max(K, c.nublock);
+        size_t lds = (size_t)(K * K + 4 * K) * sizeof(cplx);
+        hipLaunchKernelGGL(k_update_submatrix, dim3(ctx->nch), dim3(256), lds, ctx->st, ctx->dc, nt, slice_in_pass);
+    }
+}
+
+static void measure(sh_ctx *ctx)
+{
+    Tic t(ctx, T_MEASURE);
+    hipLaunchKernelGGL(k_measure, dim3(ctx->nch), dim3(256), 0, ctx->st, ctx->dc);
+}
+
+static int check_launch(sh_ctx *ctx)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, SH_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
+// ---- C ABI ---------------------------------------------------------------------------------
+extern "C" {
+
+int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
+{
+    if (!out || nchains <= 0 || !m) return SH_E_ARG;
+    *out = nullptr;
+    sh_ctx *ctx = new sh_ctx();
+    ctx->dev = device;
+    ctx->nch = nchains;
+    ctx->ltrot = m[0].ltrot;
+    ctx->nwrap = m[0].nwrap;
+    ctx->nst = m[0].nst;
+    ctx->kmode = m[0].kmode;
+    ctx->update = m[0].update;
+    for (int c = 0; c < nchains; c++) {
+        const sh_model &q = m[c];
+        if (q.ltrot != ctx->ltrot || q.nwrap != ctx->nwrap || q.nst != ctx->nst || q.kmode != ctx->kmode ||
+            q.update != ctx->update) {
+            int rc = fail(ctx, SH_E_ARG, "all chains of a batch must share ltrot, nwrap, nst, kmode and update");
+            delete ctx;
+            return rc;
+        }
+        if (!q.iwrap_nt || !q.wrap_step || !q.bonds || !q.urt || !q.urtm1 || !q.bmat || !q.bmat_inv ||
+            !q.phase_arg || !q.delta_bmat || !q.phase_ratio || !q.gaml || !q.stag || q.ltrot <= 0) {
+            delete ctx;
+            return SH_E_ARG;
+        }
+        if (q.ndim <= 0 || q.ndim > 1024 || q.nst <= 0 || q.lcomp < 2 || q.lcomp > 4 || q.nublock < 1 ||
+            q.nublock > 64 || (q.update == SH_U_SUBMATRIX && q.nublock > 64) || q.nflr < 1) {
+            delete ctx;
+            return SH_E_ARG;
+        }
+        if (q.kmode == SH_K_CHECKERBOARD && (q.nn_nf <= 0 || q.nn_lf <= 0)) { delete ctx; return SH_E_ARG; }
+        for (int b = 0; b < q.nn_nf * q.nn_lf * 2; b++)
+            if (q.bonds[b] < 0 || q.bonds[b] >= q.ndim) { delete ctx; return SH_E_ARG; }
+        for (int i = 0; i <= q.ltrot; i++)
+            if (q.iwrap_nt[i] < 0 || q.iwrap_nt[i] > q.nst) { delete ctx; return SH_E_ARG; }
+        for (int i = 0; i < 2 * q.nst; i++)
+            if (q.wrap_step[i] < 1 || q.wrap_step[i] > q.ltrot) { delete ctx; return SH_E_ARG; }
+        ctx->nmax = std::max(ctx->nmax, q.ndim);
+    }
+    // arguments are validated before the first HIP call
+    if (hipSetDevice(device) != hipSuccess) { delete ctx; return SH_E_HIP; }
+    ctx->iwrap_nt.assign(m[0].iwrap_nt, m[0].iwrap_nt + m[0].ltrot + 1);
+    ctx->wrap_step.assign(m[0].wrap_step, m[0].wrap_step + 2 * m[0].nst);
+    if (hipStreamCreate(&ctx->st) != hipSuccess) { delete ctx; return SH_E_HIP; }
+    // allow the large-LDS passes at big N
+    hipFuncSetAttribute((const void *)k_cb_rows, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipFuncSetAttribute((const void *)k_cb_cols, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    std::vector<int32_t> seeds(nchains);
+    ctx->hc.resize(nchains);
+    for (int ci = 0; ci < nchains; ci++) {
+        const sh_model &q = m[ci];
+        DChain &c = ctx->hc[ci];
+        memset(&c, 0, sizeof(c));
+        const int n = q.ndim;
+        const size_t nn = (size_t)n * n;
+        c.n = n; c.lq = q.lq; c.nn_nf = q.nn_nf; c.nn_lf = q.nn_lf; c.lcomp = q.lcomp; c.nflr = q.nflr;
+        c.kmode = q.kmode; c.nublock = q.nublock; c.update = q.update; c.lwrapT = q.lwrapT; c.lwrapu = q.lwrapu;
+        c.phase_from_conf = q.phase_from_conf; c.ltrot = q.ltrot; c.nst = q.nst;
+        c.G = dalloc<cplx>(ctx, nn); c.Gs = dalloc<cplx>(ctx, nn);
+        c.W1 = dalloc<cplx>(ctx, nn); c.W2 = dalloc<cplx>(ctx, nn); c.W3 = dalloc<cplx>(ctx, nn);
+        c.W4 = dalloc<cplx>(ctx, nn); c.Us = dalloc<cplx>(ctx, nn); c.Vs = dalloc<cplx>(ctx, nn);
+        c.U = dalloc<cplx>(ctx, nn * (q.nst + 1)); c.V = dalloc<cplx>(ctx, nn * (q.nst + 1));
+        c.D = dalloc<double>(ctx, (size_t)n * (q.nst + 1)); c.Ds = dalloc<double>(ctx, n);
+        c.ldq = dalloc<cplx>(ctx, q.nst + 1);
+        int kb = std::max(q.nublock, 1);
+        c.avec = dalloc<cplx>(ctx, (size_t)n * kb); c.bvec = dalloc<cplx>(ctx, (size_t)n * kb);
+        c.tau = dalloc<cplx>(ctx, n); c.jpvt = dalloc<int32_t>(ctx, n); c.piv = dalloc<int32_t>(ctx, n);
+        c.rwork = dalloc<double>(ctx, 4 * (size_t)n);
+        c.conf = dalloc<int32_t>(ctx, (size_t)n * q.ltrot);
+        c.unif = dalloc<double>(ctx, (size_t)2 * n * q.ltrot);
+        c.rng = dalloc<uint32_t>(ctx, 625);
+        c.bonds = dupload<int32_t>(ctx, q.bonds, (size_t)2 * q.nn_nf * q.nn_lf);
+        size_t kl = (q.kmode == SH_K_CHECKERBOARD) ? (size_t)4 * q.nn_nf * q.nn_lf : nn;
+        c.kt = dupload<cplx>(ctx, (const cplx *)q.urt, kl);
+        c.ktm1 = dupload<cplx>(ctx, (const cplx *)q.urtm1, kl);
+        c.stag = dupload<double>(ctx, q.stag, n);
+        c.sc = dalloc<double>(ctx, SC_SIZE);
+        if (!c.G || !c.Gs || !c.W1 || !c.W2 || !c.W3 || !c.W4 || !c.Us || !c.Vs || !c.U || !c.V || !c.D || !c.Ds ||
+            !c.ldq || !c.avec || !c.bvec || !c.tau || !c.jpvt || !c.piv || !c.rwork || !c.conf || !c.unif ||
+            !c.rng || !c.bonds || !c.kt || !c.ktm1 || !c.stag || !c.sc) {
+            int rc = fail(ctx, SH_E_HIP, "device allocation failed");
+            sh_destroy(ctx);
+            return rc;
+        }
+        for (int l = 0; l < q.lcomp; l++) {
+            c.bmat[l] = make_double2(q.bmat[2 * l], q.bmat[2 * l + 1]);
+            c.bmat_inv[l] = make_double2(q.bmat_inv[2 * l], q.bmat_inv[2 * l + 1]);
+            c.gaml[l] = q.gaml[l];
+            c.parg[l] = q.phase_arg[l];
+        }
+        for (int k = 0; k < (q.lcomp - 1) * q.lcomp; k++) {
+            c.delta_bmat[k] = make_double2(q.delta_bmat[2 * k], q.delta_bmat[2 * k + 1]);
+            c.phase_ratio[k] = make_double2(q.phase_ratio[2 * k], q.phase_ratio[2 * k + 1]);
+        }
+        // every conf entry starts at 1 (valid index) until init_conf / set_conf
+        std::vector<int32_t> ones((size_t)n * q.ltrot, 1);
+        hipMemcpy(c.conf, ones.data(), ones.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+        double sc0[SC_SIZE] = {0};
+        sc0[SC_PHASE] = 1.0;
+        hipMemcpy(c.sc, sc0, sizeof(sc0), hipMemcpyHostToDevice);
+        seeds[ci] = q.seed;
+    }
+    ctx->dc = dupload<DChain>(ctx, ctx->hc.data(), nchains);
+    ctx->d_seeds = dupload<int32_t>(ctx, seeds.data(), nchains);
+    if (!ctx->dc || !ctx->d_seeds) { sh_destroy(ctx); return SH_E_HIP; }
+    hipLaunchKernelGGL(k_sfmt_seed, dim3((nchains + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, nchains, ctx->d_seeds);
+    if (hipStreamSynchronize(ctx->st) != hipSuccess || hipGetLastError() != hipSuccess) {
+        sh_destroy(ctx);
+        return SH_E_HIP;
+    }
+    *out = ctx;
+    return 0;
+}
+
+int sh_destroy(sh_ctx *ctx)
+{
+    if (!ctx) return 0;
+    if (ctx->st) hipStreamSynchronize(ctx->st);
+    for (void *p : ctx->allocs) hipFree(p);
+    for (void *p : ctx->stage) hipFree(p);
+    for (cplx *p : ctx->trace_bufs) if (p) hipFree(p);
+    for (auto e : ctx->pool) hipEventDestroy(e);
+    if (ctx->st) hipStreamDestroy(ctx->st);
+    delete ctx;
+    return 0;
+}
+
+const char *sh_last_error(const sh_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int sh_init_conf(sh_ctx *ctx)
+{
+    if (!ctx) return SH_E_ARG;
+    hipLaunchKernelGGL(k_sfmt_draw, dim3((ctx->nch + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 0);
+    return check_launch(ctx);
+}
+
+int sh_set_conf(sh_ctx *ctx, int chain, const int32_t *conf)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !conf) return SH_E_ARG;
+    const DChain &c = ctx->hc[chain];
+    size_t n = (size_t)c.n * c.ltrot;
+    for (size_t k = 0; k < n; k++)
+        if (conf[k] < 1 || conf[k] > c.lcomp) return fail(ctx, SH_E_ARG, "conf value out of 1..lcomp");
+    HIPCHK(hipMemcpyAsync(c.conf, conf, n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int sh_get_conf(sh_ctx *ctx, int chain, int32_t *conf)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !conf) return SH_E_ARG;
+    const DChain &c = ctx->hc[chain];
+    HIPCHK(hipMemcpyAsync(conf, c.conf, (size_t)c.n * c.ltrot * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+static int first_info(sh_ctx *ctx)
+{
+    // the reference stops on a failed LU in b0; report the first chain's pivot index
+    for (int ci = 0; ci < ctx->nch; ci++) {
+        double v[2];
+        if (hipMemcpy(v, ctx->hc[ci].sc + SC_INFO, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return SH_E_HIP;
+        if (v[0] != 0.0) return (int)v[0];
+    }
+    return 0;
+}
+
+int sh_sweep_start_0b(sh_ctx *ctx)
+{
+    if (!ctx) return SH_E_ARG;
+    set_identity(ctx, 0);
+    for (int n = 1; n <= ctx->nst; n++) stablize_0b(ctx, n);
+    green_bb(ctx, ctx->nst);
+    wrap_replace(ctx, 0);
+    int rc = check_launch(ctx);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    int info = first_info(ctx);
+    if (info > 0) return fail(ctx, info, "LU breakdown in sweep_start");
+    ctx->started = true;
+    return 0;
+}
+
+int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure)
+{
+    if (!ctx) return SH_E_ARG;
+    if (!ctx->started) return fail(ctx, SH_E_STATE, "sweep_b0 before sweep_start_0b");
+    set_identity(ctx, ctx->nst);
+    const bool upd = lupdate && ctx->hc[0].lwrapu;
+    if (upd) hipLaunchKernelGGL(k_sfmt_draw, dim3((ctx->nch + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
+    int slice = 0;
+    for (int nt = ctx->ltrot; nt >= 1; nt--, slice++) {
+        if (lmeasure) measure(ctx);
+        {
+            Tic t(ctx, T_WRAP);
+            // K-half: G B^{-1}: right_forward (urt), left_backward (urtm1), families 1..nn_nf
+            right_pass(ctx, 0, 1, 0, nt);
+            left_pass(ctx, M_G, 0, 1, 1, 0, nt, nt, 1);
+        }
+        if (upd) update_slice(ctx, nt, slice);
+        {
+            Tic t(ctx, T_WRAP);
+            // V (right_forward bmat, left_backward bmat_inv) then the K-half nn_nf..1
+            right_pass(ctx, 0, 8 | 4, 0, nt);
+            left_pass(ctx, M_G, 0, 1, 8 | 4, 1, nt, nt, 1);
+        }
+        if ((ctx->iwrap_nt[nt - 1] > 0 || ctx->nwrap == 1) || nt == 1) {
+            int n = ctx->iwrap_nt[nt - 1];
+            {
+                Tic t(ctx, T_OTHER);
+                hipLaunchKernelGGL(k_save_point, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, n);
+            }
+            stablize_b0(ctx, n + 1);
+            green(ctx, 1, 0, 0, n);
+            wrap_replace(ctx, 1);
+        }
+    }
+    int rc = check_launch(ctx);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    int info = first_info(ctx);
+    if (info > 0) return fail(ctx, info, "LU breakdown in green_equaltime (sweep_b0)");
+    return 0;
+}
+
+int sh_sweep_0b(sh_ctx *ctx, int lupdate, int lmeasure, int lmeasure_dyn)
+{
+    if (!ctx) return SH_E_ARG;
+    if (lmeasure_dyn) return fail(ctx, SH_E_ARG, "time-displaced measurement is not provided");
+    if (!ctx->started) return fail(ctx, SH_E_STATE, "sweep_0b before sweep_start_0b");
+    set_identity(ctx, 0);
+    const bool upd = lupdate && ctx->hc[0].lwrapu;
+    if (upd) hipLaunchKernelGGL(k_sfmt_draw, dim3((ctx->nch + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
+    for (int nt = 1; nt <= ctx->ltrot; nt++) {
+        {
+            Tic t(ctx, T_WRAP);
+            // K-half B G B^{-1}: left_forward (urt), right_backward (urtm1); then V rows/cols
+            left_pass(ctx, M_G, 0, 0, 1 | 2, 0, nt, nt, 1);
+            right_pass(ctx, 1, 1 | 2, 1, nt);
+        }
+        if (upd) update_slice(ctx, nt, nt - 1);
+        {
+            Tic t(ctx, T_WRAP);
+            left_pass(ctx, M_G, 0, 0, 4, 0, nt, nt, 1);
+            right_pass(ctx, 1, 4, 1, nt);
+        }
+        if (lmeasure) measure(ctx);
+        if (ctx->iwrap_nt[nt] > 0) {
+            int n = ctx->iwrap_nt[nt];
+            {
+                Tic t(ctx, T_OTHER);
+                hipLaunchKernelGGL(k_save_point, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, n);
+            }
+            stablize_0b(ctx, n);
+            green(ctx, 0, n, 1, 0);
+            wrap_replace(ctx, 1);
+        }
+    }
+    int rc = check_launch(ctx);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int sh_push_stage(sh_ctx *ctx)
+{
+    if (!ctx) return SH_E_ARG;
+    if (ctx->stage.empty()) {
+        for (auto &c : ctx->hc) {
+            size_t nn = (size_t)c.n * c.n;
+            size_t bytes = (2 * nn * (c.nst + 1) + nn) * sizeof(cplx) + (size_t)c.n * (c.nst + 1) * sizeof(double) +
+                           (c.nst + 1) * sizeof(cplx) + SC_SIZE * sizeof(double);
+            void *p = nullptr;
+            HIPCHK(hipMalloc(&p, bytes));
+            ctx->stage.push_back(p);
+        }
+    }
+    for (int ci = 0; ci < ctx->nch; ci++) {
+        DChain &c = ctx->hc[ci];
+        size_t nn = (size_t)c.n * c.n;
+        char *p = (char *)ctx->stage[ci];
+        HIPCHK(hipMemcpyAsync(p, c.U, nn * (c.nst + 1) * sizeof(cplx), hipMemcpyDeviceToDevice, ctx->st)); p += nn * (c.nst + 1) * sizeof(cplx);
+        HIPCHK(hipMemcpyAsync(p, c.V, nn * (c.nst + 1) * sizeof(cplx), hipMemcpyDeviceToDevice, ctx->st)); p += nn * (c.nst + 1) * sizeof(cplx);
+        HIPCHK(hipMemcpyAsync(p, c.G, nn * sizeof(cplx), hipMemcpyDeviceToDevice, ctx->st)); p += nn * sizeof(cplx);
+        HIPCHK(hipMemcpyAsync(p, c.D, (size_t)c.n * (c.nst + 1) * sizeof(double), hipMemcpyDeviceToDevice, ctx->st)); p += (size_t)c.n * (c.nst + 1) * sizeof(double);
+        HIPCHK(hipMemcpyAsync(p, c.ldq, (c.nst + 1) * sizeof(cplx), hipMemcpyDeviceToDevice, ctx->st)); p += (c.nst + 1) * sizeof(cplx);
+        HIPCHK(hipMemcpyAsync(p, c.sc, SC_SIZE * sizeof(double), hipMemcpyDeviceToDevice, ctx->st));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int sh_pop_stage(sh_ctx *ctx)
+{
+    if (!ctx) return SH_E_ARG;
+    if (ctx->stage.empty()) return fail(ctx, SH_E_STATE, "pop_stage without push_stage");
+    for (int ci = 0; ci < ctx->nch; ci++) {
+        DChain &c = ctx->hc[ci];
+        size_t nn = (size_t)c.n * c.n;
+        char *p = (char *)ctx->stage[ci];
+        HIPCHK(hipMemcpyAsync(c.U, p, nn * (c.nst + 1) * sizeof(cplx), hipMemcpyDeviceToDevice, ctx->st)); p += nn * (c.nst + 1) * sizeof(cplx);
+        HIPCHK(hipMemcpyAsync(c.V, p, nn * (c.nst + 1) * sizeof(cplx), hipMemcpyDeviceToDevice, ctx->st)); p += nn * (c.nst + 1) * sizeof(cplx);
+        HIPCHK(hipMemcpyAsync(c.G, p, nn * sizeof(cplx), hipMemcpyDeviceToDevice, ctx->st)); p += nn * sizeof(cplx);
+        HIPCHK(hipMemcpyAsync(c.D, p, (size_t)c.n * (c.nst + 1) * sizeof(double), hipMemcpyDeviceToDevice, ctx->st)); p += (size_t)c.n * (c.nst + 1) * sizeof(double);
+        HIPCHK(hipMemcpyAsync(c.ldq, p, (c.nst + 1) * sizeof(cplx), hipMemcpyDeviceToDevice, ctx->st));
+        // gfc = gf and the UDV chain come back; phase and logweightf are module state that
+        // pop_stage does not restore in the reference, so sc stays as is
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int sh_get_g(sh_ctx *ctx, int chain, double *g)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !g) return SH_E_ARG;
+    const DChain &c = ctx->hc[chain];
+    HIPCHK(hipMemcpyAsync(g, c.G, (size_t)c.n * c.n * sizeof(cplx), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int sh_get_diag(sh_ctx *ctx, int chain, double *d)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !d) return SH_E_ARG;
+    double sc[SC_SIZE];
+    HIPCHK(hipMemcpyAsync(sc, ctx->hc[chain].sc, sizeof(sc), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    d[0] = sc[SC_MAXWRAP]; d[1] = sc[SC_MAXPH];
+    d[2] = sc[SC_PHASE]; d[3] = sc[SC_PHASE + 1];
+    d[4] = sc[SC_LOGW]; d[5] = sc[SC_LOGW + 1];
+    d[6] = sc[SC_ACC]; d[7] = sc[SC_TRIED]; d[8] = sc[SC_NCHK]; d[9] = sc[SC_SUMLOG];
+    if (sc[SC_NONFINITE] != 0.0) return fail(ctx, SH_E_NONFINITE, "non-finite scratch Green's function");
+    return 0;
+}
+
+int sh_reset_diag(sh_ctx *ctx)
+{
+    if (!ctx) return SH_E_ARG;
+    for (auto &c : ctx->hc) {
+        HIPCHK(hipMemsetAsync(c.sc + SC_MAXWRAP, 0, 6 * sizeof(double), ctx->st));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int sh_get_obs(sh_ctx *ctx, int chain, double *o)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !o) return SH_E_ARG;
+    double sc[SC_SIZE];
+    HIPCHK(hipMemcpyAsync(sc, ctx->hc[chain].sc, sizeof(sc), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    o[0] = sc[SC_NOBS];
+    for (int k = 0; k < 10; k++) o[1 + k] = sc[SC_OBS + k];
+    return 0;
+}
+
+int sh_reset_obs(sh_ctx *ctx)
+{
+    if (!ctx) return SH_E_ARG;
+    for (auto &c : ctx->hc) {
+        HIPCHK(hipMemsetAsync(c.sc + SC_NOBS, 0, 11 * sizeof(double), ctx->st));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int sh_set_trace(sh_ctx *ctx, int chain, int cap)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || cap < 0) return SH_E_ARG;
+    if ((int)ctx->trace_bufs.size() < ctx->nch) ctx->trace_bufs.resize(ctx->nch, nullptr);
+    DChain &c = ctx->hc[chain];
+    if (ctx->trace_bufs[chain]) { HIPCHK(hipFree(ctx->trace_bufs[chain])); ctx->trace_bufs[chain] = nullptr; }
+    c.trace = nullptr;
+    c.trace_cap = 0;
+    if (cap > 0) {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, (size_t)cap * c.n * c.n * sizeof(cplx)));
+        ctx->trace_bufs[chain] = (cplx *)p;
+        c.trace = (cplx *)p;
+        c.trace_cap = cap;
+    }
+    double zero = 0.0;
+    HIPCHK(hipMemcpy(c.sc + SC_TRACE_N, &zero, sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->dc + chain, &c, sizeof(DChain), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int sh_get_trace(sh_ctx *ctx, int chain, double *buf, int *count)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !count) return SH_E_ARG;
+    const DChain &c = ctx->hc[chain];
+    double tn = 0.0;
+    HIPCHK(hipMemcpy(&tn, c.sc + SC_TRACE_N, sizeof(double), hipMemcpyDeviceToHost));
+    *count = (int)tn;
+    if (buf && c.trace && *count > 0)
+        HIPCHK(hipMemcpy(buf, c.trace, (size_t)(*count) * c.n * c.n * sizeof(cplx), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int sh_timers(sh_ctx *ctx, int enable, double *t)
+{
+    if (!ctx) return SH_E_ARG;
+    timers_collect(ctx);
+    if (t) {
+        for (int k = 0; k < T_NCLS; k++) { t[k] = ctx->tms[k]; t[T_NCLS + k] = ctx->tcnt[k]; }
+    }
+    for (int k = 0; k < T_NCLS; k++) { ctx->tms[k] = 0; ctx->tcnt[k] = 0; }
+    ctx->timing = enable != 0;
+    return 0;
+}
+
+int sh_synchronize(sh_ctx *ctx)
+{
+    if (!ctx) return SH_E_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return check_launch(ctx);
+}
+
+}  // extern "C"
+
+// ---- test hooks (include/sunhip_testing.h) --------------------------------------------------
+#include "../../include/sunhip_testing.h"
+
+static void *dbg_ptr(sh_ctx *ctx, int chain, int sel, int pt, size_t *bytes)
+{
+    const DChain &c = ctx->hc[chain];
+    size_t n = c.n, nn = n * n;
+    *bytes = nn * sizeof(cplx);
+    switch (sel) {
+    case 0: return c.G;
+    case 1: return c.Gs;
+    case 2: return c.W1;
+    case 3: return c.W2;
+    case 4: return c.W3;
+    case 5: return c.W4;
+    case 6: return c.Us;
+    case 7: return c.Vs;
+    case 8: return c.U + nn * pt;
+    case 9: return c.V + nn * pt;
+    case 100: *bytes = n * sizeof(double); return c.D + n * pt;
+    case 102: *bytes = n * sizeof(cplx); return c.tau;
+    case 103: *bytes = sizeof(cplx); return c.ldq + pt;
+    case 104: *bytes = SC_SIZE * sizeof(double); return c.sc;
+    }
+    return nullptr;
+}
+
+extern "C" int sh_dbg_put(sh_ctx *ctx, int chain, int sel, int pt, const double *src)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !src) return SH_E_ARG;
+    if (sel == 8 || sel == 9 || sel == 100 || sel == 103)
+        if (pt < 0 || pt > ctx->hc[chain].nst) return SH_E_ARG;
+    size_t bytes = 0;
+    void *p = dbg_ptr(ctx, chain, sel, pt, &bytes);
+    if (!p) return SH_E_ARG;
+    HIPCHK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int sh_dbg_get(sh_ctx *ctx, int chain, int sel, int pt, double *dst)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !dst) return SH_E_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    const DChain &c = ctx->hc[chain];
+    if (sel == 101 || sel == 105) {
+        std::vector<int32_t> v(c.n);
+        HIPCHK(hipMemcpy(v.data(), sel == 101 ? c.jpvt : c.piv, c.n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (int i = 0; i < c.n; i++) dst[i] = v[i];
+        return 0;
+    }
+    if (sel == 8 || sel == 9 || sel == 100 || sel == 103)
+        if (pt < 0 || pt > c.nst) return SH_E_ARG;
+    size_t bytes = 0;
+    void *p = dbg_ptr(ctx, chain, sel, pt, &bytes);
+    if (!p) return SH_E_ARG;
+    HIPCHK(hipMemcpy(dst, p, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int sh_dbg_op(sh_ctx *ctx, int op, const int *a)
+{
+    if (!ctx || !a) return SH_E_ARG;
+    auto okpt = [&](int p) { return p >= 0 && p <= ctx->nst; };
+    auto okslice = [&](int s) { return s >= 1 && s <= ctx->ltrot; };
+    switch (op) {
+    case 0: gemm<0, 0>(ctx, M_W1, 0, M_W2, 0, M_W3, 0); break;
+    case 1: gemm<1, 0>(ctx, M_W1, 0, M_W2, 0, M_W3, 0); break;
+    case 2: gemm<0, 1>(ctx, M_W1, 0, M_W2, 0, M_W3, 0); break;
+    case 3:
+        if (!okpt(a[0])) return SH_E_ARG;
+        udv_step(ctx, true, a[0], 0);
+        break;
+    case 4:
+        if (!okpt(a[0]) || !okpt(a[1]) || a[0] == a[1]) return SH_E_ARG;
+        udv_step(ctx, false, a[0], a[1]);
+        break;
+    case 5:
+        hipLaunchKernelGGL(k_lu, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * 2 * sizeof(cplx), ctx->st, ctx->dc);
+        hipLaunchKernelGGL(k_lu_solve, dim3(ctx->nch), dim3(512), 0, ctx->st, ctx->dc, (int)M_W4);
+        break;
+    case 6:
+        if (!okslice(a[3]) || !okslice(a[4]) || (a[5] != 1 && a[5] != -1)) return SH_E_ARG;
+        if ((a[5] > 0) != (a[4] >= a[3]) && a[3] != a[4]) return SH_E_ARG;
+        left_pass(ctx, M_W1, 0, a[1], a[0], a[2], a[3], a[4], a[5]);
+        break;
+    case 7:
+        if (!okslice(a[3])) return SH_E_ARG;
+        right_pass(ctx, a[1], a[0], a[2], a[3]);
+        break;
+    case 8:
+        if ((!a[0] && !okpt(a[1])) || (!a[2] && !okpt(a[3]))) return SH_E_ARG;
+        green(ctx, a[0], a[1], a[2], a[3]);
+        break;
+    case 9:
+        if (!okpt(a[0])) return SH_E_ARG;
+        green_bb(ctx, a[0]);
+        break;
+    case 10:
+        if (!okslice(a[0]) || a[1] < 0 || a[1] >= ctx->ltrot) return SH_E_ARG;
+        update_slice(ctx, a[0], a[1]);
+        break;
+    case 11:
+        hipLaunchKernelGGL(k_sfmt_draw, dim3((ctx->nch + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
+        break;
+    case 12: measure(ctx); break;
+    case 13: wrap_replace(ctx, a[0]); break;
+    default: return SH_E_ARG;
+    }
+    int rc = check_launch(ctx);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}

@@ -1,0 +1,111 @@
+// sunhip_kernels.h -- device-side data layout and small helpers shared by the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double2 cplx;                                   // (re, im), the complex(dp) layout
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// per-chain scalar slots (double array `sc`)
+enum {
+    SC_PHASE = 0,      // 2: phase (re, im)
+    SC_LOGW = 2,       // 2: logweightf
+    SC_LOGW_TMP = 4,   // 2: scratch logweightf from green_equaltime
+    SC_ZLD = 6,        // 2: log det of the LU-factored matrix
+    SC_MAXWRAP = 8,
+    SC_MAXPH = 9,
+    SC_NCHK = 10,
+    SC_SUMLOG = 11,
+    SC_ACC = 12,
+    SC_TRIED = 13,
+    SC_INFO = 14,      // first zero pivot (+1) of the last LU, 0 if none
+    SC_NOBS = 15,
+    SC_OBS = 16,       // 10: complex sums of sign, density/lq, zkint, zint_u, S_stag
+    SC_TRACE_N = 26,
+    SC_LDQ_SAVE = 27,  // 2: logdetQ of the saved chain point
+    SC_NONFINITE = 29,
+    SC_SIZE = 32
+};
+
+// matrix selectors resolved per chain on the device
+enum { M_G = 0, M_GS, M_W1, M_W2, M_W3, M_W4, M_US, M_VS, M_U, M_V, M_KT, M_KTM1 };
+
+struct DChain {
+    int n, lq, nn_nf, nn_lf, lcomp, nflr, kmode, nublock, update, lwrapT, lwrapu, phase_from_conf;
+    int ltrot, nst, pad0, pad1;
+    cplx *G, *Gs, *W1, *W2, *W3, *W4, *Us, *Vs;
+    cplx *U, *V;          // (nst+1) points, n*n each
+    double *D, *Ds;       // (nst+1)*n, n
+    cplx *ldq;            // nst+1
+    cplx *avec, *bvec;    // n*nublock
+    cplx *tau;            // n
+    int32_t *jpvt, *piv;  // n
+    double *rwork;        // 4n
+    int32_t *conf;        // ltrot*n
+    double *unif;         // 2*n*ltrot
+    uint32_t *rng;        // 624 words + index
+    const int32_t *bonds; // nn_nf*nn_lf*2
+    const cplx *kt, *ktm1;
+    const double *stag;
+    double *sc;           // SC_SIZE
+    cplx *trace;
+    int trace_cap, pad2;
+    cplx bmat[4], bmat_inv[4], delta_bmat[12], phase_ratio[12];
+    double gaml[4], parg[4];
+};
+
+__device__ __forceinline__ cplx cmk(double r, double i) { return make_double2(r, i); }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cplx cmul(cplx a, cplx b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ cplx cconj(cplx a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ cplx cscale(cplx a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double cabs2(cplx a) { return a.x * a.x + a.y * a.y; }
+__device__ __forceinline__ cplx cdiv(cplx a, cplx b)
+{
+    // Smith's algorithm, as compilers lower Fortran complex division
+    if (fabs(b.x) >= fabs(b.y)) {
+        double r = b.y / b.x, d = b.x + b.y * r;
+        return make_double2((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+    } else {
+        double r = b.x / b.y, d = b.x * r + b.y;
+        return make_double2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+    }
+}
+__device__ __forceinline__ cplx cinv(cplx b) { return cdiv(make_double2(1.0, 0.0), b); }
+// complex log with the principal branch (Im in (-pi, pi])
+__device__ __forceinline__ cplx clog_(cplx z) { return make_double2(log(hypot(z.x, z.y)), atan2(z.y, z.x)); }
+__device__ __forceinline__ cplx cexp_(cplx z)
+{
+    double e = exp(z.x);
+    return make_double2(e * cos(z.y), e * sin(z.y));
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ cplx *chain_mat(const DChain &c, int sel, int pt)
+{
+    size_t nn = (size_t)c.n * c.n;
+    switch (sel) {
+    case M_G: return c.G;
+    case M_GS: return c.Gs;
+    case M_W1: return c.W1;
+    case M_W2: return c.W2;
+    case M_W3: return c.W3;
+    case M_W4: return c.W4;
+    case M_US: return c.Us;
+    case M_VS: return c.Vs;
+    case M_U: return c.U + nn * pt;
+    case M_V: return c.V + nn * pt;
+    case M_KT: return (cplx *)c.kt;
+    case M_KTM1: return (cplx *)c.ktm1;
+    }
+    return nullptr;
+}

@@ -1,0 +1,176 @@
+"""FTWalkers: a batch of independent finite-temperature DQMC chains advanced on one GPU.
+
+Host mirror of the reference's sweep interface (src/dqmc_ft_ctrl.f90:9-126 calls these):
+``allocate_core`` -> ``FTWalkers(models)``, ``u0conf%input_conf`` -> ``init_conf``,
+``dqmc_ft_sweep_start_0b`` -> ``sweep_start_0b``, ``dqmc_ft_sweep_b0(lupdate, lmeasure)`` ->
+``sweep_b0``, ``dqmc_ft_sweep_0b(lupdate, lmeasure, ldyn)`` -> ``sweep_0b``, ``push_stage`` /
+``pop_stage``.  Each chain is what one MPI rank of the reference runs; the batch replaces the
+ranks.  All compute goes through libsunhip.so; nothing here falls back to the CPU.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .model import Model, make_tables
+
+
+class SweepError(RuntimeError):
+    pass
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class FTWalkers:
+    def __init__(self, models, device=0):
+        if isinstance(models, Model):
+            models = [models]
+        self.models = list(models)
+        self.tables = [make_tables(m) for m in self.models]
+        self._keep = []
+        arr = (_lib.ShModel * len(self.tables))()
+        for k, t in enumerate(self.tables):
+            arr[k] = self._shmodel(t)
+        L = _lib.lib()
+        self._ctx = ctypes.c_void_p()
+        rc = L.sh_create(ctypes.byref(self._ctx), len(self.tables), arr, int(device))
+        self._keep = []
+        if rc != 0:
+            raise SweepError("sh_create failed (%d)" % rc)
+        self.nchains = len(self.tables)
+
+    def _shmodel(self, t):
+        hs, lat = t["hs"], t["lat"]
+
+        def keep(a, dtype):
+            a = _c(a, dtype)
+            self._keep.append(a)
+            return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32 if dtype == np.int32 else ctypes.c_double))
+
+        cz = lambda z: np.stack([np.real(z), np.imag(z)], axis=-1).reshape(-1)
+        m = t["model"]
+        s = _lib.ShModel()
+        s.ndim, s.lq, s.ltrot, s.nwrap, s.nst = t["ndim"], t["lq"], t["ltrot"], t["nwrap"], t["nst"]
+        s.iwrap_nt = keep(t["iwrap_nt"], np.int32)
+        s.wrap_step = keep(t["wrap_step"].reshape(-1), np.int32)
+        s.lwrapT, s.lwrapu, s.kmode = t["lwrapT"], t["lwrapu"], t["kmode"]
+        s.nn_nf, s.nn_lf = lat["nn_nf"], lat["nn_lf"]
+        s.bonds = keep(lat["bonds"].reshape(-1), np.int32)
+        s.urt = keep(cz(t["urt"]), np.float64)
+        s.urtm1 = keep(cz(t["urtm1"]), np.float64)
+        s.lcomp, s.nflr = hs["lcomp"], m.nflr
+        s.bmat = keep(cz(hs["bmat"]), np.float64)
+        s.bmat_inv = keep(cz(hs["bmat_inv"]), np.float64)
+        s.phase_arg = keep(hs["phase_arg"], np.float64)
+        s.delta_bmat = keep(cz(hs["delta_bmat"].reshape(-1, order="F")), np.float64)
+        s.phase_ratio = keep(cz(hs["phase_ratio"].reshape(-1, order="F")), np.float64)
+        s.gaml = keep(hs["gaml"], np.float64)
+        s.phase_from_conf = t["phase_from_conf"]
+        s.stag = keep(lat["stag"], np.float64)
+        s.update, s.nublock, s.seed = t["update"], t["nublock"], int(np.int32(m.seed))
+        return s
+
+    # -- lifecycle ---------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None) and self._ctx.value:
+            _lib.lib().sh_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = _lib.lib().sh_last_error(self._ctx)
+            raise SweepError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    # -- configuration -----------------------------------------------------------------------
+    def init_conf(self):
+        self._check(_lib.lib().sh_init_conf(self._ctx), "init_conf")
+
+    def get_conf(self, chain=0):
+        t = self.tables[chain]
+        c = np.zeros((t["ltrot"], t["ndim"]), np.int32)
+        self._check(_lib.lib().sh_get_conf(self._ctx, chain, c.ctypes.data_as(_lib.ip)), "get_conf")
+        return c
+
+    def set_conf(self, conf, chain=0):
+        t = self.tables[chain]
+        c = _c(conf, np.int32).reshape(t["ltrot"], t["ndim"])
+        self._check(_lib.lib().sh_set_conf(self._ctx, chain, c.ctypes.data_as(_lib.ip)), "set_conf")
+
+    # -- sweeps ------------------------------------------------------------------------------
+    def sweep_start_0b(self):
+        self._check(_lib.lib().sh_sweep_start_0b(self._ctx), "sweep_start_0b")
+
+    def sweep_b0(self, lupdate=True, lmeasure_equaltime=True):
+        self._check(_lib.lib().sh_sweep_b0(self._ctx, int(lupdate), int(lmeasure_equaltime)), "sweep_b0")
+
+    def sweep_0b(self, lupdate=True, lmeasure_equaltime=True, lmeasure_dyn=False):
+        self._check(_lib.lib().sh_sweep_0b(self._ctx, int(lupdate), int(lmeasure_equaltime), int(lmeasure_dyn)),
+                    "sweep_0b")
+
+    def sweep(self, lupdate=True, lmeasure=True):
+        """One reference sweep: a b0 pass then a 0b pass (src/dqmc_ft_ctrl.f90:90-115)."""
+        self.sweep_b0(lupdate, lmeasure)
+        self.sweep_0b(lupdate, lmeasure)
+
+    def push_stage(self):
+        self._check(_lib.lib().sh_push_stage(self._ctx), "push_stage")
+
+    def pop_stage(self):
+        self._check(_lib.lib().sh_pop_stage(self._ctx), "pop_stage")
+
+    def synchronize(self):
+        self._check(_lib.lib().sh_synchronize(self._ctx), "synchronize")
+
+    # -- state readback ------------------------------------------------------------------------
+    def G(self, chain=0):
+        n = self.tables[chain]["ndim"]
+        g = np.zeros((n, n), np.complex128, order="F")
+        self._check(_lib.lib().sh_get_g(self._ctx, chain, g.ctypes.data_as(_lib.dp)), "get_g")
+        return g
+
+    def diag(self, chain=0):
+        d = np.zeros(10)
+        self._check(_lib.lib().sh_get_diag(self._ctx, chain, d.ctypes.data_as(_lib.dp)), "get_diag")
+        return {"max_wrap_error": d[0], "max_phasediff": d[1], "phase": complex(d[2], d[3]),
+                "logweightf": complex(d[4], d[5]), "accepted": d[6], "tried": d[7], "nchecks": d[8],
+                "sumlog10err": d[9]}
+
+    def reset_diag(self):
+        self._check(_lib.lib().sh_reset_diag(self._ctx), "reset_diag")
+
+    def obs(self, chain=0):
+        o = np.zeros(11)
+        self._check(_lib.lib().sh_get_obs(self._ctx, chain, o.ctypes.data_as(_lib.dp)), "get_obs")
+        z = o[1::2] + 1j * o[2::2]
+        return {"nobs": o[0], "sign": z[0], "density": z[1], "ekin": z[2], "dblocc": z[3], "s_stag": z[4]}
+
+    def reset_obs(self):
+        self._check(_lib.lib().sh_reset_obs(self._ctx), "reset_obs")
+
+    def set_trace(self, cap, chain=0):
+        self._check(_lib.lib().sh_set_trace(self._ctx, chain, int(cap)), "set_trace")
+        self._trace_cap = cap
+
+    def trace(self, chain=0):
+        n = self.tables[chain]["ndim"]
+        cnt = ctypes.c_int32(0)
+        buf = np.zeros((max(self._trace_cap, 1), n, n), np.complex128)
+        self._check(_lib.lib().sh_get_trace(self._ctx, chain, buf.ctypes.data_as(_lib.dp), ctypes.byref(cnt)),
+                    "get_trace")
+        # snapshots are column-major matrices
+        return np.array([b.reshape(n, n).T for b in buf[:cnt.value]])
+
+    def timers(self, enable=True):
+        """Device ms per kernel class since the last call, and launch counts."""
+        t = np.zeros(14)
+        self._check(_lib.lib().sh_timers(self._ctx, int(enable), t.ctypes.data_as(_lib.dp)), "timers")
+        names = ["wrap", "update", "bprod", "qr", "green", "measure", "other"]
+        return {n: (t[k], int(t[7 + k])) for k, n in enumerate(names)}

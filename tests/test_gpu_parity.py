@@ -1,0 +1,84 @@
+"""GPU parity: the HIP sweep (through the C ABI) against the CPU oracle on identical seeds.
+
+Bar: identical accept/reject sequences (the HS field after every pass is
+bit-identical), G within 1e-10 relative max-norm after every stabilization (nwrap), phase within
+1e-10, observable bin sums within 1e-9 relative.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from sunhubbardmy_amd import FTWalkers, Model, chain_seed
+
+pytestmark = pytest.mark.gpu
+
+BASE = dict(lattice="square", la=4, lb=4, beta=2.0, dtau=0.1, rt=1.0, mu=0.0, rhub=4.0, nu=0.0, nflr=2,
+            exact_hs=True, kmode="checkerboard", update="delay", nwrap=10, nublock=4)
+
+CASES = {
+    "square4_delay_checkerboard": {},
+    "square4_fast_checkerboard": dict(update="fast"),
+    "square4_delay_dense": dict(kmode="dense"),
+    "square4_submatrix_checkerboard": dict(update="submatrix", nublock=5),
+    "square6_su4_general_submatrix_dense": dict(la=6, lb=6, nflr=4, exact_hs=False, rhub=6.0, update="submatrix",
+                                                 kmode="dense", nublock=8),
+    "honeycomb3_delay_dense": dict(lattice="honeycomb", la=3, lb=3, kmode="dense", nublock=8),
+    "honeycomb3_delay_checkerboard_nwrap7": dict(lattice="honeycomb", la=3, lb=3, nwrap=7, beta=1.6),
+    "square4_su4_exact_delay": dict(nflr=4, exact_hs=True, rhub=6.0, nublock=3),
+}
+
+
+def relmax(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_parity(kw, nsweeps=2, chain=0):
+    seed = chain_seed(chain)
+    m = Model(seed=seed, **kw)
+    say("case", kw, "seed", seed)
+    w = FTWalkers([m])
+    o = oracle.OracleChain(seed=seed, **kw)
+    w.init_conf()
+    o.init_conf()
+    assert np.array_equal(w.get_conf(), o.conf), "initial HS field differs"
+    w.sweep_start_0b()
+    o.sweep_start()
+    assert relmax(w.G(), o.G) < 1e-10
+    cap = 4 * o.nst * nsweeps + 4
+    w.set_trace(cap)
+    o.set_trace(cap)
+    for s in range(nsweeps):
+        for p in ("b0", "0b"):
+            getattr(w, "sweep_" + p)(True, True)
+            getattr(o, "sweep_" + p)(True, True)
+            say("  sweep", s, p)
+            cw, co = w.get_conf(), o.conf
+            assert np.array_equal(cw, co), "HS field differs after pass %d %s (%d sites)" % (s, p, (cw != co).sum())
+    tw, to = w.trace(), o.trace()
+    assert len(tw) == len(to) and len(tw) > 0
+    for k in range(len(tw)):
+        assert relmax(tw[k], to[k]) < 1e-10, "G differs at stabilization %d: %g" % (k, relmax(tw[k], to[k]))
+    dw, do = w.diag(), o.diag()
+    assert abs(dw["phase"] - do["phase"]) < 1e-10
+    assert dw["accepted"] == do["accepted"] and dw["tried"] == do["tried"]
+    assert dw["max_wrap_error"] < 1e-8
+    ow, oo = w.obs(), o.obs()
+    assert ow["nobs"] == oo["nobs"]
+    for k in ("sign", "density", "ekin", "dblocc", "s_stag"):
+        assert abs(ow[k] - oo[k]) <= 1e-9 * max(1.0, abs(oo[k])), k
+    w.close()
+    return dw, do
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_sweep_parity(name):
+    kw = dict(BASE)
+    kw.update(CASES[name])
+    run_parity(kw)

@@ -1,0 +1,128 @@
+"""CPU tests of the host side: model tables, the C ABI library's exports and its argument checks
+(no compute calls; there is no GPU here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from sunhubbardmy_amd import Model, SweepError, chain_seed, make_tables
+from sunhubbardmy_amd import _lib
+from sunhubbardmy_amd.model import default_nublock, schedule
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(sh_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = _declared("sunhip.h") + _declared("sunhip_testing.h")
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(_declared("sunhip.h")) == set(_lib.EXPORTS)
+
+
+@pytest.mark.parametrize("lattice,la,kmode", [("square", 4, "checkerboard"), ("square", 6, "dense"),
+                                              ("honeycomb", 3, "checkerboard"), ("honeycomb", 4, "dense")])
+def test_tables_match_oracle(lattice, la, kmode):
+    kw = dict(lattice=lattice, la=la, lb=la, kmode=kmode, rhub=4.0, beta=2.3, nwrap=7)
+    t = make_tables(Model(**kw))
+    o = oracle.OracleChain(**kw)
+    assert np.abs(t["urt"] - o.table(4)).max() < 1e-14
+    assert np.abs(t["urtm1"] - o.table(5)).max() < 1e-14
+    assert np.array_equal(t["lat"]["bonds"].reshape(-1), o.table(6).astype(int))
+    assert np.array_equal(t["iwrap_nt"], o.table(7).astype(int))
+    assert np.array_equal(t["wrap_step"].reshape(-1), o.table(8).astype(int))
+    assert t["nst"] == o.nst and t["ltrot"] == o.ltrot
+
+
+@pytest.mark.parametrize("nflr,exact", [(2, True), (4, True), (4, False), (6, True)])
+def test_hs_tables_match_oracle(nflr, exact):
+    hs = make_tables(Model(nflr=nflr, exact_hs=exact, rhub=6.0))["hs"]
+    o = oracle.OracleChain(nflr=nflr, exact_hs=exact, rhub=6.0)
+    t0 = o.table(0).reshape(-1, 3)
+    assert np.abs(hs["bmat"] - t0[:, 0]).max() == 0
+    assert np.abs(hs["bmat_inv"] - t0[:, 1]).max() == 0
+    assert np.abs(np.exp(1j * hs["phase_arg"]) - t0[:, 2]).max() < 1e-15
+    assert np.abs(hs["delta_bmat"].reshape(-1, order="F") - o.table(1)).max() == 0
+    assert np.abs(hs["phase_ratio"].reshape(-1, order="F") - o.table(2)).max() == 0
+
+
+def test_schedule_matches_reference_rules():
+    # ltrot multiple of nwrap
+    nst, nw, iw, ws = schedule(80, 10)
+    assert nst == 8 and nw == 10 and list(np.nonzero(iw)[0]) == list(range(10, 81, 10))
+    assert ws[0].tolist() == [1, 10] and ws[-1].tolist() == [71, 80]
+    # remainder slice block, and nwrap > ltrot clamps to ltrot
+    nst, nw, iw, ws = schedule(23, 10)
+    assert nst == 3 and iw[23] == 3 and ws[-1].tolist() == [21, 23]
+    nst, nw, iw, ws = schedule(8, 10)
+    assert nst == 1 and nw == 8 and ws[0].tolist() == [1, 8]
+
+
+def test_default_nublock_tables():
+    assert default_nublock(16, "delay") == 4
+    assert default_nublock(256, "delay") == 16     # 256/5 = 51 < 64
+    assert default_nublock(288, "delay") == 16
+    assert default_nublock(400, "delay") == 32
+    assert default_nublock(1400, "delay") == 64
+    assert default_nublock(256, "submatrix") == 32
+    # the reference's non-monotone entry (ndim/20 in [64,128) -> 16)
+    assert default_nublock(1500, "submatrix") == 16
+
+
+def test_chain_seed_matches_oracle():
+    for c in range(50):
+        assert chain_seed(c) == oracle.chain_seed(c)
+
+
+def _model_struct(t, **over):
+    from sunhubbardmy_amd.walkers import FTWalkers
+    w = FTWalkers.__new__(FTWalkers)
+    w._keep = []
+    s = FTWalkers._shmodel(w, t)
+    for k, v in over.items():
+        setattr(s, k, v)
+    return s, w._keep
+
+
+def _create(structs):
+    arr = (_lib.ShModel * len(structs))(*structs)
+    ctx = ctypes.c_void_p()
+    rc = _lib.lib().sh_create(ctypes.byref(ctx), len(structs), arr, 0)
+    return rc, ctx
+
+
+def test_create_rejects_bad_arguments_before_touching_the_device():
+    t = make_tables(Model(la=4, lb=4, beta=2.0))
+    t2 = make_tables(Model(la=4, lb=4, beta=3.0))
+    s1, k1 = _model_struct(t)
+    s2, k2 = _model_struct(t2)
+    assert _create([s1, s2])[0] == -1          # chains disagree on ltrot
+    s3, k3 = _model_struct(t, ndim=2048)
+    assert _create([s3])[0] == -1              # above the 1024 supported
+    s4, k4 = _model_struct(t, nublock=65)
+    assert _create([s4])[0] == -1
+    bad = np.array(t["lat"]["bonds"]).reshape(-1).copy()
+    bad[3] = 16
+    s5, k5 = _model_struct(t)
+    s5.bonds = bad.ctypes.data_as(_lib.ip)
+    assert _create([s5])[0] == -1              # bond index outside the lattice
+    s6, k6 = _model_struct(t, lcomp=5)
+    assert _create([s6])[0] == -1
+    assert _lib.lib().sh_destroy(None) == 0
+    assert _lib.lib().sh_sweep_b0(None, 1, 1) == -1
+    assert _lib.lib().sh_last_error(None) == b"null context"
+
+
+def test_walkers_raise_on_mixed_batches():
+    with pytest.raises(SweepError):
+        from sunhubbardmy_amd import FTWalkers
+        FTWalkers([Model(beta=2.0), Model(beta=3.0)])
