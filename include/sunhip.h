@@ -100,6 +100,9 @@ int sh_reset_diag(sh_ctx *ctx);
  * (energy_bin(3)), zint_u (energy_bin(4)), S_stag (zspsm Fourier component at the
  * staggering wave vector), all weighted by zphi as in equaltime_measure */
 int sh_get_obs(sh_ctx *ctx, int chain, double *obs);
+/* obs[11*nchains] of every chain in one readback, then (reset != 0) zero the sums: the
+ * per-sweep bins of a batch (equaltime_output with nsweep = 1) */
+int sh_get_obs_all(sh_ctx *ctx, double *obs, int reset);
 int sh_reset_obs(sh_ctx *ctx);
 
 /* Record G after every stabilization (nwrap) of chain `chain`, up to cap snapshots; buf is
@@ -107,9 +110,10 @@ int sh_reset_obs(sh_ctx *ctx);
 int sh_set_trace(sh_ctx *ctx, int chain, int cap);
 int sh_get_trace(sh_ctx *ctx, int chain, double *buf, int *count);
 
-/* per-kernel-class device time in ms since the last reset (hipEvent pairs around classes):
- * t[0] wrap (K and V), t[1] update, t[2] stabilization B-products, t[3] QR, t[4] green
- * (GEMM + LU + solve), t[5] measure, t[6] other; enable = 0 turns the events off */
+/* per-kernel-group device time since the last call, from hipEvent pairs on the library's
+ * stream: t[0..9] ms of wrap (K and V on G), update, stabilization B products, QR (prepivot +
+ * Householder), form Q, ZGEMM, LU, LU solve, measure, other; t[10..19] launch counts of the
+ * same groups.  enable = 0 stops recording (events cost a little launch overhead). */
 int sh_timers(sh_ctx *ctx, int enable, double *t);
 int sh_synchronize(sh_ctx *ctx);
 

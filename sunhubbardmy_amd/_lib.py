@@ -29,7 +29,7 @@ class ShModel(ctypes.Structure):
 EXPORTS = [
     "sh_create", "sh_destroy", "sh_last_error", "sh_init_conf", "sh_set_conf", "sh_get_conf",
     "sh_sweep_start_0b", "sh_sweep_b0", "sh_sweep_0b", "sh_push_stage", "sh_pop_stage",
-    "sh_get_g", "sh_get_diag", "sh_reset_diag", "sh_get_obs", "sh_reset_obs", "sh_set_trace",
+    "sh_get_g", "sh_get_diag", "sh_reset_diag", "sh_get_obs", "sh_get_obs_all", "sh_reset_obs", "sh_set_trace",
     "sh_get_trace", "sh_timers", "sh_synchronize",
 ]
 
@@ -58,6 +58,7 @@ def lib():
     L.sh_get_diag.argtypes = [P, ctypes.c_int, dp]
     L.sh_reset_diag.argtypes = [P]
     L.sh_get_obs.argtypes = [P, ctypes.c_int, dp]
+    L.sh_get_obs_all.argtypes = [P, dp, ctypes.c_int]
     L.sh_reset_obs.argtypes = [P]
     L.sh_set_trace.argtypes = [P, ctypes.c_int, ctypes.c_int]
     L.sh_get_trace.argtypes = [P, ctypes.c_int, dp, ip]

@@ -12,7 +12,20 @@
 
 namespace {
 
-enum { T_WRAP = 0, T_UPDATE, T_BPROD, T_QR, T_GREEN, T_MEASURE, T_OTHER, T_NCLS };
+// timer classes: one per kernel group, in the order reported by sh_timers
+enum {
+    T_WRAP = 0,   // k_cb_cols / k_cb_rows / dense-K GEMMs on G, per slice
+    T_UPDATE,     // k_update_delay / k_update_submatrix
+    T_BPROD,      // B products of the stabilization (k_cb_cols over nwrap slices)
+    T_QR,         // k_prepivot + k_qr
+    T_FORMQ,      // k_formq
+    T_GEMM,       // ZGEMMs of the stabilization and green_equaltime
+    T_LU,         // k_lu
+    T_SOLVE,      // k_lu_solve
+    T_MEASURE,    // k_measure
+    T_OTHER,      // elementwise helpers, wrap check
+    T_NCLS
+};
 
 struct EvRec {
     int cls;
@@ -192,6 +205,9 @@ static void udv_step(sh_ctx *ctx, bool first, int pt, int dpt)
         {
             Tic t(ctx, T_QR);
             hipLaunchKernelGGL(k_qr, dim3(ctx->nch), dim3(1024), lds_qr(ctx), ctx->st, ctx->dc, M_W1, 1, pt);
+        }
+        {
+            Tic t(ctx, T_FORMQ);
             hipLaunchKernelGGL(k_formq, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * sizeof(cplx), ctx->st, ctx->dc,
                                M_W1, pt);
         }
@@ -203,12 +219,18 @@ static void udv_step(sh_ctx *ctx, bool first, int pt, int dpt)
         Tic t(ctx, T_QR);
         hipLaunchKernelGGL(k_prepivot, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * 12, ctx->st, ctx->dc, dpt);
         hipLaunchKernelGGL(k_qr, dim3(ctx->nch), dim3(1024), lds_qr(ctx), ctx->st, ctx->dc, M_W2, 0, pt);
+    }
+    {
+        Tic t(ctx, T_FORMQ);
         hipLaunchKernelGGL(k_formq, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * sizeof(cplx), ctx->st, ctx->dc,
                            M_W2, pt);
     }
-    Tic t(ctx, T_GREEN);
-    // V_new = D^-1 R P^T V_prev: W3 = D^-1 triu R, W4 = P^T V[dpt]
-    hipLaunchKernelGGL(k_after_qr, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, pt, (int)M_V, dpt);
+    {
+        Tic t(ctx, T_OTHER);
+        // V_new = D^-1 R P^T V_prev: W3 = D^-1 triu R, W4 = P^T V[dpt]
+        hipLaunchKernelGGL(k_after_qr, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, pt, (int)M_V, dpt);
+    }
+    Tic t(ctx, T_GEMM);
     gemm<0, 0>(ctx, M_W3, 0, M_W4, 0, M_V, pt);
 }
 
@@ -241,17 +263,38 @@ static void stablize_b0(sh_ctx *ctx, int n)
 // green_equaltime into Gs + scratch logweight (rsaved/lsaved select the saved chain point)
 static void green(sh_ctx *ctx, int rsaved, int rpt, int lsaved, int lpt)
 {
-    Tic t(ctx, T_GREEN);
     const int URs = rsaved ? M_US : M_U, VRs = rsaved ? M_VS : M_V;
     const int ULs = lsaved ? M_US : M_U, VLs = lsaved ? M_VS : M_V;
-    gemm<1, 0>(ctx, URs, rsaved ? 0 : rpt, ULs, lsaved ? 0 : lpt, M_W1, 0);   // uu = UR^H UL
-    gemm<0, 1>(ctx, VRs, rsaved ? 0 : rpt, VLs, lsaved ? 0 : lpt, M_W2, 0);   // vv = VR VL^H
-    hipLaunchKernelGGL(k_green_m, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, 0, rsaved, rpt, lsaved, lpt);
-    hipLaunchKernelGGL(k_lu, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * 2 * sizeof(cplx), ctx->st, ctx->dc);
-    hipLaunchKernelGGL(k_green_rhs, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, rsaved, rpt);
-    hipLaunchKernelGGL(k_lu_solve, dim3(ctx->nch), dim3(512), 0, ctx->st, ctx->dc, (int)M_W4);
-    hipLaunchKernelGGL(k_green_ulscale, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, lsaved, lpt);
-    gemm<0, 0>(ctx, M_W1, 0, M_W4, 0, M_GS, 0);
+    {
+        Tic t(ctx, T_GEMM);
+        gemm<1, 0>(ctx, URs, rsaved ? 0 : rpt, ULs, lsaved ? 0 : lpt, M_W1, 0);   // uu = UR^H UL
+        gemm<0, 1>(ctx, VRs, rsaved ? 0 : rpt, VLs, lsaved ? 0 : lpt, M_W2, 0);   // vv = VR VL^H
+    }
+    {
+        Tic t(ctx, T_OTHER);
+        hipLaunchKernelGGL(k_green_m, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, 0, rsaved, rpt, lsaved, lpt);
+    }
+    {
+        Tic t(ctx, T_LU);
+        hipLaunchKernelGGL(k_lu, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * 2 * sizeof(cplx), ctx->st, ctx->dc);
+    }
+    {
+        Tic t(ctx, T_OTHER);
+        hipLaunchKernelGGL(k_green_rhs, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, rsaved, rpt);
+    }
+    {
+        Tic t(ctx, T_SOLVE);
+        hipLaunchKernelGGL(k_lu_solve, dim3(ctx->nch), dim3(512), 0, ctx->st, ctx->dc, (int)M_W4);
+    }
+    {
+        Tic t(ctx, T_OTHER);
+        hipLaunchKernelGGL(k_green_ulscale, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, lsaved, lpt);
+    }
+    {
+        Tic t(ctx, T_GEMM);
+        gemm<0, 0>(ctx, M_W1, 0, M_W4, 0, M_GS, 0);
+    }
+    Tic t(ctx, T_OTHER);
     hipLaunchKernelGGL(k_green_logw, dim3(ctx->nch), dim3(256), 0, ctx->st, ctx->dc, 0, rsaved, rpt, lsaved, lpt,
                        (int)SC_LOGW_TMP);
 }
@@ -259,7 +302,7 @@ static void green(sh_ctx *ctx, int rsaved, int rpt, int lsaved, int lpt)
 // green_equaltimebb: G(beta,beta) from the right chain point nst
 static void green_bb(sh_ctx *ctx, int rpt)
 {
-    Tic t(ctx, T_GREEN);
+    Tic t(ctx, T_OTHER);
     hipLaunchKernelGGL(k_green_m, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, 1, 0, rpt, 0, 0);
     hipLaunchKernelGGL(k_lu, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * 2 * sizeof(cplx), ctx->st, ctx->dc);
     hipLaunchKernelGGL(k_green_rhs, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, 0, rpt);
@@ -663,6 +706,24 @@ int sh_get_obs(sh_ctx *ctx, int chain, double *o)
     HIPCHK(hipStreamSynchronize(ctx->st));
     o[0] = sc[SC_NOBS];
     for (int k = 0; k < 10; k++) o[1 + k] = sc[SC_OBS + k];
+    return 0;
+}
+
+int sh_get_obs_all(sh_ctx *ctx, double *o, int reset)
+{
+    if (!ctx || !o) return SH_E_ARG;
+    std::vector<double> sc((size_t)SC_SIZE * ctx->nch);
+    for (int ci = 0; ci < ctx->nch; ci++)
+        HIPCHK(hipMemcpyAsync(sc.data() + (size_t)SC_SIZE * ci, ctx->hc[ci].sc, SC_SIZE * sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->st));
+    if (reset)
+        for (auto &c : ctx->hc) HIPCHK(hipMemsetAsync(c.sc + SC_NOBS, 0, 11 * sizeof(double), ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (int ci = 0; ci < ctx->nch; ci++) {
+        const double *q = sc.data() + (size_t)SC_SIZE * ci;
+        o[11 * ci] = q[SC_NOBS];
+        for (int k = 0; k < 10; k++) o[11 * ci + 1 + k] = q[SC_OBS + k];
+    }
     return 0;
 }
 

@@ -15,6 +15,9 @@ from . import _lib
 from .model import Model, make_tables
 
 
+TIMER_GROUPS = ["wrap", "update", "bprod", "qr", "formq", "gemm", "lu", "solve", "measure", "other"]
+
+
 class SweepError(RuntimeError):
     pass
 
@@ -152,6 +155,12 @@ class FTWalkers:
         z = o[1::2] + 1j * o[2::2]
         return {"nobs": o[0], "sign": z[0], "density": z[1], "ekin": z[2], "dblocc": z[3], "s_stag": z[4]}
 
+    def obs_all(self, reset=True):
+        """Observable bin sums of every chain (array [nchains, 5] complex) and nobs; resets them."""
+        o = np.zeros((self.nchains, 11))
+        self._check(_lib.lib().sh_get_obs_all(self._ctx, o.ctypes.data_as(_lib.dp), int(reset)), "get_obs_all")
+        return o[:, 1::2] + 1j * o[:, 2::2], o[:, 0]
+
     def reset_obs(self):
         self._check(_lib.lib().sh_reset_obs(self._ctx), "reset_obs")
 
@@ -170,7 +179,7 @@ class FTWalkers:
 
     def timers(self, enable=True):
         """Device ms per kernel class since the last call, and launch counts."""
-        t = np.zeros(14)
+        t = np.zeros(2 * len(TIMER_GROUPS))
         self._check(_lib.lib().sh_timers(self._ctx, int(enable), t.ctypes.data_as(_lib.dp)), "timers")
-        names = ["wrap", "update", "bprod", "qr", "green", "measure", "other"]
-        return {n: (t[k], int(t[7 + k])) for k, n in enumerate(names)}
+        g = len(TIMER_GROUPS)
+        return {n: (t[k], int(t[g + k])) for k, n in enumerate(TIMER_GROUPS)}
