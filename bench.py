@@ -1,0 +1,165 @@
+"""Throughput of the DQMC sweep on one MI355X.
+
+Workload: SU(2) Hubbard on the 16x16 periodic square lattice, t = 1, U = 6, half filling,
+beta = 8, dtau = 0.1 (Ltrot = 80), nwrap = 10, exact SU(2) HS field, checkerboard kinetic
+term, delay update; a batch of independent chains (seeds abs((c*1981+2008)*951049)), each
+what one MPI rank of the reference runs.  Synthetic: random initial HS fields from each
+chain's SFMT stream.  A step is one sweep (b0 pass + 0b pass, measurement on every slice) of
+every chain.  Metric: tau_int-corrected samples/s = chain sweeps/s / (2 tau_int), tau_int the
+largest of double occupancy, kinetic energy and S(pi,pi) from the per-sweep bins.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--chains C] [--nublock k]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "τ_int-corrected samples/s, SU(2) 16×16 β=8, learned vs plain, 1 GPU, FP64 G"
+FP64_MFMA_PEAK = 78.6e12   # MI355X dense FP64 matrix, FLOP/s
+HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
+
+
+def model_kw(args):
+    return dict(lattice="square", la=16, lb=16, beta=8.0, dtau=0.1, rt=1.0, mu=0.0, rhub=6.0, nu=0.0, nflr=2,
+                exact_hs=True, kmode="checkerboard", update="delay", nwrap=10, nublock=args.nublock)
+
+
+def algorithmic_work(group, n, nchains, ltrot, nwrap, acc, k):
+    """(kind, amount per launch) of the kernel group, all chains of the launch."""
+    nn = float(n) * n
+    if group == "wrap":
+        return "bytes", nchains * 2 * 16 * nn                       # read + write G
+    if group == "measure":
+        return "bytes", nchains * 16 * nn                           # read G
+    if group == "bprod":
+        return "bytes", nchains * 2 * 16 * nn                       # one pass over the propagated U
+    if group == "update":
+        return "flops", nchains * 8.0 * acc * n * nn * (1.0 + k / n)  # rank-k flushes + column builds
+    if group in ("qr", "formq"):
+        return "flops", nchains * 16.0 / 3.0 * n * nn
+    if group == "gemm":
+        return "flops", nchains * 8.0 * n * nn
+    if group == "lu":
+        return "flops", nchains * 8.0 / 3.0 * n * nn
+    if group == "solve":
+        return "flops", nchains * 8.0 * n * nn
+    return "bytes", nchains * 2 * 16 * nn
+
+
+def cpu_baseline(kw, tau):
+    """The CPU oracle (plain C, one thread) on a bounded sample: one sweep of one chain."""
+    import oracle
+    o = oracle.OracleChain(seed=oracle.chain_seed(0), **kw)
+    o.init_conf()
+    o.sweep_start()
+    t0 = time.perf_counter()
+    o.sweep(lupdate=True, lmeasure=True)
+    dt = time.perf_counter() - t0
+    sweeps_per_s = 1.0 / dt
+    return {"value": sweeps_per_s / (2.0 * tau), "unit": "samples/s", "cores": 1, "kind": "port",
+            "sample": "1 sweep (b0+0b, measured) of one chain of the same workload, %.1f s; "
+                      "sweeps/s %.4g divided by 2 tau_int of the GPU run" % (dt, sweeps_per_s),
+            "sweeps_per_s": sweeps_per_s}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--chains", type=int, default=256)
+    ap.add_argument("--nublock", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timers", dest="timers", action="store_false",
+                    help="skip the per-kernel hipEvent timing of the timed steps")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; chains are independent, so N GPUs would run N "
+              "independent batches (no collective on the sweep path). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from sunhubbardmy_amd import FTWalkers, Model, chain_seed
+    from sunhubbardmy_amd.autocorr import tau_int
+
+    kw = model_kw(args)
+    models = [Model(seed=chain_seed(c), **kw) for c in range(args.chains)]
+    w = FTWalkers(models)
+    t = w.tables[0]
+    n, ltrot = t["ndim"], t["ltrot"]
+    print("# %d chains, N=%d, Ltrot=%d, nublock=%d" % (args.chains, n, ltrot, t["nublock"]), file=sys.stderr,
+          flush=True)
+    w.init_conf()
+    w.sweep_start_0b()
+    for s in range(args.warmup):
+        w.sweep(lupdate=True, lmeasure=False)
+        if s % 10 == 0:
+            print("# warmup %d" % s, file=sys.stderr, flush=True)
+    w.reset_obs()
+    w.reset_diag()
+    w.synchronize()
+    w.timers(enable=args.timers)
+    series = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        w.sweep(lupdate=True, lmeasure=True)
+        z, nobs = w.obs_all(reset=True)
+        series.append(np.real(z) / nobs[:, None])
+        if s % 10 == 0:
+            print("# step %d" % s, file=sys.stderr, flush=True)
+    w.synchronize()
+    t1 = time.perf_counter()
+    tm = w.timers(enable=False)
+    dt = t1 - t0
+    series = np.array(series)            # [steps, chains, 5]
+    taus = {}
+    for name, col in (("dblocc", 3), ("ekin", 2), ("s_pipi", 4)):
+        tau, win, _ = tau_int(series[:, :, col].T)
+        taus[name] = round(tau, 3)
+    tau = max(taus.values())
+    sweeps_per_s = args.chains * args.steps / dt
+    value = sweeps_per_s / (2.0 * tau)
+    acc = np.mean([w.diag(c)["accepted"] / max(w.diag(c)["tried"], 1) for c in range(min(args.chains, 8))])
+    means = series.mean(axis=(0, 1))
+    line = {
+        "metric": METRIC, "value": value, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1000.0 * dt / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "SU(2) Hubbard square 16x16 (N=256), U=6, t=1, half filling, beta=8, dtau=0.1 "
+                               "(Ltrot=80), nwrap=10, checkerboard K, delay update, plain local sweeps",
+                   "chains": args.chains, "nublock": t["nublock"], "proposal": "plain (learned proposal not built)"},
+        "sweeps_per_s": sweeps_per_s, "tau_int": taus, "acceptance": round(float(acc), 4),
+        "observables": {"dblocc_per_site": means[3] / n, "ekin_per_site_flavour": means[2] / n,
+                        "s_pipi": means[4]},
+    }
+    if args.timers:
+        a = float(acc)
+        best = max(tm, key=lambda g: tm[g][0])
+        ms, cnt = tm[best]
+        kind, amount = algorithmic_work(best, n, args.chains, ltrot, 10, a, t["nublock"])
+        avg_s = ms / 1000.0 / max(cnt, 1)
+        achieved = amount / avg_s
+        if kind == "flops":
+            roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
+                    "frac": achieved / FP64_MFMA_PEAK, "traffic": None}
+        else:
+            roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK, "traffic": None}
+        roof["kernel_group"] = best
+        roof["avg_launch_ms"] = ms / max(cnt, 1)
+        line["roofline"] = roof
+        line["kernel_ms"] = {g: round(v[0], 3) for g, v in tm.items()}
+        line["kernel_launches"] = {g: v[1] for g, v in tm.items()}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(kw, tau)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

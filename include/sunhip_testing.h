@@ -31,7 +31,8 @@ int sh_dbg_get(sh_ctx *ctx, int chain, int sel, int pt, double *dst);
  * 10: delay (or submatrix) update of slice a0 using the pass uniforms of slice index a1
  * 11: draw the pass uniforms (2 N ltrot) from each chain's generator
  * 12: measurement of G into the observable sums
- * 13: wrap check/replace (compare = a0) */
+ * 13: wrap check/replace (compare = a0)
+ * 14: a0 != 0 selects the delay update that keeps its vectors in L2 instead of registers */
 int sh_dbg_op(sh_ctx *ctx, int op, const int *a);
 
 #ifdef __cplusplus

@@ -1,0 +1,510 @@
+// blocked.hip -- blocked factorizations of one chain's N x N matrix inside one workgroup:
+// 16-wide panels are factored in LDS, and every trailing update is a rank-16 product on FP64
+// MFMA (v_mfma_f64_16x16x4_f64), so each panel streams the trailing matrix once instead of
+// once per column.  Used for N <= BLK_NMAX; larger chains keep the column-at-a-time kernels.
+//   k_qr_blk     Householder QR without pivoting (zgeqrf + zlarft), T factors kept per panel
+//   k_formq_blk  Q = H_1 ... H_N from the reflectors and the T factors (zungqr)
+//   k_lu_blk     LU with partial pivoting, pivot = largest |Re|+|Im| (zgetrf)
+//   k_solve_blk  (P L U) Y = X for N right-hand sides
+#include "sunhip_kernels.h"
+
+#define PB 16          // panel width
+#define BLK_NT 512     // threads per workgroup (8 waves)
+#define BLK_NMAX 512   // LDS holds a 512 x 16 complex panel (128 KiB)
+
+// MFMA operand layout (gfx950): A(16x4): lane l holds A[l%16][l/16]; B(4x16): lane l holds
+// B[l/16][l%16]; D(16x16): lane l, element r holds D[l/16 + 4r][l%16].
+__device__ __forceinline__ double4_t mfma64(double a, double b, double4_t c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ static double blk_sum(double v, double *red)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < nw; i++) s += red[i];
+    return s;
+}
+
+__device__ static void blk_argmax(double v, int idx, double *rv, int *ri, int *outi)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) {
+        double v2 = __shfl_xor(v, o, 64);
+        int i2 = __shfl_xor(idx, o, 64);
+        if (v2 > v || (v2 == v && i2 < idx)) { v = v2; idx = i2; }
+    }
+    __syncthreads();
+    if (lane == 0) { rv[w] = v; ri[w] = idx; }
+    __syncthreads();
+    double bv = rv[0];
+    int bi = ri[0];
+    for (int i = 1; i < nw; i++)
+        if (rv[i] > bv || (rv[i] == bv && ri[i] < bi)) { bv = rv[i]; bi = ri[i]; }
+    *outi = bi;
+}
+
+// V[r][p] of a Householder panel stored LAPACK-style (unit diagonal implicit, zeros above)
+__device__ __forceinline__ cplx vpanel(const cplx *P, int ldp, int r, int p)
+{
+    if (r > p) return P[p * ldp + r];
+    return make_double2(r == p ? 1.0 : 0.0, 0.0);
+}
+
+// W (16 x nc, global, ld 16) = V^H C, V = m x 16 Householder panel in LDS, C global (m x nc, ldc)
+__device__ static void wg_vhc(cplx *W, const cplx *P, int ldp, int m, int b, const cplx *C, int ldc, int nc)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const int ntile = (nc + 15) >> 4;
+    for (int t = w; t < ntile; t += nw) {
+        const int c0 = t * 16;
+        double4_t ar = {0, 0, 0, 0}, ai = {0, 0, 0, 0};
+        for (int r0 = 0; r0 < m; r0 += 4) {
+            const int r = r0 + kq;
+            cplx x = make_double2(0.0, 0.0), y = make_double2(0.0, 0.0);
+            if (r < m) {
+                if (l16 < b) x = cconj(vpanel(P, ldp, r, l16));         // X[p][k] = conj V[k][p]
+                if (c0 + l16 < nc) y = C[(size_t)r + (size_t)(c0 + l16) * ldc];  // Y[k][q] = C[k][c0+q]
+            }
+            ar = mfma64(x.x, y.x, ar);
+            ar = mfma64(-x.y, y.y, ar);
+            ai = mfma64(x.x, y.y, ai);
+            ai = mfma64(x.y, y.x, ai);
+        }
+        for (int rr = 0; rr < 4; rr++) {
+            const int p = kq + 4 * rr, q = c0 + l16;
+            if (q < nc) W[p + 16 * (size_t)q] = make_double2(ar[rr], ai[rr]);
+        }
+    }
+}
+
+// W <- op(T) W, T 16 x 16 upper triangular in LDS (col-major), op = T^H (herm) or T
+__device__ static void wg_tmul(cplx *W, int nc, const cplx *T, int b, bool herm)
+{
+    for (int q = threadIdx.x; q < nc; q += blockDim.x) {
+        cplx x[PB], y[PB];
+#pragma unroll
+        for (int p = 0; p < PB; p++) x[p] = (p < b) ? W[p + 16 * (size_t)q] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int p = 0; p < PB; p++) {
+            cplx s = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int pp = 0; pp < PB; pp++) {
+                if (herm) {            // (T^H)[p][pp] = conj T[pp][p], nonzero for pp <= p
+                    if (pp <= p) s = cadd(s, cmul(cconj(T[p * PB + pp]), x[pp]));
+                } else {               // T[p][pp], nonzero for pp >= p
+                    if (pp >= p) s = cadd(s, cmul(T[pp * PB + p], x[pp]));
+                }
+            }
+            y[p] = s;
+        }
+#pragma unroll
+        for (int p = 0; p < PB; p++)
+            if (p < b) W[p + 16 * (size_t)q] = y[p];
+    }
+}
+
+// C (mrows x ncols, global, ldc) -= A B with A = 16 columns in LDS (A[p*lda + r], unit-panel
+// accessor when vp = 1) and B = 16 x ncols in global memory (B[p + c*ldb]).  The accumulator
+// holds C^T tiles so that the global reads and writes of C are 16-element column runs.
+__device__ static void wg_rank16(cplx *C, int ldc, int mrows, int ncols, const cplx *A, int lda, int roff, int b,
+                                 bool vp, const cplx *B, int ldb)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const int mt = (mrows + 15) >> 4, nt = (ncols + 15) >> 4;
+    for (int t = w; t < mt * nt; t += nw) {
+        const int r0 = (t % mt) * 16, c0 = (t / mt) * 16;
+        const int row = r0 + l16;
+        double4_t ar, ai;
+        for (int j = 0; j < 4; j++) {
+            const int col = c0 + kq + 4 * j;
+            cplx v = (row < mrows && col < ncols) ? C[(size_t)row + (size_t)col * ldc] : make_double2(0.0, 0.0);
+            ar[j] = v.x;
+            ai[j] = v.y;
+        }
+#pragma unroll
+        for (int p0 = 0; p0 < PB; p0 += 4) {
+            const int p = p0 + kq;
+            cplx x = make_double2(0.0, 0.0), y = make_double2(0.0, 0.0);
+            if (p < b) {
+                if (c0 + l16 < ncols) x = B[p + (size_t)(c0 + l16) * ldb];               // X[c][p] = B[p][c]
+                if (row < mrows) y = vp ? vpanel(A, lda, roff + row, p) : A[p * lda + roff + row];  // Y[p][r]
+            }
+            ar = mfma64(-x.x, y.x, ar);
+            ar = mfma64(x.y, y.y, ar);
+            ai = mfma64(-x.x, y.y, ai);
+            ai = mfma64(-x.y, y.x, ai);
+        }
+        for (int j = 0; j < 4; j++) {
+            const int col = c0 + kq + 4 * j;
+            if (row < mrows && col < ncols) C[(size_t)row + (size_t)col * ldc] = make_double2(ar[j], ai[j]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// QR without pivoting, blocked
+// ------------------------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(BLK_NT)
+k_qr_blk(const DChain *chains, int sel, int out_pt)
+{
+    extern __shared__ cplx bsh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    cplx *P = bsh;                      // n x PB panel, ld n
+    cplx *T = bsh + (size_t)n * PB;     // PB x PB
+    cplx *S = T + PB * PB;              // PB x PB Gram matrix
+    __shared__ double red[16];
+    __shared__ cplx s_par[4];
+    cplx *A = chain_mat(c, sel, 0);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int k0 = 0; k0 < n; k0 += PB) {
+        const int m = n - k0, b = min(PB, m);
+        for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
+            int r = e % m, p = e / m;
+            P[p * n + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * n];
+        }
+        __syncthreads();
+        for (int j = 0; j < b; j++) {
+            double s = 0.0;
+            for (int r = j + 1 + threadIdx.x; r < m; r += blockDim.x) s += cabs2(P[j * n + r]);
+            s = blk_sum(s, red);
+            if (threadIdx.x == 0) {
+                cplx alpha = P[j * n + j];
+                cplx tau = make_double2(0.0, 0.0), scal = make_double2(1.0, 0.0);
+                double beta = alpha.x;
+                if (!(s == 0.0 && alpha.y == 0.0)) {
+                    double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + s);
+                    beta = alpha.x >= 0 ? -nrm : nrm;
+                    tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+                    scal = cinv(make_double2(alpha.x - beta, alpha.y));
+                    P[j * n + j] = make_double2(beta, 0.0);
+                }
+                s_par[0] = tau;
+                s_par[1] = scal;
+                c.tau[k0 + j] = tau;
+            }
+            __syncthreads();
+            const cplx tau = s_par[0], scal = s_par[1];
+            for (int r = j + 1 + threadIdx.x; r < m; r += blockDim.x) P[j * n + r] = cmul(P[j * n + r], scal);
+            __syncthreads();
+            // H_j^H on the remaining panel columns, one wave per column
+            const cplx tc = cconj(tau);
+            for (int q = j + 1 + w; q < b; q += nw) {
+                double sr = 0.0, si = 0.0;
+                for (int r = j + lane; r < m; r += 64) {
+                    cplx v = (r == j) ? make_double2(1.0, 0.0) : P[j * n + r], a = P[q * n + r];
+                    sr += v.x * a.x + v.y * a.y;
+                    si += v.x * a.y - v.y * a.x;
+                }
+                sr = wave_sum(sr);
+                si = wave_sum(si);
+                cplx f = cmul(tc, make_double2(sr, si));
+                for (int r = j + lane; r < m; r += 64) {
+                    cplx v = (r == j) ? make_double2(1.0, 0.0) : P[j * n + r];
+                    P[q * n + r] = csub(P[q * n + r], cmul(v, f));
+                }
+            }
+            __syncthreads();
+        }
+        // T factor (zlarft, forward, columnwise): S = V^H V, T(:,j) = -tau_j T(:,0:j) S(0:j, j)
+        for (int pr = w; pr < b * b; pr += nw) {
+            int i = pr % b, j = pr / b;
+            if (i >= j) continue;
+            double sr = 0.0, si = 0.0;
+            for (int r = j + lane; r < m; r += 64) {
+                cplx vi = vpanel(P, n, r, i), vj = vpanel(P, n, r, j);
+                sr += vi.x * vj.x + vi.y * vj.y;
+                si += vi.x * vj.y - vi.y * vj.x;
+            }
+            sr = wave_sum(sr);
+            si = wave_sum(si);
+            if (lane == 0) S[j * PB + i] = make_double2(sr, si);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int j = 0; j < PB; j++)
+                for (int i = 0; i < PB; i++) T[j * PB + i] = make_double2(0.0, 0.0);
+            for (int j = 0; j < b; j++) {
+                cplx tj = c.tau[k0 + j];
+                T[j * PB + j] = tj;
+                for (int i = 0; i < j; i++) {
+                    cplx acc = make_double2(0.0, 0.0);
+                    for (int q = i; q < j; q++) acc = cadd(acc, cmul(T[q * PB + i], S[j * PB + q]));
+                    T[j * PB + i] = cmul(make_double2(-tj.x, -tj.y), acc);
+                }
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < PB * PB; e += blockDim.x) c.tblk[(size_t)(k0 / PB) * PB * PB + e] = T[e];
+        for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
+            int r = e % m, p = e / m;
+            A[(size_t)(k0 + r) + (size_t)(k0 + p) * n] = P[p * n + r];
+        }
+        const int nc = n - k0 - b;
+        if (nc > 0) {
+            cplx *Ct = A + (size_t)k0 + (size_t)(k0 + b) * n;
+            wg_vhc(c.wk, P, n, m, b, Ct, n, nc);
+            __syncthreads();
+            wg_tmul(c.wk, nc, T, b, true);
+            __syncthreads();
+            wg_rank16(Ct, n, m, nc, P, n, 0, b, true, c.wk, PB);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cplx s = make_double2(0.0, 0.0);
+        for (int i = 0; i < n; i++) {
+            cplx t = c.tau[i];
+            if (t.x != 0.0 || t.y != 0.0) {
+                cplx q = cdiv(cscale(t, 2.0 * t.x), cmul(t, cconj(t)));
+                s = cadd(s, clog_(make_double2(1.0 - q.x, -q.y)));
+            }
+        }
+        c.ldq[out_pt] = s;
+    }
+}
+
+// Q = H_1 ... H_n into U[out_pt]: backward accumulation by panels, Q(k0:, k0:) <- (I - V T V^H) Q(k0:, k0:)
+extern "C" __global__ void __launch_bounds__(BLK_NT)
+k_formq_blk(const DChain *chains, int sel, int out_pt)
+{
+    extern __shared__ cplx bsh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    cplx *P = bsh;
+    cplx *T = bsh + (size_t)n * PB;
+    const cplx *A = chain_mat(c, sel, 0);
+    cplx *Q = c.U + (size_t)n * n * out_pt;
+    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+        int i = (int)(e % n), j = (int)(e / n);
+        Q[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+    const int np = (n + PB - 1) / PB;
+    for (int pi = np - 1; pi >= 0; pi--) {
+        const int k0 = pi * PB, m = n - k0, b = min(PB, m);
+        for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
+            int r = e % m, p = e / m;
+            P[p * n + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * n];
+        }
+        for (int e = threadIdx.x; e < PB * PB; e += blockDim.x) T[e] = c.tblk[(size_t)pi * PB * PB + e];
+        __syncthreads();
+        cplx *Ct = Q + (size_t)k0 + (size_t)k0 * n;
+        wg_vhc(c.wk, P, n, m, b, Ct, n, m);
+        __syncthreads();
+        wg_tmul(c.wk, m, T, b, false);
+        __syncthreads();
+        wg_rank16(Ct, n, m, m, P, n, 0, b, true, c.wk, PB);
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// LU with partial pivoting, blocked (right-looking); log det as s_inv_logdet_lu_z
+// ------------------------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(BLK_NT)
+k_lu_blk(const DChain *chains)
+{
+    extern __shared__ cplx bsh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    cplx *P = bsh;                       // panel, ld n
+    __shared__ double rv[16];
+    __shared__ int ri[16];
+    __shared__ int s_info;
+    __shared__ int s_pv[PB];
+    cplx *A = c.W3;
+    if (threadIdx.x == 0) s_info = 0;
+    for (int k0 = 0; k0 < n; k0 += PB) {
+        const int m = n - k0, b = min(PB, m);
+        for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
+            int r = e % m, p = e / m;
+            P[p * n + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * n];
+        }
+        __syncthreads();
+        for (int j = 0; j < b; j++) {
+            double bv = -1.0;
+            int bi = j;
+            for (int r = j + threadIdx.x; r < m; r += blockDim.x) {
+                cplx a = P[j * n + r];
+                double v = fabs(a.x) + fabs(a.y);
+                if (v > bv) { bv = v; bi = r; }
+            }
+            int p;
+            blk_argmax(bv, bi, rv, ri, &p);
+            if (threadIdx.x == 0) { s_pv[j] = p; c.piv[k0 + j] = k0 + p; }
+            if (p != j)
+                for (int q = threadIdx.x; q < b; q += blockDim.x) {
+                    cplx t = P[q * n + p];
+                    P[q * n + p] = P[q * n + j];
+                    P[q * n + j] = t;
+                }
+            __syncthreads();
+            cplx piv = P[j * n + j];
+            bool zero = (piv.x == 0.0 && piv.y == 0.0);
+            if (zero && threadIdx.x == 0 && s_info == 0) s_info = k0 + j + 1;
+            if (!zero) {
+                cplx pinv = cinv(piv);
+                for (int r = j + 1 + threadIdx.x; r < m; r += blockDim.x) P[j * n + r] = cmul(P[j * n + r], pinv);
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < (m - j - 1) * (b - j - 1); e += blockDim.x) {
+                int r = j + 1 + e % (m - j - 1), q = j + 1 + e / (m - j - 1);
+                P[q * n + r] = csub(P[q * n + r], cmul(P[j * n + r], P[q * n + j]));
+            }
+            __syncthreads();
+        }
+        for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
+            int r = e % m, p = e / m;
+            A[(size_t)(k0 + r) + (size_t)(k0 + p) * n] = P[p * n + r];
+        }
+        // the panel's interchanges on the columns left and right of it, in order
+        for (int q = threadIdx.x; q < n; q += blockDim.x) {
+            if (q >= k0 && q < k0 + b) continue;
+            cplx *col = A + (size_t)q * n;
+            for (int j = 0; j < b; j++) {
+                int p = s_pv[j];
+                if (p != j) {
+                    cplx t = col[k0 + p];
+                    col[k0 + p] = col[k0 + j];
+                    col[k0 + j] = t;
+                }
+            }
+        }
+        __syncthreads();
+        const int nc = n - k0 - b;
+        if (nc > 0) {
+            // U12 = L11^-1 A12 (unit lower 16 x 16 from the panel), one thread per column
+            for (int q = threadIdx.x; q < nc; q += blockDim.x) {
+                cplx *col = A + (size_t)k0 + (size_t)(k0 + b + q) * n;
+                cplx x[PB];
+#pragma unroll
+                for (int i = 0; i < PB; i++) x[i] = (i < b) ? col[i] : make_double2(0.0, 0.0);
+#pragma unroll
+                for (int i = 0; i < PB; i++)
+#pragma unroll
+                    for (int ii = 0; ii < PB; ii++)
+                        if (ii < i && i < b) x[i] = csub(x[i], cmul(P[ii * n + i], x[ii]));
+#pragma unroll
+                for (int i = 0; i < PB; i++)
+                    if (i < b) col[i] = x[i];
+            }
+            __syncthreads();
+            // A22 -= L21 U12
+            cplx *C = A + (size_t)(k0 + b) + (size_t)(k0 + b) * n;
+            wg_rank16(C, n, m - b, nc, P, n, b, b, false, A + (size_t)k0 + (size_t)(k0 + b) * n, n);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cplx ld = make_double2(0.0, 0.0);
+        for (int i = 0; i < n; i++) {
+            cplx d = A[(size_t)i + (size_t)i * n];
+            if (c.piv[i] != i) d = make_double2(-d.x, -d.y);
+            ld = cadd(ld, clog_(d));
+        }
+        double im = ld.y;
+        double k2 = round(im / 2.0 / M_PI);
+        ld.y = im - 2.0 * M_PI * k2;
+        c.sc[SC_ZLD] = ld.x;
+        c.sc[SC_ZLD + 1] = ld.y;
+        c.sc[SC_INFO] = (double)s_info;
+    }
+}
+
+// (P L U) Y = X in place of X = matrix xsel (N right-hand sides): row interchanges, then
+// block forward and back substitution with rank-16 updates on MFMA
+extern "C" __global__ void __launch_bounds__(BLK_NT)
+k_solve_blk(const DChain *chains, int xsel)
+{
+    extern __shared__ cplx bsh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    cplx *P = bsh;                            // 16 columns of L or U, ld n
+    int *perm = (int *)(bsh + (size_t)n * PB);
+    const cplx *A = c.W3;
+    cplx *X = chain_mat(c, xsel, 0);
+    cplx *Y = c.W2;
+    // final row order of the interchanges
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < n; i++) perm[i] = i;
+        for (int k = 0; k < n; k++) {
+            int p = c.piv[k];
+            int t = perm[p]; perm[p] = perm[k]; perm[k] = t;
+        }
+    }
+    __syncthreads();
+    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+        int i = (int)(e % n), j = (int)(e / n);
+        Y[e] = X[(size_t)perm[i] + (size_t)j * n];
+    }
+    __syncthreads();
+    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) X[e] = Y[e];
+    __syncthreads();
+    // forward: unit lower L
+    for (int k0 = 0; k0 < n; k0 += PB) {
+        const int b = min(PB, n - k0), mb = n - k0 - b;
+        for (int e = threadIdx.x; e < (n - k0) * b; e += blockDim.x) {
+            int r = e % (n - k0), p = e / (n - k0);
+            P[p * n + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * n];
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < n; q += blockDim.x) {
+            cplx *col = X + (size_t)k0 + (size_t)q * n;
+            cplx x[PB];
+#pragma unroll
+            for (int i = 0; i < PB; i++) x[i] = (i < b) ? col[i] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int i = 0; i < PB; i++)
+#pragma unroll
+                for (int ii = 0; ii < PB; ii++)
+                    if (ii < i && i < b) x[i] = csub(x[i], cmul(P[ii * n + i], x[ii]));
+#pragma unroll
+            for (int i = 0; i < PB; i++)
+                if (i < b) col[i] = x[i];
+        }
+        __syncthreads();
+        if (mb > 0) wg_rank16(X + (size_t)(k0 + b), n, mb, n, P, n, b, b, false, X + (size_t)k0, n);
+        __syncthreads();
+    }
+    // backward: upper U
+    const int np = (n + PB - 1) / PB;
+    for (int pi = np - 1; pi >= 0; pi--) {
+        const int k0 = pi * PB, b = min(PB, n - k0);
+        // rows 0 .. k0+b of the 16 columns
+        for (int e = threadIdx.x; e < (k0 + b) * b; e += blockDim.x) {
+            int r = e % (k0 + b), p = e / (k0 + b);
+            P[p * n + r] = A[(size_t)r + (size_t)(k0 + p) * n];
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < n; q += blockDim.x) {
+            cplx *col = X + (size_t)k0 + (size_t)q * n;
+            cplx x[PB];
+#pragma unroll
+            for (int i = 0; i < PB; i++) x[i] = (i < b) ? col[i] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int i = PB - 1; i >= 0; i--) {
+                if (i < b) {
+#pragma unroll
+                    for (int ii = 0; ii < PB; ii++)
+                        if (ii > i && ii < b) x[i] = csub(x[i], cmul(P[ii * n + k0 + i], x[ii]));
+                    x[i] = cdiv(x[i], P[i * n + k0 + i]);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < PB; i++)
+                if (i < b) col[i] = x[i];
+        }
+        __syncthreads();
+        if (k0 > 0) wg_rank16(X, n, k0, n, P, n, 0, b, false, X + (size_t)k0, n);
+        __syncthreads();
+    }
+}

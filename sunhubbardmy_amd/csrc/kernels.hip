@@ -1376,3 +1376,109 @@ k_save_point(const DChain *chains, int pt)
     if (e < (size_t)n) c.Ds[e] = c.D[(size_t)n * pt + e];
     if (e == 0) { c.sc[SC_LDQ_SAVE] = c.ldq[pt].x; c.sc[SC_LDQ_SAVE + 1] = c.ldq[pt].y; }
 }
+
+// ------------------------------------------------------------------------------------------
+// Delay update with the delayed vectors held in registers (src/dqmc_config_u/dqmc_update_delay.f90:39-128).
+// 2R threads, R = rows rounded up to a wave: thread r < R keeps avec(r, 0:K) and thread R + r
+// keeps bvec(r, 0:K) in registers.  An accepted site broadcasts row s of both through LDS,
+// every thread extends its own row, and avec/bvec reach memory only for the MFMA flush.
+// ------------------------------------------------------------------------------------------
+template <int K>
+__global__ void __launch_bounds__(K >= 16 ? 512 : 1024)
+k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
+{
+    extern __shared__ cplx sh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n, kb = c.nublock;
+    const int R = blockDim.x >> 1;
+    const bool isA = threadIdx.x < R;
+    const int r = isA ? threadIdx.x : threadIdx.x - R;
+    cplx *diagg = sh;            // n
+    cplx *anew = diagg + n;      // n
+    cplx *bnew = anew + n;       // n
+    cplx *rowa = bnew + n;       // K
+    cplx *rowb = rowa + K;       // K
+    cplx *G = c.G;
+    int32_t *cf = c.conf + (size_t)(nt - 1) * n;
+    const double *u = c.unif + (size_t)2 * n * slice_in_pass;
+    cplx phase = make_double2(c.sc[SC_PHASE], c.sc[SC_PHASE + 1]);
+    cplx reg[K];
+#pragma unroll
+    for (int m = 0; m < K; m++) reg[m] = make_double2(0.0, 0.0);
+    for (int q = threadIdx.x; q < n; q += blockDim.x) diagg[q] = G[(size_t)q + (size_t)q * n];
+    __syncthreads();
+    int ik = 0;
+    double accm = 0.0;
+    for (int s = 0; s < n; s++) {
+        const SiteDraw d = site_draw(c, cf[s], u[2 * s]);
+        const cplx dg = diagg[s];
+        const cplx ratio1 = cadd(cmul(make_double2(1.0 - dg.x, -dg.y), d.db), make_double2(1.0, 0.0));
+        cplx rt;
+        const bool acc = site_accept(c, ratio1, d, u[2 * s + 1], phase, &rt);
+        if (acc) {
+            accm += 1.0;
+            phase = phase_after(phase, rt);
+            const cplx sscl = cdiv(d.db, ratio1);
+            if (r == s) {
+#pragma unroll
+                for (int m = 0; m < K; m++)
+                    if (m < ik) (isA ? rowa : rowb)[m] = reg[m];
+            }
+            __syncthreads();
+            if (r < n) {
+                cplx v;
+                if (isA) {
+                    v = G[(size_t)r + (size_t)s * n];
+#pragma unroll
+                    for (int m = 0; m < K; m++)
+                        if (m < ik) v = cadd(v, cmul(rowb[m], reg[m]));
+                    v = cmul(v, sscl);
+                    anew[r] = v;
+                } else {
+                    v = G[(size_t)s + (size_t)r * n];
+#pragma unroll
+                    for (int m = 0; m < K; m++)
+                        if (m < ik) v = cadd(v, cmul(rowa[m], reg[m]));
+                    if (r == s) v.x -= 1.0;
+                    bnew[r] = v;
+                }
+#pragma unroll
+                for (int m = 0; m < K; m++)
+                    if (m == ik) reg[m] = v;
+            }
+            __syncthreads();
+            if (isA && r < n) diagg[r] = cadd(diagg[r], cmul(anew[r], bnew[r]));
+            if (threadIdx.x == 0) cf[s] = d.isp;
+            ik++;
+            __syncthreads();
+        }
+        if (ik == kb || s == n - 1) {
+            if (ik > 0) {
+                if (r < n) {
+                    cplx *dst = isA ? c.avec : c.bvec;
+#pragma unroll
+                    for (int m = 0; m < K; m++)
+                        if (m < ik) dst[(size_t)r + (size_t)m * n] = reg[m];
+                }
+                __syncthreads();
+                wg_rank_k(G, n, c.avec, c.bvec, ik, nullptr, nullptr);
+            }
+            __syncthreads();
+            ik = 0;
+            if (s < n - 1)
+                for (int q = threadIdx.x; q < n; q += blockDim.x) diagg[q] = G[(size_t)q + (size_t)q * n];
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        c.sc[SC_PHASE] = phase.x;
+        c.sc[SC_PHASE + 1] = phase.y;
+        c.sc[SC_ACC] += accm;
+        c.sc[SC_TRIED] += n;
+    }
+}
+
+template __global__ void k_update_delay_reg<4>(const DChain *, int, int);
+template __global__ void k_update_delay_reg<8>(const DChain *, int, int);
+template __global__ void k_update_delay_reg<16>(const DChain *, int, int);
+template __global__ void k_update_delay_reg<32>(const DChain *, int, int);

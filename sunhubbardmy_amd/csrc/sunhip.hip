@@ -2,6 +2,7 @@
 // The slice loop mirrors src/dqmc_ft_core/dqmc_ft_sweep_b0.f90 and dqmc_ft_sweep_0b.f90; every
 // launch advances all chains of the batch at once.
 #include "kernels.hip"
+#include "blocked.hip"
 #include "../../include/sunhip.h"
 
 #include <cmath>
@@ -44,6 +45,7 @@ struct sh_ctx {
     std::vector<void *> allocs;
     int32_t *d_seeds = nullptr;
     bool started = false;
+    bool legacy_update = false;   // test hook: the L2-resident delay update kernel
     std::string err;
     // timing
     bool timing = false;
@@ -197,6 +199,26 @@ static void right_pass(sh_ctx *ctx, int ksel, int prog, int vsel, int nt)
 
 // ---- stabilization -------------------------------------------------------------------------
 static size_t lds_qr(sh_ctx *ctx) { return (size_t)ctx->nmax * 32; }
+static bool blocked(sh_ctx *ctx) { return ctx->nmax <= BLK_NMAX; }
+static size_t lds_blk(sh_ctx *ctx) { return ((size_t)ctx->nmax * PB + 2 * PB * PB) * sizeof(cplx) + ctx->nmax * 4; }
+
+static void lu_factor(sh_ctx *ctx)
+{
+    Tic t(ctx, T_LU);
+    if (blocked(ctx))
+        hipLaunchKernelGGL(k_lu_blk, dim3(ctx->nch), dim3(BLK_NT), lds_blk(ctx), ctx->st, ctx->dc);
+    else
+        hipLaunchKernelGGL(k_lu, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * 2 * sizeof(cplx), ctx->st, ctx->dc);
+}
+
+static void lu_solve(sh_ctx *ctx, int xsel)
+{
+    Tic t(ctx, T_SOLVE);
+    if (blocked(ctx))
+        hipLaunchKernelGGL(k_solve_blk, dim3(ctx->nch), dim3(BLK_NT), lds_blk(ctx), ctx->st, ctx->dc, xsel);
+    else
+        hipLaunchKernelGGL(k_lu_solve, dim3(ctx->nch), dim3(512), 0, ctx->st, ctx->dc, xsel);
+}
 
 // QR of W1 (first point: column pivoting) or of the prepivoted W1 diag(D[dpt]) -> point `pt`
 static void udv_step(sh_ctx *ctx, bool first, int pt, int dpt)
@@ -218,12 +240,18 @@ static void udv_step(sh_ctx *ctx, bool first, int pt, int dpt)
     {
         Tic t(ctx, T_QR);
         hipLaunchKernelGGL(k_prepivot, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * 12, ctx->st, ctx->dc, dpt);
-        hipLaunchKernelGGL(k_qr, dim3(ctx->nch), dim3(1024), lds_qr(ctx), ctx->st, ctx->dc, M_W2, 0, pt);
+        if (blocked(ctx))
+            hipLaunchKernelGGL(k_qr_blk, dim3(ctx->nch), dim3(BLK_NT), lds_blk(ctx), ctx->st, ctx->dc, M_W2, pt);
+        else
+            hipLaunchKernelGGL(k_qr, dim3(ctx->nch), dim3(1024), lds_qr(ctx), ctx->st, ctx->dc, M_W2, 0, pt);
     }
     {
         Tic t(ctx, T_FORMQ);
-        hipLaunchKernelGGL(k_formq, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * sizeof(cplx), ctx->st, ctx->dc,
-                           M_W2, pt);
+        if (blocked(ctx))
+            hipLaunchKernelGGL(k_formq_blk, dim3(ctx->nch), dim3(BLK_NT), lds_blk(ctx), ctx->st, ctx->dc, M_W2, pt);
+        else
+            hipLaunchKernelGGL(k_formq, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * sizeof(cplx), ctx->st,
+                               ctx->dc, M_W2, pt);
     }
     {
         Tic t(ctx, T_OTHER);
@@ -274,18 +302,12 @@ static void green(sh_ctx *ctx, int rsaved, int rpt, int lsaved, int lpt)
         Tic t(ctx, T_OTHER);
         hipLaunchKernelGGL(k_green_m, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, 0, rsaved, rpt, lsaved, lpt);
     }
-    {
-        Tic t(ctx, T_LU);
-        hipLaunchKernelGGL(k_lu, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * 2 * sizeof(cplx), ctx->st, ctx->dc);
-    }
+    lu_factor(ctx);
     {
         Tic t(ctx, T_OTHER);
         hipLaunchKernelGGL(k_green_rhs, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, rsaved, rpt);
     }
-    {
-        Tic t(ctx, T_SOLVE);
-        hipLaunchKernelGGL(k_lu_solve, dim3(ctx->nch), dim3(512), 0, ctx->st, ctx->dc, (int)M_W4);
-    }
+    lu_solve(ctx, M_W4);
     {
         Tic t(ctx, T_OTHER);
         hipLaunchKernelGGL(k_green_ulscale, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, lsaved, lpt);
@@ -302,11 +324,11 @@ static void green(sh_ctx *ctx, int rsaved, int rpt, int lsaved, int lpt)
 // green_equaltimebb: G(beta,beta) from the right chain point nst
 static void green_bb(sh_ctx *ctx, int rpt)
 {
-    Tic t(ctx, T_OTHER);
     hipLaunchKernelGGL(k_green_m, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, 1, 0, rpt, 0, 0);
-    hipLaunchKernelGGL(k_lu, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * 2 * sizeof(cplx), ctx->st, ctx->dc);
+    lu_factor(ctx);
     hipLaunchKernelGGL(k_green_rhs, grid_tiles32(ctx), dim3(256), 0, ctx->st, ctx->dc, 0, rpt);
-    hipLaunchKernelGGL(k_lu_solve, dim3(ctx->nch), dim3(512), 0, ctx->st, ctx->dc, (int)M_W4);
+    lu_solve(ctx, M_W4);
+    Tic t(ctx, T_OTHER);
     copy_mat(ctx, M_GS, 0, M_W4, 0);
     hipLaunchKernelGGL(k_green_logw, dim3(ctx->nch), dim3(256), 0, ctx->st, ctx->dc, 1, 0, rpt, 0, 0, (int)SC_LOGW_TMP);
 }
@@ -322,10 +344,34 @@ static void set_identity(sh_ctx *ctx, int pt)
     hipLaunchKernelGGL(k_set_identity, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, pt);
 }
 
+// register-resident delay update: the template depth covering every chain's nublock, and the
+// thread count 2 * rows; 0 when the register budget does not allow it
+static int delay_reg_depth(sh_ctx *ctx, int *threads)
+{
+    int kmax = 0;
+    for (auto &c : ctx->hc) kmax = std::max(kmax, c.nublock);
+    int kt = kmax <= 4 ? 4 : kmax <= 8 ? 8 : kmax <= 16 ? 16 : kmax <= 32 ? 32 : 0;
+    int rows = (ctx->nmax + 63) / 64 * 64;
+    int nt = 2 * rows;
+    if (kt == 0 || nt > (kt >= 16 ? 512 : 1024)) return 0;
+    *threads = nt;
+    return kt;
+}
+
 static void update_slice(sh_ctx *ctx, int nt, int slice_in_pass)
 {
     Tic t(ctx, T_UPDATE);
-    if (ctx->update == SH_U_DELAY) {
+    int nthr = 0;
+    int kt = (ctx->update == SH_U_DELAY && !ctx->legacy_update) ? delay_reg_depth(ctx, &nthr) : 0;
+    if (kt) {
+        size_t lds = (size_t)(3 * ctx->nmax + 2 * kt) * sizeof(cplx);
+        switch (kt) {
+        case 4: hipLaunchKernelGGL(k_update_delay_reg<4>, dim3(ctx->nch), dim3(nthr), lds, ctx->st, ctx->dc, nt, slice_in_pass); break;
+        case 8: hipLaunchKernelGGL(k_update_delay_reg<8>, dim3(ctx->nch), dim3(nthr), lds, ctx->st, ctx->dc, nt, slice_in_pass); break;
+        case 16: hipLaunchKernelGGL(k_update_delay_reg<16>, dim3(ctx->nch), dim3(nthr), lds, ctx->st, ctx->dc, nt, slice_in_pass); break;
+        default: hipLaunchKernelGGL(k_update_delay_reg<32>, dim3(ctx->nch), dim3(nthr), lds, ctx->st, ctx->dc, nt, slice_in_pass); break;
+        }
+    } else if (ctx->update == SH_U_DELAY) {
         size_t lds = (size_t)(ctx->nmax + 2 * 64) * sizeof(cplx);
         hipLaunchKernelGGL(k_update_delay, dim3(ctx->nch), dim3(256), lds, ctx->st, ctx->dc, nt, slice_in_pass);
     } else {
@@ -396,9 +442,24 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
     ctx->iwrap_nt.assign(m[0].iwrap_nt, m[0].iwrap_nt + m[0].ltrot + 1);
     ctx->wrap_step.assign(m[0].wrap_step, m[0].wrap_step + 2 * m[0].nst);
     if (hipStreamCreate(&ctx->st) != hipSuccess) { delete ctx; return SH_E_HIP; }
-    // allow the large-LDS passes at big N
-    hipFuncSetAttribute((const void *)k_cb_rows, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    hipFuncSetAttribute((const void *)k_cb_cols, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    // kernels whose dynamic LDS grows with N get exactly what the largest chain needs
+    {
+        struct { const void *f; size_t bytes; } big[] = {
+            {(const void *)k_cb_rows, lds_rows(ctx)}, {(const void *)k_cb_cols, lds_cols(ctx)},
+            {(const void *)k_qr_blk, blocked(ctx) ? lds_blk(ctx) : 0},
+            {(const void *)k_formq_blk, blocked(ctx) ? lds_blk(ctx) : 0},
+            {(const void *)k_lu_blk, blocked(ctx) ? lds_blk(ctx) : 0},
+            {(const void *)k_solve_blk, blocked(ctx) ? lds_blk(ctx) : 0}};
+        for (auto &b : big) {
+            if (b.bytes <= 64 * 1024) continue;
+            if (hipFuncSetAttribute(b.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                int rc = fail(ctx, SH_E_HIP, "dynamic LDS request above the device limit");
+                sh_destroy(ctx);
+                return rc;
+            }
+        }
+    }
     std::vector<int32_t> seeds(nchains);
     ctx->hc.resize(nchains);
     for (int ci = 0; ci < nchains; ci++) {
@@ -419,6 +480,8 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
         int kb = std::max(q.nublock, 1);
         c.avec = dalloc<cplx>(ctx, (size_t)n * kb); c.bvec = dalloc<cplx>(ctx, (size_t)n * kb);
         c.tau = dalloc<cplx>(ctx, n); c.jpvt = dalloc<int32_t>(ctx, n); c.piv = dalloc<int32_t>(ctx, n);
+        c.wk = dalloc<cplx>(ctx, (size_t)PB * n);
+        c.tblk = dalloc<cplx>(ctx, (size_t)(n / PB + 1) * PB * PB);
         c.rwork = dalloc<double>(ctx, 4 * (size_t)n);
         c.conf = dalloc<int32_t>(ctx, (size_t)n * q.ltrot);
         c.unif = dalloc<double>(ctx, (size_t)2 * n * q.ltrot);
@@ -430,7 +493,7 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
         c.stag = dupload<double>(ctx, q.stag, n);
         c.sc = dalloc<double>(ctx, SC_SIZE);
         if (!c.G || !c.Gs || !c.W1 || !c.W2 || !c.W3 || !c.W4 || !c.Us || !c.Vs || !c.U || !c.V || !c.D || !c.Ds ||
-            !c.ldq || !c.avec || !c.bvec || !c.tau || !c.jpvt || !c.piv || !c.rwork || !c.conf || !c.unif ||
+            !c.ldq || !c.avec || !c.bvec || !c.tau || !c.jpvt || !c.piv || !c.rwork || !c.conf || !c.unif || !c.wk || !c.tblk ||
             !c.rng || !c.bonds || !c.kt || !c.ktm1 || !c.stag || !c.sc) {
             int rc = fail(ctx, SH_E_HIP, "device allocation failed");
             sh_destroy(ctx);
@@ -868,8 +931,8 @@ extern "C" int sh_dbg_op(sh_ctx *ctx, int op, const int *a)
         udv_step(ctx, false, a[0], a[1]);
         break;
     case 5:
-        hipLaunchKernelGGL(k_lu, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * 2 * sizeof(cplx), ctx->st, ctx->dc);
-        hipLaunchKernelGGL(k_lu_solve, dim3(ctx->nch), dim3(512), 0, ctx->st, ctx->dc, (int)M_W4);
+        lu_factor(ctx);
+        lu_solve(ctx, M_W4);
         break;
     case 6:
         if (!okslice(a[3]) || !okslice(a[4]) || (a[5] != 1 && a[5] != -1)) return SH_E_ARG;
@@ -897,6 +960,7 @@ extern "C" int sh_dbg_op(sh_ctx *ctx, int op, const int *a)
         break;
     case 12: measure(ctx); break;
     case 13: wrap_replace(ctx, a[0]); break;
+    case 14: ctx->legacy_update = a[0] != 0; break;
     default: return SH_E_ARG;
     }
     int rc = check_launch(ctx);

@@ -39,6 +39,8 @@ struct DChain {
     cplx *ldq;            // nst+1
     cplx *avec, *bvec;    // n*nublock
     cplx *tau;            // n
+    cplx *wk;             // 16 x n work block of the blocked factorizations
+    cplx *tblk;           // (n/16 + 1) triangular 16 x 16 factors of the block reflectors
     int32_t *jpvt, *piv;  // n
     double *rwork;        // 4n
     int32_t *conf;        // ltrot*n

@@ -1,0 +1,112 @@
+! sunhip_bridge.f90 -- ISO_C_BINDING interface to libsunhip.so for the reference's Fortran
+! control layer.  A maintainer adds this module next to src/dqmc_ft_core.f90 and calls the
+! sh_* entry points where src/dqmc_ft_ctrl.f90 calls allocate_core, dqmc_ft_sweep_start_0b,
+! dqmc_ft_sweep_b0/0b and push_stage/pop_stage.  Nonzero return codes map onto the
+! reference's `stop` (dqmc_ft_sweep_b0.f90:188-194, dqmc_ft_sweep_start_0b.f90:44-47).
+module sunhip_bridge
+  use, intrinsic :: iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: SH_K_CHECKERBOARD = 0, SH_K_DENSE = 1
+  integer(c_int), parameter :: SH_U_DELAY = 0, SH_U_SUBMATRIX = 1
+
+  ! mirrors struct sh_model of include/sunhip.h
+  type, bind(C) :: sh_model
+     integer(c_int) :: ndim, lq, ltrot, nwrap, nst
+     type(c_ptr) :: iwrap_nt, wrap_step
+     integer(c_int) :: lwrapT, lwrapu, kmode, nn_nf, nn_lf
+     type(c_ptr) :: bonds, urt, urtm1
+     integer(c_int) :: lcomp, nflr
+     type(c_ptr) :: bmat, bmat_inv, phase_arg, delta_bmat, phase_ratio, gaml
+     integer(c_int) :: phase_from_conf
+     type(c_ptr) :: stag
+     integer(c_int) :: update, nublock
+     integer(c_int32_t) :: seed
+  end type sh_model
+
+  interface
+     integer(c_int) function sh_create(ctx, nchains, models, device) bind(C, name="sh_create")
+       import :: c_ptr, c_int, sh_model
+       type(c_ptr), intent(out) :: ctx
+       integer(c_int), value :: nchains, device
+       type(sh_model), intent(in) :: models(*)
+     end function sh_create
+     integer(c_int) function sh_destroy(ctx) bind(C, name="sh_destroy")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+     end function sh_destroy
+     integer(c_int) function sh_init_conf(ctx) bind(C, name="sh_init_conf")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+     end function sh_init_conf
+     integer(c_int) function sh_set_conf(ctx, chain, conf) bind(C, name="sh_set_conf")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(in) :: conf(*)
+     end function sh_set_conf
+     integer(c_int) function sh_get_conf(ctx, chain, conf) bind(C, name="sh_get_conf")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(out) :: conf(*)
+     end function sh_get_conf
+     integer(c_int) function sh_sweep_start_0b(ctx) bind(C, name="sh_sweep_start_0b")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+     end function sh_sweep_start_0b
+     integer(c_int) function sh_sweep_b0(ctx, lupdate, lmeasure) bind(C, name="sh_sweep_b0")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: lupdate, lmeasure
+     end function sh_sweep_b0
+     integer(c_int) function sh_sweep_0b(ctx, lupdate, lmeasure, ldyn) bind(C, name="sh_sweep_0b")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: lupdate, lmeasure, ldyn
+     end function sh_sweep_0b
+     integer(c_int) function sh_push_stage(ctx) bind(C, name="sh_push_stage")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+     end function sh_push_stage
+     integer(c_int) function sh_pop_stage(ctx) bind(C, name="sh_pop_stage")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+     end function sh_pop_stage
+     integer(c_int) function sh_get_g(ctx, chain, g) bind(C, name="sh_get_g")
+       import :: c_ptr, c_int, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       complex(c_double_complex), intent(out) :: g(*)
+     end function sh_get_g
+     integer(c_int) function sh_get_diag(ctx, chain, d) bind(C, name="sh_get_diag")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       real(c_double), intent(out) :: d(10)
+     end function sh_get_diag
+     integer(c_int) function sh_get_obs(ctx, chain, o) bind(C, name="sh_get_obs")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       real(c_double), intent(out) :: o(11)
+     end function sh_get_obs
+     integer(c_int) function sh_reset_obs(ctx) bind(C, name="sh_reset_obs")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+     end function sh_reset_obs
+  end interface
+
+contains
+
+  ! the reference's failure convention: print and stop on a nonzero code
+  subroutine sh_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    if (rc /= 0) then
+       write(*, '(a,a,i6)') ' Error in ', where, rc
+       stop
+    end if
+  end subroutine sh_check
+
+end module sunhip_bridge

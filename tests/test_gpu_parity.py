@@ -11,6 +11,7 @@ import pytest
 
 import oracle
 from sunhubbardmy_amd import FTWalkers, Model, chain_seed
+from sunhubbardmy_amd import _lib
 
 pytestmark = pytest.mark.gpu
 
@@ -39,11 +40,15 @@ def say(*a):
     sys.stdout.flush()
 
 
-def run_parity(kw, nsweeps=2, chain=0):
+def run_parity(kw, nsweeps=2, chain=0, legacy_update=False):
     seed = chain_seed(chain)
     m = Model(seed=seed, **kw)
-    say("case", kw, "seed", seed)
+    say("case", kw, "seed", seed, "legacy_update", legacy_update)
     w = FTWalkers([m])
+    if legacy_update:
+        a = np.zeros(8, np.int32)
+        a[0] = 1
+        assert _lib.testing_lib().sh_dbg_op(w._ctx, 14, a.ctypes.data_as(_lib.ip)) == 0
     o = oracle.OracleChain(seed=seed, **kw)
     w.init_conf()
     o.init_conf()
@@ -82,3 +87,18 @@ def test_sweep_parity(name):
     kw = dict(BASE)
     kw.update(CASES[name])
     run_parity(kw)
+
+
+@pytest.mark.parametrize("name", ["square4_delay_checkerboard", "honeycomb3_delay_dense"])
+def test_sweep_parity_l2_delay_kernel(name):
+    kw = dict(BASE)
+    kw.update(CASES[name])
+    run_parity(kw, legacy_update=True)
+
+
+def test_sweep_parity_metric_shape_short():
+    # the bench shape (16x16, k = 32) at a short beta: register-resident delay kernel at K = 32,
+    # blocked stabilization at N = 256
+    kw = dict(BASE)
+    kw.update(la=16, lb=16, beta=1.0, rhub=6.0, nublock=32, nwrap=5)
+    run_parity(kw, nsweeps=1)
