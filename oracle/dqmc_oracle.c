@@ -1416,3 +1416,687 @@ int orc_get_table(const orc_ctx *c, int kind, double *out)
     default: return -1;
     }
 }
+
+/* ==================================================================================== */
+/* Projective DQMC (t-V model, bond HS field, submatrix-LR update)                       */
+/* src/dqmc_proj_core.f90 and the files under src/dqmc_proj_core/ and src/dqmc_config_v/          */
+/* ==================================================================================== */
+struct orc_pctx {
+    orc_pparams p;
+    orc_ctx *b;            /* lattice, schedule, kinetic tables, measurement sums, RNG */
+    int N, ne, ltrot, nst, nn_nf, nn_lf, nustock;
+    double alpha;
+    zc bp[2], bm[2], bpi[2], bmi[2], dp[2], dm[2];
+    int32_t *conf;         /* [ltrot][nn_nf][N]; entries i < nn_lf are the bonds' fields */
+    double *wtrial;        /* trial Hamiltonian eigenvalues, ascending */
+    zc *projR;             /* N x ne (projL = projR) */
+    zc *UR, *UL, *ULR, *ULRINV, *G, *tmp;
+    zc **Ust;              /* (nst+1) x (N x ne) */
+    double *logwDV;
+    zc phase, logweightf;
+    double max_wrap_error, max_phasediff, nchecks, sumlog10err, accepted, tried;
+};
+
+/* vconf%set_conf, lprojv = .false. (src/dqmc_config_v/dqmc_set_conf.f90:14-77):
+ * alpha = acosh(exp(dtau V / 2)), eta = (-1, +1) */
+static void pq_setup_v(orc_pctx *c)
+{
+    double al = acosh(exp(c->p.dtau * c->p.rv * 0.5));
+    const double eta[2] = {-1.0, 1.0};
+    c->alpha = al;
+    for (int is = 0; is < 2; is++) {
+        c->bp[is] = exp(al * eta[is]);
+        c->bpi[is] = exp(-al * eta[is]);
+        c->bm[is] = exp(-al * eta[is]);
+        c->bmi[is] = exp(al * eta[is]);
+        int isp = (is + 1) % 2;              /* iflip = 1 */
+        c->dp[is] = exp(al * (eta[isp] - eta[is])) - 1.0;
+        c->dm[is] = exp(-al * (eta[isp] - eta[is])) - 1.0;
+    }
+}
+
+orc_pctx *orc_pcreate(const orc_pparams *p)
+{
+    orc_params q;
+    memset(&q, 0, sizeof(q));
+    q.lattice = p->lattice; q.la = p->la; q.lb = p->lb; q.beta = p->beta; q.dtau = p->dtau;
+    q.rt = p->rt; q.mu = p->mu; q.rhub = 0.0; q.nu = p->nu; q.nflr = p->nflr; q.exact_hs = 1;
+    q.kmode = p->kmode; q.update = 2; q.nwrap = p->nwrap; q.nublock = 1; q.seed = p->seed;
+    orc_pctx *c = (orc_pctx *)calloc(1, sizeof(orc_pctx));
+    c->p = *p;
+    orc_ctx *b = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+    b->p = q;
+    setup_lattice(b);
+    b->ltrot = (int)lround(p->beta / p->dtau);
+    setup_schedule(b);
+    b->lwrapu = 0;
+    setup_kinetic(b);
+    sfmt_init(&b->rng, p->seed);
+    c->b = b;
+    c->N = b->ndim;
+    c->ne = c->N / 2 + (int)(p->nu * (double)c->N / 8.0);   /* src/model_para.f90:376 */
+    c->ltrot = b->ltrot; c->nst = b->nst;
+    c->nn_nf = b->nn_nf; c->nn_lf = b->nn_lf;
+    c->nustock = p->nustock;
+    pq_setup_v(c);
+    int N = c->N, ne = c->ne;
+    c->conf = (int32_t *)calloc((size_t)N * c->nn_nf * c->ltrot, sizeof(int32_t));
+    for (size_t k = 0; k < (size_t)N * c->nn_nf * c->ltrot; k++) c->conf[k] = 1;
+    c->wtrial = (double *)calloc(N, sizeof(double));
+    c->projR = zalloc((size_t)N * ne);
+    c->UR = zalloc((size_t)N * ne);
+    c->UL = zalloc((size_t)N * ne);
+    c->ULR = zalloc((size_t)ne * ne);
+    c->ULRINV = zalloc((size_t)ne * ne);
+    c->G = zalloc((size_t)N * N);
+    c->tmp = zalloc((size_t)N * (N > ne ? N : ne));
+    c->Ust = (zc **)calloc(c->nst + 1, sizeof(zc *));
+    for (int n = 0; n <= c->nst; n++) c->Ust[n] = zalloc((size_t)N * ne);
+    c->logwDV = (double *)calloc(c->nst + 2, sizeof(double));
+    c->phase = 1.0;
+    b->G = c->G;
+    return c;
+}
+
+void orc_pdestroy(orc_pctx *c)
+{
+    if (!c) return;
+    orc_ctx *b = c->b;
+    free(b->urt); free(b->urtm1); free(b->nnlf); free(b->nnlist); free(b->cx); free(b->cy); free(b->sub);
+    free(b->iwrap_nt); free(b->wrap_step); free(b);
+    for (int n = 0; n <= c->nst; n++) free(c->Ust[n]);
+    free(c->Ust); free(c->logwDV); free(c->conf); free(c->wtrial); free(c->projR);
+    free(c->UR); free(c->UL); free(c->ULR); free(c->ULRINV); free(c->G); free(c->tmp);
+    free(c);
+}
+
+void orc_psizes(const orc_pctx *c, int *out)
+{
+    out[0] = c->N; out[1] = c->ne; out[2] = c->ltrot; out[3] = c->nst;
+    out[4] = c->nn_nf; out[5] = c->nn_lf; out[6] = c->nustock; out[7] = c->b->lq;
+}
+
+/* vconf%input_conf from scratch (src/dqmc_config_v/dqmc_input_conf.f90:55-66, one rank):
+ * nt outer, site i, family nf innermost; conf(i,nf,nt) = ceiling(u * 2) */
+void orc_pinit_conf(orc_pctx *c)
+{
+    int N = c->N, nf = c->nn_nf;
+    for (int nt = 0; nt < c->ltrot; nt++)
+        for (int i = 0; i < N; i++)
+            for (int f = 0; f < nf; f++)
+                c->conf[((size_t)nt * nf + f) * N + i] = (int32_t)ceil(sfmt_stream(&c->b->rng) * 2.0);
+}
+void orc_pget_conf(const orc_pctx *c, int32_t *conf) { memcpy(conf, c->conf, sizeof(int32_t) * (size_t)c->N * c->nn_nf * c->ltrot); }
+void orc_pset_conf(orc_pctx *c, const int32_t *conf) { memcpy(c->conf, conf, sizeof(int32_t) * (size_t)c->N * c->nn_nf * c->ltrot); }
+
+/* cyclic Jacobi diagonalisation of a real symmetric matrix (a destroyed, v = eigenvectors,
+ * w ascending); the trial Hamiltonians of this path are real, so zheev's eigenvalues and an
+ * orthonormal basis of each eigenspace are what it returns up to rounding and phases */
+static void jacobi_sym(int n, double *a, double *v, double *w)
+{
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) v[IDX(i, j, n)] = (i == j) ? 1.0 : 0.0;
+    double scale = 0;
+    for (size_t k = 0; k < (size_t)n * n; k++) scale += a[k] * a[k];
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0;
+        for (int q = 0; q < n; q++)
+            for (int p = 0; p < q; p++) off += a[IDX(p, q, n)] * a[IDX(p, q, n)];
+        if (off <= 1e-34 * scale || off == 0) break;
+        for (int q = 1; q < n; q++)
+            for (int p = 0; p < q; p++) {
+                double apq = a[IDX(p, q, n)];
+                if (apq == 0) continue;
+                double th = (a[IDX(q, q, n)] - a[IDX(p, p, n)]) / (2.0 * apq);
+                double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+                for (int k = 0; k < n; k++) {   /* columns p, q */
+                    double akp = a[IDX(k, p, n)], akq = a[IDX(k, q, n)];
+                    a[IDX(k, p, n)] = cs * akp - sn * akq;
+                    a[IDX(k, q, n)] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < n; k++) {   /* rows p, q */
+                    double apk = a[IDX(p, k, n)], aqk = a[IDX(q, k, n)];
+                    a[IDX(p, k, n)] = cs * apk - sn * aqk;
+                    a[IDX(q, k, n)] = sn * apk + cs * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    double vkp = v[IDX(k, p, n)], vkq = v[IDX(k, q, n)];
+                    v[IDX(k, p, n)] = cs * vkp - sn * vkq;
+                    v[IDX(k, q, n)] = sn * vkp + cs * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < n; i++) w[i] = a[IDX(i, i, n)];
+    /* ascending order (selection sort, swapping eigenvector columns) */
+    for (int i = 0; i < n; i++) {
+        int m = i;
+        for (int j = i + 1; j < n; j++) if (w[j] < w[m]) m = j;
+        if (m != i) {
+            double t = w[i]; w[i] = w[m]; w[m] = t;
+            for (int k = 0; k < n; k++) { double s = v[IDX(k, i, n)]; v[IDX(k, i, n)] = v[IDX(k, m, n)]; v[IDX(k, m, n)] = s; }
+        }
+    }
+}
+
+/* set_trial_h0 (src/dqmc_config_h0/set_trial_h0.f90:1-29) with expar = 1, then dqmc_set_proj
+ * (src/dqmc_proj_core/dqmc_set_proj.f90:1-78): the lowest ne eigenvectors.  Returns the gap
+ * w(ne+1) - w(ne) ("degen"). */
+double orc_pset_trial(orc_pctx *c)
+{
+    int N = c->N, ne = c->ne;
+    double *H = (double *)calloc((size_t)N * N, sizeof(double));
+    double *V = (double *)calloc((size_t)N * N, sizeof(double));
+    orc_ctx *b = c->b;
+    for (int i = 0; i < c->nn_lf; i++) {
+        int i0 = b->nnlf[i];
+        for (int nf = 0; nf < c->nn_nf; nf++) {
+            int in = b->nnlist[i0 * c->nn_nf + nf];
+            double z1 = -c->p.rt + c->p.rndness * (sfmt_stream(&b->rng) - 1.0);
+            H[IDX(i0, in, N)] += z1;
+            H[IDX(in, i0, N)] += z1;
+        }
+    }
+    for (int i = 0; i < N; i++) H[IDX(i, i, N)] -= c->p.mu;
+    jacobi_sym(N, H, V, c->wtrial);
+    for (int j = 0; j < ne; j++)
+        for (int i = 0; i < N; i++) c->projR[IDX(i, j, N)] = V[IDX(i, j, N)];
+    free(H); free(V);
+    return (ne < N) ? c->wtrial[ne] - c->wtrial[ne - 1] : 0.0;
+}
+void orc_pset_proj(orc_pctx *c, const double *projR) { memcpy(c->projR, projR, sizeof(zc) * (size_t)c->N * c->ne); }
+void orc_pget_proj(const orc_pctx *c, double *projR, double *w)
+{
+    memcpy(projR, c->projR, sizeof(zc) * (size_t)c->N * c->ne);
+    if (w) memcpy(w, c->wtrial, sizeof(double) * c->N);
+}
+
+/* ---- operators on rectangular matrices ------------------------------------------------ */
+/* kinetic family nf from the left on an N x m matrix (rows mixed) */
+static void pk_left(orc_pctx *c, int nf, const zc *T, zc *M, int m)
+{
+    orc_ctx *b = c->b;
+    int N = c->N;
+    if (!b->lwrapT || !(c->p.rt > 0)) return;
+    if (c->p.kmode == 1) {
+        zgemm('N', 'N', N, m, N, T, N, M, N, 0, c->tmp, N);
+        memcpy(M, c->tmp, sizeof(zc) * (size_t)N * m);
+        return;
+    }
+    for (int q = 0; q < c->nn_lf; q++) {
+        int ist = q + nf * c->nn_lf;
+        int i1 = b->nnlf[q], i2 = b->nnlist[i1 * c->nn_nf + nf];
+        const zc *u = T + 4 * ist;
+        for (int j = 0; j < m; j++) {
+            zc x = M[IDX(i1, j, N)], y = M[IDX(i2, j, N)];
+            M[IDX(i1, j, N)] = u[0] * x + u[2] * y;
+            M[IDX(i2, j, N)] = u[1] * x + u[3] * y;
+        }
+    }
+}
+/* kinetic family nf from the right on an m x N matrix (columns mixed) */
+static void pk_right(orc_pctx *c, int nf, const zc *T, zc *M, int m)
+{
+    orc_ctx *b = c->b;
+    int N = c->N;
+    if (!b->lwrapT || !(c->p.rt > 0)) return;
+    if (c->p.kmode == 1) {
+        zgemm('N', 'N', m, N, N, M, m, T, N, 0, c->tmp, m);
+        memcpy(M, c->tmp, sizeof(zc) * (size_t)m * N);
+        return;
+    }
+    for (int q = 0; q < c->nn_lf; q++) {
+        int ist = q + nf * c->nn_lf;
+        int i1 = b->nnlf[q], i2 = b->nnlist[i1 * c->nn_nf + nf];
+        const zc *u = T + 4 * ist;
+        zc *ca = M + (size_t)i1 * m, *cb = M + (size_t)i2 * m;
+        for (int j = 0; j < m; j++) {
+            zc x = ca[j], y = cb[j];
+            ca[j] = x * u[0] + y * u[1];
+            cb[j] = x * u[2] + y * u[3];
+        }
+    }
+}
+static int pq_nfam(const orc_pctx *c) { return (c->p.kmode == 1) ? 1 : c->nn_nf; }
+
+#define HAD 0.70710678118654752440   /* entries of u = ut = [[1,1],[1,-1]]/sqrt(2) (dqmc_config_v.f90:19-20) */
+static const double pq_u[4] = {HAD, HAD, HAD, -HAD};   /* column-major u(1,1),u(2,1),u(1,2),u(2,2) */
+
+static int pq_is(const orc_pctx *c, int nt, int nf, int q) { return c->conf[((size_t)(nt - 1) * c->nn_nf + nf) * c->N + q]; }
+
+/* vconf left props on an N x m matrix (rows i1, i2 of each bond of family nf):
+ *   mode 0: left_forward nflag 2 (dqmc_config_v/dqmc_left_forward_prop.f90:28-45): rows <- diag(bp,bm) ut rows
+ *   mode 1: left_forward nflag 1 (:50-69) and left_backward nflag 2 (dqmc_left_backward_prop.f90): rows <- u rows
+ *   mode 2: left_backward nflag 1 (dqmc_left_backward_prop.f90): rows <- u diag(bpi,bmi) rows */
+static void pv_left(orc_pctx *c, int nt, int nf, int mode, zc *M, int m)
+{
+    int N = c->N;
+    for (int q = 0; q < c->nn_lf; q++) {
+        int i1 = c->b->nnlf[q], i2 = c->b->nnlist[i1 * c->nn_nf + nf];
+        int is = pq_is(c, nt, nf, q) - 1;
+        for (int j = 0; j < m; j++) {
+            zc x = M[IDX(i1, j, N)], y = M[IDX(i2, j, N)];
+            if (mode == 2) { x = c->bpi[is] * x; y = c->bmi[is] * y; }
+            zc v1 = pq_u[0] * x + pq_u[2] * y;
+            zc v2 = pq_u[1] * x + pq_u[3] * y;
+            if (mode == 0) { v1 = c->bp[is] * v1; v2 = c->bm[is] * v2; }
+            M[IDX(i1, j, N)] = v1;
+            M[IDX(i2, j, N)] = v2;
+        }
+    }
+}
+/* vconf right props on an m x N matrix (columns i1, i2):
+ *   mode 0: right_backward nflag 2 (dqmc_right_backward_prop.f90): cols <- (cols u) diag(bpi,bmi)
+ *   mode 1: right_backward nflag 1 and right_forward nflag 2: cols <- cols u (u = ut symmetric)
+ *   mode 2: right_forward nflag 1 (dqmc_right_forward_prop.f90): cols <- (cols diag(bp,bm)) ut */
+static void pv_right(orc_pctx *c, int nt, int nf, int mode, zc *M, int m)
+{
+    for (int q = 0; q < c->nn_lf; q++) {
+        int i1 = c->b->nnlf[q], i2 = c->b->nnlist[i1 * c->nn_nf + nf];
+        int is = pq_is(c, nt, nf, q) - 1;
+        zc *ca = M + (size_t)i1 * m, *cb = M + (size_t)i2 * m;
+        for (int j = 0; j < m; j++) {
+            zc x = ca[j], y = cb[j];
+            if (mode == 2) { x = c->bp[is] * x; y = c->bm[is] * y; }
+            zc v1 = x * pq_u[0] + y * pq_u[1];
+            zc v2 = x * pq_u[2] + y * pq_u[3];
+            if (mode == 0) { v1 = c->bpi[is] * v1; v2 = c->bmi[is] * v2; }
+            ca[j] = v1;
+            cb[j] = v2;
+        }
+    }
+}
+
+/* Bmat_left_forward of the projector core (src/dqmc_proj_core.f90:68-116), lwrapv only */
+static void pq_bmat_left_forward(orc_pctx *c, int nt1, int nt2, zc *M)
+{
+    int nfam = pq_nfam(c);
+    for (int nt = nt2; nt <= nt1; nt++) {
+        for (int q = 0; q < nfam; q++) pk_left(c, q, c->b->urt, M, c->ne);
+        for (int nf = 0; nf < c->nn_nf; nf++) {
+            pv_left(c, nt, nf, 0, M, c->ne);
+            pv_left(c, nt, nf, 1, M, c->ne);
+        }
+        for (int q = nfam - 1; q >= 0; q--) pk_left(c, q, c->b->urt, M, c->ne);
+    }
+}
+
+/* s_zgeQR_pqmc (lib/s_matrix.f90:1822-1996): Householder QR (zgeqrf) of the tall orientation,
+ * Q formed by zungqr, log|det R| from the real diagonal, and Q's first column negated when an
+ * odd number of R's diagonal entries is negative.  trans = 1: LQ of an ne x N matrix done as
+ * QR of its plain transpose.  a (tall: N x ne, or wide: ne x N) is replaced by Q. */
+static double pq_qr(orc_pctx *c, zc *a, int trans)
+{
+    int N = c->N, ne = c->ne;
+    zc *w = zalloc((size_t)N * ne), *tau = zalloc(ne);
+    if (!trans) memcpy(w, a, sizeof(zc) * (size_t)N * ne);
+    else
+        for (int j = 0; j < N; j++)
+            for (int i = 0; i < ne; i++) w[IDX(j, i, N)] = a[IDX(i, j, ne)];
+    zgeqr_core(N, ne, w, N, tau, NULL, 0);
+    int isgn = 0;
+    double rl = 0;
+    for (int i = 0; i < ne; i++) {
+        double r = creal(w[IDX(i, i, N)]);
+        if (r < 0) { isgn++; rl += log(-r); } else rl += log(r);
+    }
+    for (int j = 0; j < ne; j++)
+        for (int i = 0; i <= j; i++) w[IDX(i, j, N)] = (i == j) ? 1.0 : 0.0;
+    zungqr(N, ne, w, N, tau);
+    if (isgn % 2 == 1)
+        for (int i = 0; i < N; i++) w[IDX(i, 0, N)] = -w[IDX(i, 0, N)];
+    if (!trans) memcpy(a, w, sizeof(zc) * (size_t)N * ne);
+    else
+        for (int j = 0; j < N; j++)
+            for (int i = 0; i < ne; i++) a[IDX(i, j, ne)] = w[IDX(j, i, N)];
+    free(w); free(tau);
+    return rl;
+}
+
+/* ULR = UL UR, ULRINV = ULR^-1 (s_inv_logdet_lu_z, or s_invlu_z when logdet is not wanted) */
+static int pq_ulrinv(orc_pctx *c, const zc *ul, const zc *ur, zc *ulrinv, zc *logdet)
+{
+    int N = c->N, ne = c->ne;
+    zgemm('N', 'N', ne, ne, N, ul, ne, ur, N, 0, ulrinv, ne);
+    zc ld;
+    int info = zinv_logdet(ne, ulrinv, &ld);
+    if (logdet) *logdet = ld;
+    return info;
+}
+
+/* green_equaltime (src/dqmc_proj_core/green_equaltime.f90:1-25): G = I - UR ULRINV UL */
+static void pq_green(orc_pctx *c, const zc *ul, const zc *ur, const zc *ulrinv, zc *g)
+{
+    int N = c->N, ne = c->ne;
+    zc *t = zalloc((size_t)N * ne);
+    zgemm('N', 'N', N, ne, ne, ur, N, ulrinv, ne, 0, t, N);
+    zgemm('N', 'N', N, N, ne, t, N, ul, ne, 0, g, N);
+    for (size_t k = 0; k < (size_t)N * N; k++) g[k] = -g[k];
+    for (int i = 0; i < N; i++) g[IDX(i, i, N)] += 1.0;
+    free(t);
+}
+
+/* set_phase (src/dqmc_proj_core/set_phase.f90:1-24) for the t-V model: no field factors */
+static zc pq_phase(const orc_pctx *c, zc logw) { return cexp(CMPLX(0.0, (double)c->p.nflr * cimag(logw))); }
+
+/* submatrix-LR update of family nf at slice nt (src/dqmc_config_v/dqmc_proj_update_submatrix_LR.f90:1-298) */
+static void pq_update(orc_pctx *c, int nt, int nf)
+{
+    int N = c->N, ne = c->ne, ns = c->nustock, nl = c->nn_lf;
+    orc_ctx *b = c->b;
+    zc *ulP = zalloc((size_t)ne * ns), *urP = zalloc((size_t)ns * ne);
+    zc *Rrows = zalloc((size_t)ns * ne), *Felms = zalloc((size_t)ns * ns);
+    zc *Ginv = zalloc((size_t)ns * ns), *Vmat = zalloc((size_t)ns * ne), *Umat = zalloc((size_t)ne * ns);
+    zc *Vtmp = zalloc((size_t)ns * ne), *Utmp = zalloc((size_t)ne * ns);
+    zc *ikk = zalloc((size_t)ns * 2), *kik = zalloc((size_t)2 * ns), *Pikk = zalloc((size_t)ns * 2), *Pkik = zalloc((size_t)2 * ns);
+    int *xvec = (int *)calloc(ns, sizeof(int)), *svec = (int *)calloc(ns, sizeof(int));
+    zc *Dvec = zalloc(ns);
+    int ik = 0, istk_s = 0, istk_e = -1, nstk = 0;
+    double accm = 0;
+    for (int ib = 0; ib < nl; ib++) {
+        int i1 = b->nnlf[ib], i2 = b->nnlist[i1 * c->nn_nf + nf];
+        int32_t *cf = &c->conf[((size_t)(nt - 1) * c->nn_nf + nf) * N + ib];
+        int is = *cf;
+        int iflip = (int)ceil(sfmt_stream(&b->rng) * 1.0);
+        int isp = (is + iflip - 1) % 2 + 1;
+        if (nstk == 0) {
+            int nup = ns < (nl - ib) * 2 ? ns : (nl - ib) * 2;
+            memset(ulP, 0, sizeof(zc) * (size_t)ne * ns);
+            memset(urP, 0, sizeof(zc) * (size_t)ns * ne);
+            for (int q = 0; q < nup / 2; q++) {
+                int i3 = b->nnlf[ib + q], i4 = b->nnlist[i3 * c->nn_nf + nf];
+                for (int r = 0; r < ne; r++) {
+                    ulP[IDX(r, 2 * q, ne)] = c->UL[IDX(r, i3, ne)];
+                    ulP[IDX(r, 2 * q + 1, ne)] = c->UL[IDX(r, i4, ne)];
+                    urP[IDX(2 * q, r, ns)] = c->UR[IDX(i3, r, N)];
+                    urP[IDX(2 * q + 1, r, ns)] = c->UR[IDX(i4, r, N)];
+                }
+            }
+            zgemm('N', 'N', nup, ne, ne, urP, ns, c->ULRINV, ne, 0, Rrows, ns);
+            zgemm('N', 'N', nup, nup, ne, Rrows, ns, ulP, ne, 0, Felms, ns);
+            istk_s = ib; istk_e = ib + nup / 2 - 1; nstk = nup / 2;
+        }
+        int s1 = (ib - istk_s) * 2, s2 = s1 + 1;
+        zc v[4] = {0, 0, 0, 0};   /* column-major 2x2: v11, v21, v12, v22 */
+        if (ik > 0) {
+            for (int m = 0; m < ik; m++) {
+                Pkik[IDX(0, m, 2)] = Felms[IDX(s1, svec[m], ns)];
+                Pkik[IDX(1, m, 2)] = Felms[IDX(s2, svec[m], ns)];
+                Pikk[IDX(m, 0, ns)] = Felms[IDX(svec[m], s1, ns)];
+                Pikk[IDX(m, 1, ns)] = Felms[IDX(svec[m], s2, ns)];
+            }
+            zgemm('N', 'N', ik, 2, ik, Ginv, ns, Pikk, ns, 0, ikk, ns);
+            zgemm('N', 'N', 2, 2, ik, Pkik, 2, ikk, ns, 1, v, 2);
+        }
+        v[0] = Felms[IDX(s1, s1, ns)] - v[0];
+        v[1] = Felms[IDX(s2, s1, ns)] - v[1];
+        v[2] = Felms[IDX(s1, s2, ns)] - v[2];
+        v[3] = Felms[IDX(s2, s2, ns)] - v[3];
+        zc dp = c->dp[is - 1], dm = c->dm[is - 1];
+        v[0] = 1.0 + v[0] * dp;
+        v[1] = v[1] * dp;
+        v[2] = v[2] * dm;
+        v[3] = 1.0 + v[3] * dm;
+        zc ratio1 = v[0] * v[3] - v[2] * v[1];
+        double ratio_re = creal(ratio1 * c->phase) / creal(c->phase);
+        double ratio_abs = ratio_re < 0 ? -ratio_re : ratio_re;
+        double random = sfmt_stream(&b->rng);
+        if (ratio_abs > random) {
+            accm += 1;
+            double weight = sqrt(creal(ratio1 * conj(ratio1)));
+            c->phase = c->phase * ratio1 / weight;
+            zc gk[4];
+            gk[0] = v[3] * dp / ratio1;
+            gk[2] = -v[2] * dp / ratio1;
+            gk[1] = -v[1] * dm / ratio1;
+            gk[3] = v[0] * dm / ratio1;
+            Ginv[IDX(ik, ik, ns)] = gk[0]; Ginv[IDX(ik + 1, ik, ns)] = gk[1];
+            Ginv[IDX(ik, ik + 1, ns)] = gk[2]; Ginv[IDX(ik + 1, ik + 1, ns)] = gk[3];
+            if (ik > 0) {
+                zgemm('N', 'N', 2, ik, ik, Pkik, 2, Ginv, ns, 0, kik, 2);
+                /* 21 block = -gk kik */
+                for (int m = 0; m < ik; m++)
+                    for (int r = 0; r < 2; r++)
+                        Ginv[IDX(ik + r, m, ns)] = -(gk[IDX(r, 0, 2)] * kik[IDX(0, m, 2)] + gk[IDX(r, 1, 2)] * kik[IDX(1, m, 2)]);
+                /* 12 block = -ikk gk */
+                for (int r = 0; r < 2; r++)
+                    for (int m = 0; m < ik; m++)
+                        Ginv[IDX(m, ik + r, ns)] = -(ikk[IDX(m, 0, ns)] * gk[IDX(0, r, 2)] + ikk[IDX(m, 1, ns)] * gk[IDX(1, r, 2)]);
+                /* 11 block += -ikk * (21 block) */
+                for (int q = 0; q < ik; q++)
+                    for (int m = 0; m < ik; m++)
+                        Ginv[IDX(m, q, ns)] -= ikk[IDX(m, 0, ns)] * Ginv[IDX(ik, q, ns)] + ikk[IDX(m, 1, ns)] * Ginv[IDX(ik + 1, q, ns)];
+            }
+            ik += 2;
+            xvec[ik - 2] = i1; xvec[ik - 1] = i2;
+            svec[ik - 2] = s1; svec[ik - 1] = s2;
+            Dvec[ik - 2] = dp; Dvec[ik - 1] = dm;
+            for (int r = 0; r < ne; r++) {
+                Vmat[IDX(ik - 2, r, ns)] = Rrows[IDX(s1, r, ns)];
+                Vmat[IDX(ik - 1, r, ns)] = Rrows[IDX(s2, r, ns)];
+                Umat[IDX(r, ik - 2, ne)] = ulP[IDX(r, s1, ne)];
+                Umat[IDX(r, ik - 1, ne)] = ulP[IDX(r, s2, ne)];
+            }
+            *cf = isp;
+        }
+        if (ib == istk_e || ib == nl - 1) {
+            if (ik > 0) {
+                zgemm('N', 'N', ik, ne, ik, Ginv, ns, Vmat, ns, 0, Vtmp, ns);
+                zgemm('N', 'N', ne, ik, ne, c->ULRINV, ne, Umat, ne, 0, Utmp, ne);
+                zgemm('N', 'N', ne, ne, ik, Utmp, ne, Vtmp, ns, 0, c->tmp, ne);
+                for (size_t k = 0; k < (size_t)ne * ne; k++) c->ULRINV[k] -= c->tmp[k];
+                for (int m = 0; m < ik; m++) {
+                    int x = xvec[m];
+                    zc d = Dvec[m];
+                    for (int r = 0; r < ne; r++) c->UR[IDX(x, r, N)] = c->UR[IDX(x, r, N)] + d * c->UR[IDX(x, r, N)];
+                }
+            }
+            ik = 0; istk_s = 0; istk_e = -1; nstk = 0;
+            memset(Ginv, 0, sizeof(zc) * (size_t)ns * ns);
+            memset(Vmat, 0, sizeof(zc) * (size_t)ns * ne);
+            memset(Umat, 0, sizeof(zc) * (size_t)ne * ns);
+        }
+    }
+    c->accepted += accm;
+    c->tried += nl;
+    free(ulP); free(urP); free(Rrows); free(Felms); free(Ginv); free(Vmat); free(Umat);
+    free(Vtmp); free(Utmp); free(ikk); free(kik); free(Pikk); free(Pkik); free(xvec); free(svec); free(Dvec);
+}
+
+/* dqmc_proj_sweep_start_0b (src/dqmc_proj_core/dqmc_proj_sweep_start_0b.f90:1-48) */
+int orc_psweep_start(orc_pctx *c)
+{
+    int N = c->N, ne = c->ne;
+    if (c->nst <= 0) return -1;
+    memcpy(c->UR, c->projR, sizeof(zc) * (size_t)N * ne);
+    c->logweightf = 0;
+    c->logwDV[0] = 0;
+    for (int n = 1; n <= c->nst; n++) {
+        pq_bmat_left_forward(c, c->b->wrap_step[2 * (n - 1) + 1], c->b->wrap_step[2 * (n - 1)], c->UR);
+        double lw = pq_qr(c, c->UR, 0);
+        memcpy(c->Ust[n], c->UR, sizeof(zc) * (size_t)N * ne);
+        c->logwDV[n] = c->logwDV[n - 1] + lw;
+    }
+    for (int l = 0; l < ne; l++)
+        for (int i = 0; i < N; i++) c->UL[IDX(l, i, ne)] = conj(c->projR[IDX(i, l, N)]);
+    zc z;
+    int info = pq_ulrinv(c, c->UL, c->UR, c->ULRINV, &z);
+    c->phase = pq_phase(c, z);
+    c->logweightf = c->logwDV[c->nst] + z;
+    return info;
+}
+
+static int pq_in_window(const orc_pctx *c, int nt)
+{
+    int lt = c->ltrot, ml = c->p.mid_len;
+    return nt > (lt / 2 - (ml + 1) / 2) && nt <= (lt / 2 + ml / 2);
+}
+
+static void pq_measure(orc_pctx *c)
+{
+    pq_ulrinv(c, c->UL, c->UR, c->ULRINV, NULL);
+    pq_green(c, c->UL, c->UR, c->ULRINV, c->G);
+    c->b->G = c->G;
+    c->b->phase = c->phase;
+    measure(c->b);
+}
+
+/* the scratch recomputation at a stabilization point, and the wrap-error check of
+ * dqmc_proj_sweep_0b.f90:121-190 (under WRAPERROR): G from the propagated UL, UR against G
+ * from the orthonormalised ones, and the phase difference */
+static int pq_scratch(orc_pctx *c, const zc *ul_old, const zc *ur_old, double logw_dv)
+{
+    int N = c->N, ne = c->ne;
+    zc z;
+    int info = pq_ulrinv(c, c->UL, c->UR, c->ULRINV, &z);
+    zc lw = logw_dv + z;
+    zc ph = pq_phase(c, lw);
+    pq_green(c, c->UL, c->UR, c->ULRINV, c->G);
+    if (c->p.wrap_check) {
+        zc *inv = zalloc((size_t)ne * ne), *g2 = zalloc((size_t)N * N);
+        pq_ulrinv(c, ul_old, ur_old, inv, NULL);
+        pq_green(c, ul_old, ur_old, inv, g2);
+        double mx = 0;
+        for (size_t k = 0; k < (size_t)N * N; k++) { double d = cabs(g2[k] - c->G[k]); if (d > mx) mx = d; }
+        if (mx > c->max_wrap_error) c->max_wrap_error = mx;
+        double pd = cabs(c->phase - ph);
+        if (pd > c->max_phasediff) c->max_phasediff = pd;
+        c->nchecks += 1;
+        c->sumlog10err += log10(mx > 0 ? mx : 1e-300);
+        free(inv); free(g2);
+    }
+    c->phase = ph;
+    c->logweightf = lw;
+    if (c->b->trace && c->b->trace_n < c->b->trace_cap) {
+        memcpy(c->b->trace + (size_t)2 * N * N * c->b->trace_n, c->G, sizeof(zc) * (size_t)N * N);
+        c->b->trace_n++;
+    }
+    return info;
+}
+
+/* dqmc_proj_sweep_b0 (src/dqmc_proj_core/dqmc_proj_sweep_b0.f90:1-220) */
+int orc_psweep_b0(orc_pctx *c, int lupdate, int lmeasure)
+{
+    int N = c->N, ne = c->ne, nst = c->nst, nfam = pq_nfam(c);
+    orc_ctx *b = c->b;
+    if (nst <= 0) return -1;
+    for (int l = 0; l < ne; l++)
+        for (int i = 0; i < N; i++) c->UL[IDX(l, i, ne)] = conj(c->projR[IDX(i, l, N)]);
+    memcpy(c->Ust[0], c->projR, sizeof(zc) * (size_t)N * ne);
+    zc *ul_old = zalloc((size_t)N * ne), *ur_old = zalloc((size_t)N * ne);
+    int rc = 0;
+    for (int nt = c->ltrot; nt >= 1; nt--) {
+        if (lmeasure && pq_in_window(c, nt)) pq_measure(c);
+        for (int q = 0; q < nfam; q++) { pk_right(c, q, b->urt, c->UL, ne); pk_left(c, q, b->urtm1, c->UR, ne); }
+        for (int nf = c->nn_nf - 1; nf >= 0; nf--) {
+            pv_right(c, nt, nf, 1, c->UL, ne);
+            pv_left(c, nt, nf, 1, c->UR, ne);
+            if (lupdate) pq_update(c, nt, nf);
+            pv_right(c, nt, nf, 2, c->UL, ne);
+            pv_left(c, nt, nf, 2, c->UR, ne);
+        }
+        for (int q = nfam - 1; q >= 0; q--) { pk_right(c, q, b->urt, c->UL, ne); pk_left(c, q, b->urtm1, c->UR, ne); }
+        if ((b->iwrap_nt[nt - 1] > 0 || b->nwrap == 1) || nt == 1) {
+            int n = b->iwrap_nt[nt - 1];
+            memcpy(ul_old, c->UL, sizeof(zc) * (size_t)N * ne);
+            memcpy(ur_old, c->UR, sizeof(zc) * (size_t)N * ne);
+            memcpy(c->UR, c->Ust[n], sizeof(zc) * (size_t)N * ne);
+            double lw = pq_qr(c, c->UL, 1);
+            if (n + 1 == nst) c->logwDV[n + 1] = lw;
+            else c->logwDV[n + 1] = c->logwDV[n + 2] + lw;
+            double lsum = (n == 0) ? c->logwDV[n + 1] : c->logwDV[n + 1] + c->logwDV[n];
+            int info = pq_scratch(c, ul_old, ur_old, lsum);
+            for (int l = 0; l < ne; l++)
+                for (int i = 0; i < N; i++) c->Ust[n][IDX(i, l, N)] = c->UL[IDX(l, i, ne)];
+            if (info) { rc = info; break; }
+        }
+    }
+    free(ul_old); free(ur_old);
+    return rc;
+}
+
+/* dqmc_proj_sweep_0b (src/dqmc_proj_core/dqmc_proj_sweep_0b.f90:1-235), without the ltau branch */
+int orc_psweep_0b(orc_pctx *c, int lupdate, int lmeasure)
+{
+    int N = c->N, ne = c->ne, nst = c->nst, nfam = pq_nfam(c);
+    orc_ctx *b = c->b;
+    if (nst <= 0) return -1;
+    memcpy(c->UR, c->projR, sizeof(zc) * (size_t)N * ne);
+    for (int l = 0; l < ne; l++)
+        for (int i = 0; i < N; i++) c->Ust[nst][IDX(i, l, N)] = conj(c->projR[IDX(i, l, N)]);
+    c->logwDV[0] = 0;
+    zc *ul_old = zalloc((size_t)N * ne), *ur_old = zalloc((size_t)N * ne);
+    int rc = 0;
+    for (int nt = 1; nt <= c->ltrot; nt++) {
+        for (int q = 0; q < nfam; q++) { pk_left(c, q, b->urt, c->UR, ne); pk_right(c, q, b->urtm1, c->UL, ne); }
+        for (int nf = 0; nf < c->nn_nf; nf++) {
+            pv_left(c, nt, nf, 0, c->UR, ne);
+            pv_right(c, nt, nf, 0, c->UL, ne);
+            if (lupdate) pq_update(c, nt, nf);
+            pv_left(c, nt, nf, 1, c->UR, ne);
+            pv_right(c, nt, nf, 1, c->UL, ne);
+        }
+        for (int q = nfam - 1; q >= 0; q--) { pk_left(c, q, b->urt, c->UR, ne); pk_right(c, q, b->urtm1, c->UL, ne); }
+        if (lmeasure && pq_in_window(c, nt)) pq_measure(c);
+        if (b->iwrap_nt[nt] > 0) {
+            int n = b->iwrap_nt[nt];
+            memcpy(ul_old, c->UL, sizeof(zc) * (size_t)N * ne);
+            memcpy(ur_old, c->UR, sizeof(zc) * (size_t)N * ne);
+            for (int l = 0; l < ne; l++)
+                for (int i = 0; i < N; i++) c->UL[IDX(l, i, ne)] = c->Ust[n][IDX(i, l, N)];
+            double lw = pq_qr(c, c->UR, 0);
+            c->logwDV[n] = c->logwDV[n - 1] + lw;
+            double lsum = (n == nst) ? c->logwDV[n] : c->logwDV[n] + c->logwDV[n + 1];
+            int info = pq_scratch(c, ul_old, ur_old, lsum);
+            memcpy(c->Ust[n], c->UR, sizeof(zc) * (size_t)N * ne);
+            if (info) { rc = info; break; }
+        }
+    }
+    free(ul_old); free(ur_old);
+    return rc;
+}
+
+/* G = I - UR (UL UR)^-1 UL from the current UL, UR (fresh inverse) */
+void orc_pget_g(orc_pctx *c, double *g)
+{
+    zc *inv = zalloc((size_t)c->ne * c->ne);
+    pq_ulrinv(c, c->UL, c->UR, inv, NULL);
+    pq_green(c, c->UL, c->UR, inv, (zc *)g);
+    free(inv);
+}
+void orc_pget_state(const orc_pctx *c, double *ul, double *ur, double *ulrinv)
+{
+    if (ul) memcpy(ul, c->UL, sizeof(zc) * (size_t)c->N * c->ne);
+    if (ur) memcpy(ur, c->UR, sizeof(zc) * (size_t)c->N * c->ne);
+    if (ulrinv) memcpy(ulrinv, c->ULRINV, sizeof(zc) * (size_t)c->ne * c->ne);
+}
+void orc_pget_diag(const orc_pctx *c, double *d)
+{
+    d[0] = c->max_wrap_error; d[1] = c->max_phasediff;
+    d[2] = creal(c->phase); d[3] = cimag(c->phase);
+    d[4] = creal(c->logweightf); d[5] = cimag(c->logweightf);
+    d[6] = c->accepted; d[7] = c->tried; d[8] = c->nchecks; d[9] = c->sumlog10err;
+}
+void orc_preset_diag(orc_pctx *c)
+{
+    c->max_wrap_error = c->max_phasediff = c->nchecks = c->sumlog10err = c->accepted = c->tried = 0;
+}
+void orc_pget_obs(const orc_pctx *c, double *o) { orc_get_obs(c->b, o); }
+void orc_preset_obs(orc_pctx *c) { orc_reset_obs(c->b); }
+void orc_pset_trace(orc_pctx *c, double *buf, int cap) { orc_set_trace(c->b, buf, cap); }
+int orc_ptrace_count(const orc_pctx *c) { return c->b->trace_n; }
+/* kind 0: (bp, bm, bpi, bmi) for is = 1, 2 -> 8 complex; kind 1: (dp, dm) for is = 1, 2 -> 4 complex;
+ * other kinds as orc_get_table */
+int orc_pget_table(const orc_pctx *c, int kind, double *out)
+{
+    zc *z = (zc *)out;
+    if (kind == 100) {
+        for (int is = 0; is < 2; is++) { z[4 * is] = c->bp[is]; z[4 * is + 1] = c->bm[is]; z[4 * is + 2] = c->bpi[is]; z[4 * is + 3] = c->bmi[is]; }
+        return 8;
+    }
+    if (kind == 101) {
+        for (int is = 0; is < 2; is++) { z[2 * is] = c->dp[is]; z[2 * is + 1] = c->dm[is]; }
+        return 4;
+    }
+    return orc_get_table(c->b, kind, out);
+}

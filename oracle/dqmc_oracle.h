@@ -79,6 +79,42 @@ void orc_qr(int m, int n, const double *a, double *q, double *r, int32_t *jpvt, 
 /* inverse with log det via LU (s_inv_logdet_lu_z); returns 0 or the singular column */
 int orc_inv_logdet(int n, double *a, double *logdet);
 
+
+/* ---- projective DQMC: spinless t-V model, bond HS field, submatrix-LR update ---------- */
+typedef struct orc_pparams {
+    int lattice, la, lb;
+    double beta, dtau, rt, mu, rv, nu, rndness;
+    int nflr;
+    int kmode;      /* 0 = checkerboard, 1 = dense */
+    int nwrap, nustock, mid_len;
+    int wrap_check; /* 1: the WRAPERROR comparison at every stabilization */
+    int seed;
+} orc_pparams;
+
+typedef struct orc_pctx orc_pctx;
+orc_pctx *orc_pcreate(const orc_pparams *p);
+void orc_pdestroy(orc_pctx *c);
+/* sizes: [0]=N [1]=ne [2]=ltrot [3]=nst [4]=nn_nf [5]=nn_lf [6]=nustock [7]=lq */
+void orc_psizes(const orc_pctx *c, int *out);
+void orc_pinit_conf(orc_pctx *c);                          /* N*nn_nf*ltrot draws */
+void orc_pget_conf(const orc_pctx *c, int32_t *conf);      /* [ltrot][nn_nf][N] */
+void orc_pset_conf(orc_pctx *c, const int32_t *conf);
+double orc_pset_trial(orc_pctx *c);                        /* nn_lf*nn_nf draws; returns the gap */
+void orc_pset_proj(orc_pctx *c, const double *projR);      /* N x ne complex */
+void orc_pget_proj(const orc_pctx *c, double *projR, double *w);
+int orc_psweep_start(orc_pctx *c);
+int orc_psweep_b0(orc_pctx *c, int lupdate, int lmeasure);
+int orc_psweep_0b(orc_pctx *c, int lupdate, int lmeasure);
+void orc_pget_g(orc_pctx *c, double *g);
+void orc_pget_state(const orc_pctx *c, double *ul, double *ur, double *ulrinv);
+void orc_pget_diag(const orc_pctx *c, double *d);
+void orc_preset_diag(orc_pctx *c);
+void orc_pget_obs(const orc_pctx *c, double *o);
+void orc_preset_obs(orc_pctx *c);
+void orc_pset_trace(orc_pctx *c, double *buf, int cap);
+int orc_ptrace_count(const orc_pctx *c);
+int orc_pget_table(const orc_pctx *c, int kind, double *out);
+
 #ifdef __cplusplus
 }
 #endif

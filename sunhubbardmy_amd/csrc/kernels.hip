@@ -87,7 +87,7 @@ extern "C" __global__ void k_sfmt_draw(const DChain *chains, int nchains, int mo
 //               8 = V scaling before everything.  Several slices can be chained (B products).
 // Bond 2x2 blocks are column-major [T11, T21, T12, T22] (dqmc_left_forward_prop.f90:18-44).
 // ------------------------------------------------------------------------------------------
-#define CPB 4
+#define CPB 8
 #define RPB 8
 
 __device__ __forceinline__ cplx vtab(const DChain &c, int vsel, int is)
@@ -98,19 +98,25 @@ __device__ __forceinline__ cplx vtab(const DChain &c, int vsel, int is)
     return b;
 }
 
+// one bond per thread (and a stride of columns): the bond's endpoints and 2x2 block are read
+// once per family and applied to every column the thread covers
 __device__ static void cb_left_fams(const DChain &c, cplx *lds, int ncol, const cplx *T, bool rev)
 {
     const int nf = c.nn_nf, nl = c.nn_lf, n = c.n;
+    const int groups = max(1, (int)blockDim.x / nl);
+    const int b = threadIdx.x % nl, g = threadIdx.x / nl;
     for (int q = 0; q < nf; q++) {
         int f = rev ? nf - 1 - q : q;
-        for (int e = threadIdx.x; e < nl * ncol; e += blockDim.x) {
-            int b = e % nl, cc = e / nl;
-            int ist = f * nl + b;
-            int i1 = c.bonds[2 * ist], i2 = c.bonds[2 * ist + 1];
-            const cplx *t = T + 4 * ist;
-            cplx x = lds[cc * n + i1], y = lds[cc * n + i2];
-            lds[cc * n + i1] = cadd(cmul(t[0], x), cmul(t[2], y));
-            lds[cc * n + i2] = cadd(cmul(t[1], x), cmul(t[3], y));
+        for (int bb = b; bb < nl; bb += blockDim.x) {
+            if (g >= groups) break;
+            const int ist = f * nl + bb;
+            const int i1 = c.bonds[2 * ist], i2 = c.bonds[2 * ist + 1];
+            const cplx t0 = T[4 * ist], t1 = T[4 * ist + 1], t2 = T[4 * ist + 2], t3 = T[4 * ist + 3];
+            for (int cc = g; cc < ncol; cc += groups) {
+                cplx x = lds[cc * n + i1], y = lds[cc * n + i2];
+                lds[cc * n + i1] = cadd(cmul(t0, x), cmul(t2, y));
+                lds[cc * n + i2] = cadd(cmul(t1, x), cmul(t3, y));
+            }
         }
         __syncthreads();
     }
@@ -161,17 +167,20 @@ k_cb_cols(const DChain *chains, int msel, int pt, int ksel, int prog, int vsel, 
 __device__ static void cb_right_fams(const DChain &c, cplx *lds, int nrow, const cplx *T, bool rev)
 {
     const int nf = c.nn_nf, nl = c.nn_lf;
+    const int groups = max(1, (int)blockDim.x / nl);
+    const int b = threadIdx.x % nl, g = threadIdx.x / nl;
     for (int q = 0; q < nf; q++) {
         int f = rev ? nf - 1 - q : q;
-        for (int e = threadIdx.x; e < nl * RPB; e += blockDim.x) {
-            int rr = e % RPB, b = e / RPB;
-            if (rr >= nrow) continue;
-            int ist = f * nl + b;
-            int i1 = c.bonds[2 * ist], i2 = c.bonds[2 * ist + 1];
-            const cplx *t = T + 4 * ist;
-            cplx x = lds[i1 * RPB + rr], y = lds[i2 * RPB + rr];
-            lds[i1 * RPB + rr] = cadd(cmul(x, t[0]), cmul(y, t[1]));
-            lds[i2 * RPB + rr] = cadd(cmul(x, t[2]), cmul(y, t[3]));
+        for (int bb = b; bb < nl; bb += blockDim.x) {
+            if (g >= groups) break;
+            const int ist = f * nl + bb;
+            const int i1 = c.bonds[2 * ist], i2 = c.bonds[2 * ist + 1];
+            const cplx t0 = T[4 * ist], t1 = T[4 * ist + 1], t2 = T[4 * ist + 2], t3 = T[4 * ist + 3];
+            for (int rr = g; rr < nrow; rr += groups) {
+                cplx x = lds[i1 * RPB + rr], y = lds[i2 * RPB + rr];
+                lds[i1 * RPB + rr] = cadd(cmul(x, t0), cmul(y, t1));
+                lds[i2 * RPB + rr] = cadd(cmul(x, t2), cmul(y, t3));
+            }
         }
         __syncthreads();
     }
@@ -605,15 +614,59 @@ __device__ static double block_sum(double v, double *red)
     return s;
 }
 
+// partial sums of S = sum_ij s_i s_j G(i,j) G(j,i) over one 32x32 tile pair (bi <= bj) per
+// workgroup, written to rwork[2 tp]
+extern "C" __global__ void __launch_bounds__(256)
+k_measure_tiles(const DChain *chains)
+{
+    __shared__ cplx ta[32][33], tb[32][33];
+    __shared__ double red[16];
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    const int nb = (n + 31) / 32;
+    int tp = blockIdx.x;
+    if (tp >= nb * (nb + 1) / 2) return;
+    int bi = 0;
+    while (tp >= nb - bi) { tp -= nb - bi; bi++; }
+    const int bj = bi + tp;
+    const cplx *G = c.G;
+    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+        int q = e % 32, p = e / 32;
+        int r1 = bi * 32 + q, c1 = bj * 32 + p;
+        ta[q][p] = (r1 < n && c1 < n) ? G[(size_t)r1 + (size_t)c1 * n] : make_double2(0.0, 0.0);
+        int r2 = bj * 32 + q, c2 = bi * 32 + p;
+        tb[q][p] = (r2 < n && c2 < n) ? G[(size_t)r2 + (size_t)c2 * n] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const double wgt = (bi == bj) ? 1.0 : 2.0;
+    double ssr = 0.0, ssi = 0.0;
+    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+        int q = e % 32, p = e / 32;
+        int i = bi * 32 + q, j = bj * 32 + p;
+        if (i < n && j < n) {
+            cplx pr = cmul(ta[q][p], tb[p][q]);   // G(i,j) G(j,i)
+            double sg = c.stag[i] * c.stag[j] * wgt;
+            ssr += sg * pr.x;
+            ssi += sg * pr.y;
+        }
+    }
+    ssr = block_sum(ssr, red);
+    ssi = block_sum(ssi, red);
+    if (threadIdx.x == 0) {
+        c.rwork[2 * blockIdx.x] = ssr;
+        c.rwork[2 * blockIdx.x + 1] = ssi;
+    }
+}
+
+// diagonal and bond terms, the tile partials, and the zphi-weighted bin sums; one workgroup per chain
 extern "C" __global__ void __launch_bounds__(256)
 k_measure(const DChain *chains)
 {
-    __shared__ cplx ta[32][33], tb[32][33];
     __shared__ double red[16];
     const DChain &c = chains[blockIdx.x];
     const int n = c.n;
     const cplx *G = c.G;
-    double zne_r = 0, zne_i = 0, zu_r = 0, zu_i = 0, zk = 0, trr = 0, tri = 0;
+    double zne_r = 0, zne_i = 0, zu_r = 0, zu_i = 0, zk = 0, trr = 0, tri = 0, ssr = 0, ssi = 0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         cplx g = G[(size_t)i + (size_t)i * n];
         cplx gc = make_double2(1.0 - g.x, -g.y);
@@ -622,38 +675,15 @@ k_measure(const DChain *chains)
         zu_r += g2.x; zu_i += g2.y;
         trr += g.x; tri += g.y;
     }
-    const int nb = c.nn_nf * c.nn_lf;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    const int nbond = c.nn_nf * c.nn_lf;
+    for (int b = threadIdx.x; b < nbond; b += blockDim.x) {
         int i0 = c.bonds[2 * b], i1 = c.bonds[2 * b + 1];
         // gfc(i0,i1) = delta - gf(i1,i0); kinetic term -2 Re(gfc(i0,i1))
         double gcr = (i0 == i1 ? 1.0 : 0.0) - G[(size_t)i1 + (size_t)i0 * n].x;
         zk += -2.0 * gcr;
     }
-    // S = sum_ij s_i s_j G(i,j) G(j,i)
-    double ssr = 0, ssi = 0;
-    const int ntile = (n + 31) / 32;
-    for (int bi = 0; bi < ntile; bi++)
-        for (int bj = bi; bj < ntile; bj++) {
-            __syncthreads();
-            for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
-                int q = e % 32, p = e / 32;
-                int r1 = bi * 32 + q, c1 = bj * 32 + p;
-                ta[q][p] = (r1 < n && c1 < n) ? G[(size_t)r1 + (size_t)c1 * n] : make_double2(0.0, 0.0);
-                int r2 = bj * 32 + q, c2 = bi * 32 + p;
-                tb[q][p] = (r2 < n && c2 < n) ? G[(size_t)r2 + (size_t)c2 * n] : make_double2(0.0, 0.0);
-            }
-            __syncthreads();
-            double wgt = (bi == bj) ? 1.0 : 2.0;
-            for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
-                int q = e % 32, p = e / 32;
-                int i = bi * 32 + q, j = bj * 32 + p;
-                if (i < n && j < n) {
-                    cplx pr = cmul(ta[q][p], tb[p][q]);   // G(i,j) G(j,i)
-                    double s = c.stag[i] * c.stag[j] * wgt;
-                    ssr += s * pr.x; ssi += s * pr.y;
-                }
-            }
-        }
+    const int nb = (n + 31) / 32, ntp = nb * (nb + 1) / 2;
+    for (int t = threadIdx.x; t < ntp; t += blockDim.x) { ssr += c.rwork[2 * t]; ssi += c.rwork[2 * t + 1]; }
     zne_r = block_sum(zne_r, red); zne_i = block_sum(zne_i, red);
     zu_r = block_sum(zu_r, red); zu_i = block_sum(zu_i, red);
     zk = block_sum(zk, red);
@@ -896,17 +926,16 @@ k_v_from_qrcp(const DChain *chains, int sel, int pt)
 }
 
 // prepivot: Btmp = Bd diag(Dprev); columns sorted by decreasing norm (s_zmcpt); the sorted
-// matrix goes to W2, the order to jpvt.  One workgroup per chain.
+// matrix goes to W2, the order to jpvt.  One workgroup per chain; the selection sort (first
+// maximum wins, the displaced entry takes the maximum's slot) runs in wave 0 without barriers.
 extern "C" __global__ void __launch_bounds__(1024)
 k_prepivot(const DChain *chains, int dpt)
 {
     extern __shared__ double psh[];
     const DChain &c = chains[blockIdx.x];
     const int n = c.n;
-    double *vn = psh;
-    int *jp = (int *)(psh + n);
-    __shared__ double rv[16];
-    __shared__ int ri[16];
+    volatile double *vn = psh;
+    volatile int *jp = (volatile int *)(psh + n);
     const double *Dp = c.D + (size_t)n * dpt;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     cplx *B = c.W1;
@@ -922,20 +951,27 @@ k_prepivot(const DChain *chains, int dpt)
         if (lane == 0) { vn[j] = sqrt(s); jp[j] = j + 1; }
     }
     __syncthreads();
-    for (int k = 0; k < n; k++) {
-        double bv = -1.0;
-        int bi = k;
-        for (int j = k + threadIdx.x; j < n; j += blockDim.x)
-            if (fabs(vn[j]) > bv) { bv = fabs(vn[j]); bi = j; }
-        double mv;
-        int p;
-        block_argmax(bv, bi, rv, ri, &mv, &p);
-        if (threadIdx.x == 0 && p != k) {
-            int it = jp[p]; jp[p] = jp[k]; jp[k] = it;
-            vn[p] = vn[k];
+    if (w == 0) {
+        for (int k = 0; k < n; k++) {
+            double bv = -1.0;
+            int bi = k;
+            for (int j = k + lane; j < n; j += 64) {
+                double a = fabs(vn[j]);
+                if (a > bv) { bv = a; bi = j; }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                double v2 = __shfl_xor(bv, o, 64);
+                int i2 = __shfl_xor(bi, o, 64);
+                if (v2 > bv || (v2 == bv && i2 < bi)) { bv = v2; bi = i2; }
+            }
+            if (lane == 0 && bi != k) {
+                int it = jp[bi]; jp[bi] = jp[k]; jp[k] = it;
+                vn[bi] = vn[k];
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        __syncthreads();
     }
+    __syncthreads();
     for (int j = threadIdx.x; j < n; j += blockDim.x) c.jpvt[j] = jp[j];
     for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
         int r = (int)(e % n), k = (int)(e / n);
@@ -1383,6 +1419,8 @@ k_save_point(const DChain *chains, int pt)
 // keeps bvec(r, 0:K) in registers.  An accepted site broadcasts row s of both through LDS,
 // every thread extends its own row, and avec/bvec reach memory only for the MFMA flush.
 // ------------------------------------------------------------------------------------------
+#define DSB 8   // sites per staged block of G columns and rows
+
 template <int K>
 __global__ void __launch_bounds__(K >= 16 ? 512 : 1024)
 k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
@@ -1393,11 +1431,15 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
     const int R = blockDim.x >> 1;
     const bool isA = threadIdx.x < R;
     const int r = isA ? threadIdx.x : threadIdx.x - R;
-    cplx *diagg = sh;            // n
-    cplx *anew = diagg + n;      // n
-    cplx *bnew = anew + n;       // n
-    cplx *rowa = bnew + n;       // K
-    cplx *rowb = rowa + K;       // K
+    cplx *diagg = sh;                   // n
+    cplx *anew = diagg + n;             // n
+    cplx *bnew = anew + n;              // n
+    cplx *gcol = bnew + n;              // DSB x n: G(:, s0+j)
+    cplx *grow = gcol + DSB * n;        // DSB x n: G(s0+j, :)
+    cplx *rowa = grow + DSB * n;        // K
+    cplx *rowb = rowa + K;              // K
+    double *us = (double *)(rowb + K);  // 2n uniforms of this slice
+    int *cfs = (int *)(us + 2 * n);     // n: HS field of this slice
     cplx *G = c.G;
     int32_t *cf = c.conf + (size_t)(nt - 1) * n;
     const double *u = c.unif + (size_t)2 * n * slice_in_pass;
@@ -1405,16 +1447,33 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
     cplx reg[K];
 #pragma unroll
     for (int m = 0; m < K; m++) reg[m] = make_double2(0.0, 0.0);
-    for (int q = threadIdx.x; q < n; q += blockDim.x) diagg[q] = G[(size_t)q + (size_t)q * n];
-    __syncthreads();
-    int ik = 0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        diagg[q] = G[(size_t)q + (size_t)q * n];
+        cfs[q] = cf[q];
+    }
+    for (int q = threadIdx.x; q < 2 * n; q += blockDim.x) us[q] = u[q];
+    int ik = 0, s0 = -1;
     double accm = 0.0;
     for (int s = 0; s < n; s++) {
-        const SiteDraw d = site_draw(c, cf[s], u[2 * s]);
+        if (s0 < 0 || s >= s0 + DSB) {
+            // stage G(:, s..s+DSB) and G(s..s+DSB, :); valid until the next flush changes G
+            s0 = s;
+            const int nb = min(DSB, n - s);
+            for (int e = threadIdx.x; e < nb * n; e += blockDim.x) {
+                int j = e / n, q = e % n;
+                gcol[j * n + q] = G[(size_t)q + (size_t)(s + j) * n];
+            }
+            for (int e = threadIdx.x; e < nb * n; e += blockDim.x) {
+                int q = e / nb, j = e % nb;
+                grow[j * n + q] = G[(size_t)(s + j) + (size_t)q * n];
+            }
+        }
+        __syncthreads();
+        const SiteDraw d = site_draw(c, cfs[s], us[2 * s]);
         const cplx dg = diagg[s];
         const cplx ratio1 = cadd(cmul(make_double2(1.0 - dg.x, -dg.y), d.db), make_double2(1.0, 0.0));
         cplx rt;
-        const bool acc = site_accept(c, ratio1, d, u[2 * s + 1], phase, &rt);
+        const bool acc = site_accept(c, ratio1, d, us[2 * s + 1], phase, &rt);
         if (acc) {
             accm += 1.0;
             phase = phase_after(phase, rt);
@@ -1428,14 +1487,14 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
             if (r < n) {
                 cplx v;
                 if (isA) {
-                    v = G[(size_t)r + (size_t)s * n];
+                    v = gcol[(s - s0) * n + r];
 #pragma unroll
                     for (int m = 0; m < K; m++)
                         if (m < ik) v = cadd(v, cmul(rowb[m], reg[m]));
                     v = cmul(v, sscl);
                     anew[r] = v;
                 } else {
-                    v = G[(size_t)s + (size_t)r * n];
+                    v = grow[(s - s0) * n + r];
 #pragma unroll
                     for (int m = 0; m < K; m++)
                         if (m < ik) v = cadd(v, cmul(rowa[m], reg[m]));
@@ -1448,11 +1507,11 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
             }
             __syncthreads();
             if (isA && r < n) diagg[r] = cadd(diagg[r], cmul(anew[r], bnew[r]));
-            if (threadIdx.x == 0) cf[s] = d.isp;
+            if (threadIdx.x == 0) { cf[s] = d.isp; cfs[s] = d.isp; }
             ik++;
-            __syncthreads();
         }
         if (ik == kb || s == n - 1) {
+            __syncthreads();
             if (ik > 0) {
                 if (r < n) {
                     cplx *dst = isA ? c.avec : c.bvec;
@@ -1465,9 +1524,9 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
             }
             __syncthreads();
             ik = 0;
+            s0 = -1;
             if (s < n - 1)
                 for (int q = threadIdx.x; q < n; q += blockDim.x) diagg[q] = G[(size_t)q + (size_t)q * n];
-            __syncthreads();
         }
     }
     if (threadIdx.x == 0) {

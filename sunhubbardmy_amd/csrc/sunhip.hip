@@ -354,6 +354,8 @@ static int delay_reg_depth(sh_ctx *ctx, int *threads)
     int rows = (ctx->nmax + 63) / 64 * 64;
     int nt = 2 * rows;
     if (kt == 0 || nt > (kt >= 16 ? 512 : 1024)) return 0;
+    // LDS: diag, two exchange vectors, DSB staged columns and rows, uniforms and HS field
+    if ((size_t)(3 + 2 * DSB) * ctx->nmax * sizeof(cplx) + (size_t)ctx->nmax * 20 > 150 * 1024) return 0;
     *threads = nt;
     return kt;
 }
@@ -364,7 +366,7 @@ static void update_slice(sh_ctx *ctx, int nt, int slice_in_pass)
     int nthr = 0;
     int kt = (ctx->update == SH_U_DELAY && !ctx->legacy_update) ? delay_reg_depth(ctx, &nthr) : 0;
     if (kt) {
-        size_t lds = (size_t)(3 * ctx->nmax + 2 * kt) * sizeof(cplx);
+        size_t lds = (size_t)(3 * ctx->nmax + 2 * DSB * ctx->nmax + 2 * kt) * sizeof(cplx) + (size_t)ctx->nmax * 20;
         switch (kt) {
         case 4: hipLaunchKernelGGL(k_update_delay_reg<4>, dim3(ctx->nch), dim3(nthr), lds, ctx->st, ctx->dc, nt, slice_in_pass); break;
         case 8: hipLaunchKernelGGL(k_update_delay_reg<8>, dim3(ctx->nch), dim3(nthr), lds, ctx->st, ctx->dc, nt, slice_in_pass); break;
@@ -385,6 +387,8 @@ static void update_slice(sh_ctx *ctx, int nt, int slice_in_pass)
 static void measure(sh_ctx *ctx)
 {
     Tic t(ctx, T_MEASURE);
+    unsigned nb = (ctx->nmax + 31) / 32;
+    hipLaunchKernelGGL(k_measure_tiles, dim3(nb * (nb + 1) / 2, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dc);
     hipLaunchKernelGGL(k_measure, dim3(ctx->nch), dim3(256), 0, ctx->st, ctx->dc);
 }
 
@@ -446,12 +450,16 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
     {
         struct { const void *f; size_t bytes; } big[] = {
             {(const void *)k_cb_rows, lds_rows(ctx)}, {(const void *)k_cb_cols, lds_cols(ctx)},
+            {(const void *)k_update_delay_reg<4>, (size_t)(3 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 8 * sizeof(cplx)},
+            {(const void *)k_update_delay_reg<8>, (size_t)(3 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 16 * sizeof(cplx)},
+            {(const void *)k_update_delay_reg<16>, (size_t)(3 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 32 * sizeof(cplx)},
+            {(const void *)k_update_delay_reg<32>, (size_t)(3 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 64 * sizeof(cplx)},
             {(const void *)k_qr_blk, blocked(ctx) ? lds_blk(ctx) : 0},
             {(const void *)k_formq_blk, blocked(ctx) ? lds_blk(ctx) : 0},
             {(const void *)k_lu_blk, blocked(ctx) ? lds_blk(ctx) : 0},
             {(const void *)k_solve_blk, blocked(ctx) ? lds_blk(ctx) : 0}};
         for (auto &b : big) {
-            if (b.bytes <= 64 * 1024) continue;
+            if (b.bytes <= 64 * 1024 || b.bytes > 150 * 1024) continue;
             if (hipFuncSetAttribute(b.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.bytes) != hipSuccess) {
                 (void)hipGetLastError();
                 int rc = fail(ctx, SH_E_HIP, "dynamic LDS request above the device limit");
