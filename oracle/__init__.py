@@ -53,6 +53,30 @@ def lib():
         L.orc_chain_seed.restype = ctypes.c_int32
         L.orc_qr.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, ip, dp, ctypes.c_int]
         L.orc_inv_logdet.argtypes = [ctypes.c_int, dp, dp]
+        L.orc_pcreate.restype = P
+        L.orc_pcreate.argtypes = [ctypes.POINTER(OrcPParams)]
+        for name in ("orc_pdestroy", "orc_pinit_conf", "orc_preset_diag", "orc_preset_obs"):
+            getattr(L, name).argtypes = [P]
+            getattr(L, name).restype = None
+        L.orc_psizes.argtypes = [P, ip]
+        L.orc_pget_conf.argtypes = [P, ip]
+        L.orc_pset_conf.argtypes = [P, ip]
+        L.orc_pset_trial.argtypes = [P]
+        L.orc_pset_trial.restype = ctypes.c_double
+        L.orc_pset_proj.argtypes = [P, dp]
+        L.orc_pget_proj.argtypes = [P, dp, dp]
+        L.orc_psweep_start.argtypes = [P]
+        L.orc_psweep_b0.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.orc_psweep_0b.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.orc_pget_g.argtypes = [P, dp]
+        L.orc_pget_state.argtypes = [P, dp, dp, dp]
+        L.orc_pget_diag.argtypes = [P, dp]
+        L.orc_pget_obs.argtypes = [P, dp]
+        L.orc_pset_trace.argtypes = [P, dp, ctypes.c_int]
+        L.orc_ptrace_count.argtypes = [P]
+        L.orc_pget_table.argtypes = [P, ctypes.c_int, dp]
+        L.orc_qr_pqmc.argtypes = [ctypes.c_int, ctypes.c_int, dp]
+        L.orc_qr_pqmc.restype = ctypes.c_double
         _LIB = L
     return _LIB
 
@@ -65,6 +89,17 @@ class OrcParams(ctypes.Structure):
         ("nflr", ctypes.c_int), ("exact_hs", ctypes.c_int), ("kmode", ctypes.c_int),
         ("update", ctypes.c_int), ("nwrap", ctypes.c_int), ("nublock", ctypes.c_int),
         ("seed", ctypes.c_int32),
+    ]
+
+
+class OrcPParams(ctypes.Structure):
+    _fields_ = [
+        ("lattice", ctypes.c_int), ("la", ctypes.c_int), ("lb", ctypes.c_int),
+        ("beta", ctypes.c_double), ("dtau", ctypes.c_double), ("rt", ctypes.c_double),
+        ("mu", ctypes.c_double), ("rv", ctypes.c_double), ("nu", ctypes.c_double),
+        ("rndness", ctypes.c_double), ("nflr", ctypes.c_int), ("kmode", ctypes.c_int),
+        ("nwrap", ctypes.c_int), ("nustock", ctypes.c_int), ("mid_len", ctypes.c_int),
+        ("wrap_check", ctypes.c_int), ("seed", ctypes.c_int32),
     ]
 
 
@@ -214,3 +249,130 @@ def inv_logdet(a):
     ld = np.zeros(2)
     info = lib().orc_inv_logdet(a.shape[0], _dptr(a), _dptr(ld))
     return a, complex(ld[0], ld[1]), info
+
+
+class OracleProjChain:
+    """One projective (t-V, submatrix-LR) chain of the CPU restatement, driven like the
+    reference's dqmc_proj_ctrl: init_conf, set_trial (or set_proj), sweep_start, sweeps."""
+
+    def __init__(self, lattice="honeycomb", la=2, lb=2, beta=2.0, dtau=0.1, rt=1.0, mu=0.0,
+                 rv=1.2, nu=0.0, rndness=1e-4, nflr=1, kmode="dense", nwrap=10, nustock=8,
+                 mid_len=1, wrap_check=True, seed=None):
+        L = lib()
+        if seed is None:
+            seed = L.orc_chain_seed(0)
+        self.params = OrcPParams(LATTICES[lattice], la, lb, beta, dtau, rt, mu, rv, nu, rndness, nflr,
+                                 KMODES[kmode], nwrap, nustock, mid_len, int(bool(wrap_check)),
+                                 int(np.int32(seed)))
+        self._c = L.orc_pcreate(ctypes.byref(self.params))
+        s = np.zeros(8, np.int32)
+        L.orc_psizes(self._c, _iptr(s))
+        (self.ndim, self.ne, self.ltrot, self.nst, self.nn_nf, self.nn_lf, self.nustock,
+         self.lq) = [int(v) for v in s]
+        self._trace = None
+
+    def __del__(self):
+        if getattr(self, "_c", None):
+            lib().orc_pdestroy(self._c)
+            self._c = None
+
+    def init_conf(self):
+        lib().orc_pinit_conf(self._c)
+
+    @property
+    def conf(self):
+        c = np.zeros((self.ltrot, self.nn_nf, self.ndim), np.int32)
+        lib().orc_pget_conf(self._c, _iptr(c))
+        return c
+
+    @conf.setter
+    def conf(self, value):
+        c = np.ascontiguousarray(value, dtype=np.int32).reshape(self.ltrot, self.nn_nf, self.ndim)
+        lib().orc_pset_conf(self._c, _iptr(c))
+
+    def set_trial(self):
+        return float(lib().orc_pset_trial(self._c))
+
+    @property
+    def proj(self):
+        p = np.zeros((self.ndim, self.ne), np.complex128, order="F")
+        w = np.zeros(self.ndim)
+        lib().orc_pget_proj(self._c, _dptr(p), _dptr(w))
+        return p, w
+
+    def set_proj(self, p):
+        p = np.asfortranarray(p, dtype=np.complex128)
+        assert p.shape == (self.ndim, self.ne)
+        lib().orc_pset_proj(self._c, _dptr(p))
+
+    def _check(self, rc, what):
+        if rc:
+            raise RuntimeError("oracle %s failed: %d" % (what, rc))
+
+    def sweep_start(self):
+        self._check(lib().orc_psweep_start(self._c), "proj sweep_start")
+
+    def sweep_b0(self, lupdate=True, lmeasure=True):
+        self._check(lib().orc_psweep_b0(self._c, int(lupdate), int(lmeasure)), "proj sweep_b0")
+
+    def sweep_0b(self, lupdate=True, lmeasure=True):
+        self._check(lib().orc_psweep_0b(self._c, int(lupdate), int(lmeasure)), "proj sweep_0b")
+
+    def sweep(self, lupdate=True, lmeasure=True):
+        self.sweep_b0(lupdate, lmeasure)
+        self.sweep_0b(lupdate, lmeasure)
+
+    @property
+    def G(self):
+        g = np.zeros((self.ndim, self.ndim), np.complex128, order="F")
+        lib().orc_pget_g(self._c, _dptr(g))
+        return g
+
+    def state(self):
+        ul = np.zeros((self.ne, self.ndim), np.complex128, order="F")
+        ur = np.zeros((self.ndim, self.ne), np.complex128, order="F")
+        inv = np.zeros((self.ne, self.ne), np.complex128, order="F")
+        lib().orc_pget_state(self._c, _dptr(ul), _dptr(ur), _dptr(inv))
+        return ul, ur, inv
+
+    def diag(self):
+        d = np.zeros(10)
+        lib().orc_pget_diag(self._c, _dptr(d))
+        return {"max_wrap_error": d[0], "max_phasediff": d[1], "phase": complex(d[2], d[3]),
+                "logweightf": complex(d[4], d[5]), "accepted": d[6], "tried": d[7],
+                "nchecks": d[8], "sumlog10err": d[9]}
+
+    def reset_diag(self):
+        lib().orc_preset_diag(self._c)
+
+    def obs(self):
+        o = np.zeros(11)
+        lib().orc_pget_obs(self._c, _dptr(o))
+        z = o[1::2] + 1j * o[2::2]
+        return {"nobs": o[0], "sign": z[0], "density": z[1], "ekin": z[2], "dblocc": z[3],
+                "s_stag": z[4]}
+
+    def reset_obs(self):
+        lib().orc_preset_obs(self._c)
+
+    def set_trace(self, cap):
+        self._trace = np.zeros((cap, self.ndim, self.ndim), np.complex128)
+        lib().orc_pset_trace(self._c, _dptr(self._trace), cap)
+
+    def trace(self):
+        n = lib().orc_ptrace_count(self._c)
+        return np.array([t.reshape(self.ndim, self.ndim).T for t in self._trace[:n]])
+
+    def table(self, kind):
+        buf = np.zeros(2 * max(self.ndim * self.ndim, 4 * self.nn_nf * self.nn_lf, 64) + 64)
+        n = lib().orc_pget_table(self._c, kind, _dptr(buf))
+        if kind in (3, 6, 7, 8):
+            return buf[:n].copy()
+        return (buf[0:2 * n:2] + 1j * buf[1:2 * n:2]).copy()
+
+
+def qr_pqmc(a):
+    """s_zgeQR_pqmc: returns (Q, log|det R|) for a tall or wide complex matrix."""
+    a = np.array(a, dtype=np.complex128, order="F")
+    rl = lib().orc_qr_pqmc(a.shape[0], a.shape[1], _dptr(a))
+    return a, float(rl)

@@ -221,7 +221,7 @@ static void apply_hh_left(int m, int n, const zc *v, zc tauc, zc *A, int lda)
 static void zgeqr_core(int m, int n, zc *A, int lda, zc *tau, int32_t *jpvt, int pivot)
 {
     double *vn1 = NULL, *vn2 = NULL;
-    for (int j = 0; j < n; j++) jpvt[j] = j + 1;
+    if (jpvt) for (int j = 0; j < n; j++) jpvt[j] = j + 1;
     if (pivot) {
         vn1 = (double *)malloc(sizeof(double) * n);
         vn2 = (double *)malloc(sizeof(double) * n);
@@ -1725,33 +1725,40 @@ static void pq_bmat_left_forward(orc_pctx *c, int nt1, int nt2, zc *M)
  * Q formed by zungqr, log|det R| from the real diagonal, and Q's first column negated when an
  * odd number of R's diagonal entries is negative.  trans = 1: LQ of an ne x N matrix done as
  * QR of its plain transpose.  a (tall: N x ne, or wide: ne x N) is replaced by Q. */
-static double pq_qr(orc_pctx *c, zc *a, int trans)
+static double zgeqr_pqmc(int rdim, int cdim, zc *a)
 {
-    int N = c->N, ne = c->ne;
-    zc *w = zalloc((size_t)N * ne), *tau = zalloc(ne);
-    if (!trans) memcpy(w, a, sizeof(zc) * (size_t)N * ne);
+    int trans = rdim < cdim;
+    int m = trans ? cdim : rdim, k = trans ? rdim : cdim;
+    zc *w = zalloc((size_t)m * k), *tau = zalloc(k);
+    if (!trans) memcpy(w, a, sizeof(zc) * (size_t)m * k);
     else
-        for (int j = 0; j < N; j++)
-            for (int i = 0; i < ne; i++) w[IDX(j, i, N)] = a[IDX(i, j, ne)];
-    zgeqr_core(N, ne, w, N, tau, NULL, 0);
+        for (int j = 0; j < m; j++)
+            for (int i = 0; i < k; i++) w[IDX(j, i, m)] = a[IDX(i, j, k)];
+    zgeqr_core(m, k, w, m, tau, NULL, 0);
     int isgn = 0;
     double rl = 0;
-    for (int i = 0; i < ne; i++) {
-        double r = creal(w[IDX(i, i, N)]);
+    for (int i = 0; i < k; i++) {
+        double r = creal(w[IDX(i, i, m)]);
         if (r < 0) { isgn++; rl += log(-r); } else rl += log(r);
     }
-    for (int j = 0; j < ne; j++)
-        for (int i = 0; i <= j; i++) w[IDX(i, j, N)] = (i == j) ? 1.0 : 0.0;
-    zungqr(N, ne, w, N, tau);
+    for (int j = 0; j < k; j++)
+        for (int i = 0; i <= j; i++) w[IDX(i, j, m)] = (i == j) ? 1.0 : 0.0;
+    zungqr(m, k, w, m, tau);
     if (isgn % 2 == 1)
-        for (int i = 0; i < N; i++) w[IDX(i, 0, N)] = -w[IDX(i, 0, N)];
-    if (!trans) memcpy(a, w, sizeof(zc) * (size_t)N * ne);
+        for (int i = 0; i < m; i++) w[IDX(i, 0, m)] = -w[IDX(i, 0, m)];
+    if (!trans) memcpy(a, w, sizeof(zc) * (size_t)m * k);
     else
-        for (int j = 0; j < N; j++)
-            for (int i = 0; i < ne; i++) a[IDX(i, j, ne)] = w[IDX(j, i, N)];
+        for (int j = 0; j < m; j++)
+            for (int i = 0; i < k; i++) a[IDX(i, j, k)] = w[IDX(j, i, m)];
     free(w); free(tau);
     return rl;
 }
+static double pq_qr(orc_pctx *c, zc *a, int trans)
+{
+    return trans ? zgeqr_pqmc(c->ne, c->N, a) : zgeqr_pqmc(c->N, c->ne, a);
+}
+/* exported for the invariant tests: a (rdim x cdim) is replaced by Q, returns log|det R| */
+double orc_qr_pqmc(int rdim, int cdim, double *a) { return zgeqr_pqmc(rdim, cdim, (zc *)a); }
 
 /* ULR = UL UR, ULRINV = ULR^-1 (s_inv_logdet_lu_z, or s_invlu_z when logdet is not wanted) */
 static int pq_ulrinv(orc_pctx *c, const zc *ul, const zc *ur, zc *ulrinv, zc *logdet)

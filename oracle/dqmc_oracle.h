@@ -114,6 +114,8 @@ void orc_preset_obs(orc_pctx *c);
 void orc_pset_trace(orc_pctx *c, double *buf, int cap);
 int orc_ptrace_count(const orc_pctx *c);
 int orc_pget_table(const orc_pctx *c, int kind, double *out);
+/* s_zgeQR_pqmc: a (rdim x cdim) -> Q in place (LQ by plain transpose when rdim < cdim); returns log|det R| */
+double orc_qr_pqmc(int rdim, int cdim, double *a);
 
 #ifdef __cplusplus
 }

@@ -33,6 +33,14 @@ EXPORTS = [
     "sh_get_trace", "sh_timers", "sh_synchronize",
 ]
 
+# include/sunhip_proj.h
+EXPORTS_PROJ = [
+    "shp_create", "shp_destroy", "shp_last_error", "shp_init_conf", "shp_set_conf", "shp_get_conf",
+    "shp_draw_uniforms", "shp_set_proj", "shp_sweep_start_0b", "shp_sweep_b0", "shp_sweep_0b", "shp_get_g",
+    "shp_get_state", "shp_get_diag", "shp_reset_diag", "shp_get_obs", "shp_reset_obs", "shp_set_trace",
+    "shp_get_trace", "shp_timers", "shp_synchronize",
+]
+
 
 def lib():
     global _LIB
@@ -66,6 +74,31 @@ def lib():
     L.sh_synchronize.argtypes = [P]
     for name in EXPORTS:
         if name != "sh_last_error":
+            getattr(L, name).restype = ctypes.c_int
+    L.shp_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, P, ctypes.c_int]
+    L.shp_destroy.argtypes = [P]
+    L.shp_last_error.argtypes = [P]
+    L.shp_last_error.restype = ctypes.c_char_p
+    L.shp_init_conf.argtypes = [P]
+    L.shp_set_conf.argtypes = [P, ctypes.c_int, ip]
+    L.shp_get_conf.argtypes = [P, ctypes.c_int, ip]
+    L.shp_draw_uniforms.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
+    L.shp_set_proj.argtypes = [P, ctypes.c_int, dp]
+    L.shp_sweep_start_0b.argtypes = [P]
+    L.shp_sweep_b0.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.shp_sweep_0b.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.shp_get_g.argtypes = [P, ctypes.c_int, dp]
+    L.shp_get_state.argtypes = [P, ctypes.c_int, dp, dp, dp]
+    L.shp_get_diag.argtypes = [P, ctypes.c_int, dp]
+    L.shp_reset_diag.argtypes = [P]
+    L.shp_get_obs.argtypes = [P, ctypes.c_int, dp]
+    L.shp_reset_obs.argtypes = [P]
+    L.shp_set_trace.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.shp_get_trace.argtypes = [P, ctypes.c_int, dp, ip]
+    L.shp_timers.argtypes = [P, ctypes.c_int, dp]
+    L.shp_synchronize.argtypes = [P]
+    for name in EXPORTS_PROJ:
+        if name != "shp_last_error":
             getattr(L, name).restype = ctypes.c_int
     _LIB = L
     return L

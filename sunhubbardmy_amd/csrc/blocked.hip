@@ -1,14 +1,16 @@
-// blocked.hip -- blocked factorizations of one chain's N x N matrix inside one workgroup:
-// 16-wide panels are factored in LDS, and every trailing update is a rank-16 product on FP64
-// MFMA (v_mfma_f64_16x16x4_f64), so each panel streams the trailing matrix once instead of
-// once per column.  Used for N <= BLK_NMAX; larger chains keep the column-at-a-time kernels.
-//   k_qr_blk     Householder QR without pivoting (zgeqrf + zlarft), T factors kept per panel
-//   k_formq_blk  Q = H_1 ... H_N from the reflectors and the T factors (zungqr)
-//   k_lu_blk     LU with partial pivoting, pivot = largest |Re|+|Im| (zgetrf)
-//   k_solve_blk  (P L U) Y = X for N right-hand sides
+// blocked.hip -- blocked factorizations of one chain's matrix inside one workgroup:
+// panels of up to 16 columns are factored in LDS, and every trailing update is a rank-16
+// product on FP64 MFMA (v_mfma_f64_16x16x4_f64), so each panel streams the trailing matrix
+// once instead of once per column.  The bodies take explicit sizes and leading dimensions so
+// that the square N x N matrices of the finite-temperature sweep and the tall N x ne
+// matrices of the projective sweep share them.
+//   qr_blk_body     Householder QR without pivoting (zgeqrf + zlarft), T factors kept per panel
+//   formq_blk_body  Q = H_1 ... H_n from the reflectors and the T factors (zungqr)
+//   lu_blk_body     LU with partial pivoting, pivot = largest |Re|+|Im| (zgetrf)
+//   solve_blk_body  (P L U) Y = X for nrhs right-hand sides
 #include "sunhip_kernels.h"
 
-#define PB 16          // panel width
+#define PB 16          // largest panel width (T factors are stored PB x PB)
 #define BLK_NT 512     // threads per workgroup (8 waves)
 #define BLK_NMAX 512   // LDS holds a 512 x 16 complex panel (128 KiB)
 
@@ -150,34 +152,32 @@ __device__ static void wg_rank16(cplx *C, int ldc, int mrows, int ncols, const c
 }
 
 // ------------------------------------------------------------------------------------------
-// QR without pivoting, blocked
+// QR without pivoting, blocked: A (M x Nc, M >= Nc, lda) -> reflectors below the diagonal and
+// R on and above it; tau[Nc]; the panel's T factor to tblk[panel]; pb = panel width <= PB;
+// lds holds M x pb panel + 2 PB x PB; wk holds PB x Nc.
 // ------------------------------------------------------------------------------------------
-extern "C" __global__ void __launch_bounds__(BLK_NT)
-k_qr_blk(const DChain *chains, int sel, int out_pt)
+__device__ static void qr_blk_body(cplx *A, int M, int Nc, int lda, cplx *tau_out, cplx *tblk, cplx *wk, int pb,
+                                   cplx *lds)
 {
-    extern __shared__ cplx bsh[];
-    const DChain &c = chains[blockIdx.x];
-    const int n = c.n;
-    cplx *P = bsh;                      // n x PB panel, ld n
-    cplx *T = bsh + (size_t)n * PB;     // PB x PB
+    cplx *P = lds;                      // M x pb panel, ld M
+    cplx *T = lds + (size_t)M * pb;     // PB x PB
     cplx *S = T + PB * PB;              // PB x PB Gram matrix
     __shared__ double red[16];
     __shared__ cplx s_par[4];
-    cplx *A = chain_mat(c, sel, 0);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int k0 = 0; k0 < n; k0 += PB) {
-        const int m = n - k0, b = min(PB, m);
+    for (int k0 = 0; k0 < Nc; k0 += pb) {
+        const int m = M - k0, b = min(pb, Nc - k0);
         for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
             int r = e % m, p = e / m;
-            P[p * n + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * n];
+            P[p * M + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * lda];
         }
         __syncthreads();
         for (int j = 0; j < b; j++) {
             double s = 0.0;
-            for (int r = j + 1 + threadIdx.x; r < m; r += blockDim.x) s += cabs2(P[j * n + r]);
+            for (int r = j + 1 + threadIdx.x; r < m; r += blockDim.x) s += cabs2(P[j * M + r]);
             s = blk_sum(s, red);
             if (threadIdx.x == 0) {
-                cplx alpha = P[j * n + j];
+                cplx alpha = P[j * M + j];
                 cplx tau = make_double2(0.0, 0.0), scal = make_double2(1.0, 0.0);
                 double beta = alpha.x;
                 if (!(s == 0.0 && alpha.y == 0.0)) {
@@ -185,22 +185,22 @@ k_qr_blk(const DChain *chains, int sel, int out_pt)
                     beta = alpha.x >= 0 ? -nrm : nrm;
                     tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
                     scal = cinv(make_double2(alpha.x - beta, alpha.y));
-                    P[j * n + j] = make_double2(beta, 0.0);
+                    P[j * M + j] = make_double2(beta, 0.0);
                 }
                 s_par[0] = tau;
                 s_par[1] = scal;
-                c.tau[k0 + j] = tau;
+                tau_out[k0 + j] = tau;
             }
             __syncthreads();
             const cplx tau = s_par[0], scal = s_par[1];
-            for (int r = j + 1 + threadIdx.x; r < m; r += blockDim.x) P[j * n + r] = cmul(P[j * n + r], scal);
+            for (int r = j + 1 + threadIdx.x; r < m; r += blockDim.x) P[j * M + r] = cmul(P[j * M + r], scal);
             __syncthreads();
             // H_j^H on the remaining panel columns, one wave per column
             const cplx tc = cconj(tau);
             for (int q = j + 1 + w; q < b; q += nw) {
                 double sr = 0.0, si = 0.0;
                 for (int r = j + lane; r < m; r += 64) {
-                    cplx v = (r == j) ? make_double2(1.0, 0.0) : P[j * n + r], a = P[q * n + r];
+                    cplx v = (r == j) ? make_double2(1.0, 0.0) : P[j * M + r], a = P[q * M + r];
                     sr += v.x * a.x + v.y * a.y;
                     si += v.x * a.y - v.y * a.x;
                 }
@@ -208,8 +208,8 @@ k_qr_blk(const DChain *chains, int sel, int out_pt)
                 si = wave_sum(si);
                 cplx f = cmul(tc, make_double2(sr, si));
                 for (int r = j + lane; r < m; r += 64) {
-                    cplx v = (r == j) ? make_double2(1.0, 0.0) : P[j * n + r];
-                    P[q * n + r] = csub(P[q * n + r], cmul(v, f));
+                    cplx v = (r == j) ? make_double2(1.0, 0.0) : P[j * M + r];
+                    P[q * M + r] = csub(P[q * M + r], cmul(v, f));
                 }
             }
             __syncthreads();
@@ -220,7 +220,7 @@ k_qr_blk(const DChain *chains, int sel, int out_pt)
             if (i >= j) continue;
             double sr = 0.0, si = 0.0;
             for (int r = j + lane; r < m; r += 64) {
-                cplx vi = vpanel(P, n, r, i), vj = vpanel(P, n, r, j);
+                cplx vi = vpanel(P, M, r, i), vj = vpanel(P, M, r, j);
                 sr += vi.x * vj.x + vi.y * vj.y;
                 si += vi.x * vj.y - vi.y * vj.x;
             }
@@ -233,7 +233,7 @@ k_qr_blk(const DChain *chains, int sel, int out_pt)
             for (int j = 0; j < PB; j++)
                 for (int i = 0; i < PB; i++) T[j * PB + i] = make_double2(0.0, 0.0);
             for (int j = 0; j < b; j++) {
-                cplx tj = c.tau[k0 + j];
+                cplx tj = tau_out[k0 + j];
                 T[j * PB + j] = tj;
                 for (int i = 0; i < j; i++) {
                     cplx acc = make_double2(0.0, 0.0);
@@ -243,91 +243,73 @@ k_qr_blk(const DChain *chains, int sel, int out_pt)
             }
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < PB * PB; e += blockDim.x) c.tblk[(size_t)(k0 / PB) * PB * PB + e] = T[e];
+        for (int e = threadIdx.x; e < PB * PB; e += blockDim.x) tblk[(size_t)(k0 / pb) * PB * PB + e] = T[e];
         for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
             int r = e % m, p = e / m;
-            A[(size_t)(k0 + r) + (size_t)(k0 + p) * n] = P[p * n + r];
+            A[(size_t)(k0 + r) + (size_t)(k0 + p) * lda] = P[p * M + r];
         }
-        const int nc = n - k0 - b;
+        const int nc = Nc - k0 - b;
         if (nc > 0) {
-            cplx *Ct = A + (size_t)k0 + (size_t)(k0 + b) * n;
-            wg_vhc(c.wk, P, n, m, b, Ct, n, nc);
+            cplx *Ct = A + (size_t)k0 + (size_t)(k0 + b) * lda;
+            wg_vhc(wk, P, M, m, b, Ct, lda, nc);
             __syncthreads();
-            wg_tmul(c.wk, nc, T, b, true);
+            wg_tmul(wk, nc, T, b, true);
             __syncthreads();
-            wg_rank16(Ct, n, m, nc, P, n, 0, b, true, c.wk, PB);
+            wg_rank16(Ct, lda, m, nc, P, M, 0, b, true, wk, PB);
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        cplx s = make_double2(0.0, 0.0);
-        for (int i = 0; i < n; i++) {
-            cplx t = c.tau[i];
-            if (t.x != 0.0 || t.y != 0.0) {
-                cplx q = cdiv(cscale(t, 2.0 * t.x), cmul(t, cconj(t)));
-                s = cadd(s, clog_(make_double2(1.0 - q.x, -q.y)));
-            }
-        }
-        c.ldq[out_pt] = s;
-    }
 }
 
-// Q = H_1 ... H_n into U[out_pt]: backward accumulation by panels, Q(k0:, k0:) <- (I - V T V^H) Q(k0:, k0:)
-extern "C" __global__ void __launch_bounds__(BLK_NT)
-k_formq_blk(const DChain *chains, int sel, int out_pt)
+// Q (M x Nc, ldq) = H_1 ... H_Nc applied to the first Nc columns of the identity: backward
+// accumulation by panels, Q(k0:, k0:) <- (I - V T V^H) Q(k0:, k0:)
+__device__ static void formq_blk_body(const cplx *A, int M, int Nc, int lda, cplx *Q, int ldq, const cplx *tblk,
+                                      cplx *wk, int pb, cplx *lds)
 {
-    extern __shared__ cplx bsh[];
-    const DChain &c = chains[blockIdx.x];
-    const int n = c.n;
-    cplx *P = bsh;
-    cplx *T = bsh + (size_t)n * PB;
-    const cplx *A = chain_mat(c, sel, 0);
-    cplx *Q = c.U + (size_t)n * n * out_pt;
-    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
-        int i = (int)(e % n), j = (int)(e / n);
-        Q[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    cplx *P = lds;
+    cplx *T = lds + (size_t)M * pb;
+    for (size_t e = threadIdx.x; e < (size_t)M * Nc; e += blockDim.x) {
+        int i = (int)(e % M), j = (int)(e / M);
+        Q[(size_t)i + (size_t)j * ldq] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
-    const int np = (n + PB - 1) / PB;
+    const int np = (Nc + pb - 1) / pb;
     for (int pi = np - 1; pi >= 0; pi--) {
-        const int k0 = pi * PB, m = n - k0, b = min(PB, m);
+        const int k0 = pi * pb, m = M - k0, b = min(pb, Nc - k0), nc = Nc - k0;
         for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
             int r = e % m, p = e / m;
-            P[p * n + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * n];
+            P[p * M + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * lda];
         }
-        for (int e = threadIdx.x; e < PB * PB; e += blockDim.x) T[e] = c.tblk[(size_t)pi * PB * PB + e];
+        for (int e = threadIdx.x; e < PB * PB; e += blockDim.x) T[e] = tblk[(size_t)pi * PB * PB + e];
         __syncthreads();
-        cplx *Ct = Q + (size_t)k0 + (size_t)k0 * n;
-        wg_vhc(c.wk, P, n, m, b, Ct, n, m);
+        cplx *Ct = Q + (size_t)k0 + (size_t)k0 * ldq;
+        wg_vhc(wk, P, M, m, b, Ct, ldq, nc);
         __syncthreads();
-        wg_tmul(c.wk, m, T, b, false);
+        wg_tmul(wk, nc, T, b, false);
         __syncthreads();
-        wg_rank16(Ct, n, m, m, P, n, 0, b, true, c.wk, PB);
+        wg_rank16(Ct, ldq, m, nc, P, M, 0, b, true, wk, PB);
         __syncthreads();
     }
 }
 
 // ------------------------------------------------------------------------------------------
-// LU with partial pivoting, blocked (right-looking); log det as s_inv_logdet_lu_z
+// LU with partial pivoting, blocked (right-looking), in place on A (n x n, lda); piv[k] = row
+// interchanged with k (0-based); log det as s_inv_logdet_lu_z (Im folded into (-pi, pi]);
+// thread 0 writes *ldet and *info (first zero pivot + 1, or 0)
 // ------------------------------------------------------------------------------------------
-extern "C" __global__ void __launch_bounds__(BLK_NT)
-k_lu_blk(const DChain *chains)
+__device__ static void lu_blk_body(cplx *A, int n, int lda, int32_t *piv, cplx *ldet, double *info, cplx *lds)
 {
-    extern __shared__ cplx bsh[];
-    const DChain &c = chains[blockIdx.x];
-    const int n = c.n;
-    cplx *P = bsh;                       // panel, ld n
+    cplx *P = lds;                       // panel, ld n
     __shared__ double rv[16];
     __shared__ int ri[16];
     __shared__ int s_info;
     __shared__ int s_pv[PB];
-    cplx *A = c.W3;
     if (threadIdx.x == 0) s_info = 0;
     for (int k0 = 0; k0 < n; k0 += PB) {
         const int m = n - k0, b = min(PB, m);
         for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
             int r = e % m, p = e / m;
-            P[p * n + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * n];
+            P[p * n + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * lda];
         }
         __syncthreads();
         for (int j = 0; j < b; j++) {
@@ -340,7 +322,7 @@ k_lu_blk(const DChain *chains)
             }
             int p;
             blk_argmax(bv, bi, rv, ri, &p);
-            if (threadIdx.x == 0) { s_pv[j] = p; c.piv[k0 + j] = k0 + p; }
+            if (threadIdx.x == 0) { s_pv[j] = p; piv[k0 + j] = k0 + p; }
             if (p != j)
                 for (int q = threadIdx.x; q < b; q += blockDim.x) {
                     cplx t = P[q * n + p];
@@ -348,11 +330,11 @@ k_lu_blk(const DChain *chains)
                     P[q * n + j] = t;
                 }
             __syncthreads();
-            cplx piv = P[j * n + j];
-            bool zero = (piv.x == 0.0 && piv.y == 0.0);
+            cplx pv = P[j * n + j];
+            bool zero = (pv.x == 0.0 && pv.y == 0.0);
             if (zero && threadIdx.x == 0 && s_info == 0) s_info = k0 + j + 1;
             if (!zero) {
-                cplx pinv = cinv(piv);
+                cplx pinv = cinv(pv);
                 for (int r = j + 1 + threadIdx.x; r < m; r += blockDim.x) P[j * n + r] = cmul(P[j * n + r], pinv);
             }
             __syncthreads();
@@ -364,12 +346,12 @@ k_lu_blk(const DChain *chains)
         }
         for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
             int r = e % m, p = e / m;
-            A[(size_t)(k0 + r) + (size_t)(k0 + p) * n] = P[p * n + r];
+            A[(size_t)(k0 + r) + (size_t)(k0 + p) * lda] = P[p * n + r];
         }
         // the panel's interchanges on the columns left and right of it, in order
         for (int q = threadIdx.x; q < n; q += blockDim.x) {
             if (q >= k0 && q < k0 + b) continue;
-            cplx *col = A + (size_t)q * n;
+            cplx *col = A + (size_t)q * lda;
             for (int j = 0; j < b; j++) {
                 int p = s_pv[j];
                 if (p != j) {
@@ -384,7 +366,7 @@ k_lu_blk(const DChain *chains)
         if (nc > 0) {
             // U12 = L11^-1 A12 (unit lower 16 x 16 from the panel), one thread per column
             for (int q = threadIdx.x; q < nc; q += blockDim.x) {
-                cplx *col = A + (size_t)k0 + (size_t)(k0 + b + q) * n;
+                cplx *col = A + (size_t)k0 + (size_t)(k0 + b + q) * lda;
                 cplx x[PB];
 #pragma unroll
                 for (int i = 0; i < PB; i++) x[i] = (i < b) ? col[i] : make_double2(0.0, 0.0);
@@ -399,66 +381,63 @@ k_lu_blk(const DChain *chains)
             }
             __syncthreads();
             // A22 -= L21 U12
-            cplx *C = A + (size_t)(k0 + b) + (size_t)(k0 + b) * n;
-            wg_rank16(C, n, m - b, nc, P, n, b, b, false, A + (size_t)k0 + (size_t)(k0 + b) * n, n);
+            cplx *C = A + (size_t)(k0 + b) + (size_t)(k0 + b) * lda;
+            wg_rank16(C, lda, m - b, nc, P, n, b, b, false, A + (size_t)k0 + (size_t)(k0 + b) * lda, lda);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         cplx ld = make_double2(0.0, 0.0);
         for (int i = 0; i < n; i++) {
-            cplx d = A[(size_t)i + (size_t)i * n];
-            if (c.piv[i] != i) d = make_double2(-d.x, -d.y);
+            cplx d = A[(size_t)i + (size_t)i * lda];
+            if (piv[i] != i) d = make_double2(-d.x, -d.y);
             ld = cadd(ld, clog_(d));
         }
         double im = ld.y;
         double k2 = round(im / 2.0 / M_PI);
         ld.y = im - 2.0 * M_PI * k2;
-        c.sc[SC_ZLD] = ld.x;
-        c.sc[SC_ZLD + 1] = ld.y;
-        c.sc[SC_INFO] = (double)s_info;
+        *ldet = ld;
+        *info = (double)s_info;
     }
 }
 
-// (P L U) Y = X in place of X = matrix xsel (N right-hand sides): row interchanges, then
-// block forward and back substitution with rank-16 updates on MFMA
-extern "C" __global__ void __launch_bounds__(BLK_NT)
-k_solve_blk(const DChain *chains, int xsel)
+// (P L U) Y = X in place of X (n x nrhs, ldx), with the LU factors of lu_blk_body in A: row
+// interchanges through Y (n x nrhs work, ld n), then block forward and back substitution with
+// rank-16 updates on MFMA; lds holds n x PB + n ints
+__device__ static void solve_blk_body(const cplx *A, int n, int lda, const int32_t *piv, cplx *X, int ldx, int nrhs,
+                                      cplx *Y, cplx *lds)
 {
-    extern __shared__ cplx bsh[];
-    const DChain &c = chains[blockIdx.x];
-    const int n = c.n;
-    cplx *P = bsh;                            // 16 columns of L or U, ld n
-    int *perm = (int *)(bsh + (size_t)n * PB);
-    const cplx *A = c.W3;
-    cplx *X = chain_mat(c, xsel, 0);
-    cplx *Y = c.W2;
+    cplx *P = lds;                            // 16 columns of L or U, ld n
+    int *perm = (int *)(lds + (size_t)n * PB);
     // final row order of the interchanges
     if (threadIdx.x == 0) {
         for (int i = 0; i < n; i++) perm[i] = i;
         for (int k = 0; k < n; k++) {
-            int p = c.piv[k];
+            int p = piv[k];
             int t = perm[p]; perm[p] = perm[k]; perm[k] = t;
         }
     }
     __syncthreads();
-    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+    for (size_t e = threadIdx.x; e < (size_t)n * nrhs; e += blockDim.x) {
         int i = (int)(e % n), j = (int)(e / n);
-        Y[e] = X[(size_t)perm[i] + (size_t)j * n];
+        Y[e] = X[(size_t)perm[i] + (size_t)j * ldx];
     }
     __syncthreads();
-    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) X[e] = Y[e];
+    for (size_t e = threadIdx.x; e < (size_t)n * nrhs; e += blockDim.x) {
+        int i = (int)(e % n), j = (int)(e / n);
+        X[(size_t)i + (size_t)j * ldx] = Y[e];
+    }
     __syncthreads();
     // forward: unit lower L
     for (int k0 = 0; k0 < n; k0 += PB) {
         const int b = min(PB, n - k0), mb = n - k0 - b;
         for (int e = threadIdx.x; e < (n - k0) * b; e += blockDim.x) {
             int r = e % (n - k0), p = e / (n - k0);
-            P[p * n + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * n];
+            P[p * n + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * lda];
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < n; q += blockDim.x) {
-            cplx *col = X + (size_t)k0 + (size_t)q * n;
+        for (int q = threadIdx.x; q < nrhs; q += blockDim.x) {
+            cplx *col = X + (size_t)k0 + (size_t)q * ldx;
             cplx x[PB];
 #pragma unroll
             for (int i = 0; i < PB; i++) x[i] = (i < b) ? col[i] : make_double2(0.0, 0.0);
@@ -472,7 +451,7 @@ k_solve_blk(const DChain *chains, int xsel)
                 if (i < b) col[i] = x[i];
         }
         __syncthreads();
-        if (mb > 0) wg_rank16(X + (size_t)(k0 + b), n, mb, n, P, n, b, b, false, X + (size_t)k0, n);
+        if (mb > 0) wg_rank16(X + (size_t)(k0 + b), ldx, mb, nrhs, P, n, b, b, false, X + (size_t)k0, ldx);
         __syncthreads();
     }
     // backward: upper U
@@ -482,11 +461,11 @@ k_solve_blk(const DChain *chains, int xsel)
         // rows 0 .. k0+b of the 16 columns
         for (int e = threadIdx.x; e < (k0 + b) * b; e += blockDim.x) {
             int r = e % (k0 + b), p = e / (k0 + b);
-            P[p * n + r] = A[(size_t)r + (size_t)(k0 + p) * n];
+            P[p * n + r] = A[(size_t)r + (size_t)(k0 + p) * lda];
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < n; q += blockDim.x) {
-            cplx *col = X + (size_t)k0 + (size_t)q * n;
+        for (int q = threadIdx.x; q < nrhs; q += blockDim.x) {
+            cplx *col = X + (size_t)k0 + (size_t)q * ldx;
             cplx x[PB];
 #pragma unroll
             for (int i = 0; i < PB; i++) x[i] = (i < b) ? col[i] : make_double2(0.0, 0.0);
@@ -504,7 +483,62 @@ k_solve_blk(const DChain *chains, int xsel)
                 if (i < b) col[i] = x[i];
         }
         __syncthreads();
-        if (k0 > 0) wg_rank16(X, n, k0, n, P, n, 0, b, false, X + (size_t)k0, n);
+        if (k0 > 0) wg_rank16(X, ldx, k0, nrhs, P, n, 0, b, false, X + (size_t)k0, ldx);
         __syncthreads();
     }
+}
+
+// ------------------------------------------------------------------------------------------
+// The finite-temperature chain's N x N factorizations (one workgroup per chain)
+// ------------------------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(BLK_NT)
+k_qr_blk(const DChain *chains, int sel, int out_pt)
+{
+    extern __shared__ cplx bsh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    qr_blk_body(chain_mat(c, sel, 0), n, n, n, c.tau, c.tblk, c.wk, PB, bsh);
+    if (threadIdx.x == 0) {
+        cplx s = make_double2(0.0, 0.0);
+        for (int i = 0; i < n; i++) {
+            cplx t = c.tau[i];
+            if (t.x != 0.0 || t.y != 0.0) {
+                cplx q = cdiv(cscale(t, 2.0 * t.x), cmul(t, cconj(t)));
+                s = cadd(s, clog_(make_double2(1.0 - q.x, -q.y)));
+            }
+        }
+        c.ldq[out_pt] = s;
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(BLK_NT)
+k_formq_blk(const DChain *chains, int sel, int out_pt)
+{
+    extern __shared__ cplx bsh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    formq_blk_body(chain_mat(c, sel, 0), n, n, n, c.U + (size_t)n * n * out_pt, n, c.tblk, c.wk, PB, bsh);
+}
+
+extern "C" __global__ void __launch_bounds__(BLK_NT)
+k_lu_blk(const DChain *chains)
+{
+    extern __shared__ cplx bsh[];
+    const DChain &c = chains[blockIdx.x];
+    __shared__ cplx s_ld;
+    __shared__ double s_inf;
+    lu_blk_body(c.W3, c.n, c.n, c.piv, &s_ld, &s_inf, bsh);
+    if (threadIdx.x == 0) {
+        c.sc[SC_ZLD] = s_ld.x;
+        c.sc[SC_ZLD + 1] = s_ld.y;
+        c.sc[SC_INFO] = s_inf;
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(BLK_NT)
+k_solve_blk(const DChain *chains, int xsel)
+{
+    extern __shared__ cplx bsh[];
+    const DChain &c = chains[blockIdx.x];
+    solve_blk_body(c.W3, c.n, c.n, c.piv, chain_mat(c, xsel, 0), c.n, c.n, c.W2, bsh);
 }

@@ -6,8 +6,8 @@
 #define FULL_DBL 1
 
 // ------------------------------------------------------------------------------------------
-// SFMT19937 on the device (lib/m_spring.f90:187-379): one thread per chain walks the chain's
-// own generator; state = 624 uint32 lanes + index, kept in global memory.
+// SFMT19937 on the device (lib/m_spring.f90:187-379): every chain owns its generator; state =
+// 624 uint32 lanes + index, kept in global memory between calls.
 // ------------------------------------------------------------------------------------------
 __device__ static void sfmt_refill(uint32_t *w)
 {
@@ -36,13 +36,11 @@ __device__ static double sfmt_next(uint32_t *w, int &idx)
     return ((double)w[idx++] + 0.5) * (1.0 / 4294967296.0);
 }
 
-extern "C" __global__ void k_sfmt_seed(const DChain *chains, int nchains, const int32_t *seeds)
+// init_gen_rand(seed) and the period certification (lib/m_spring.f90:187-243)
+__device__ static void sfmt_seed_state(uint32_t *w, int32_t seed)
 {
-    int ci = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ci >= nchains) return;
-    uint32_t *w = chains[ci].rng;
     const uint32_t parity[4] = {0x00000001u, 0x00000000u, 0x00000000u, 0x13c9e684u};
-    w[0] = (uint32_t)seeds[ci];
+    w[0] = (uint32_t)seed;
     for (int i = 1; i < 624; i++) w[i] = 1812433253u * (w[i - 1] ^ (w[i - 1] >> 30)) + (uint32_t)i;
     uint32_t inner = 0;
     for (int i = 0; i < 4; i++) inner ^= w[i] & parity[i];
@@ -60,23 +58,57 @@ extern "C" __global__ void k_sfmt_seed(const DChain *chains, int nchains, const 
     w[624] = 624;
 }
 
-// mode 0: fresh HS field conf(i,nt) = ceiling(u*lcomp); mode 1: the pass's 2*N*ltrot uniforms
-extern "C" __global__ void k_sfmt_draw(const DChain *chains, int nchains, int mode)
+extern "C" __global__ void k_sfmt_seed(const DChain *chains, int nchains, const int32_t *seeds)
 {
     int ci = blockIdx.x * blockDim.x + threadIdx.x;
     if (ci >= nchains) return;
-    const DChain &c = chains[ci];
-    uint32_t *w = c.rng;
-    int idx = (int)w[624];
+    sfmt_seed_state(chains[ci].rng, seeds[ci]);
+}
+
+// Draws tot numbers from the generator whose state (624 words + index) is at g, one workgroup
+// of 64 threads: the state is kept in LDS, lane 0 runs each refill (the recursion is sequential
+// over the 156 words), and the whole wave converts and stores the block; emit(k, u) receives
+// draw k (0-based) of this call.
+template <typename Emit>
+__device__ static void sfmt_draw_block(uint32_t *g, size_t tot, Emit emit)
+{
+    __shared__ uint32_t w[624];
+    for (int i = threadIdx.x; i < 624; i += blockDim.x) w[i] = g[i];
+    int idx = (int)g[624];
+    __syncthreads();
+    size_t k = 0;
+    while (k < tot) {
+        if (idx >= 624) {
+            if (threadIdx.x == 0) sfmt_refill(w);
+            __syncthreads();
+            idx = 0;
+        }
+        const int cnt = (int)min((size_t)(624 - idx), tot - k);
+        for (int j = threadIdx.x; j < cnt; j += blockDim.x)
+            emit(k + j, ((double)w[idx + j] + 0.5) * (1.0 / 4294967296.0));
+        idx += cnt;
+        k += cnt;
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < 624; i += blockDim.x) g[i] = w[i];
+    if (threadIdx.x == 0) g[624] = (uint32_t)idx;
+}
+
+// one workgroup per chain; mode 0: fresh HS field conf(i,nt) = ceiling(u*lcomp) (nt outer);
+// mode 1: the pass's 2*N*ltrot uniforms
+extern "C" __global__ void __launch_bounds__(64)
+k_sfmt_draw(const DChain *chains, int nchains, int mode)
+{
+    const DChain &c = chains[blockIdx.x];
     if (mode == 0) {
         if (!c.lwrapu) return;
-        size_t tot = (size_t)c.n * c.ltrot;
-        for (size_t k = 0; k < tot; k++) c.conf[k] = (int32_t)ceil(sfmt_next(w, idx) * c.lcomp);
+        const int lc = c.lcomp;
+        int32_t *conf = c.conf;
+        sfmt_draw_block(c.rng, (size_t)c.n * c.ltrot, [=](size_t k, double u) { conf[k] = (int32_t)ceil(u * lc); });
     } else {
-        size_t tot = (size_t)2 * c.n * c.ltrot;
-        for (size_t k = 0; k < tot; k++) c.unif[k] = sfmt_next(w, idx);
+        double *un = c.unif;
+        sfmt_draw_block(c.rng, (size_t)2 * c.n * c.ltrot, [=](size_t k, double u) { un[k] = u; });
     }
-    w[624] = (uint32_t)idx;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -100,7 +132,8 @@ __device__ __forceinline__ cplx vtab(const DChain &c, int vsel, int is)
 
 // one bond per thread (and a stride of columns): the bond's endpoints and 2x2 block are read
 // once per family and applied to every column the thread covers
-__device__ static void cb_left_fams(const DChain &c, cplx *lds, int ncol, const cplx *T, bool rev)
+template <class Chain>
+__device__ static void cb_left_fams(const Chain &c, cplx *lds, int ncol, const cplx *T, bool rev)
 {
     const int nf = c.nn_nf, nl = c.nn_lf, n = c.n;
     const int groups = max(1, (int)blockDim.x / nl);
@@ -164,7 +197,8 @@ k_cb_cols(const DChain *chains, int msel, int pt, int ksel, int prog, int vsel, 
     }
 }
 
-__device__ static void cb_right_fams(const DChain &c, cplx *lds, int nrow, const cplx *T, bool rev)
+template <class Chain>
+__device__ static void cb_right_fams(const Chain &c, cplx *lds, int nrow, const cplx *T, bool rev)
 {
     const int nf = c.nn_nf, nl = c.nn_lf;
     const int groups = max(1, (int)blockDim.x / nl);
@@ -258,49 +292,43 @@ __device__ __forceinline__ double4_t mfma(double a, double b, double4_t c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// one 64 x 64 tile (tm, tn) of C (m x n, ldc) = alpha op(A) op(B) + beta C, op(A) m x kk,
+// op(B) kk x n; op = 0: as stored, 1: conjugate transpose
 template <int OPA, int OPB>
-__global__ void __launch_bounds__(256)
-k_zgemm(const DChain *chains, int selA, int ptA, int selB, int ptB, int selC, int ptC,
-        double ar, double ai, double br, double bi)
+__device__ static void zgemm_tile(const cplx *A, int lda, const cplx *B, int ldb, cplx *C, int ldc, int m, int n,
+                                  int kk, int tm, int tn, double ar, double ai, double br, double bi)
 {
     __shared__ double sAr[GT][GK + 1], sAi[GT][GK + 1];
     __shared__ double sBr[GK][GT + 1], sBi[GK][GT + 1];
-    const DChain &c = chains[blockIdx.z];
-    const int n = c.n;
-    const int tm = blockIdx.x * GT, tn = blockIdx.y * GT;
-    if (tm >= n || tn >= n) return;
-    const cplx *A = chain_mat(c, selA, ptA);
-    const cplx *B = chain_mat(c, selB, ptB);
-    cplx *C = chain_mat(c, selC, ptC);
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
     double4_t accr[2][2], acci[2][2];
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++)
             for (int r = 0; r < 4; r++) { accr[i][j][r] = 0.0; acci[i][j][r] = 0.0; }
-    for (int k0 = 0; k0 < n; k0 += GK) {
+    for (int k0 = 0; k0 < kk; k0 += GK) {
         for (int q = 0; q < 4; q++) {
             int idx = t + 256 * q;
-            int i, kk;
+            int i, k;
             cplx v = make_double2(0.0, 0.0);
-            if (OPA == 0) { i = idx % GT; kk = idx / GT; if (tm + i < n && k0 + kk < n) v = A[(size_t)(tm + i) + (size_t)(k0 + kk) * n]; }
-            else { kk = idx % GK; i = idx / GK; if (tm + i < n && k0 + kk < n) v = cconj(A[(size_t)(k0 + kk) + (size_t)(tm + i) * n]); }
-            sAr[i][kk] = v.x; sAi[i][kk] = v.y;
+            if (OPA == 0) { i = idx % GT; k = idx / GT; if (tm + i < m && k0 + k < kk) v = A[(size_t)(tm + i) + (size_t)(k0 + k) * lda]; }
+            else { k = idx % GK; i = idx / GK; if (tm + i < m && k0 + k < kk) v = cconj(A[(size_t)(k0 + k) + (size_t)(tm + i) * lda]); }
+            sAr[i][k] = v.x; sAi[i][k] = v.y;
             int j;
             cplx u = make_double2(0.0, 0.0);
-            if (OPB == 0) { kk = idx % GK; j = idx / GK; if (tn + j < n && k0 + kk < n) u = B[(size_t)(k0 + kk) + (size_t)(tn + j) * n]; }
-            else { j = idx % GT; kk = idx / GT; if (tn + j < n && k0 + kk < n) u = cconj(B[(size_t)(tn + j) + (size_t)(k0 + kk) * n]); }
-            sBr[kk][j] = u.x; sBi[kk][j] = u.y;
+            if (OPB == 0) { k = idx % GK; j = idx / GK; if (tn + j < n && k0 + k < kk) u = B[(size_t)(k0 + k) + (size_t)(tn + j) * ldb]; }
+            else { j = idx % GT; k = idx / GT; if (tn + j < n && k0 + k < kk) u = cconj(B[(size_t)(tn + j) + (size_t)(k0 + k) * ldb]); }
+            sBr[k][j] = u.x; sBi[k][j] = u.y;
         }
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < GK / 4; ks++) {
-            const int kk = 4 * ks + (lane >> 4), l16 = lane & 15;
+            const int k = 4 * ks + (lane >> 4), l16 = lane & 15;
             double xr[2], xi[2], yr[2], yi[2];
             for (int s = 0; s < 2; s++) {
                 // transposed product: X = op(B)^T rows (output columns), Y = op(A)^T cols (output rows)
-                xr[s] = sBr[kk][32 * wn + 16 * s + l16]; xi[s] = sBi[kk][32 * wn + 16 * s + l16];
-                yr[s] = sAr[32 * wm + 16 * s + l16][kk]; yi[s] = sAi[32 * wm + 16 * s + l16][kk];
+                xr[s] = sBr[k][32 * wn + 16 * s + l16]; xi[s] = sBi[k][32 * wn + 16 * s + l16];
+                yr[s] = sAr[32 * wm + 16 * s + l16][k]; yi[s] = sAi[32 * wm + 16 * s + l16][k];
             }
             for (int sn = 0; sn < 2; sn++)
                 for (int sm = 0; sm < 2; sm++) {
@@ -318,14 +346,27 @@ k_zgemm(const DChain *chains, int selA, int ptA, int selB, int ptB, int selC, in
             for (int r = 0; r < 4; r++) {
                 int row = tm + 32 * wm + 16 * sm + (lane & 15);
                 int col = tn + 32 * wn + 16 * sn + (lane >> 4) + 4 * r;
-                if (row < n && col < n) {
-                    size_t o = (size_t)row + (size_t)col * n;
+                if (row < m && col < n) {
+                    size_t o = (size_t)row + (size_t)col * ldc;
                     double vr = accr[sm][sn][r], vi = acci[sm][sn][r];
                     double outr = ar * vr - ai * vi, outi = ar * vi + ai * vr;
                     if (!zb) { cplx cv = C[o]; outr += br * cv.x - bi * cv.y; outi += br * cv.y + bi * cv.x; }
                     C[o] = make_double2(outr, outi);
                 }
             }
+}
+
+template <int OPA, int OPB>
+__global__ void __launch_bounds__(256)
+k_zgemm(const DChain *chains, int selA, int ptA, int selB, int ptB, int selC, int ptC,
+        double ar, double ai, double br, double bi)
+{
+    const DChain &c = chains[blockIdx.z];
+    const int n = c.n;
+    const int tm = blockIdx.x * GT, tn = blockIdx.y * GT;
+    if (tm >= n || tn >= n) return;
+    zgemm_tile<OPA, OPB>(chain_mat(c, selA, ptA), n, chain_mat(c, selB, ptB), n, chain_mat(c, selC, ptC), n, n, n, n,
+                         tm, tn, ar, ai, br, bi);
 }
 
 template __global__ void k_zgemm<0, 0>(const DChain *, int, int, int, int, int, int, double, double, double, double);
@@ -371,6 +412,68 @@ __device__ static void wg_rank_k(cplx *G, int n, const cplx *A, const cplx *B, i
                 cplx v = make_double2(accr[r], acci[r]);
                 if (csr) v = cmul(v, make_double2(csr[col], csi[col]));
                 G[(size_t)row + (size_t)col * n] = v;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// The same rank-k update G += A B^T for k <= K known at compile time, organised for latency:
+// a wave owns 16-row strips of G; the strip's A fragment (K/4 complex per lane) stays in
+// registers while the wave walks the 16-column tiles, and the next tile's B fragment and G
+// tile are loaded before the current tile's MFMAs are issued.
+// ------------------------------------------------------------------------------------------
+template <int K>
+__device__ static void wg_rank_k_strips(cplx *G, int n, const cplx *A, const cplx *B, int k)
+{
+    constexpr int KS = K / 4;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int nt = (n + 15) >> 4;
+    const int l16 = lane & 15, kq = lane >> 4;
+    for (int si = w; si < nt; si += nw) {
+        const int i0 = si * 16, row = i0 + l16;
+        const bool rok = row < n;
+        cplx y[KS], x[KS], g[4];
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int m = 4 * s + kq;
+            y[s] = (rok && m < k) ? A[(size_t)row + (size_t)m * n] : make_double2(0.0, 0.0);
+            x[s] = (l16 < n && m < k) ? B[(size_t)l16 + (size_t)m * n] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int col = kq + 4 * r;
+            g[r] = (rok && col < n) ? G[(size_t)row + (size_t)col * n] : make_double2(0.0, 0.0);
+        }
+        for (int tj = 0; tj < nt; tj++) {
+            const int j0 = tj * 16, j1 = j0 + 16;
+            const bool more = tj + 1 < nt;
+            double4_t accr, acci;
+#pragma unroll
+            for (int r = 0; r < 4; r++) { accr[r] = g[r].x; acci[r] = g[r].y; }
+            // each register is refilled with the next tile's value as soon as it is consumed
+            if (more) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int col = j1 + kq + 4 * r;
+                    g[r] = (rok && col < n) ? G[(size_t)row + (size_t)col * n] : make_double2(0.0, 0.0);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < KS; s++) {
+                accr = mfma(x[s].x, y[s].x, accr);
+                accr = mfma(-x[s].y, y[s].y, accr);
+                acci = mfma(x[s].x, y[s].y, acci);
+                acci = mfma(x[s].y, y[s].x, acci);
+                if (more) {
+                    const int m = 4 * s + kq;
+                    x[s] = (j1 + l16 < n && m < k) ? B[(size_t)(j1 + l16) + (size_t)m * n] : make_double2(0.0, 0.0);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int col = j0 + kq + 4 * r;
+                if (rok && col < n) G[(size_t)row + (size_t)col * n] = make_double2(accr[r], acci[r]);
             }
         }
     }
@@ -1422,7 +1525,7 @@ k_save_point(const DChain *chains, int pt)
 #define DSB 8   // sites per staged block of G columns and rows
 
 template <int K>
-__global__ void __launch_bounds__(K >= 16 ? 512 : 1024)
+__global__ void __launch_bounds__(K >= 8 ? 512 : 1024)
 k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
 {
     extern __shared__ cplx sh[];
@@ -1473,7 +1576,9 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
         const cplx dg = diagg[s];
         const cplx ratio1 = cadd(cmul(make_double2(1.0 - dg.x, -dg.y), d.db), make_double2(1.0, 0.0));
         cplx rt;
-        const bool acc = site_accept(c, ratio1, d, us[2 * s + 1], phase, &rt);
+        // every thread evaluates the same decision; reading it from lane 0 makes the branch
+        // and the delay count scalar
+        const bool acc = __builtin_amdgcn_readfirstlane((int)site_accept(c, ratio1, d, us[2 * s + 1], phase, &rt)) != 0;
         if (acc) {
             accm += 1.0;
             phase = phase_after(phase, rt);
@@ -1519,8 +1624,12 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
                     for (int m = 0; m < K; m++)
                         if (m < ik) dst[(size_t)r + (size_t)m * n] = reg[m];
                 }
+                // the vectors restart from zero after the flush; clearing them here frees
+                // their registers for the flush
+#pragma unroll
+                for (int m = 0; m < K; m++) reg[m] = make_double2(0.0, 0.0);
                 __syncthreads();
-                wg_rank_k(G, n, c.avec, c.bvec, ik, nullptr, nullptr);
+                wg_rank_k_strips<K>(G, n, c.avec, c.bvec, ik);
             }
             __syncthreads();
             ik = 0;

@@ -3,6 +3,7 @@
 // launch advances all chains of the batch at once.
 #include "kernels.hip"
 #include "blocked.hip"
+#include "pqmc_kernels.hip"
 #include "../../include/sunhip.h"
 
 #include <cmath>
@@ -35,25 +36,29 @@ struct EvRec {
 
 }  // namespace
 
-struct sh_ctx {
+// what every context of the library holds: the stream, its allocations, the last error and
+// the per-kernel-group timers
+struct ctx_base {
     int dev = 0;
     hipStream_t st = nullptr;
-    int nch = 0, nmax = 0, ltrot = 0, nwrap = 0, nst = 0, kmode = 0, update = 0;
-    std::vector<int> iwrap_nt, wrap_step;
-    std::vector<DChain> hc;
-    DChain *dc = nullptr;
     std::vector<void *> allocs;
-    int32_t *d_seeds = nullptr;
-    bool started = false;
-    bool legacy_update = false;   // test hook: the L2-resident delay update kernel
     std::string err;
-    // timing
     bool timing = false;
     std::vector<EvRec> evs;
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
     double tms[T_NCLS] = {0};
     double tcnt[T_NCLS] = {0};
+};
+
+struct sh_ctx : ctx_base {
+    int nch = 0, nmax = 0, ltrot = 0, nwrap = 0, nst = 0, kmode = 0, update = 0;
+    std::vector<int> iwrap_nt, wrap_step;
+    std::vector<DChain> hc;
+    DChain *dc = nullptr;
+    int32_t *d_seeds = nullptr;
+    bool started = false;
+    bool legacy_update = false;   // test hook: the L2-resident delay update kernel
     // push/pop_stage backups
     std::vector<void *> stage;
     std::vector<cplx *> trace_bufs;
@@ -68,14 +73,14 @@ struct sh_ctx {
         }                                                                          \
     } while (0)
 
-static int fail(sh_ctx *ctx, int code, const std::string &m)
+static int fail(ctx_base *ctx, int code, const std::string &m)
 {
     if (ctx) ctx->err = m;
     return code;
 }
 
 template <typename T>
-static T *dalloc(sh_ctx *ctx, size_t n)
+static T *dalloc(ctx_base *ctx, size_t n)
 {
     void *p = nullptr;
     if (hipMalloc(&p, n * sizeof(T) + 256) != hipSuccess) return nullptr;
@@ -85,7 +90,7 @@ static T *dalloc(sh_ctx *ctx, size_t n)
 }
 
 template <typename T>
-static T *dupload(sh_ctx *ctx, const T *src, size_t n)
+static T *dupload(ctx_base *ctx, const T *src, size_t n)
 {
     T *p = dalloc<T>(ctx, n);
     if (p && src && n) hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice);
@@ -93,7 +98,7 @@ static T *dupload(sh_ctx *ctx, const T *src, size_t n)
 }
 
 // ---- timing ------------------------------------------------------------------------------
-static hipEvent_t ev_get(sh_ctx *ctx)
+static hipEvent_t ev_get(ctx_base *ctx)
 {
     if (ctx->pool_used == ctx->pool.size()) {
         hipEvent_t e;
@@ -104,10 +109,10 @@ static hipEvent_t ev_get(sh_ctx *ctx)
 }
 
 struct Tic {
-    sh_ctx *ctx;
+    ctx_base *ctx;
     int cls;
     hipEvent_t a = nullptr;
-    Tic(sh_ctx *c, int k) : ctx(c), cls(k)
+    Tic(ctx_base *c, int k) : ctx(c), cls(k)
     {
         if (ctx->timing) {
             a = ev_get(ctx);
@@ -124,7 +129,7 @@ struct Tic {
     }
 };
 
-static void timers_collect(sh_ctx *ctx)
+static void timers_collect(ctx_base *ctx)
 {
     if (ctx->evs.empty()) return;
     hipStreamSynchronize(ctx->st);
@@ -353,7 +358,7 @@ static int delay_reg_depth(sh_ctx *ctx, int *threads)
     int kt = kmax <= 4 ? 4 : kmax <= 8 ? 8 : kmax <= 16 ? 16 : kmax <= 32 ? 32 : 0;
     int rows = (ctx->nmax + 63) / 64 * 64;
     int nt = 2 * rows;
-    if (kt == 0 || nt > (kt >= 16 ? 512 : 1024)) return 0;
+    if (kt == 0 || nt > (kt >= 8 ? 512 : 1024)) return 0;
     // LDS: diag, two exchange vectors, DSB staged columns and rows, uniforms and HS field
     if ((size_t)(3 + 2 * DSB) * ctx->nmax * sizeof(cplx) + (size_t)ctx->nmax * 20 > 150 * 1024) return 0;
     *threads = nt;
@@ -555,7 +560,7 @@ const char *sh_last_error(const sh_ctx *ctx) { return ctx ? ctx->err.c_str() : "
 int sh_init_conf(sh_ctx *ctx)
 {
     if (!ctx) return SH_E_ARG;
-    hipLaunchKernelGGL(k_sfmt_draw, dim3((ctx->nch + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 0);
+    hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 0);
     return check_launch(ctx);
 }
 
@@ -613,7 +618,7 @@ int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure)
     if (!ctx->started) return fail(ctx, SH_E_STATE, "sweep_b0 before sweep_start_0b");
     set_identity(ctx, ctx->nst);
     const bool upd = lupdate && ctx->hc[0].lwrapu;
-    if (upd) hipLaunchKernelGGL(k_sfmt_draw, dim3((ctx->nch + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
+    if (upd) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
     int slice = 0;
     for (int nt = ctx->ltrot; nt >= 1; nt--, slice++) {
         if (lmeasure) measure(ctx);
@@ -656,7 +661,7 @@ int sh_sweep_0b(sh_ctx *ctx, int lupdate, int lmeasure, int lmeasure_dyn)
     if (!ctx->started) return fail(ctx, SH_E_STATE, "sweep_0b before sweep_start_0b");
     set_identity(ctx, 0);
     const bool upd = lupdate && ctx->hc[0].lwrapu;
-    if (upd) hipLaunchKernelGGL(k_sfmt_draw, dim3((ctx->nch + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
+    if (upd) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
     for (int nt = 1; nt <= ctx->ltrot; nt++) {
         {
             Tic t(ctx, T_WRAP);
@@ -964,7 +969,7 @@ extern "C" int sh_dbg_op(sh_ctx *ctx, int op, const int *a)
         update_slice(ctx, a[0], a[1]);
         break;
     case 11:
-        hipLaunchKernelGGL(k_sfmt_draw, dim3((ctx->nch + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
+        hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
         break;
     case 12: measure(ctx); break;
     case 13: wrap_replace(ctx, a[0]); break;
@@ -976,3 +981,6 @@ extern "C" int sh_dbg_op(sh_ctx *ctx, int op, const int *a)
     HIPCHK(hipStreamSynchronize(ctx->st));
     return 0;
 }
+
+// ---- projective sweep (include/sunhip_proj.h) -------------------------------------------------
+#include "pqmc.hip"

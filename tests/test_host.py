@@ -17,16 +17,17 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared(header):
     txt = open(os.path.join(ROOT, "include", header)).read()
-    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(sh_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(shp?_\w+)\s*\(", txt, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
     L = _lib.lib()
-    names = _declared("sunhip.h") + _declared("sunhip_testing.h")
-    assert len(names) >= 20
+    names = _declared("sunhip.h") + _declared("sunhip_testing.h") + _declared("sunhip_proj.h")
+    assert len(names) >= 40
     for n in names:
         assert hasattr(L, n), n
     assert set(_declared("sunhip.h")) == set(_lib.EXPORTS)
+    assert set(_declared("sunhip_proj.h")) == set(_lib.EXPORTS_PROJ)
 
 
 @pytest.mark.parametrize("lattice,la,kmode", [("square", 4, "checkerboard"), ("square", 6, "dense"),
@@ -126,3 +127,71 @@ def test_walkers_raise_on_mixed_batches():
     with pytest.raises(SweepError):
         from sunhubbardmy_amd import FTWalkers
         FTWalkers([Model(beta=2.0), Model(beta=3.0)])
+
+
+# ---- projective (t-V) host side --------------------------------------------------------------
+from sunhubbardmy_amd.proj import ProjModel, ShpModel, make_proj_tables, trial_hamiltonian  # noqa: E402
+
+
+@pytest.mark.parametrize("lattice,la,kmode", [("honeycomb", 3, "dense"), ("honeycomb", 2, "checkerboard"),
+                                              ("square", 4, "checkerboard")])
+def test_proj_tables_match_oracle(lattice, la, kmode):
+    m = ProjModel(lattice=lattice, la=la, lb=la, kmode=kmode, beta=1.3, nwrap=4, rv=1.7, nustock=6)
+    t = make_proj_tables(m)
+    o = oracle.OracleProjChain(lattice=lattice, la=la, lb=la, kmode=kmode, beta=1.3, nwrap=4, rv=1.7, nustock=6)
+    assert (t["ndim"], t["ne"], t["ltrot"], t["nst"]) == (o.ndim, o.ne, o.ltrot, o.nst)
+    assert np.abs(t["urt"] - o.table(4)).max() < 1e-14
+    assert np.abs(t["urtm1"] - o.table(5)).max() < 1e-14
+    assert np.array_equal(t["lat"]["bonds"].reshape(-1), o.table(6).astype(int))
+    assert np.array_equal(t["iwrap_nt"], o.table(7).astype(int))
+    assert np.abs(t["v"]["vt"].reshape(-1) - o.table(100)).max() < 1e-15
+    assert np.abs(t["v"]["vd"].reshape(-1) - o.table(101)).max() < 1e-15
+
+
+def test_proj_trial_hamiltonian_matches_oracle():
+    seed = chain_seed(4)
+    m = ProjModel(lattice="honeycomb", la=3, lb=3, beta=1.0, rndness=0.25, mu=0.1, seed=seed)
+    t = make_proj_tables(m)
+    o = oracle.OracleProjChain(lattice="honeycomb", la=3, lb=3, beta=1.0, rndness=0.25, mu=0.1, seed=seed)
+    o.init_conf()
+    o.set_trial()
+    p_o, w_o = o.proj
+    lat = t["lat"]
+    nconf = t["ndim"] * lat["nn_nf"] * t["ltrot"]
+    u = oracle.sfmt_stream(seed, nconf + lat["nn_lf"] * lat["nn_nf"])[nconf:]
+    w, v = np.linalg.eigh(trial_hamiltonian(m, lat, u))
+    assert np.abs(w - w_o).max() < 1e-12
+    p = v[:, :t["ne"]]
+    assert np.abs(p @ p.T - p_o @ p_o.conj().T).max() < 1e-11
+
+
+def test_proj_default_stock_size():
+    assert make_proj_tables(ProjModel(lattice="honeycomb", la=18, lb=18))["nustock"] == 64   # N = 648
+    assert make_proj_tables(ProjModel(lattice="honeycomb", la=2, lb=2))["nustock"] == 32
+
+
+def _proj_create(tables, **override):
+    from sunhubbardmy_amd.proj import ProjWalkers
+    w = ProjWalkers.__new__(ProjWalkers)
+    w._keep = []
+    arr = (ShpModel * len(tables))()
+    for k, t in enumerate(tables):
+        arr[k] = w._shpmodel(t)
+        for key, val in override.items():
+            setattr(arr[k], key, val)
+    ctx = ctypes.c_void_p()
+    return _lib.lib().shp_create(ctypes.byref(ctx), len(tables), ctypes.cast(arr, ctypes.c_void_p), 0)
+
+
+def test_proj_create_rejects_bad_arguments():
+    """Argument errors come back as SH_E_ARG (-1) before any device call."""
+    t1 = make_proj_tables(ProjModel(beta=1.0))
+    t2 = make_proj_tables(ProjModel(beta=2.0))
+    assert _proj_create([t1, t2]) == -1            # chains disagree on ltrot
+    assert _proj_create([t1], nustock=66) == -1    # stock above 64 rows
+    assert _proj_create([t1], nustock=7) == -1     # odd stock
+    assert _proj_create([t1], ne=0) == -1
+    assert _proj_create([t1], ndim=2048) == -1
+    assert _lib.lib().shp_destroy(None) == 0
+    assert _lib.lib().shp_sweep_b0(None, 1, 1) == -1
+    assert _lib.lib().shp_last_error(None) == b"null context"
