@@ -69,6 +69,48 @@ def cpu_baseline(kw, tau):
             "sweeps_per_s": sweeps_per_s}
 
 
+def bench_pqmc(args):
+    """Projective t-V sweep: honeycomb 18x18 (N=648, ne=324), t=1, V=1.2, theta=20 (Ltrot=200),
+    dtau=0.1, rndness 1e-4, nwrap=10, dense kinetic term, submatrix-LR with the reference's
+    default stock (64 rows).  A step is one sweep (b0 + 0b, measurement in the middle slice)."""
+    from sunhubbardmy_amd import chain_seed
+    from sunhubbardmy_amd.proj import ProjModel, ProjWalkers
+
+    chains = args.chains if args.chains != 256 else 16
+    steps = args.steps if args.steps != 60 else 2
+    warmup = args.warmup if args.warmup != 30 else 1
+    kw = dict(lattice="honeycomb", la=18, lb=18, beta=20.0, dtau=0.1, rt=1.0, rv=1.2, rndness=1e-4, kmode="dense",
+              nwrap=10, nustock=0, mid_len=1, wrap_check=False)
+    w = ProjWalkers([ProjModel(seed=chain_seed(c), **kw) for c in range(chains)])
+    print("# pqmc %d chains, N=%d, ne=%d, Ltrot=%d" % (chains, w.ndim, w.ne, w.ltrot), file=sys.stderr, flush=True)
+    w.init_conf()
+    w.set_trial()
+    w.sweep_start_0b()
+    for s in range(warmup):
+        w.sweep(lupdate=True, lmeasure=False)
+        print("# warmup %d" % s, file=sys.stderr, flush=True)
+    w.synchronize()
+    w.timers(enable=args.timers)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        w.sweep(lupdate=True, lmeasure=True)
+        print("# step %d" % s, file=sys.stderr, flush=True)
+    w.synchronize()
+    dt = time.perf_counter() - t0
+    tm = w.timers(enable=False)
+    d = [w.diag(c) for c in range(min(chains, 4))]
+    line = {"metric": "projective t-V chain sweeps/s", "value": chains * steps / dt, "unit": "sweeps/s",
+            "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": 1000.0 * dt / steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "spinless t-V honeycomb 18x18 (N=648, 324 particles), V=1.2, theta=20, dtau=0.1, "
+                                   "nwrap=10, dense K, submatrix-LR (64-row stocks)", "chains": chains},
+            "acceptance": float(np.mean([x["accepted"] / max(x["tried"], 1) for x in d]))}
+    if args.timers:
+        line["kernel_ms"] = {g: round(v[0], 3) for g, v in tm.items()}
+        line["kernel_launches"] = {g: v[1] for g, v in tm.items()}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,11 +121,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timers", dest="timers", action="store_false",
                     help="skip the per-kernel hipEvent timing of the timed steps")
+    ap.add_argument("--workload", choices=("ft", "pqmc"), default="ft",
+                    help="ft: the headline finite-temperature workload; pqmc: the projective t-V workload "
+                         "(honeycomb L=18, N=648, 324 particles, V=1.2, theta=20, submatrix-LR)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; chains are independent, so N GPUs would run N "
               "independent batches (no collective on the sweep path). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "pqmc":
+        return bench_pqmc(args)
 
     from sunhubbardmy_amd import FTWalkers, Model, chain_seed
     from sunhubbardmy_amd.autocorr import tau_int

@@ -130,6 +130,24 @@ __device__ __forceinline__ cplx vtab(const DChain &c, int vsel, int is)
     return b;
 }
 
+// the lcomp (<= 4) entries of one V table in registers: the loads are uniform (scalar) and a
+// per-element lookup is a select, not a memory access
+struct VTab {
+    cplx v[4];
+    __device__ VTab(const DChain &c, int vsel)
+    {
+        for (int k = 0; k < 4; k++) v[k] = (k < c.lcomp) ? vtab(c, vsel, k + 1) : make_double2(0.0, 0.0);
+    }
+    __device__ __forceinline__ cplx operator()(int is) const
+    {
+        cplx r = v[0];
+        r = (is == 2) ? v[1] : r;
+        r = (is == 3) ? v[2] : r;
+        r = (is == 4) ? v[3] : r;
+        return r;
+    }
+};
+
 // one bond per thread (and a stride of columns): the bond's endpoints and 2x2 block are read
 // once per family and applied to every column the thread covers
 template <class Chain>
@@ -159,9 +177,10 @@ __device__ static void cb_left_v(const DChain &c, cplx *lds, int ncol, int nt, i
 {
     const int n = c.n;
     const int32_t *cf = c.conf + (size_t)(nt - 1) * n;
+    const VTab vt(c, vsel);
     for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
         int r = e % n;
-        lds[e] = cmul(vtab(c, vsel, cf[r]), lds[e]);
+        lds[e] = cmul(vt(cf[r]), lds[e]);
     }
     __syncthreads();
 }
@@ -224,9 +243,10 @@ __device__ static void cb_right_v(const DChain &c, cplx *lds, int nt, int vsel)
 {
     const int n = c.n;
     const int32_t *cf = c.conf + (size_t)(nt - 1) * n;
+    const VTab vt(c, vsel);
     for (int e = threadIdx.x; e < n * RPB; e += blockDim.x) {
         int col = e / RPB;
-        lds[e] = cmul(lds[e], vtab(c, vsel, cf[col]));
+        lds[e] = cmul(lds[e], vt(cf[col]));
     }
     __syncthreads();
 }
@@ -271,9 +291,9 @@ k_vscale(const DChain *chains, int msel, int pt, int side, int vsel, int vsel2, 
     const int32_t *cf = c.conf + (size_t)(nt - 1) * n;
     cplx *M = chain_mat(c, msel, pt);
     cplx v = M[e];
-    if (side == 0 || side == 2) v = cmul(vtab(c, vsel, cf[r]), v);
-    if (side == 1) v = cmul(v, vtab(c, vsel, cf[col]));
-    if (side == 2) v = cmul(v, vtab(c, vsel2, cf[col]));
+    if (side == 0 || side == 2) v = cmul(VTab(c, vsel)(cf[r]), v);
+    if (side == 1) v = cmul(v, VTab(c, vsel)(cf[col]));
+    if (side == 2) v = cmul(v, VTab(c, vsel2)(cf[col]));
     M[e] = v;
 }
 
@@ -488,7 +508,26 @@ struct SiteDraw {
     cplx db;
 };
 
-__device__ __forceinline__ bool site_accept(const DChain &c, cplx ratio1, const SiteDraw &d, double u2,
+// the HS tables a site decision reads, copied to LDS once per launch (the same field names as
+// DChain so that the decision helpers take either)
+struct HsTab {
+    int lcomp, nflr;
+    double gaml[4];
+    cplx delta_bmat[12], phase_ratio[12];
+};
+
+__device__ static void hs_tab_load(HsTab &t, const DChain &c)
+{
+    if (threadIdx.x == 0) {
+        t.lcomp = c.lcomp;
+        t.nflr = c.nflr;
+        for (int k = 0; k < 4; k++) t.gaml[k] = c.gaml[k];
+        for (int k = 0; k < 12; k++) { t.delta_bmat[k] = c.delta_bmat[k]; t.phase_ratio[k] = c.phase_ratio[k]; }
+    }
+}
+
+template <class Tab>
+__device__ __forceinline__ bool site_accept(const Tab &c, cplx ratio1, const SiteDraw &d, double u2,
                                             cplx phase, cplx *ratiotot)
 {
     cplx rt = make_double2(1.0, 0.0);
@@ -501,7 +540,8 @@ __device__ __forceinline__ bool site_accept(const DChain &c, cplx ratio1, const 
     return ratio_abs > u2;
 }
 
-__device__ __forceinline__ SiteDraw site_draw(const DChain &c, int is, double u1)
+template <class Tab>
+__device__ __forceinline__ SiteDraw site_draw(const Tab &c, int is, double u1)
 {
     SiteDraw d;
     d.is = is;
@@ -1555,6 +1595,8 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
         cfs[q] = cf[q];
     }
     for (int q = threadIdx.x; q < 2 * n; q += blockDim.x) us[q] = u[q];
+    __shared__ HsTab tab;
+    hs_tab_load(tab, c);
     int ik = 0, s0 = -1;
     double accm = 0.0;
     for (int s = 0; s < n; s++) {
@@ -1572,13 +1614,13 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
             }
         }
         __syncthreads();
-        const SiteDraw d = site_draw(c, cfs[s], us[2 * s]);
+        const SiteDraw d = site_draw(tab, cfs[s], us[2 * s]);
         const cplx dg = diagg[s];
         const cplx ratio1 = cadd(cmul(make_double2(1.0 - dg.x, -dg.y), d.db), make_double2(1.0, 0.0));
         cplx rt;
         // every thread evaluates the same decision; reading it from lane 0 makes the branch
         // and the delay count scalar
-        const bool acc = __builtin_amdgcn_readfirstlane((int)site_accept(c, ratio1, d, us[2 * s + 1], phase, &rt)) != 0;
+        const bool acc = __builtin_amdgcn_readfirstlane((int)site_accept(tab, ratio1, d, us[2 * s + 1], phase, &rt)) != 0;
         if (acc) {
             accm += 1.0;
             phase = phase_after(phase, rt);

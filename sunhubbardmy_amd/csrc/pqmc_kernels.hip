@@ -149,10 +149,20 @@ struct POps {
 
 #define HADAMARD 0.70710678118654752440
 
-__device__ __forceinline__ void v_left_pair(const PChain &c, int is, int mode, cplx &x, cplx &y)
+// the eight bond-field factors in registers (uniform loads); t(is, k) selects without memory access
+struct BondTab {
+    cplx v[8];
+    __device__ BondTab(const PChain &c)
+    {
+        for (int k = 0; k < 8; k++) v[k] = c.vt[k];
+    }
+    __device__ __forceinline__ cplx operator()(int is, int k) const { return is == 2 ? v[4 + k] : v[k]; }
+};
+
+__device__ __forceinline__ void v_left_pair(const BondTab &tb, int is, int mode, cplx &x, cplx &y)
 {
     const cplx s = make_double2(HADAMARD, 0.0), ms = make_double2(-HADAMARD, 0.0);
-    const cplx *t = c.vt + 4 * (is - 1);
+    const cplx t[4] = {tb(is, 0), tb(is, 1), tb(is, 2), tb(is, 3)};
     if (mode == 2) { x = cmul(t[2], x); y = cmul(t[3], y); }
     cplx v1 = cadd(cmul(s, x), cmul(s, y));
     cplx v2 = cadd(cmul(s, x), cmul(ms, y));
@@ -161,10 +171,10 @@ __device__ __forceinline__ void v_left_pair(const PChain &c, int is, int mode, c
     y = v2;
 }
 
-__device__ __forceinline__ void v_right_pair(const PChain &c, int is, int mode, cplx &x, cplx &y)
+__device__ __forceinline__ void v_right_pair(const BondTab &tb, int is, int mode, cplx &x, cplx &y)
 {
     const cplx s = make_double2(HADAMARD, 0.0), ms = make_double2(-HADAMARD, 0.0);
-    const cplx *t = c.vt + 4 * (is - 1);
+    const cplx t[4] = {tb(is, 0), tb(is, 1), tb(is, 2), tb(is, 3)};
     if (mode == 2) { x = cmul(x, t[0]); y = cmul(y, t[1]); }
     cplx v1 = cadd(cmul(x, s), cmul(y, s));
     cplx v2 = cadd(cmul(x, s), cmul(y, ms));
@@ -182,6 +192,7 @@ kp_prop(const PChain *chains, POps ops)
     const int groups = max(1, (int)blockDim.x / nl);
     const int bt = threadIdx.x % nl, g = threadIdx.x / nl;
     if (!(ops.sides & (1 << blockIdx.y))) return;
+    const BondTab tb(c);
     if (blockIdx.y == 0) {
         // UR from the left, CPB columns
         const int c0 = blockIdx.x * CPB;
@@ -204,7 +215,7 @@ kp_prop(const PChain *chains, POps ops)
                 const int is = cf[bb];
                 for (int cc = g; cc < ncol; cc += groups) {
                     cplx x = lds[cc * n + i1], y = lds[cc * n + i2];
-                    v_left_pair(c, is, mode, x, y);
+                    v_left_pair(tb, is, mode, x, y);
                     lds[cc * n + i1] = x;
                     lds[cc * n + i2] = y;
                 }
@@ -237,7 +248,7 @@ kp_prop(const PChain *chains, POps ops)
                 const int is = cf[bb];
                 for (int rr = g; rr < nrow; rr += groups) {
                     cplx x = lds[i1 * RPB + rr], y = lds[i2 * RPB + rr];
-                    v_right_pair(c, is, mode, x, y);
+                    v_right_pair(tb, is, mode, x, y);
                     lds[i1 * RPB + rr] = x;
                     lds[i2 * RPB + rr] = y;
                 }
