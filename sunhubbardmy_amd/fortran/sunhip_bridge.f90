@@ -1,8 +1,10 @@
 ! sunhip_bridge.f90 -- ISO_C_BINDING interface to libsunhip.so for the reference's Fortran
 ! control layer.  A maintainer adds this module next to src/dqmc_ft_core.f90 and calls the
 ! sh_* entry points where src/dqmc_ft_ctrl.f90 calls allocate_core, dqmc_ft_sweep_start_0b,
-! dqmc_ft_sweep_b0/0b and push_stage/pop_stage.  Nonzero return codes map onto the
-! reference's `stop` (dqmc_ft_sweep_b0.f90:188-194, dqmc_ft_sweep_start_0b.f90:44-47).
+! dqmc_ft_sweep_b0/0b and push_stage/pop_stage, and the shp_* entry points where
+! src/dqmc_proj_ctrl.f90 calls allocate_core, dqmc_set_proj, dqmc_proj_sweep_start_0b and
+! dqmc_proj_sweep_b0/0b.  Nonzero return codes map onto the reference's `stop`
+! (dqmc_ft_sweep_b0.f90:188-194, dqmc_ft_sweep_start_0b.f90:44-47).
 module sunhip_bridge
   use, intrinsic :: iso_c_binding
   implicit none
@@ -23,6 +25,16 @@ module sunhip_bridge
      integer(c_int) :: update, nublock
      integer(c_int32_t) :: seed
   end type sh_model
+
+  ! mirrors struct shp_model of include/sunhip_proj.h
+  type, bind(C) :: shp_model
+     integer(c_int) :: ndim, ne, ltrot, nwrap, nst
+     type(c_ptr) :: iwrap_nt, wrap_step
+     integer(c_int) :: kmode, nn_nf, nn_lf
+     type(c_ptr) :: bonds, urt, urtm1, vtab, vdelta, stag
+     integer(c_int) :: lq, nflr, nustock, mid_len, wrap_check
+     integer(c_int32_t) :: seed
+  end type shp_model
 
   interface
      integer(c_int) function sh_create(ctx, nchains, models, device) bind(C, name="sh_create")
@@ -95,6 +107,82 @@ module sunhip_bridge
        import :: c_ptr, c_int
        type(c_ptr), value :: ctx
      end function sh_reset_obs
+
+     ! projective sweep (include/sunhip_proj.h)
+     integer(c_int) function shp_create(ctx, nchains, models, device) bind(C, name="shp_create")
+       import :: c_ptr, c_int, shp_model
+       type(c_ptr), intent(out) :: ctx
+       integer(c_int), value :: nchains, device
+       type(shp_model), intent(in) :: models(*)
+     end function shp_create
+     integer(c_int) function shp_destroy(ctx) bind(C, name="shp_destroy")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+     end function shp_destroy
+     integer(c_int) function shp_init_conf(ctx) bind(C, name="shp_init_conf")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+     end function shp_init_conf
+     integer(c_int) function shp_set_conf(ctx, chain, conf) bind(C, name="shp_set_conf")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(in) :: conf(*)
+     end function shp_set_conf
+     integer(c_int) function shp_get_conf(ctx, chain, conf) bind(C, name="shp_get_conf")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(out) :: conf(*)
+     end function shp_get_conf
+     integer(c_int) function shp_draw_uniforms(ctx, chain, n, u) bind(C, name="shp_draw_uniforms")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain, n
+       real(c_double), intent(out) :: u(*)
+     end function shp_draw_uniforms
+     integer(c_int) function shp_set_proj(ctx, chain, proj) bind(C, name="shp_set_proj")
+       import :: c_ptr, c_int, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       complex(c_double_complex), intent(in) :: proj(*)
+     end function shp_set_proj
+     integer(c_int) function shp_sweep_start_0b(ctx) bind(C, name="shp_sweep_start_0b")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+     end function shp_sweep_start_0b
+     integer(c_int) function shp_sweep_b0(ctx, lupdate, lmeasure) bind(C, name="shp_sweep_b0")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: lupdate, lmeasure
+     end function shp_sweep_b0
+     integer(c_int) function shp_sweep_0b(ctx, lupdate, lmeasure, ldyn) bind(C, name="shp_sweep_0b")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: lupdate, lmeasure, ldyn
+     end function shp_sweep_0b
+     integer(c_int) function shp_get_g(ctx, chain, g) bind(C, name="shp_get_g")
+       import :: c_ptr, c_int, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       complex(c_double_complex), intent(out) :: g(*)
+     end function shp_get_g
+     integer(c_int) function shp_get_diag(ctx, chain, d) bind(C, name="shp_get_diag")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       real(c_double), intent(out) :: d(10)
+     end function shp_get_diag
+     integer(c_int) function shp_get_obs(ctx, chain, o) bind(C, name="shp_get_obs")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       real(c_double), intent(out) :: o(11)
+     end function shp_get_obs
+     integer(c_int) function shp_reset_obs(ctx) bind(C, name="shp_reset_obs")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+     end function shp_reset_obs
   end interface
 
 contains
