@@ -151,12 +151,47 @@ __device__ static void wg_rank16(cplx *C, int ldc, int mrows, int ncols, const c
     }
 }
 
+// T factor of one panel (zlarft, forward, columnwise): S = V^H V over the panel's m rows,
+// T(:,j) = -tau_j T(:,0:j) S(0:j, j); V is the m x b panel in LDS (ld M, unit diagonal implicit)
+__device__ static void panel_tfactor(const cplx *P, int M, int m, int b, const cplx *tau, cplx *T, cplx *S)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int pr = w; pr < b * b; pr += nw) {
+        int i = pr % b, j = pr / b;
+        if (i >= j) continue;
+        double sr = 0.0, si = 0.0;
+        for (int r = j + lane; r < m; r += 64) {
+            cplx vi = vpanel(P, M, r, i), vj = vpanel(P, M, r, j);
+            sr += vi.x * vj.x + vi.y * vj.y;
+            si += vi.x * vj.y - vi.y * vj.x;
+        }
+        sr = wave_sum(sr);
+        si = wave_sum(si);
+        if (lane == 0) S[j * PB + i] = make_double2(sr, si);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 0; j < PB; j++)
+            for (int i = 0; i < PB; i++) T[j * PB + i] = make_double2(0.0, 0.0);
+        for (int j = 0; j < b; j++) {
+            cplx tj = tau[j];
+            T[j * PB + j] = tj;
+            for (int i = 0; i < j; i++) {
+                cplx acc = make_double2(0.0, 0.0);
+                for (int q = i; q < j; q++) acc = cadd(acc, cmul(T[q * PB + i], S[j * PB + q]));
+                T[j * PB + i] = cmul(make_double2(-tj.x, -tj.y), acc);
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // ------------------------------------------------------------------------------------------
 // QR without pivoting, blocked: A (M x Nc, M >= Nc, lda) -> reflectors below the diagonal and
 // R on and above it; tau[Nc]; the panel's T factor to tblk[panel]; pb = panel width <= PB;
 // lds holds M x pb panel + 2 PB x PB; wk holds PB x Nc.
 // ------------------------------------------------------------------------------------------
-__device__ static void qr_blk_body(cplx *A, int M, int Nc, int lda, cplx *tau_out, cplx *tblk, cplx *wk, int pb,
+__device__ __forceinline__ void qr_blk_body(cplx *A, int M, int Nc, int lda, cplx *tau_out, cplx *tblk, cplx *wk, int pb,
                                    cplx *lds)
 {
     cplx *P = lds;                      // M x pb panel, ld M
@@ -214,35 +249,7 @@ __device__ static void qr_blk_body(cplx *A, int M, int Nc, int lda, cplx *tau_ou
             }
             __syncthreads();
         }
-        // T factor (zlarft, forward, columnwise): S = V^H V, T(:,j) = -tau_j T(:,0:j) S(0:j, j)
-        for (int pr = w; pr < b * b; pr += nw) {
-            int i = pr % b, j = pr / b;
-            if (i >= j) continue;
-            double sr = 0.0, si = 0.0;
-            for (int r = j + lane; r < m; r += 64) {
-                cplx vi = vpanel(P, M, r, i), vj = vpanel(P, M, r, j);
-                sr += vi.x * vj.x + vi.y * vj.y;
-                si += vi.x * vj.y - vi.y * vj.x;
-            }
-            sr = wave_sum(sr);
-            si = wave_sum(si);
-            if (lane == 0) S[j * PB + i] = make_double2(sr, si);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int j = 0; j < PB; j++)
-                for (int i = 0; i < PB; i++) T[j * PB + i] = make_double2(0.0, 0.0);
-            for (int j = 0; j < b; j++) {
-                cplx tj = tau_out[k0 + j];
-                T[j * PB + j] = tj;
-                for (int i = 0; i < j; i++) {
-                    cplx acc = make_double2(0.0, 0.0);
-                    for (int q = i; q < j; q++) acc = cadd(acc, cmul(T[q * PB + i], S[j * PB + q]));
-                    T[j * PB + i] = cmul(make_double2(-tj.x, -tj.y), acc);
-                }
-            }
-        }
-        __syncthreads();
+        panel_tfactor(P, M, m, b, tau_out + k0, T, S);
         for (int e = threadIdx.x; e < PB * PB; e += blockDim.x) tblk[(size_t)(k0 / pb) * PB * PB + e] = T[e];
         for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
             int r = e % m, p = e / m;
@@ -263,7 +270,7 @@ __device__ static void qr_blk_body(cplx *A, int M, int Nc, int lda, cplx *tau_ou
 
 // Q (M x Nc, ldq) = H_1 ... H_Nc applied to the first Nc columns of the identity: backward
 // accumulation by panels, Q(k0:, k0:) <- (I - V T V^H) Q(k0:, k0:)
-__device__ static void formq_blk_body(const cplx *A, int M, int Nc, int lda, cplx *Q, int ldq, const cplx *tblk,
+__device__ __forceinline__ void formq_blk_body(const cplx *A, int M, int Nc, int lda, cplx *Q, int ldq, const cplx *tblk,
                                       cplx *wk, int pb, cplx *lds)
 {
     cplx *P = lds;
@@ -297,7 +304,7 @@ __device__ static void formq_blk_body(const cplx *A, int M, int Nc, int lda, cpl
 // interchanged with k (0-based); log det as s_inv_logdet_lu_z (Im folded into (-pi, pi]);
 // thread 0 writes *ldet and *info (first zero pivot + 1, or 0)
 // ------------------------------------------------------------------------------------------
-__device__ static void lu_blk_body(cplx *A, int n, int lda, int32_t *piv, cplx *ldet, double *info, cplx *lds)
+__device__ __forceinline__ void lu_blk_body(cplx *A, int n, int lda, int32_t *piv, cplx *ldet, double *info, cplx *lds)
 {
     cplx *P = lds;                       // panel, ld n
     __shared__ double rv[16];
@@ -404,7 +411,7 @@ __device__ static void lu_blk_body(cplx *A, int n, int lda, int32_t *piv, cplx *
 // (P L U) Y = X in place of X (n x nrhs, ldx), with the LU factors of lu_blk_body in A: row
 // interchanges through Y (n x nrhs work, ld n), then block forward and back substitution with
 // rank-16 updates on MFMA; lds holds n x PB + n ints
-__device__ static void solve_blk_body(const cplx *A, int n, int lda, const int32_t *piv, cplx *X, int ldx, int nrhs,
+__device__ __forceinline__ void solve_blk_body(const cplx *A, int n, int lda, const int32_t *piv, cplx *X, int ldx, int nrhs,
                                       cplx *Y, cplx *lds)
 {
     cplx *P = lds;                            // 16 columns of L or U, ld n
@@ -541,4 +548,29 @@ k_solve_blk(const DChain *chains, int xsel)
     extern __shared__ cplx bsh[];
     const DChain &c = chains[blockIdx.x];
     solve_blk_body(c.W3, c.n, c.n, c.piv, chain_mat(c, xsel, 0), c.n, c.n, c.W2, bsh);
+}
+
+// T factors of the reflectors that the column-at-a-time (pivoted) QR left in matrix sel, panel
+// by panel, so that k_formq_blk forms Q by rank-16 MFMA products
+extern "C" __global__ void __launch_bounds__(BLK_NT)
+k_tfactors(const DChain *chains, int sel)
+{
+    extern __shared__ cplx bsh[];
+    const DChain &c = chains[blockIdx.x];
+    const int n = c.n;
+    cplx *P = bsh;
+    cplx *T = bsh + (size_t)n * PB;
+    cplx *S = T + PB * PB;
+    const cplx *A = chain_mat(c, sel, 0);
+    for (int k0 = 0; k0 < n; k0 += PB) {
+        const int m = n - k0, b = min(PB, m);
+        for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
+            int r = e % m, p = e / m;
+            P[p * n + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * n];
+        }
+        __syncthreads();
+        panel_tfactor(P, n, m, b, c.tau + k0, T, S);
+        for (int e = threadIdx.x; e < PB * PB; e += blockDim.x) c.tblk[(size_t)(k0 / PB) * PB * PB + e] = T[e];
+        __syncthreads();
+    }
 }

@@ -130,28 +130,10 @@ __device__ __forceinline__ cplx vtab(const DChain &c, int vsel, int is)
     return b;
 }
 
-// the lcomp (<= 4) entries of one V table in registers: the loads are uniform (scalar) and a
-// per-element lookup is a select, not a memory access
-struct VTab {
-    cplx v[4];
-    __device__ VTab(const DChain &c, int vsel)
-    {
-        for (int k = 0; k < 4; k++) v[k] = (k < c.lcomp) ? vtab(c, vsel, k + 1) : make_double2(0.0, 0.0);
-    }
-    __device__ __forceinline__ cplx operator()(int is) const
-    {
-        cplx r = v[0];
-        r = (is == 2) ? v[1] : r;
-        r = (is == 3) ? v[2] : r;
-        r = (is == 4) ? v[3] : r;
-        return r;
-    }
-};
-
 // one bond per thread (and a stride of columns): the bond's endpoints and 2x2 block are read
 // once per family and applied to every column the thread covers
 template <class Chain>
-__device__ static void cb_left_fams(const Chain &c, cplx *lds, int ncol, const cplx *T, bool rev)
+__device__ __forceinline__ void cb_left_fams(const Chain &c, cplx *lds, int ncol, const cplx *T, bool rev)
 {
     const int nf = c.nn_nf, nl = c.nn_lf, n = c.n;
     const int groups = max(1, (int)blockDim.x / nl);
@@ -177,10 +159,9 @@ __device__ static void cb_left_v(const DChain &c, cplx *lds, int ncol, int nt, i
 {
     const int n = c.n;
     const int32_t *cf = c.conf + (size_t)(nt - 1) * n;
-    const VTab vt(c, vsel);
     for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
         int r = e % n;
-        lds[e] = cmul(vt(cf[r]), lds[e]);
+        lds[e] = cmul(vtab(c, vsel, cf[r]), lds[e]);
     }
     __syncthreads();
 }
@@ -217,7 +198,7 @@ k_cb_cols(const DChain *chains, int msel, int pt, int ksel, int prog, int vsel, 
 }
 
 template <class Chain>
-__device__ static void cb_right_fams(const Chain &c, cplx *lds, int nrow, const cplx *T, bool rev)
+__device__ __forceinline__ void cb_right_fams(const Chain &c, cplx *lds, int nrow, const cplx *T, bool rev)
 {
     const int nf = c.nn_nf, nl = c.nn_lf;
     const int groups = max(1, (int)blockDim.x / nl);
@@ -243,10 +224,9 @@ __device__ static void cb_right_v(const DChain &c, cplx *lds, int nt, int vsel)
 {
     const int n = c.n;
     const int32_t *cf = c.conf + (size_t)(nt - 1) * n;
-    const VTab vt(c, vsel);
     for (int e = threadIdx.x; e < n * RPB; e += blockDim.x) {
         int col = e / RPB;
-        lds[e] = cmul(lds[e], vt(cf[col]));
+        lds[e] = cmul(lds[e], vtab(c, vsel, cf[col]));
     }
     __syncthreads();
 }
@@ -291,9 +271,9 @@ k_vscale(const DChain *chains, int msel, int pt, int side, int vsel, int vsel2, 
     const int32_t *cf = c.conf + (size_t)(nt - 1) * n;
     cplx *M = chain_mat(c, msel, pt);
     cplx v = M[e];
-    if (side == 0 || side == 2) v = cmul(VTab(c, vsel)(cf[r]), v);
-    if (side == 1) v = cmul(v, VTab(c, vsel)(cf[col]));
-    if (side == 2) v = cmul(v, VTab(c, vsel2)(cf[col]));
+    if (side == 0 || side == 2) v = cmul(vtab(c, vsel, cf[r]), v);
+    if (side == 1) v = cmul(v, vtab(c, vsel, cf[col]));
+    if (side == 2) v = cmul(v, vtab(c, vsel2, cf[col]));
     M[e] = v;
 }
 
@@ -315,7 +295,7 @@ __device__ __forceinline__ double4_t mfma(double a, double b, double4_t c)
 // one 64 x 64 tile (tm, tn) of C (m x n, ldc) = alpha op(A) op(B) + beta C, op(A) m x kk,
 // op(B) kk x n; op = 0: as stored, 1: conjugate transpose
 template <int OPA, int OPB>
-__device__ static void zgemm_tile(const cplx *A, int lda, const cplx *B, int ldb, cplx *C, int ldc, int m, int n,
+__device__ __forceinline__ void zgemm_tile(const cplx *A, int lda, const cplx *B, int ldb, cplx *C, int ldc, int m, int n,
                                   int kk, int tm, int tn, double ar, double ai, double br, double bi)
 {
     __shared__ double sAr[GT][GK + 1], sAi[GT][GK + 1];
@@ -514,6 +494,7 @@ struct HsTab {
     int lcomp, nflr;
     double gaml[4];
     cplx delta_bmat[12], phase_ratio[12];
+    cplx qf[12];   // gaml(isp)/gaml(is) phase_ratio(iflip, is): the field-only factor of ratiotot
 };
 
 __device__ static void hs_tab_load(HsTab &t, const DChain &c)
@@ -523,7 +504,26 @@ __device__ static void hs_tab_load(HsTab &t, const DChain &c)
         t.nflr = c.nflr;
         for (int k = 0; k < 4; k++) t.gaml[k] = c.gaml[k];
         for (int k = 0; k < 12; k++) { t.delta_bmat[k] = c.delta_bmat[k]; t.phase_ratio[k] = c.phase_ratio[k]; }
+        for (int is = 1; is <= c.lcomp; is++)
+            for (int fl = 1; fl < c.lcomp; fl++) {
+                const int k = (fl - 1) + (c.lcomp - 1) * (is - 1), isp = (is + fl - 1) % c.lcomp + 1;
+                t.qf[k] = cscale(c.phase_ratio[k], c.gaml[isp - 1] / c.gaml[is - 1]);
+            }
     }
+}
+
+// The decision of dqmc_update_delay.f90:55-76 without divisions: ratiotot = ratio1^nflr qf and
+// accept when |Re(ratiotot phase)| > u2 |Re(phase)|, which is |ratio_re| > u2 for the reference's
+// ratio_re = Re(ratiotot phase) / Re(phase).  Only the direction of phase matters here, so the
+// caller may carry it unnormalised.
+__device__ __forceinline__ bool site_accept_fast(const HsTab &t, cplx ratio1, const SiteDraw &d, double u2,
+                                                 cplx phase, cplx *ratiotot)
+{
+    cplx rt = ratio1;
+    for (int f = 1; f < t.nflr; f++) rt = cmul(rt, ratio1);
+    rt = cmul(rt, t.qf[(d.fl - 1) + (t.lcomp - 1) * (d.is - 1)]);
+    *ratiotot = rt;
+    return fabs(cmul(rt, phase).x) > u2 * fabs(phase.x);
 }
 
 template <class Tab>
@@ -1558,9 +1558,12 @@ k_save_point(const DChain *chains, int pt)
 
 // ------------------------------------------------------------------------------------------
 // Delay update with the delayed vectors held in registers (src/dqmc_config_u/dqmc_update_delay.f90:39-128).
-// 2R threads, R = rows rounded up to a wave: thread r < R keeps avec(r, 0:K) and thread R + r
-// keeps bvec(r, 0:K) in registers.  An accepted site broadcasts row s of both through LDS,
-// every thread extends its own row, and avec/bvec reach memory only for the MFMA flush.
+// Wave w covers rows 32w .. 32w+31: lane l < 32 keeps avec(32w + l, 0:K), lane l + 32 keeps
+// bvec of the same row, so a row's two new entries meet through a lane shuffle.  Each site
+// costs one workgroup barrier: the rows avec(s, :) and bvec(s, :) that site s needs are
+// published by row s's lanes at the end of site s - 1 into one of two LDS buffers (by the
+// parity of s), and diagg(s) is only read at site s, so no lane rewrites a value another lane
+// may still be reading.  avec/bvec reach memory only for the MFMA flush.
 // ------------------------------------------------------------------------------------------
 #define DSB 8   // sites per staged block of G columns and rows
 
@@ -1571,18 +1574,16 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
     extern __shared__ cplx sh[];
     const DChain &c = chains[blockIdx.x];
     const int n = c.n, kb = c.nublock;
-    const int R = blockDim.x >> 1;
-    const bool isA = threadIdx.x < R;
-    const int r = isA ? threadIdx.x : threadIdx.x - R;
+    const int lane = threadIdx.x & 63;
+    const bool isA = lane < 32;
+    const int r = 32 * (threadIdx.x >> 6) + (lane & 31);
     cplx *diagg = sh;                   // n
-    cplx *anew = diagg + n;             // n
-    cplx *bnew = anew + n;              // n
-    cplx *gcol = bnew + n;              // DSB x n: G(:, s0+j)
+    cplx *gcol = diagg + n;             // DSB x n: G(:, s0+j)
     cplx *grow = gcol + DSB * n;        // DSB x n: G(s0+j, :)
-    cplx *rowa = grow + DSB * n;        // K
-    cplx *rowb = rowa + K;              // K
-    double *us = (double *)(rowb + K);  // 2n uniforms of this slice
-    int *cfs = (int *)(us + 2 * n);     // n: HS field of this slice
+    cplx *rows = grow + DSB * n;        // 2 buffers x (avec row K, bvec row K)
+    double *us = (double *)(rows + 4 * K);  // 2n uniforms of this slice
+    int *cfs = (int *)(us + 2 * n);     // n: HS field of this slice (read-only here)
+    __shared__ HsTab tab;
     cplx *G = c.G;
     int32_t *cf = c.conf + (size_t)(nt - 1) * n;
     const double *u = c.unif + (size_t)2 * n * slice_in_pass;
@@ -1595,13 +1596,15 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
         cfs[q] = cf[q];
     }
     for (int q = threadIdx.x; q < 2 * n; q += blockDim.x) us[q] = u[q];
-    __shared__ HsTab tab;
+    for (int q = threadIdx.x; q < 4 * K; q += blockDim.x) rows[q] = make_double2(0.0, 0.0);
     hs_tab_load(tab, c);
     int ik = 0, s0 = -1;
     double accm = 0.0;
     for (int s = 0; s < n; s++) {
         if (s0 < 0 || s >= s0 + DSB) {
-            // stage G(:, s..s+DSB) and G(s..s+DSB, :); valid until the next flush changes G
+            // stage G(:, s..s+DSB) and G(s..s+DSB, :); valid until the next flush changes G.
+            // The previous site's lanes may still read the old block.
+            if (s0 >= 0) __syncthreads();
             s0 = s;
             const int nb = min(DSB, n - s);
             for (int e = threadIdx.x; e < nb * n; e += blockDim.x) {
@@ -1620,42 +1623,53 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
         cplx rt;
         // every thread evaluates the same decision; reading it from lane 0 makes the branch
         // and the delay count scalar
-        const bool acc = __builtin_amdgcn_readfirstlane((int)site_accept(tab, ratio1, d, us[2 * s + 1], phase, &rt)) != 0;
+        const bool acc = __builtin_amdgcn_readfirstlane((int)site_accept_fast(tab, ratio1, d, us[2 * s + 1], phase, &rt)) != 0;
         if (acc) {
             accm += 1.0;
-            phase = phase_after(phase, rt);
+            phase = cmul(phase, rt);   // normalised at the next flush
             const cplx sscl = cdiv(d.db, ratio1);
-            if (r == s) {
+            const cplx *rw = rows + (s & 1) * 2 * K;      // avec(s, :), then bvec(s, :)
+            // A lanes: sum_m bvec(s,m) avec(r,m); B lanes: sum_m avec(s,m) bvec(r,m)
+            const cplx *other = isA ? rw + K : rw;
+            // over all K entries without a branch: reg[m] is zero for m >= ik (cleared at each
+            // flush), so stale buffer entries there contribute nothing, and the loads can all
+            // be in flight at once
+            double vr = 0.0, vi = 0.0, wr = 0.0, wi = 0.0;
 #pragma unroll
-                for (int m = 0; m < K; m++)
-                    if (m < ik) (isA ? rowa : rowb)[m] = reg[m];
+            for (int m = 0; m < K; m += 2) {
+                const cplx o0 = other[m], o1 = other[m + 1];
+                vr += o0.x * reg[m].x - o0.y * reg[m].y;
+                vi += o0.x * reg[m].y + o0.y * reg[m].x;
+                wr += o1.x * reg[m + 1].x - o1.y * reg[m + 1].y;
+                wi += o1.x * reg[m + 1].y + o1.y * reg[m + 1].x;
             }
-            __syncthreads();
+            vr += wr;
+            vi += wi;
+            cplx v = make_double2(0.0, 0.0);
             if (r < n) {
-                cplx v;
                 if (isA) {
-                    v = gcol[(s - s0) * n + r];
-#pragma unroll
-                    for (int m = 0; m < K; m++)
-                        if (m < ik) v = cadd(v, cmul(rowb[m], reg[m]));
-                    v = cmul(v, sscl);
-                    anew[r] = v;
+                    const cplx gc = gcol[(s - s0) * n + r];
+                    v = cmul(make_double2(gc.x + vr, gc.y + vi), sscl);
                 } else {
-                    v = grow[(s - s0) * n + r];
-#pragma unroll
-                    for (int m = 0; m < K; m++)
-                        if (m < ik) v = cadd(v, cmul(rowa[m], reg[m]));
+                    const cplx gr = grow[(s - s0) * n + r];
+                    v = make_double2(gr.x + vr, gr.y + vi);
                     if (r == s) v.x -= 1.0;
-                    bnew[r] = v;
                 }
-#pragma unroll
-                for (int m = 0; m < K; m++)
-                    if (m == ik) reg[m] = v;
             }
-            __syncthreads();
-            if (isA && r < n) diagg[r] = cadd(diagg[r], cmul(anew[r], bnew[r]));
-            if (threadIdx.x == 0) { cf[s] = d.isp; cfs[s] = d.isp; }
+            // the row's other entry from the partner lane (A <-> B)
+            const cplx w = make_double2(__shfl_xor(v.x, 32, 64), __shfl_xor(v.y, 32, 64));
+            if (isA && r > s && r < n) diagg[r] = cadd(diagg[r], cmul(v, w));
+#pragma unroll
+            for (int m = 0; m < K; m++)
+                if (m == ik) reg[m] = v;
+            if (threadIdx.x == 0) cf[s] = d.isp;
             ik++;
+        }
+        // row s+1 publishes its vectors for the next site
+        if (r == s + 1 && r < n) {
+            cplx *pw = rows + ((s + 1) & 1) * 2 * K + (isA ? 0 : K);
+#pragma unroll
+            for (int m = 0; m < K; m++) pw[m] = reg[m];
         }
         if (ik == kb || s == n - 1) {
             __syncthreads();
@@ -1674,6 +1688,7 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
                 wg_rank_k_strips<K>(G, n, c.avec, c.bvec, ik);
             }
             __syncthreads();
+            phase = cscale(phase, 1.0 / sqrt(cabs2(phase)));
             ik = 0;
             s0 = -1;
             if (s < n - 1)

@@ -328,22 +328,39 @@ kp_stock_bonds(const PChain *chains, int nt, int nf, int ib0, int nb, int uoff)
         const int ik = s_ik;
         const int s1 = 2 * q, s2 = s1 + 1;
         if (ik > 0) {
-            // ikk = Gamma^-1 PFP_ikk (ik x 2)
-            for (int e = threadIdx.x; e < 2 * ik; e += blockDim.x) {
-                const int m = e % ik, col = e / ik;
+            // ikk = Gamma^-1 PFP_ikk (ik x 2): two threads per entry, each half of the sum
+            {
+                const int o = threadIdx.x >> 1, h = threadIdx.x & 1;
+                const int m = o % ik, col = o / ik;
                 const int sc_ = col ? s2 : s1;
-                cplx a = make_double2(0.0, 0.0);
-                for (int p = 0; p < ik; p++) a = cadd(a, cmul(Gi[m + p * ns], F[svec[p] + sc_ * ns]));
-                ikk[m][col] = a;
+                double ar = 0.0, ai = 0.0;
+                if (o < 2 * ik)
+                    for (int p = h; p < ik; p += 2) {
+                        const cplx g = Gi[m + p * ns], f = F[svec[p] + sc_ * ns];
+                        ar += g.x * f.x - g.y * f.y;
+                        ai += g.x * f.y + g.y * f.x;
+                    }
+                ar += __shfl_xor(ar, 1, 64);
+                ai += __shfl_xor(ai, 1, 64);
+                if (o < 2 * ik && h == 0) ikk[m][col] = make_double2(ar, ai);
             }
             __syncthreads();
-            // vuk = PFP_kik ikk (2 x 2)
-            if (threadIdx.x < 4) {
-                const int r = threadIdx.x & 1, col = threadIdx.x >> 1;
-                const int sr = r ? s2 : s1;
-                cplx a = make_double2(0.0, 0.0);
-                for (int m = 0; m < ik; m++) a = cadd(a, cmul(F[sr + svec[m] * ns], ikk[m][col]));
-                vuk[r + 2 * col] = a;
+            // vuk = PFP_kik ikk (2 x 2): one wave per entry, one lane per term
+            {
+                const int w = threadIdx.x >> 6, m = threadIdx.x & 63;
+                if (w < 4) {
+                    const int r = w & 1, col = w >> 1;
+                    const int sr = r ? s2 : s1;
+                    double ar = 0.0, ai = 0.0;
+                    if (m < ik) {
+                        const cplx f = F[sr + svec[m] * ns], x = ikk[m][col];
+                        ar = f.x * x.x - f.y * x.y;
+                        ai = f.x * x.y + f.y * x.x;
+                    }
+                    ar = wave_sum(ar);
+                    ai = wave_sum(ai);
+                    if (m == 0) vuk[r + 2 * col] = make_double2(ar, ai);
+                }
             }
         } else if (threadIdx.x < 4) {
             vuk[threadIdx.x] = make_double2(0.0, 0.0);
@@ -383,13 +400,21 @@ kp_stock_bonds(const PChain *chains, int nt, int nf, int ib0, int nb, int uoff)
         __syncthreads();
         if (!s_acc) continue;
         if (ik > 0) {
-            // kik = PFP_kik Gamma^-1 (2 x ik)
-            for (int e = threadIdx.x; e < 2 * ik; e += blockDim.x) {
-                const int r = e % 2, p = e / 2;
+            // kik = PFP_kik Gamma^-1 (2 x ik): two threads per entry
+            {
+                const int o = threadIdx.x >> 1, h = threadIdx.x & 1;
+                const int r = o % 2, p = o / 2;
                 const int sr = r ? s2 : s1;
-                cplx a = make_double2(0.0, 0.0);
-                for (int m = 0; m < ik; m++) a = cadd(a, cmul(F[sr + svec[m] * ns], Gi[m + p * ns]));
-                kik[r][p] = a;
+                double ar = 0.0, ai = 0.0;
+                if (o < 2 * ik)
+                    for (int m = h; m < ik; m += 2) {
+                        const cplx f = F[sr + svec[m] * ns], g = Gi[m + p * ns];
+                        ar += f.x * g.x - f.y * g.y;
+                        ai += f.x * g.y + f.y * g.x;
+                    }
+                ar += __shfl_xor(ar, 1, 64);
+                ai += __shfl_xor(ai, 1, 64);
+                if (o < 2 * ik && h == 0) kik[r][p] = make_double2(ar, ai);
             }
             __syncthreads();
             // 21 block = -gkk kik

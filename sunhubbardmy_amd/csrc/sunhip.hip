@@ -235,8 +235,13 @@ static void udv_step(sh_ctx *ctx, bool first, int pt, int dpt)
         }
         {
             Tic t(ctx, T_FORMQ);
-            hipLaunchKernelGGL(k_formq, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * sizeof(cplx), ctx->st, ctx->dc,
-                               M_W1, pt);
+            if (blocked(ctx)) {
+                hipLaunchKernelGGL(k_tfactors, dim3(ctx->nch), dim3(BLK_NT), lds_blk(ctx), ctx->st, ctx->dc, M_W1);
+                hipLaunchKernelGGL(k_formq_blk, dim3(ctx->nch), dim3(BLK_NT), lds_blk(ctx), ctx->st, ctx->dc, M_W1, pt);
+            } else {
+                hipLaunchKernelGGL(k_formq, dim3(ctx->nch), dim3(1024), (size_t)ctx->nmax * sizeof(cplx), ctx->st,
+                                   ctx->dc, M_W1, pt);
+            }
         }
         Tic t(ctx, T_OTHER);
         hipLaunchKernelGGL(k_v_from_qrcp, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, M_W1, pt);
@@ -356,11 +361,12 @@ static int delay_reg_depth(sh_ctx *ctx, int *threads)
     int kmax = 0;
     for (auto &c : ctx->hc) kmax = std::max(kmax, c.nublock);
     int kt = kmax <= 4 ? 4 : kmax <= 8 ? 8 : kmax <= 16 ? 16 : kmax <= 32 ? 32 : 0;
-    int rows = (ctx->nmax + 63) / 64 * 64;
+    int rows = (ctx->nmax + 31) / 32 * 32;
     int nt = 2 * rows;
     if (kt == 0 || nt > (kt >= 8 ? 512 : 1024)) return 0;
-    // LDS: diag, two exchange vectors, DSB staged columns and rows, uniforms and HS field
-    if ((size_t)(3 + 2 * DSB) * ctx->nmax * sizeof(cplx) + (size_t)ctx->nmax * 20 > 150 * 1024) return 0;
+    // LDS: diag, DSB staged columns and rows, two row buffers, uniforms and HS field
+    if ((size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + 4 * kt * sizeof(cplx) + (size_t)ctx->nmax * 20 > 150 * 1024)
+        return 0;
     *threads = nt;
     return kt;
 }
@@ -371,7 +377,7 @@ static void update_slice(sh_ctx *ctx, int nt, int slice_in_pass)
     int nthr = 0;
     int kt = (ctx->update == SH_U_DELAY && !ctx->legacy_update) ? delay_reg_depth(ctx, &nthr) : 0;
     if (kt) {
-        size_t lds = (size_t)(3 * ctx->nmax + 2 * DSB * ctx->nmax + 2 * kt) * sizeof(cplx) + (size_t)ctx->nmax * 20;
+        size_t lds = (size_t)(ctx->nmax + 2 * DSB * ctx->nmax + 4 * kt) * sizeof(cplx) + (size_t)ctx->nmax * 20;
         switch (kt) {
         case 4: hipLaunchKernelGGL(k_update_delay_reg<4>, dim3(ctx->nch), dim3(nthr), lds, ctx->st, ctx->dc, nt, slice_in_pass); break;
         case 8: hipLaunchKernelGGL(k_update_delay_reg<8>, dim3(ctx->nch), dim3(nthr), lds, ctx->st, ctx->dc, nt, slice_in_pass); break;
@@ -455,12 +461,13 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
     {
         struct { const void *f; size_t bytes; } big[] = {
             {(const void *)k_cb_rows, lds_rows(ctx)}, {(const void *)k_cb_cols, lds_cols(ctx)},
-            {(const void *)k_update_delay_reg<4>, (size_t)(3 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 8 * sizeof(cplx)},
-            {(const void *)k_update_delay_reg<8>, (size_t)(3 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 16 * sizeof(cplx)},
-            {(const void *)k_update_delay_reg<16>, (size_t)(3 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 32 * sizeof(cplx)},
-            {(const void *)k_update_delay_reg<32>, (size_t)(3 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 64 * sizeof(cplx)},
+            {(const void *)k_update_delay_reg<4>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 16 * sizeof(cplx)},
+            {(const void *)k_update_delay_reg<8>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 32 * sizeof(cplx)},
+            {(const void *)k_update_delay_reg<16>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 64 * sizeof(cplx)},
+            {(const void *)k_update_delay_reg<32>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 128 * sizeof(cplx)},
             {(const void *)k_qr_blk, blocked(ctx) ? lds_blk(ctx) : 0},
             {(const void *)k_formq_blk, blocked(ctx) ? lds_blk(ctx) : 0},
+            {(const void *)k_tfactors, blocked(ctx) ? lds_blk(ctx) : 0},
             {(const void *)k_lu_blk, blocked(ctx) ? lds_blk(ctx) : 0},
             {(const void *)k_solve_blk, blocked(ctx) ? lds_blk(ctx) : 0}};
         for (auto &b : big) {
