@@ -8,7 +8,13 @@ chain's SFMT stream.  A step is one sweep (b0 pass + 0b pass, measurement on eve
 every chain.  Metric: tau_int-corrected samples/s = chain sweeps/s / (2 tau_int), tau_int the
 largest of double occupancy, kinetic energy and S(pi,pi) from the per-sweep bins.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--chains C] [--nublock k]
+Learned vs plain: the warm-up sweeps (plain delay update) feed every chain's configuration after
+every pass to the effective-action network (40 sweeps x 2 passes x 256 chains = 20480
+configurations), which the library's FP32 kernels then train.  The same chains are timed first
+with the plain sweep, then with block proposals after every slice's local update; value is the
+learned form's tau_int-corrected samples/s, and the line carries both phases and their ratio.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--chains C] [--nublock k] [--proposal both]
 """
 import argparse
 import json
@@ -69,6 +75,17 @@ def cpu_baseline(kw, tau):
             "sweeps_per_s": sweeps_per_s}
 
 
+def copy_calibration():
+    """Three copies of 512 MiB between two device buffers, for counter calibration."""
+    import torch
+    a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": a.numel() * 8, "launches": 3}), flush=True)
+
+
 def bench_pqmc(args):
     """Projective t-V sweep: honeycomb 18x18 (N=648, ne=324), t=1, V=1.2, theta=20 (Ltrot=200),
     dtau=0.1, rndness 1e-4, nwrap=10, dense kinetic term, submatrix-LR with the reference's
@@ -78,7 +95,7 @@ def bench_pqmc(args):
 
     chains = args.chains if args.chains != 256 else 16
     steps = args.steps if args.steps != 60 else 2
-    warmup = args.warmup if args.warmup != 30 else 1
+    warmup = args.warmup if args.warmup != 40 else 1
     kw = dict(lattice="honeycomb", la=18, lb=18, beta=20.0, dtau=0.1, rt=1.0, rv=1.2, rndness=1e-4, kmode="dense",
               nwrap=10, nustock=0, mid_len=1, wrap_check=False)
     w = ProjWalkers([ProjModel(seed=chain_seed(c), **kw) for c in range(chains)])
@@ -111,20 +128,71 @@ def bench_pqmc(args):
     print(json.dumps(line), flush=True)
 
 
+def timed_phase(w, steps, timers, label):
+    """steps sweeps with measurement, bracketed by device synchronisation; per-sweep bins."""
+    from sunhubbardmy_amd.autocorr import tau_int
+    w.reset_obs()
+    w.reset_diag()
+    w.synchronize()
+    w.timers(enable=timers)
+    series = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        w.sweep(lupdate=True, lmeasure=True)
+        z, nobs = w.obs_all(reset=True)
+        series.append(np.real(z) / nobs[:, None])
+        if s % 10 == 0:
+            print("# %s step %d" % (label, s), file=sys.stderr, flush=True)
+    w.synchronize()
+    dt = time.perf_counter() - t0
+    tm = w.timers(enable=False)
+    series = np.array(series)            # [steps, chains, 5]
+    taus = {}
+    for name, col in (("dblocc", 3), ("ekin", 2), ("s_pipi", 4)):
+        tau, win, _ = tau_int(series[:, :, col].T)
+        taus[name] = round(tau, 3)
+    tau = max(taus.values())
+    sweeps_per_s = w.nchains * steps / dt
+    nd = min(w.nchains, 8)
+    acc = np.mean([w.diag(c)["accepted"] / max(w.diag(c)["tried"], 1) for c in range(nd)])
+    means = series.mean(axis=(0, 1))
+    return {"dt": dt, "tm": tm, "tau": tau, "taus": taus, "sweeps_per_s": sweeps_per_s,
+            "samples_per_s": sweeps_per_s / (2.0 * tau), "acc": float(acc), "means": means}
+
+
+def read_traffic(chains, n, nublock):
+    """HBM bytes per launch of the update kernel from the committed counter summary, when the
+    run is the workload it was measured on."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_ft_hubbard16x16_256chains.txt")
+    if not (chains == 256 and n == 256 and nublock == 32) or not os.path.exists(path):
+        return None
+    kv = dict(l.strip().split("=", 1) for l in open(path) if "=" in l and not l.startswith("#"))
+    return float(kv["fetch_bytes"]) + float(kv["write_bytes"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--chains", type=int, default=256)
     ap.add_argument("--nublock", type=int, default=32)
+    ap.add_argument("--nprop", type=int, default=64, help="inner Metropolis attempts per block proposal (<= 64)")
+    ap.add_argument("--epochs", type=int, default=30, help="training epochs of the effective action")
+    ap.add_argument("--proposal", choices=("both", "plain", "learned"), default="both",
+                    help="time the plain delay sweep, the learned-proposal sweep, or both (value = learned)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timers", dest="timers", action="store_false",
                     help="skip the per-kernel hipEvent timing of the timed steps")
     ap.add_argument("--workload", choices=("ft", "pqmc"), default="ft",
                     help="ft: the headline finite-temperature workload; pqmc: the projective t-V workload "
                          "(honeycomb L=18, N=648, 324 particles, V=1.2, theta=20, submatrix-LR)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 512 MiB device-to-device copies (the known-size kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; chains are independent, so N GPUs would run N "
               "independent batches (no collective on the sweep path). Use --gpus 1.", file=sys.stderr)
@@ -133,7 +201,7 @@ def main():
         return bench_pqmc(args)
 
     from sunhubbardmy_amd import FTWalkers, Model, chain_seed
-    from sunhubbardmy_amd.autocorr import tau_int
+    from sunhubbardmy_amd.learned import EffectiveAction
 
     kw = model_kw(args)
     models = [Model(seed=chain_seed(c), **kw) for c in range(args.chains)]
@@ -142,69 +210,89 @@ def main():
     n, ltrot = t["ndim"], t["ltrot"]
     print("# %d chains, N=%d, Ltrot=%d, nublock=%d" % (args.chains, n, ltrot, t["nublock"]), file=sys.stderr,
           flush=True)
+    w.set_neighbours()
     w.init_conf()
     w.sweep_start_0b()
+    # warm-up with the plain sweep; its configurations after every pass train the effective action
+    learn = args.proposal != "plain"
+    net = EffectiveAction(capacity=max(2 * args.warmup * args.chains, 1), seed=chain_seed(0)) if learn else None
     for s in range(args.warmup):
-        w.sweep(lupdate=True, lmeasure=False)
+        w.sweep_b0(True, False)
+        if learn:
+            net.add_samples(w)
+        w.sweep_0b(True, False)
+        if learn:
+            net.add_samples(w)
         if s % 10 == 0:
             print("# warmup %d" % s, file=sys.stderr, flush=True)
-    w.reset_obs()
-    w.reset_diag()
-    w.synchronize()
-    w.timers(enable=args.timers)
-    series = []
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        w.sweep(lupdate=True, lmeasure=True)
-        z, nobs = w.obs_all(reset=True)
-        series.append(np.real(z) / nobs[:, None])
-        if s % 10 == 0:
-            print("# step %d" % s, file=sys.stderr, flush=True)
-    w.synchronize()
-    t1 = time.perf_counter()
-    tm = w.timers(enable=False)
-    dt = t1 - t0
-    series = np.array(series)            # [steps, chains, 5]
-    taus = {}
-    for name, col in (("dblocc", 3), ("ekin", 2), ("s_pipi", 4)):
-        tau, win, _ = tau_int(series[:, :, col].T)
-        taus[name] = round(tau, 3)
-    tau = max(taus.values())
-    sweeps_per_s = args.chains * args.steps / dt
-    value = sweeps_per_s / (2.0 * tau)
-    acc = np.mean([w.diag(c)["accepted"] / max(w.diag(c)["tried"], 1) for c in range(min(args.chains, 8))])
-    means = series.mean(axis=(0, 1))
+    train = None
+    if learn:
+        t0 = time.perf_counter()
+        loss = net.train(epochs=args.epochs, batch=512, lr=1e-3)
+        train = {"samples": net.num_samples(), "epochs": args.epochs, "seconds": round(time.perf_counter() - t0, 3),
+                 "loss_first_epoch": float(loss[0]), "loss_last_epoch": float(loss[-1])}
+        print("# trained on %d configurations: loss %.4g -> %.4g" % (net.num_samples(), loss[0], loss[-1]),
+              file=sys.stderr, flush=True)
+    plain = learned = None
+    if args.proposal in ("both", "plain"):
+        plain = timed_phase(w, args.steps, args.timers, "plain")
+    if learn:
+        w.set_proposal(net, nprop=args.nprop, local_updates=True)
+        w.reset_proposal_stats()
+        learned = timed_phase(w, args.steps, args.timers, "learned")
+        pst = [w.proposal_stats(c) for c in range(min(args.chains, 16))]
+        learned["block"] = {
+            "acceptance": sum(x["accepted"] for x in pst) / max(sum(x["moves"] for x in pst), 1),
+            "spins_per_accepted_block": sum(x["spins_accepted"] for x in pst) / max(sum(x["accepted"] for x in pst), 1),
+            "spins_proposed_per_block": sum(x["spins_proposed"] for x in pst) / max(sum(x["moves"] for x in pst), 1)}
+    head = learned if learned is not None else plain
+    wl = ("SU(2) Hubbard square 16x16 (N=256), U=6, t=1, half filling, beta=8, dtau=0.1 (Ltrot=80), nwrap=10, "
+          "checkerboard K, delay update k=%d" % t["nublock"])
+
+    def summary(r):
+        return {"samples_per_s": r["samples_per_s"], "sweeps_per_s": r["sweeps_per_s"], "tau_int": r["taus"],
+                "acceptance": round(r["acc"], 4), "ms_per_step": 1000.0 * r["dt"] / args.steps,
+                "observables": {"dblocc_per_site": r["means"][3] / n, "ekin_per_site_flavour": r["means"][2] / n,
+                                "s_pipi": r["means"][4]}}
+
     line = {
-        "metric": METRIC, "value": value, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1000.0 * dt / args.steps, "higher_is_better": True,
+        "metric": METRIC, "value": head["samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1000.0 * head["dt"] / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "SU(2) Hubbard square 16x16 (N=256), U=6, t=1, half filling, beta=8, dtau=0.1 "
-                               "(Ltrot=80), nwrap=10, checkerboard K, delay update, plain local sweeps",
-                   "chains": args.chains, "nublock": t["nublock"], "proposal": "plain (learned proposal not built)"},
-        "sweeps_per_s": sweeps_per_s, "tau_int": taus, "acceptance": round(float(acc), 4),
-        "observables": {"dblocc_per_site": means[3] / n, "ekin_per_site_flavour": means[2] / n,
-                        "s_pipi": means[4]},
+        "config": {"workload": wl, "chains": args.chains, "nublock": t["nublock"],
+                   "proposal": "learned (block moves of %d attempts after each slice's local update)" % args.nprop
+                   if learned is not None else "plain local delay update"},
     }
+    if plain is not None:
+        line["plain"] = summary(plain)
+    if learned is not None:
+        line["learned"] = summary(learned)
+        line["learned"]["block"] = learned["block"]
+        line["training"] = train
+    if plain is not None and learned is not None:
+        line["learned_over_plain"] = learned["samples_per_s"] / plain["samples_per_s"]
     if args.timers:
-        a = float(acc)
+        tm = head["tm"]
+        a = head["acc"]
         best = max(tm, key=lambda g: tm[g][0])
         ms, cnt = tm[best]
         kind, amount = algorithmic_work(best, n, args.chains, ltrot, 10, a, t["nublock"])
         avg_s = ms / 1000.0 / max(cnt, 1)
         achieved = amount / avg_s
+        traffic = read_traffic(args.chains, n, t["nublock"]) if best == "update" else None
         if kind == "flops":
             roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
-                    "frac": achieved / FP64_MFMA_PEAK, "traffic": None}
+                    "frac": achieved / FP64_MFMA_PEAK, "traffic": traffic}
         else:
             roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK, "traffic": None}
+                    "frac": achieved / HBM_PEAK, "traffic": traffic}
         roof["kernel_group"] = best
         roof["avg_launch_ms"] = ms / max(cnt, 1)
         line["roofline"] = roof
         line["kernel_ms"] = {g: round(v[0], 3) for g, v in tm.items()}
         line["kernel_launches"] = {g: v[1] for g, v in tm.items()}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(kw, tau)
+        line["cpu_baseline"] = cpu_baseline(kw, (plain or head)["tau"])
     print(json.dumps(line), flush=True)
 
 

@@ -111,8 +111,8 @@ int sh_set_trace(sh_ctx *ctx, int chain, int cap);
 int sh_get_trace(sh_ctx *ctx, int chain, double *buf, int *count);
 
 /* per-kernel-group device time since the last call, from hipEvent pairs on the library's
- * stream: t[0..9] ms of wrap (K and V on G), update, stabilization B products, QR (prepivot +
- * Householder), form Q, ZGEMM, LU, LU solve, measure, other; t[10..19] launch counts of the
+ * stream: t[0..10] ms of wrap (K and V on G), update, stabilization B products, QR (prepivot +
+ * Householder), form Q, ZGEMM, LU, LU solve, measure, other, block proposals; t[11..21] launch counts of the
  * same groups.  enable = 0 stops recording (events cost a little launch overhead). */
 int sh_timers(sh_ctx *ctx, int enable, double *t);
 int sh_synchronize(sh_ctx *ctx);

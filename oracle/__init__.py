@@ -75,6 +75,11 @@ def lib():
         L.orc_pset_trace.argtypes = [P, dp, ctypes.c_int]
         L.orc_ptrace_count.argtypes = [P]
         L.orc_pget_table.argtypes = [P, ctypes.c_int, dp]
+        L.orc_set_proposal.argtypes = [P, ctypes.c_int, ip, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_float), dp]
+        L.orc_set_proposal.restype = None
+        L.orc_get_proposal_stats.argtypes = [P, dp]
+        L.orc_get_proposal_stats.restype = None
         L.orc_qr_pqmc.argtypes = [ctypes.c_int, ctypes.c_int, dp]
         L.orc_qr_pqmc.restype = ctypes.c_double
         _LIB = L
@@ -207,6 +212,23 @@ class OracleChain:
         n = lib().orc_trace_count(self._c)
         # snapshots are stored column-major per matrix
         return np.array([t.reshape(self.ndim, self.ndim).T for t in self._trace[:n]])
+
+    def set_proposal(self, nbr, nprop, weights, norm, local_updates=True):
+        """Block proposals restated from the HIP library (weights=None switches them off)."""
+        if weights is None:
+            lib().orc_set_proposal(self._c, 0, None, 0, 1, None, None)
+            return
+        self._pnbr = np.ascontiguousarray(nbr, np.int32)
+        self._pw = np.ascontiguousarray(weights, np.float32)
+        self._pnorm = np.ascontiguousarray(norm, np.float64)
+        lib().orc_set_proposal(self._c, self._pnbr.shape[1], _iptr(self._pnbr), int(nprop), int(local_updates),
+                               self._pw.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _dptr(self._pnorm))
+
+    def proposal_stats(self):
+        s = np.zeros(5)
+        lib().orc_get_proposal_stats(self._c, _dptr(s))
+        return {"moves": s[0], "accepted": s[1], "spins_proposed": s[2], "spins_accepted": s[3],
+                "inner_accepted": s[4]}
 
     def table(self, kind):
         buf = np.zeros(2 * max(self.ndim * self.ndim, 4 * self.nn_nf * self.nn_lf, 64) + 64)
@@ -362,6 +384,23 @@ class OracleProjChain:
     def trace(self):
         n = lib().orc_ptrace_count(self._c)
         return np.array([t.reshape(self.ndim, self.ndim).T for t in self._trace[:n]])
+
+    def set_proposal(self, nbr, nprop, weights, norm, local_updates=True):
+        """Block proposals restated from the HIP library (weights=None switches them off)."""
+        if weights is None:
+            lib().orc_set_proposal(self._c, 0, None, 0, 1, None, None)
+            return
+        self._pnbr = np.ascontiguousarray(nbr, np.int32)
+        self._pw = np.ascontiguousarray(weights, np.float32)
+        self._pnorm = np.ascontiguousarray(norm, np.float64)
+        lib().orc_set_proposal(self._c, self._pnbr.shape[1], _iptr(self._pnbr), int(nprop), int(local_updates),
+                               self._pw.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _dptr(self._pnorm))
+
+    def proposal_stats(self):
+        s = np.zeros(5)
+        lib().orc_get_proposal_stats(self._c, _dptr(s))
+        return {"moves": s[0], "accepted": s[1], "spins_proposed": s[2], "spins_accepted": s[3],
+                "inner_accepted": s[4]}
 
     def table(self, kind):
         buf = np.zeros(2 * max(self.ndim * self.ndim, 4 * self.nn_nf * self.nn_lf, 64) + 64)

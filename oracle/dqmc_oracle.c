@@ -449,6 +449,13 @@ struct orc_ctx {
     zc obs[5];
     double *trace;
     int trace_cap, trace_n;
+    /* learned block proposals (no counterpart in the reference; see orc_set_proposal) */
+    int prop_on, prop_local, prop_nprop, prop_z;
+    int32_t *prop_nbr;
+    float *prop_w;
+    double prop_norm[10];
+    uint64_t prop_rng;
+    double prop_stats[5];
 };
 
 /* src/lattice.f90:792-961 (square) and :611-776 (honeycomb); 0-based site indices */
@@ -703,6 +710,7 @@ orc_ctx *orc_create(const orc_params *p)
     setup_hs(c);
     setup_kinetic(c);
     sfmt_init(&c->rng, p->seed);
+    c->prop_local = 1;
     int N = c->ndim;
     c->conf = (int32_t *)calloc((size_t)N * c->ltrot, sizeof(int32_t));
     for (size_t k = 0; k < (size_t)N * c->ltrot; k++) c->conf[k] = 1;
@@ -730,6 +738,7 @@ void orc_destroy(orc_ctx *c)
     free(c->urt); free(c->urtm1);
     free(c->nnlf); free(c->nnlist); free(c->cx); free(c->cy); free(c->sub);
     free(c->iwrap_nt); free(c->wrap_step);
+    free(c->prop_nbr); free(c->prop_w);
     free(c);
 }
 
@@ -1031,6 +1040,173 @@ static void do_update(orc_ctx *c, int nt)
 }
 
 /* ------------------------------------------------------------------------------------ */
+/* learned block proposals: a restatement of the block move of the HIP library, which has */
+/* no counterpart in the reference.  It reuses the local update's Delta, field factor and  */
+/* Metropolis test (dqmc_update_delay.f90:45-76) for a whole flipped set F:                 */
+/* R = det(1 + Delta_F (1 - G)_FF)^nflr prod qf, accepted with                              */
+/* min(1, |Re(R phase)| / |Re(phase)| exp(E_eff(C') - E_eff(C))).  The features are         */
+/* recomputed from the whole field at every move (the library updates them per slice).     */
+/* ------------------------------------------------------------------------------------ */
+#define PROP_H 128
+#define PROP_NW (PROP_H * PROP_H + 7 * PROP_H + 1)
+
+static uint64_t smix_next(uint64_t *s)
+{
+    /* splitmix64 (Steele, Lea, Flood 2014) */
+    *s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = *s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static double smix_unif(uint64_t *s) { return (double)(smix_next(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+void orc_set_proposal(orc_ctx *c, int z, const int32_t *nbr, int nprop, int local, const float *w, const double *norm)
+{
+    free(c->prop_nbr);
+    free(c->prop_w);
+    c->prop_nbr = NULL;
+    c->prop_w = NULL;
+    c->prop_on = (w != NULL);
+    c->prop_local = c->prop_on ? local : 1;
+    if (!c->prop_on) return;
+    c->prop_z = z;
+    c->prop_nprop = nprop;
+    c->prop_nbr = (int32_t *)malloc(sizeof(int32_t) * (size_t)c->ndim * z);
+    memcpy(c->prop_nbr, nbr, sizeof(int32_t) * (size_t)c->ndim * z);
+    c->prop_w = (float *)malloc(sizeof(float) * PROP_NW);
+    memcpy(c->prop_w, w, sizeof(float) * PROP_NW);
+    memcpy(c->prop_norm, norm, sizeof(double) * 10);
+    c->prop_rng = 0x9E3779B97F4A7C15ull * (uint64_t)(uint32_t)c->p.seed + 0x632BE59BD9B4E019ull;
+    memset(c->prop_stats, 0, sizeof(c->prop_stats));
+}
+
+void orc_get_proposal_stats(const orc_ctx *c, double *out) { memcpy(out, c->prop_stats, sizeof(c->prop_stats)); }
+
+/* the four global couplings of the field: sum s, sum_i s_i sum_{j~i} s_j, sum s s(t+1),
+ * sum_i s_i sum_{j~i} s_j(t+1) (s = 2 conf - 3, slices periodic) */
+static void prop_totals(const orc_ctx *c, double T[4])
+{
+    int N = c->ndim, lt = c->ltrot, z = c->prop_z;
+    T[0] = T[1] = T[2] = T[3] = 0;
+    for (int t = 0; t < lt; t++) {
+        const int32_t *r0 = c->conf + (size_t)t * N, *r1 = c->conf + (size_t)((t + 1) % lt) * N;
+        for (int i = 0; i < N; i++) {
+            int s = 2 * r0[i] - 3;
+            T[0] += s;
+            T[2] += s * (2 * r1[i] - 3);
+            for (int k = 0; k < z; k++) {
+                int j = c->prop_nbr[(size_t)i * z + k];
+                T[1] += s * (2 * r0[j] - 3);
+                T[3] += s * (2 * r1[j] - 3);
+            }
+        }
+    }
+}
+
+/* g(x) of the 4 -> 128 -> 128 -> 1 tanh network in double from the float weights */
+static double prop_mlp(const orc_ctx *c, const double T[4])
+{
+    const float *w = c->prop_w;
+    const float *W1 = w, *b1 = w + 4 * PROP_H, *W2 = w + 5 * PROP_H, *b2 = W2 + PROP_H * PROP_H, *w3 = b2 + PROP_H,
+                *b3 = w3 + PROP_H;
+    double ns = (double)c->ndim * c->ltrot, nbl = (double)c->ndim * c->prop_z * c->ltrot;
+    double f[4] = {T[0] / ns, T[1] / nbl, T[2] / ns, T[3] / nbl}, x[4], h1[PROP_H];
+    for (int k = 0; k < 4; k++) x[k] = (f[k] - c->prop_norm[k]) / c->prop_norm[4 + k];
+    for (int j = 0; j < PROP_H; j++) {
+        double a = b1[j];
+        for (int k = 0; k < 4; k++) a += (double)W1[4 * j + k] * x[k];
+        h1[j] = tanh(a);
+    }
+    double out = b3[0];
+    for (int j = 0; j < PROP_H; j++) {
+        double a = b2[j];
+        for (int k = 0; k < PROP_H; k++) a += (double)W2[j * PROP_H + k] * h1[k];
+        out += (double)w3[j] * tanh(a);
+    }
+    return out;
+}
+
+static void block_move(orc_ctx *c, int nt)
+{
+    int N = c->ndim, lt = c->ltrot, z = c->prop_z;
+    int32_t *row = c->conf + (size_t)(nt - 1) * N;
+    const int32_t *rm = c->conf + (size_t)((nt - 2 + lt) % lt) * N, *rp = c->conf + (size_t)(nt % lt) * N;
+    int32_t *orig = (int32_t *)malloc(sizeof(int32_t) * N);
+    memcpy(orig, row, sizeof(int32_t) * N);
+    double escale = (double)N * lt * c->prop_norm[9];
+    double T[4];
+    prop_totals(c, T);
+    double g0 = prop_mlp(c, T), gc = g0;
+    int ninner = 0;
+    for (int a = 0; a < c->prop_nprop; a++) {
+        double u1 = smix_unif(&c->prop_rng), u2 = smix_unif(&c->prop_rng);
+        int i = (int)(u1 * N);
+        if (i > N - 1) i = N - 1;
+        int s = 2 * row[i] - 3, nn = 0, st = 0;
+        for (int k = 0; k < z; k++) {
+            int j = c->prop_nbr[(size_t)i * z + k];
+            nn += 2 * row[j] - 3;
+            st += (2 * rm[j] - 3) + (2 * rp[j] - 3);
+        }
+        double T2[4] = {T[0] - 2 * s, T[1] - 4 * s * nn, T[2] - 2 * s * ((2 * rm[i] - 3) + (2 * rp[i] - 3)), T[3] - 2 * s * st};
+        double gn = prop_mlp(c, T2), de = escale * (gn - gc);
+        if (de <= 0.0 || u2 < exp(-de)) {
+            memcpy(T, T2, sizeof(T));
+            gc = gn;
+            ninner++;
+            row[i] = 3 - row[i];
+        }
+    }
+    double u3 = smix_unif(&c->prop_rng);
+    int F[64], m = 0;
+    for (int i = 0; i < N; i++)
+        if (row[i] != orig[i] && m < 64) F[m++] = i;
+    int acc = 0;
+    if (m > 0) {
+        zc *M = zalloc((size_t)m * m), dl[64], ld;
+        zc qprod = 1;
+        for (int a = 0; a < m; a++) {
+            int is = orig[F[a]], isp = 3 - is;
+            dl[a] = c->delta_bmat[is - 1];
+            qprod *= c->phase_ratio[is - 1] * (c->gaml[isp - 1] / c->gaml[is - 1]);
+        }
+        for (int b = 0; b < m; b++)
+            for (int a = 0; a < m; a++)
+                M[IDX(a, b, m)] = (a == b ? 1.0 : 0.0) + dl[a] * ((a == b ? 1.0 : 0.0) - c->G[IDX(F[a], F[b], N)]);
+        int info = zinv_logdet(m, M, &ld);
+        if (!info) {
+            zc ratiotot = cexp((double)c->p.nflr * ld) * qprod;
+            double ratio_re = creal(ratiotot * c->phase) / creal(c->phase);
+            double lacc = log(fabs(ratio_re)) + escale * (gc - g0);
+            acc = fabs(ratio_re) > 0.0 && log(u3) < lacc;
+            if (acc) {
+                /* X = M^-1 Delta_F (E_F - G_F:) (m x N); G <- G - G_:F X */
+                zc *R = zalloc((size_t)m * N), *X = zalloc((size_t)m * N);
+                for (int col = 0; col < N; col++)
+                    for (int a = 0; a < m; a++)
+                        R[IDX(a, col, m)] = dl[a] * ((F[a] == col ? 1.0 : 0.0) - c->G[IDX(F[a], col, N)]);
+                zgemm('N', 'N', m, N, m, M, m, R, m, 0.0, X, m);
+                zc *GF = zalloc((size_t)N * m);
+                for (int a = 0; a < m; a++)
+                    for (int i = 0; i < N; i++) GF[IDX(i, a, N)] = -c->G[IDX(i, F[a], N)];
+                zgemm('N', 'N', N, N, m, GF, N, X, m, 1.0, c->G, N);
+                free(R); free(X); free(GF);
+                accept_phase(c, ratiotot);
+            }
+        }
+        free(M);
+    }
+    if (!acc) memcpy(row, orig, sizeof(int32_t) * N);
+    c->prop_stats[0] += 1;
+    c->prop_stats[1] += acc;
+    c->prop_stats[2] += m;
+    c->prop_stats[3] += acc ? m : 0;
+    c->prop_stats[4] += ninner;
+    free(orig);
+}
+
+/* ------------------------------------------------------------------------------------ */
 /* stabilization and Green's functions                                                    */
 /* ------------------------------------------------------------------------------------ */
 /* s_zmcpt (lib/s_matrix.f90:785-823): columns sorted by decreasing 2-norm (norms computed
@@ -1294,7 +1470,8 @@ int orc_sweep_b0(orc_ctx *c, int lupdate, int lmeasure)
             for (int q = 0; q < nfam; q++) { k_right(c, q, c->urt, c->G); k_left(c, q, c->urtm1, c->G); }
         }
         if (c->lwrapu) {
-            if (lupdate) do_update(c, nt);
+            if (lupdate && c->prop_local) do_update(c, nt);
+            if (lupdate && c->prop_on) block_move(c, nt);
             v_cols(c, nt, c->bmat, c->G);
             v_rows(c, nt, c->bmat_inv, 0, c->G);
         }
@@ -1338,7 +1515,8 @@ int orc_sweep_0b(orc_ctx *c, int lupdate, int lmeasure)
         if (c->lwrapu) {
             v_rows(c, nt, c->bmat, 0, c->G);
             v_cols(c, nt, c->bmat_inv, c->G);
-            if (lupdate) do_update(c, nt);
+            if (lupdate && c->prop_local) do_update(c, nt);
+            if (lupdate && c->prop_on) block_move(c, nt);
         }
         if (c->lwrapT) {
             int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;

@@ -56,6 +56,16 @@ void orc_reset_diag(orc_ctx *c);
 void orc_get_obs(const orc_ctx *c, double *o);
 void orc_reset_obs(orc_ctx *c);
 
+/* Learned block proposals (the HIP library's block move restated; the reference has none):
+ * after the local update of every slice (skipped when local == 0), nprop Metropolis attempts
+ * on E_eff = Ns (ymu + ysd g(x)) at uniform sites of the slice (splitmix64 stream seeded from
+ * the chain seed), then the flipped set is accepted with the exact determinant ratio.
+ * nbr[ndim][z] neighbour lists, w the float network weights, norm[10] = xmu, xsd, ymu, ysd.
+ * w == NULL switches proposals off.  stats[5]: moves, accepted, spins proposed, spins accepted,
+ * inner attempts accepted. */
+void orc_set_proposal(orc_ctx *c, int z, const int32_t *nbr, int nprop, int local, const float *w, const double *norm);
+void orc_get_proposal_stats(const orc_ctx *c, double *stats);
+
 /* per-stabilization trace of G (after replacement by the scratch G): cap snapshots */
 void orc_set_trace(orc_ctx *c, double *buf, int cap);
 int orc_trace_count(const orc_ctx *c);

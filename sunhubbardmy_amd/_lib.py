@@ -42,6 +42,15 @@ EXPORTS_PROJ = [
 ]
 
 
+# include/sunhip_learn.h
+EXPORTS_LEARN = [
+    "shl_create", "shl_destroy", "shl_last_error", "shl_get_weights", "shl_set_weights", "shl_set_norm",
+    "shl_get_norm", "shl_add_samples", "shl_push_samples", "shl_num_samples", "shl_get_samples",
+    "shl_clear_samples", "shl_train", "shl_loss_grad", "shl_predict", "sh_set_neighbours", "sh_features",
+    "sh_set_proposal", "sh_get_proposal_stats", "sh_reset_proposal_stats",
+]
+
+
 def lib():
     global _LIB
     if _LIB is not None:
@@ -99,6 +108,31 @@ def lib():
     L.shp_synchronize.argtypes = [P]
     for name in EXPORTS_PROJ:
         if name != "shp_last_error":
+            getattr(L, name).restype = ctypes.c_int
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.shl_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int32, ctypes.c_int]
+    L.shl_destroy.argtypes = [P]
+    L.shl_last_error.argtypes = [P]
+    L.shl_last_error.restype = ctypes.c_char_p
+    L.shl_get_weights.argtypes = [P, fp]
+    L.shl_set_weights.argtypes = [P, fp]
+    L.shl_set_norm.argtypes = [P, dp]
+    L.shl_get_norm.argtypes = [P, dp]
+    L.shl_add_samples.argtypes = [P, P]
+    L.shl_push_samples.argtypes = [P, dp, ctypes.c_int]
+    L.shl_num_samples.argtypes = [P]
+    L.shl_get_samples.argtypes = [P, dp, ctypes.c_int, ctypes.c_int]
+    L.shl_clear_samples.argtypes = [P]
+    L.shl_train.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
+    L.shl_loss_grad.argtypes = [P, dp, ctypes.c_int, dp, fp]
+    L.shl_predict.argtypes = [P, dp, ctypes.c_int, dp]
+    L.sh_set_neighbours.argtypes = [P, ctypes.c_int, ip]
+    L.sh_features.argtypes = [P, dp]
+    L.sh_set_proposal.argtypes = [P, P, ctypes.c_int, ctypes.c_int]
+    L.sh_get_proposal_stats.argtypes = [P, ctypes.c_int, dp]
+    L.sh_reset_proposal_stats.argtypes = [P]
+    for name in EXPORTS_LEARN:
+        if name != "shl_last_error":
             getattr(L, name).restype = ctypes.c_int
     _LIB = L
     return L

@@ -4,7 +4,9 @@
 #include "kernels.hip"
 #include "blocked.hip"
 #include "pqmc_kernels.hip"
+#include "learn_kernels.hip"
 #include "../../include/sunhip.h"
+#include "../../include/sunhip_learn.h"
 
 #include <cmath>
 #include <cstdio>
@@ -26,6 +28,7 @@ enum {
     T_SOLVE,      // k_lu_solve
     T_MEASURE,    // k_measure
     T_OTHER,      // elementwise helpers, wrap check
+    T_PROPOSAL,   // k_block_move
     T_NCLS
 };
 
@@ -62,6 +65,16 @@ struct sh_ctx : ctx_base {
     // push/pop_stage backups
     std::vector<void *> stage;
     std::vector<cplx *> trace_bufs;
+    std::vector<int32_t> seeds;
+    // learned block proposals (learn.hip)
+    int prop_z = 0;
+    bool prop_on = false, prop_local = true;
+    int32_t *d_nbr = nullptr;
+    double *d_feat = nullptr;
+    std::vector<PropChain> hprop;
+    PropChain *d_prop = nullptr;
+    float *pw = nullptr;
+    PropNet pnet = {};
 };
 
 #define HIPCHK(x)                                                                  \
@@ -410,6 +423,11 @@ static int check_launch(sh_ctx *ctx)
     return 0;
 }
 
+// learned block proposals (learn.hip): the per-pass refresh of the feature sums and one block
+// move of a slice, both no-ops while proposals are off
+static void prop_pass_start(sh_ctx *ctx);
+static void block_move(sh_ctx *ctx, int nt);
+
 // ---- C ABI ---------------------------------------------------------------------------------
 extern "C" {
 
@@ -539,6 +557,7 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
     }
     ctx->dc = dupload<DChain>(ctx, ctx->hc.data(), nchains);
     ctx->d_seeds = dupload<int32_t>(ctx, seeds.data(), nchains);
+    ctx->seeds = seeds;
     if (!ctx->dc || !ctx->d_seeds) { sh_destroy(ctx); return SH_E_HIP; }
     hipLaunchKernelGGL(k_sfmt_seed, dim3((nchains + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, nchains, ctx->d_seeds);
     if (hipStreamSynchronize(ctx->st) != hipSuccess || hipGetLastError() != hipSuccess) {
@@ -626,6 +645,7 @@ int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure)
     set_identity(ctx, ctx->nst);
     const bool upd = lupdate && ctx->hc[0].lwrapu;
     if (upd) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
+    if (upd) prop_pass_start(ctx);
     int slice = 0;
     for (int nt = ctx->ltrot; nt >= 1; nt--, slice++) {
         if (lmeasure) measure(ctx);
@@ -635,7 +655,8 @@ int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure)
             right_pass(ctx, 0, 1, 0, nt);
             left_pass(ctx, M_G, 0, 1, 1, 0, nt, nt, 1);
         }
-        if (upd) update_slice(ctx, nt, slice);
+        if (upd && ctx->prop_local) update_slice(ctx, nt, slice);
+        if (upd) block_move(ctx, nt);
         {
             Tic t(ctx, T_WRAP);
             // V (right_forward bmat, left_backward bmat_inv) then the K-half nn_nf..1
@@ -669,6 +690,7 @@ int sh_sweep_0b(sh_ctx *ctx, int lupdate, int lmeasure, int lmeasure_dyn)
     set_identity(ctx, 0);
     const bool upd = lupdate && ctx->hc[0].lwrapu;
     if (upd) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
+    if (upd) prop_pass_start(ctx);
     for (int nt = 1; nt <= ctx->ltrot; nt++) {
         {
             Tic t(ctx, T_WRAP);
@@ -676,7 +698,8 @@ int sh_sweep_0b(sh_ctx *ctx, int lupdate, int lmeasure, int lmeasure_dyn)
             left_pass(ctx, M_G, 0, 0, 1 | 2, 0, nt, nt, 1);
             right_pass(ctx, 1, 1 | 2, 1, nt);
         }
-        if (upd) update_slice(ctx, nt, nt - 1);
+        if (upd && ctx->prop_local) update_slice(ctx, nt, nt - 1);
+        if (upd) block_move(ctx, nt);
         {
             Tic t(ctx, T_WRAP);
             left_pass(ctx, M_G, 0, 0, 4, 0, nt, nt, 1);
@@ -991,3 +1014,4 @@ extern "C" int sh_dbg_op(sh_ctx *ctx, int op, const int *a)
 
 // ---- projective sweep (include/sunhip_proj.h) -------------------------------------------------
 #include "pqmc.hip"
+#include "learn.hip"

@@ -15,7 +15,7 @@ from . import _lib
 from .model import Model, make_tables
 
 
-TIMER_GROUPS = ["wrap", "update", "bprod", "qr", "formq", "gemm", "lu", "solve", "measure", "other"]
+TIMER_GROUPS = ["wrap", "update", "bprod", "qr", "formq", "gemm", "lu", "solve", "measure", "other", "proposal"]
 
 
 class SweepError(RuntimeError):
@@ -183,3 +183,33 @@ class FTWalkers:
         self._check(_lib.lib().sh_timers(self._ctx, int(enable), t.ctypes.data_as(_lib.dp)), "timers")
         g = len(TIMER_GROUPS)
         return {n: (t[k], int(t[g + k])) for k, n in enumerate(TIMER_GROUPS)}
+
+    # -- learned block proposals (include/sunhip_learn.h) ----------------------------------
+    def set_neighbours(self):
+        """Hand the nearest-neighbour lists of every chain's lattice to the library (features
+        and block proposals need them)."""
+        z = {t["lat"]["nnlist"].shape[1] for t in self.tables}
+        if len(z) != 1:
+            raise SweepError("all chains need the same coordination number")
+        nbr = _c(np.concatenate([t["lat"]["nnlist"].reshape(-1) for t in self.tables]), np.int32)
+        self._check(_lib.lib().sh_set_neighbours(self._ctx, z.pop(), nbr.ctypes.data_as(_lib.ip)), "set_neighbours")
+
+    def features(self):
+        """[nchains, 5]: the four field features f0..f3 and E/Ns of each chain's current field."""
+        out = np.zeros((self.nchains, 5))
+        self._check(_lib.lib().sh_features(self._ctx, out.ctypes.data_as(_lib.dp)), "features")
+        return out
+
+    def set_proposal(self, net, nprop=64, local_updates=True):
+        """Block proposals from the effective action `net` (None disables them)."""
+        h = None if net is None else net._net
+        self._check(_lib.lib().sh_set_proposal(self._ctx, h, int(nprop), int(local_updates)), "set_proposal")
+
+    def proposal_stats(self, chain=0):
+        s = np.zeros(5)
+        self._check(_lib.lib().sh_get_proposal_stats(self._ctx, chain, s.ctypes.data_as(_lib.dp)), "proposal_stats")
+        return {"moves": s[0], "accepted": s[1], "spins_proposed": s[2], "spins_accepted": s[3],
+                "inner_accepted": s[4]}
+
+    def reset_proposal_stats(self):
+        self._check(_lib.lib().sh_reset_proposal_stats(self._ctx), "reset_proposal_stats")

@@ -17,17 +17,19 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared(header):
     txt = open(os.path.join(ROOT, "include", header)).read()
-    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(shp?_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(sh[pl]?_\w+)\s*\(", txt, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
     L = _lib.lib()
-    names = _declared("sunhip.h") + _declared("sunhip_testing.h") + _declared("sunhip_proj.h")
-    assert len(names) >= 40
+    names = (_declared("sunhip.h") + _declared("sunhip_testing.h") + _declared("sunhip_proj.h") +
+             _declared("sunhip_learn.h"))
+    assert len(names) >= 60
     for n in names:
         assert hasattr(L, n), n
     assert set(_declared("sunhip.h")) == set(_lib.EXPORTS)
     assert set(_declared("sunhip_proj.h")) == set(_lib.EXPORTS_PROJ)
+    assert set(_declared("sunhip_learn.h")) == set(_lib.EXPORTS_LEARN)
 
 
 @pytest.mark.parametrize("lattice,la,kmode", [("square", 4, "checkerboard"), ("square", 6, "dense"),
