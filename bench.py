@@ -6,11 +6,15 @@ term, delay update; a batch of independent chains (seeds abs((c*1981+2008)*95104
 what one MPI rank of the reference runs.  Synthetic: random initial HS fields from each
 chain's SFMT stream.  A step is one sweep (b0 pass + 0b pass, measurement on every slice) of
 every chain.  Metric: tau_int-corrected samples/s = chain sweeps/s / (2 tau_int), tau_int the
-largest of double occupancy, kinetic energy and S(pi,pi) from the per-sweep bins.
+largest of double occupancy, kinetic energy and S(pi,pi) from the per-sweep bins: the
+reference's auto_cor rho(t) with Sokal's window, on the deviations from the batch mean of each
+sweep (sunhubbardmy_amd/autocorr.py tau_int_batch), so that the relaxation every chain of the
+batch still shares after the warm-up is not read as autocorrelation.  The line also carries the
+estimate with per-chain means only (tau_int_per_chain_means).
 
-Learned vs plain: the warm-up sweeps (plain delay update) feed every chain's configuration after
-every pass to the effective-action network (40 sweeps x 2 passes x 256 chains = 20480
-configurations), which the library's FP32 kernels then train.  The same chains are timed first
+Learned vs plain: the last 40 warm-up sweeps (plain delay update) feed every chain's
+configuration after every pass to the effective-action network (40 sweeps x 2 passes x 256
+chains = 20480 configurations), which the library's FP32 kernels then train.  The same chains are timed first
 with the plain sweep, then with block proposals after every slice's local update; value is the
 learned form's tau_int-corrected samples/s, and the line carries both phases and their ratio.
 
@@ -95,7 +99,7 @@ def bench_pqmc(args):
 
     chains = args.chains if args.chains != 256 else 16
     steps = args.steps if args.steps != 60 else 2
-    warmup = args.warmup if args.warmup != 40 else 1
+    warmup = args.warmup if args.warmup != 60 else 1
     kw = dict(lattice="honeycomb", la=18, lb=18, beta=20.0, dtau=0.1, rt=1.0, rv=1.2, rndness=1e-4, kmode="dense",
               nwrap=10, nustock=0, mid_len=1, wrap_check=False)
     w = ProjWalkers([ProjModel(seed=chain_seed(c), **kw) for c in range(chains)])
@@ -130,7 +134,7 @@ def bench_pqmc(args):
 
 def timed_phase(w, steps, timers, label):
     """steps sweeps with measurement, bracketed by device synchronisation; per-sweep bins."""
-    from sunhubbardmy_amd.autocorr import tau_int
+    from sunhubbardmy_amd.autocorr import tau_int, tau_int_batch
     w.reset_obs()
     w.reset_diag()
     w.synchronize()
@@ -147,17 +151,18 @@ def timed_phase(w, steps, timers, label):
     dt = time.perf_counter() - t0
     tm = w.timers(enable=False)
     series = np.array(series)            # [steps, chains, 5]
-    taus = {}
+    taus, taus_chain = {}, {}
     for name, col in (("dblocc", 3), ("ekin", 2), ("s_pipi", 4)):
-        tau, win, _ = tau_int(series[:, :, col].T)
+        tau, win, _ = tau_int_batch(series[:, :, col].T)
         taus[name] = round(tau, 3)
+        taus_chain[name] = round(tau_int(series[:, :, col].T)[0], 3)
     tau = max(taus.values())
     sweeps_per_s = w.nchains * steps / dt
     nd = min(w.nchains, 8)
     acc = np.mean([w.diag(c)["accepted"] / max(w.diag(c)["tried"], 1) for c in range(nd)])
     means = series.mean(axis=(0, 1))
-    return {"dt": dt, "tm": tm, "tau": tau, "taus": taus, "sweeps_per_s": sweeps_per_s,
-            "samples_per_s": sweeps_per_s / (2.0 * tau), "acc": float(acc), "means": means}
+    return {"dt": dt, "tm": tm, "tau": tau, "taus": taus, "taus_chain": taus_chain, "sweeps_per_s": sweeps_per_s,
+            "samples_per_s": sweeps_per_s / (2.0 * tau), "acc": float(acc), "means": means, "series": series}
 
 
 def read_traffic(chains, n, nublock):
@@ -174,13 +179,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--chains", type=int, default=256)
     ap.add_argument("--nublock", type=int, default=32)
-    ap.add_argument("--nprop", type=int, default=64, help="inner Metropolis attempts per block proposal (<= 64)")
+    ap.add_argument("--nprop", type=int, default=16, help="inner Metropolis attempts per block proposal (<= 64)")
     ap.add_argument("--epochs", type=int, default=30, help="training epochs of the effective action")
     ap.add_argument("--proposal", choices=("both", "plain", "learned"), default="both",
                     help="time the plain delay sweep, the learned-proposal sweep, or both (value = learned)")
+    ap.add_argument("--series-out", default=None,
+                    help="write the per-sweep observable series of the timed phases to this .npz file")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timers", dest="timers", action="store_false",
                     help="skip the per-kernel hipEvent timing of the timed steps")
@@ -216,12 +223,13 @@ def main():
     # warm-up with the plain sweep; its configurations after every pass train the effective action
     learn = args.proposal != "plain"
     net = EffectiveAction(capacity=max(2 * args.warmup * args.chains, 1), seed=chain_seed(0)) if learn else None
+    first_sample = max(0, args.warmup - 40)
     for s in range(args.warmup):
         w.sweep_b0(True, False)
-        if learn:
+        if learn and s >= first_sample:
             net.add_samples(w)
         w.sweep_0b(True, False)
-        if learn:
+        if learn and s >= first_sample:
             net.add_samples(w)
         if s % 10 == 0:
             print("# warmup %d" % s, file=sys.stderr, flush=True)
@@ -245,12 +253,16 @@ def main():
             "acceptance": sum(x["accepted"] for x in pst) / max(sum(x["moves"] for x in pst), 1),
             "spins_per_accepted_block": sum(x["spins_accepted"] for x in pst) / max(sum(x["accepted"] for x in pst), 1),
             "spins_proposed_per_block": sum(x["spins_proposed"] for x in pst) / max(sum(x["moves"] for x in pst), 1)}
+    if args.series_out:
+        np.savez(args.series_out, **{k: r["series"] for k, r in (("plain", plain), ("learned", learned))
+                                     if r is not None})
     head = learned if learned is not None else plain
     wl = ("SU(2) Hubbard square 16x16 (N=256), U=6, t=1, half filling, beta=8, dtau=0.1 (Ltrot=80), nwrap=10, "
           "checkerboard K, delay update k=%d" % t["nublock"])
 
     def summary(r):
         return {"samples_per_s": r["samples_per_s"], "sweeps_per_s": r["sweeps_per_s"], "tau_int": r["taus"],
+                "tau_int_per_chain_means": r["taus_chain"],
                 "acceptance": round(r["acc"], 4), "ms_per_step": 1000.0 * r["dt"] / args.steps,
                 "observables": {"dblocc_per_site": r["means"][3] / n, "ekin_per_site_flavour": r["means"][2] / n,
                                 "s_pipi": r["means"][4]}}

@@ -73,7 +73,8 @@ int shl_loss_grad(shl_net *net, const double *x, int n, double *loss, float *gra
 int shl_predict(shl_net *net, const double *f, int n, double *e);
 
 /* Neighbour lists of the lattice sites, int32 [sum over chains of ndim][z], 0-based, symmetric
- * (j in list(i) as often as i in list(j)); z the same for every chain. */
+ * (j in list(i) as often as i in list(j)); z the same for every chain.  Switches block
+ * proposals off until the next sh_set_proposal. */
 int sh_set_neighbours(sh_ctx *ctx, int z, const int32_t *nbr);
 /* Features of every chain's current field: out[nchains][5] = f0..f3, E/Ns. */
 int sh_features(sh_ctx *ctx, double *out);

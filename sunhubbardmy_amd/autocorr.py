@@ -5,6 +5,13 @@ means over the first nb - t points of x_i x_{i+t}, x_i^2 and x_i give
 rho(t) = (<x x_t> - <x>^2) / (<x^2> - <x>^2).  For a batch of independent chains of equal
 length rho(t) is averaged over the chains.  tau_int = 1/2 + sum_{t=1}^{W} rho(t) with Sokal's
 automatic window, the smallest W with W >= c tau_int(W) (c = 5).
+
+``tau_int_batch`` first removes the batch mean of every sweep: the chains of a batch are
+independent and identically distributed, so the mean over chains at sweep t estimates the
+(possibly still relaxing) ensemble mean at t.  A slow relaxation common to every chain, which
+a per-chain estimator reads as correlation at every lag, then drops out, and rho(t) measures
+the equilibrium fluctuations only (removing the mean scales every lag's covariance by the
+same factor 1 - 1/nchains, which cancels in rho).
 """
 import numpy as np
 
@@ -41,3 +48,12 @@ def tau_int(series, c=5.0, max_lag=None):
         if w >= c * tau:
             return max(tau, 0.5), w, rho
     return max(tau, 0.5), max_lag, rho
+
+
+def tau_int_batch(series, c=5.0, max_lag=None):
+    """tau_int of a batch [nchains, nsteps] from the deviations from the batch mean of every
+    step (see the module docstring); returns (tau, window, rho)."""
+    x = np.atleast_2d(np.asarray(series, dtype=np.float64))
+    if x.shape[0] < 2:
+        raise ValueError("the batch estimator needs at least two chains")
+    return tau_int(x - x.mean(axis=0, keepdims=True), c=c, max_lag=max_lag)

@@ -1567,6 +1567,47 @@ k_save_point(const DChain *chains, int pt)
 // ------------------------------------------------------------------------------------------
 #define DSB 8   // sites per staged block of G columns and rows
 
+// sum_{m < ik} other[m] reg[m] in chunks of four entries: ik is uniform over the workgroup,
+// so each chunk is a scalar branch and only the filled chunks are read (entries of a partly
+// filled chunk beyond ik are zero in reg).  Two accumulators alternate over m.
+template <int K>
+__device__ __forceinline__ cplx delay_dot(int ik, const cplx (&reg)[K], const cplx *other)
+{
+    double vr = 0.0, vi = 0.0, wr = 0.0, wi = 0.0;
+#pragma unroll
+    for (int c = 0; c < K / 4; c++) {
+        if (4 * c < ik) {
+            const cplx o0 = other[4 * c], o1 = other[4 * c + 1], o2 = other[4 * c + 2], o3 = other[4 * c + 3];
+            vr += o0.x * reg[4 * c].x - o0.y * reg[4 * c].y;
+            vi += o0.x * reg[4 * c].y + o0.y * reg[4 * c].x;
+            wr += o1.x * reg[4 * c + 1].x - o1.y * reg[4 * c + 1].y;
+            wi += o1.x * reg[4 * c + 1].y + o1.y * reg[4 * c + 1].x;
+            vr += o2.x * reg[4 * c + 2].x - o2.y * reg[4 * c + 2].y;
+            vi += o2.x * reg[4 * c + 2].y + o2.y * reg[4 * c + 2].x;
+            wr += o3.x * reg[4 * c + 3].x - o3.y * reg[4 * c + 3].y;
+            wi += o3.x * reg[4 * c + 3].y + o3.y * reg[4 * c + 3].x;
+        }
+    }
+    return make_double2(vr + wr, vi + wi);
+}
+
+// reg[ik] = v by a binary tree of uniform branches; the empty volatile asm keeps each leaf a
+// real branch target instead of a select over all K registers
+template <int K, int LO, int HI>
+__device__ __forceinline__ void delay_put(int ik, cplx (&reg)[K], cplx v)
+{
+    if constexpr (HI - LO == 1) {
+        reg[LO] = v;
+        asm volatile("" ::: "memory");
+    } else {
+        constexpr int MID = (LO + HI) / 2;
+        if (ik < MID)
+            delay_put<K, LO, MID>(ik, reg, v);
+        else
+            delay_put<K, MID, HI>(ik, reg, v);
+    }
+}
+
 template <int K>
 __global__ void __launch_bounds__(K >= 8 ? 512 : 1024)
 k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
@@ -1631,37 +1672,23 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
             const cplx *rw = rows + (s & 1) * 2 * K;      // avec(s, :), then bvec(s, :)
             // A lanes: sum_m bvec(s,m) avec(r,m); B lanes: sum_m avec(s,m) bvec(r,m)
             const cplx *other = isA ? rw + K : rw;
-            // over all K entries without a branch: reg[m] is zero for m >= ik (cleared at each
-            // flush), so stale buffer entries there contribute nothing, and the loads can all
-            // be in flight at once
-            double vr = 0.0, vi = 0.0, wr = 0.0, wi = 0.0;
-#pragma unroll
-            for (int m = 0; m < K; m += 2) {
-                const cplx o0 = other[m], o1 = other[m + 1];
-                vr += o0.x * reg[m].x - o0.y * reg[m].y;
-                vi += o0.x * reg[m].y + o0.y * reg[m].x;
-                wr += o1.x * reg[m + 1].x - o1.y * reg[m + 1].y;
-                wi += o1.x * reg[m + 1].y + o1.y * reg[m + 1].x;
-            }
-            vr += wr;
-            vi += wi;
+            const int iku = __builtin_amdgcn_readfirstlane(ik);
+            const cplx dot = delay_dot<K>(iku, reg, other);
             cplx v = make_double2(0.0, 0.0);
             if (r < n) {
                 if (isA) {
                     const cplx gc = gcol[(s - s0) * n + r];
-                    v = cmul(make_double2(gc.x + vr, gc.y + vi), sscl);
+                    v = cmul(make_double2(gc.x + dot.x, gc.y + dot.y), sscl);
                 } else {
                     const cplx gr = grow[(s - s0) * n + r];
-                    v = make_double2(gr.x + vr, gr.y + vi);
+                    v = make_double2(gr.x + dot.x, gr.y + dot.y);
                     if (r == s) v.x -= 1.0;
                 }
             }
+            delay_put<K, 0, K>(iku, reg, v);
             // the row's other entry from the partner lane (A <-> B)
             const cplx w = make_double2(__shfl_xor(v.x, 32, 64), __shfl_xor(v.y, 32, 64));
             if (isA && r > s && r < n) diagg[r] = cadd(diagg[r], cmul(v, w));
-#pragma unroll
-            for (int m = 0; m < K; m++)
-                if (m == ik) reg[m] = v;
             if (threadIdx.x == 0) cf[s] = d.isp;
             ik++;
         }

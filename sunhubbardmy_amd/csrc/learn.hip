@@ -13,10 +13,10 @@ struct shl_net : ctx_base {
     uint64_t rng = 0;
 };
 
-static size_t lds_block_move(int nmax)
+static size_t lds_block_move(int nmax, int z)
 {
     return (size_t)SHL_MB * SHL_MB * sizeof(cplx) + 3 * SHL_MB * sizeof(cplx) + (SHL_H + 8) * sizeof(double) +
-           7 * SHL_H * sizeof(float) + (4 * SHL_MB + 16 * 12) * sizeof(int) + (size_t)3 * nmax * sizeof(int);
+           7 * SHL_H * sizeof(float) + (4 * SHL_MB + 16 * 12) * sizeof(int) + (size_t)(3 + z) * nmax * sizeof(int);
 }
 static size_t lds_mlp_batch() { return (size_t)(SHL_H * SHL_H + 7 * SHL_H + 8) * sizeof(float) + 2 * SHL_H * sizeof(float); }
 
@@ -65,7 +65,7 @@ static void block_move(sh_ctx *ctx, int nt)
 {
     if (!ctx->prop_on) return;
     Tic t(ctx, T_PROPOSAL);
-    hipLaunchKernelGGL(k_block_move, dim3(ctx->nch), dim3(256), lds_block_move(ctx->nmax), ctx->st, ctx->dc,
+    hipLaunchKernelGGL(k_block_move, dim3(ctx->nch), dim3(256), lds_block_move(ctx->nmax, ctx->prop_z), ctx->st, ctx->dc,
                        ctx->d_prop, ctx->pnet, nt);
 }
 
@@ -105,6 +105,8 @@ int sh_set_neighbours(sh_ctx *ctx, int z, const int32_t *nbr)
     }
     HIPCHK(hipMemcpy(ctx->d_nbr, nbr, tot * sizeof(int32_t), hipMemcpyHostToDevice));
     ctx->prop_z = z;
+    ctx->prop_on = false;      // the proposal's LDS size depends on z: sh_set_proposal again
+    ctx->prop_local = true;
     off = 0;
     for (int ci = 0; ci < ctx->nch; ci++) {
         PropChain &p = ctx->hprop[ci];
@@ -145,7 +147,8 @@ int sh_set_proposal(sh_ctx *ctx, const shl_net *net, int nprop, int local_update
         if (c.lcomp != 2 || c.ltrot < 2 || !c.lwrapu)
             return fail(ctx, SH_E_ARG, "block proposals need lcomp = 2, ltrot >= 2 and an interaction term");
     if (net->dev != ctx->dev) return fail(ctx, SH_E_ARG, "network and context are on different devices");
-    const size_t lds = lds_block_move(ctx->nmax);
+    const size_t lds = lds_block_move(ctx->nmax, ctx->prop_z);
+    if (lds > 150 * 1024) return fail(ctx, SH_E_ARG, "block proposals: ndim * (3 + z) too large for LDS");
     if (lds > 64 * 1024 && hipFuncSetAttribute((const void *)k_block_move, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                (int)lds) != hipSuccess) {
         (void)hipGetLastError();

@@ -83,13 +83,17 @@ __device__ __forceinline__ void wg_sum_i(int (&v)[NV], int *scr)
     __syncthreads();
 }
 
-// features (f0..f3) of the totals T = sum over slices of the per-slice sums
-__device__ __forceinline__ void feat_of(const int (&T)[4], double ns, double nbl, const PropNet &pn, double (&x)[4])
+// normalised network inputs x_k = (T_k / norm_k - xmu_k) / xsd_k of the totals T (sums over
+// slices of the per-slice sums), as x_k = T_k fa_k + fb_k with the factors formed once
+__device__ __forceinline__ void feat_of(const int (&T)[4], const double (&fa)[4], const double (&fb)[4], double (&x)[4])
 {
-    const double f[4] = {T[0] / ns, T[1] / nbl, T[2] / ns, T[3] / nbl};
 #pragma unroll
-    for (int k = 0; k < 4; k++) x[k] = (f[k] - pn.xmu[k]) / pn.xsd[k];
+    for (int k = 0; k < 4; k++) x[k] = fma((double)T[k], fa[k], fb[k]);
 }
+
+// tanh(a) = 1 - 2 / (e^{2a} + 1): one exponential and one division; exact limits +-1, absolute
+// error a few ulp of 1 (the network only needs tanh to that absolute accuracy)
+__device__ __forceinline__ double tanh_exp(double a) { return 1.0 - 2.0 / (exp(2.0 * a) + 1.0); }
 
 // g(x) of the 4 -> 128 -> 128 -> 1 tanh network, evaluated by a workgroup of exactly 256
 // threads; thread t owns half (t & 1) of row t >> 1 of W2 in w2r.  Every thread returns the
@@ -102,7 +106,7 @@ __device__ __forceinline__ double mlp_eval64(const double (&x)[4], const float *
         double a = (double)b1s[t];
         a += (double)W1s[4 * t] * x[0] + (double)W1s[4 * t + 1] * x[1] + (double)W1s[4 * t + 2] * x[2] +
              (double)W1s[4 * t + 3] * x[3];
-        h1[t] = tanh(a);
+        h1[t] = tanh_exp(a);
     }
     __syncthreads();
     const int j = t >> 1, half = t & 1;
@@ -115,7 +119,7 @@ __device__ __forceinline__ double mlp_eval64(const double (&x)[4], const float *
     }
     double p = p0 + p1;
     p += __shfl_xor(p, 1, 64);
-    const double h2 = tanh(p + (double)b2s[j]);
+    const double h2 = tanh_exp(p + (double)b2s[j]);
     double q = half ? 0.0 : (double)w3s[j] * h2;
     q = wave_sum(q);
     if (lane == 0) red[w] = q;
@@ -189,7 +193,8 @@ __global__ __launch_bounds__(64) void k_prop_features(const DChain *chains, cons
 //     R = det(M)^nflr prod qf
 //  5. on accept G <- G - G_:F M^-1 Delta_F (E_F - G_F:) as a rank-|F| MFMA update, flip the
 //     field, rotate the phase
-// LDS: 3n spins, M (64 x 64), pivot rows/columns, network vectors and small weights.
+// LDS: 3n spins, the neighbour lists, M (64 x 64), pivot rows/columns, network vectors and
+// small weights.
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_block_move(const DChain *chains, const PropChain *props, PropNet pn, int nt)
 {
@@ -212,6 +217,7 @@ __global__ __launch_bounds__(256) void k_block_move(const DChain *chains, const 
     int *cols = piv + SHL_MB;                       // 64
     int *iscr = cols + SHL_MB;                      // 16 * 12
     int *R = iscr + 16 * 12;                        // 3n spins (+-1): slices tm, tau, tp
+    int *nb = R + 3 * n;                            // n * z neighbour lists
     int32_t *cf = c.conf + (size_t)tau * n;
     const int32_t *cm = c.conf + (size_t)tm * n, *cp = c.conf + (size_t)tp * n;
     cplx *G = c.G;
@@ -234,6 +240,7 @@ __global__ __launch_bounds__(256) void k_block_move(const DChain *chains, const 
         R[n + i] = 2 * cf[i] - 3;
         R[2 * n + i] = 2 * cp[i] - 3;
     }
+    for (int e = t; e < n * z; e += 256) nb[e] = p.nbr[e];
     __syncthreads();
 
     // partial sums of slices tm (rows tm, tau) and tau (rows tau, tp), and the other slices
@@ -243,7 +250,7 @@ __global__ __launch_bounds__(256) void k_block_move(const DChain *chains, const 
             const int s0 = R[i], s1 = R[n + i], s2 = R[2 * n + i];
             int n0 = 0, n1 = 0, n2 = 0;
             for (int k = 0; k < z; k++) {
-                const int j = p.nbr[(size_t)i * z + k];
+                const int j = nb[i * z + k];
                 n0 += R[j];
                 n1 += R[n + j];
                 n2 += R[2 * n + j];
@@ -263,9 +270,15 @@ __global__ __launch_bounds__(256) void k_block_move(const DChain *chains, const 
 
     const double ns = (double)n * lt, nbl = (double)p.nb2 * lt;
     const double escale = ns * pn.ysd;
+    double fa[4], fb[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        fa[k] = 1.0 / (((k & 1) ? nbl : ns) * pn.xsd[k]);
+        fb[k] = -pn.xmu[k] / pn.xsd[k];
+    }
     uint64_t rs = p.rng[0];
     double x[4];
-    feat_of(T, ns, nbl, pn, x);
+    feat_of(T, fa, fb, x);
     const double g0 = mlp_eval64(x, W1s, b1s, b2s, w3s, b3, w2r, h1, red);
     double gc = g0;
     int ninner = 0;
@@ -275,12 +288,12 @@ __global__ __launch_bounds__(256) void k_block_move(const DChain *chains, const 
         const int s = R[n + i];
         int nn = 0, st = 0;
         for (int k = 0; k < z; k++) {
-            const int j = p.nbr[(size_t)i * z + k];
+            const int j = nb[i * z + k];
             nn += R[n + j];
             st += R[j] + R[2 * n + j];
         }
         const int T2[4] = {T[0] - 2 * s, T[1] - 4 * s * nn, T[2] - 2 * s * (R[i] + R[2 * n + i]), T[3] - 2 * s * st};
-        feat_of(T2, ns, nbl, pn, x);
+        feat_of(T2, fa, fb, x);
         const double gn = mlp_eval64(x, W1s, b1s, b2s, w3s, b3, w2r, h1, red);
         const double de = escale * (gn - gc);
         // every thread holds the same de and u2, so the decision is uniform; the spins of this

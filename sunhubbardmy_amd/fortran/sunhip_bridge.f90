@@ -183,6 +183,45 @@ module sunhip_bridge
        import :: c_ptr, c_int
        type(c_ptr), value :: ctx
      end function shp_reset_obs
+     ! learned block proposals (include/sunhip_learn.h)
+     integer(c_int) function shl_create(net, capacity, seed, device) bind(C, name="shl_create")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr) :: net
+       integer(c_int), value :: capacity, device
+       integer(c_int32_t), value :: seed
+     end function shl_create
+     integer(c_int) function shl_destroy(net) bind(C, name="shl_destroy")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: net
+     end function shl_destroy
+     integer(c_int) function shl_add_samples(net, ctx) bind(C, name="shl_add_samples")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: net, ctx
+     end function shl_add_samples
+     integer(c_int) function shl_train(net, epochs, batch, lr, loss) bind(C, name="shl_train")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: net
+       integer(c_int), value :: epochs, batch
+       real(c_double), value :: lr
+       real(c_double) :: loss(*)
+     end function shl_train
+     integer(c_int) function sh_set_neighbours(ctx, z, nbr) bind(C, name="sh_set_neighbours")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: z
+       integer(c_int32_t) :: nbr(*)
+     end function sh_set_neighbours
+     integer(c_int) function sh_set_proposal(ctx, net, nprop, local_updates) bind(C, name="sh_set_proposal")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, net
+       integer(c_int), value :: nprop, local_updates
+     end function sh_set_proposal
+     integer(c_int) function sh_get_proposal_stats(ctx, chain, stats) bind(C, name="sh_get_proposal_stats")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       real(c_double) :: stats(*)
+     end function sh_get_proposal_stats
   end interface
 
 contains
