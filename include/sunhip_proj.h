@@ -1,11 +1,12 @@
 /*
- * sunhip_proj.h -- C ABI of libsunhip.so for the projective (ground-state) DQMC sweep of the
- * spinless t-V model: bond Hubbard-Stratonovich field, submatrix-LR local update, QR of the
- * rectangular propagators every nwrap slices.
+ * sunhip_proj.h -- C ABI of libsunhip.so for the projective (ground-state) DQMC sweep: the
+ * bond Hubbard-Stratonovich field of the t-V model and/or the on-site field of the Hubbard
+ * model, submatrix-LR local updates, QR of the rectangular propagators every nwrap slices.
  *
  * Drop-in for the module procedures of src/dqmc_proj_core.f90 that the reference's
- * src/dqmc_proj_ctrl.f90 calls, and for the vconf operator plugin they drive
- * (src/dqmc_config_v.f90).  Each entry point below names the reference routine it replaces.
+ * src/dqmc_proj_ctrl.f90 calls, and for the vconf and uconf operator plugins they drive
+ * (src/dqmc_config_v.f90, src/dqmc_config_u.f90).  Each entry point below names the reference
+ * routine it replaces.
  * Conventions as in sunhip.h: int return codes (0 = success, SH_E_*), column-major complex128
  * (re, im interleaved) host buffers owned by the caller and read or written only during the
  * call, one context per host thread, all chains of a context advanced together.
@@ -21,9 +22,10 @@ extern "C" {
 
 typedef struct shp_ctx shp_ctx;
 
-/* One chain's model: what make_tables, h0c%set_h0conf and v0conf%set_conf leave in module
- * state (src/model_para.f90:376-453, src/dqmc_config_h0/dqmc_set_h0conf.f90,
- * src/dqmc_config_v/dqmc_set_conf.f90:1-79). */
+/* One chain's model: what make_tables, h0c%set_h0conf, v0conf%set_conf and u0conf%set_conf
+ * leave in module state (src/model_para.f90:376-453, src/dqmc_config_h0/dqmc_set_h0conf.f90,
+ * src/dqmc_config_v/dqmc_set_conf.f90:1-79, src/dqmc_config_u/dqmc_set_conf.f90:1-176).
+ * vtab == NULL: no bond field (rv = 0); ubmat == NULL: no on-site field (rhub = 0). */
 typedef struct shp_model {
     int ndim, ne;                 /* sites, particles (ne = ndim/2 + int(nu*ndim/8)) */
     int ltrot, nwrap, nst;        /* slices, stabilization interval and count */
@@ -41,6 +43,15 @@ typedef struct shp_model {
     int mid_len;                  /* obs_eqt_mid_len: measured slices around ltrot/2 */
     int wrap_check;               /* 1: the WRAPERROR comparison at every stabilization */
     int32_t seed;                 /* SFMT19937 seed of the chain (src/dqmc_util.f90:68-75) */
+    /* the on-site field (u0conf, lproju = .false.) */
+    int ulcomp;                   /* components 2 or 4 */
+    const double *ubmat;          /* [ulcomp] complex bmat */
+    const double *ubmat_inv;      /* [ulcomp] complex */
+    const double *udelta;         /* [(ulcomp-1)*ulcomp] complex delta_bmat(iflip, is), column-major */
+    const double *uratio;         /* [(ulcomp-1)*ulcomp] complex phase_ratio(iflip, is) */
+    const double *ugaml;          /* [ulcomp] gaml */
+    const double *uphase_arg;     /* [ulcomp] arg of phase(is), for set_phase */
+    int uphase_from_conf;         /* 1 when set_phase multiplies the field's phases (rhub > 0) */
 } shp_model;
 
 /* allocate_core (src/dqmc_proj_core.f90:23-49) for a batch of chains sharing ndim, ne, ltrot,
@@ -54,6 +65,11 @@ const char *shp_last_error(const shp_ctx *ctx);
 int shp_init_conf(shp_ctx *ctx);
 int shp_set_conf(shp_ctx *ctx, int chain, const int32_t *conf);   /* [ltrot][nn_nf][ndim] */
 int shp_get_conf(shp_ctx *ctx, int chain, int32_t *conf);
+/* the on-site field u0conf%conf(1:ndim, 1:ltrot) as int32 [ltrot][ndim], values 1..ulcomp;
+ * shp_init_conf draws it first (u0conf%input_conf precedes v0conf%input_conf in dqmc_start,
+ * src/dqmc_proj_ctrl.f90:27-36) */
+int shp_set_uconf(shp_ctx *ctx, int chain, const int32_t *conf);
+int shp_get_uconf(shp_ctx *ctx, int chain, int32_t *conf);
 
 /* the next n numbers of the chain's stream (set_trial_h0 draws them after the field:
  * src/dqmc_config_h0/set_trial_h0.f90:20) */

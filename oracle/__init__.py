@@ -61,6 +61,10 @@ def lib():
         L.orc_psizes.argtypes = [P, ip]
         L.orc_pget_conf.argtypes = [P, ip]
         L.orc_pset_conf.argtypes = [P, ip]
+        L.orc_pget_uconf.argtypes = [P, ip]
+        L.orc_pset_uconf.argtypes = [P, ip]
+        L.orc_pget_uconf.restype = None
+        L.orc_pset_uconf.restype = None
         L.orc_pset_trial.argtypes = [P]
         L.orc_pset_trial.restype = ctypes.c_double
         L.orc_pset_proj.argtypes = [P, dp]
@@ -105,6 +109,7 @@ class OrcPParams(ctypes.Structure):
         ("rndness", ctypes.c_double), ("nflr", ctypes.c_int), ("kmode", ctypes.c_int),
         ("nwrap", ctypes.c_int), ("nustock", ctypes.c_int), ("mid_len", ctypes.c_int),
         ("wrap_check", ctypes.c_int), ("seed", ctypes.c_int32),
+        ("rhub", ctypes.c_double), ("exact_hs", ctypes.c_int),
     ]
 
 
@@ -274,18 +279,19 @@ def inv_logdet(a):
 
 
 class OracleProjChain:
-    """One projective (t-V, submatrix-LR) chain of the CPU restatement, driven like the
-    reference's dqmc_proj_ctrl: init_conf, set_trial (or set_proj), sweep_start, sweeps."""
+    """One projective chain of the CPU restatement (bond field for rv != 0, on-site field for
+    rhub != 0, both with the submatrix-LR update), driven like the reference's
+    dqmc_proj_ctrl: init_conf, set_trial (or set_proj), sweep_start, sweeps."""
 
     def __init__(self, lattice="honeycomb", la=2, lb=2, beta=2.0, dtau=0.1, rt=1.0, mu=0.0,
                  rv=1.2, nu=0.0, rndness=1e-4, nflr=1, kmode="dense", nwrap=10, nustock=8,
-                 mid_len=1, wrap_check=True, seed=None):
+                 mid_len=1, wrap_check=True, seed=None, rhub=0.0, exact_hs=True):
         L = lib()
         if seed is None:
             seed = L.orc_chain_seed(0)
         self.params = OrcPParams(LATTICES[lattice], la, lb, beta, dtau, rt, mu, rv, nu, rndness, nflr,
                                  KMODES[kmode], nwrap, nustock, mid_len, int(bool(wrap_check)),
-                                 int(np.int32(seed)))
+                                 int(np.int32(seed)), float(rhub), int(bool(exact_hs)))
         self._c = L.orc_pcreate(ctypes.byref(self.params))
         s = np.zeros(8, np.int32)
         L.orc_psizes(self._c, _iptr(s))
@@ -311,6 +317,17 @@ class OracleProjChain:
     def conf(self, value):
         c = np.ascontiguousarray(value, dtype=np.int32).reshape(self.ltrot, self.nn_nf, self.ndim)
         lib().orc_pset_conf(self._c, _iptr(c))
+
+    @property
+    def uconf(self):
+        c = np.zeros((self.ltrot, self.ndim), np.int32)
+        lib().orc_pget_uconf(self._c, _iptr(c))
+        return c
+
+    @uconf.setter
+    def uconf(self, value):
+        c = np.ascontiguousarray(value, dtype=np.int32).reshape(self.ltrot, self.ndim)
+        lib().orc_pset_uconf(self._c, _iptr(c))
 
     def set_trial(self):
         return float(lib().orc_pset_trial(self._c))

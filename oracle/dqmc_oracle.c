@@ -1613,6 +1613,7 @@ struct orc_pctx {
     double *logwDV;
     zc phase, logweightf;
     double max_wrap_error, max_phasediff, nchecks, sumlog10err, accepted, tried;
+    int lwrapv;            /* bond field present (rv != 0); the on-site field lives in b (b->lwrapu) */
 };
 
 /* vconf%set_conf, lprojv = .false. (src/dqmc_config_v/dqmc_set_conf.f90:14-77):
@@ -1638,7 +1639,7 @@ orc_pctx *orc_pcreate(const orc_pparams *p)
     orc_params q;
     memset(&q, 0, sizeof(q));
     q.lattice = p->lattice; q.la = p->la; q.lb = p->lb; q.beta = p->beta; q.dtau = p->dtau;
-    q.rt = p->rt; q.mu = p->mu; q.rhub = 0.0; q.nu = p->nu; q.nflr = p->nflr; q.exact_hs = 1;
+    q.rt = p->rt; q.mu = p->mu; q.rhub = p->rhub; q.nu = p->nu; q.nflr = p->nflr; q.exact_hs = p->exact_hs;
     q.kmode = p->kmode; q.update = 2; q.nwrap = p->nwrap; q.nublock = 1; q.seed = p->seed;
     orc_pctx *c = (orc_pctx *)calloc(1, sizeof(orc_pctx));
     c->p = *p;
@@ -1647,10 +1648,16 @@ orc_pctx *orc_pcreate(const orc_pparams *p)
     setup_lattice(b);
     b->ltrot = (int)lround(p->beta / p->dtau);
     setup_schedule(b);
-    b->lwrapu = 0;
+    b->lwrapu = (p->rhub != 0.0);
+    if (b->lwrapu) {
+        setup_hs(b);      /* u0conf%set_conf (src/dqmc_config_u/dqmc_set_conf.f90), lproju = .false. */
+        b->conf = (int32_t *)calloc((size_t)b->ndim * b->ltrot, sizeof(int32_t));
+        for (size_t k = 0; k < (size_t)b->ndim * b->ltrot; k++) b->conf[k] = 1;
+    }
     setup_kinetic(b);
     sfmt_init(&b->rng, p->seed);
     c->b = b;
+    c->lwrapv = (p->rv != 0.0);
     c->N = b->ndim;
     c->ne = c->N / 2 + (int)(p->nu * (double)c->N / 8.0);   /* src/model_para.f90:376 */
     c->ltrot = b->ltrot; c->nst = b->nst;
@@ -1681,7 +1688,7 @@ void orc_pdestroy(orc_pctx *c)
     if (!c) return;
     orc_ctx *b = c->b;
     free(b->urt); free(b->urtm1); free(b->nnlf); free(b->nnlist); free(b->cx); free(b->cy); free(b->sub);
-    free(b->iwrap_nt); free(b->wrap_step); free(b);
+    free(b->iwrap_nt); free(b->wrap_step); free(b->conf); free(b);
     for (int n = 0; n <= c->nst; n++) free(c->Ust[n]);
     free(c->Ust); free(c->logwDV); free(c->conf); free(c->wtrial); free(c->projR);
     free(c->UR); free(c->UL); free(c->ULR); free(c->ULRINV); free(c->G); free(c->tmp);
@@ -1699,12 +1706,23 @@ void orc_psizes(const orc_pctx *c, int *out)
 void orc_pinit_conf(orc_pctx *c)
 {
     int N = c->N, nf = c->nn_nf;
+    /* dqmc_start draws u0conf%input_conf before v0conf%input_conf (src/dqmc_proj_ctrl.f90:27-36) */
+    if (c->b->lwrapu) orc_init_conf(c->b);
+    if (!c->lwrapv) return;
     for (int nt = 0; nt < c->ltrot; nt++)
         for (int i = 0; i < N; i++)
             for (int f = 0; f < nf; f++)
                 c->conf[((size_t)nt * nf + f) * N + i] = (int32_t)ceil(sfmt_stream(&c->b->rng) * 2.0);
 }
 void orc_pget_conf(const orc_pctx *c, int32_t *conf) { memcpy(conf, c->conf, sizeof(int32_t) * (size_t)c->N * c->nn_nf * c->ltrot); }
+void orc_pget_uconf(const orc_pctx *c, int32_t *conf)
+{
+    if (c->b->lwrapu) memcpy(conf, c->b->conf, sizeof(int32_t) * (size_t)c->N * c->ltrot);
+}
+void orc_pset_uconf(orc_pctx *c, const int32_t *conf)
+{
+    if (c->b->lwrapu) memcpy(c->b->conf, conf, sizeof(int32_t) * (size_t)c->N * c->ltrot);
+}
 void orc_pset_conf(orc_pctx *c, const int32_t *conf) { memcpy(c->conf, conf, sizeof(int32_t) * (size_t)c->N * c->nn_nf * c->ltrot); }
 
 /* cyclic Jacobi diagonalisation of a real symmetric matrix (a destroyed, v = eigenvectors,
@@ -1885,16 +1903,44 @@ static void pv_right(orc_pctx *c, int nt, int nf, int mode, zc *M, int m)
     }
 }
 
-/* Bmat_left_forward of the projector core (src/dqmc_proj_core.f90:68-116), lwrapv only */
+/* uconf props on the rows of an N x ne matrix (UR) or the columns of an ne x N matrix (UL),
+ * by bmat (inv = 0) or bmat_inv (inv = 1) of the field at slice nt:
+ * left_forward / left_backward (src/dqmc_config_u/dqmc_left_forward_prop.f90:23-32,
+ * dqmc_left_backward_prop.f90:24-33), right_forward / right_backward
+ * (dqmc_right_forward_prop.f90:24-33, dqmc_right_backward_prop.f90:24-33) */
+static void pu_left(orc_pctx *c, int nt, int inv, zc *M)
+{
+    orc_ctx *b = c->b;
+    int N = c->N;
+    for (int i = 0; i < N; i++) {
+        int is = b->conf[(size_t)(nt - 1) * N + i];
+        zc f = inv ? b->bmat_inv[is - 1] : b->bmat[is - 1];
+        for (int j = 0; j < c->ne; j++) M[IDX(i, j, N)] *= f;
+    }
+}
+static void pu_right(orc_pctx *c, int nt, int inv, zc *M)
+{
+    orc_ctx *b = c->b;
+    int N = c->N, ne = c->ne;
+    for (int i = 0; i < N; i++) {
+        int is = b->conf[(size_t)(nt - 1) * N + i];
+        zc f = inv ? b->bmat_inv[is - 1] : b->bmat[is - 1];
+        for (int r = 0; r < ne; r++) M[IDX(r, i, ne)] *= f;
+    }
+}
+
+/* Bmat_left_forward of the projector core (src/dqmc_proj_core.f90:68-116) */
 static void pq_bmat_left_forward(orc_pctx *c, int nt1, int nt2, zc *M)
 {
     int nfam = pq_nfam(c);
     for (int nt = nt2; nt <= nt1; nt++) {
         for (int q = 0; q < nfam; q++) pk_left(c, q, c->b->urt, M, c->ne);
-        for (int nf = 0; nf < c->nn_nf; nf++) {
-            pv_left(c, nt, nf, 0, M, c->ne);
-            pv_left(c, nt, nf, 1, M, c->ne);
-        }
+        if (c->lwrapv)
+            for (int nf = 0; nf < c->nn_nf; nf++) {
+                pv_left(c, nt, nf, 0, M, c->ne);
+                pv_left(c, nt, nf, 1, M, c->ne);
+            }
+        if (c->b->lwrapu) pu_left(c, nt, 0, M);
         for (int q = nfam - 1; q >= 0; q--) pk_left(c, q, c->b->urt, M, c->ne);
     }
 }
@@ -1961,8 +2007,15 @@ static void pq_green(orc_pctx *c, const zc *ul, const zc *ur, const zc *ulrinv, 
     free(t);
 }
 
-/* set_phase (src/dqmc_proj_core/set_phase.f90:1-24) for the t-V model: no field factors */
-static zc pq_phase(const orc_pctx *c, zc logw) { return cexp(CMPLX(0.0, (double)c->p.nflr * cimag(logw))); }
+/* set_phase (src/dqmc_proj_core/set_phase.f90:1-24): the on-site field's phases when U > 0 */
+static zc pq_phase(const orc_pctx *c, zc logw)
+{
+    zc w = cexp(CMPLX(0.0, (double)c->p.nflr * cimag(logw)));
+    orc_ctx *b = c->b;
+    if (b->lwrapu && c->p.rhub > 0.0)
+        for (size_t k = 0; k < (size_t)c->ltrot * c->N; k++) w *= b->phase_tab[b->conf[k] - 1];
+    return w;
+}
 
 /* submatrix-LR update of family nf at slice nt (src/dqmc_config_v/dqmc_proj_update_submatrix_LR.f90:1-298) */
 static void pq_update(orc_pctx *c, int nt, int nf)
@@ -2088,6 +2141,111 @@ static void pq_update(orc_pctx *c, int nt, int nf)
     free(Vtmp); free(Utmp); free(ikk); free(kik); free(Pikk); free(Pkik); free(xvec); free(svec); free(Dvec);
 }
 
+/* uconf submatrix-LR update at slice nt (src/dqmc_config_u/dqmc_proj_update_submatrix_LR.f90:83-276):
+ * stocks of nustock consecutive sites; Rrows = UR(P,:) ULRINV, Felms = Rrows UL(:,P); per site
+ * the 1x1 ratio 1 + (F_kk - F_kP Gamma^-1 F_Pk) Delta, Gamma^-1 grown by bordering; at the end
+ * of a stock ULRINV -= (ULRINV Umat)(Gamma^-1 Vmat) and the accepted rows of UR scaled by 1 + Delta */
+static void pq_update_u(orc_pctx *c, int nt)
+{
+    int N = c->N, ne = c->ne, ns = c->nustock;
+    orc_ctx *b = c->b;
+    int lc = b->lcomp;
+    zc *ulP = zalloc((size_t)ne * ns), *urP = zalloc((size_t)ns * ne);
+    zc *Rrows = zalloc((size_t)ns * ne), *Felms = zalloc((size_t)ns * ns);
+    zc *Ginv = zalloc((size_t)ns * ns), *Vmat = zalloc((size_t)ns * ne), *Umat = zalloc((size_t)ne * ns);
+    zc *Vtmp = zalloc((size_t)ns * ne), *Utmp = zalloc((size_t)ne * ns);
+    zc *ikk = zalloc(ns), *kik = zalloc(ns), *Pikk = zalloc(ns), *Pkik = zalloc(ns), *Dvec = zalloc(ns);
+    int *xvec = (int *)calloc(ns, sizeof(int)), *svec = (int *)calloc(ns, sizeof(int));
+    int ik = 0, istk_s = 0, istk_e = -1, nstk = 0;
+    double accm = 0;
+    for (int i = 0; i < N; i++) {
+        int32_t *cf = &b->conf[(size_t)(nt - 1) * N + i];
+        int is = *cf;
+        int iflip = (int)ceil(sfmt_stream(&b->rng) * (lc - 1));
+        int isp = (is + iflip - 1) % lc + 1;
+        if (nstk == 0) {
+            int nup = ns < N - i ? ns : N - i;
+            memset(ulP, 0, sizeof(zc) * (size_t)ne * ns);
+            memset(urP, 0, sizeof(zc) * (size_t)ns * ne);
+            for (int q = 0; q < nup; q++)
+                for (int r = 0; r < ne; r++) {
+                    ulP[IDX(r, q, ne)] = c->UL[IDX(r, i + q, ne)];
+                    urP[IDX(q, r, ns)] = c->UR[IDX(i + q, r, N)];
+                }
+            zgemm('N', 'N', nup, ne, ne, urP, ns, c->ULRINV, ne, 0, Rrows, ns);
+            zgemm('N', 'N', nup, nup, ne, Rrows, ns, ulP, ne, 0, Felms, ns);
+            istk_s = i; istk_e = i + nup - 1; nstk = nup;
+        }
+        int s = i - istk_s;
+        zc vuk = 0;
+        if (ik > 0) {
+            for (int m = 0; m < ik; m++) {
+                Pkik[m] = Felms[IDX(s, svec[m], ns)];
+                Pikk[m] = Felms[IDX(svec[m], s, ns)];
+            }
+            for (int m = 0; m < ik; m++) {
+                zc t = 0;
+                for (int q = 0; q < ik; q++) t += Ginv[IDX(m, q, ns)] * Pikk[q];
+                ikk[m] = t;
+            }
+            for (int m = 0; m < ik; m++) vuk += Pkik[m] * ikk[m];
+        }
+        vuk = Felms[IDX(s, s, ns)] - vuk;
+        zc d = b->delta_bmat[(iflip - 1) + (lc - 1) * (is - 1)];
+        zc ratio1 = 1.0 + vuk * d;
+        zc ratiotot = zpow(ratio1, c->p.nflr) * (b->gaml[isp - 1] / b->gaml[is - 1]) *
+                      b->phase_ratio[(iflip - 1) + (lc - 1) * (is - 1)];
+        double ratio_re = creal(ratiotot * c->phase) / creal(c->phase);
+        double random = sfmt_stream(&b->rng);
+        if ((ratio_re < 0 ? -ratio_re : ratio_re) > random) {
+            accm += 1;
+            double weight = sqrt(creal(ratiotot * conj(ratiotot)));
+            c->phase = c->phase * ratiotot / weight;
+            zc gkk = d / ratio1;
+            Ginv[IDX(ik, ik, ns)] = gkk;
+            if (ik > 0) {
+                for (int q = 0; q < ik; q++) {
+                    zc t = 0;
+                    for (int m = 0; m < ik; m++) t += Pkik[m] * Ginv[IDX(m, q, ns)];
+                    kik[q] = t;
+                }
+                for (int q = 0; q < ik; q++) Ginv[IDX(ik, q, ns)] = -gkk * kik[q];
+                for (int m = 0; m < ik; m++) Ginv[IDX(m, ik, ns)] = -ikk[m] * gkk;
+                for (int q = 0; q < ik; q++)
+                    for (int m = 0; m < ik; m++) Ginv[IDX(m, q, ns)] -= ikk[m] * Ginv[IDX(ik, q, ns)];
+            }
+            ik += 1;
+            xvec[ik - 1] = i; svec[ik - 1] = s; Dvec[ik - 1] = d;
+            for (int r = 0; r < ne; r++) {
+                Vmat[IDX(ik - 1, r, ns)] = Rrows[IDX(s, r, ns)];
+                Umat[IDX(r, ik - 1, ne)] = ulP[IDX(r, s, ne)];
+            }
+            *cf = isp;
+        }
+        if (i == istk_e || i == N - 1) {
+            if (ik > 0) {
+                zgemm('N', 'N', ik, ne, ik, Ginv, ns, Vmat, ns, 0, Vtmp, ns);
+                zgemm('N', 'N', ne, ik, ne, c->ULRINV, ne, Umat, ne, 0, Utmp, ne);
+                zgemm('N', 'N', ne, ne, ik, Utmp, ne, Vtmp, ns, 0, c->tmp, ne);
+                for (size_t k = 0; k < (size_t)ne * ne; k++) c->ULRINV[k] -= c->tmp[k];
+                for (int m = 0; m < ik; m++) {
+                    int x = xvec[m];
+                    zc dd = Dvec[m];
+                    for (int r = 0; r < ne; r++) c->UR[IDX(x, r, N)] = c->UR[IDX(x, r, N)] + dd * c->UR[IDX(x, r, N)];
+                }
+            }
+            ik = 0; istk_s = 0; istk_e = -1; nstk = 0;
+            memset(Ginv, 0, sizeof(zc) * (size_t)ns * ns);
+            memset(Vmat, 0, sizeof(zc) * (size_t)ns * ne);
+            memset(Umat, 0, sizeof(zc) * (size_t)ne * ns);
+        }
+    }
+    c->accepted += accm;
+    c->tried += N;
+    free(ulP); free(urP); free(Rrows); free(Felms); free(Ginv); free(Vmat); free(Umat);
+    free(Vtmp); free(Utmp); free(ikk); free(kik); free(Pikk); free(Pkik); free(Dvec); free(xvec); free(svec);
+}
+
 /* dqmc_proj_sweep_start_0b (src/dqmc_proj_core/dqmc_proj_sweep_start_0b.f90:1-48) */
 int orc_psweep_start(orc_pctx *c)
 {
@@ -2173,7 +2331,12 @@ int orc_psweep_b0(orc_pctx *c, int lupdate, int lmeasure)
     for (int nt = c->ltrot; nt >= 1; nt--) {
         if (lmeasure && pq_in_window(c, nt)) pq_measure(c);
         for (int q = 0; q < nfam; q++) { pk_right(c, q, b->urt, c->UL, ne); pk_left(c, q, b->urtm1, c->UR, ne); }
-        for (int nf = c->nn_nf - 1; nf >= 0; nf--) {
+        if (b->lwrapu) {
+            if (lupdate) pq_update_u(c, nt);
+            pu_right(c, nt, 0, c->UL);
+            pu_left(c, nt, 1, c->UR);
+        }
+        for (int nf = c->nn_nf - 1; nf >= 0 && c->lwrapv; nf--) {
             pv_right(c, nt, nf, 1, c->UL, ne);
             pv_left(c, nt, nf, 1, c->UR, ne);
             if (lupdate) pq_update(c, nt, nf);
@@ -2214,12 +2377,17 @@ int orc_psweep_0b(orc_pctx *c, int lupdate, int lmeasure)
     int rc = 0;
     for (int nt = 1; nt <= c->ltrot; nt++) {
         for (int q = 0; q < nfam; q++) { pk_left(c, q, b->urt, c->UR, ne); pk_right(c, q, b->urtm1, c->UL, ne); }
-        for (int nf = 0; nf < c->nn_nf; nf++) {
+        for (int nf = 0; nf < c->nn_nf && c->lwrapv; nf++) {
             pv_left(c, nt, nf, 0, c->UR, ne);
             pv_right(c, nt, nf, 0, c->UL, ne);
             if (lupdate) pq_update(c, nt, nf);
             pv_left(c, nt, nf, 1, c->UR, ne);
             pv_right(c, nt, nf, 1, c->UL, ne);
+        }
+        if (b->lwrapu) {
+            pu_left(c, nt, 0, c->UR);
+            pu_right(c, nt, 1, c->UL);
+            if (lupdate) pq_update_u(c, nt);
         }
         for (int q = nfam - 1; q >= 0; q--) { pk_left(c, q, b->urt, c->UR, ne); pk_right(c, q, b->urtm1, c->UL, ne); }
         if (lmeasure && pq_in_window(c, nt)) pq_measure(c);

@@ -99,9 +99,14 @@ typedef struct orc_pparams {
     int nwrap, nustock, mid_len;
     int wrap_check; /* 1: the WRAPERROR comparison at every stabilization */
     int seed;
+    double rhub;    /* on-site U: the uconf field (src/dqmc_config_u) joins the bond field when != 0 */
+    int exact_hs;   /* uconf tables: 1 = EXACTHS, 0 = general 4-component */
 } orc_pparams;
 
 typedef struct orc_pctx orc_pctx;
+/* the on-site field of a projective chain with rhub != 0: [ltrot][N], values 1..lcomp */
+void orc_pget_uconf(const orc_pctx *c, int32_t *conf);
+void orc_pset_uconf(orc_pctx *c, const int32_t *conf);
 orc_pctx *orc_pcreate(const orc_pparams *p);
 void orc_pdestroy(orc_pctx *c);
 /* sizes: [0]=N [1]=ne [2]=ltrot [3]=nst [4]=nn_nf [5]=nn_lf [6]=nustock [7]=lq */

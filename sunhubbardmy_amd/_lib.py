@@ -38,7 +38,7 @@ EXPORTS_PROJ = [
     "shp_create", "shp_destroy", "shp_last_error", "shp_init_conf", "shp_set_conf", "shp_get_conf",
     "shp_draw_uniforms", "shp_set_proj", "shp_sweep_start_0b", "shp_sweep_b0", "shp_sweep_0b", "shp_get_g",
     "shp_get_state", "shp_get_diag", "shp_reset_diag", "shp_get_obs", "shp_reset_obs", "shp_set_trace",
-    "shp_get_trace", "shp_timers", "shp_synchronize",
+    "shp_get_trace", "shp_timers", "shp_synchronize", "shp_set_uconf", "shp_get_uconf",
 ]
 
 
@@ -90,6 +90,8 @@ def lib():
     L.shp_last_error.restype = ctypes.c_char_p
     L.shp_init_conf.argtypes = [P]
     L.shp_set_conf.argtypes = [P, ctypes.c_int, ip]
+    L.shp_set_uconf.argtypes = [P, ctypes.c_int, ip]
+    L.shp_get_uconf.argtypes = [P, ctypes.c_int, ip]
     L.shp_get_conf.argtypes = [P, ctypes.c_int, ip]
     L.shp_draw_uniforms.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
     L.shp_set_proj.argtypes = [P, ctypes.c_int, dp]

@@ -92,6 +92,7 @@ __device__ static void sfmt_draw_block(uint32_t *g, size_t tot, Emit emit)
     }
     for (int i = threadIdx.x; i < 624; i += blockDim.x) g[i] = w[i];
     if (threadIdx.x == 0) g[624] = (uint32_t)idx;
+    __syncthreads();   // a following call in the same workgroup reads the state back
 }
 
 // one workgroup per chain; mode 0: fresh HS field conf(i,nt) = ceiling(u*lcomp) (nt outer);

@@ -1,12 +1,15 @@
 // pqmc.hip -- host side of the projective sweep: the C ABI of include/sunhip_proj.h.
 // The slice loops mirror src/dqmc_proj_core/dqmc_proj_sweep_b0.f90 and dqmc_proj_sweep_0b.f90
-// and the update mirrors src/dqmc_config_v/dqmc_proj_update_submatrix_LR.f90; every launch
-// advances all chains of the batch.  Included by sunhip.hip after the finite-temperature code.
+// and the updates mirror src/dqmc_config_v/dqmc_proj_update_submatrix_LR.f90 (bond field) and
+// src/dqmc_config_u/dqmc_proj_update_submatrix_LR.f90 (on-site field); every launch advances
+// all chains of the batch.  Included by sunhip.hip after the finite-temperature code.
 #include "../../include/sunhip_proj.h"
 
 struct shp_ctx : ctx_base {
     int nch = 0, n = 0, ne = 0, ltrot = 0, nwrap = 0, nst = 0, kmode = 0, ns = 0, mid_len = 1;
     int nn_nf = 0, nn_lf = 0, wrap_check = 1, pb = PB;
+    bool lwrapu = false, lwrapv = true;
+    size_t slice_draws = 0;       // uniforms per slice of a pass
     bool tracing = false, started = false;
     std::vector<int> iwrap_nt, wrap_step;
     std::vector<PChain> hc;
@@ -61,6 +64,12 @@ void op_v(POps &o, int f, int modeL, int modeR)
     o.kind[o.nops] = 2; o.arg[o.nops] = f; o.modeL[o.nops] = modeL; o.modeR[o.nops] = modeR;
     o.nops++;
 }
+// the on-site field: UR rows / UL columns times bmat (0) or bmat_inv (1)
+void op_u(POps &o, int modeL, int modeR)
+{
+    o.kind[o.nops] = 3; o.arg[o.nops] = 0; o.modeL[o.nops] = modeL; o.modeR[o.nops] = modeR;
+    o.nops++;
+}
 
 void launch_prop(shp_ctx *ctx, const POps &o)
 {
@@ -97,14 +106,16 @@ void run_ops(shp_ctx *ctx, const POps &o)
             if (part.nops) { launch_prop(ctx, part); part.nops = 0; }
             kin_dense(ctx, o.sides, o.kselL, o.kselR);
         } else {
-            op_v(part, o.arg[k], o.modeL[k], o.modeR[k]);
+            part.kind[part.nops] = o.kind[k]; part.arg[part.nops] = o.arg[k];
+            part.modeL[part.nops] = o.modeL[k]; part.modeR[part.nops] = o.modeR[k];
+            part.nops++;
         }
     }
     if (part.nops) launch_prop(ctx, part);
 }
 
-// ---- submatrix-LR update of family nf at slice nt; upos = (slice, family) index in the pass ----
-void update_family(shp_ctx *ctx, int nt, int nf, int upos)
+// ---- submatrix-LR update of family nf at slice nt; uoff = its first uniform in the pass ----
+void update_family(shp_ctx *ctx, int nt, int nf, size_t uoff0)
 {
     Tic t(ctx, T_UPDATE);
     const int ns = ctx->ns, ne = ctx->ne, nl = ctx->nn_lf;
@@ -117,11 +128,32 @@ void update_family(shp_ctx *ctx, int nt, int nf, int upos)
         // Rrows = urP ULRINV, Felms = Rrows ulP (:121-133)
         pgemm(ctx, P_URP, 0, P_ULRINV, 0, P_RROWS, 0, nup, ne, ne, ns, ne, ns);
         pgemm(ctx, P_RROWS, 0, P_ULP, 0, P_FELMS, 0, nup, nup, ne, ns, ne, ns);
-        const int uoff = 2 * (upos * nl + ib0);
+        const int uoff = (int)(uoff0 + 2 * (size_t)ib0);
         hipLaunchKernelGGL(kp_stock_bonds, dim3(ctx->nch), dim3(256), p_lds_bonds(ctx), ctx->st, ctx->dc, nt, nf, ib0,
                            nb, uoff);
         // Woodbury flush ULRINV -= (ULRINV Umat)(Gamma^-1 Vmat) over the whole stock width; rows
         // and columns past the accepted count are zero (:240-266)
+        pgemm(ctx, P_GINV, 0, P_VMAT, 0, P_VTMP, 0, ns, ne, ns, ns, ns, ns);
+        pgemm(ctx, P_ULRINV, 0, P_UMAT, 0, P_UTMP, 0, ne, ns, ne, ne, ne, ne);
+        pgemm(ctx, P_UTMP, 0, P_VTMP, 0, P_ULRINV, 0, ne, ne, ns, ne, ns, ne, -1.0, 1.0);
+        hipLaunchKernelGGL(kp_stock_scale, dim3((ns * ne + 255) / 256, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dc);
+    }
+}
+
+// ---- submatrix-LR update of the on-site field at slice nt, stocks of ns consecutive sites ----
+void update_u(shp_ctx *ctx, int nt, size_t uoff0)
+{
+    Tic t(ctx, T_UPDATE);
+    const int ns = ctx->ns, ne = ctx->ne, n = ctx->n;
+    for (int i0 = 0; i0 < n; i0 += ns) {
+        const int nup = std::min(ns, n - i0);
+        const int cnt = std::max(ns * ne, ns * ns);
+        hipLaunchKernelGGL(kp_stock_gather, dim3((cnt + 255) / 256, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dc,
+                           -1, i0, nup);
+        pgemm(ctx, P_URP, 0, P_ULRINV, 0, P_RROWS, 0, nup, ne, ne, ns, ne, ns);
+        pgemm(ctx, P_RROWS, 0, P_ULP, 0, P_FELMS, 0, nup, nup, ne, ns, ne, ns);
+        hipLaunchKernelGGL(kp_stock_sites, dim3(ctx->nch), dim3(256), p_lds_bonds(ctx), ctx->st, ctx->dc, nt, i0, nup,
+                           uoff0 + 2 * (size_t)i0);
         pgemm(ctx, P_GINV, 0, P_VMAT, 0, P_VTMP, 0, ns, ne, ns, ns, ns, ns);
         pgemm(ctx, P_ULRINV, 0, P_UMAT, 0, P_UTMP, 0, ne, ns, ne, ne, ne, ne);
         pgemm(ctx, P_UTMP, 0, P_VTMP, 0, P_ULRINV, 0, ne, ne, ns, ne, ns, ne, -1.0, 1.0);
@@ -171,6 +203,7 @@ bool in_window(const shp_ctx *ctx, int nt)
 
 void weights(shp_ctx *ctx, int op, int n, int phasediff)
 {
+    if (ctx->lwrapu && op <= 2) hipLaunchKernelGGL(kp_uphase, dim3(ctx->nch), dim3(256), 0, ctx->st, ctx->dc);
     hipLaunchKernelGGL(kp_weights, dim3((ctx->nch + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, op, n,
                        phasediff);
 }
@@ -218,10 +251,11 @@ void bmat_left_forward(shp_ctx *ctx, int nt1, int nt2)
     for (int nt = nt2; nt <= nt1; nt++) {
         POps o = ops_begin(nt, 1, 0, 0);
         op_kin(o, false);
-        for (int f = 0; f < ctx->nn_nf; f++) {
+        for (int f = 0; f < ctx->nn_nf && ctx->lwrapv; f++) {
             op_v(o, f, 0, 0);
             op_v(o, f, 1, 0);
         }
+        if (ctx->lwrapu) op_u(o, 0, 0);
         op_kin(o, true);
         run_ops(ctx, o);
     }
@@ -286,6 +320,9 @@ int shp_create(shp_ctx **out, int nchains, const shp_model *m, int device)
     ctx->n = q0.ndim; ctx->ne = q0.ne; ctx->ltrot = q0.ltrot; ctx->nwrap = q0.nwrap; ctx->nst = q0.nst;
     ctx->kmode = q0.kmode; ctx->ns = q0.nustock; ctx->mid_len = q0.mid_len; ctx->nn_nf = q0.nn_nf;
     ctx->nn_lf = q0.nn_lf; ctx->wrap_check = q0.wrap_check ? 1 : 0;
+    ctx->lwrapv = q0.vtab != nullptr;
+    ctx->lwrapu = q0.ubmat != nullptr;
+    ctx->slice_draws = (ctx->lwrapu ? (size_t)2 * q0.ndim : 0) + (ctx->lwrapv ? (size_t)2 * q0.nn_lf * q0.nn_nf : 0);
     for (int c = 0; c < nchains; c++) {
         const shp_model &q = m[c];
         if (q.ndim != ctx->n || q.ne != ctx->ne || q.ltrot != ctx->ltrot || q.nwrap != ctx->nwrap ||
@@ -295,9 +332,16 @@ int shp_create(shp_ctx **out, int nchains, const shp_model *m, int device)
             delete ctx;
             return rc;
         }
-        if (!q.iwrap_nt || !q.wrap_step || !q.bonds || !q.urt || !q.urtm1 || !q.vtab || !q.vdelta || !q.stag) {
+        if (!q.iwrap_nt || !q.wrap_step || !q.bonds || !q.urt || !q.urtm1 || (q.vtab && !q.vdelta) || !q.stag ||
+            (q.vtab != nullptr) != (m[0].vtab != nullptr) || (q.ubmat != nullptr) != (m[0].ubmat != nullptr)) {
             delete ctx;
             return SH_E_ARG;
+        }
+        if (q.ubmat && (q.ulcomp < 2 || q.ulcomp > 4 || !q.ubmat_inv || !q.udelta || !q.uratio || !q.ugaml ||
+                        !q.uphase_arg || q.nn_nf * 2 + 3 > POPS_MAX)) {
+            int rc = fail(ctx, SH_E_ARG, "incomplete on-site field tables");
+            delete ctx;
+            return rc;
         }
         if (q.ndim < 2 || q.ndim > 1024 || q.ne < 1 || q.ne > q.ndim || q.ne > 512 || q.ltrot < 1 || q.nst < 1 ||
             q.nustock < 2 || q.nustock > 64 || (q.nustock & 1) || q.nn_nf < 1 || q.nn_lf < 1 ||
@@ -328,6 +372,7 @@ int shp_create(shp_ctx **out, int nchains, const shp_model *m, int device)
     {
         struct { const void *f; size_t bytes; } big[] = {
             {(const void *)kp_prop, p_lds_prop(ctx)}, {(const void *)kp_stock_bonds, p_lds_bonds(ctx)},
+            {(const void *)kp_stock_sites, p_lds_bonds(ctx)},
             {(const void *)kp_qr_formq, p_lds_qr(ctx)}, {(const void *)kp_lu_inv, p_lds_lu(ctx)}};
         for (auto &b : big) {
             if (b.bytes <= 64 * 1024) continue;
@@ -366,7 +411,24 @@ int shp_create(shp_ctx **out, int nchains, const shp_model *m, int device)
         c.piv = dalloc<int32_t>(ctx, ne);
         c.conf = dalloc<int32_t>(ctx, (size_t)n * q.nn_nf * q.ltrot);
         c.xvec = dalloc<int32_t>(ctx, ns); c.svec = dalloc<int32_t>(ctx, ns); c.ikd = dalloc<int32_t>(ctx, 4);
-        c.unif = dalloc<double>(ctx, (size_t)2 * q.nn_lf * q.nn_nf * q.ltrot);
+        c.unif = dalloc<double>(ctx, std::max(ctx->slice_draws, (size_t)1) * q.ltrot);
+        c.lwrapu = ctx->lwrapu ? 1 : 0;
+        c.lwrapv = ctx->lwrapv ? 1 : 0;
+        if (ctx->lwrapu) {
+            c.ulcomp = q.ulcomp;
+            c.uphase_from_conf = q.uphase_from_conf ? 1 : 0;
+            c.uconf = dalloc<int32_t>(ctx, (size_t)n * q.ltrot);
+            for (int l = 0; l < q.ulcomp; l++) {
+                c.ub[l] = make_double2(q.ubmat[2 * l], q.ubmat[2 * l + 1]);
+                c.ubi[l] = make_double2(q.ubmat_inv[2 * l], q.ubmat_inv[2 * l + 1]);
+                c.ug[l] = q.ugaml[l];
+                c.upa[l] = q.uphase_arg[l];
+            }
+            for (int k = 0; k < (q.ulcomp - 1) * q.ulcomp; k++) {
+                c.ud[k] = make_double2(q.udelta[2 * k], q.udelta[2 * k + 1]);
+                c.uratio[k] = make_double2(q.uratio[2 * k], q.uratio[2 * k + 1]);
+            }
+        }
         c.sc = dalloc<double>(ctx, SC_SIZE);
         c.logwDV = dalloc<double>(ctx, q.nst + 2);
         c.rwork = dalloc<double>(ctx, 4 * (size_t)n);
@@ -380,15 +442,18 @@ int shp_create(shp_ctx **out, int nchains, const shp_model *m, int device)
             !c.Ust || !c.proj || !c.urP || !c.ulP || !c.Rrows || !c.Felms || !c.Ginv || !c.Vmat || !c.Umat ||
             !c.Vtmp || !c.Utmp || !c.Dvec || !c.tau || !c.tblk || !c.wk || !c.piv || !c.conf || !c.xvec ||
             !c.svec || !c.ikd || !c.unif || !c.sc || !c.logwDV || !c.rwork || !c.rng || !c.bonds || !c.kt ||
-            !c.ktm1 || !c.stag) {
+            !c.ktm1 || !c.stag || (ctx->lwrapu && !c.uconf)) {
             int rc = fail(ctx, SH_E_HIP, "device allocation failed");
             shp_destroy(ctx);
             return rc;
         }
-        for (int k = 0; k < 8; k++) c.vt[k] = make_double2(q.vtab[2 * k], q.vtab[2 * k + 1]);
-        for (int k = 0; k < 4; k++) c.vd[k] = make_double2(q.vdelta[2 * k], q.vdelta[2 * k + 1]);
+        if (q.vtab) {
+            for (int k = 0; k < 8; k++) c.vt[k] = make_double2(q.vtab[2 * k], q.vtab[2 * k + 1]);
+            for (int k = 0; k < 4; k++) c.vd[k] = make_double2(q.vdelta[2 * k], q.vdelta[2 * k + 1]);
+        }
         std::vector<int32_t> ones((size_t)n * q.nn_nf * q.ltrot, 1);
         hipMemcpy(c.conf, ones.data(), ones.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+        if (ctx->lwrapu) hipMemcpy(c.uconf, ones.data(), (size_t)n * q.ltrot * sizeof(int32_t), hipMemcpyHostToDevice);
         double sc0[SC_SIZE] = {0};
         sc0[SC_PHASE] = 1.0;
         hipMemcpy(c.sc, sc0, sizeof(sc0), hipMemcpyHostToDevice);
@@ -438,6 +503,29 @@ int shp_get_conf(shp_ctx *ctx, int chain, int32_t *conf)
     if (!ctx || chain < 0 || chain >= ctx->nch || !conf) return SH_E_ARG;
     const size_t n = (size_t)ctx->n * ctx->nn_nf * ctx->ltrot;
     HIPCHK(hipMemcpyAsync(conf, ctx->hc[chain].conf, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int shp_set_uconf(shp_ctx *ctx, int chain, const int32_t *conf)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !conf) return SH_E_ARG;
+    if (!ctx->lwrapu) return fail(ctx, SH_E_STATE, "the model has no on-site field");
+    const PChain &c = ctx->hc[chain];
+    const size_t n = (size_t)ctx->n * ctx->ltrot;
+    for (size_t k = 0; k < n; k++)
+        if (conf[k] < 1 || conf[k] > c.ulcomp) return fail(ctx, SH_E_ARG, "on-site field values out of 1..ulcomp");
+    HIPCHK(hipMemcpyAsync(c.uconf, conf, n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int shp_get_uconf(shp_ctx *ctx, int chain, int32_t *conf)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !conf) return SH_E_ARG;
+    if (!ctx->lwrapu) return fail(ctx, SH_E_STATE, "the model has no on-site field");
+    const size_t n = (size_t)ctx->n * ctx->ltrot;
+    HIPCHK(hipMemcpyAsync(conf, ctx->hc[chain].uconf, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     return 0;
 }
@@ -500,12 +588,23 @@ int shp_sweep_b0(shp_ctx *ctx, int lupdate, int lmeasure)
         if (lmeasure && in_window(ctx, nt)) measure(ctx);
         // K half (right_forward on UL with urt, left_backward on UR with urtm1), then the
         // rotation of the last family (nflag 2)
+        // the on-site field first: its update, then right_forward on UL (bmat) and
+        // left_backward on UR (bmat_inv) (dqmc_proj_sweep_b0.f90:72-77)
+        const size_t sb = (size_t)s * ctx->slice_draws;
+        const size_t ub = ctx->lwrapu ? (size_t)2 * n : 0;
         POps o = ops_begin(nt, 3, 1, 0);
         op_kin(o, false);
-        op_v(o, nf - 1, 1, 1);
+        if (ctx->lwrapu) {
+            run_ops(ctx, o);
+            if (lupdate) update_u(ctx, nt, sb);
+            o = ops_begin(nt, 3, 1, 0);
+            op_u(o, 1, 0);
+        }
+        if (ctx->lwrapv) op_v(o, nf - 1, 1, 1);
+        else op_kin(o, true);
         run_ops(ctx, o);
-        for (int f = nf - 1; f >= 0; f--) {
-            if (lupdate) update_family(ctx, nt, f, s * nf + (nf - 1 - f));
+        for (int f = nf - 1; f >= 0 && ctx->lwrapv; f--) {
+            if (lupdate) update_family(ctx, nt, f, sb + ub + (size_t)2 * ctx->nn_lf * (nf - 1 - f));
             POps p = ops_begin(nt, 3, 1, 0);
             op_v(p, f, 2, 2);                       // nflag 1 of this family
             if (f > 0) op_v(p, f - 1, 1, 1);        // nflag 2 of the next one
@@ -542,16 +641,29 @@ int shp_sweep_0b(shp_ctx *ctx, int lupdate, int lmeasure, int ldyn)
     for (int nt = 1; nt <= ctx->ltrot; nt++) {
         // K half (left_forward on UR with urt, right_backward on UL with urtm1), then nflag 2
         // of the first family
+        // the bond families, then the on-site field: left_forward on UR (bmat), right_backward
+        // on UL (bmat_inv), its update (dqmc_proj_sweep_0b.f90:70-86), then the K half
+        const size_t sb = (size_t)(nt - 1) * ctx->slice_draws;
+        const size_t vb = ctx->lwrapv ? (size_t)2 * ctx->nn_lf * nf : 0;
         POps o = ops_begin(nt, 3, 0, 1);
         op_kin(o, false);
-        op_v(o, 0, 0, 0);
+        if (ctx->lwrapv) op_v(o, 0, 0, 0);
+        else if (ctx->lwrapu) op_u(o, 0, 1);
+        else op_kin(o, true);
         run_ops(ctx, o);
-        for (int f = 0; f < nf; f++) {
-            if (lupdate) update_family(ctx, nt, f, (nt - 1) * nf + f);
+        for (int f = 0; f < nf && ctx->lwrapv; f++) {
+            if (lupdate) update_family(ctx, nt, f, sb + (size_t)2 * ctx->nn_lf * f);
             POps p = ops_begin(nt, 3, 0, 1);
             op_v(p, f, 1, 1);                       // nflag 1 of this family
             if (f + 1 < nf) op_v(p, f + 1, 0, 0);   // nflag 2 of the next one
+            else if (ctx->lwrapu) op_u(p, 0, 1);
             else op_kin(p, true);
+            run_ops(ctx, p);
+        }
+        if (ctx->lwrapu) {
+            if (lupdate) update_u(ctx, nt, sb + vb);
+            POps p = ops_begin(nt, 3, 0, 1);
+            op_kin(p, true);
             run_ops(ctx, p);
         }
         if (lmeasure && in_window(ctx, nt)) measure(ctx);

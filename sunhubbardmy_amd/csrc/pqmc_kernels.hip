@@ -8,7 +8,9 @@
 
 // scalar slots of the projective chain beyond the shared layout (SC_PHASE, SC_LOGW, SC_ZLD,
 // SC_INFO, SC_OBS ... keep their meaning)
-enum { PS_RLOG = 30, PS_ISGN = 31 };
+// (the projective sweep has no saved chain point, so SC_LDQ_SAVE's first slot holds the angle
+// sum of the on-site field's phases)
+enum { PS_RLOG = 30, PS_ISGN = 31, PS_UANG = SC_LDQ_SAVE };
 
 enum {
     P_UR = 0, P_UL, P_ULRINV, P_ULR, P_T1, P_T2, P_ULO, P_URO, P_INV2, P_G, P_G2, P_KT, P_KTM1,
@@ -28,7 +30,18 @@ struct PChain {
     const double *stag;
     cplx vt[8];   // (bp, bm, bpi, bmi) for field value 1 then 2
     cplx vd[4];   // (dp, dm) for field value 1 then 2
+    // the on-site field (src/dqmc_config_u), when lwrapu: conf_u [ltrot][n], values 1..ulcomp
+    int lwrapu, lwrapv, ulcomp, uphase_from_conf;
+    int32_t *uconf;
+    cplx ub[4], ubi[4], ud[12], uratio[12];   // bmat, bmat_inv, delta_bmat(iflip, is), phase_ratio
+    double ug[4], upa[4];                      // gaml, arg of phase(is)
 };
+
+// uniforms of one slice in a pass: 2 per site of the on-site field, 2 per bond of every family
+__host__ __device__ __forceinline__ size_t p_slice_draws(const PChain &c)
+{
+    return (c.lwrapu ? (size_t)2 * c.n : 0) + (c.lwrapv ? (size_t)2 * c.nn_lf * c.nn_nf : 0);
+}
 
 __device__ __forceinline__ cplx *pmat(const PChain &c, int sel, int pt)
 {
@@ -141,6 +154,8 @@ kp_add_identity(const PChain *chains, int sel)
 //   right 1: cols <- cols u                 (right_backward nflag 1, right_forward nflag 2)
 //   right 2: cols <- cols diag(bp,bm) ut    (dqmc_right_forward_prop.f90, nflag 1)
 // u = ut = [[1, 1], [1, -1]] / sqrt(2) (src/dqmc_config_v.f90:19-20).
+// kind 3: the on-site field, UR rows / UL columns times bmat (mode 0) or bmat_inv (mode 1) of
+// the site's value (src/dqmc_config_u/dqmc_left_forward_prop.f90:23-32 and its three siblings).
 #define POPS_MAX 12
 struct POps {
     int nops, nt, kselL, kselR, sides;   // sides: 1 = UR, 2 = UL
@@ -206,6 +221,16 @@ kp_prop(const PChain *chains, POps ops)
                 if (c.kmode == 0) cb_left_fams(c, lds, ncol, ops.kselL ? c.ktm1 : c.kt, ops.kind[o] == 1);
                 continue;
             }
+            if (ops.kind[o] == 3) {
+                const int32_t *uc = c.uconf + (size_t)(ops.nt - 1) * n;
+                for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                    const int is = uc[i] - 1;
+                    const cplx fct = ops.modeL[o] ? c.ubi[is] : c.ub[is];
+                    for (int cc = 0; cc < ncol; cc++) lds[cc * n + i] = cmul(fct, lds[cc * n + i]);
+                }
+                __syncthreads();
+                continue;
+            }
             const int f = ops.arg[o], mode = ops.modeL[o];
             const int32_t *cf = c.conf + ((size_t)(ops.nt - 1) * c.nn_nf + f) * n;
             for (int bb = bt; bb < nl; bb += blockDim.x) {
@@ -239,6 +264,18 @@ kp_prop(const PChain *chains, POps ops)
                 if (c.kmode == 0) cb_right_fams(c, lds, nrow, ops.kselR ? c.ktm1 : c.kt, ops.kind[o] == 1);
                 continue;
             }
+            if (ops.kind[o] == 3) {
+                const int32_t *uc = c.uconf + (size_t)(ops.nt - 1) * n;
+                for (int e = threadIdx.x; e < n * RPB; e += blockDim.x) {
+                    const int rr = e % RPB, col = e / RPB;
+                    const int is = uc[col] - 1;
+                    const cplx fct = ops.modeR[o] ? c.ubi[is] : c.ub[is];
+                    lds[e] = cmul(lds[e], fct);
+                    (void)rr;
+                }
+                __syncthreads();
+                continue;
+            }
             const int f = ops.arg[o], mode = ops.modeR[o];
             const int32_t *cf = c.conf + ((size_t)(ops.nt - 1) * c.nn_nf + f) * n;
             for (int bb = bt; bb < nl; bb += blockDim.x) {
@@ -265,29 +302,29 @@ kp_prop(const PChain *chains, POps ops)
 // ---- submatrix-LR update (src/dqmc_config_v/dqmc_proj_update_submatrix_LR.f90) -----------
 // stock start: the stock's rows of UR (urP, nup x ne, ld ns) and columns of UL (ulP, ne x
 // nup, ld ne) in bond order (i1, i2 per bond); Gamma^-1, Vmat, Umat and the count cleared
+// nf < 0: the on-site field's stock, the consecutive sites ib0 .. ib0 + nup - 1
 extern "C" __global__ void __launch_bounds__(256)
 kp_stock_gather(const PChain *chains, int nf, int ib0, int nup)
 {
     const PChain &c = chains[blockIdx.z];
     const int ns = c.ns, ne = c.ne, n = c.n;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    auto site = [&](int s) {
+        if (nf < 0) return ib0 + s;
+        const int ist = nf * c.nn_lf + ib0 + s / 2;
+        return (int)c.bonds[2 * ist + (s & 1)];
+    };
     if (e < ns * ne) {
         const int s = e % ns, l = e / ns;       // urP(s, l)
         cplx v = make_double2(0.0, 0.0);
-        if (s < nup) {
-            const int ist = nf * c.nn_lf + ib0 + s / 2;
-            v = c.UR[(size_t)c.bonds[2 * ist + (s & 1)] + (size_t)l * n];
-        }
+        if (s < nup) v = c.UR[(size_t)site(s) + (size_t)l * n];
         c.urP[e] = v;
         c.Vmat[e] = make_double2(0.0, 0.0);
     }
     if (e < ne * ns) {
         const int l = e % ne, s = e / ne;       // ulP(l, s)
         cplx v = make_double2(0.0, 0.0);
-        if (s < nup) {
-            const int ist = nf * c.nn_lf + ib0 + s / 2;
-            v = c.UL[(size_t)l + (size_t)c.bonds[2 * ist + (s & 1)] * ne];
-        }
+        if (s < nup) v = c.UL[(size_t)l + (size_t)site(s) * ne];
         c.ulP[e] = v;
         c.Umat[e] = make_double2(0.0, 0.0);
     }
@@ -464,6 +501,155 @@ kp_stock_bonds(const PChain *chains, int nt, int nf, int ib0, int nb, int uoff)
     }
 }
 
+// the on-site field's stock of consecutive sites i0 .. i0 + nup - 1, one workgroup per chain
+// (src/dqmc_config_u/dqmc_proj_update_submatrix_LR.f90:83-237): per site the 1 x 1 ratio
+// 1 + (F_kk - F_kP Gamma^-1 F_Pk) Delta(iflip, is), ratiotot = ratio1^nflr gaml(isp)/gaml(is)
+// phase_ratio(iflip, is), the Metropolis test on Re(ratiotot phase)/Re(phase), and on
+// acceptance Gamma^-1 bordered by one row and column.  Felms and Gamma^-1 live in LDS.
+extern "C" __global__ void __launch_bounds__(256)
+kp_stock_sites(const PChain *chains, int nt, int i0, int nup, size_t uoff)
+{
+    extern __shared__ cplx bsh[];
+    const PChain &c = chains[blockIdx.x];
+    const int ns = c.ns, ne = c.ne, lc = c.ulcomp;
+    cplx *F = bsh;                    // Felms, ld ns
+    cplx *Gi = bsh + ns * ns;         // Gamma^-1, ld ns
+    __shared__ cplx ikk[64], kik[64], s_vuk, s_gkk;
+    __shared__ int svec[64];
+    __shared__ int s_acc, s_ik;
+    __shared__ double s_accm;
+    __shared__ cplx s_phase;
+    for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
+        F[e] = c.Felms[e];
+        Gi[e] = make_double2(0.0, 0.0);
+    }
+    if (threadIdx.x == 0) {
+        s_ik = 0;
+        s_accm = 0.0;
+        s_phase = make_double2(c.sc[SC_PHASE], c.sc[SC_PHASE + 1]);
+    }
+    __syncthreads();
+    int32_t *cf = c.uconf + (size_t)(nt - 1) * c.n + i0;
+    const double *u = c.unif + uoff;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int q = 0; q < nup; q++) {
+        const int ik = s_ik;
+        if (ik > 0) {
+            // ikk = Gamma^-1 F(P, q): four threads per entry, each a quarter of the sum
+            {
+                const int m = threadIdx.x >> 2, h = threadIdx.x & 3;
+                double ar = 0.0, ai = 0.0;
+                if (m < ik)
+                    for (int p = h; p < ik; p += 4) {
+                        const cplx g = Gi[m + p * ns], f = F[svec[p] + q * ns];
+                        ar += g.x * f.x - g.y * f.y;
+                        ai += g.x * f.y + g.y * f.x;
+                    }
+                ar += __shfl_xor(ar, 1, 64);
+                ai += __shfl_xor(ai, 1, 64);
+                ar += __shfl_xor(ar, 2, 64);
+                ai += __shfl_xor(ai, 2, 64);
+                if (m < ik && h == 0) ikk[m] = make_double2(ar, ai);
+            }
+            __syncthreads();
+            // vuk = F(q, P) ikk: wave 0, one lane per term
+            if (w == 0) {
+                double ar = 0.0, ai = 0.0;
+                if (lane < ik) {
+                    const cplx f = F[q + svec[lane] * ns], x = ikk[lane];
+                    ar = f.x * x.x - f.y * x.y;
+                    ai = f.x * x.y + f.y * x.x;
+                }
+                ar = wave_sum(ar);
+                ai = wave_sum(ai);
+                if (lane == 0) s_vuk = make_double2(ar, ai);
+            }
+        } else if (threadIdx.x == 0) {
+            s_vuk = make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int is = cf[q];
+            const int iflip = (int)ceil(u[2 * q] * (lc - 1));
+            const int isp = (is + iflip - 1) % lc + 1;
+            const int k = (iflip - 1) + (lc - 1) * (is - 1);
+            const cplx d = c.ud[k];
+            const cplx ratio1 = cadd(make_double2(1.0, 0.0), cmul(csub(F[q + q * ns], s_vuk), d));
+            cplx rt = ratio1;
+            for (int f = 1; f < c.nflr; f++) rt = cmul(rt, ratio1);
+            rt = cmul(cscale(rt, c.ug[isp - 1] / c.ug[is - 1]), c.uratio[k]);
+            const cplx ph = s_phase;
+            const double ratio_re = cmul(rt, ph).x / ph.x;
+            const double ratio_abs = ratio_re < 0.0 ? -ratio_re : ratio_re;
+            const int acc = ratio_abs > u[2 * q + 1];
+            s_acc = acc;
+            if (acc) {
+                s_accm += 1.0;
+                const double wgt = sqrt(cmul(rt, cconj(rt)).x);
+                s_phase = cscale(cmul(ph, rt), 1.0 / wgt);
+                s_gkk = cdiv(d, ratio1);
+                cf[q] = isp;
+                c.xvec[ik] = i0 + q;
+                c.Dvec[ik] = d;
+            }
+        }
+        __syncthreads();
+        if (!s_acc) continue;
+        const cplx gkk = s_gkk;
+        if (ik > 0) {
+            // kik = F(q, P) Gamma^-1: four threads per entry
+            {
+                const int p = threadIdx.x >> 2, h = threadIdx.x & 3;
+                double ar = 0.0, ai = 0.0;
+                if (p < ik)
+                    for (int m = h; m < ik; m += 4) {
+                        const cplx f = F[q + svec[m] * ns], g = Gi[m + p * ns];
+                        ar += f.x * g.x - f.y * g.y;
+                        ai += f.x * g.y + f.y * g.x;
+                    }
+                ar += __shfl_xor(ar, 1, 64);
+                ai += __shfl_xor(ai, 1, 64);
+                ar += __shfl_xor(ar, 2, 64);
+                ai += __shfl_xor(ai, 2, 64);
+                if (p < ik && h == 0) kik[p] = make_double2(ar, ai);
+            }
+            __syncthreads();
+            // 11 block -= ikk (-gkk kik); 12 block = -ikk gkk; 21 block = -gkk kik
+            for (int e = threadIdx.x; e < ik * ik; e += blockDim.x) {
+                const int m = e % ik, p = e / ik;
+                const cplx n21 = cmul(gkk, kik[p]);          // minus the 21 entry
+                Gi[m + p * ns] = cadd(Gi[m + p * ns], cmul(ikk[m], n21));
+            }
+            for (int e = threadIdx.x; e < ik; e += blockDim.x) {
+                const cplx a = cmul(ikk[e], gkk), b = cmul(gkk, kik[e]);
+                Gi[e + ik * ns] = make_double2(-a.x, -a.y);
+                Gi[ik + e * ns] = make_double2(-b.x, -b.y);
+            }
+        }
+        if (threadIdx.x == 0) {
+            Gi[ik + ik * ns] = gkk;
+            svec[ik] = q;
+            s_ik = ik + 1;
+        }
+        __syncthreads();
+    }
+    const int ik = s_ik;
+    for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) c.Ginv[e] = Gi[e];
+    for (int e = threadIdx.x; e < ik * ne; e += blockDim.x) {
+        const int m = e % ik, l = e / ik;
+        c.Vmat[m + (size_t)l * ns] = c.Rrows[svec[m] + (size_t)l * ns];
+        const int l2 = e % ne, m2 = e / ne;
+        c.Umat[l2 + (size_t)m2 * ne] = c.ulP[l2 + (size_t)svec[m2] * ne];
+    }
+    if (threadIdx.x == 0) {
+        c.ikd[0] = ik;
+        c.sc[SC_PHASE] = s_phase.x;
+        c.sc[SC_PHASE + 1] = s_phase.y;
+        c.sc[SC_ACC] += s_accm;
+        c.sc[SC_TRIED] += nup;
+    }
+}
+
 // UR(x_m, :) += D_m UR(x_m, :) for the stock's accepted rows (:249-254)
 extern "C" __global__ void __launch_bounds__(256)
 kp_stock_scale(const PChain *chains)
@@ -563,7 +749,8 @@ extern "C" __global__ void kp_weights(const PChain *chains, int nchains, int op,
         dv = (n == 0) ? lw[n + 1] : lw[n + 1] + lw[n];
     }
     const cplx logw = make_double2(dv + z.x, z.y);
-    const double a = (double)c.nflr * logw.y;
+    // set_phase: exp(i nflr Im logw) times the on-site field's phases (angle sum from kp_uphase)
+    const double a = (double)c.nflr * logw.y + ((c.lwrapu && c.uphase_from_conf) ? sc[PS_UANG] : 0.0);
     const cplx ph = make_double2(cos(a), sin(a));
     if (phasediff) {
         const cplx old = make_double2(sc[SC_PHASE], sc[SC_PHASE + 1]);
@@ -616,9 +803,27 @@ kp_wrap_check(const PChain *chains, int compare, int trace)
     }
 }
 
-// draws of the projective chain: mode 0 the fresh bond field (N nn_nf ltrot draws, conf(i,nf,nt)
-// with nf innermost), mode 1 the pass's 2 nn_lf nn_nf ltrot uniforms, mode 2 count uniforms of
-// chain `which` into out
+// sum over the on-site field of the arguments of phase(conf) into sc[PS_UANG] (set_phase,
+// src/dqmc_proj_core/set_phase.f90:16-22); one workgroup per chain
+extern "C" __global__ void __launch_bounds__(256)
+kp_uphase(const PChain *chains)
+{
+    const PChain &c = chains[blockIdx.x];
+    __shared__ double red[4];
+    double a = 0.0;
+    if (c.lwrapu)
+        for (size_t k = threadIdx.x; k < (size_t)c.n * c.ltrot; k += blockDim.x) a += c.upa[c.uconf[k] - 1];
+    a = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) c.sc[PS_UANG] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// draws of the projective chain: mode 0 the fresh fields (the on-site field first, N ltrot
+// draws conf(i,nt) = ceiling(u lcomp) with i inner, as u0conf%input_conf precedes
+// v0conf%input_conf in dqmc_start; then the bond field, N nn_nf ltrot draws, conf(i,nf,nt)
+// with nf innermost), mode 1 the pass's uniforms (per slice 2 N for the on-site field and
+// 2 nn_lf nn_nf for the bonds), mode 2 count uniforms of chain `which` into out
 extern "C" __global__ void __launch_bounds__(64)
 kp_draw(const PChain *chains, int mode, int which, int count, double *out)
 {
@@ -626,14 +831,21 @@ kp_draw(const PChain *chains, int mode, int which, int count, double *out)
     const PChain &c = chains[ci];
     if (mode == 0) {
         const int n = c.n, nf = c.nn_nf;
-        int32_t *conf = c.conf;
-        sfmt_draw_block(c.rng, (size_t)n * nf * c.ltrot, [=](size_t k, double u) {
-            const size_t f = k % nf, i = (k / nf) % n, t = k / ((size_t)nf * n);
-            conf[(t * nf + f) * n + i] = (int32_t)ceil(u * 2.0);
-        });
+        if (c.lwrapu) {
+            int32_t *uc = c.uconf;
+            const int lc = c.ulcomp;
+            sfmt_draw_block(c.rng, (size_t)n * c.ltrot, [=](size_t k, double u) { uc[k] = (int32_t)ceil(u * lc); });
+        }
+        if (c.lwrapv) {
+            int32_t *conf = c.conf;
+            sfmt_draw_block(c.rng, (size_t)n * nf * c.ltrot, [=](size_t k, double u) {
+                const size_t f = k % nf, i = (k / nf) % n, t = k / ((size_t)nf * n);
+                conf[(t * nf + f) * n + i] = (int32_t)ceil(u * 2.0);
+            });
+        }
     } else if (mode == 1) {
         double *un = c.unif;
-        sfmt_draw_block(c.rng, (size_t)2 * c.nn_lf * c.nn_nf * c.ltrot, [=](size_t k, double u) { un[k] = u; });
+        sfmt_draw_block(c.rng, p_slice_draws(c) * c.ltrot, [=](size_t k, double u) { un[k] = u; });
     } else {
         sfmt_draw_block(c.rng, (size_t)count, [=](size_t k, double u) { out[k] = u; });
     }

@@ -24,7 +24,7 @@ enum {
     SC_TRACE_N = 26,
     SC_LDQ_SAVE = 27,  // 2: logdetQ of the saved chain point
     SC_NONFINITE = 29,
-    SC_SIZE = 32
+    SC_SIZE = 32       // 30, 31: per-sweep-type extras
 };
 
 // matrix selectors resolved per chain on the device

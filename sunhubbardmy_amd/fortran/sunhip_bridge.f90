@@ -34,6 +34,10 @@ module sunhip_bridge
      type(c_ptr) :: bonds, urt, urtm1, vtab, vdelta, stag
      integer(c_int) :: lq, nflr, nustock, mid_len, wrap_check
      integer(c_int32_t) :: seed
+     ! the on-site field (c_null_ptr ubmat: none); vtab = c_null_ptr: no bond field
+     integer(c_int) :: ulcomp
+     type(c_ptr) :: ubmat, ubmat_inv, udelta, uratio, ugaml, uphase_arg
+     integer(c_int) :: uphase_from_conf
   end type shp_model
 
   interface
@@ -135,6 +139,18 @@ module sunhip_bridge
        integer(c_int), value :: chain
        integer(c_int32_t), intent(out) :: conf(*)
      end function shp_get_conf
+     integer(c_int) function shp_set_uconf(ctx, chain, conf) bind(C, name="shp_set_uconf")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(in) :: conf(*)
+     end function shp_set_uconf
+     integer(c_int) function shp_get_uconf(ctx, chain, conf) bind(C, name="shp_get_uconf")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(out) :: conf(*)
+     end function shp_get_uconf
      integer(c_int) function shp_draw_uniforms(ctx, chain, n, u) bind(C, name="shp_draw_uniforms")
        import :: c_ptr, c_int, c_double
        type(c_ptr), value :: ctx

@@ -1,12 +1,13 @@
-"""ProjWalkers: a batch of projective (ground-state) DQMC chains of the spinless t-V model on
-one GPU.
+"""ProjWalkers: a batch of projective (ground-state) DQMC chains on one GPU: the t-V model's
+bond field (rv != 0) and/or the Hubbard model's on-site field (rhub != 0).
 
 Host mirror of the reference's projective interface (src/dqmc_proj_ctrl.f90:8-129 calls
 these): ``allocate_core`` -> ``ProjWalkers(models)``, ``v0conf%input_conf`` -> ``init_conf``,
 ``dqmc_set_proj`` -> ``set_trial`` (or ``set_proj``), ``dqmc_proj_sweep_start_0b`` ->
 ``sweep_start_0b``, ``dqmc_proj_sweep_b0 / _0b`` -> ``sweep_b0 / sweep_0b``.  Tables restate
 ``v0conf%set_conf`` (src/dqmc_config_v/dqmc_set_conf.f90:14-77, lprojv = .false.),
-``set_trial_h0`` (src/dqmc_config_h0/set_trial_h0.f90:1-29, expar = 1) and ``make_tables``
+``u0conf%set_conf`` (src/dqmc_config_u/dqmc_set_conf.f90, lproju = .false., via
+``model.hs_tables``), ``set_trial_h0`` (src/dqmc_config_h0/set_trial_h0.f90:1-29, expar = 1) and ``make_tables``
 (src/model_para.f90:376-453).  All compute goes through libsunhip.so.
 """
 import ctypes
@@ -15,7 +16,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .model import KMODES, default_nublock, kinetic_tables, lattice_tables, schedule
+from .model import KMODES, Model, default_nublock, hs_tables, kinetic_tables, lattice_tables, schedule
 from .walkers import TIMER_GROUPS, SweepError
 
 
@@ -38,6 +39,8 @@ class ProjModel:
     mid_len: int = 1           # obs_eqt_mid_len
     wrap_check: bool = True
     seed: int = 1909706392
+    rhub: float = 0.0          # on-site U (the uconf field when != 0)
+    exact_hs: bool = True      # EXACTHS tables of the on-site field
 
 
 def vconf_tables(m):
@@ -62,9 +65,13 @@ def make_proj_tables(m):
     ns = m.nustock if m.nustock > 0 else default_nublock(n, "submatrix")
     ns = min(ns, 64)
     ns -= ns % 2
+    u = None
+    if m.rhub != 0.0:
+        u = hs_tables(Model(dtau=m.dtau, rhub=m.rhub, nu=m.nu, nflr=m.nflr, exact_hs=m.exact_hs))
     return {"model": m, "lat": lat, "ndim": n, "ne": n // 2 + int(m.nu * n / 8.0), "lq": lat["lq"],
             "ltrot": ltrot, "nst": nst, "nwrap": nw, "iwrap_nt": iw, "wrap_step": ws, "urt": urt,
-            "urtm1": urtm1, "v": vconf_tables(m), "nustock": max(ns, 2), "kmode": KMODES[m.kmode]}
+            "urtm1": urtm1, "v": vconf_tables(m) if m.rv != 0.0 else None, "u": u, "nustock": max(ns, 2),
+            "kmode": KMODES[m.kmode]}
 
 
 def trial_hamiltonian(m, lat, u):
@@ -92,6 +99,8 @@ class ShpModel(ctypes.Structure):
         ("urt", _lib.dp), ("urtm1", _lib.dp), ("vtab", _lib.dp), ("vdelta", _lib.dp), ("stag", _lib.dp),
         ("lq", ctypes.c_int), ("nflr", ctypes.c_int), ("nustock", ctypes.c_int), ("mid_len", ctypes.c_int),
         ("wrap_check", ctypes.c_int), ("seed", ctypes.c_int32),
+        ("ulcomp", ctypes.c_int), ("ubmat", _lib.dp), ("ubmat_inv", _lib.dp), ("udelta", _lib.dp),
+        ("uratio", _lib.dp), ("ugaml", _lib.dp), ("uphase_arg", _lib.dp), ("uphase_from_conf", ctypes.c_int),
     ]
 
 
@@ -134,8 +143,19 @@ class ProjWalkers:
         s.bonds = keep(lat["bonds"].reshape(-1), np.int32)
         s.urt = keep(cz(t["urt"]), np.float64)
         s.urtm1 = keep(cz(t["urtm1"]), np.float64)
-        s.vtab = keep(cz(t["v"]["vt"].reshape(-1)), np.float64)
-        s.vdelta = keep(cz(t["v"]["vd"].reshape(-1)), np.float64)
+        if t["v"] is not None:
+            s.vtab = keep(cz(t["v"]["vt"].reshape(-1)), np.float64)
+            s.vdelta = keep(cz(t["v"]["vd"].reshape(-1)), np.float64)
+        hs = t["u"]
+        if hs is not None:
+            s.ulcomp = hs["lcomp"]
+            s.ubmat = keep(cz(hs["bmat"]), np.float64)
+            s.ubmat_inv = keep(cz(hs["bmat_inv"]), np.float64)
+            s.udelta = keep(cz(hs["delta_bmat"].reshape(-1, order="F")), np.float64)
+            s.uratio = keep(cz(hs["phase_ratio"].reshape(-1, order="F")), np.float64)
+            s.ugaml = keep(hs["gaml"], np.float64)
+            s.uphase_arg = keep(hs["phase_arg"], np.float64)
+            s.uphase_from_conf = int(m.rhub > 0.0)
         s.stag = keep(lat["stag"], np.float64)
         s.lq, s.nflr, s.nustock, s.mid_len = t["lq"], m.nflr, t["nustock"], m.mid_len
         s.wrap_check, s.seed = int(bool(m.wrap_check)), int(np.int32(m.seed))
@@ -169,6 +189,16 @@ class ProjWalkers:
     def set_conf(self, chain, conf):
         c = np.ascontiguousarray(conf, dtype=np.int32).reshape(self.ltrot, self.nn_nf, self.ndim)
         self._check(_lib.lib().shp_set_conf(self._ctx, chain, c.ctypes.data_as(_lib.ip)), "set_conf")
+
+    def get_uconf(self, chain=0):
+        """the on-site field [ltrot, ndim] (models with rhub != 0)"""
+        c = np.zeros((self.ltrot, self.ndim), np.int32)
+        self._check(_lib.lib().shp_get_uconf(self._ctx, chain, c.ctypes.data_as(_lib.ip)), "get_uconf")
+        return c
+
+    def set_uconf(self, chain, conf):
+        c = np.ascontiguousarray(conf, dtype=np.int32).reshape(self.ltrot, self.ndim)
+        self._check(_lib.lib().shp_set_uconf(self._ctx, chain, c.ctypes.data_as(_lib.ip)), "set_uconf")
 
     def draw_uniforms(self, chain, count):
         out = np.zeros(count)

@@ -1,5 +1,6 @@
-"""GPU parity of the projective (t-V, submatrix-LR) sweep through the C ABI of
-include/sunhip_proj.h against the CPU oracle on identical seeds.
+"""GPU parity of the projective (submatrix-LR) sweep through the C ABI of include/sunhip_proj.h
+against the CPU oracle on identical seeds: the t-V bond field, the Hubbard on-site field and
+both together.
 
 Bar: the bond field after every pass is bit-identical (identical accept/reject sequences), G
 after every stabilization within 1e-10 relative max-norm, the phase within 1e-10, the
@@ -26,6 +27,11 @@ CASES = {
     "honeycomb3_checkerboard_stock6": dict(la=3, lb=3, kmode="checkerboard", nustock=6),
     "square4_checkerboard_nwrap3": dict(lattice="square", la=4, lb=4, kmode="checkerboard", nwrap=3, nustock=4),
     "honeycomb3_dense_stock64_v2": dict(la=3, lb=3, nustock=64, rv=2.0, beta=1.2, nwrap=4),
+    # the Hubbard on-site field (src/dqmc_config_u): alone, with 4 components, and with bonds
+    "square4_hubbard_cb": dict(lattice="square", la=4, lb=4, rv=0.0, rhub=4.0, nflr=2, kmode="checkerboard",
+                               nustock=6, nwrap=4),
+    "square4_su4_dense": dict(lattice="square", la=4, lb=4, rv=0.0, rhub=3.0, nflr=4, kmode="dense", nustock=8),
+    "honeycomb3_tuv": dict(la=3, lb=3, rv=1.0, rhub=3.0, nflr=2, nustock=8, kmode="checkerboard"),
 }
 
 
@@ -46,6 +52,8 @@ def run_parity(kw, nsweeps=2, chain=1):
     w.init_conf()
     o.init_conf()
     assert np.array_equal(w.get_conf(), o.conf), "initial bond field differs"
+    if kw.get("rhub", 0.0) != 0.0:
+        assert np.array_equal(w.get_uconf(), o.uconf), "initial on-site field differs"
     w.set_trial()
     o.set_trial()                      # the oracle's stream advances by the same draws
     o.set_proj(w.projs[0])
@@ -65,6 +73,9 @@ def run_parity(kw, nsweeps=2, chain=1):
             say("  sweep", s, p)
             cw, co = w.get_conf(), o.conf
             assert np.array_equal(cw, co), "bond field differs after pass %d %s (%d bonds)" % (s, p, (cw != co).sum())
+            if kw.get("rhub", 0.0) != 0.0:
+                uw, uo = w.get_uconf(), o.uconf
+                assert np.array_equal(uw, uo), "on-site field differs after pass %d %s (%d)" % (s, p, (uw != uo).sum())
     tw, to = w.trace(), o.trace()
     assert len(tw) == len(to) and len(tw) > 0
     for k in range(len(tw)):

@@ -270,3 +270,139 @@ def test_exact_enumeration_square_2x2():
         jk = np.array([(vb.sum() - vb[i]) / (sb.sum() - sb[i]) for i in range(nbin)])
         err = np.sqrt((nbin - 1) * np.mean((jk - jk.mean()) ** 2))
         assert abs(est - exact) < 4 * err + 1e-12, (est, exact, err)
+
+
+# ---- the on-site (Hubbard) field in the projective sweep: src/dqmc_config_u/*_prop.f90,
+# ---- src/dqmc_config_u/dqmc_proj_update_submatrix_LR.f90, src/dqmc_proj_core.f90:68-116
+
+def _u_tables(dtau, rhub, nflr=2):
+    from sunhubbardmy_amd.model import Model, hs_tables
+    return hs_tables(Model(dtau=dtau, rhub=rhub, nflr=nflr, exact_hs=True))
+
+
+def slice_matrix_tuv(o, conf_v_t, conf_u_t, dtau, rv, hs, kin):
+    """B(tau) with the bond field (when rv != 0) and the on-site field: K-half, bond fields,
+    exp(V_U) = diag(bmat(conf_u)), K-half (src/dqmc_proj_core.f90:68-116)."""
+    N = o.ndim
+    M = np.eye(N, dtype=complex)
+    for f in kin[0]:
+        M = f @ M
+    if rv != 0.0:
+        alpha = np.arccosh(np.exp(dtau * rv / 2))
+        b = bonds_by_family(o)
+        for f in range(o.nn_nf):
+            V = np.eye(N, dtype=complex)
+            for q in range(o.nn_lf):
+                i1, i2 = b[f, q]
+                eta = -1.0 if conf_v_t[f, q] == 1 else 1.0
+                V[np.ix_([i1, i2], [i1, i2])] = scipy.linalg.expm(alpha * eta * SX)
+            M = V @ M
+    M = np.diag(hs["bmat"][conf_u_t - 1]) @ M
+    for f in kin[1]:
+        M = f @ M
+    return M
+
+
+@pytest.mark.parametrize("kmode,rv", [("dense", 0.0), ("checkerboard", 0.0), ("checkerboard", 1.0)])
+def test_hubbard_direct_product(kmode, rv):
+    """Hubbard (and t-U-V) projective chain: after a sweep, G = 1 - R (P^H R)^-1 P^H with
+    R = B(theta, 0) P built densely from the final on-site and bond fields."""
+    dtau, rhub = 0.1, 4.0
+    o = oracle.OracleProjChain(lattice="square", la=4, lb=4, beta=1.0, dtau=dtau, rv=rv, rhub=rhub, nflr=2,
+                               rndness=0.2, kmode=kmode, nwrap=5, nustock=6, seed=oracle.chain_seed(4))
+    o.init_conf()
+    o.set_trial()
+    o.sweep_start()
+    u0 = o.uconf.copy()
+    assert set(np.unique(u0)) <= {1, 2} and len(np.unique(u0)) == 2
+    o.sweep()
+    assert not np.array_equal(o.uconf, u0)                # the on-site field was updated
+    hs = _u_tables(dtau, rhub)
+    p, _ = o.proj
+    kin = kinetic_factors(o, kmode, dtau)
+    B = np.eye(o.ndim, dtype=complex)
+    conf_v = o.conf[:, :, :o.nn_lf]
+    for t in range(o.ltrot):
+        B = slice_matrix_tuv(o, conf_v[t], o.uconf[t], dtau, rv, hs, kin) @ B
+    R = B @ p
+    G = np.eye(o.ndim) - R @ np.linalg.solve(p.conj().T @ R, p.conj().T)
+    np.testing.assert_allclose(o.G, G, atol=1e-10)
+    d = o.diag()
+    assert d["max_wrap_error"] < 1e-10 and 0 < d["accepted"] < d["tried"]
+
+
+def test_hubbard_stock_size_is_invisible():
+    """The on-site submatrix-LR update with 2 and with 8 sites per stock accepts the same flips."""
+    kw = dict(lattice="square", la=4, lb=4, beta=1.0, rv=0.0, rhub=4.0, nflr=2, rndness=0.2, nwrap=5,
+              kmode="checkerboard", seed=oracle.chain_seed(6))
+    a = oracle.OracleProjChain(nustock=2, **kw)
+    b = oracle.OracleProjChain(nustock=8, **kw)
+    for o in (a, b):
+        o.init_conf()
+        o.set_trial()
+        o.sweep_start()
+    for _ in range(2):
+        a.sweep()
+        b.sweep()
+        assert np.array_equal(a.uconf, b.uconf)
+        np.testing.assert_allclose(a.G, b.G, atol=1e-11)
+    assert a.diag()["accepted"] == b.diag()["accepted"]
+
+
+def test_hubbard_exact_enumeration_square_2x2():
+    """4 sites, 2 particles per flavour, SU(2), 2 slices: all 2^8 on-site fields enumerated
+    with weight Re[det(P^H B P)^2 prod phase(conf)]; the Monte Carlo kinetic energy and
+    sum_i (1 - G_ii)^2 at the middle slice agree with the exact averages within 4 errors."""
+    dtau, rhub = 0.25, 4.0
+    kw = dict(lattice="square", la=2, lb=2, beta=0.5, dtau=dtau, rv=0.0, rhub=rhub, nflr=2, rndness=0.4,
+              kmode="dense", nwrap=1, nustock=4, mid_len=1)
+    o = oracle.OracleProjChain(seed=oracle.chain_seed(3), **kw)
+    o.init_conf()
+    o.set_trial()
+    p, _ = o.proj
+    assert o.ltrot == 2 and o.ndim == 4 and o.ne == 2
+    hs = _u_tables(dtau, rhub)
+    kin = kinetic_factors(o, "dense", dtau)
+    b = bonds_by_family(o)
+    sl, ph = [], []
+    for bits in itertools.product([1, 2], repeat=o.ndim):
+        ct = np.array(bits)
+        sl.append(slice_matrix_tuv(o, None, ct, dtau, 0.0, hs, kin))
+        ph.append(np.prod(hs["phase"][ct - 1]))
+    sl, ph = np.array(sl), np.array(ph)
+    n1 = len(sl)
+    B1 = np.broadcast_to(sl[:, None], (n1, n1, 4, 4))
+    B2 = np.broadcast_to(sl[None, :], (n1, n1, 4, 4))
+    R = B1 @ p
+    L = p.conj().T @ B2
+    LR = L @ R
+    # complex weights; the chain samples |Re W| and measures O W / |Re W| (the phase-weighted
+    # bins of equaltime_measure), so the exact value of a bin ratio is sum Re(W O) / sum Re(W)
+    wc = np.linalg.det(LR) ** 2 * ph[:, None] * ph[None, :]
+    assert wc.real.min() > 0
+    G = np.eye(4) - R @ np.linalg.solve(LR, L)
+    ekin = np.zeros(wc.shape)
+    for f in range(o.nn_nf):
+        for q in range(o.nn_lf):
+            i0, i1 = b[f, q]
+            ekin += 2.0 * G[..., i1, i0].real
+    dbl = np.sum((1.0 - np.diagonal(G, axis1=-2, axis2=-1)) ** 2, axis=-1)
+    exact_ekin = np.sum((wc * ekin).real) / np.sum(wc.real)
+    exact_dbl = np.sum((wc * dbl).real) / np.sum(wc.real)
+    o.sweep_start()
+    for _ in range(200):
+        o.sweep(lmeasure=False)
+    nsw, nbin = 16000, 40
+    ek, db, sg = np.zeros(nsw), np.zeros(nsw), np.zeros(nsw)
+    for s in range(nsw):
+        o.reset_obs()
+        o.sweep()
+        ob = o.obs()
+        ek[s], db[s], sg[s] = ob["ekin"].real, ob["dblocc"].real, ob["sign"].real
+    for vals, exact in ((ek, exact_ekin), (db, exact_dbl)):
+        vb = vals.reshape(nbin, -1).sum(1)
+        sb = sg.reshape(nbin, -1).sum(1)
+        est = vb.sum() / sb.sum()
+        jk = np.array([(vb.sum() - vb[i]) / (sb.sum() - sb[i]) for i in range(nbin)])
+        err = np.sqrt((nbin - 1) * np.mean((jk - jk.mean()) ** 2))
+        assert abs(est - exact) < 4 * err + 1e-12, (est, exact, err)
