@@ -123,6 +123,53 @@ k_sfmt_draw(const DChain *chains, int nchains, int mode)
 #define CPB 8
 #define RPB 8
 
+// Global <-> LDS staging with ST_BATCH independent 16-byte accesses in flight per thread: a
+// batch's loads are all issued before its first LDS store, so a workgroup waits for one memory
+// latency per batch instead of one per element.  ld(e) returns element e of the tile, st(e, v)
+// writes it back.
+#define ST_BATCH 8
+template <int B, class Ld, class Ix>
+__device__ __forceinline__ void stage_in_map(cplx *lds, int total, Ld ld, Ix ix)
+{
+    for (int base = threadIdx.x; base < total; base += B * blockDim.x) {
+        cplx v[B];
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            const int e = base + u * (int)blockDim.x;
+            v[u] = e < total ? ld(e) : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            const int e = base + u * (int)blockDim.x;
+            if (e < total) lds[ix(e)] = v[u];
+        }
+    }
+}
+
+template <class Ld>
+__device__ __forceinline__ void stage_in(cplx *lds, int total, Ld ld)
+{
+    stage_in_map<ST_BATCH>(lds, total, ld, [](int e) { return e; });
+}
+
+template <class St>
+__device__ __forceinline__ void stage_out(const cplx *lds, int total, St st)
+{
+    for (int base = threadIdx.x; base < total; base += ST_BATCH * blockDim.x) {
+        cplx v[ST_BATCH];
+#pragma unroll
+        for (int u = 0; u < ST_BATCH; u++) {
+            const int e = base + u * (int)blockDim.x;
+            v[u] = e < total ? lds[e] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < ST_BATCH; u++) {
+            const int e = base + u * (int)blockDim.x;
+            if (e < total) st(e, v[u]);
+        }
+    }
+}
+
 __device__ __forceinline__ cplx vtab(const DChain &c, int vsel, int is)
 {
     // vsel 0: bmat, 1: bmat_inv, 2: conj(bmat)
@@ -178,10 +225,9 @@ k_cb_cols(const DChain *chains, int msel, int pt, int ksel, int prog, int vsel, 
     const int ncol = min(CPB, n - c0);
     cplx *M = chain_mat(c, msel, pt);
     const cplx *T = ksel ? c.ktm1 : c.kt;
-    for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
-        int r = e % n, cc = e / n;
-        lds[e] = M[(size_t)r + (size_t)(c0 + cc) * n];
-    }
+    // the CPB columns are one contiguous run of n * ncol elements
+    GAS cplx *Mc = gp(M) + (size_t)c0 * n;
+    stage_in(lds, n * ncol, [=](int e) { return gld(Mc + (e)); });
     __syncthreads();
     // slices nt0, nt0+ntstep, ..., nt1 (ntstep = +-1); the count bounds the loop
     const int nslice = (ntstep > 0 ? nt1 - nt0 : nt0 - nt1) + 1;
@@ -192,10 +238,7 @@ k_cb_cols(const DChain *chains, int msel, int pt, int ksel, int prog, int vsel, 
         if ((prog & 2) && c.lwrapu) cb_left_v(c, lds, ncol, nt, vsel);
         if ((prog & 4) && c.lwrapT) cb_left_fams(c, lds, ncol, T, true);
     }
-    for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
-        int r = e % n, cc = e / n;
-        M[(size_t)r + (size_t)(c0 + cc) * n] = lds[e];
-    }
+    stage_out(lds, n * ncol, [=](int e, cplx v) { gst(Mc + (e), v); });
 }
 
 template <class Chain>
@@ -243,19 +286,21 @@ k_cb_rows(const DChain *chains, int msel, int pt, int ksel, int prog, int vsel, 
     const int nrow = min(RPB, n - r0);
     cplx *M = chain_mat(c, msel, pt);
     const cplx *T = ksel ? c.ktm1 : c.kt;
-    for (int e = threadIdx.x; e < n * RPB; e += blockDim.x) {
-        int rr = e % RPB, col = e / RPB;
-        lds[e] = (rr < nrow) ? M[(size_t)(r0 + rr) + (size_t)col * n] : make_double2(0.0, 0.0);
-    }
+    // lds[col * RPB + rr] = M(r0 + rr, col): eight consecutive threads read one 128-byte run
+    GAS cplx *Mg = gp(M);
+    stage_in(lds, n * RPB, [=](int e) {
+        const int rr = e % RPB, col = e / RPB;
+        return rr < nrow ? gld(Mg + ((size_t)(r0 + rr) + (size_t)col * n)) : make_double2(0.0, 0.0);
+    });
     __syncthreads();
     if ((prog & 8) && c.lwrapu) cb_right_v(c, lds, nt, vsel);
     if ((prog & 1) && c.lwrapT) cb_right_fams(c, lds, nrow, T, false);
     if ((prog & 2) && c.lwrapu) cb_right_v(c, lds, nt, vsel);
     if ((prog & 4) && c.lwrapT) cb_right_fams(c, lds, nrow, T, true);
-    for (int e = threadIdx.x; e < n * RPB; e += blockDim.x) {
-        int rr = e % RPB, col = e / RPB;
-        if (rr < nrow) M[(size_t)(r0 + rr) + (size_t)col * n] = lds[e];
-    }
+    stage_out(lds, n * RPB, [=](int e, cplx v) {
+        const int rr = e % RPB, col = e / RPB;
+        if (rr < nrow) gst(Mg + ((size_t)(r0 + rr) + (size_t)col * n), v);
+    });
 }
 
 // exp(V) scalings for the dense-K form: rows (side 0) or columns (side 1), or both (side 2:
@@ -296,9 +341,10 @@ __device__ __forceinline__ double4_t mfma(double a, double b, double4_t c)
 // one 64 x 64 tile (tm, tn) of C (m x n, ldc) = alpha op(A) op(B) + beta C, op(A) m x kk,
 // op(B) kk x n; op = 0: as stored, 1: conjugate transpose
 template <int OPA, int OPB>
-__device__ __forceinline__ void zgemm_tile(const cplx *A, int lda, const cplx *B, int ldb, cplx *C, int ldc, int m, int n,
+__device__ __forceinline__ void zgemm_tile(const cplx *A_, int lda, const cplx *B_, int ldb, cplx *C_, int ldc, int m, int n,
                                   int kk, int tm, int tn, double ar, double ai, double br, double bi)
 {
+    GAS cplx *A = gp(A_), *B = gp(B_), *C = gp(C_);
     __shared__ double sAr[GT][GK + 1], sAi[GT][GK + 1];
     __shared__ double sBr[GK][GT + 1], sBi[GK][GT + 1];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -312,13 +358,13 @@ __device__ __forceinline__ void zgemm_tile(const cplx *A, int lda, const cplx *B
             int idx = t + 256 * q;
             int i, k;
             cplx v = make_double2(0.0, 0.0);
-            if (OPA == 0) { i = idx % GT; k = idx / GT; if (tm + i < m && k0 + k < kk) v = A[(size_t)(tm + i) + (size_t)(k0 + k) * lda]; }
-            else { k = idx % GK; i = idx / GK; if (tm + i < m && k0 + k < kk) v = cconj(A[(size_t)(k0 + k) + (size_t)(tm + i) * lda]); }
+            if (OPA == 0) { i = idx % GT; k = idx / GT; if (tm + i < m && k0 + k < kk) v = gld(A + ((size_t)(tm + i) + (size_t)(k0 + k) * lda)); }
+            else { k = idx % GK; i = idx / GK; if (tm + i < m && k0 + k < kk) v = cconj(gld(A + ((size_t)(k0 + k) + (size_t)(tm + i) * lda))); }
             sAr[i][k] = v.x; sAi[i][k] = v.y;
             int j;
             cplx u = make_double2(0.0, 0.0);
-            if (OPB == 0) { k = idx % GK; j = idx / GK; if (tn + j < n && k0 + k < kk) u = B[(size_t)(k0 + k) + (size_t)(tn + j) * ldb]; }
-            else { j = idx % GT; k = idx / GT; if (tn + j < n && k0 + k < kk) u = cconj(B[(size_t)(tn + j) + (size_t)(k0 + k) * ldb]); }
+            if (OPB == 0) { k = idx % GK; j = idx / GK; if (tn + j < n && k0 + k < kk) u = gld(B + ((size_t)(k0 + k) + (size_t)(tn + j) * ldb)); }
+            else { j = idx % GT; k = idx / GT; if (tn + j < n && k0 + k < kk) u = cconj(gld(B + ((size_t)(tn + j) + (size_t)(k0 + k) * ldb))); }
             sBr[k][j] = u.x; sBi[k][j] = u.y;
         }
         __syncthreads();
@@ -351,8 +397,8 @@ __device__ __forceinline__ void zgemm_tile(const cplx *A, int lda, const cplx *B
                     size_t o = (size_t)row + (size_t)col * ldc;
                     double vr = accr[sm][sn][r], vi = acci[sm][sn][r];
                     double outr = ar * vr - ai * vi, outi = ar * vi + ai * vr;
-                    if (!zb) { cplx cv = C[o]; outr += br * cv.x - bi * cv.y; outi += br * cv.y + bi * cv.x; }
-                    C[o] = make_double2(outr, outi);
+                    if (!zb) { cplx cv = gld(C + (o)); outr += br * cv.x - bi * cv.y; outi += br * cv.y + bi * cv.x; }
+                    gst(C + (o), make_double2(outr, outi));
                 }
             }
 }
@@ -421,61 +467,78 @@ __device__ static void wg_rank_k(cplx *G, int n, const cplx *A, const cplx *B, i
 // ------------------------------------------------------------------------------------------
 // The same rank-k update G += A B^T for k <= K known at compile time, organised for latency:
 // a wave owns 16-row strips of G; the strip's A fragment (K/4 complex per lane) stays in
-// registers while the wave walks the 16-column tiles, and the next tile's B fragment and G
-// tile are loaded before the current tile's MFMAs are issued.
+// registers while the wave walks the 16-column tiles.  Every load has a clamped, in-bounds
+// address and no branch around it (rows and columns past n compute values that are never
+// stored; entries past k are zero in the A fragment), so nothing waits on a load before its
+// use: G tiles are loaded two tiles ahead into one of two alternating register sets, the B
+// fragment one tile ahead (a second B set would spill in the delay kernel).  At k = 32 the
+// update reads and writes G for 8 flops per byte, so it is bound by HBM, not by the MFMAs.
 // ------------------------------------------------------------------------------------------
+template <int KS>
+__device__ __forceinline__ void rank_k_tile(GAS cplx *G, GAS cplx *B, int n, int k, int row, bool rok, int kq, int l16,
+                                            int tj, const cplx (&y)[KS], cplx (&x)[KS], cplx (&g)[4])
+{
+    const int j0 = tj * 16;
+    double4_t accr, acci;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        accr[r] = g[r].x;
+        acci[r] = g[r].y;
+    }
+    // this register set now takes tile tj + 2 (clamped: past the last tile it reloads the last
+    // columns, which are never used); the B fragment takes tile tj + 1
+    const size_t rowc = (size_t)min(row, n - 1);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int col = min(j0 + 32 + kq + 4 * r, n - 1);
+        g[r] = gld(G + (rowc + (size_t)col * n));
+    }
+    const size_t xcol = (size_t)min(j0 + 16 + l16, n - 1);
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+        accr = mfma(x[s].x, y[s].x, accr);
+        accr = mfma(-x[s].y, y[s].y, accr);
+        acci = mfma(x[s].x, y[s].y, acci);
+        acci = mfma(x[s].y, y[s].x, acci);
+        const int mc = min(4 * s + kq, k - 1);
+        x[s] = gld(B + (xcol + (size_t)mc * n));
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int col = j0 + kq + 4 * r;
+        if (rok && col < n) gst(G + ((size_t)row + (size_t)col * n), make_double2(accr[r], acci[r]));
+    }
+}
+
 template <int K>
-__device__ static void wg_rank_k_strips(cplx *G, int n, const cplx *A, const cplx *B, int k)
+__device__ static void wg_rank_k_strips(cplx *G_, int n, const cplx *A_, const cplx *B_, int k)
 {
     constexpr int KS = K / 4;
+    GAS cplx *G = gp(G_), *A = gp(A_), *B = gp(B_);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int nt = (n + 15) >> 4;
     const int l16 = lane & 15, kq = lane >> 4;
     for (int si = w; si < nt; si += nw) {
-        const int i0 = si * 16, row = i0 + l16;
+        const int row = si * 16 + l16;
         const bool rok = row < n;
-        cplx y[KS], x[KS], g[4];
+        const size_t rowc = (size_t)min(row, n - 1);
+        const size_t xcol = (size_t)min(l16, n - 1);
+        cplx y[KS], x[KS], ga[4], gb[4];
 #pragma unroll
         for (int s = 0; s < KS; s++) {
-            const int m = 4 * s + kq;
-            y[s] = (rok && m < k) ? A[(size_t)row + (size_t)m * n] : make_double2(0.0, 0.0);
-            x[s] = (l16 < n && m < k) ? B[(size_t)l16 + (size_t)m * n] : make_double2(0.0, 0.0);
+            const int m = 4 * s + kq, mc = min(m, k - 1);
+            const cplx a = gld(A + (rowc + (size_t)mc * n));
+            y[s] = m < k ? a : make_double2(0.0, 0.0);
+            x[s] = gld(B + (xcol + (size_t)mc * n));
         }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int col = kq + 4 * r;
-            g[r] = (rok && col < n) ? G[(size_t)row + (size_t)col * n] : make_double2(0.0, 0.0);
+            ga[r] = gld(G + (rowc + (size_t)min(kq + 4 * r, n - 1) * n));
+            gb[r] = gld(G + (rowc + (size_t)min(16 + kq + 4 * r, n - 1) * n));
         }
-        for (int tj = 0; tj < nt; tj++) {
-            const int j0 = tj * 16, j1 = j0 + 16;
-            const bool more = tj + 1 < nt;
-            double4_t accr, acci;
-#pragma unroll
-            for (int r = 0; r < 4; r++) { accr[r] = g[r].x; acci[r] = g[r].y; }
-            // each register is refilled with the next tile's value as soon as it is consumed
-            if (more) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int col = j1 + kq + 4 * r;
-                    g[r] = (rok && col < n) ? G[(size_t)row + (size_t)col * n] : make_double2(0.0, 0.0);
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < KS; s++) {
-                accr = mfma(x[s].x, y[s].x, accr);
-                accr = mfma(-x[s].y, y[s].y, accr);
-                acci = mfma(x[s].x, y[s].y, acci);
-                acci = mfma(x[s].y, y[s].x, acci);
-                if (more) {
-                    const int m = 4 * s + kq;
-                    x[s] = (j1 + l16 < n && m < k) ? B[(size_t)(j1 + l16) + (size_t)m * n] : make_double2(0.0, 0.0);
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int col = j0 + kq + 4 * r;
-                if (rok && col < n) G[(size_t)row + (size_t)col * n] = make_double2(accr[r], acci[r]);
-            }
+        for (int tj = 0; tj < nt; tj += 2) {
+            rank_k_tile<KS>(G, B, n, k, row, rok, kq, l16, tj, y, x, ga);
+            if (tj + 1 < nt) rank_k_tile<KS>(G, B, n, k, row, rok, kq, l16, tj + 1, y, x, gb);
         }
     }
 }
@@ -1592,6 +1655,16 @@ __device__ __forceinline__ cplx delay_dot(int ik, const cplx (&reg)[K], const cp
     return make_double2(vr + wr, vi + wi);
 }
 
+// the value of lane l ^ 32 (v_permlane32_swap exchanges the two 32-lane halves)
+__device__ __forceinline__ double partner32(double v)
+{
+    const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    const bool upper = (threadIdx.x & 32) != 0;
+    return __hiloint2double((int)(upper ? b[0] : b[1]), (int)(upper ? a[0] : a[1]));
+}
+
 // reg[ik] = v by a binary tree of uniform branches; the empty volatile asm keeps each leaf a
 // real branch target instead of a select over all K registers
 template <int K, int LO, int HI>
@@ -1626,7 +1699,7 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
     double *us = (double *)(rows + 4 * K);  // 2n uniforms of this slice
     int *cfs = (int *)(us + 2 * n);     // n: HS field of this slice (read-only here)
     __shared__ HsTab tab;
-    cplx *G = c.G;
+    GAS cplx *G = gp(c.G);
     int32_t *cf = c.conf + (size_t)(nt - 1) * n;
     const double *u = c.unif + (size_t)2 * n * slice_in_pass;
     cplx phase = make_double2(c.sc[SC_PHASE], c.sc[SC_PHASE + 1]);
@@ -1634,7 +1707,7 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
 #pragma unroll
     for (int m = 0; m < K; m++) reg[m] = make_double2(0.0, 0.0);
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        diagg[q] = G[(size_t)q + (size_t)q * n];
+        diagg[q] = gld(G + ((size_t)q + (size_t)q * n));
         cfs[q] = cf[q];
     }
     for (int q = threadIdx.x; q < 2 * n; q += blockDim.x) us[q] = u[q];
@@ -1649,19 +1722,19 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
             if (s0 >= 0) __syncthreads();
             s0 = s;
             const int nb = min(DSB, n - s);
-            for (int e = threadIdx.x; e < nb * n; e += blockDim.x) {
-                int j = e / n, q = e % n;
-                gcol[j * n + q] = G[(size_t)q + (size_t)(s + j) * n];
-            }
-            for (int e = threadIdx.x; e < nb * n; e += blockDim.x) {
-                int q = e / nb, j = e % nb;
-                grow[j * n + q] = G[(size_t)(s + j) + (size_t)q * n];
-            }
+            // columns s..s+nb are one contiguous run; a row block is read as nb-element runs
+            GAS cplx *Gc = G + (size_t)s * n;
+            stage_in_map<4>(gcol, nb * n, [=](int e) { return gld(Gc + (e)); }, [](int e) { return e; });
+            stage_in_map<4>(grow, nb * n, [=](int e) { return gld(G + ((size_t)(s + e % nb) + (size_t)(e / nb) * n)); },
+                            [=](int e) { return (e % nb) * n + e / nb; });
         }
         __syncthreads();
         const SiteDraw d = site_draw(tab, cfs[s], us[2 * s]);
         const cplx dg = diagg[s];
         const cplx ratio1 = cadd(cmul(make_double2(1.0 - dg.x, -dg.y), d.db), make_double2(1.0, 0.0));
+        // delta / ratio1, evaluated next to the decision (independent of it) so that the two
+        // dependency chains overlap
+        const cplx sscl = cdiv(d.db, ratio1);
         cplx rt;
         // every thread evaluates the same decision; reading it from lane 0 makes the branch
         // and the delay count scalar
@@ -1669,7 +1742,6 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
         if (acc) {
             accm += 1.0;
             phase = cmul(phase, rt);   // normalised at the next flush
-            const cplx sscl = cdiv(d.db, ratio1);
             const cplx *rw = rows + (s & 1) * 2 * K;      // avec(s, :), then bvec(s, :)
             // A lanes: sum_m bvec(s,m) avec(r,m); B lanes: sum_m avec(s,m) bvec(r,m)
             const cplx *other = isA ? rw + K : rw;
@@ -1688,39 +1760,47 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
             }
             delay_put<K, 0, K>(iku, reg, v);
             // the row's other entry from the partner lane (A <-> B)
-            const cplx w = make_double2(__shfl_xor(v.x, 32, 64), __shfl_xor(v.y, 32, 64));
+            const cplx w = make_double2(partner32(v.x), partner32(v.y));
             if (isA && r > s && r < n) diagg[r] = cadd(diagg[r], cmul(v, w));
             if (threadIdx.x == 0) cf[s] = d.isp;
             ik++;
         }
-        // row s+1 publishes its vectors for the next site
+        // row s+1 publishes its filled vector entries for the next site (the reader's products
+        // cover the same chunks, and its entries past ik are zero)
         if (r == s + 1 && r < n) {
             cplx *pw = rows + ((s + 1) & 1) * 2 * K + (isA ? 0 : K);
+            const int iku = __builtin_amdgcn_readfirstlane(ik);
 #pragma unroll
-            for (int m = 0; m < K; m++) pw[m] = reg[m];
+            for (int c4 = 0; c4 < K / 4; c4++)
+                if (4 * c4 < iku) {
+                    pw[4 * c4] = reg[4 * c4];
+                    pw[4 * c4 + 1] = reg[4 * c4 + 1];
+                    pw[4 * c4 + 2] = reg[4 * c4 + 2];
+                    pw[4 * c4 + 3] = reg[4 * c4 + 3];
+                }
         }
         if (ik == kb || s == n - 1) {
             __syncthreads();
             if (ik > 0) {
                 if (r < n) {
-                    cplx *dst = isA ? c.avec : c.bvec;
+                    GAS cplx *dst = gp(isA ? c.avec : c.bvec);
 #pragma unroll
                     for (int m = 0; m < K; m++)
-                        if (m < ik) dst[(size_t)r + (size_t)m * n] = reg[m];
+                        if (m < ik) gst(dst + ((size_t)r + (size_t)m * n), reg[m]);
                 }
                 // the vectors restart from zero after the flush; clearing them here frees
                 // their registers for the flush
 #pragma unroll
                 for (int m = 0; m < K; m++) reg[m] = make_double2(0.0, 0.0);
                 __syncthreads();
-                wg_rank_k_strips<K>(G, n, c.avec, c.bvec, ik);
+                wg_rank_k_strips<K>(c.G, n, c.avec, c.bvec, ik);
             }
             __syncthreads();
             phase = cscale(phase, 1.0 / sqrt(cabs2(phase)));
             ik = 0;
             s0 = -1;
             if (s < n - 1)
-                for (int q = threadIdx.x; q < n; q += blockDim.x) diagg[q] = G[(size_t)q + (size_t)q * n];
+                for (int q = threadIdx.x; q < n; q += blockDim.x) diagg[q] = gld(G + ((size_t)q + (size_t)q * n));
         }
     }
     if (threadIdx.x == 0) {

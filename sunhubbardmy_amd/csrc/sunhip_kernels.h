@@ -6,6 +6,33 @@
 typedef double2 cplx;                                   // (re, im), the complex(dp) layout
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
+// Global-memory view of a device pointer.  The chain descriptors hold generic pointers, whose
+// accesses compile to FLAT instructions; the result of a FLAT load can only be awaited together
+// with every other outstanding memory and LDS access, so loads issued ahead of use stall the
+// first consumer anyway.  Through an address-space-1 pointer the same accesses compile to
+// GLOBAL instructions with per-load wait counts.  Only for pointers into device allocations.
+// HIP's vector class has no members for address-space-1 objects, so complex elements are read
+// and written through gld / gst (one 16-byte access each).
+#define GAS __attribute__((address_space(1)))
+typedef double gdbl2 __attribute__((ext_vector_type(2)));
+template <class T>
+__device__ __forceinline__ GAS T *gp(const T *p)
+{
+    return (GAS T *)p;
+}
+__device__ __forceinline__ double2 gld(const GAS double2 *p)
+{
+    const gdbl2 v = *(const GAS gdbl2 *)p;
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void gst(GAS double2 *p, double2 v)
+{
+    gdbl2 w;
+    w.x = v.x;
+    w.y = v.y;
+    *(GAS gdbl2 *)p = w;
+}
+
 // per-chain scalar slots (double array `sc`)
 enum {
     SC_PHASE = 0,      // 2: phase (re, im)
