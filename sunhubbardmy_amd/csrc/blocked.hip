@@ -58,41 +58,70 @@ __device__ __forceinline__ cplx vpanel(const cplx *P, int ldp, int r, int p)
     return make_double2(r == p ? 1.0 : 0.0, 0.0);
 }
 
-// W (16 x nc, global, ld 16) = V^H C, V = m x 16 Householder panel in LDS, C global (m x nc, ldc)
-__device__ static void wg_vhc(cplx *W, const cplx *P, int ldp, int m, int b, const cplx *C, int ldc, int nc)
+// The products below read their global operands through clamped, in-bounds addresses with no
+// branch around the load (rows and columns past the edge compute values that are never stored),
+// so a tile's loads are all in flight together and the next tile's are issued before the
+// current tile's MFMAs.
+
+// W (16 x nc, global, ld 16) = V^H C, V = m x 16 Householder panel in LDS, C global (m x nc, ldc).
+// The rows are walked in blocks of 16 whose C values are loaded two blocks ahead into one of
+// two alternating register sets.
+__device__ __forceinline__ void vhc_block(double4_t &ar, double4_t &ai, const cplx *P, int ldp, int m, int b,
+                                          GAS cplx *C, size_t ccol, int bk, int kq, int l16, cplx (&y)[4])
 {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int r = 16 * bk + 4 * u + kq;
+        cplx x = make_double2(0.0, 0.0);
+        const cplx yy = r < m ? y[u] : make_double2(0.0, 0.0);
+        if (r < m && l16 < b) x = cconj(vpanel(P, ldp, r, l16));         // X[p][k] = conj V[k][p]
+        ar = mfma64(x.x, yy.x, ar);
+        ar = mfma64(-x.y, yy.y, ar);
+        ai = mfma64(x.x, yy.y, ai);
+        ai = mfma64(x.y, yy.x, ai);
+        y[u] = gld(C + ((size_t)min(16 * (bk + 2) + 4 * u + kq, m - 1) + ccol));   // Y[k][q] = C[k][c0+q]
+    }
+}
+
+__device__ static void wg_vhc(cplx *W_, const cplx *P, int ldp, int m, int b, const cplx *C_, int ldc, int nc)
+{
+    GAS cplx *W = gp(W_), *C = gp(C_);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int l16 = lane & 15, kq = lane >> 4;
-    const int ntile = (nc + 15) >> 4;
+    const int ntile = (nc + 15) >> 4, nb = (m + 15) >> 4;
     for (int t = w; t < ntile; t += nw) {
         const int c0 = t * 16;
+        const size_t ccol = (size_t)min(c0 + l16, nc - 1) * ldc;
         double4_t ar = {0, 0, 0, 0}, ai = {0, 0, 0, 0};
-        for (int r0 = 0; r0 < m; r0 += 4) {
-            const int r = r0 + kq;
-            cplx x = make_double2(0.0, 0.0), y = make_double2(0.0, 0.0);
-            if (r < m) {
-                if (l16 < b) x = cconj(vpanel(P, ldp, r, l16));         // X[p][k] = conj V[k][p]
-                if (c0 + l16 < nc) y = C[(size_t)r + (size_t)(c0 + l16) * ldc];  // Y[k][q] = C[k][c0+q]
-            }
-            ar = mfma64(x.x, y.x, ar);
-            ar = mfma64(-x.y, y.y, ar);
-            ai = mfma64(x.x, y.y, ai);
-            ai = mfma64(x.y, y.x, ai);
+        cplx ya[4], yb[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            ya[u] = gld(C + ((size_t)min(4 * u + kq, m - 1) + ccol));
+            yb[u] = gld(C + ((size_t)min(16 + 4 * u + kq, m - 1) + ccol));
         }
+        for (int bk = 0; bk < nb; bk += 2) {
+            vhc_block(ar, ai, P, ldp, m, b, C, ccol, bk, kq, l16, ya);
+            if (bk + 1 < nb) vhc_block(ar, ai, P, ldp, m, b, C, ccol, bk + 1, kq, l16, yb);
+        }
+#pragma unroll
         for (int rr = 0; rr < 4; rr++) {
             const int p = kq + 4 * rr, q = c0 + l16;
-            if (q < nc) W[p + 16 * (size_t)q] = make_double2(ar[rr], ai[rr]);
+            if (q < nc) gst(W + (p + 16 * (size_t)q), make_double2(ar[rr], ai[rr]));
         }
     }
 }
 
 // W <- op(T) W, T 16 x 16 upper triangular in LDS (col-major), op = T^H (herm) or T
-__device__ static void wg_tmul(cplx *W, int nc, const cplx *T, int b, bool herm)
+__device__ static void wg_tmul(cplx *W_, int nc, const cplx *T, int b, bool herm)
 {
+    GAS cplx *W = gp(W_);
     for (int q = threadIdx.x; q < nc; q += blockDim.x) {
         cplx x[PB], y[PB];
 #pragma unroll
-        for (int p = 0; p < PB; p++) x[p] = (p < b) ? W[p + 16 * (size_t)q] : make_double2(0.0, 0.0);
+        for (int p = 0; p < PB; p++) {
+            const cplx v = gld(W + (min(p, b - 1) + 16 * (size_t)q));
+            x[p] = (p < b) ? v : make_double2(0.0, 0.0);
+        }
 #pragma unroll
         for (int p = 0; p < PB; p++) {
             cplx s = make_double2(0.0, 0.0);
@@ -108,51 +137,101 @@ __device__ static void wg_tmul(cplx *W, int nc, const cplx *T, int b, bool herm)
         }
 #pragma unroll
         for (int p = 0; p < PB; p++)
-            if (p < b) W[p + 16 * (size_t)q] = y[p];
+            if (p < b) gst(W + (p + 16 * (size_t)q), y[p]);
     }
 }
 
 // C (mrows x ncols, global, ldc) -= A B with A = 16 columns in LDS (A[p*lda + r], unit-panel
 // accessor when vp = 1) and B = 16 x ncols in global memory (B[p + c*ldb]).  The accumulator
-// holds C^T tiles so that the global reads and writes of C are 16-element column runs.
+// holds C^T tiles so that the global reads and writes of C are 16-element column runs.  A
+// wave's tiles alternate between two register sets; a set is refilled with the C tile and B
+// fragment of the wave's tile after next as soon as they are consumed.
+struct R16Args {
+    GAS cplx *C;
+    GAS cplx *B;
+    const cplx *A;
+    int ldc, mrows, ncols, lda, roff, b, ldb, mt, ntiles;
+    bool vp;
+};
+
+__device__ __forceinline__ void r16_load(const R16Args &g, int t, int kq, int l16, cplx (&cv)[4], cplx (&xv)[4])
+{
+    t = min(t, g.ntiles - 1);
+    const int r0 = (t % g.mt) * 16, c0 = (t / g.mt) * 16;
+    const size_t rowc = (size_t)min(r0 + l16, g.mrows - 1);
+#pragma unroll
+    for (int j = 0; j < 4; j++) cv[j] = gld(g.C + (rowc + (size_t)min(c0 + kq + 4 * j, g.ncols - 1) * g.ldc));
+    const size_t xc = (size_t)min(c0 + l16, g.ncols - 1) * g.ldb;
+#pragma unroll
+    for (int s = 0; s < 4; s++) xv[s] = gld(g.B + (min(4 * s + kq, g.b - 1) + xc));
+}
+
+__device__ __forceinline__ void r16_tile(const R16Args &g, int t, int step, int kq, int l16, cplx (&cv)[4], cplx (&xv)[4])
+{
+    const int r0 = (t % g.mt) * 16, c0 = (t / g.mt) * 16;
+    const int row = r0 + l16;
+    double4_t ar, ai;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        ar[j] = cv[j].x;
+        ai[j] = cv[j].y;
+    }
+    // the C tile of the wave's tile after next into the registers just read
+    const int tn = min(t + step, g.ntiles - 1);
+    const int nr0 = (tn % g.mt) * 16, nc0 = (tn / g.mt) * 16;
+    const size_t nrowc = (size_t)min(nr0 + l16, g.mrows - 1);
+#pragma unroll
+    for (int j = 0; j < 4; j++) cv[j] = gld(g.C + (nrowc + (size_t)min(nc0 + kq + 4 * j, g.ncols - 1) * g.ldc));
+    const size_t nxc = (size_t)min(nc0 + l16, g.ncols - 1) * g.ldb;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int p = 4 * s + kq;
+        cplx x = make_double2(0.0, 0.0), y = make_double2(0.0, 0.0);
+        if (p < g.b) {
+            x = xv[s];                                                          // X[c][p] = B[p][c]
+            if (row < g.mrows) y = g.vp ? vpanel(g.A, g.lda, g.roff + row, p) : g.A[p * g.lda + g.roff + row];  // Y[p][r]
+        }
+        ar = mfma64(-x.x, y.x, ar);
+        ar = mfma64(x.y, y.y, ar);
+        ai = mfma64(-x.x, y.y, ai);
+        ai = mfma64(-x.y, y.x, ai);
+        // and its B fragment, entry by entry as the current one is consumed
+        xv[s] = gld(g.B + (min(4 * s + kq, g.b - 1) + nxc));
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int col = c0 + kq + 4 * j;
+        if (row < g.mrows && col < g.ncols) gst(g.C + ((size_t)row + (size_t)col * g.ldc), make_double2(ar[j], ai[j]));
+    }
+}
+
 __device__ static void wg_rank16(cplx *C, int ldc, int mrows, int ncols, const cplx *A, int lda, int roff, int b,
                                  bool vp, const cplx *B, int ldb)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int l16 = lane & 15, kq = lane >> 4;
-    const int mt = (mrows + 15) >> 4, nt = (ncols + 15) >> 4;
-    for (int t = w; t < mt * nt; t += nw) {
-        const int r0 = (t % mt) * 16, c0 = (t / mt) * 16;
-        const int row = r0 + l16;
-        double4_t ar, ai;
-        for (int j = 0; j < 4; j++) {
-            const int col = c0 + kq + 4 * j;
-            cplx v = (row < mrows && col < ncols) ? C[(size_t)row + (size_t)col * ldc] : make_double2(0.0, 0.0);
-            ar[j] = v.x;
-            ai[j] = v.y;
-        }
-#pragma unroll
-        for (int p0 = 0; p0 < PB; p0 += 4) {
-            const int p = p0 + kq;
-            cplx x = make_double2(0.0, 0.0), y = make_double2(0.0, 0.0);
-            if (p < b) {
-                if (c0 + l16 < ncols) x = B[p + (size_t)(c0 + l16) * ldb];               // X[c][p] = B[p][c]
-                if (row < mrows) y = vp ? vpanel(A, lda, roff + row, p) : A[p * lda + roff + row];  // Y[p][r]
-            }
-            ar = mfma64(-x.x, y.x, ar);
-            ar = mfma64(x.y, y.y, ar);
-            ai = mfma64(-x.x, y.y, ai);
-            ai = mfma64(-x.y, y.x, ai);
-        }
-        for (int j = 0; j < 4; j++) {
-            const int col = c0 + kq + 4 * j;
-            if (row < mrows && col < ncols) C[(size_t)row + (size_t)col * ldc] = make_double2(ar[j], ai[j]);
-        }
+    R16Args g;
+    g.C = gp(C);
+    g.B = gp(B);
+    g.A = A;
+    g.ldc = ldc; g.mrows = mrows; g.ncols = ncols; g.lda = lda; g.roff = roff; g.b = b; g.ldb = ldb;
+    g.mt = (mrows + 15) >> 4;
+    g.ntiles = g.mt * ((ncols + 15) >> 4);
+    g.vp = vp;
+    if (w >= g.ntiles) return;
+    cplx ca[4], xa[4], cb[4], xb[4];
+    r16_load(g, w, kq, l16, ca, xa);
+    r16_load(g, w + nw, kq, l16, cb, xb);
+    for (int t = w; t < g.ntiles; t += 2 * nw) {
+        r16_tile(g, t, 2 * nw, kq, l16, ca, xa);
+        if (t + nw < g.ntiles) r16_tile(g, t + nw, 2 * nw, kq, l16, cb, xb);
     }
 }
 
 // T factor of one panel (zlarft, forward, columnwise): S = V^H V over the panel's m rows,
-// T(:,j) = -tau_j T(:,0:j) S(0:j, j); V is the m x b panel in LDS (ld M, unit diagonal implicit)
+// T(:,j) = -tau_j T(:,0:j) S(0:j, j); V is the m x b panel in LDS (ld M, unit diagonal implicit).
+// Row i of T only needs row i itself and S (T(i, j) = -tau_j sum_{i <= q < j} T(i, q) S(q, j)),
+// so lane i builds its row in registers, all rows at once.
 __device__ static void panel_tfactor(const cplx *P, int M, int m, int b, const cplx *tau, cplx *T, cplx *S)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -170,18 +249,29 @@ __device__ static void panel_tfactor(const cplx *P, int M, int m, int b, const c
         if (lane == 0) S[j * PB + i] = make_double2(sr, si);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int j = 0; j < PB; j++)
-            for (int i = 0; i < PB; i++) T[j * PB + i] = make_double2(0.0, 0.0);
-        for (int j = 0; j < b; j++) {
-            cplx tj = tau[j];
-            T[j * PB + j] = tj;
-            for (int i = 0; i < j; i++) {
+    if (threadIdx.x < PB) {
+        const int i = threadIdx.x;
+        GAS cplx *tg = gp(tau);
+        cplx tv[PB], trow[PB];
+#pragma unroll
+        for (int j = 0; j < PB; j++) {
+            tv[j] = gld(tg + min(j, b - 1));
+            trow[j] = make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int j = 0; j < PB; j++) {
+            if (j < b && j > i) {
                 cplx acc = make_double2(0.0, 0.0);
-                for (int q = i; q < j; q++) acc = cadd(acc, cmul(T[q * PB + i], S[j * PB + q]));
-                T[j * PB + i] = cmul(make_double2(-tj.x, -tj.y), acc);
+#pragma unroll
+                for (int q = 0; q < PB; q++)
+                    if (q >= i && q < j) acc = cadd(acc, cmul(trow[q], S[j * PB + q]));
+                trow[j] = cmul(make_double2(-tv[j].x, -tv[j].y), acc);
+            } else if (j < b && j == i) {
+                trow[j] = tv[j];
             }
         }
+#pragma unroll
+        for (int j = 0; j < PB; j++) T[j * PB + i] = trow[j];
     }
     __syncthreads();
 }
@@ -393,17 +483,21 @@ __device__ __forceinline__ void lu_blk_body(cplx *A, int n, int lda, int32_t *pi
         }
         __syncthreads();
     }
+    // log det = sum_i log(+-U_ii), summed by the whole workgroup
+    __syncthreads();
+    double lr = 0.0, li = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        cplx d = gld(gp(A) + ((size_t)i + (size_t)i * lda));
+        if (piv[i] != i) d = make_double2(-d.x, -d.y);
+        const cplx l = clog_(d);
+        lr += l.x;
+        li += l.y;
+    }
+    lr = blk_sum(lr, rv);
+    li = blk_sum(li, rv);
     if (threadIdx.x == 0) {
-        cplx ld = make_double2(0.0, 0.0);
-        for (int i = 0; i < n; i++) {
-            cplx d = A[(size_t)i + (size_t)i * lda];
-            if (piv[i] != i) d = make_double2(-d.x, -d.y);
-            ld = cadd(ld, clog_(d));
-        }
-        double im = ld.y;
-        double k2 = round(im / 2.0 / M_PI);
-        ld.y = im - 2.0 * M_PI * k2;
-        *ldet = ld;
+        const double k2 = round(li / 2.0 / M_PI);
+        *ldet = make_double2(lr, li - 2.0 * M_PI * k2);
         *info = (double)s_info;
     }
 }
@@ -502,20 +596,24 @@ extern "C" __global__ void __launch_bounds__(BLK_NT)
 k_qr_blk(const DChain *chains, int sel, int out_pt)
 {
     extern __shared__ cplx bsh[];
+    __shared__ double red[16];
     const DChain &c = chains[blockIdx.x];
     const int n = c.n;
     qr_blk_body(chain_mat(c, sel, 0), n, n, n, c.tau, c.tblk, c.wk, PB, bsh);
-    if (threadIdx.x == 0) {
-        cplx s = make_double2(0.0, 0.0);
-        for (int i = 0; i < n; i++) {
-            cplx t = c.tau[i];
-            if (t.x != 0.0 || t.y != 0.0) {
-                cplx q = cdiv(cscale(t, 2.0 * t.x), cmul(t, cconj(t)));
-                s = cadd(s, clog_(make_double2(1.0 - q.x, -q.y)));
-            }
+    // log det Q = sum_i log(1 - tau_i 2 Re(tau_i) / |tau_i|^2), summed by the whole workgroup
+    double sr = 0.0, si = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const cplx t = gld(gp(c.tau) + i);
+        if (t.x != 0.0 || t.y != 0.0) {
+            const cplx q = cdiv(cscale(t, 2.0 * t.x), cmul(t, cconj(t)));
+            const cplx l = clog_(make_double2(1.0 - q.x, -q.y));
+            sr += l.x;
+            si += l.y;
         }
-        c.ldq[out_pt] = s;
     }
+    sr = blk_sum(sr, red);
+    si = blk_sum(si, red);
+    if (threadIdx.x == 0) c.ldq[out_pt] = make_double2(sr, si);
 }
 
 extern "C" __global__ void __launch_bounds__(BLK_NT)
