@@ -115,7 +115,7 @@ k_sfmt_draw(const DChain *chains, int nchains, int mode)
 // ------------------------------------------------------------------------------------------
 // Kinetic and interaction propagation, checkerboard form (BREAKUP_T).
 //   column pass (left products, rows mixed): a workgroup owns CPB full columns in LDS.
-//   row pass (right products, columns mixed): a workgroup owns RPB full rows in LDS.
+//   row pass (right products, columns mixed): a workgroup owns RPB full rows in LDS (ld n + 1).
 // Program bits: 1 = families 0..nn_nf-1, 2 = V scaling, 4 = families nn_nf-1..0,
 //               8 = V scaling before everything.  Several slices can be chained (B products).
 // Bond 2x2 blocks are column-major [T11, T21, T12, T22] (dqmc_left_forward_prop.f90:18-44).
@@ -203,13 +203,25 @@ __device__ __forceinline__ void cb_left_fams(const Chain &c, cplx *lds, int ncol
     }
 }
 
+// the four factors of the HS components, staged in LDS once per pass (a register table indexed by
+// the component is lowered to a private-memory lookup)
+__device__ __forceinline__ const cplx *vtab_lds(const DChain &c, int vsel)
+{
+    __shared__ cplx vt[4];
+    if (threadIdx.x < 4) vt[threadIdx.x] = vtab(c, vsel, threadIdx.x + 1);
+    __syncthreads();
+    return vt;
+}
+
 __device__ static void cb_left_v(const DChain &c, cplx *lds, int ncol, int nt, int vsel)
 {
     const int n = c.n;
-    const int32_t *cf = c.conf + (size_t)(nt - 1) * n;
+    GAS const int32_t *cf = gp(c.conf + (size_t)(nt - 1) * n);
+    const cplx *vt = vtab_lds(c, vsel);
+#pragma unroll 8
     for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
         int r = e % n;
-        lds[e] = cmul(vtab(c, vsel, cf[r]), lds[e]);
+        lds[e] = cmul(vt[cf[r] - 1], lds[e]);
     }
     __syncthreads();
 }
@@ -241,9 +253,15 @@ k_cb_cols(const DChain *chains, int msel, int pt, int ksel, int prog, int vsel, 
     stage_out(lds, n * ncol, [=](int e, cplx v) { gst(Mc + (e), v); });
 }
 
+// rows in LDS are row-major with the padded leading dimension ldr = n + 1 (lds[rr * ldr + col]):
+// the lanes of a family pass touch consecutive sites of one row, and the staging writes of
+// eight consecutive rows fall on different banks
+__host__ __device__ __forceinline__ int cb_ldr(int n) { return n + 1; }
+
 template <class Chain>
 __device__ __forceinline__ void cb_right_fams(const Chain &c, cplx *lds, int nrow, const cplx *T, bool rev)
 {
+    const int ldr = cb_ldr(c.n);
     const int nf = c.nn_nf, nl = c.nn_lf;
     const int groups = max(1, (int)blockDim.x / nl);
     const int b = threadIdx.x % nl, g = threadIdx.x / nl;
@@ -255,9 +273,9 @@ __device__ __forceinline__ void cb_right_fams(const Chain &c, cplx *lds, int nro
             const int i1 = c.bonds[2 * ist], i2 = c.bonds[2 * ist + 1];
             const cplx t0 = T[4 * ist], t1 = T[4 * ist + 1], t2 = T[4 * ist + 2], t3 = T[4 * ist + 3];
             for (int rr = g; rr < nrow; rr += groups) {
-                cplx x = lds[i1 * RPB + rr], y = lds[i2 * RPB + rr];
-                lds[i1 * RPB + rr] = cadd(cmul(x, t0), cmul(y, t1));
-                lds[i2 * RPB + rr] = cadd(cmul(x, t2), cmul(y, t3));
+                cplx x = lds[rr * ldr + i1], y = lds[rr * ldr + i2];
+                lds[rr * ldr + i1] = cadd(cmul(x, t0), cmul(y, t1));
+                lds[rr * ldr + i2] = cadd(cmul(x, t2), cmul(y, t3));
             }
         }
         __syncthreads();
@@ -267,10 +285,13 @@ __device__ __forceinline__ void cb_right_fams(const Chain &c, cplx *lds, int nro
 __device__ static void cb_right_v(const DChain &c, cplx *lds, int nt, int vsel)
 {
     const int n = c.n;
-    const int32_t *cf = c.conf + (size_t)(nt - 1) * n;
+    GAS const int32_t *cf = gp(c.conf + (size_t)(nt - 1) * n);
+    const cplx *vt = vtab_lds(c, vsel);
+    const int ldr = cb_ldr(n);
+#pragma unroll 8
     for (int e = threadIdx.x; e < n * RPB; e += blockDim.x) {
-        int col = e / RPB;
-        lds[e] = cmul(lds[e], vtab(c, vsel, cf[col]));
+        const int rr = e / n, col = e % n;
+        lds[rr * ldr + col] = cmul(lds[rr * ldr + col], vt[cf[col] - 1]);
     }
     __syncthreads();
 }
@@ -286,21 +307,31 @@ k_cb_rows(const DChain *chains, int msel, int pt, int ksel, int prog, int vsel, 
     const int nrow = min(RPB, n - r0);
     cplx *M = chain_mat(c, msel, pt);
     const cplx *T = ksel ? c.ktm1 : c.kt;
-    // lds[col * RPB + rr] = M(r0 + rr, col): eight consecutive threads read one 128-byte run
+    // lds[rr * ldr + col] = M(r0 + rr, col): eight consecutive threads read one 128-byte run
     GAS cplx *Mg = gp(M);
-    stage_in(lds, n * RPB, [=](int e) {
+    const int ldr = cb_ldr(n);
+    stage_in_map<ST_BATCH>(lds, n * RPB, [=](int e) {
         const int rr = e % RPB, col = e / RPB;
         return rr < nrow ? gld(Mg + ((size_t)(r0 + rr) + (size_t)col * n)) : make_double2(0.0, 0.0);
-    });
+    }, [=](int e) { return (e % RPB) * ldr + e / RPB; });
     __syncthreads();
     if ((prog & 8) && c.lwrapu) cb_right_v(c, lds, nt, vsel);
     if ((prog & 1) && c.lwrapT) cb_right_fams(c, lds, nrow, T, false);
     if ((prog & 2) && c.lwrapu) cb_right_v(c, lds, nt, vsel);
     if ((prog & 4) && c.lwrapT) cb_right_fams(c, lds, nrow, T, true);
-    stage_out(lds, n * RPB, [=](int e, cplx v) {
-        const int rr = e % RPB, col = e / RPB;
-        if (rr < nrow) gst(Mg + ((size_t)(r0 + rr) + (size_t)col * n), v);
-    });
+    for (int base = threadIdx.x; base < n * RPB; base += ST_BATCH * blockDim.x) {
+        cplx v[ST_BATCH];
+#pragma unroll
+        for (int u = 0; u < ST_BATCH; u++) {
+            const int e = base + u * (int)blockDim.x;
+            v[u] = e < n * RPB ? lds[(e % RPB) * ldr + e / RPB] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < ST_BATCH; u++) {
+            const int e = base + u * (int)blockDim.x, rr = e % RPB, col = e / RPB;
+            if (e < n * RPB && rr < nrow) gst(Mg + ((size_t)(r0 + rr) + (size_t)col * n), v[u]);
+        }
+    }
 }
 
 // exp(V) scalings for the dense-K form: rows (side 0) or columns (side 1), or both (side 2:
@@ -353,21 +384,37 @@ __device__ __forceinline__ void zgemm_tile(const cplx *A_, int lda, const cplx *
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++)
             for (int r = 0; r < 4; r++) { accr[i][j][r] = 0.0; acci[i][j][r] = 0.0; }
-    for (int k0 = 0; k0 < kk; k0 += GK) {
+    // the next k block's operands are loaded into registers (clamped, branch-free addresses)
+    // while the current block's MFMAs run from LDS
+    cplx va[4], vb[4];
+    auto load_k = [&](int k0) {
+#pragma unroll
         for (int q = 0; q < 4; q++) {
-            int idx = t + 256 * q;
-            int i, k;
-            cplx v = make_double2(0.0, 0.0);
-            if (OPA == 0) { i = idx % GT; k = idx / GT; if (tm + i < m && k0 + k < kk) v = gld(A + ((size_t)(tm + i) + (size_t)(k0 + k) * lda)); }
-            else { k = idx % GK; i = idx / GK; if (tm + i < m && k0 + k < kk) v = cconj(gld(A + ((size_t)(k0 + k) + (size_t)(tm + i) * lda))); }
-            sAr[i][k] = v.x; sAi[i][k] = v.y;
-            int j;
-            cplx u = make_double2(0.0, 0.0);
-            if (OPB == 0) { k = idx % GK; j = idx / GK; if (tn + j < n && k0 + k < kk) u = gld(B + ((size_t)(k0 + k) + (size_t)(tn + j) * ldb)); }
-            else { j = idx % GT; k = idx / GT; if (tn + j < n && k0 + k < kk) u = cconj(gld(B + ((size_t)(tn + j) + (size_t)(k0 + k) * ldb))); }
-            sBr[k][j] = u.x; sBi[k][j] = u.y;
+            const int idx = t + 256 * q;
+            int i, k, j;
+            if (OPA == 0) { i = idx % GT; k = idx / GT; } else { k = idx % GK; i = idx / GK; }
+            const int ic = min(tm + i, m - 1), kc = min(k0 + k, kk - 1);
+            const cplx a = OPA == 0 ? gld(A + ((size_t)ic + (size_t)kc * lda)) : cconj(gld(A + ((size_t)kc + (size_t)ic * lda)));
+            va[q] = (tm + i < m && k0 + k < kk) ? a : make_double2(0.0, 0.0);
+            if (OPB == 0) { k = idx % GK; j = idx / GK; } else { j = idx % GT; k = idx / GT; }
+            const int jc = min(tn + j, n - 1), kc2 = min(k0 + k, kk - 1);
+            const cplx u = OPB == 0 ? gld(B + ((size_t)kc2 + (size_t)jc * ldb)) : cconj(gld(B + ((size_t)jc + (size_t)kc2 * ldb)));
+            vb[q] = (tn + j < n && k0 + k < kk) ? u : make_double2(0.0, 0.0);
+        }
+    };
+    load_k(0);
+    for (int k0 = 0; k0 < kk; k0 += GK) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int idx = t + 256 * q;
+            int i, k, j;
+            if (OPA == 0) { i = idx % GT; k = idx / GT; } else { k = idx % GK; i = idx / GK; }
+            sAr[i][k] = va[q].x; sAi[i][k] = va[q].y;
+            if (OPB == 0) { k = idx % GK; j = idx / GK; } else { j = idx % GT; k = idx / GT; }
+            sBr[k][j] = vb[q].x; sBi[k][j] = vb[q].y;
         }
         __syncthreads();
+        if (k0 + GK < kk) load_k(k0 + GK);
 #pragma unroll
         for (int ks = 0; ks < GK / 4; ks++) {
             const int k = 4 * ks + (lane >> 4), l16 = lane & 15;

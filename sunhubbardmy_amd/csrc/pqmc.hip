@@ -21,7 +21,7 @@ struct shp_ctx : ctx_base {
 
 namespace {
 
-size_t p_lds_prop(const shp_ctx *c) { return (size_t)c->n * std::max(CPB, RPB) * sizeof(cplx); }
+size_t p_lds_prop(const shp_ctx *c) { return (size_t)cb_ldr(c->n) * std::max(CPB, RPB) * sizeof(cplx); }
 size_t p_lds_bonds(const shp_ctx *c) { return (size_t)2 * c->ns * c->ns * sizeof(cplx); }
 size_t p_lds_qr(const shp_ctx *c) { return ((size_t)c->n * c->pb + 2 * PB * PB) * sizeof(cplx); }
 size_t p_lds_lu(const shp_ctx *c) { return (size_t)c->ne * PB * sizeof(cplx) + (size_t)c->ne * sizeof(int); }

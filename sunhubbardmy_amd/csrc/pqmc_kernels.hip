@@ -249,14 +249,14 @@ kp_prop(const PChain *chains, POps ops)
         }
         for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) M[(size_t)(e % n) + (size_t)(c0 + e / n) * n] = lds[e];
     } else {
-        // UL from the right, RPB rows; lds[col * RPB + rr]
+        // UL from the right, RPB rows; lds[rr * ldr + col]
         const int r0 = blockIdx.x * RPB;
         if (r0 >= ne) return;
-        const int nrow = min(RPB, ne - r0);
+        const int nrow = min(RPB, ne - r0), ldr = cb_ldr(n);
         cplx *M = c.UL;
         for (int e = threadIdx.x; e < n * RPB; e += blockDim.x) {
             int rr = e % RPB, col = e / RPB;
-            lds[e] = (rr < nrow) ? M[(size_t)(r0 + rr) + (size_t)col * ne] : make_double2(0.0, 0.0);
+            lds[rr * ldr + col] = (rr < nrow) ? M[(size_t)(r0 + rr) + (size_t)col * ne] : make_double2(0.0, 0.0);
         }
         __syncthreads();
         for (int o = 0; o < ops.nops; o++) {
@@ -267,11 +267,10 @@ kp_prop(const PChain *chains, POps ops)
             if (ops.kind[o] == 3) {
                 const int32_t *uc = c.uconf + (size_t)(ops.nt - 1) * n;
                 for (int e = threadIdx.x; e < n * RPB; e += blockDim.x) {
-                    const int rr = e % RPB, col = e / RPB;
+                    const int rr = e / n, col = e % n;
                     const int is = uc[col] - 1;
                     const cplx fct = ops.modeR[o] ? c.ubi[is] : c.ub[is];
-                    lds[e] = cmul(lds[e], fct);
-                    (void)rr;
+                    lds[rr * ldr + col] = cmul(lds[rr * ldr + col], fct);
                 }
                 __syncthreads();
                 continue;
@@ -284,17 +283,17 @@ kp_prop(const PChain *chains, POps ops)
                 const int i1 = c.bonds[2 * ist], i2 = c.bonds[2 * ist + 1];
                 const int is = cf[bb];
                 for (int rr = g; rr < nrow; rr += groups) {
-                    cplx x = lds[i1 * RPB + rr], y = lds[i2 * RPB + rr];
+                    cplx x = lds[rr * ldr + i1], y = lds[rr * ldr + i2];
                     v_right_pair(tb, is, mode, x, y);
-                    lds[i1 * RPB + rr] = x;
-                    lds[i2 * RPB + rr] = y;
+                    lds[rr * ldr + i1] = x;
+                    lds[rr * ldr + i2] = y;
                 }
             }
             __syncthreads();
         }
         for (int e = threadIdx.x; e < n * RPB; e += blockDim.x) {
             int rr = e % RPB, col = e / RPB;
-            if (rr < nrow) M[(size_t)(r0 + rr) + (size_t)col * ne] = lds[e];
+            if (rr < nrow) M[(size_t)(r0 + rr) + (size_t)col * ne] = lds[rr * ldr + col];
         }
     }
 }

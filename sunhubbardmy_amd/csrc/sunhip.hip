@@ -179,7 +179,7 @@ static void copy_mat(sh_ctx *ctx, int dsel, int dpt, int ssel, int spt)
 }
 
 static size_t lds_cols(sh_ctx *ctx) { return (size_t)ctx->nmax * CPB * sizeof(cplx); }
-static size_t lds_rows(sh_ctx *ctx) { return (size_t)ctx->nmax * RPB * sizeof(cplx); }
+static size_t lds_rows(sh_ctx *ctx) { return (size_t)cb_ldr(ctx->nmax) * RPB * sizeof(cplx); }
 
 // left product (rows mixed) on matrix sel: program bits as in k_cb_cols
 static void left_pass(sh_ctx *ctx, int sel, int pt, int ksel, int prog, int vsel, int nt0, int nt1, int step)
