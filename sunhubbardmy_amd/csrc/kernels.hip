@@ -986,26 +986,93 @@ __device__ static void block_argmax(double v, int idx, double *rv, int *ri, doub
     *outi = bi;
 }
 
-extern "C" __global__ void __launch_bounds__(1024)
+// One wave's share of rows r0 + lane + 64 q (q < NQ) of a column, loaded with clamped addresses
+// in one batch (entries past n are never used).
+template <int NQ>
+__device__ __forceinline__ void qr_col_load(GAS cplx *col, int r0, int n, int lane, cplx (&x)[NQ])
+{
+#pragma unroll
+    for (int q = 0; q < NQ; q++) x[q] = gld(col + min(r0 + lane + 64 * q, n - 1));
+}
+
+// H^H = I - conj(tau) v v^H on one trailing column held in registers (rows i + lane + 64 q), its
+// norm downdate as zlaqp2 (the vn1/vn2 recomputation below tol3z)
+template <int NQ>
+__device__ __forceinline__ void qr_apply_col(GAS cplx *a, const cplx (&x)[NQ], int i, int j, int n, int lane, cplx tc,
+                                             const cplx *v, bool pivot, double *vn1, double *vn2, double tol3z)
+{
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int r = i + lane + 64 * q;
+        if (r < n) {
+            const cplx vv = v[r], aa = x[q];
+            sr += vv.x * aa.x + vv.y * aa.y;     // conj(v) a
+            si += vv.x * aa.y - vv.y * aa.x;
+        }
+    }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    const cplx f = cmul(tc, make_double2(sr, si));
+    double nrm2 = 0.0;
+    cplx ai_new = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int r = i + lane + 64 * q;
+        if (r < n) {
+            const cplx aa = csub(x[q], cmul(v[r], f));
+            gst(a + r, aa);
+            if (r > i) nrm2 += cabs2(aa);
+            else ai_new = aa;   // r == i happens on lane 0 only
+        }
+    }
+    if (pivot) {
+        nrm2 = wave_sum(nrm2);
+        if (lane == 0 && vn1[j] != 0.0) {
+            double t = sqrt(cabs2(ai_new)) / vn1[j];
+            t = 1.0 - t * t;
+            if (t < 0.0) t = 0.0;
+            double t2 = t * (vn1[j] / vn2[j]) * (vn1[j] / vn2[j]);
+            if (t2 <= tol3z) {
+                vn1[j] = (i < n - 1) ? sqrt(nrm2) : 0.0;
+                vn2[j] = vn1[j];
+            } else {
+                vn1[j] *= sqrt(t);
+            }
+        }
+    }
+}
+
+// Householder QR, column-at-a-time, with column pivoting (pivot = 1, zgeqp3's choice by the
+// downdated partial norms) or in the given order (pivot = 0); R above the diagonal, the
+// reflectors below, tau to c.tau, log det Q to ldq[out_pt].  Every column a wave works on is
+// read once into registers (NQ = rows / 64), and the wave's next trailing column is loaded while
+// the current one is reduced.
+template <int NQ>
+__global__ void __launch_bounds__(1024)
 k_qr(const DChain *chains, int sel, int pivot, int out_pt)
 {
-    extern __shared__ double qsh[];
+    extern __shared__ double qrsh[];
     const DChain &c = chains[blockIdx.x];
     const int n = c.n;
-    double *vn1 = qsh, *vn2 = qsh + n;
-    cplx *v = (cplx *)(qsh + 2 * n);          // n
+    double *vn1 = qrsh, *vn2 = qrsh + n;
+    cplx *v = (cplx *)(qrsh + 2 * n);          // n
     __shared__ double rv[16];
     __shared__ int ri[16];
     __shared__ cplx s_tau;
-    cplx *A = chain_mat(c, sel, 0);
+    GAS cplx *A = gp(chain_mat(c, sel, 0));
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     // the unpivoted call keeps the prepivot order that k_prepivot left in jpvt
     if (pivot)
         for (int j = threadIdx.x; j < n; j += blockDim.x) c.jpvt[j] = j + 1;
     if (pivot) {
         for (int j = w; j < n; j += nw) {
+            cplx x[NQ];
+            qr_col_load<NQ>(A + (size_t)j * n, 0, n, lane, x);
             double s = 0.0;
-            for (int r = lane; r < n; r += 64) s += cabs2(A[(size_t)r + (size_t)j * n]);
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+                if (lane + 64 * q < n) s += cabs2(x[q]);
             s = wave_sum(s);
             if (lane == 0) { vn1[j] = sqrt(s); vn2[j] = vn1[j]; }
         }
@@ -1025,9 +1092,9 @@ k_qr(const DChain *chains, int sel, int pivot, int out_pt)
             block_argmax(bv, bi, rv, ri, &mv, &p);
             if (p != i) {
                 for (int r = threadIdx.x; r < n; r += blockDim.x) {
-                    cplx t = A[(size_t)r + (size_t)p * n];
-                    A[(size_t)r + (size_t)p * n] = A[(size_t)r + (size_t)i * n];
-                    A[(size_t)r + (size_t)i * n] = t;
+                    const cplx t = gld(A + ((size_t)r + (size_t)p * n)), u = gld(A + ((size_t)r + (size_t)i * n));
+                    gst(A + ((size_t)r + (size_t)p * n), u);
+                    gst(A + ((size_t)r + (size_t)i * n), t);
                 }
                 __syncthreads();
                 if (threadIdx.x == 0) {
@@ -1040,11 +1107,15 @@ k_qr(const DChain *chains, int sel, int pivot, int out_pt)
         }
         // reflector of column i (zlarfg), wave 0
         if (w == 0) {
-            cplx *col = A + (size_t)i * n;
+            GAS cplx *col = A + (size_t)i * n;
+            cplx x[NQ];
+            qr_col_load<NQ>(col, i + 1, n, lane, x);
+            const cplx alpha = gld(col + i);
             double s = 0.0;
-            for (int r = i + 1 + lane; r < n; r += 64) s += cabs2(col[r]);
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+                if (i + 1 + lane + 64 * q < n) s += cabs2(x[q]);
             s = wave_sum(s);
-            cplx alpha = col[i];
             double xnorm = sqrt(s);
             cplx tau = make_double2(0.0, 0.0);
             cplx scal = make_double2(0.0, 0.0);
@@ -1056,13 +1127,17 @@ k_qr(const DChain *chains, int sel, int pivot, int out_pt)
                 tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
                 scal = cinv(make_double2(alpha.x - beta, alpha.y));
             }
-            for (int r = i + 1 + lane; r < n; r += 64) {
-                cplx x = triv ? col[r] : cmul(col[r], scal);
-                col[r] = x;
-                v[r] = x;
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int r = i + 1 + lane + 64 * q;
+                if (r < n) {
+                    const cplx xs = triv ? x[q] : cmul(x[q], scal);
+                    gst(col + r, xs);
+                    v[r] = xs;
+                }
             }
             if (lane == 0) {
-                if (!triv) col[i] = make_double2(beta, 0.0);
+                if (!triv) gst(col + i, make_double2(beta, 0.0));
                 v[i] = make_double2(1.0, 0.0);
                 s_tau = tau;
                 c.tau[i] = tau;
@@ -1070,56 +1145,51 @@ k_qr(const DChain *chains, int sel, int pivot, int out_pt)
         }
         __syncthreads();
         const cplx tc = cconj(s_tau);
-        // apply H^H = I - conj(tau) v v^H to the trailing columns, one wave per column
-        for (int j = i + 1 + w; j < n; j += nw) {
-            cplx *a = A + (size_t)j * n;
-            double sr = 0.0, si = 0.0;
-            for (int r = i + lane; r < n; r += 64) {
-                cplx vv = v[r], aa = a[r];
-                sr += vv.x * aa.x + vv.y * aa.y;     // conj(v) a
-                si += vv.x * aa.y - vv.y * aa.x;
+        // apply H^H to the trailing columns, one wave per column; the wave's next column is
+        // in flight while the current one is reduced
+        int j = i + 1 + w;
+        if (NQ > 4) {
+            // (two register sets of more than four slots would spill at 1024 threads)
+            for (; j < n; j += nw) {
+                cplx xa[NQ];
+                qr_col_load<NQ>(A + (size_t)j * n, i, n, lane, xa);
+                qr_apply_col<NQ>(A + (size_t)j * n, xa, i, j, n, lane, tc, v, pivot, vn1, vn2, tol3z);
             }
-            sr = wave_sum(sr);
-            si = wave_sum(si);
-            cplx f = cmul(tc, make_double2(sr, si));
-            double nrm2 = 0.0;
-            cplx ai_new = make_double2(0.0, 0.0);
-            for (int r = i + lane; r < n; r += 64) {
-                cplx aa = csub(a[r], cmul(v[r], f));
-                a[r] = aa;
-                if (r > i) nrm2 += cabs2(aa);
-                else ai_new = aa;   // r == i happens on lane 0 only
-            }
-            if (pivot) {
-                nrm2 = wave_sum(nrm2);
-                if (lane == 0 && vn1[j] != 0.0) {
-                    double t = sqrt(cabs2(ai_new)) / vn1[j];
-                    t = 1.0 - t * t;
-                    if (t < 0.0) t = 0.0;
-                    double t2 = t * (vn1[j] / vn2[j]) * (vn1[j] / vn2[j]);
-                    if (t2 <= tol3z) {
-                        vn1[j] = (i < n - 1) ? sqrt(nrm2) : 0.0;
-                        vn2[j] = vn1[j];
-                    } else {
-                        vn1[j] *= sqrt(t);
-                    }
+        } else if (j < n) {
+            cplx xa[NQ], xb[NQ];
+            qr_col_load<NQ>(A + (size_t)j * n, i, n, lane, xa);
+            for (; j < n; j += 2 * nw) {
+                const int j2 = j + nw;
+                if (j2 < n) qr_col_load<NQ>(A + (size_t)j2 * n, i, n, lane, xb);
+                qr_apply_col<NQ>(A + (size_t)j * n, xa, i, j, n, lane, tc, v, pivot, vn1, vn2, tol3z);
+                if (j2 < n) {
+                    if (j2 + nw < n) qr_col_load<NQ>(A + (size_t)(j2 + nw) * n, i, n, lane, xa);
+                    qr_apply_col<NQ>(A + (size_t)j2 * n, xb, i, j2, n, lane, tc, v, pivot, vn1, vn2, tol3z);
                 }
             }
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        cplx s = make_double2(0.0, 0.0);
-        for (int i = 0; i < n; i++) {
-            cplx t = c.tau[i];
-            if (t.x != 0.0 || t.y != 0.0) {
-                cplx q = cdiv(cscale(t, 2.0 * t.x), cmul(t, cconj(t)));
-                s = cadd(s, clog_(make_double2(1.0 - q.x, -q.y)));
-            }
+    // log det Q, summed by the whole workgroup
+    double sr = 0.0, si = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const cplx t = gld(gp(c.tau) + q);
+        if (t.x != 0.0 || t.y != 0.0) {
+            const cplx qq = cdiv(cscale(t, 2.0 * t.x), cmul(t, cconj(t)));
+            const cplx l = clog_(make_double2(1.0 - qq.x, -qq.y));
+            sr += l.x;
+            si += l.y;
         }
-        c.ldq[out_pt] = s;
     }
+    __shared__ double red[16];
+    sr = block_sum(sr, red);
+    si = block_sum(si, red);
+    if (threadIdx.x == 0) c.ldq[out_pt] = make_double2(sr, si);
 }
+
+template __global__ void k_qr<4>(const DChain *, int, int, int);
+template __global__ void k_qr<8>(const DChain *, int, int, int);
+template __global__ void k_qr<16>(const DChain *, int, int, int);
 
 // Q = H_1 ... H_n from the reflectors in matrix `sel` (zung2r), written to chain point U[pt]
 // (dest 0) or Us (dest 1)

@@ -238,13 +238,21 @@ static void lu_solve(sh_ctx *ctx, int xsel)
         hipLaunchKernelGGL(k_lu_solve, dim3(ctx->nch), dim3(512), 0, ctx->st, ctx->dc, xsel);
 }
 
+// column-at-a-time QR of matrix sel (pivot: zgeqp3's column pivoting), rows held in registers
+// in NQ = ceil(N / 64) slots
+static void launch_qr(sh_ctx *ctx, int sel, int pivot, int pt)
+{
+    auto k = ctx->nmax <= 256 ? k_qr<4> : ctx->nmax <= 512 ? k_qr<8> : k_qr<16>;
+    hipLaunchKernelGGL(k, dim3(ctx->nch), dim3(1024), lds_qr(ctx), ctx->st, ctx->dc, sel, pivot, pt);
+}
+
 // QR of W1 (first point: column pivoting) or of the prepivoted W1 diag(D[dpt]) -> point `pt`
 static void udv_step(sh_ctx *ctx, bool first, int pt, int dpt)
 {
     if (first) {
         {
             Tic t(ctx, T_QR);
-            hipLaunchKernelGGL(k_qr, dim3(ctx->nch), dim3(1024), lds_qr(ctx), ctx->st, ctx->dc, M_W1, 1, pt);
+            launch_qr(ctx, M_W1, 1, pt);
         }
         {
             Tic t(ctx, T_FORMQ);
@@ -266,7 +274,7 @@ static void udv_step(sh_ctx *ctx, bool first, int pt, int dpt)
         if (blocked(ctx))
             hipLaunchKernelGGL(k_qr_blk, dim3(ctx->nch), dim3(BLK_NT), lds_blk(ctx), ctx->st, ctx->dc, M_W2, pt);
         else
-            hipLaunchKernelGGL(k_qr, dim3(ctx->nch), dim3(1024), lds_qr(ctx), ctx->st, ctx->dc, M_W2, 0, pt);
+            launch_qr(ctx, M_W2, 0, pt);
     }
     {
         Tic t(ctx, T_FORMQ);
