@@ -633,11 +633,193 @@ k_lu_blk(const DChain *chains)
     __shared__ cplx s_ld;
     __shared__ double s_inf;
     lu_blk_body(c.W3, c.n, c.n, c.piv, &s_ld, &s_inf, bsh);
+    // the row order of the interchanges, for k_solve_cb: the pivots into LDS, the swaps in order
+    // by one thread, the result out
+    const int n = c.n;
+    int *perm = (int *)bsh, *pv = perm + n;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        perm[i] = i;
+        pv[i] = c.piv[i];
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         c.sc[SC_ZLD] = s_ld.x;
         c.sc[SC_ZLD + 1] = s_ld.y;
         c.sc[SC_INFO] = s_inf;
+        for (int k = 0; k < n; k++) {
+            const int p = pv[k], t = perm[p];
+            perm[p] = perm[k];
+            perm[k] = t;
+        }
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) c.lperm[i] = perm[i];
+}
+
+// ------------------------------------------------------------------------------------------
+// (P L U) Y = X in place of X for N <= 256, 16 right-hand sides per workgroup (zgetrs with the
+// factors of k_lu_blk).  The workgroup's 16 columns stay in registers through the forward and
+// the backward substitution, in the MFMA accumulator layout: wave w holds the 16-row tiles
+// t = w, w + 4, w + 8, w + 12; lane l, element r of a tile is row 16 t + 4 r + l / 16, column
+// l % 16.  Panel p (rows 16 p .. 16 p + 15) is solved by the wave that holds tile p against the
+// 16 x 16 diagonal block (staged in LDS), published to LDS, and every wave then updates its
+// tiles below (forward, L) or above (backward, U) by one rank-16 MFMA product per tile whose
+// factor block is read from global memory.  A chain's workgroups run on one XCD
+// (blockIdx.x % 8 picks it) so that they share the factor's panels in that XCD's L2.
+// ------------------------------------------------------------------------------------------
+#define SCB_T 4   // tiles per wave: N <= 256
+
+// forward (unit lower, lower = 1) or backward (upper with diagonal reciprocals dinv) solve of
+// one 16 x 16 tile held in accumulator layout, against D11[a * 16 + b] in LDS
+template <bool LOWER>
+__device__ __forceinline__ void cb_tri(double4_t &xr, double4_t &xi, const cplx *D11, const cplx *dinv, int lq, int l16)
+{
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int i = LOWER ? q : 15 - q;
+        const int ri = i >> 2, qi = i & 3;
+        if (!LOWER && lq == qi) {
+            const cplx d = dinv[i];
+            const double a = xr[ri], b = xi[ri];
+            xr[ri] = a * d.x - b * d.y;
+            xi[ri] = a * d.y + b * d.x;
+        }
+        const int src = l16 + 16 * qi;
+        const double vr = __shfl(xr[ri], src, 64), vi = __shfl(xi[ri], src, 64);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int a = 4 * r + lq;
+            if (LOWER ? a > i : a < i) {
+                const cplx m = D11[a * 16 + i];
+                xr[r] -= m.x * vr - m.y * vi;
+                xi[r] -= m.x * vi + m.y * vr;
+            }
+        }
+    }
+}
+
+// tiles of this wave past (forward) or before (backward) panel p -= F(tile rows, panel cols) Xp
+template <bool FWD>
+__device__ __forceinline__ void cb_update(double4_t (&xr)[SCB_T], double4_t (&xi)[SCB_T], GAS cplx *A, int n, int p,
+                                          int w, int lq, int l16, const cplx *Xp)
+{
+    const int k0 = 16 * p;
+    cplx bv[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) bv[s] = Xp[(4 * s + lq) * 16 + l16];
+#pragma unroll
+    for (int tt = 0; tt < SCB_T; tt++) {
+        const int tile = w + 4 * tt;
+        if (FWD ? tile > p : tile < p) {
+            const int row = 16 * tile + l16;
+            const size_t rowc = (size_t)min(row, n - 1);
+            cplx av[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int kk = k0 + 4 * s + lq;
+                const cplx a = gld(A + (rowc + (size_t)min(kk, n - 1) * n));
+                av[s] = (row < n && kk < n) ? a : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const cplx a = av[s], b = bv[s];
+                xr[tt] = mfma64(-a.x, b.x, xr[tt]);
+                xr[tt] = mfma64(a.y, b.y, xr[tt]);
+                xi[tt] = mfma64(-a.x, b.y, xi[tt]);
+                xi[tt] = mfma64(-a.y, b.x, xi[tt]);
+            }
+        }
+    }
+}
+
+// the diagonal block of panel p (identity past n), four entries per lane, into registers
+__device__ __forceinline__ void cb_diag_load(GAS cplx *A, int n, int p, int lane, cplx (&d)[4])
+{
+    const int k0 = 16 * p;
+#pragma unroll
+    for (int e4 = 0; e4 < 4; e4++) {
+        const int e = lane + 64 * e4, a = e >> 4, b = e & 15;
+        const cplx v = gld(A + ((size_t)min(k0 + a, n - 1) + (size_t)min(k0 + b, n - 1) * n));
+        d[e4] = (k0 + a < n && k0 + b < n) ? v : make_double2(a == b ? 1.0 : 0.0, 0.0);
+    }
+}
+
+// the owner of panel p stages the diagonal block (loaded ahead into d), solves its tile,
+// publishes it
+template <bool FWD>
+__device__ __forceinline__ void cb_panel(double4_t (&xr)[SCB_T], double4_t (&xi)[SCB_T], const cplx (&d)[4], int p,
+                                         int lane, int lq, int l16, cplx *D11, cplx *dinv, cplx *Xp)
+{
+#pragma unroll
+    for (int e4 = 0; e4 < 4; e4++) D11[lane + 64 * e4] = d[e4];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (!FWD && lane < 16) dinv[lane] = cinv(D11[lane * 16 + lane]);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const int tp = p >> 2;
+#pragma unroll
+    for (int tt = 0; tt < SCB_T; tt++) {
+        if (tt == tp) {
+            cb_tri<FWD>(xr[tt], xi[tt], D11, dinv, lq, l16);
+#pragma unroll
+            for (int r = 0; r < 4; r++) Xp[(4 * r + lq) * 16 + l16] = make_double2(xr[tt][r], xi[tt][r]);
+        }
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_solve_cb(const DChain *chains, int nch, int ncb, int xsel)
+{
+    const int bid = blockIdx.x, xcd = bid & 7, kb = bid >> 3;
+    const int chain = xcd + 8 * (kb / ncb), cb = kb % ncb;
+    if (chain >= nch) return;
+    const DChain &c = chains[chain];
+    const int n = c.n, j0 = 16 * cb;
+    if (j0 >= n) return;
+    GAS cplx *A = gp(c.W3), *X = gp(chain_mat(c, xsel, 0));
+    GAS int32_t *perm = gp(c.lperm);
+    __shared__ cplx D11[256], Xp[2][256], dinv[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int nt = (n + 15) >> 4;
+    const int col = j0 + l16;
+    const size_t colc = (size_t)min(col, n - 1) * n;
+    double4_t xr[SCB_T], xi[SCB_T];
+    // the rows of P X
+#pragma unroll
+    for (int tt = 0; tt < SCB_T; tt++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = 16 * (w + 4 * tt) + 4 * r + lq;
+            const int pr = perm[min(row, n - 1)];
+            const cplx v = gld(X + ((size_t)pr + colc));
+            xr[tt][r] = row < n ? v.x : 0.0;
+            xi[tt][r] = row < n ? v.y : 0.0;
+        }
+    // the owner of the next panel loads its diagonal block while the current panel is solved
+    // and applied
+    cplx d[4];
+    if (w == 0) cb_diag_load(A, n, 0, lane, d);
+    for (int p = 0; p < nt; p++) {
+        if (w == (p & 3)) cb_panel<true>(xr, xi, d, p, lane, lq, l16, D11, dinv, Xp[p & 1]);
+        if (w == ((p + 1) & 3)) cb_diag_load(A, n, p + 1 < nt ? p + 1 : nt - 1, lane, d);
+        __syncthreads();
+        cb_update<true>(xr, xi, A, n, p, w, lq, l16, Xp[p & 1]);
+    }
+    __syncthreads();
+    for (int p = nt - 1; p >= 0; p--) {
+        if (w == (p & 3)) cb_panel<false>(xr, xi, d, p, lane, lq, l16, D11, dinv, Xp[p & 1]);
+        if (w == ((p + 3) & 3)) cb_diag_load(A, n, p > 0 ? p - 1 : 0, lane, d);
+        __syncthreads();
+        cb_update<false>(xr, xi, A, n, p, w, lq, l16, Xp[p & 1]);
+    }
+#pragma unroll
+    for (int tt = 0; tt < SCB_T; tt++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = 16 * (w + 4 * tt) + 4 * r + lq;
+            if (row < n && col < n) gst(X + ((size_t)row + (size_t)col * n), make_double2(xr[tt][r], xi[tt][r]));
+        }
 }
 
 extern "C" __global__ void __launch_bounds__(BLK_NT)

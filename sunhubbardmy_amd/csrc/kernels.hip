@@ -519,7 +519,9 @@ __device__ static void wg_rank_k(cplx *G, int n, const cplx *A, const cplx *B, i
 // stored; entries past k are zero in the A fragment), so nothing waits on a load before its
 // use: G tiles are loaded two tiles ahead into one of two alternating register sets, the B
 // fragment one tile ahead (a second B set would spill in the delay kernel).  At k = 32 the
-// update reads and writes G for 8 flops per byte, so it is bound by HBM, not by the MFMAs.
+// update reads and writes G for 8 flops per byte, so it is bound by HBM, not by the MFMAs; G
+// streams through with nontemporal accesses so that the A and B fragments, re-read for every
+// tile, stay in L2.
 // ------------------------------------------------------------------------------------------
 template <int KS>
 __device__ __forceinline__ void rank_k_tile(GAS cplx *G, GAS cplx *B, int n, int k, int row, bool rok, int kq, int l16,
@@ -538,7 +540,7 @@ __device__ __forceinline__ void rank_k_tile(GAS cplx *G, GAS cplx *B, int n, int
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int col = min(j0 + 32 + kq + 4 * r, n - 1);
-        g[r] = gld(G + (rowc + (size_t)col * n));
+        g[r] = gld_nt(G + (rowc + (size_t)col * n));
     }
     const size_t xcol = (size_t)min(j0 + 16 + l16, n - 1);
 #pragma unroll
@@ -553,7 +555,7 @@ __device__ __forceinline__ void rank_k_tile(GAS cplx *G, GAS cplx *B, int n, int
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int col = j0 + kq + 4 * r;
-        if (rok && col < n) gst(G + ((size_t)row + (size_t)col * n), make_double2(accr[r], acci[r]));
+        if (rok && col < n) gst_nt(G + ((size_t)row + (size_t)col * n), make_double2(accr[r], acci[r]));
     }
 }
 
@@ -580,8 +582,8 @@ __device__ static void wg_rank_k_strips(cplx *G_, int n, const cplx *A_, const c
         }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            ga[r] = gld(G + (rowc + (size_t)min(kq + 4 * r, n - 1) * n));
-            gb[r] = gld(G + (rowc + (size_t)min(16 + kq + 4 * r, n - 1) * n));
+            ga[r] = gld_nt(G + (rowc + (size_t)min(kq + 4 * r, n - 1) * n));
+            gb[r] = gld_nt(G + (rowc + (size_t)min(16 + kq + 4 * r, n - 1) * n));
         }
         for (int tj = 0; tj < nt; tj += 2) {
             rank_k_tile<KS>(G, B, n, k, row, rok, kq, l16, tj, y, x, ga);
@@ -1841,8 +1843,8 @@ k_update_delay_reg(const DChain *chains, int nt, int slice_in_pass)
             const int nb = min(DSB, n - s);
             // columns s..s+nb are one contiguous run; a row block is read as nb-element runs
             GAS cplx *Gc = G + (size_t)s * n;
-            stage_in_map<4>(gcol, nb * n, [=](int e) { return gld(Gc + (e)); }, [](int e) { return e; });
-            stage_in_map<4>(grow, nb * n, [=](int e) { return gld(G + ((size_t)(s + e % nb) + (size_t)(e / nb) * n)); },
+            stage_in_map<4>(gcol, nb * n, [=](int e) { return gld_nt(Gc + (e)); }, [](int e) { return e; });
+            stage_in_map<4>(grow, nb * n, [=](int e) { return gld_nt(G + ((size_t)(s + e % nb) + (size_t)(e / nb) * n)); },
                             [=](int e) { return (e % nb) * n + e / nb; });
         }
         __syncthreads();

@@ -232,7 +232,11 @@ static void lu_factor(sh_ctx *ctx)
 static void lu_solve(sh_ctx *ctx, int xsel)
 {
     Tic t(ctx, T_SOLVE);
-    if (blocked(ctx))
+    if (ctx->nmax <= 16 * 4 * SCB_T) {
+        // 16 right-hand sides per workgroup, a chain's workgroups on one XCD
+        const int ncb = (ctx->nmax + 15) / 16, nch8 = (ctx->nch + 7) / 8 * 8;
+        hipLaunchKernelGGL(k_solve_cb, dim3(nch8 * ncb), dim3(256), 0, ctx->st, ctx->dc, ctx->nch, ncb, xsel);
+    } else if (blocked(ctx))
         hipLaunchKernelGGL(k_solve_blk, dim3(ctx->nch), dim3(BLK_NT), lds_blk(ctx), ctx->st, ctx->dc, xsel);
     else
         hipLaunchKernelGGL(k_lu_solve, dim3(ctx->nch), dim3(512), 0, ctx->st, ctx->dc, xsel);
@@ -526,6 +530,7 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
         int kb = std::max(q.nublock, 1);
         c.avec = dalloc<cplx>(ctx, (size_t)n * kb); c.bvec = dalloc<cplx>(ctx, (size_t)n * kb);
         c.tau = dalloc<cplx>(ctx, n); c.jpvt = dalloc<int32_t>(ctx, n); c.piv = dalloc<int32_t>(ctx, n);
+        c.lperm = dalloc<int32_t>(ctx, n);
         c.wk = dalloc<cplx>(ctx, (size_t)PB * n);
         c.tblk = dalloc<cplx>(ctx, (size_t)(n / PB + 1) * PB * PB);
         c.rwork = dalloc<double>(ctx, 4 * (size_t)n);

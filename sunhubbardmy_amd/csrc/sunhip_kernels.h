@@ -25,6 +25,20 @@ __device__ __forceinline__ double2 gld(const GAS double2 *p)
     const gdbl2 v = *(const GAS gdbl2 *)p;
     return make_double2(v.x, v.y);
 }
+// streaming variants (nontemporal): for data read or written once per pass, so that it does not
+// displace operands that are re-read from L2
+__device__ __forceinline__ double2 gld_nt(const GAS double2 *p)
+{
+    const gdbl2 v = __builtin_nontemporal_load((const GAS gdbl2 *)p);
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void gst_nt(GAS double2 *p, double2 v)
+{
+    gdbl2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, (GAS gdbl2 *)p);
+}
 __device__ __forceinline__ void gst(GAS double2 *p, double2 v)
 {
     gdbl2 w;
@@ -69,6 +83,7 @@ struct DChain {
     cplx *wk;             // 16 x n work block of the blocked factorizations
     cplx *tblk;           // (n/16 + 1) triangular 16 x 16 factors of the block reflectors
     int32_t *jpvt, *piv;  // n
+    int32_t *lperm;       // n: row order of the LU's interchanges (row i of P X is row lperm[i] of X)
     double *rwork;        // 4n
     int32_t *conf;        // ltrot*n
     double *unif;         // 2*n*ltrot
