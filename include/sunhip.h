@@ -79,6 +79,16 @@ int sh_init_conf(sh_ctx *ctx);
 int sh_set_conf(sh_ctx *ctx, int chain, const int32_t *conf);
 int sh_get_conf(sh_ctx *ctx, int chain, int32_t *conf);
 
+/* The chain's random-number state, SH_RNG_WORDS uint32: the SFMT19937 state that spring_sfmt
+ * keeps in module storage (lib/m_spring.f90:66-71: 624 words, then the read index 0..624) and,
+ * for chains with learned block proposals, the proposal generator's 64-bit state (low word
+ * first; zeros when proposals were never set up, and ignored by the setter then).  Together with
+ * the HS field it resumes a chain bit-exactly; the reference itself restarts from conf_u.in with
+ * a fresh seed (src/dqmc_util.f90:68-75), so this pair has no reference counterpart. */
+#define SH_RNG_WORDS 627
+int sh_get_rng_state(sh_ctx *ctx, int chain, uint32_t *state);
+int sh_set_rng_state(sh_ctx *ctx, int chain, const uint32_t *state);
+
 /* dqmc_ft_sweep_start_0b (src/dqmc_ft_core/dqmc_ft_sweep_start_0b.f90:1-71) */
 int sh_sweep_start_0b(sh_ctx *ctx);
 /* dqmc_ft_sweep_b0(lupdate, lmeasure_equaltime) (src/dqmc_ft_core/dqmc_ft_sweep_b0.f90:1-216) */

@@ -71,6 +71,13 @@ int shp_get_conf(shp_ctx *ctx, int chain, int32_t *conf);
 int shp_set_uconf(shp_ctx *ctx, int chain, const int32_t *conf);
 int shp_get_uconf(shp_ctx *ctx, int chain, int32_t *conf);
 
+/* the chain's SFMT19937 state (lib/m_spring.f90:66-71): 624 words, then the read index 0..624;
+ * with the fields it resumes a chain bit-exactly (no reference counterpart: the reference
+ * restarts from conf_v.in / conf_u.in with a fresh seed) */
+#define SHP_RNG_WORDS 625
+int shp_get_rng_state(shp_ctx *ctx, int chain, uint32_t *state);
+int shp_set_rng_state(shp_ctx *ctx, int chain, const uint32_t *state);
+
 /* the next n numbers of the chain's stream (set_trial_h0 draws them after the field:
  * src/dqmc_config_h0/set_trial_h0.f90:20) */
 int shp_draw_uniforms(shp_ctx *ctx, int chain, int n, double *out);

@@ -30,15 +30,19 @@ EXPORTS = [
     "sh_create", "sh_destroy", "sh_last_error", "sh_init_conf", "sh_set_conf", "sh_get_conf",
     "sh_sweep_start_0b", "sh_sweep_b0", "sh_sweep_0b", "sh_push_stage", "sh_pop_stage",
     "sh_get_g", "sh_get_diag", "sh_reset_diag", "sh_get_obs", "sh_get_obs_all", "sh_reset_obs", "sh_set_trace",
-    "sh_get_trace", "sh_timers", "sh_synchronize",
+    "sh_get_trace", "sh_timers", "sh_synchronize", "sh_get_rng_state", "sh_set_rng_state",
 ]
+
+SH_RNG_WORDS = 627     # include/sunhip.h
+SHP_RNG_WORDS = 625    # include/sunhip_proj.h
 
 # include/sunhip_proj.h
 EXPORTS_PROJ = [
     "shp_create", "shp_destroy", "shp_last_error", "shp_init_conf", "shp_set_conf", "shp_get_conf",
     "shp_draw_uniforms", "shp_set_proj", "shp_sweep_start_0b", "shp_sweep_b0", "shp_sweep_0b", "shp_get_g",
     "shp_get_state", "shp_get_diag", "shp_reset_diag", "shp_get_obs", "shp_reset_obs", "shp_set_trace",
-    "shp_get_trace", "shp_timers", "shp_synchronize", "shp_set_uconf", "shp_get_uconf",
+    "shp_get_trace", "shp_timers", "shp_synchronize", "shp_set_uconf", "shp_get_uconf", "shp_get_rng_state",
+    "shp_set_rng_state",
 ]
 
 
@@ -81,6 +85,9 @@ def lib():
     L.sh_get_trace.argtypes = [P, ctypes.c_int, dp, ip]
     L.sh_timers.argtypes = [P, ctypes.c_int, dp]
     L.sh_synchronize.argtypes = [P]
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    L.sh_get_rng_state.argtypes = [P, ctypes.c_int, u32p]
+    L.sh_set_rng_state.argtypes = [P, ctypes.c_int, u32p]
     for name in EXPORTS:
         if name != "sh_last_error":
             getattr(L, name).restype = ctypes.c_int
@@ -108,6 +115,8 @@ def lib():
     L.shp_get_trace.argtypes = [P, ctypes.c_int, dp, ip]
     L.shp_timers.argtypes = [P, ctypes.c_int, dp]
     L.shp_synchronize.argtypes = [P]
+    L.shp_get_rng_state.argtypes = [P, ctypes.c_int, u32p]
+    L.shp_set_rng_state.argtypes = [P, ctypes.c_int, u32p]
     for name in EXPORTS_PROJ:
         if name != "shp_last_error":
             getattr(L, name).restype = ctypes.c_int

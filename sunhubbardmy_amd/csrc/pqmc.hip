@@ -530,6 +530,23 @@ int shp_get_uconf(shp_ctx *ctx, int chain, int32_t *conf)
     return 0;
 }
 
+int shp_get_rng_state(shp_ctx *ctx, int chain, uint32_t *state)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !state) return SH_E_ARG;
+    HIPCHK(hipMemcpyAsync(state, ctx->hc[chain].rng, 625 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int shp_set_rng_state(shp_ctx *ctx, int chain, const uint32_t *state)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !state) return SH_E_ARG;
+    if (state[624] > 624) return fail(ctx, SH_E_ARG, "SFMT read index out of 0..624");
+    HIPCHK(hipMemcpyAsync(ctx->hc[chain].rng, state, 625 * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
 int shp_draw_uniforms(shp_ctx *ctx, int chain, int count, double *out)
 {
     if (!ctx || chain < 0 || chain >= ctx->nch || count < 0 || (count > 0 && !out)) return SH_E_ARG;

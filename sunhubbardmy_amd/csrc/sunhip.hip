@@ -624,6 +624,29 @@ int sh_get_conf(sh_ctx *ctx, int chain, int32_t *conf)
     return 0;
 }
 
+int sh_get_rng_state(sh_ctx *ctx, int chain, uint32_t *state)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !state) return SH_E_ARG;
+    HIPCHK(hipMemcpyAsync(state, ctx->hc[chain].rng, 625 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->st));
+    uint64_t ps = 0;
+    if (!ctx->hprop.empty()) HIPCHK(hipMemcpyAsync(&ps, ctx->hprop[chain].rng, sizeof(ps), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    state[625] = (uint32_t)ps;
+    state[626] = (uint32_t)(ps >> 32);
+    return 0;
+}
+
+int sh_set_rng_state(sh_ctx *ctx, int chain, const uint32_t *state)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !state) return SH_E_ARG;
+    if (state[624] > 624) return fail(ctx, SH_E_ARG, "SFMT read index out of 0..624");
+    HIPCHK(hipMemcpyAsync(ctx->hc[chain].rng, state, 625 * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->st));
+    const uint64_t ps = (uint64_t)state[625] | ((uint64_t)state[626] << 32);
+    if (!ctx->hprop.empty()) HIPCHK(hipMemcpyAsync(ctx->hprop[chain].rng, &ps, sizeof(ps), hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
 static int first_info(sh_ctx *ctx)
 {
     // the reference stops on a failed LU in b0; report the first chain's pivot index

@@ -67,6 +67,19 @@ module sunhip_bridge
        integer(c_int), value :: chain
        integer(c_int32_t), intent(out) :: conf(*)
      end function sh_get_conf
+     ! the chain's generator state, SH_RNG_WORDS = 627 words (uint32 as int32): bit-exact resume
+     integer(c_int) function sh_get_rng_state(ctx, chain, state) bind(C, name="sh_get_rng_state")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(out) :: state(*)
+     end function sh_get_rng_state
+     integer(c_int) function sh_set_rng_state(ctx, chain, state) bind(C, name="sh_set_rng_state")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(in) :: state(*)
+     end function sh_set_rng_state
      integer(c_int) function sh_sweep_start_0b(ctx) bind(C, name="sh_sweep_start_0b")
        import :: c_ptr, c_int
        type(c_ptr), value :: ctx
@@ -151,6 +164,19 @@ module sunhip_bridge
        integer(c_int), value :: chain
        integer(c_int32_t), intent(out) :: conf(*)
      end function shp_get_uconf
+     ! SHP_RNG_WORDS = 625 words
+     integer(c_int) function shp_get_rng_state(ctx, chain, state) bind(C, name="shp_get_rng_state")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(out) :: state(*)
+     end function shp_get_rng_state
+     integer(c_int) function shp_set_rng_state(ctx, chain, state) bind(C, name="shp_set_rng_state")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(in) :: state(*)
+     end function shp_set_rng_state
      integer(c_int) function shp_draw_uniforms(ctx, chain, n, u) bind(C, name="shp_draw_uniforms")
        import :: c_ptr, c_int, c_double
        type(c_ptr), value :: ctx

@@ -200,6 +200,51 @@ class ProjWalkers:
         c = np.ascontiguousarray(conf, dtype=np.int32).reshape(self.ltrot, self.ndim)
         self._check(_lib.lib().shp_set_uconf(self._ctx, chain, c.ctypes.data_as(_lib.ip)), "set_uconf")
 
+    @property
+    def has_uconf(self):
+        return self.tables[0].get("u") is not None
+
+    @property
+    def has_vconf(self):
+        return self.tables[0].get("v") is not None
+
+    def rng_state(self, chain=0):
+        """The chain's SFMT19937 state (SHP_RNG_WORDS uint32: words, read index)."""
+        s = np.zeros(_lib.SHP_RNG_WORDS, np.uint32)
+        self._check(_lib.lib().shp_get_rng_state(self._ctx, chain, s.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+                    "get_rng_state")
+        return s
+
+    def set_rng_state(self, chain, state):
+        s = np.ascontiguousarray(state, dtype=np.uint32)
+        if s.size != _lib.SHP_RNG_WORDS:
+            raise ValueError("a generator state has %d words" % _lib.SHP_RNG_WORDS)
+        self._check(_lib.lib().shp_set_rng_state(self._ctx, chain, s.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+                    "set_rng_state")
+
+    def output_conf(self, path_v="conf_v.out", path_u="conf_u.out"):
+        """v0conf%output_conf and u0conf%output_conf for the fields the model has."""
+        from .confio import write_conf_u, write_conf_v
+        if self.has_vconf:
+            write_conf_v(path_v, [self.get_conf(c) for c in range(self.nchains)])
+        if self.has_uconf:
+            write_conf_u(path_u, [self.get_uconf(c) for c in range(self.nchains)])
+
+    def input_conf(self, path_v="conf_v.in", path_u="conf_u.in"):
+        """u0conf%input_conf then v0conf%input_conf: fresh fields (drawn by the library in that
+        order) unless every field the model has is in its file.  Returns lwarmup."""
+        from .confio import read_conf_u, read_conf_v
+        lw_u, cu = read_conf_u(path_u, self.nchains, self.ndim, self.ltrot) if self.has_uconf else (False, None)
+        lw_v, cv = read_conf_v(path_v, self.nchains, self.ndim, self.nn_nf, self.ltrot) if self.has_vconf else (False, None)
+        if (self.has_uconf and cu is None) or (self.has_vconf and cv is None):
+            self.init_conf()
+        for c in range(self.nchains):
+            if cu is not None:
+                self.set_uconf(c, cu[c])
+            if cv is not None:
+                self.set_conf(c, cv[c])
+        return lw_u or lw_v
+
     def draw_uniforms(self, chain, count):
         out = np.zeros(count)
         self._check(_lib.lib().shp_draw_uniforms(self._ctx, chain, count, out.ctypes.data_as(_lib.dp)),

@@ -107,6 +107,39 @@ class FTWalkers:
         c = _c(conf, np.int32).reshape(t["ltrot"], t["ndim"])
         self._check(_lib.lib().sh_set_conf(self._ctx, chain, c.ctypes.data_as(_lib.ip)), "set_conf")
 
+    def rng_state(self, chain=0):
+        """The chain's generator state (SH_RNG_WORDS uint32: SFMT19937 words, read index, block
+        proposal state)."""
+        s = np.zeros(_lib.SH_RNG_WORDS, np.uint32)
+        self._check(_lib.lib().sh_get_rng_state(self._ctx, chain, s.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+                    "get_rng_state")
+        return s
+
+    def set_rng_state(self, chain, state):
+        s = _c(state, np.uint32)
+        if s.size != _lib.SH_RNG_WORDS:
+            raise ValueError("a generator state has %d words" % _lib.SH_RNG_WORDS)
+        self._check(_lib.lib().sh_set_rng_state(self._ctx, chain, s.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+                    "set_rng_state")
+
+    def output_conf(self, path="conf_u.out"):
+        """u0conf%output_conf: every chain's field to `path` in the reference's text format."""
+        from .confio import write_conf_u
+        write_conf_u(path, [self.get_conf(c) for c in range(self.nchains)])
+
+    def input_conf(self, path="conf_u.in"):
+        """u0conf%input_conf: the fields from `path`, or fresh ones when it is missing or starts
+        with 0.  Returns lwarmup (True for a fresh start)."""
+        from .confio import read_conf_u
+        t = self.tables[0]
+        lw, confs = read_conf_u(path, self.nchains, t["ndim"], t["ltrot"])
+        if confs is None:
+            self.init_conf()
+        else:
+            for c, conf in enumerate(confs):
+                self.set_conf(conf, c)
+        return lw
+
     # -- sweeps ------------------------------------------------------------------------------
     def sweep_start_0b(self):
         self._check(_lib.lib().sh_sweep_start_0b(self._ctx), "sweep_start_0b")
