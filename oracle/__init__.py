@@ -46,6 +46,19 @@ def lib():
         L.orc_get_obs.argtypes = [P, dp]
         L.orc_set_trace.argtypes = [P, dp, ctypes.c_int]
         L.orc_trace_count.argtypes = [P]
+        L.orc_set_dyn.argtypes = [P, ctypes.c_int]
+        L.orc_sweep_0b_dyn.argtypes = [P, ctypes.c_int]
+        L.orc_get_gtau.argtypes = [P, dp, dp, dp]
+        L.orc_get_gtau.restype = None
+        L.orc_get_znn.argtypes = [P, dp]
+        L.orc_get_znn.restype = None
+        L.orc_reset_znn.argtypes = [P]
+        L.orc_reset_znn.restype = None
+        L.orc_get_dyn_error.argtypes = [P, dp]
+        L.orc_get_dyn_error.restype = None
+        L.orc_set_dyn_trace.argtypes = [P, dp, ctypes.c_int]
+        L.orc_set_dyn_trace.restype = None
+        L.orc_dyn_trace_count.argtypes = [P]
         L.orc_get_table.argtypes = [P, ctypes.c_int, dp]
         L.orc_sfmt_u32.argtypes = [ctypes.c_int32, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
         L.orc_sfmt_stream.argtypes = [ctypes.c_int32, ctypes.c_int, dp]
@@ -217,6 +230,47 @@ class OracleChain:
         n = lib().orc_trace_count(self._c)
         # snapshots are stored column-major per matrix
         return np.array([t.reshape(self.ndim, self.ndim).T for t in self._trace[:n]])
+
+    # -- time-displaced measurement (ltau) ---------------------------------------------------
+    def set_dyn(self, ntdm=None):
+        """Enable G(t,0), G(0,t) and the znn_tau bins for t = 0 .. ntdm slices (default ltrot)."""
+        self.ntdm = self.ltrot if ntdm is None else int(ntdm)
+        if lib().orc_set_dyn(self._c, self.ntdm):
+            raise ValueError("ntdm must be in 0..ltrot")
+
+    def sweep_0b_dyn(self, lmeasure=False):
+        rc = lib().orc_sweep_0b_dyn(self._c, int(lmeasure))
+        if rc:
+            raise RuntimeError("oracle sweep_0b_dyn failed: %d" % rc)
+
+    def gtau(self):
+        """(G(t,0), G(0,t), G(0,0)) at the end of the last dynamic pass"""
+        m = [np.zeros((self.ndim, self.ndim), np.complex128, order="F") for _ in range(3)]
+        lib().orc_get_gtau(self._c, _dptr(m[0]), _dptr(m[1]), _dptr(m[2]))
+        return tuple(m)
+
+    def znn(self):
+        """znn_tau bins [ntdm+1][orbital j][orbital i][lq] (the Fortran znn_tau(lq, 2, 2, ntdm+1))"""
+        z = np.zeros((self.ntdm + 1, 2, 2, self.lq), np.complex128)
+        lib().orc_get_znn(self._c, _dptr(z))
+        return z
+
+    def reset_znn(self):
+        lib().orc_reset_znn(self._c)
+
+    def dyn_error(self):
+        e = np.zeros(3)
+        lib().orc_get_dyn_error(self._c, _dptr(e))
+        return {"xmax_dyn": e[0], "count": e[1], "sumlog10": e[2]}
+
+    def set_dyn_trace(self, cap):
+        self._dtrace = np.zeros((cap, 4, self.ndim, self.ndim), np.complex128)
+        lib().orc_set_dyn_trace(self._c, _dptr(self._dtrace), cap)
+
+    def dyn_trace(self):
+        """per slice t = 0, 1, ...: (G(t,0), G(0,t), G(t,t), G(0,0)) as [n, 4, N, N]"""
+        n = lib().orc_dyn_trace_count(self._c)
+        return np.array([[m.reshape(self.ndim, self.ndim).T for m in t] for t in self._dtrace[:n]])
 
     def set_proposal(self, nbr, nprop, weights, norm, local_updates=True):
         """Block proposals restated from the HIP library (weights=None switches them off)."""

@@ -456,6 +456,12 @@ struct orc_ctx {
     double prop_norm[10];
     uint64_t prop_rng;
     double prop_stats[5];
+    /* time-displaced measurement (ltau; see orc_set_dyn) */
+    int ltau, ntdm;
+    zc *gt0, *g0t, *g00, *znn;
+    double xmax_dyn, dyncount, sumlog10dyn;
+    double *dtrace;
+    int dtrace_cap, dtrace_n;
 };
 
 /* src/lattice.f90:792-961 (square) and :611-776 (honeycomb); 0-based site indices */
@@ -739,6 +745,7 @@ void orc_destroy(orc_ctx *c)
     free(c->nnlf); free(c->nnlist); free(c->cx); free(c->cy); free(c->sub);
     free(c->iwrap_nt); free(c->wrap_step);
     free(c->prop_nbr); free(c->prop_w);
+    free(c->gt0); free(c->g0t); free(c->g00); free(c->znn);
     free(c);
 }
 
@@ -1544,6 +1551,220 @@ int orc_sweep_0b(orc_ctx *c, int lupdate, int lmeasure)
 }
 
 void orc_get_g(const orc_ctx *c, double *g) { memcpy(g, c->G, sizeof(zc) * (size_t)c->ndim * c->ndim); }
+
+/* ------------------------------------------------------------------------------------ */
+/* time-displaced Green's functions (ltau)                                                */
+/* ------------------------------------------------------------------------------------ */
+/* ltau with ntdm = nint(dyntau/dtau) (src/model_para.f90:457): gt0, g0t, g00 and the bins
+ * znn_tau(lq, 2, 2, ntdm+1) (src/dqmc_measure_2d.f90:30, :61) */
+int orc_set_dyn(orc_ctx *c, int ntdm)
+{
+    if (ntdm < 0 || ntdm > c->ltrot) return -1;
+    size_t nn = (size_t)c->ndim * c->ndim;
+    free(c->gt0); free(c->g0t); free(c->g00); free(c->znn);
+    c->gt0 = zalloc(nn); c->g0t = zalloc(nn); c->g00 = zalloc(nn);
+    c->znn = zalloc((size_t)c->lq * 4 * (ntdm + 1));
+    c->ltau = 1;
+    c->ntdm = ntdm;
+    c->xmax_dyn = 0; c->dyncount = 0; c->sumlog10dyn = 0;
+    return 0;
+}
+
+/* green_tau (src/dqmc_ft_core/green_tau.f90:1-122): G(t,t), G(t,0), G(0,t), G(0,0) from the right
+ * chain (ur, dr, vr) and the left chain (vl, dl, ul) at the same point; s_v_invd_u(V, d, U) =
+ * V d^-1 U, s_v_d_u(V, d, U) = V d U and the uH forms with U^H (lib/s_matrix.f90:597-720) */
+static int green_tau(orc_ctx *c, const zc *ur, const double *dr, const zc *vr_in, const zc *vl, const double *dl,
+                     const zc *ul, zc *g00, zc *gt0, zc *g0t, zc *gtt, zc ldqr, zc ldql, zc *logw)
+{
+    int N = c->ndim;
+    size_t nn = (size_t)N * N;
+    double *drmax = malloc(sizeof(double) * N), *drmin = malloc(sizeof(double) * N);
+    double *dlmax = malloc(sizeof(double) * N), *dlmin = malloc(sizeof(double) * N);
+    dvec_min_max(N, dr, drmax, drmin);
+    dvec_min_max(N, dl, dlmax, dlmin);
+    zc *vlh = zalloc(nn), *vr = zalloc(nn), *uu = zalloc(nn), *vv = zalloc(nn), *M = zalloc(nn), *B = zalloc(nn), *T = zalloc(nn);
+    memcpy(vr, vr_in, sizeof(zc) * nn);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) vlh[IDX(i, j, N)] = conj(vl[IDX(j, i, N)]);
+    zgemm('C', 'N', N, N, N, ur, N, ul, N, 0, uu, N);
+    zgemm('N', 'C', N, N, N, vr, N, vl, N, 0, vv, N);
+    /* G(t,t) */
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++)
+            M[IDX(i, j, N)] = uu[IDX(i, j, N)] / (drmax[i] * dlmax[j]) + vv[IDX(i, j, N)] * drmin[i] * dlmin[j];
+    zc zld;
+    int info = zinv_logdet(N, M, &zld);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) T[IDX(i, j, N)] = ul[IDX(i, j, N)] / dlmax[j];
+    zgemm('N', 'N', N, N, N, T, N, M, N, 0, B, N);                      /* B = ul dlmax^-1 M^-1 */
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) T[IDX(i, j, N)] = B[IDX(i, j, N)] / drmax[j];
+    zgemm('N', 'C', N, N, N, T, N, ur, N, 0, gtt, N);
+    double r1 = 0, r2 = 0;
+    for (int i = 0; i < N; i++) r1 += log(dlmax[i]);
+    for (int i = 0; i < N; i++) r2 += log(drmax[i]);
+    *logw = -ldql + ldqr + zld + (r1 + r2);
+    /* G(t,0) = B drmin vr */
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) T[IDX(i, j, N)] = B[IDX(i, j, N)] * drmin[j];
+    zgemm('N', 'N', N, N, N, T, N, vr, N, 0, gt0, N);
+    /* vr <- vr^-1, vlh <- (vl^H)^-1, vv = vlh vr */
+    zc dummy;
+    int info2 = zinv_logdet(N, vr, &dummy);
+    int info3 = zinv_logdet(N, vlh, &dummy);
+    zgemm('N', 'N', N, N, N, vlh, N, vr, N, 0, vv, N);
+    /* G(0,0) */
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++)
+            M[IDX(i, j, N)] = vv[IDX(i, j, N)] / (dlmax[i] * drmax[j]) + conj(uu[IDX(j, i, N)]) * dlmin[i] * drmin[j];
+    int info4 = zinv_logdet(N, M, &dummy);
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) T[IDX(i, j, N)] = vr[IDX(i, j, N)] / drmax[j];
+    zgemm('N', 'N', N, N, N, T, N, M, N, 0, B, N);                      /* B = vr^-1 drmax^-1 M^-1 */
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) T[IDX(i, j, N)] = B[IDX(i, j, N)] / dlmax[j];
+    zgemm('N', 'N', N, N, N, T, N, vlh, N, 0, g00, N);
+    /* G(0,t) = B (-dlmin) ul^H */
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) T[IDX(i, j, N)] = -B[IDX(i, j, N)] * dlmin[j];
+    zgemm('N', 'C', N, N, N, T, N, ul, N, 0, g0t, N);
+    free(drmax); free(drmin); free(dlmax); free(dlmin);
+    free(vlh); free(vr); free(uu); free(vv); free(M); free(B); free(T);
+    return info ? info : (info2 ? info2 : (info3 ? info3 : info4));
+}
+
+/* dyn_measure (src/dqmc_measure/dyn_measure.f90:1-101), the active znn_tau term: bin nt (1-based)
+ * += ((1/2 - Re gtt(i,i)) (1/2 - Re g00(j,j)) - g0t(j,i) gt0(i,j) / nflr) zphi, summed into
+ * (imj, orbital of i, orbital of j); imj = npbc(yi - yj) + (npbc(xi - xj) - 1) l2
+ * (src/lattice.f90:471-479) */
+static void dyn_measure(orc_ctx *c, int nt, const zc *gtt)
+{
+    int N = c->ndim, l1 = c->p.la, l2 = c->p.lb, lq = c->lq;
+    double sg = creal(c->phase) < 0 ? -1.0 : 1.0;
+    zc zphi = (sg / creal(c->phase)) * c->phase;
+    zc *bin = c->znn + (size_t)lq * 4 * (nt - 1);
+#define NPBC(v, l) ((v) > (l) ? (v) - (l) : ((v) < 1 ? (v) + (l) : (v)))
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+            int imj = NPBC(c->cy[i] - c->cy[j], l2) + (NPBC(c->cx[i] - c->cx[j], l1) - 1) * l2;
+            zc v = (0.5 - creal(gtt[IDX(i, i, N)])) * (0.5 - creal(c->g00[IDX(j, j, N)]))
+                 - c->g0t[IDX(j, i, N)] * c->gt0[IDX(i, j, N)] / (double)c->p.nflr;
+            bin[(imj - 1) + (size_t)lq * (c->sub[i] + 2 * c->sub[j])] += v * zphi;
+        }
+#undef NPBC
+}
+
+static void dyn_trace(orc_ctx *c)
+{
+    int N = c->ndim;
+    size_t nn = (size_t)N * N;
+    if (!c->dtrace || c->dtrace_n >= c->dtrace_cap) return;
+    double *d = c->dtrace + (size_t)2 * 4 * nn * c->dtrace_n;
+    memcpy(d, c->gt0, sizeof(zc) * nn);
+    memcpy(d + 2 * nn, c->g0t, sizeof(zc) * nn);
+    memcpy(d + 4 * nn, c->G, sizeof(zc) * nn);
+    memcpy(d + 6 * nn, c->g00, sizeof(zc) * nn);
+    c->dtrace_n++;
+}
+
+/* M <- M B(nt)^-1, the right-hand half of the 0b wrap (Bmat_right_backward, one slice) */
+static void bmat_right_backward_slice(orc_ctx *c, int nt, zc *M)
+{
+    int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
+    if (c->lwrapT) for (int q = 0; q < nfam; q++) k_right(c, q, c->urtm1, M);
+    v_cols(c, nt, c->bmat_inv, M);
+    if (c->lwrapT) for (int q = nfam - 1; q >= 0; q--) k_right(c, q, c->urtm1, M);
+}
+
+static double max_abs_diff(size_t n, const zc *a, const zc *b)
+{
+    double mx = 0;
+    for (size_t k = 0; k < n; k++) { double d = cabs(a[k] - b[k]); if (d > mx) mx = d; }
+    return mx;
+}
+
+/* dqmc_ft_sweep_0b(lupdate = .false., lmeasure_equaltime, lmeasure_dyn = .true.) with ltau
+ * (src/dqmc_ft_core/dqmc_ft_sweep_0b.f90:36-49, :160-204 with -DDYNERROR, dqmc_ft_core/dyn.f90):
+ * the field is not updated; G(t,0), G(0,t) follow B(t) slice by slice and are replaced by
+ * green_tau at every stabilization point; bins 1 .. ntdm+1 are measured (the reference writes
+ * bin nt+1 for every slice, past the array when ntdm < ltrot) */
+int orc_sweep_0b_dyn(orc_ctx *c, int lmeasure)
+{
+    int N = c->ndim;
+    size_t nn = (size_t)N * N;
+    if (c->nst <= 0 || !c->ltau) return -1;
+    zc *gs = zalloc(nn), *gt0s = zalloc(nn), *g0ts = zalloc(nn);
+    set_identity_point(c, 0);
+    memcpy(c->g00, c->G, sizeof(zc) * nn);
+    memcpy(c->gt0, c->G, sizeof(zc) * nn);
+    memcpy(c->g0t, c->G, sizeof(zc) * nn);
+    for (int i = 0; i < N; i++) c->g0t[IDX(i, i, N)] -= 1;
+    dyn_measure(c, 1, c->G);
+    dyn_trace(c);
+    int rc = 0;
+    for (int nt = 1; nt <= c->ltrot; nt++) {
+        if (c->lwrapT) {
+            int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
+            for (int q = 0; q < nfam; q++) { k_left(c, q, c->urt, c->G); k_right(c, q, c->urtm1, c->G); }
+        }
+        if (c->lwrapu) {
+            v_rows(c, nt, c->bmat, 0, c->G);
+            v_cols(c, nt, c->bmat_inv, c->G);
+        }
+        if (c->lwrapT) {
+            int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
+            for (int q = nfam - 1; q >= 0; q--) { k_left(c, q, c->urt, c->G); k_right(c, q, c->urtm1, c->G); }
+        }
+        if (lmeasure) measure(c);
+        if (c->iwrap_nt[nt] > 0) {
+            int n = c->iwrap_nt[nt];
+            zc *UL = zalloc(nn), *VL = zalloc(nn);
+            double *DL = (double *)malloc(sizeof(double) * N);
+            memcpy(UL, c->Ust[n], sizeof(zc) * nn);
+            memcpy(VL, c->Vst[n], sizeof(zc) * nn);
+            memcpy(DL, c->Dst[n], sizeof(double) * N);
+            zc ldql = c->logdetQst[n];
+            stablize_0b(c, n);
+            /* -DDYNERROR: the propagated G(t,0), G(0,t) against green_tau's */
+            bmat_left_forward(c, nt, nt, c->gt0);
+            bmat_right_backward_slice(c, nt, c->g0t);
+            zc lw;
+            int info = green_tau(c, c->Ust[n], c->Dst[n], c->Vst[n], VL, DL, UL, c->g00, gt0s, g0ts, gs,
+                                 c->logdetQst[n], ldql, &lw);
+            double e1 = max_abs_diff(nn, c->gt0, gt0s), e2 = max_abs_diff(nn, c->g0t, g0ts);
+            if (e1 > c->xmax_dyn) c->xmax_dyn = e1;
+            if (e2 > c->xmax_dyn) c->xmax_dyn = e2;
+            c->dyncount += 2;
+            c->sumlog10dyn += log10(e1 > 0 ? e1 : 1e-300) + log10(e2 > 0 ? e2 : 1e-300);
+            memcpy(c->gt0, gt0s, sizeof(zc) * nn);
+            memcpy(c->g0t, g0ts, sizeof(zc) * nn);
+            record_wrap(c, gs, lw, info);
+            free(UL); free(VL); free(DL);
+            if (info != 0) { rc = info; break; }
+        } else {
+            bmat_left_forward(c, nt, nt, c->gt0);
+            bmat_right_backward_slice(c, nt, c->g0t);
+        }
+        if (nt + 1 <= c->ntdm + 1) dyn_measure(c, nt + 1, c->G);
+        dyn_trace(c);
+    }
+    free(gs); free(gt0s); free(g0ts);
+    return rc;
+}
+
+void orc_get_gtau(const orc_ctx *c, double *gt0, double *g0t, double *g00)
+{
+    size_t nn = (size_t)c->ndim * c->ndim;
+    memcpy(gt0, c->gt0, sizeof(zc) * nn);
+    memcpy(g0t, c->g0t, sizeof(zc) * nn);
+    memcpy(g00, c->g00, sizeof(zc) * nn);
+}
+void orc_get_znn(const orc_ctx *c, double *znn) { memcpy(znn, c->znn, sizeof(zc) * (size_t)c->lq * 4 * (c->ntdm + 1)); }
+void orc_reset_znn(orc_ctx *c) { memset(c->znn, 0, sizeof(zc) * (size_t)c->lq * 4 * (c->ntdm + 1)); }
+void orc_get_dyn_error(const orc_ctx *c, double *out) { out[0] = c->xmax_dyn; out[1] = c->dyncount; out[2] = c->sumlog10dyn; }
+/* per slice tau = 0 .. ltrot: gt0, g0t, gtt, g00 (4 N x N complex) into buf (cap entries) */
+void orc_set_dyn_trace(orc_ctx *c, double *buf, int cap) { c->dtrace = buf; c->dtrace_cap = cap; c->dtrace_n = 0; }
+int orc_dyn_trace_count(const orc_ctx *c) { return c->dtrace_n; }
 
 void orc_get_diag(const orc_ctx *c, double *d)
 {

@@ -68,6 +68,15 @@ void orc_get_proposal_stats(const orc_ctx *c, double *stats);
 
 /* per-stabilization trace of G (after replacement by the scratch G): cap snapshots */
 void orc_set_trace(orc_ctx *c, double *buf, int cap);
+/* time-displaced measurement (ltau): green_tau, the 0b pass without updates, znn_tau bins */
+int orc_set_dyn(orc_ctx *c, int ntdm);
+int orc_sweep_0b_dyn(orc_ctx *c, int lmeasure);
+void orc_get_gtau(const orc_ctx *c, double *gt0, double *g0t, double *g00);
+void orc_get_znn(const orc_ctx *c, double *znn);
+void orc_reset_znn(orc_ctx *c);
+void orc_get_dyn_error(const orc_ctx *c, double *out);
+void orc_set_dyn_trace(orc_ctx *c, double *buf, int cap);
+int orc_dyn_trace_count(const orc_ctx *c);
 int orc_trace_count(const orc_ctx *c);
 
 /* tables, for cross-checking the product's host-side setup:
