@@ -79,6 +79,24 @@ int sh_init_conf(sh_ctx *ctx);
 int sh_set_conf(sh_ctx *ctx, int chain, const int32_t *conf);
 int sh_get_conf(sh_ctx *ctx, int chain, int32_t *conf);
 
+/* Time-displaced measurement (ltau, dyntau = ntdm * dtau: src/model_para.f90:457).  sh_set_dyn
+ * gives a chain the unit-cell coordinates of its sites, [ndim][3] int32: x in 1..l1, y in 1..l2,
+ * orbital 0 or 1 (latt%list(i,1:2), src/lattice.f90:386-491, 611-776) and switches on G(t,0),
+ * G(0,t), G(0,0) and the znn_tau bins; every chain of a batch takes the same ntdm (<= ltrot).
+ * sh_sweep_0b(ctx, 0, lmeasure, 1) is then the reference's dynamic pass (dqmc_ft_ctrl.f90:99-101
+ * between sh_push_stage and sh_pop_stage): no updates, green_tau at every stabilization point
+ * (src/dqmc_ft_core/green_tau.f90:1-122), znn_tau(imj, no_i, no_j, t) += ... for t = 0 .. ntdm
+ * (src/dqmc_measure/dyn_measure.f90:51-64; the reference writes past its array when ntdm < ltrot,
+ * here the slices past ntdm are not measured). */
+int sh_set_dyn(sh_ctx *ctx, int chain, int ntdm, int l1, int l2, const int32_t *sites);
+/* znn_tau_bin as complex [ntdm+1][2][2][lq] (the Fortran znn_tau(lq, 2, 2, ntdm+1)); dynerr (may be
+ * NULL): the largest |propagated - green_tau| of G(t,0) and G(0,t) and the number of such
+ * comparisons (xmax_dyn, logdyncount of -DDYNERROR) */
+int sh_get_dyn_obs(sh_ctx *ctx, int chain, double *znn, double *dynerr);
+int sh_reset_dyn_obs(sh_ctx *ctx);
+/* G(t,0), G(0,t), G(0,0) (N x N complex, column-major) as the last dynamic pass left them */
+int sh_get_gtau(sh_ctx *ctx, int chain, double *gt0, double *g0t, double *g00);
+
 /* The chain's random-number state, SH_RNG_WORDS uint32: the SFMT19937 state that spring_sfmt
  * keeps in module storage (lib/m_spring.f90:66-71: 624 words, then the read index 0..624) and,
  * for chains with learned block proposals, the proposal generator's 64-bit state (low word
@@ -94,7 +112,7 @@ int sh_sweep_start_0b(sh_ctx *ctx);
 /* dqmc_ft_sweep_b0(lupdate, lmeasure_equaltime) (src/dqmc_ft_core/dqmc_ft_sweep_b0.f90:1-216) */
 int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure);
 /* dqmc_ft_sweep_0b(lupdate, lmeasure_equaltime, lmeasure_dyn)
- * (src/dqmc_ft_core/dqmc_ft_sweep_0b.f90:1-288); lmeasure_dyn must be 0 */
+ * (src/dqmc_ft_core/dqmc_ft_sweep_0b.f90:1-288); lmeasure_dyn = 1 needs sh_set_dyn and lupdate = 0 */
 int sh_sweep_0b(sh_ctx *ctx, int lupdate, int lmeasure, int lmeasure_dyn);
 /* push_stage / pop_stage (src/dqmc_ft_core.f90:290-312) */
 int sh_push_stage(sh_ctx *ctx);

@@ -30,7 +30,8 @@ EXPORTS = [
     "sh_create", "sh_destroy", "sh_last_error", "sh_init_conf", "sh_set_conf", "sh_get_conf",
     "sh_sweep_start_0b", "sh_sweep_b0", "sh_sweep_0b", "sh_push_stage", "sh_pop_stage",
     "sh_get_g", "sh_get_diag", "sh_reset_diag", "sh_get_obs", "sh_get_obs_all", "sh_reset_obs", "sh_set_trace",
-    "sh_get_trace", "sh_timers", "sh_synchronize", "sh_get_rng_state", "sh_set_rng_state",
+    "sh_get_trace", "sh_timers", "sh_synchronize", "sh_get_rng_state", "sh_set_rng_state", "sh_set_dyn",
+    "sh_get_dyn_obs", "sh_reset_dyn_obs", "sh_get_gtau",
 ]
 
 SH_RNG_WORDS = 627     # include/sunhip.h
@@ -88,6 +89,10 @@ def lib():
     u32p = ctypes.POINTER(ctypes.c_uint32)
     L.sh_get_rng_state.argtypes = [P, ctypes.c_int, u32p]
     L.sh_set_rng_state.argtypes = [P, ctypes.c_int, u32p]
+    L.sh_set_dyn.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]
+    L.sh_get_dyn_obs.argtypes = [P, ctypes.c_int, dp, dp]
+    L.sh_reset_dyn_obs.argtypes = [P]
+    L.sh_get_gtau.argtypes = [P, ctypes.c_int, dp, dp, dp]
     for name in EXPORTS:
         if name != "sh_last_error":
             getattr(L, name).restype = ctypes.c_int

@@ -5,6 +5,7 @@
 #include "blocked.hip"
 #include "pqmc_kernels.hip"
 #include "learn_kernels.hip"
+#include "dyn_kernels.hip"
 #include "../../include/sunhip.h"
 #include "../../include/sunhip_learn.h"
 
@@ -62,6 +63,7 @@ struct sh_ctx : ctx_base {
     int32_t *d_seeds = nullptr;
     bool started = false;
     bool legacy_update = false;   // test hook: the L2-resident delay update kernel
+    int ntdm = -1;                // time-displaced measurement: bins 0 .. ntdm, -1 = off
     // push/pop_stage backups
     std::vector<void *> stage;
     std::vector<cplx *> trace_bufs;
@@ -200,19 +202,19 @@ static void left_pass(sh_ctx *ctx, int sel, int pt, int ksel, int prog, int vsel
     }
 }
 
-// right product (columns mixed) on G
-static void right_pass(sh_ctx *ctx, int ksel, int prog, int vsel, int nt)
+// right product (columns mixed) on matrix sel (G, or G(0,t) of the time-displaced pass)
+static void right_pass(sh_ctx *ctx, int ksel, int prog, int vsel, int nt, int sel = M_G)
 {
     if (ctx->kmode == SH_K_CHECKERBOARD) {
         hipLaunchKernelGGL(k_cb_rows, dim3((ctx->nmax + RPB - 1) / RPB, 1, ctx->nch), dim3(256), lds_rows(ctx),
-                           ctx->st, ctx->dc, M_G, 0, ksel, prog, vsel, nt);
+                           ctx->st, ctx->dc, sel, 0, ksel, prog, vsel, nt);
         return;
     }
     const int kmat = ksel ? M_KTM1 : M_KT;
-    if (prog & 8) hipLaunchKernelGGL(k_vscale, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, M_G, 0, 1, vsel, 0, nt);
-    if (prog & 1) { gemm<0, 0>(ctx, M_G, 0, kmat, 0, M_W2, 0); copy_mat(ctx, M_G, 0, M_W2, 0); }
-    if (prog & 2) hipLaunchKernelGGL(k_vscale, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, M_G, 0, 1, vsel, 0, nt);
-    if (prog & 4) { gemm<0, 0>(ctx, M_G, 0, kmat, 0, M_W2, 0); copy_mat(ctx, M_G, 0, M_W2, 0); }
+    if (prog & 8) hipLaunchKernelGGL(k_vscale, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, sel, 0, 1, vsel, 0, nt);
+    if (prog & 1) { gemm<0, 0>(ctx, sel, 0, kmat, 0, M_W2, 0); copy_mat(ctx, sel, 0, M_W2, 0); }
+    if (prog & 2) hipLaunchKernelGGL(k_vscale, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, sel, 0, 1, vsel, 0, nt);
+    if (prog & 4) { gemm<0, 0>(ctx, sel, 0, kmat, 0, M_W2, 0); copy_mat(ctx, sel, 0, M_W2, 0); }
 }
 
 // ---- stabilization -------------------------------------------------------------------------
@@ -354,6 +356,81 @@ static void green(sh_ctx *ctx, int rsaved, int rpt, int lsaved, int lpt)
     Tic t(ctx, T_OTHER);
     hipLaunchKernelGGL(k_green_logw, dim3(ctx->nch), dim3(256), 0, ctx->st, ctx->dc, 0, rsaved, rpt, lsaved, lpt,
                        (int)SC_LOGW_TMP);
+}
+
+// green_tau (src/dqmc_ft_core/green_tau.f90:1-122) at right point rpt against the saved left
+// point: G(t,t) into Gs with the scratch weight (as green), G(t,0) and G(0,t) into the scratch
+// gt0s / g0ts, G(0,0) into g00.  The explicit inverses of the reference become LU solves:
+// G(t,0) = (UL dlmax^-1) M^-1 (drmin VR); with A = VR^-1 drmax^-1 and M2 the G(0,0) matrix,
+// G(0,0) = A M2^-1 (dlmax^-1 (VL^H)^-1) and G(0,t) = A M2^-1 (-dlmin UL^H).
+static void green_tau(sh_ctx *ctx, int rpt)
+{
+    auto elem = [&](int op, int dsel) {
+        hipLaunchKernelGGL(k_dyn_elem, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, op, dsel, rpt);
+    };
+    green(ctx, 0, rpt, 1, 0);          // Gs = G(t,t), W1 = UL dlmax^-1, W3 = LU of M, SC_LOGW_TMP
+    {
+        Tic t(ctx, T_OTHER);
+        hipLaunchKernelGGL(k_dyn_info, dim3((ctx->nch + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 0);
+        elem(2, M_W4);                 // W4 = drmin VR
+    }
+    lu_solve(ctx, M_W4);
+    {
+        Tic t(ctx, T_GEMM);
+        gemm<0, 0>(ctx, M_W1, 0, M_W4, 0, M_GT0S, 0);
+    }
+    // VR^-1 into W4, (VL^H)^-1 into W2
+    {
+        Tic t(ctx, T_OTHER);
+        copy_mat(ctx, M_W3, 0, M_V, rpt);
+    }
+    lu_factor(ctx);
+    {
+        Tic t(ctx, T_OTHER);
+        elem(1, M_W4);
+    }
+    lu_solve(ctx, M_W4);
+    {
+        Tic t(ctx, T_OTHER);
+        elem(3, M_W3);
+    }
+    lu_factor(ctx);
+    {
+        Tic t(ctx, T_OTHER);
+        elem(1, M_W2);
+    }
+    lu_solve(ctx, M_W2);
+    {
+        Tic t(ctx, T_GEMM);
+        gemm<0, 0>(ctx, M_W2, 0, M_W4, 0, M_W1, 0);     // (VL^H)^-1 VR^-1
+        gemm<1, 0>(ctx, M_US, 0, M_U, rpt, M_W3, 0);     // UL^H UR
+    }
+    {
+        Tic t(ctx, T_OTHER);
+        elem(4, M_W3);                                   // M2
+    }
+    lu_factor(ctx);
+    {
+        Tic t(ctx, T_OTHER);
+        elem(5, M_W2);                                   // dlmax^-1 (VL^H)^-1
+        elem(6, M_W4);                                   // A = VR^-1 drmax^-1
+    }
+    lu_solve(ctx, M_W2);
+    {
+        Tic t(ctx, T_GEMM);
+        gemm<0, 0>(ctx, M_W4, 0, M_W2, 0, M_G00, 0);
+    }
+    {
+        Tic t(ctx, T_OTHER);
+        elem(7, M_W2);                                   // -dlmin UL^H
+    }
+    lu_solve(ctx, M_W2);
+    {
+        Tic t(ctx, T_GEMM);
+        gemm<0, 0>(ctx, M_W4, 0, M_W2, 0, M_G0TS, 0);
+    }
+    Tic t(ctx, T_OTHER);
+    hipLaunchKernelGGL(k_dyn_info, dim3((ctx->nch + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
 }
 
 // green_equaltimebb: G(beta,beta) from the right chain point nst
@@ -624,6 +701,74 @@ int sh_get_conf(sh_ctx *ctx, int chain, int32_t *conf)
     return 0;
 }
 
+int sh_set_dyn(sh_ctx *ctx, int chain, int ntdm, int l1, int l2, const int32_t *sites)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !sites || ntdm < 0 || ntdm > ctx->ltrot || l1 < 1 || l2 < 1)
+        return SH_E_ARG;
+    if (ctx->ntdm >= 0 && ntdm != ctx->ntdm) return fail(ctx, SH_E_ARG, "all chains of a batch share ntdm");
+    DChain &c = ctx->hc[chain];
+    if (l1 * l2 != c.lq) return fail(ctx, SH_E_ARG, "l1 * l2 must be the number of unit cells");
+    for (int i = 0; i < c.n; i++)
+        if (sites[3 * i] < 1 || sites[3 * i] > l1 || sites[3 * i + 1] < 1 || sites[3 * i + 1] > l2 ||
+            sites[3 * i + 2] < 0 || sites[3 * i + 2] > 1)
+            return fail(ctx, SH_E_ARG, "site cell coordinates out of 1..l1, 1..l2 or orbital out of 0..1");
+    if (c.lq > 512) return fail(ctx, SH_E_ARG, "more unit cells than the znn_tau bins in LDS hold (512)");
+    const size_t nn = (size_t)c.n * c.n;
+    if (!c.ltau) {
+        c.gt0 = dalloc<cplx>(ctx, nn); c.g0t = dalloc<cplx>(ctx, nn); c.g00 = dalloc<cplx>(ctx, nn);
+        c.gt0s = dalloc<cplx>(ctx, nn); c.g0ts = dalloc<cplx>(ctx, nn);
+        c.znn = dalloc<cplx>(ctx, (size_t)4 * c.lq * (ctx->ltrot + 1));
+        c.dynerr = dalloc<double>(ctx, 2);
+        c.sxy = dalloc<int32_t>(ctx, (size_t)3 * c.n);
+        if (!c.gt0 || !c.g0t || !c.g00 || !c.gt0s || !c.g0ts || !c.znn || !c.dynerr || !c.sxy)
+            return fail(ctx, SH_E_HIP, "device allocation failed");
+    }
+    HIPCHK(hipMemcpy((void *)c.sxy, sites, sizeof(int32_t) * 3 * c.n, hipMemcpyHostToDevice));
+    c.dl1 = l1;
+    c.dl2 = l2;
+    c.ntdm = ntdm;
+    c.ltau = 1;
+    ctx->ntdm = ntdm;
+    HIPCHK(hipMemcpy(ctx->dc + chain, &c, sizeof(DChain), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int sh_get_dyn_obs(sh_ctx *ctx, int chain, double *znn, double *dynerr)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !znn) return SH_E_ARG;
+    const DChain &c = ctx->hc[chain];
+    if (!c.ltau) return fail(ctx, SH_E_STATE, "time-displaced measurement not set up (sh_set_dyn)");
+    HIPCHK(hipMemcpyAsync(znn, c.znn, sizeof(cplx) * 4 * c.lq * (c.ntdm + 1), hipMemcpyDeviceToHost, ctx->st));
+    if (dynerr) HIPCHK(hipMemcpyAsync(dynerr, c.dynerr, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int sh_reset_dyn_obs(sh_ctx *ctx)
+{
+    if (!ctx) return SH_E_ARG;
+    for (auto &c : ctx->hc)
+        if (c.ltau) {
+            HIPCHK(hipMemsetAsync(c.znn, 0, sizeof(cplx) * 4 * c.lq * (ctx->ltrot + 1), ctx->st));
+            HIPCHK(hipMemsetAsync(c.dynerr, 0, 2 * sizeof(double), ctx->st));
+        }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int sh_get_gtau(sh_ctx *ctx, int chain, double *gt0, double *g0t, double *g00)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !gt0 || !g0t || !g00) return SH_E_ARG;
+    const DChain &c = ctx->hc[chain];
+    if (!c.ltau) return fail(ctx, SH_E_STATE, "time-displaced measurement not set up (sh_set_dyn)");
+    const size_t nn = (size_t)c.n * c.n * sizeof(cplx);
+    HIPCHK(hipMemcpyAsync(gt0, c.gt0, nn, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(g0t, c.g0t, nn, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(g00, c.g00, nn, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
 int sh_get_rng_state(sh_ctx *ctx, int chain, uint32_t *state)
 {
     if (!ctx || chain < 0 || chain >= ctx->nch || !state) return SH_E_ARG;
@@ -718,11 +863,73 @@ int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure)
     return 0;
 }
 
+// dqmc_ft_sweep_0b(lupdate = .false., lmeasure_equaltime, lmeasure_dyn = .true.) with ltau
+// (src/dqmc_ft_core/dqmc_ft_sweep_0b.f90:36-49, :160-204, dqmc_ft_core/dyn.f90): no updates;
+// G(t,0) <- B(t) G(t,0) and G(0,t) <- G(0,t) B(t)^-1 slice by slice, replaced by green_tau at
+// every stabilization point after the -DDYNERROR comparison; bins 0 .. ntdm measured
+static int sweep_0b_dyn(sh_ctx *ctx, int lmeasure)
+{
+    set_identity(ctx, 0);
+    // the bins of up to 512 cells and two diagonals (<= 48 KiB)
+    const size_t dyn_lds = (size_t)(8 * 512 + 2 * ctx->nmax) * sizeof(double);
+    auto dyn_measure = [&](int b) {
+        Tic t(ctx, T_MEASURE);
+        hipLaunchKernelGGL(k_dyn_measure, dim3(ctx->nch), dim3(256), dyn_lds, ctx->st, ctx->dc, b);
+    };
+    {
+        Tic t(ctx, T_OTHER);
+        hipLaunchKernelGGL(k_dyn_elem, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, 0, (int)M_G, 0);
+    }
+    dyn_measure(0);
+    for (int nt = 1; nt <= ctx->ltrot; nt++) {
+        {
+            Tic t(ctx, T_WRAP);
+            left_pass(ctx, M_G, 0, 0, 1 | 2, 0, nt, nt, 1);
+            right_pass(ctx, 1, 1 | 2, 1, nt);
+            left_pass(ctx, M_G, 0, 0, 4, 0, nt, nt, 1);
+            right_pass(ctx, 1, 4, 1, nt);
+        }
+        if (lmeasure) measure(ctx);
+        {
+            // G(t,0) <- B(t) G(t,0), G(0,t) <- G(0,t) B(t)^-1 (Bmat_left_forward / _right_backward)
+            Tic t(ctx, T_WRAP);
+            left_pass(ctx, M_GT0, 0, 0, 1 | 2 | 4, 0, nt, nt, 1);
+            right_pass(ctx, 1, 1 | 2 | 4, 1, nt, M_G0T);
+        }
+        if (ctx->iwrap_nt[nt] > 0) {
+            int n = ctx->iwrap_nt[nt];
+            {
+                Tic t(ctx, T_OTHER);
+                hipLaunchKernelGGL(k_save_point, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, n);
+            }
+            stablize_0b(ctx, n);
+            green_tau(ctx, n);
+            {
+                Tic t(ctx, T_OTHER);
+                hipLaunchKernelGGL(k_dyn_err, dim3(ctx->nch), dim3(256), 0, ctx->st, ctx->dc);
+                hipLaunchKernelGGL(k_dyn_elem, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, 8, (int)M_GT0, 0);
+                hipLaunchKernelGGL(k_dyn_elem, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, 8, (int)M_G0T, 0);
+            }
+            wrap_replace(ctx, 1);
+        }
+        if (nt <= ctx->ntdm) dyn_measure(nt);
+    }
+    int rc = check_launch(ctx);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
 int sh_sweep_0b(sh_ctx *ctx, int lupdate, int lmeasure, int lmeasure_dyn)
 {
     if (!ctx) return SH_E_ARG;
-    if (lmeasure_dyn) return fail(ctx, SH_E_ARG, "time-displaced measurement is not provided");
     if (!ctx->started) return fail(ctx, SH_E_STATE, "sweep_0b before sweep_start_0b");
+    if (lmeasure_dyn) {
+        if (lupdate) return fail(ctx, SH_E_ARG, "time-displaced measurement runs without updates (lupdate = 0)");
+        for (const auto &c : ctx->hc)
+            if (!c.ltau) return fail(ctx, SH_E_STATE, "time-displaced measurement not set up on every chain (sh_set_dyn)");
+        return sweep_0b_dyn(ctx, lmeasure);
+    }
     set_identity(ctx, 0);
     const bool upd = lupdate && ctx->hc[0].lwrapu;
     if (upd) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);

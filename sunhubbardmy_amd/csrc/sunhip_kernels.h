@@ -69,7 +69,7 @@ enum {
 };
 
 // matrix selectors resolved per chain on the device
-enum { M_G = 0, M_GS, M_W1, M_W2, M_W3, M_W4, M_US, M_VS, M_U, M_V, M_KT, M_KTM1 };
+enum { M_G = 0, M_GS, M_W1, M_W2, M_W3, M_W4, M_US, M_VS, M_U, M_V, M_KT, M_KTM1, M_GT0, M_G0T, M_G00, M_GT0S, M_G0TS };
 
 struct DChain {
     int n, lq, nn_nf, nn_lf, lcomp, nflr, kmode, nublock, update, lwrapT, lwrapu, phase_from_conf;
@@ -96,6 +96,12 @@ struct DChain {
     int trace_cap, pad2;
     cplx bmat[4], bmat_inv[4], delta_bmat[12], phase_ratio[12];
     double gaml[4], parg[4];
+    // time-displaced measurement (ltau): G(t,0), G(0,t), G(0,0), green_tau's scratch G(t,0) and
+    // G(0,t), the znn_tau bins [ntdm+1][2][2][lq], cell x, y (1-based) and orbital of every site
+    cplx *gt0, *g0t, *g00, *gt0s, *g0ts, *znn;
+    const int32_t *sxy;
+    double *dynerr;       // largest |propagated - green_tau| of G(t,0), G(0,t)
+    int dl1, dl2, ntdm, ltau;
 };
 
 __device__ __forceinline__ cplx cmk(double r, double i) { return make_double2(r, i); }
@@ -150,6 +156,11 @@ __device__ __forceinline__ cplx *chain_mat(const DChain &c, int sel, int pt)
     case M_V: return c.V + nn * pt;
     case M_KT: return (cplx *)c.kt;
     case M_KTM1: return (cplx *)c.ktm1;
+    case M_GT0: return c.gt0;
+    case M_G0T: return c.g0t;
+    case M_G00: return c.g00;
+    case M_GT0S: return c.gt0s;
+    case M_G0TS: return c.g0ts;
     }
     return nullptr;
 }

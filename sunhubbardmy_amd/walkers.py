@@ -156,6 +156,40 @@ class FTWalkers:
         self.sweep_b0(lupdate, lmeasure)
         self.sweep_0b(lupdate, lmeasure)
 
+    # -- time-displaced measurement (ltau) ---------------------------------------------------
+    def set_dyn(self, ntdm=None):
+        """Switch on G(t,0), G(0,t) and the znn_tau bins for t = 0 .. ntdm slices (default ltrot);
+        sweep_0b(lupdate=False, lmeasure_dyn=True) then runs the dynamic pass."""
+        for c, t in enumerate(self.tables):
+            m = t["model"]
+            n = t["ndim"]
+            nsub = n // t["lq"]
+            cell = np.arange(n) // nsub
+            sites = np.stack([cell // m.lb + 1, cell % m.lb + 1, np.arange(n) % nsub], axis=1).astype(np.int32)
+            nd = t["ltrot"] if ntdm is None else int(ntdm)
+            self._check(_lib.lib().sh_set_dyn(self._ctx, c, nd, m.la, m.lb, _c(sites, np.int32).ctypes.data_as(_lib.ip)),
+                        "set_dyn")
+        self.ntdm = self.tables[0]["ltrot"] if ntdm is None else int(ntdm)
+
+    def dyn_obs(self, chain=0):
+        """(znn_tau bins [ntdm+1][orbital j][orbital i][lq], {xmax_dyn, count})"""
+        lq = self.tables[chain]["lq"]
+        z = np.zeros((self.ntdm + 1, 2, 2, lq), np.complex128)
+        e = np.zeros(2)
+        self._check(_lib.lib().sh_get_dyn_obs(self._ctx, chain, z.ctypes.data_as(_lib.dp), e.ctypes.data_as(_lib.dp)),
+                    "get_dyn_obs")
+        return z, {"xmax_dyn": e[0], "count": e[1]}
+
+    def reset_dyn_obs(self):
+        self._check(_lib.lib().sh_reset_dyn_obs(self._ctx), "reset_dyn_obs")
+
+    def gtau(self, chain=0):
+        """(G(t,0), G(0,t), G(0,0)) after the last dynamic pass"""
+        n = self.tables[chain]["ndim"]
+        m = [np.zeros((n, n), np.complex128, order="F") for _ in range(3)]
+        self._check(_lib.lib().sh_get_gtau(self._ctx, chain, *[x.ctypes.data_as(_lib.dp) for x in m]), "get_gtau")
+        return tuple(m)
+
     def push_stage(self):
         self._check(_lib.lib().sh_push_stage(self._ctx), "push_stage")
 
