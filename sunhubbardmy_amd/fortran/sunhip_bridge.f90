@@ -67,6 +67,30 @@ module sunhip_bridge
        integer(c_int), value :: chain
        integer(c_int32_t), intent(out) :: conf(*)
      end function sh_get_conf
+     ! ltau: sites(3, ndim) = cell x (1..l1), cell y (1..l2), orbital (0, 1) per site
+     integer(c_int) function sh_set_dyn(ctx, chain, ntdm, l1, l2, sites) bind(C, name="sh_set_dyn")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain, ntdm, l1, l2
+       integer(c_int32_t), intent(in) :: sites(*)
+     end function sh_set_dyn
+     ! znn_tau_bin(lq, 2, 2, ntdm+1) as complex(c_double_complex); dynerr(2) = xmax_dyn, count
+     integer(c_int) function sh_get_dyn_obs(ctx, chain, znn, dynerr) bind(C, name="sh_get_dyn_obs")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       real(c_double), intent(out) :: znn(*), dynerr(2)
+     end function sh_get_dyn_obs
+     integer(c_int) function sh_reset_dyn_obs(ctx) bind(C, name="sh_reset_dyn_obs")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx
+     end function sh_reset_dyn_obs
+     integer(c_int) function sh_get_gtau(ctx, chain, gt0, g0t, g00) bind(C, name="sh_get_gtau")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       real(c_double), intent(out) :: gt0(*), g0t(*), g00(*)
+     end function sh_get_gtau
      ! the chain's generator state, SH_RNG_WORDS = 627 words (uint32 as int32): bit-exact resume
      integer(c_int) function sh_get_rng_state(ctx, chain, state) bind(C, name="sh_get_rng_state")
        import :: c_ptr, c_int, c_int32_t
