@@ -61,6 +61,8 @@ typedef struct sh_model {
     int update;             /* SH_U_* */
     int nublock;            /* delay depth / submatrix block */
     int32_t seed;           /* SFMT19937 seed of this chain (spring_sfmt_init) */
+    double kin_weight;      /* factor of the zkint bin: 1 (0 is read as 1) in 2D and 3D, nflr * rt for
+                               the chain (src/dqmc_measure/equaltime_measure1d.f90:76) */
 } sh_model;
 
 typedef struct sh_ctx sh_ctx;
@@ -80,15 +82,16 @@ int sh_set_conf(sh_ctx *ctx, int chain, const int32_t *conf);
 int sh_get_conf(sh_ctx *ctx, int chain, int32_t *conf);
 
 /* Time-displaced measurement (ltau, dyntau = ntdm * dtau: src/model_para.f90:457).  sh_set_dyn
- * gives a chain the unit-cell coordinates of its sites, [ndim][3] int32: x in 1..l1, y in 1..l2,
- * orbital 0 or 1 (latt%list(i,1:2), src/lattice.f90:386-491, 611-776) and switches on G(t,0),
+ * gives a chain the unit-cell coordinates of its sites, [ndim][4] int32: x in 1..l1, y in 1..l2,
+ * z in 1..l3, orbital 0 or 1 (latt%list(i,1:2) with latt%ulist, src/lattice.f90:67-135, 386-491,
+ * 1033-1157; l2 = l3 = 1 for the chain, l3 = 1 in 2D) and switches on G(t,0),
  * G(0,t), G(0,0) and the znn_tau bins; every chain of a batch takes the same ntdm (<= ltrot).
  * sh_sweep_0b(ctx, 0, lmeasure, 1) is then the reference's dynamic pass (dqmc_ft_ctrl.f90:99-101
  * between sh_push_stage and sh_pop_stage): no updates, green_tau at every stabilization point
  * (src/dqmc_ft_core/green_tau.f90:1-122), znn_tau(imj, no_i, no_j, t) += ... for t = 0 .. ntdm
  * (src/dqmc_measure/dyn_measure.f90:51-64; the reference writes past its array when ntdm < ltrot,
  * here the slices past ntdm are not measured). */
-int sh_set_dyn(sh_ctx *ctx, int chain, int ntdm, int l1, int l2, const int32_t *sites);
+int sh_set_dyn(sh_ctx *ctx, int chain, int ntdm, int l1, int l2, int l3, const int32_t *sites);
 /* znn_tau_bin as complex [ntdm+1][2][2][lq] (the Fortran znn_tau(lq, 2, 2, ntdm+1)); dynerr (may be
  * NULL): the largest |propagated - green_tau| of G(t,0) and G(0,t) and the number of such
  * comparisons (xmax_dyn, logdyncount of -DDYNERROR) */

@@ -52,6 +52,8 @@ typedef struct shp_model {
     const double *ugaml;          /* [ulcomp] gaml */
     const double *uphase_arg;     /* [ulcomp] arg of phase(is), for set_phase */
     int uphase_from_conf;         /* 1 when set_phase multiplies the field's phases (rhub > 0) */
+    double kin_weight;            /* factor of the zkint bin: 1 (0 is read as 1), nflr * rt for the
+                                     chain (src/dqmc_measure/equaltime_measure1d.f90:76) */
 } shp_model;
 
 /* allocate_core (src/dqmc_proj_core.f90:23-49) for a batch of chains sharing ndim, ne, ltrot,

@@ -110,7 +110,7 @@ class OrcParams(ctypes.Structure):
         ("mu", ctypes.c_double), ("rhub", ctypes.c_double), ("nu", ctypes.c_double),
         ("nflr", ctypes.c_int), ("exact_hs", ctypes.c_int), ("kmode", ctypes.c_int),
         ("update", ctypes.c_int), ("nwrap", ctypes.c_int), ("nublock", ctypes.c_int),
-        ("seed", ctypes.c_int32),
+        ("seed", ctypes.c_int32), ("lc", ctypes.c_int),
     ]
 
 
@@ -134,7 +134,7 @@ def _iptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
 
 
-LATTICES = {"square": 0, "honeycomb": 1}
+LATTICES = {"square": 0, "honeycomb": 1, "chain": 2, "cubic": 3}
 KMODES = {"checkerboard": 0, "dense": 1, "fft": 1}
 UPDATES = {"fast": 0, "delay": 1, "submatrix": 2}
 
@@ -144,13 +144,13 @@ class OracleChain:
 
     def __init__(self, lattice="square", la=4, lb=4, beta=4.0, dtau=0.1, rt=1.0, mu=0.0,
                  rhub=4.0, nu=0.0, nflr=2, exact_hs=True, kmode="checkerboard",
-                 update="delay", nwrap=10, nublock=8, seed=None):
+                 update="delay", nwrap=10, nublock=8, seed=None, lc=1):
         L = lib()
         if seed is None:
             seed = L.orc_chain_seed(0)
         self.params = OrcParams(LATTICES[lattice], la, lb, beta, dtau, rt, mu, rhub, nu, nflr,
                                 int(bool(exact_hs)), KMODES[kmode], UPDATES[update], nwrap,
-                                nublock, int(np.int32(seed)))
+                                nublock, int(np.int32(seed)), lc)
         self._c = L.orc_create(ctypes.byref(self.params))
         s = np.zeros(8, np.int32)
         L.orc_sizes(self._c, _iptr(s))

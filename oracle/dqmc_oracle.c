@@ -426,7 +426,7 @@ struct orc_ctx {
     int nn_nf, nn_lf;
     int *nnlf;          /* [nn_lf] A sites, 0-based */
     int *nnlist;        /* [nsites*nn_nf] */
-    int *cx, *cy, *sub; /* unit-cell coordinates and sublattice of each site */
+    int *cx, *cy, *cz, *sub; /* unit-cell coordinates and sublattice of each site */
     int *iwrap_nt;      /* [ltrot+1] */
     int *wrap_step;     /* [2*nst], 1-based slice numbers */
     /* kinetic */
@@ -464,12 +464,60 @@ struct orc_ctx {
     int dtrace_cap, dtrace_n;
 };
 
-/* src/lattice.f90:792-961 (square) and :611-776 (honeycomb); 0-based site indices */
+/* src/lattice.f90:792-961 (square), :611-776 (honeycomb), :240-316 (chain), :1278-1400 (cubic);
+ * 0-based site indices */
 static void setup_lattice(orc_ctx *c)
 {
     int l1 = c->p.la, l2 = c->p.lb;
 #define NPBC(v, l) ((v) > (l) ? (v) - (l) : ((v) < 1 ? (v) + (l) : (v)))
-    if (c->p.lattice == 0) {
+    if (c->p.lattice == 2) {
+        c->nsub = 1; c->lq = l1; c->ndim = l1;
+        c->nn_nf = 2; c->nn_lf = l1 / 2;
+        c->nnlf = (int *)calloc(c->nn_lf > 0 ? c->nn_lf : 1, sizeof(int));
+        c->nnlist = (int *)calloc((size_t)c->ndim * 2, sizeof(int));
+        c->cx = (int *)calloc(c->ndim, sizeof(int));
+        c->cy = (int *)calloc(c->ndim, sizeof(int));
+        c->sub = (int *)calloc(c->ndim, sizeof(int));
+        int ind = 0;
+        for (int i1 = 1; i1 <= l1; i1++) {
+            c->cx[i1 - 1] = i1; c->cy[i1 - 1] = 1;
+            if (i1 % 2 == 0 && ind < c->nn_lf) c->nnlf[ind++] = i1 - 1;
+        }
+        for (int s = 0; s < c->ndim; s++) {
+            c->nnlist[s * 2 + 0] = NPBC(c->cx[s] + 1, l1) - 1;
+            c->nnlist[s * 2 + 1] = NPBC(c->cx[s] - 1, l1) - 1;
+        }
+    } else if (c->p.lattice == 3) {
+        int l3 = c->p.lc;
+        c->nsub = 1; c->lq = l1 * l2 * l3; c->ndim = c->lq;
+        c->nn_nf = 6; c->nn_lf = c->ndim / 2;
+        c->nnlf = (int *)calloc(c->nn_lf > 0 ? c->nn_lf : 1, sizeof(int));
+        c->nnlist = (int *)calloc((size_t)c->ndim * 6, sizeof(int));
+        c->cx = (int *)calloc(c->ndim, sizeof(int));
+        c->cy = (int *)calloc(c->ndim, sizeof(int));
+        c->cz = (int *)calloc(c->ndim, sizeof(int));
+        c->sub = (int *)calloc(c->ndim, sizeof(int));
+#define CSITE(x, y, z) (((z) - 1) * l1 * l2 + ((x) - 1) * l2 + (y) - 1)
+        int ind = 0;
+        for (int i1 = 1; i1 <= l1; i1++)
+            for (int i2 = 1; i2 <= l2; i2++)
+                for (int i3 = 1; i3 <= l3; i3++) {
+                    int s = CSITE(i1, i2, i3);
+                    c->cx[s] = i1; c->cy[s] = i2; c->cz[s] = i3;
+                    if ((i1 + i2 + i3) % 2 == 0 && ind < c->nn_lf) c->nnlf[ind++] = s;
+                }
+        for (int s = 0; s < c->ndim; s++) {
+            int x = c->cx[s], y = c->cy[s], z = c->cz[s];
+            int *nl = c->nnlist + s * 6;
+            nl[0] = CSITE(x, y, NPBC(z + 1, l3));
+            nl[1] = CSITE(NPBC(x + 1, l1), y, z);
+            nl[2] = CSITE(x, NPBC(y + 1, l2), z);
+            nl[3] = CSITE(x, y, NPBC(z - 1, l3));
+            nl[4] = CSITE(NPBC(x - 1, l1), y, z);
+            nl[5] = CSITE(x, NPBC(y - 1, l2), z);
+        }
+#undef CSITE
+    } else if (c->p.lattice == 0) {
         c->nsub = 1; c->lq = l1 * l2; c->ndim = c->lq;
         c->nn_nf = 4; c->nn_lf = l1 * l2 / 2;
         c->nnlf = (int *)calloc(c->nn_lf > 0 ? c->nn_lf : 1, sizeof(int));
@@ -516,6 +564,10 @@ static void setup_lattice(orc_ctx *c)
                 c->nnlist[s * 3 + 2] = ((NPBC(x, l1) - 1) * l2 + NPBC(y + 1, l2)) * 2 - 2;
             }
         }
+    }
+    if (!c->cz) {
+        c->cz = (int *)calloc(c->ndim, sizeof(int));
+        for (int s = 0; s < c->ndim; s++) c->cz[s] = 1;
     }
 #undef NPBC
 }
@@ -742,7 +794,7 @@ void orc_destroy(orc_ctx *c)
     free(c->Ust); free(c->Vst); free(c->Dst); free(c->logdetQst);
     free(c->G); free(c->Gtmp); free(c->conf);
     free(c->urt); free(c->urtm1);
-    free(c->nnlf); free(c->nnlist); free(c->cx); free(c->cy); free(c->sub);
+    free(c->nnlf); free(c->nnlist); free(c->cx); free(c->cy); free(c->cz); free(c->sub);
     free(c->iwrap_nt); free(c->wrap_step);
     free(c->prop_nbr); free(c->prop_w);
     free(c->gt0); free(c->g0t); free(c->g00); free(c->znn);
@@ -1400,6 +1452,20 @@ int orc_sweep_start(orc_ctx *c)
     return info;
 }
 
+/* the staggering of site s: + on even x + y (square), x (chain), x + y + z (cubic), on sublattice A
+ * (honeycomb) */
+static double stag_of(const orc_ctx *c, int s)
+{
+    int par;
+    switch (c->p.lattice) {
+    case 1: return c->sub[s] ? -1.0 : 1.0;
+    case 2: par = c->cx[s]; break;
+    case 3: par = c->cx[s] + c->cy[s] + c->cz[s]; break;
+    default: par = c->cx[s] + c->cy[s]; break;
+    }
+    return (par & 1) ? -1.0 : 1.0;
+}
+
 /* equaltime_measure (src/dqmc_measure/equaltime_measure.f90:1-69): sign, density, zkint,
  * zint_u and the staggered spin structure factor built from zspsm (:283-299) */
 static void measure(orc_ctx *c)
@@ -1423,11 +1489,13 @@ static void measure(orc_ctx *c)
             zk += -2.0 * creal(gc);
         }
     }
+    /* the chain's zkint carries nflr * rt (src/dqmc_measure/equaltime_measure1d.f90:76) */
+    if (c->p.lattice == 2) zk *= c->p.nflr * c->p.rt;
     double fac = 1.0 - 1.0 / (double)(c->p.nflr * c->p.nflr);
     for (int j = 0; j < N; j++) {
-        double sj = (c->p.lattice == 0) ? (((c->cx[j] + c->cy[j]) & 1) ? -1.0 : 1.0) : (c->sub[j] ? -1.0 : 1.0);
+        double sj = stag_of(c, j);
         for (int i = 0; i < N; i++) {
-            double si = (c->p.lattice == 0) ? (((c->cx[i] + c->cy[i]) & 1) ? -1.0 : 1.0) : (c->sub[i] ? -1.0 : 1.0);
+            double si = stag_of(c, i);
             zc gc = (i == j ? 1.0 : 0.0) - G[IDX(j, i, N)];
             ss += si * sj * gc * G[IDX(i, j, N)];
         }
@@ -1635,18 +1703,21 @@ static int green_tau(orc_ctx *c, const zc *ur, const double *dr, const zc *vr_in
 
 /* dyn_measure (src/dqmc_measure/dyn_measure.f90:1-101), the active znn_tau term: bin nt (1-based)
  * += ((1/2 - Re gtt(i,i)) (1/2 - Re g00(j,j)) - g0t(j,i) gt0(i,j) / nflr) zphi, summed into
- * (imj, orbital of i, orbital of j); imj = npbc(yi - yj) + (npbc(xi - xj) - 1) l2
- * (src/lattice.f90:471-479) */
+ * (imj, orbital of i, orbital of j); imj = npbc(yi - yj) + (npbc(xi - xj) - 1) l2 in 2D
+ * (src/lattice.f90:471-479), + (npbc(zi - zj) - 1) l1 l2 in 3D (:1136-1145), npbc(xi - xj) in 1D
+ * (:116-123) */
 static void dyn_measure(orc_ctx *c, int nt, const zc *gtt)
 {
-    int N = c->ndim, l1 = c->p.la, l2 = c->p.lb, lq = c->lq;
+    int N = c->ndim, l1 = c->p.la, l2 = c->p.lb, l3 = c->p.lattice == 3 ? c->p.lc : 1, lq = c->lq;
+    if (c->p.lattice == 2) l2 = 1;
     double sg = creal(c->phase) < 0 ? -1.0 : 1.0;
     zc zphi = (sg / creal(c->phase)) * c->phase;
     zc *bin = c->znn + (size_t)lq * 4 * (nt - 1);
 #define NPBC(v, l) ((v) > (l) ? (v) - (l) : ((v) < 1 ? (v) + (l) : (v)))
     for (int j = 0; j < N; j++)
         for (int i = 0; i < N; i++) {
-            int imj = NPBC(c->cy[i] - c->cy[j], l2) + (NPBC(c->cx[i] - c->cx[j], l1) - 1) * l2;
+            int imj = (NPBC(c->cz[i] - c->cz[j], l3) - 1) * l1 * l2 + NPBC(c->cy[i] - c->cy[j], l2) +
+                      (NPBC(c->cx[i] - c->cx[j], l1) - 1) * l2;
             zc v = (0.5 - creal(gtt[IDX(i, i, N)])) * (0.5 - creal(c->g00[IDX(j, j, N)]))
                  - c->g0t[IDX(j, i, N)] * c->gt0[IDX(i, j, N)] / (double)c->p.nflr;
             bin[(imj - 1) + (size_t)lq * (c->sub[i] + 2 * c->sub[j])] += v * zphi;
@@ -1861,7 +1932,7 @@ orc_pctx *orc_pcreate(const orc_pparams *p)
     memset(&q, 0, sizeof(q));
     q.lattice = p->lattice; q.la = p->la; q.lb = p->lb; q.beta = p->beta; q.dtau = p->dtau;
     q.rt = p->rt; q.mu = p->mu; q.rhub = p->rhub; q.nu = p->nu; q.nflr = p->nflr; q.exact_hs = p->exact_hs;
-    q.kmode = p->kmode; q.update = 2; q.nwrap = p->nwrap; q.nublock = 1; q.seed = p->seed;
+    q.kmode = p->kmode; q.update = 2; q.nwrap = p->nwrap; q.nublock = 1; q.seed = p->seed; q.lc = 1;
     orc_pctx *c = (orc_pctx *)calloc(1, sizeof(orc_pctx));
     c->p = *p;
     orc_ctx *b = (orc_ctx *)calloc(1, sizeof(orc_ctx));

@@ -19,7 +19,7 @@ extern "C" {
 #endif
 
 typedef struct orc_params {
-    int lattice;    /* 0 = square (src/lattice.f90:792), 1 = honeycomb (:611) */
+    int lattice;    /* 0 = square (src/lattice.f90:792), 1 = honeycomb (:611), 2 = chain (:240), 3 = cubic (:1278) */
     int la, lb;
     double beta, dtau, rt, mu, rhub, nu;
     int nflr;
@@ -28,6 +28,7 @@ typedef struct orc_params {
     int update;     /* 0 = fast, 1 = delay, 2 = submatrix */
     int nwrap, nublock;
     int seed;       /* SFMT19937 seed, int32 */
+    int lc;         /* third linear size (cubic) */
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;

@@ -23,6 +23,7 @@ class ShModel(ctypes.Structure):
         ("bmat", dp), ("bmat_inv", dp), ("phase_arg", dp), ("delta_bmat", dp), ("phase_ratio", dp),
         ("gaml", dp), ("phase_from_conf", ctypes.c_int), ("stag", dp),
         ("update", ctypes.c_int), ("nublock", ctypes.c_int), ("seed", ctypes.c_int32),
+        ("kin_weight", ctypes.c_double),
     ]
 
 
@@ -89,7 +90,7 @@ def lib():
     u32p = ctypes.POINTER(ctypes.c_uint32)
     L.sh_get_rng_state.argtypes = [P, ctypes.c_int, u32p]
     L.sh_set_rng_state.argtypes = [P, ctypes.c_int, u32p]
-    L.sh_set_dyn.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]
+    L.sh_set_dyn.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]
     L.sh_get_dyn_obs.argtypes = [P, ctypes.c_int, dp, dp]
     L.sh_reset_dyn_obs.argtypes = [P]
     L.sh_get_gtau.argtypes = [P, ctypes.c_int, dp, dp, dp]

@@ -85,15 +85,16 @@ extern "C" __global__ void k_dyn_info(const DChain *chains, int nch, int mode)
 }
 
 // dyn_measure: bin b (0-based slice) of znn_tau += ((1/2 - Re G(t,t)_ii) (1/2 - Re G(0,0)_jj)
-// - G(0,t)_ji G(t,0)_ij / nflr) zphi at (imj, orbital i, orbital j), imj = npbc(yi - yj) +
-// (npbc(xi - xj) - 1) l2 (src/lattice.f90:471-479).  One workgroup per chain; the bins are summed
+// - G(0,t)_ji G(t,0)_ij / nflr) zphi at (imj, orbital i, orbital j), imj = (npbc(zi - zj) - 1)
+// l1 l2 + npbc(yi - yj) + (npbc(xi - xj) - 1) l2 (src/lattice.f90:471-479, :1136-1145; the 1D
+// and 2D maps are the cases l2 = l3 = 1 and l3 = 1).  One workgroup per chain; the bins are summed
 // in LDS (lq x 4 complex) and added to the chain's znn once.
 extern "C" __global__ void __launch_bounds__(256)
 k_dyn_measure(const DChain *chains, int b)
 {
     extern __shared__ double dbin[];        // 2 * 4 * lq, then 2 * n diagonals
     const DChain &c = chains[blockIdx.x];
-    const int n = c.n, lq = c.lq, l1 = c.dl1, l2 = c.dl2;
+    const int n = c.n, lq = c.lq, l1 = c.dl1, l2 = c.dl2, l3 = c.dl3;
     if (!c.ltau || b > c.ntdm) return;
     double *dtt = dbin + 8 * lq, *d00 = dtt + n;
     for (int k = threadIdx.x; k < 8 * lq; k += blockDim.x) dbin[k] = 0.0;
@@ -114,11 +115,12 @@ k_dyn_measure(const DChain *chains, int b)
         const cplx p = cmul(g0, gt);
         const cplx v = make_double2(dtt[i] * d00[j] - rf * p.x, -rf * p.y);
         const cplx w = cmul(v, zphi);
-        int dx = xy[3 * i] - xy[3 * j], dy = xy[3 * i + 1] - xy[3 * j + 1];
+        int dx = xy[4 * i] - xy[4 * j], dy = xy[4 * i + 1] - xy[4 * j + 1], dz = xy[4 * i + 2] - xy[4 * j + 2];
         dx = dx < 1 ? dx + l1 : dx;
         dy = dy < 1 ? dy + l2 : dy;
-        const int imj = dy + (dx - 1) * l2 - 1;
-        const int k = imj + lq * (xy[3 * i + 2] + 2 * xy[3 * j + 2]);
+        dz = dz < 1 ? dz + l3 : dz;
+        const int imj = (dz - 1) * l1 * l2 + dy + (dx - 1) * l2 - 1;
+        const int k = imj + lq * (xy[4 * i + 3] + 2 * xy[4 * j + 3]);
         atomicAdd(&dbin[2 * k], w.x);
         atomicAdd(&dbin[2 * k + 1], w.y);
     }

@@ -955,7 +955,7 @@ k_measure(const DChain *chains)
         cplx v[5];
         v[0] = make_double2(sg, 0.0);
         v[1] = cmul(cscale(make_double2(zne_r, zne_i), 1.0 / (double)c.lq), zphi);
-        v[2] = cmul(make_double2(zk, 0.0), zphi);
+        v[2] = cmul(make_double2(zk * c.kin_weight, 0.0), zphi);
         v[3] = cmul(make_double2(zu_r, zu_i), zphi);
         v[4] = cmul(ss, zphi);
         sc[SC_NOBS] += 1.0;

@@ -463,6 +463,7 @@ int shp_create(shp_ctx **out, int nchains, const shp_model *m, int device)
         memset(&v, 0, sizeof(v));
         v.n = n; v.lq = q.lq; v.nn_nf = q.nn_nf; v.nn_lf = q.nn_lf; v.nflr = q.nflr; v.lcomp = 2;
         v.G = c.G; v.bonds = c.bonds; v.stag = c.stag; v.sc = c.sc; v.rwork = c.rwork;
+        v.kin_weight = q.kin_weight != 0.0 ? q.kin_weight : 1.0;
     }
     ctx->dc = dupload<PChain>(ctx, ctx->hc.data(), nchains);
     ctx->dview = dupload<DChain>(ctx, views.data(), nchains);

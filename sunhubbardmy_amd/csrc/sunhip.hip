@@ -598,6 +598,7 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
         c.n = n; c.lq = q.lq; c.nn_nf = q.nn_nf; c.nn_lf = q.nn_lf; c.lcomp = q.lcomp; c.nflr = q.nflr;
         c.kmode = q.kmode; c.nublock = q.nublock; c.update = q.update; c.lwrapT = q.lwrapT; c.lwrapu = q.lwrapu;
         c.phase_from_conf = q.phase_from_conf; c.ltrot = q.ltrot; c.nst = q.nst;
+        c.kin_weight = q.kin_weight != 0.0 ? q.kin_weight : 1.0;
         c.G = dalloc<cplx>(ctx, nn); c.Gs = dalloc<cplx>(ctx, nn);
         c.W1 = dalloc<cplx>(ctx, nn); c.W2 = dalloc<cplx>(ctx, nn); c.W3 = dalloc<cplx>(ctx, nn);
         c.W4 = dalloc<cplx>(ctx, nn); c.Us = dalloc<cplx>(ctx, nn); c.Vs = dalloc<cplx>(ctx, nn);
@@ -701,17 +702,18 @@ int sh_get_conf(sh_ctx *ctx, int chain, int32_t *conf)
     return 0;
 }
 
-int sh_set_dyn(sh_ctx *ctx, int chain, int ntdm, int l1, int l2, const int32_t *sites)
+int sh_set_dyn(sh_ctx *ctx, int chain, int ntdm, int l1, int l2, int l3, const int32_t *sites)
 {
-    if (!ctx || chain < 0 || chain >= ctx->nch || !sites || ntdm < 0 || ntdm > ctx->ltrot || l1 < 1 || l2 < 1)
+    if (!ctx || chain < 0 || chain >= ctx->nch || !sites || ntdm < 0 || ntdm > ctx->ltrot || l1 < 1 || l2 < 1 ||
+        l3 < 1)
         return SH_E_ARG;
     if (ctx->ntdm >= 0 && ntdm != ctx->ntdm) return fail(ctx, SH_E_ARG, "all chains of a batch share ntdm");
     DChain &c = ctx->hc[chain];
-    if (l1 * l2 != c.lq) return fail(ctx, SH_E_ARG, "l1 * l2 must be the number of unit cells");
+    if (l1 * l2 * l3 != c.lq) return fail(ctx, SH_E_ARG, "l1 * l2 * l3 must be the number of unit cells");
     for (int i = 0; i < c.n; i++)
-        if (sites[3 * i] < 1 || sites[3 * i] > l1 || sites[3 * i + 1] < 1 || sites[3 * i + 1] > l2 ||
-            sites[3 * i + 2] < 0 || sites[3 * i + 2] > 1)
-            return fail(ctx, SH_E_ARG, "site cell coordinates out of 1..l1, 1..l2 or orbital out of 0..1");
+        if (sites[4 * i] < 1 || sites[4 * i] > l1 || sites[4 * i + 1] < 1 || sites[4 * i + 1] > l2 ||
+            sites[4 * i + 2] < 1 || sites[4 * i + 2] > l3 || sites[4 * i + 3] < 0 || sites[4 * i + 3] > 1)
+            return fail(ctx, SH_E_ARG, "site cell coordinates out of 1..l1, 1..l2, 1..l3 or orbital out of 0..1");
     if (c.lq > 512) return fail(ctx, SH_E_ARG, "more unit cells than the znn_tau bins in LDS hold (512)");
     const size_t nn = (size_t)c.n * c.n;
     if (!c.ltau) {
@@ -719,13 +721,14 @@ int sh_set_dyn(sh_ctx *ctx, int chain, int ntdm, int l1, int l2, const int32_t *
         c.gt0s = dalloc<cplx>(ctx, nn); c.g0ts = dalloc<cplx>(ctx, nn);
         c.znn = dalloc<cplx>(ctx, (size_t)4 * c.lq * (ctx->ltrot + 1));
         c.dynerr = dalloc<double>(ctx, 2);
-        c.sxy = dalloc<int32_t>(ctx, (size_t)3 * c.n);
+        c.sxy = dalloc<int32_t>(ctx, (size_t)4 * c.n);
         if (!c.gt0 || !c.g0t || !c.g00 || !c.gt0s || !c.g0ts || !c.znn || !c.dynerr || !c.sxy)
             return fail(ctx, SH_E_HIP, "device allocation failed");
     }
-    HIPCHK(hipMemcpy((void *)c.sxy, sites, sizeof(int32_t) * 3 * c.n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((void *)c.sxy, sites, sizeof(int32_t) * 4 * c.n, hipMemcpyHostToDevice));
     c.dl1 = l1;
     c.dl2 = l2;
+    c.dl3 = l3;
     c.ntdm = ntdm;
     c.ltau = 1;
     ctx->ntdm = ntdm;

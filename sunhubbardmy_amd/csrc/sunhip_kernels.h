@@ -97,11 +97,12 @@ struct DChain {
     cplx bmat[4], bmat_inv[4], delta_bmat[12], phase_ratio[12];
     double gaml[4], parg[4];
     // time-displaced measurement (ltau): G(t,0), G(0,t), G(0,0), green_tau's scratch G(t,0) and
-    // G(0,t), the znn_tau bins [ntdm+1][2][2][lq], cell x, y (1-based) and orbital of every site
+    // G(0,t), the znn_tau bins [ntdm+1][2][2][lq], cell x, y, z (1-based) and orbital of every site
     cplx *gt0, *g0t, *g00, *gt0s, *g0ts, *znn;
     const int32_t *sxy;
     double *dynerr;       // largest |propagated - green_tau| of G(t,0), G(0,t)
-    int dl1, dl2, ntdm, ltau;
+    int dl1, dl2, dl3, ntdm, ltau;
+    double kin_weight;    // factor of the zkint bin (nflr * rt for the chain, else 1)
 };
 
 __device__ __forceinline__ cplx cmk(double r, double i) { return make_double2(r, i); }

@@ -24,6 +24,7 @@ module sunhip_bridge
      type(c_ptr) :: stag
      integer(c_int) :: update, nublock
      integer(c_int32_t) :: seed
+     real(c_double) :: kin_weight
   end type sh_model
 
   ! mirrors struct shp_model of include/sunhip_proj.h
@@ -38,6 +39,7 @@ module sunhip_bridge
      integer(c_int) :: ulcomp
      type(c_ptr) :: ubmat, ubmat_inv, udelta, uratio, ugaml, uphase_arg
      integer(c_int) :: uphase_from_conf
+     real(c_double) :: kin_weight
   end type shp_model
 
   interface
@@ -67,11 +69,11 @@ module sunhip_bridge
        integer(c_int), value :: chain
        integer(c_int32_t), intent(out) :: conf(*)
      end function sh_get_conf
-     ! ltau: sites(3, ndim) = cell x (1..l1), cell y (1..l2), orbital (0, 1) per site
-     integer(c_int) function sh_set_dyn(ctx, chain, ntdm, l1, l2, sites) bind(C, name="sh_set_dyn")
+     ! ltau: sites(4, ndim) = cell x (1..l1), y (1..l2), z (1..l3), orbital (0, 1) per site
+     integer(c_int) function sh_set_dyn(ctx, chain, ntdm, l1, l2, l3, sites) bind(C, name="sh_set_dyn")
        import :: c_ptr, c_int, c_int32_t
        type(c_ptr), value :: ctx
-       integer(c_int), value :: chain, ntdm, l1, l2
+       integer(c_int), value :: chain, ntdm, l1, l2, l3
        integer(c_int32_t), intent(in) :: sites(*)
      end function sh_set_dyn
      ! znn_tau_bin(lq, 2, 2, ntdm+1) as complex(c_double_complex); dynerr(2) = xmax_dyn, count

@@ -2,7 +2,8 @@
 
 Restates, for the sweep's inputs only:
   * ``make_tables`` schedule and delay-depth defaults   (src/model_para.f90:376-453)
-  * square and honeycomb lattices                       (src/lattice.f90:792-961, :611-776)
+  * square, honeycomb, chain and cubic lattices         (src/lattice.f90:792-961, :611-776,
+                                                         :240-316, :1278-1400)
   * kinetic propagators, checkerboard and dense         (src/dqmc_config_h0/dqmc_set_h0conf.f90:77-123,
                                                          :484-510; seth0.f90:1-28)
   * Hubbard-Stratonovich tables                          (src/dqmc_config_u/dqmc_set_conf.f90:1-176)
@@ -35,6 +36,7 @@ class Model:
     nwrap: int = 10
     nublock: int = 0
     seed: int = 1909706392
+    lc: int = 1                    # third linear size (cubic lattice)
 
 
 def chain_seed(c):
@@ -56,7 +58,49 @@ def _npbc(v, l):
 def lattice_tables(m):
     """Site order, checkerboard bond families and staggering (0-based site indices)."""
     l1, l2 = m.la, m.lb
-    if m.lattice == "square":
+    cz = None
+    if m.lattice == "chain":
+        # src/lattice.f90:240-316: sites 1..l1, bonds from the even sites to i+1 (family 1) and
+        # i-1 (family 2)
+        nsites = lq = l1
+        nn_nf, nn_lf = 2, l1 // 2
+        cx = np.arange(1, l1 + 1)
+        cy = np.ones(l1, int)
+        nnlf = [i1 - 1 for i1 in range(1, l1 + 1) if i1 % 2 == 0][:nn_lf]
+        nnlist = np.zeros((nsites, 2), int)
+        for s in range(nsites):
+            nnlist[s, 0] = _npbc(cx[s] + 1, l1) - 1
+            nnlist[s, 1] = _npbc(cx[s] - 1, l1) - 1
+        stag = np.where(cx % 2 == 0, 1.0, -1.0)
+        sub = np.zeros(nsites, int)
+    elif m.lattice == "cubic":
+        # src/lattice.f90:1278-1400: site (i1, i2, i3) at (i3-1) l1 l2 + (i1-1) l2 + i2; bond
+        # families +z, +x, +y, -z, -x, -y from the sites with i1 + i2 + i3 even
+        l3 = m.lc
+        nsites = lq = l1 * l2 * l3
+        nn_nf, nn_lf = 6, nsites // 2
+        cx = np.zeros(nsites, int)
+        cy = np.zeros(nsites, int)
+        cz = np.zeros(nsites, int)
+        nnlf = []
+
+        def site(x, y, z):
+            return (z - 1) * l1 * l2 + (x - 1) * l2 + y - 1
+        for i1 in range(1, l1 + 1):
+            for i2 in range(1, l2 + 1):
+                for i3 in range(1, l3 + 1):
+                    s = site(i1, i2, i3)
+                    cx[s], cy[s], cz[s] = i1, i2, i3
+                    if (i1 + i2 + i3) % 2 == 0 and len(nnlf) < nn_lf:
+                        nnlf.append(s)
+        nnlist = np.zeros((nsites, 6), int)
+        for s in range(nsites):
+            x, y, z = cx[s], cy[s], cz[s]
+            nnlist[s] = [site(x, y, _npbc(z + 1, l3)), site(_npbc(x + 1, l1), y, z), site(x, _npbc(y + 1, l2), z),
+                         site(x, y, _npbc(z - 1, l3)), site(_npbc(x - 1, l1), y, z), site(x, _npbc(y - 1, l2), z)]
+        stag = np.where((cx + cy + cz) % 2 == 0, 1.0, -1.0)
+        sub = np.zeros(nsites, int)
+    elif m.lattice == "square":
         nsites = l1 * l2
         nn_nf, nn_lf = 4, l1 * l2 // 2
         cx = np.zeros(nsites, int)
@@ -77,6 +121,7 @@ def lattice_tables(m):
             nnlist[s, 3] = (_npbc(x, l1) - 1) * l2 + _npbc(y - 1, l2) - 1
         stag = np.where((cx + cy) % 2 == 0, 1.0, -1.0)
         lq = l1 * l2
+        sub = np.zeros(nsites, int)
     elif m.lattice == "honeycomb":
         lq = l1 * l2
         nsites = 2 * lq
@@ -104,15 +149,23 @@ def lattice_tables(m):
                 nnlist[s, 2] = ((_npbc(x, l1) - 1) * l2 + _npbc(y + 1, l2)) * 2 - 2
         stag = np.where(sub == 0, 1.0, -1.0)
     else:
-        raise ValueError("lattice must be 'square' or 'honeycomb'")
+        raise ValueError("lattice must be 'square', 'honeycomb', 'chain' or 'cubic'")
     nnlf = np.array(nnlf, int)
+    if cz is None:
+        cz = np.ones(nsites, int)
     bonds = np.zeros((nn_nf, nn_lf, 2), np.int32)
     for nf in range(nn_nf):
         for b in range(nn_lf):
             bonds[nf, b, 0] = nnlf[b]
             bonds[nf, b, 1] = nnlist[nnlf[b], nf]
+    # unit-cell coordinates (1-based) and orbital of every site: latt%list / latt%ulist
+    cells = np.stack([cx, cy, cz, sub], axis=1).astype(np.int32)
+    dims = (l1, 1, 1) if m.lattice == "chain" else (l1, l2, l3) if m.lattice == "cubic" else (l1, l2, 1)
+    # the chain's zkint carries nflr * rt (src/dqmc_measure/equaltime_measure1d.f90:76)
+    kin_weight = float(m.nflr * m.rt) if m.lattice == "chain" else 1.0
     return {"nsites": nsites, "lq": lq, "nn_nf": nn_nf, "nn_lf": nn_lf, "bonds": bonds,
-            "nnlist": nnlist, "stag": stag.astype(np.float64)}
+            "nnlist": nnlist, "stag": stag.astype(np.float64), "cells": cells, "dims": dims,
+            "kin_weight": kin_weight}
 
 
 def schedule(ltrot, nwrap):

@@ -101,6 +101,7 @@ class ShpModel(ctypes.Structure):
         ("wrap_check", ctypes.c_int), ("seed", ctypes.c_int32),
         ("ulcomp", ctypes.c_int), ("ubmat", _lib.dp), ("ubmat_inv", _lib.dp), ("udelta", _lib.dp),
         ("uratio", _lib.dp), ("ugaml", _lib.dp), ("uphase_arg", _lib.dp), ("uphase_from_conf", ctypes.c_int),
+        ("kin_weight", ctypes.c_double),
     ]
 
 
@@ -159,6 +160,7 @@ class ProjWalkers:
         s.stag = keep(lat["stag"], np.float64)
         s.lq, s.nflr, s.nustock, s.mid_len = t["lq"], m.nflr, t["nustock"], m.mid_len
         s.wrap_check, s.seed = int(bool(m.wrap_check)), int(np.int32(m.seed))
+        s.kin_weight = lat["kin_weight"]
         return s
 
     def close(self):

@@ -73,6 +73,7 @@ class FTWalkers:
         s.phase_from_conf = t["phase_from_conf"]
         s.stag = keep(lat["stag"], np.float64)
         s.update, s.nublock, s.seed = t["update"], t["nublock"], int(np.int32(m.seed))
+        s.kin_weight = lat["kin_weight"]
         return s
 
     # -- lifecycle ---------------------------------------------------------------------------
@@ -161,14 +162,10 @@ class FTWalkers:
         """Switch on G(t,0), G(0,t) and the znn_tau bins for t = 0 .. ntdm slices (default ltrot);
         sweep_0b(lupdate=False, lmeasure_dyn=True) then runs the dynamic pass."""
         for c, t in enumerate(self.tables):
-            m = t["model"]
-            n = t["ndim"]
-            nsub = n // t["lq"]
-            cell = np.arange(n) // nsub
-            sites = np.stack([cell // m.lb + 1, cell % m.lb + 1, np.arange(n) % nsub], axis=1).astype(np.int32)
+            l1, l2, l3 = t["lat"]["dims"]
+            sites = _c(t["lat"]["cells"], np.int32)
             nd = t["ltrot"] if ntdm is None else int(ntdm)
-            self._check(_lib.lib().sh_set_dyn(self._ctx, c, nd, m.la, m.lb, _c(sites, np.int32).ctypes.data_as(_lib.ip)),
-                        "set_dyn")
+            self._check(_lib.lib().sh_set_dyn(self._ctx, c, nd, l1, l2, l3, sites.ctypes.data_as(_lib.ip)), "set_dyn")
         self.ntdm = self.tables[0]["ltrot"] if ntdm is None else int(ntdm)
 
     def dyn_obs(self, chain=0):

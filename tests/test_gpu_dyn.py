@@ -21,6 +21,10 @@ CASES = {
                              update="fast"),
     "square6_ntdm5": dict(lattice="square", la=6, lb=6, beta=1.5, rhub=5.0, kmode="checkerboard", nwrap=5,
                           update="delay", nublock=8, ntdm=5),
+    "chain16_delay": dict(lattice="chain", la=16, lb=1, beta=1.0, rhub=4.0, kmode="checkerboard", nwrap=4,
+                          update="delay", nublock=4),
+    "cubic4_dense": dict(lattice="cubic", la=4, lb=4, lc=4, beta=0.8, rhub=4.0, kmode="dense", nwrap=4,
+                         update="fast"),
 }
 
 

@@ -28,6 +28,11 @@ CASES = {
     "honeycomb3_delay_dense": dict(lattice="honeycomb", la=3, lb=3, kmode="dense", nublock=8),
     "honeycomb3_delay_checkerboard_nwrap7": dict(lattice="honeycomb", la=3, lb=3, nwrap=7, beta=1.6),
     "square4_su4_exact_delay": dict(nflr=4, exact_hs=True, rhub=6.0, nublock=3),
+    "chain16_delay_checkerboard": dict(lattice="chain", la=16, lb=1, nublock=8),
+    "chain10_fast_dense_mu": dict(lattice="chain", la=10, lb=1, kmode="dense", update="fast", mu=0.3),
+    "cubic4_delay_checkerboard": dict(lattice="cubic", la=4, lb=4, lc=4, beta=1.0, nublock=8),
+    "cubic4_submatrix_dense": dict(lattice="cubic", la=4, lb=4, lc=4, beta=0.8, kmode="dense", update="submatrix",
+                                   nublock=8),
 }
 
 

@@ -32,6 +32,7 @@ CASES = {
                                nustock=6, nwrap=4),
     "square4_su4_dense": dict(lattice="square", la=4, lb=4, rv=0.0, rhub=3.0, nflr=4, kmode="dense", nustock=8),
     "honeycomb3_tuv": dict(la=3, lb=3, rv=1.0, rhub=3.0, nflr=2, nustock=8, kmode="checkerboard"),
+    "chain16_tuv_cb": dict(lattice="chain", la=16, lb=1, rv=1.0, rhub=2.0, nflr=2, nustock=8, kmode="checkerboard"),
 }
 
 

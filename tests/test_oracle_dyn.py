@@ -1,6 +1,7 @@
 """Pins of the oracle's time-displaced path (ltau): green_tau (src/dqmc_ft_core/green_tau.f90),
 the 0b pass without updates that carries G(t,0) and G(0,t) (dqmc_ft_sweep_0b.f90:36-49,
-:160-204, dqmc_ft_core/dyn.f90) and the znn_tau bins (src/dqmc_measure/dyn_measure.f90).
+:160-204, dqmc_ft_core/dyn.f90) and the znn_tau bins (src/dqmc_measure/dyn_measure.f90,
+dyn_measure1d.f90, dyn_measure3d.f90) on the square, honeycomb, chain and cubic lattices.
 
 For one fixed HS field every Green's function has a closed form in the dense slice matrices
 B(t) (built here with numpy from the bond tables, independently of the oracle's propagation):
@@ -18,7 +19,22 @@ from sunhubbardmy_amd.model import make_tables
 CASES = {
     "square4_u4": dict(lattice="square", la=4, lb=4, beta=1.0, rhub=4.0, kmode="checkerboard", nwrap=4),
     "honeycomb2_u3": dict(lattice="honeycomb", la=2, lb=2, beta=1.2, rhub=3.0, kmode="checkerboard", nwrap=3),
+    "chain8_u4": dict(lattice="chain", la=8, lb=1, beta=1.0, rhub=4.0, kmode="checkerboard", nwrap=4),
+    "cubic4_u4": dict(lattice="cubic", la=4, lb=4, lc=4, beta=0.6, rhub=4.0, kmode="checkerboard", nwrap=3),
 }
+
+
+def _cells(t):
+    """1-based cell coordinates (x, y, z), orbital of every site from the site numbering, and the
+    cell grid (l1, l2, l3)"""
+    m, n, lq = t["model"], t["ndim"], t["lq"]
+    if m.lattice == "chain":
+        return [(s + 1, 1, 1, 0) for s in range(n)], (m.la, 1, 1)
+    if m.lattice == "cubic":
+        a = m.la * m.lb
+        return [((s % a) // m.lb + 1, s % m.lb + 1, s // a + 1, 0) for s in range(n)], (m.la, m.lb, m.lc)
+    nsub = n // lq
+    return [((s // nsub) // m.lb + 1, (s // nsub) % m.lb + 1, 1, s % nsub) for s in range(n)], (m.la, m.lb, 1)
 
 
 def _slices(t, conf, bm):
@@ -94,22 +110,19 @@ def test_dyn_pass_matches_dense_products(name):
 def test_znn_bins(name):
     o, t = _run(name)
     z = o.znn()                                     # [t][orbital j][orbital i][imj]
-    n, lq, nflr = t["ndim"], t["lq"], t["model"].nflr
-    la, lb = t["model"].la, t["model"].lb
-    nsub = n // lq
-    cell = np.arange(n) // nsub
-    orb = np.arange(n) % nsub
-    cx, cy = cell // lb + 1, cell % lb + 1
+    n, nflr = t["ndim"], t["model"].nflr
+    cells, (l1, l2, l3) = _cells(t)
     phase = o.diag()["phase"]
     zphi = np.sign(phase.real) / phase.real * phase
     ref = np.zeros_like(z)
     for tau, (gt0, g0t, gtt, g00) in enumerate(o.dyn_trace()):
         v = np.outer(0.5 - gtt.diagonal().real, 0.5 - g00.diagonal().real) - g0t.T * gt0 / nflr
-        for i in range(n):
-            for j in range(n):
-                dx = (cx[i] - cx[j] - 1) % la + 1
-                dy = (cy[i] - cy[j] - 1) % lb + 1
-                ref[tau, orb[j], orb[i], dy + (dx - 1) * lb - 1] += v[i, j] * zphi
+        for i, (xi, yi, zi, oi) in enumerate(cells):
+            for j, (xj, yj, zj, oj) in enumerate(cells):
+                dx = (xi - xj - 1) % l1 + 1
+                dy = (yi - yj - 1) % l2 + 1
+                dz = (zi - zj - 1) % l3 + 1
+                ref[tau, oj, oi, (dz - 1) * l1 * l2 + dy + (dx - 1) * l2 - 1] += v[i, j] * zphi
     assert np.abs(z - ref).max() < 1e-10
     # number conservation: the total over all pairs does not depend on t for one field
     tot = z.sum(axis=(1, 2, 3))

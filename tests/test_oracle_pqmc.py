@@ -174,7 +174,8 @@ def kinetic_factors(o, kmode, dtau):
 
 
 @pytest.mark.parametrize("kmode,lattice,la", [("dense", "honeycomb", 2), ("checkerboard", "honeycomb", 2),
-                                              ("checkerboard", "square", 4)])
+                                              ("checkerboard", "square", 4), ("checkerboard", "chain", 10),
+                                              ("dense", "chain", 12)])
 def test_direct_product(kmode, lattice, la):
     """After a 0b pass, G = 1 - R (P^H R)^-1 P^H with R = B(theta, 0) P built densely from
     the final fields."""
