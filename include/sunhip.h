@@ -100,6 +100,20 @@ int sh_reset_dyn_obs(sh_ctx *ctx);
 /* G(t,0), G(0,t), G(0,0) (N x N complex, column-major) as the last dynamic pass left them */
 int sh_get_gtau(sh_ctx *ctx, int chain, double *gt0, double *g0t, double *g00);
 
+/* The correlation bins of equaltime_measure (src/dqmc_measure/equaltime_measure.f90:44-68), on top
+ * of the sh_get_obs sums at every measured slice from this call on (off by default).  sites as
+ * for sh_set_dyn; cells [lq][4]: the A site of each unit cell, its nnlist(., 1), its snlist(., 1)
+ * and its current partner (snlist on the honeycomb, nnlist elsewhere: equaltime_measure.f90:
+ * 378-394), 0-based and in latt%imj's cell order; plq [lq][zplq] the plaquette site cycles
+ * (latt%plq_cord, src/lattice.f90:954-957, :767-772; zplq = 0: no plaquette terms); theta in
+ * radians.  Needs 16 lq + ndim + 544 <= 10240. */
+int sh_set_eqt(sh_ctx *ctx, int chain, int l1, int l2, int l3, const int32_t *sites, const int32_t *cells, int zplq,
+               const int32_t *plq, double theta);
+/* 20 lq + 8 complex, each summed with zphi: zcpcm, zspsm, znn, pair_onsite as (imj, no_i, no_j) at
+ * q 4 lq + imj + lq (no_i + 2 no_j); zjj, zbb, pair_nn, pair_sn at (16 + q) lq + imj; then zn(1),
+ * zn(2), zj, zb, zqsq (without its 1/9), ztsq, ztq, zint_v (without rv).  Cleared by sh_reset_obs. */
+int sh_get_eqt(sh_ctx *ctx, int chain, double *bins);
+
 /* The chain's random-number state, SH_RNG_WORDS uint32: the SFMT19937 state that spring_sfmt
  * keeps in module storage (lib/m_spring.f90:66-71: 624 words, then the read index 0..624) and,
  * for chains with learned block proposals, the proposal generator's 64-bit state (low word

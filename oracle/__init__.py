@@ -47,6 +47,9 @@ def lib():
         L.orc_set_trace.argtypes = [P, dp, ctypes.c_int]
         L.orc_trace_count.argtypes = [P]
         L.orc_set_dyn.argtypes = [P, ctypes.c_int]
+        L.orc_set_eqt.argtypes = [P, ctypes.c_double]
+        L.orc_get_eqt.argtypes = [P, dp]
+        L.orc_get_eqt.restype = None
         L.orc_sweep_0b_dyn.argtypes = [P, ctypes.c_int]
         L.orc_get_gtau.argtypes = [P, dp, dp, dp]
         L.orc_get_gtau.restype = None
@@ -221,6 +224,16 @@ class OracleChain:
 
     def reset_obs(self):
         lib().orc_reset_obs(self._c)
+
+    def set_eqt(self, theta=0.0):
+        """switch on the correlation bins of equaltime_measure; theta in radians"""
+        lib().orc_set_eqt(self._c, float(theta))
+
+    def eqt(self):
+        """the raw bins, 20 lq + 8 complex (layout of orc_get_eqt)"""
+        o = np.zeros(2 * (20 * self.lq + 8))
+        lib().orc_get_eqt(self._c, _dptr(o))
+        return o[0::2] + 1j * o[1::2]
 
     def set_trace(self, cap):
         self._trace = np.zeros((cap, self.ndim, self.ndim), np.complex128)

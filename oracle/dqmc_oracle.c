@@ -462,6 +462,12 @@ struct orc_ctx {
     double xmax_dyn, dyncount, sumlog10dyn;
     double *dtrace;
     int dtrace_cap, dtrace_n;
+    /* the correlation bins and extra energy terms of equaltime_measure (see orc_set_eqt) */
+    int eqt, zplq;
+    double theta;
+    zc *eqb;
+    int *ca, *cnn, *csn, *cjj;  /* per unit cell: A site, its nnlist(.,1), snlist(.,1), current partner */
+    int *plq;                   /* [lq][zplq] plaquette site cycles */
 };
 
 /* src/lattice.f90:792-961 (square), :611-776 (honeycomb), :240-316 (chain), :1278-1400 (cubic);
@@ -795,6 +801,7 @@ void orc_destroy(orc_ctx *c)
     free(c->G); free(c->Gtmp); free(c->conf);
     free(c->urt); free(c->urtm1);
     free(c->nnlf); free(c->nnlist); free(c->cx); free(c->cy); free(c->cz); free(c->sub);
+    free(c->eqb); free(c->ca); free(c->cnn); free(c->csn); free(c->cjj); free(c->plq);
     free(c->iwrap_nt); free(c->wrap_step);
     free(c->prop_nbr); free(c->prop_w);
     free(c->gt0); free(c->g0t); free(c->g00); free(c->znn);
@@ -1452,6 +1459,175 @@ int orc_sweep_start(orc_ctx *c)
     return info;
 }
 
+/* ---- correlation bins of equaltime_measure ------------------------------------------------ */
+#define NPBC(v, l) ((v) > (l) ? (v) - (l) : ((v) < 1 ? (v) + (l) : (v)))
+
+/* the cell grid (l1, l2, l3) and the site of cell (x, y, z) (1-based, periodic) and orbital o, in
+ * the numbering of setup_lattice */
+static void cell_dims(const orc_ctx *c, int *l1, int *l2, int *l3)
+{
+    *l1 = c->p.la;
+    *l2 = c->p.lattice == 2 ? 1 : c->p.lb;
+    *l3 = c->p.lattice == 3 ? c->p.lc : 1;
+}
+static int site_at(const orc_ctx *c, int x, int y, int z, int o)
+{
+    int l1, l2, l3;
+    cell_dims(c, &l1, &l2, &l3);
+    x = NPBC(x, l1); y = NPBC(y, l2); z = NPBC(z, l3);
+    switch (c->p.lattice) {
+    case 1: return 2 * ((x - 1) * l2 + y - 1) + o;
+    case 2: return x - 1;
+    case 3: return (z - 1) * l1 * l2 + (x - 1) * l2 + y - 1;
+    default: return (x - 1) * l2 + y - 1;
+    }
+}
+/* latt%imj of the cells of sites i and j, 0-based (src/lattice.f90:471-479, :116-122, :1136-1145) */
+static int imj_of(const orc_ctx *c, int i, int j)
+{
+    int l1, l2, l3;
+    cell_dims(c, &l1, &l2, &l3);
+    return (NPBC(c->cz[i] - c->cz[j], l3) - 1) * l1 * l2 + NPBC(c->cy[i] - c->cy[j], l2) +
+           (NPBC(c->cx[i] - c->cx[j], l1) - 1) * l2 - 1;
+}
+
+/* Switches on the correlation bins of equaltime_measure (src/dqmc_measure/equaltime_measure.f90:
+ * 44-68): zcpcm, zspsm, znn, pair_onsite as (imj, no_i, no_j), zn(no), the bond correlations zjj,
+ * zbb, pair_nn, pair_sn per imj with zj and zb, and the energy terms zqsq, ztsq, ztq (plaquettes,
+ * phase theta) and zint_v.  Per unit cell u: A site a (2u on the honeycomb), its neighbour
+ * nnlist(a,1), its second neighbour snlist(a,1) (lattice.f90:887, :710, :309, :1370) and the
+ * current partner (snlist on the honeycomb, nnlist elsewhere: equaltime_measure.f90:378-394,
+ * equaltime_measure1d.f90:241-246); plaquette cycles from lattice.f90:954-957 (square) and
+ * :767-772 (honeycomb). */
+int orc_set_eqt(orc_ctx *c, double theta)
+{
+    int lq = c->lq, l1, l2, l3;
+    cell_dims(c, &l1, &l2, &l3);
+    free(c->eqb); free(c->ca); free(c->cnn); free(c->csn); free(c->cjj); free(c->plq);
+    c->eqb = (zc *)calloc((size_t)20 * lq + 8, sizeof(zc));
+    c->ca = (int *)calloc(lq, sizeof(int)); c->cnn = (int *)calloc(lq, sizeof(int));
+    c->csn = (int *)calloc(lq, sizeof(int)); c->cjj = (int *)calloc(lq, sizeof(int));
+    c->zplq = c->p.lattice == 0 ? 4 : (c->p.lattice == 1 ? 6 : 0);
+    c->plq = (int *)calloc((size_t)lq * (c->zplq ? c->zplq : 1), sizeof(int));
+    for (int u = 0; u < lq; u++) {
+        int a = c->p.lattice == 1 ? 2 * u : u;
+        int x = c->cx[a], y = c->cy[a], z = c->cz[a];
+        c->ca[u] = a;
+        c->cnn[u] = c->nnlist[a * c->nn_nf];
+        switch (c->p.lattice) {
+        case 1: c->csn[u] = site_at(c, x + 1, y + 1, 1, 0); break;
+        case 2: c->csn[u] = site_at(c, x + 2, 1, 1, 0); break;
+        case 3: c->csn[u] = site_at(c, x + 1, y + 1, z, 0); break;
+        default: c->csn[u] = site_at(c, x + 1, y + 1, 1, 0); break;
+        }
+        c->cjj[u] = c->p.lattice == 1 ? c->csn[u] : c->cnn[u];
+        int *q = c->plq + (size_t)u * c->zplq;
+        if (c->p.lattice == 0) {
+            q[0] = site_at(c, x, y, 1, 0); q[1] = site_at(c, x + 1, y, 1, 0);
+            q[2] = site_at(c, x + 1, y + 1, 1, 0); q[3] = site_at(c, x, y + 1, 1, 0);
+        } else if (c->p.lattice == 1) {
+            q[0] = site_at(c, x, y, 1, 1); q[1] = site_at(c, x, y, 1, 0);
+            q[2] = site_at(c, x, y - 1, 1, 1); q[3] = site_at(c, x - 1, y - 1, 1, 0);
+            q[4] = site_at(c, x - 1, y - 1, 1, 1); q[5] = site_at(c, x - 1, y, 1, 0);
+        }
+    }
+    c->theta = theta;
+    c->eqt = 1;
+    return 0;
+}
+
+/* eqb (complex): [0, 4lq) zcpcm, [4lq, 8lq) zspsm, [8lq, 12lq) znn, [12lq, 16lq) pair_onsite, each
+ * (imj, no_i, no_j) at imj + lq (no_i + 2 no_j); then zjj, zbb, pair_nn, pair_sn (lq each); then
+ * zn(1), zn(2), zj, zb, zqsq, ztsq, ztq, zint_v (zqsq without its 1/9, zint_v without rv); all
+ * summed with zphi */
+void orc_get_eqt(const orc_ctx *c, double *out)
+{
+    if (c->eqb) memcpy(out, c->eqb, sizeof(zc) * ((size_t)20 * c->lq + 8));
+}
+
+static void eqt_bins(orc_ctx *c, const zc *G, zc zphi)
+{
+    const int N = c->ndim, lq = c->lq;
+    const double nf = (double)c->p.nflr;
+    zc *b = c->eqb;
+#define GF(i, j) G[IDX(i, j, N)]
+#define GC(i, j) (((i) == (j) ? 1.0 : 0.0) - G[IDX(j, i, N)])
+    /* measure_cpcm, measure_spsm, measure_nn, measure_paircorr's on-site part (:265-318, :484-499) */
+    for (int j = 0; j < N; j++) {
+        b[20 * lq + c->sub[j]] += GC(j, j) * zphi;
+        for (int i = 0; i < N; i++) {
+            size_t k = imj_of(c, i, j) + (size_t)lq * (c->sub[i] + 2 * c->sub[j]);
+            zc gc = GC(i, j), g = GF(i, j);
+            b[k] += gc * zphi;
+            b[4 * lq + k] += gc * g * (1.0 - 1.0 / (nf * nf)) * zphi;
+            b[8 * lq + k] += (GC(i, i) * GC(j, j) + gc * g / nf) * zphi;
+            b[12 * lq + k] += g * g * zphi;
+        }
+    }
+    /* measure_jj, measure_bondcorr, measure_paircorr's bond parts (:367-416, :418-482, :500-543);
+     * expar = 1 (no field, flux or dimerisation) */
+    for (int v = 0; v < lq; v++) {
+        int j0 = c->ca[v], jj = c->cjj[v], jn = c->cnn[v], js = c->csn[v];
+        c->eqb[20 * lq + 3] += (GC(j0, jn) + GC(jn, j0)) * zphi;
+        for (int u = 0; u < lq; u++) {
+            int i0 = c->ca[u], ij = c->cjj[u], in = c->cnn[u], is = c->csn[u];
+            int imj = imj_of(c, i0, j0);
+            zc z1 = (GC(i0, ij) - GC(ij, i0)) * (GC(j0, jj) - GC(jj, j0));
+            zc z3 = GC(i0, jj) * GF(ij, j0) + GC(ij, j0) * GF(i0, jj) - GC(i0, j0) * GF(ij, jj) - GC(ij, jj) * GF(i0, j0);
+            b[16 * lq + imj] += (-z1 - z3 / nf) * zphi;
+            zc bb = (GC(j0, jn) + GC(jn, j0)) * (GC(i0, in) + GC(in, i0)) +
+                    (GC(i0, jn) * GF(in, j0) + GC(in, j0) * GF(i0, jn) + GC(i0, j0) * GF(in, jn) + GC(in, jn) * GF(i0, j0)) / nf;
+            b[17 * lq + imj] += bb * zphi;
+            b[18 * lq + imj] += GF(i0, j0) * GF(in, jn) * zphi;
+            b[19 * lq + imj] += GF(i0, j0) * GF(is, js) * zphi;
+        }
+    }
+    for (int u = 0; u < lq; u++) {
+        int i0 = c->ca[u], ij = c->cjj[u];
+        b[20 * lq + 2] += I * (GC(i0, ij) - GC(ij, i0)) * zphi;
+    }
+    /* zqsq, ztsq, ztq (:143-263) over the plaquettes; zint_v (:127-141) over the bonds */
+    zc qsq = 0, tsq = 0, tq = 0, iv = 0;
+    const int zp = c->zplq;
+    for (int u = 0; u < lq && zp; u++) {
+        const int *q = c->plq + (size_t)u * zp;
+        zc dq = 0, dt = 0;
+        for (int s1 = 0; s1 < zp; s1++) {
+            int i1 = q[s1], i1a = q[(s1 + 1) % zp];
+            zc z1 = ((s1 + 1) % 2 == 0) ? -cexp(I * c->theta) : cexp(-I * c->theta);
+            dq += GC(i1, i1);
+            dt += 2.0 * creal(z1 * GC(i1, i1a));
+        }
+        qsq += dq * dq * nf;
+        tsq += dt * dt * nf;
+        tq += dt * (dq / 3.0) * nf;
+        for (int s1 = 0; s1 < zp; s1++) {
+            int i1 = q[s1], i1a = q[(s1 + 1) % zp];
+            zc z1 = ((s1 + 1) % 2 == 0) ? -cexp(I * c->theta) : cexp(-I * c->theta), z2 = conj(z1);
+            for (int s2 = 0; s2 < zp; s2++) {
+                int i2 = q[s2], i2a = q[(s2 + 1) % zp];
+                zc z3 = ((s2 + 1) % 2 == 0) ? -cexp(I * c->theta) : cexp(-I * c->theta), z4 = conj(z3);
+                qsq += GC(i1, i2) * GF(i1, i2);
+                tsq += z1 * z3 * GC(i1, i2a) * GF(i1a, i2) + z1 * z4 * GC(i1, i2) * GF(i1a, i2a) +
+                       z2 * z3 * GC(i1a, i2a) * GF(i1, i2) + z2 * z4 * GC(i1a, i2) * GF(i1, i2a);
+                tq += (z1 / 3.0) * GC(i1, i2) * GF(i1a, i2) + (z2 / 3.0) * GC(i1a, i2) * GF(i1, i2);
+            }
+        }
+    }
+    for (int bnd = 0; bnd < c->nn_lf; bnd++)
+        for (int f = 0; f < c->nn_nf; f++) {
+            int i0 = c->nnlf[bnd], in = c->nnlist[i0 * c->nn_nf + f];
+            iv += creal(GC(i0, in));
+        }
+    b[20 * lq + 4] += qsq * zphi;
+    b[20 * lq + 5] += tsq * zphi;
+    b[20 * lq + 6] += tq * zphi;
+    b[20 * lq + 7] += iv * zphi;
+#undef GF
+#undef GC
+}
+#undef NPBC
+
 /* the staggering of site s: + on even x + y (square), x (chain), x + y + z (cubic), on sublattice A
  * (honeycomb) */
 static double stag_of(const orc_ctx *c, int s)
@@ -1506,6 +1682,7 @@ static void measure(orc_ctx *c)
     c->obs[2] += zk * zphi;
     c->obs[3] += zu * zphi;
     c->obs[4] += ss * zphi;
+    if (c->eqt) eqt_bins(c, G, zphi);
 }
 
 static void record_wrap(orc_ctx *c, const zc *gscr, zc logw_tmp, int info)
@@ -1853,7 +2030,12 @@ void orc_get_obs(const orc_ctx *c, double *o)
     o[0] = c->nobs;
     for (int k = 0; k < 5; k++) { o[1 + 2 * k] = creal(c->obs[k]); o[2 + 2 * k] = cimag(c->obs[k]); }
 }
-void orc_reset_obs(orc_ctx *c) { c->nobs = 0; for (int k = 0; k < 5; k++) c->obs[k] = 0; }
+void orc_reset_obs(orc_ctx *c)
+{
+    c->nobs = 0;
+    for (int k = 0; k < 5; k++) c->obs[k] = 0;
+    if (c->eqb) memset(c->eqb, 0, sizeof(zc) * ((size_t)20 * c->lq + 8));
+}
 void orc_set_trace(orc_ctx *c, double *buf, int cap) { c->trace = buf; c->trace_cap = cap; c->trace_n = 0; }
 int orc_trace_count(const orc_ctx *c) { return c->trace_n; }
 

@@ -71,6 +71,12 @@ void orc_get_proposal_stats(const orc_ctx *c, double *stats);
 void orc_set_trace(orc_ctx *c, double *buf, int cap);
 /* time-displaced measurement (ltau): green_tau, the 0b pass without updates, znn_tau bins */
 int orc_set_dyn(orc_ctx *c, int ntdm);
+/* correlation bins of equaltime_measure (zcpcm, zspsm, znn, pair_onsite, zjj, zbb, pair_nn,
+ * pair_sn, zn, zj, zb, zqsq, ztsq, ztq, zint_v), accumulated at every measured slice from now on;
+ * theta in radians (model_para theta * pi).  Layout of orc_get_eqt: 20 lq + 8 complex, see the
+ * source. */
+int orc_set_eqt(orc_ctx *c, double theta);
+void orc_get_eqt(const orc_ctx *c, double *out);
 int orc_sweep_0b_dyn(orc_ctx *c, int lmeasure);
 void orc_get_gtau(const orc_ctx *c, double *gt0, double *g0t, double *g00);
 void orc_get_znn(const orc_ctx *c, double *znn);

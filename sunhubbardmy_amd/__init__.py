@@ -3,3 +3,4 @@ Fortran package YuanyueChen/sunhubbardmy.  Compute runs in libsunhip.so (hand-wr
 gfx950); this package builds the model tables and drives the C ABI."""
 from .model import Model, chain_seed, make_tables  # noqa: F401
 from .walkers import FTWalkers, SweepError  # noqa: F401
+from . import binio  # noqa: F401

@@ -6,6 +6,7 @@
 #include "pqmc_kernels.hip"
 #include "learn_kernels.hip"
 #include "dyn_kernels.hip"
+#include "eqt_kernels.hip"
 #include "../../include/sunhip.h"
 #include "../../include/sunhip_learn.h"
 
@@ -64,6 +65,7 @@ struct sh_ctx : ctx_base {
     bool started = false;
     bool legacy_update = false;   // test hook: the L2-resident delay update kernel
     int ntdm = -1;                // time-displaced measurement: bins 0 .. ntdm, -1 = off
+    size_t eqt_lds = 0;           // LDS of k_eqt, 0 while no chain has correlation bins
     // push/pop_stage backups
     std::vector<void *> stage;
     std::vector<cplx *> trace_bufs;
@@ -503,6 +505,7 @@ static void measure(sh_ctx *ctx)
     unsigned nb = (ctx->nmax + 31) / 32;
     hipLaunchKernelGGL(k_measure_tiles, dim3(nb * (nb + 1) / 2, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dc);
     hipLaunchKernelGGL(k_measure, dim3(ctx->nch), dim3(256), 0, ctx->st, ctx->dc);
+    if (ctx->eqt_lds) hipLaunchKernelGGL(k_eqt, dim3(ctx->nch), dim3(256), ctx->eqt_lds, ctx->st, ctx->dc);
 }
 
 static int check_launch(sh_ctx *ctx)
@@ -721,7 +724,7 @@ int sh_set_dyn(sh_ctx *ctx, int chain, int ntdm, int l1, int l2, int l3, const i
         c.gt0s = dalloc<cplx>(ctx, nn); c.g0ts = dalloc<cplx>(ctx, nn);
         c.znn = dalloc<cplx>(ctx, (size_t)4 * c.lq * (ctx->ltrot + 1));
         c.dynerr = dalloc<double>(ctx, 2);
-        c.sxy = dalloc<int32_t>(ctx, (size_t)4 * c.n);
+        if (!c.sxy) c.sxy = dalloc<int32_t>(ctx, (size_t)4 * c.n);
         if (!c.gt0 || !c.g0t || !c.g00 || !c.gt0s || !c.g0ts || !c.znn || !c.dynerr || !c.sxy)
             return fail(ctx, SH_E_HIP, "device allocation failed");
     }
@@ -733,6 +736,61 @@ int sh_set_dyn(sh_ctx *ctx, int chain, int ntdm, int l1, int l2, int l3, const i
     c.ltau = 1;
     ctx->ntdm = ntdm;
     HIPCHK(hipMemcpy(ctx->dc + chain, &c, sizeof(DChain), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// LDS bytes of k_eqt for a chain
+static size_t eqt_lds(const DChain &c) { return sizeof(cplx) * ((size_t)16 * c.lq + 2 * EQT_T * (EQT_T + 1) + c.n); }
+
+int sh_set_eqt(sh_ctx *ctx, int chain, int l1, int l2, int l3, const int32_t *sites, const int32_t *cells, int zplq,
+               const int32_t *plq, double theta)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !sites || !cells || l1 < 1 || l2 < 1 || l3 < 1 || zplq < 0 ||
+        (zplq > 0 && !plq))
+        return SH_E_ARG;
+    DChain &c = ctx->hc[chain];
+    if (l1 * l2 * l3 != c.lq) return fail(ctx, SH_E_ARG, "l1 * l2 * l3 must be the number of unit cells");
+    if (eqt_lds(c) > 160 * 1024) return fail(ctx, SH_E_ARG, "the correlation bins need 16 lq + n + 544 <= 10240");
+    for (int i = 0; i < c.n; i++)
+        if (sites[4 * i] < 1 || sites[4 * i] > l1 || sites[4 * i + 1] < 1 || sites[4 * i + 1] > l2 ||
+            sites[4 * i + 2] < 1 || sites[4 * i + 2] > l3 || sites[4 * i + 3] < 0 || sites[4 * i + 3] > 1)
+            return fail(ctx, SH_E_ARG, "site cell coordinates out of 1..l1, 1..l2, 1..l3 or orbital out of 0..1");
+    for (int k = 0; k < 4 * c.lq; k++)
+        if (cells[k] < 0 || cells[k] >= c.n) return fail(ctx, SH_E_ARG, "cell partner site out of range");
+    for (int k = 0; k < zplq * c.lq; k++)
+        if (plq[k] < 0 || plq[k] >= c.n) return fail(ctx, SH_E_ARG, "plaquette site out of range");
+    if (!c.sxy) c.sxy = dalloc<int32_t>(ctx, (size_t)4 * c.n);
+    if (!c.eqb) c.eqb = dalloc<cplx>(ctx, (size_t)20 * c.lq + 8);
+    if (!c.eqc) c.eqc = dalloc<int32_t>(ctx, (size_t)4 * c.lq);
+    if (zplq > 0 && c.zplq != zplq) c.plq = dalloc<int32_t>(ctx, (size_t)zplq * c.lq);
+    if (!c.sxy || !c.eqb || !c.eqc || (zplq > 0 && !c.plq)) return fail(ctx, SH_E_HIP, "device allocation failed");
+    HIPCHK(hipMemcpy((void *)c.sxy, sites, sizeof(int32_t) * 4 * c.n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((void *)c.eqc, cells, sizeof(int32_t) * 4 * c.lq, hipMemcpyHostToDevice));
+    if (zplq > 0) HIPCHK(hipMemcpy((void *)c.plq, plq, sizeof(int32_t) * zplq * c.lq, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c.eqb, 0, sizeof(cplx) * (20 * (size_t)c.lq + 8)));
+    c.dl1 = l1;
+    c.dl2 = l2;
+    c.dl3 = l3;
+    c.zplq = zplq;
+    c.theta = theta;
+    size_t lds = 0;
+    for (const DChain &d : ctx->hc)
+        if (d.eqb || &d == &c) lds = std::max(lds, eqt_lds(d));
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void *)k_eqt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return fail(ctx, SH_E_HIP, "k_eqt: LDS request refused");
+    ctx->eqt_lds = lds;
+    HIPCHK(hipMemcpy(ctx->dc + chain, &c, sizeof(DChain), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int sh_get_eqt(sh_ctx *ctx, int chain, double *bins)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !bins) return SH_E_ARG;
+    const DChain &c = ctx->hc[chain];
+    if (!c.eqb) return fail(ctx, SH_E_STATE, "correlation bins not set up (sh_set_eqt)");
+    HIPCHK(hipMemcpyAsync(bins, c.eqb, sizeof(cplx) * (20 * (size_t)c.lq + 8), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
     return 0;
 }
 
@@ -1084,6 +1142,7 @@ int sh_reset_obs(sh_ctx *ctx)
     if (!ctx) return SH_E_ARG;
     for (auto &c : ctx->hc) {
         HIPCHK(hipMemsetAsync(c.sc + SC_NOBS, 0, 11 * sizeof(double), ctx->st));
+        if (c.eqb) HIPCHK(hipMemsetAsync(c.eqb, 0, sizeof(cplx) * (20 * (size_t)c.lq + 8), ctx->st));
     }
     HIPCHK(hipStreamSynchronize(ctx->st));
     return 0;

@@ -103,6 +103,12 @@ struct DChain {
     double *dynerr;       // largest |propagated - green_tau| of G(t,0), G(0,t)
     int dl1, dl2, dl3, ntdm, ltau;
     double kin_weight;    // factor of the zkint bin (nflr * rt for the chain, else 1)
+    // correlation bins of the equal-time measurement (null: off): 20 lq + 8 complex; per unit
+    // cell the A site, nnlist(., 1), snlist(., 1) and the current partner; plaquette site cycles
+    cplx *eqb;
+    const int32_t *eqc, *plq;
+    int zplq, pad3;
+    double theta;
 };
 
 __device__ __forceinline__ cplx cmk(double r, double i) { return make_double2(r, i); }

@@ -69,6 +69,22 @@ module sunhip_bridge
        integer(c_int), value :: chain
        integer(c_int32_t), intent(out) :: conf(*)
      end function sh_get_conf
+     ! correlation bins of equaltime_measure: cells(4, lq) = A site, nnlist(., 1), snlist(., 1),
+     ! current partner (0-based); plq(zplq, lq) plaquette cycles; bins(2, 20 lq + 8)
+     integer(c_int) function sh_set_eqt(ctx, chain, l1, l2, l3, sites, cells, zplq, plq, theta) &
+          bind(C, name="sh_set_eqt")
+       import :: c_ptr, c_int, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain, l1, l2, l3, zplq
+       integer(c_int32_t), intent(in) :: sites(*), cells(*), plq(*)
+       real(c_double), value :: theta
+     end function sh_set_eqt
+     integer(c_int) function sh_get_eqt(ctx, chain, bins) bind(C, name="sh_get_eqt")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       real(c_double), intent(out) :: bins(*)
+     end function sh_get_eqt
      ! ltau: sites(4, ndim) = cell x (1..l1), y (1..l2), z (1..l3), orbital (0, 1) per site
      integer(c_int) function sh_set_dyn(ctx, chain, ntdm, l1, l2, l3, sites) bind(C, name="sh_set_dyn")
        import :: c_ptr, c_int, c_int32_t

@@ -55,6 +55,43 @@ def _npbc(v, l):
     return v
 
 
+def _eqt_tables(lattice, dims, nnlist):
+    """Per unit cell (latt%imj's cell order) the A site, nnlist(A, 1), snlist(A, 1) and the current
+    partner of measure_jj (snlist on the honeycomb, nnlist elsewhere: src/dqmc_measure/
+    equaltime_measure.f90:378-394); the plaquette site cycles latt%plq_cord (square
+    src/lattice.f90:954-957, honeycomb :767-772; none on the chain and the cube).  snlist(., 1):
+    (x+1, y+1) on the square (:887) and the cube (:1370), the A site of cell (x+1, y+1) on the
+    honeycomb (:710), x+2 on the chain (:309)."""
+    l1, l2, l3 = dims
+
+    def site(x, y, z, o=0):
+        x, y, z = (x - 1) % l1 + 1, (y - 1) % l2 + 1, (z - 1) % l3 + 1
+        if lattice == "honeycomb":
+            return 2 * ((x - 1) * l2 + y - 1) + o
+        if lattice == "chain":
+            return x - 1
+        if lattice == "cubic":
+            return (z - 1) * l1 * l2 + (x - 1) * l2 + y - 1
+        return (x - 1) * l2 + y - 1
+    if lattice == "cubic":
+        order = sorted(((x, y, z) for x in range(1, l1 + 1) for y in range(1, l2 + 1) for z in range(1, l3 + 1)),
+                       key=lambda c: site(*c))
+    else:
+        order = [(x, y, 1) for x in range(1, l1 + 1) for y in range(1, l2 + 1)]
+    cells, plq = [], []
+    for (x, y, z) in order:
+        a = site(x, y, z)
+        nn = int(nnlist[a, 0])
+        sn = site(x + 2, 1, 1) if lattice == "chain" else site(x + 1, y + 1, z)
+        cells.append((a, nn, sn, sn if lattice == "honeycomb" else nn))
+        if lattice == "square":
+            plq.append((site(x, y, 1), site(x + 1, y, 1), site(x + 1, y + 1, 1), site(x, y + 1, 1)))
+        elif lattice == "honeycomb":
+            plq.append((site(x, y, 1, 1), site(x, y, 1, 0), site(x, y - 1, 1, 1), site(x - 1, y - 1, 1, 0),
+                        site(x - 1, y - 1, 1, 1), site(x - 1, y, 1, 0)))
+    return np.array(cells, np.int32), np.array(plq, np.int32).reshape(len(order), -1)
+
+
 def lattice_tables(m):
     """Site order, checkerboard bond families and staggering (0-based site indices)."""
     l1, l2 = m.la, m.lb
@@ -163,7 +200,8 @@ def lattice_tables(m):
     dims = (l1, 1, 1) if m.lattice == "chain" else (l1, l2, l3) if m.lattice == "cubic" else (l1, l2, 1)
     # the chain's zkint carries nflr * rt (src/dqmc_measure/equaltime_measure1d.f90:76)
     kin_weight = float(m.nflr * m.rt) if m.lattice == "chain" else 1.0
-    return {"nsites": nsites, "lq": lq, "nn_nf": nn_nf, "nn_lf": nn_lf, "bonds": bonds,
+    eqt_cells, plq = _eqt_tables(m.lattice, dims, nnlist)
+    return {"eqt_cells": eqt_cells, "plq": plq,"nsites": nsites, "lq": lq, "nn_nf": nn_nf, "nn_lf": nn_lf, "bonds": bonds,
             "nnlist": nnlist, "stag": stag.astype(np.float64), "cells": cells, "dims": dims,
             "kin_weight": kin_weight}
 

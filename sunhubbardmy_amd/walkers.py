@@ -168,6 +168,25 @@ class FTWalkers:
             self._check(_lib.lib().sh_set_dyn(self._ctx, c, nd, l1, l2, l3, sites.ctypes.data_as(_lib.ip)), "set_dyn")
         self.ntdm = self.tables[0]["ltrot"] if ntdm is None else int(ntdm)
 
+    # -- correlation bins of the equal-time measurement --------------------------------------
+    def set_eqt(self, theta=0.0):
+        """Switch on zcpcm, zspsm, znn, pair_onsite, the bond correlations and the plaquette and
+        zint_v energy terms at every measured slice (theta in radians: model_para's theta * pi)."""
+        for c, t in enumerate(self.tables):
+            lat = t["lat"]
+            l1, l2, l3 = lat["dims"]
+            sites, cells, plq = _c(lat["cells"], np.int32), _c(lat["eqt_cells"], np.int32), _c(lat["plq"], np.int32)
+            self._check(_lib.lib().sh_set_eqt(self._ctx, c, l1, l2, l3, sites.ctypes.data_as(_lib.ip),
+                                              cells.ctypes.data_as(_lib.ip), plq.shape[1],
+                                              plq.ctypes.data_as(_lib.ip), float(theta)), "set_eqt")
+
+    def eqt(self, chain=0):
+        """the flat correlation bins of a chain, 20 lq + 8 complex (binio.unpack_eqt splits them)"""
+        lq = self.tables[chain]["lq"]
+        b = np.zeros(20 * lq + 8, np.complex128)
+        self._check(_lib.lib().sh_get_eqt(self._ctx, chain, b.ctypes.data_as(_lib.dp)), "get_eqt")
+        return b
+
     def dyn_obs(self, chain=0):
         """(znn_tau bins [ntdm+1][orbital j][orbital i][lq], {xmax_dyn, count})"""
         lq = self.tables[chain]["lq"]
