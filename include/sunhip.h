@@ -63,6 +63,12 @@ typedef struct sh_model {
     int32_t seed;           /* SFMT19937 seed of this chain (spring_sfmt_init) */
     double kin_weight;      /* factor of the zkint bin: 1 (0 is read as 1) in 2D and 3D, nflr * rt for
                                the chain (src/dqmc_measure/equaltime_measure1d.f90:76) */
+    /* the bond field of the t-V model (v0conf, rv != 0; NULL: none): vtab 8 complex (bmat_p,
+     * bmat_m, bmat_p_inv, bmat_m_inv of component 1, then 2), vdelta 4 complex (delta_bmat_p,
+     * delta_bmat_m)(1, is) for is = 1, 2 (src/dqmc_config_v/dqmc_set_conf.f90:29-68); the
+     * bonds are the (nn_nf, nn_lf) table above, also for the dense kinetic term */
+    const double *vtab;
+    const double *vdelta;
 } sh_model;
 
 typedef struct sh_ctx sh_ctx;
@@ -113,6 +119,14 @@ int sh_set_eqt(sh_ctx *ctx, int chain, int l1, int l2, int l3, const int32_t *si
  * q 4 lq + imj + lq (no_i + 2 no_j); zjj, zbb, pair_nn, pair_sn at (16 + q) lq + imj; then zn(1),
  * zn(2), zj, zb, zqsq (without its 1/9), ztsq, ztq, zint_v (without rv).  Cleared by sh_reset_obs. */
 int sh_get_eqt(sh_ctx *ctx, int chain, double *bins);
+
+/* The bond field of the t-V model (v0conf%conf(i, nf, nt), src/dqmc_config_v): int32
+ * [ltrot][nn_nf][ndim], values 1..2, entries i < nn_lf are the bonds (conf_v.in / conf_v.out);
+ * sh_init_conf draws it after the on-site field.  sh_get_vacc: accepted and tried bond flips
+ * (main_obs(4) of dqmc_update.f90:97). */
+int sh_set_confv(sh_ctx *ctx, int chain, const int32_t *conf);
+int sh_get_confv(sh_ctx *ctx, int chain, int32_t *conf);
+int sh_get_vacc(sh_ctx *ctx, int chain, double *acc);
 
 /* The chain's random-number state, SH_RNG_WORDS uint32: the SFMT19937 state that spring_sfmt
  * keeps in module storage (lib/m_spring.f90:66-71: 624 words, then the read index 0..624) and,

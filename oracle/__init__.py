@@ -38,6 +38,10 @@ def lib():
         L.orc_sizes.argtypes = [P, ip]
         L.orc_get_conf.argtypes = [P, ip]
         L.orc_set_conf.argtypes = [P, ip]
+        L.orc_get_confv.argtypes = [P, ip]
+        L.orc_set_confv.argtypes = [P, ip]
+        L.orc_get_vacc.argtypes = [P, dp]
+        L.orc_get_vacc.restype = None
         L.orc_sweep_start.argtypes = [P]
         L.orc_sweep_b0.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.orc_sweep_0b.argtypes = [P, ctypes.c_int, ctypes.c_int]
@@ -113,7 +117,7 @@ class OrcParams(ctypes.Structure):
         ("mu", ctypes.c_double), ("rhub", ctypes.c_double), ("nu", ctypes.c_double),
         ("nflr", ctypes.c_int), ("exact_hs", ctypes.c_int), ("kmode", ctypes.c_int),
         ("update", ctypes.c_int), ("nwrap", ctypes.c_int), ("nublock", ctypes.c_int),
-        ("seed", ctypes.c_int32), ("lc", ctypes.c_int),
+        ("seed", ctypes.c_int32), ("lc", ctypes.c_int), ("rv", ctypes.c_double),
     ]
 
 
@@ -147,13 +151,13 @@ class OracleChain:
 
     def __init__(self, lattice="square", la=4, lb=4, beta=4.0, dtau=0.1, rt=1.0, mu=0.0,
                  rhub=4.0, nu=0.0, nflr=2, exact_hs=True, kmode="checkerboard",
-                 update="delay", nwrap=10, nublock=8, seed=None, lc=1):
+                 update="delay", nwrap=10, nublock=8, seed=None, lc=1, rv=0.0):
         L = lib()
         if seed is None:
             seed = L.orc_chain_seed(0)
         self.params = OrcParams(LATTICES[lattice], la, lb, beta, dtau, rt, mu, rhub, nu, nflr,
                                 int(bool(exact_hs)), KMODES[kmode], UPDATES[update], nwrap,
-                                nublock, int(np.int32(seed)), lc)
+                                nublock, int(np.int32(seed)), lc, rv)
         self._c = L.orc_create(ctypes.byref(self.params))
         s = np.zeros(8, np.int32)
         L.orc_sizes(self._c, _iptr(s))
@@ -179,6 +183,23 @@ class OracleChain:
     def conf(self, value):
         c = np.ascontiguousarray(value, dtype=np.int32).reshape(self.ltrot, self.ndim)
         lib().orc_set_conf(self._c, _iptr(c))
+
+    @property
+    def confv(self):
+        """bond field [ltrot][nn_nf][nsites] (t-V model; entries i < nn_lf are the bonds)"""
+        c = np.zeros((self.ltrot, self.nn_nf, self.ndim), np.int32)
+        lib().orc_get_confv(self._c, _iptr(c))
+        return c
+
+    @confv.setter
+    def confv(self, value):
+        c = np.ascontiguousarray(value, dtype=np.int32).reshape(self.ltrot, self.nn_nf, self.ndim)
+        lib().orc_set_confv(self._c, _iptr(c))
+
+    def vacc(self):
+        a = np.zeros(2)
+        lib().orc_get_vacc(self._c, _dptr(a))
+        return a
 
     def sweep_start(self):
         rc = lib().orc_sweep_start(self._c)

@@ -468,6 +468,11 @@ struct orc_ctx {
     zc *eqb;
     int *ca, *cnn, *csn, *cjj;  /* per unit cell: A site, its nnlist(.,1), snlist(.,1), current partner */
     int *plq;                   /* [lq][zplq] plaquette site cycles */
+    /* the bond field of the t-V model (vconf with lprojv = .false.) */
+    int lwrapv;
+    zc vbp[2], vbm[2], vbpi[2], vbmi[2], vdp[2], vdm[2];
+    int32_t *confv;             /* [ltrot][nn_nf][ndim] */
+    double vacc[2];
 };
 
 /* src/lattice.f90:792-961 (square), :611-776 (honeycomb), :240-316 (chain), :1278-1400 (cubic);
@@ -762,6 +767,8 @@ static void setup_kinetic(orc_ctx *c)
     }
 }
 
+static void setup_vfield(orc_ctx *c);
+
 orc_ctx *orc_create(const orc_params *p)
 {
     orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
@@ -777,6 +784,12 @@ orc_ctx *orc_create(const orc_params *p)
     c->prop_local = 1;
     int N = c->ndim;
     c->conf = (int32_t *)calloc((size_t)N * c->ltrot, sizeof(int32_t));
+    c->lwrapv = (p->rv != 0.0);
+    if (c->lwrapv) {
+        setup_vfield(c);
+        c->confv = (int32_t *)malloc(sizeof(int32_t) * (size_t)N * c->nn_nf * c->ltrot);
+        for (size_t k = 0; k < (size_t)N * c->nn_nf * c->ltrot; k++) c->confv[k] = 1;
+    }
     for (size_t k = 0; k < (size_t)N * c->ltrot; k++) c->conf[k] = 1;
     c->G = zalloc((size_t)N * N);
     c->Gtmp = zalloc((size_t)N * N);
@@ -802,6 +815,7 @@ void orc_destroy(orc_ctx *c)
     free(c->urt); free(c->urtm1);
     free(c->nnlf); free(c->nnlist); free(c->cx); free(c->cy); free(c->cz); free(c->sub);
     free(c->eqb); free(c->ca); free(c->cnn); free(c->csn); free(c->cjj); free(c->plq);
+    free(c->confv);
     free(c->iwrap_nt); free(c->wrap_step);
     free(c->prop_nbr); free(c->prop_w);
     free(c->gt0); free(c->g0t); free(c->g00); free(c->znn);
@@ -817,10 +831,121 @@ void orc_sizes(const orc_ctx *c, int *out)
 /* dqmc_input_conf.f90:55-63 (single rank): conf(i,nt) = ceiling(u*lcomp), nt outer, i inner */
 void orc_init_conf(orc_ctx *c)
 {
-    if (!c->lwrapu) return;
-    for (int nt = 0; nt < c->ltrot; nt++)
-        for (int i = 0; i < c->ndim; i++)
-            c->conf[(size_t)nt * c->ndim + i] = (int32_t)ceil(sfmt_stream(&c->rng) * c->lcomp);
+    if (c->lwrapu)
+        for (int nt = 0; nt < c->ltrot; nt++)
+            for (int i = 0; i < c->ndim; i++)
+                c->conf[(size_t)nt * c->ndim + i] = (int32_t)ceil(sfmt_stream(&c->rng) * c->lcomp);
+    /* v0conf%input_conf after u0conf%input_conf (src/dqmc_ft_ctrl.f90:25-35): conf(i, nf, nt),
+     * i fastest, two components */
+    if (c->lwrapv)
+        for (size_t k = 0; k < (size_t)c->ndim * c->nn_nf * c->ltrot; k++)
+            c->confv[k] = (int32_t)ceil(sfmt_stream(&c->rng) * 2);
+}
+void orc_get_confv(const orc_ctx *c, int32_t *conf)
+{
+    if (c->confv) memcpy(conf, c->confv, sizeof(int32_t) * (size_t)c->ndim * c->nn_nf * c->ltrot);
+}
+void orc_set_confv(orc_ctx *c, const int32_t *conf)
+{
+    if (c->confv) memcpy(c->confv, conf, sizeof(int32_t) * (size_t)c->ndim * c->nn_nf * c->ltrot);
+}
+void orc_get_vacc(const orc_ctx *c, double *acc) { acc[0] = c->vacc[0]; acc[1] = c->vacc[1]; }
+
+/* ---- the bond field of the finite-temperature t-V model ------------------------------- */
+/* vconf%set_conf with lprojv = .false. (src/dqmc_config_v/dqmc_set_conf.f90:14-77): alpha =
+ * acosh(exp(dtau V / 2)), eta = (-1, +1), bmat_p = exp(alpha eta), bmat_m = exp(-alpha eta),
+ * delta for the one flip iflip = 1 */
+static void setup_vfield(orc_ctx *c)
+{
+    double al = acosh(exp(c->p.dtau * c->p.rv * 0.5));
+    const double eta[2] = {-1.0, 1.0};
+    for (int is = 0; is < 2; is++) {
+        int isp = (is + 1) % 2;
+        c->vbp[is] = exp(al * eta[is]);
+        c->vbpi[is] = exp(-al * eta[is]);
+        c->vbm[is] = exp(-al * eta[is]);
+        c->vbmi[is] = exp(al * eta[is]);
+        c->vdp[is] = exp(al * (eta[isp] - eta[is])) - 1.0;
+        c->vdm[is] = exp(-al * (eta[isp] - eta[is])) - 1.0;
+    }
+}
+#define VS 0.70710678118654752440   /* u = ut = [[1, 1], [1, -1]] / sqrt(2) (dqmc_config_v.f90:19-20) */
+static int fv_is(const orc_ctx *c, int nt, int nf, int q) { return c->confv[((size_t)(nt - 1) * c->nn_nf + nf) * c->ndim + q] - 1; }
+/* rows i1, i2 of every bond of family nf:
+ *   mode 0: left_forward nflag 2 (dqmc_config_v/dqmc_left_forward_prop.f90:24-38): rows <- diag(bp, bm) ut rows
+ *   mode 1: left_forward nflag 1 / left_backward nflag 2: rows <- u rows
+ *   mode 2: left_backward nflag 1 (dqmc_left_backward_prop.f90:39-55): rows <- u diag(bpi, bmi) rows */
+static void fv_left(orc_ctx *c, int nt, int nf, int mode, zc *M)
+{
+    int N = c->ndim;
+    for (int q = 0; q < c->nn_lf; q++) {
+        int i1 = c->nnlf[q], i2 = c->nnlist[i1 * c->nn_nf + nf], is = fv_is(c, nt, nf, q);
+        for (int j = 0; j < N; j++) {
+            zc x = M[IDX(i1, j, N)], y = M[IDX(i2, j, N)];
+            if (mode == 2) { x *= c->vbpi[is]; y *= c->vbmi[is]; }
+            zc v1 = VS * (x + y), v2 = VS * (x - y);
+            if (mode == 0) { v1 *= c->vbp[is]; v2 *= c->vbm[is]; }
+            M[IDX(i1, j, N)] = v1;
+            M[IDX(i2, j, N)] = v2;
+        }
+    }
+}
+/* columns i1, i2:
+ *   mode 0: right_backward nflag 2 (dqmc_right_backward_prop.f90:24-38): cols <- (cols u) diag(bpi, bmi)
+ *   mode 1: right_backward nflag 1 / right_forward nflag 2: cols <- cols u
+ *   mode 2: right_forward nflag 1 (dqmc_right_forward_prop.f90:40-56): cols <- (cols diag(bp, bm)) ut */
+static void fv_right(orc_ctx *c, int nt, int nf, int mode, zc *M)
+{
+    int N = c->ndim;
+    for (int q = 0; q < c->nn_lf; q++) {
+        int i1 = c->nnlf[q], i2 = c->nnlist[i1 * c->nn_nf + nf], is = fv_is(c, nt, nf, q);
+        for (int j = 0; j < N; j++) {
+            zc x = M[IDX(j, i1, N)], y = M[IDX(j, i2, N)];
+            if (mode == 2) { x *= c->vbp[is]; y *= c->vbm[is]; }
+            zc v1 = VS * (x + y), v2 = VS * (x - y);
+            if (mode == 0) { v1 *= c->vbpi[is]; v2 *= c->vbmi[is]; }
+            M[IDX(j, i1, N)] = v1;
+            M[IDX(j, i2, N)] = v2;
+        }
+    }
+}
+/* vconf%update (src/dqmc_config_v/dqmc_update.f90:35-96): per bond of family nf in order, a flip
+ * with ratio det(1 + (1 - G)_bb diag(dp, dm)) (no flavour power, no phase factor), accepted on
+ * |Re(ratio phase)| / |Re(phase)| > u; G <- G - G_{:,b} diag(dp, dm) M^-1 (E_b - G_{b,:}) */
+static void fv_update(orc_ctx *c, int nt, int nf)
+{
+    int N = c->ndim;
+    zc *G = c->G;
+    for (int q = 0; q < c->nn_lf; q++) {
+        int i1 = c->nnlf[q], i2 = c->nnlist[i1 * c->nn_nf + nf], is = fv_is(c, nt, nf, q);
+        int iflip = (int)ceil(sfmt_stream(&c->rng) * 1.0);
+        int isp = (is + iflip) % 2;
+        zc dp = c->vdp[is], dm = c->vdm[is];
+        zc m11 = 1.0 + (1.0 - G[IDX(i1, i1, N)]) * dp, m12 = -G[IDX(i1, i2, N)] * dm;
+        zc m21 = -G[IDX(i2, i1, N)] * dp, m22 = 1.0 + (1.0 - G[IDX(i2, i2, N)]) * dm;
+        zc ratio = m11 * m22 - m12 * m21;
+        double rr = creal(ratio * c->phase) / creal(c->phase);
+        double random = sfmt_stream(&c->rng);
+        c->vacc[1] += 1.0;
+        if (fabs(rr) > random) {
+            c->vacc[0] += 1.0;
+            c->phase *= ratio / cabs(ratio);
+            /* S = diag(dp, dm) M^-1 */
+            zc s11 = dp * m22 / ratio, s12 = -dp * m12 / ratio, s21 = -dm * m21 / ratio, s22 = dm * m11 / ratio;
+            zc *r1 = zalloc(N), *r2 = zalloc(N), *c1 = zalloc(N), *c2 = zalloc(N);
+            for (int j = 0; j < N; j++) {
+                zc a = (j == i1 ? 1.0 : 0.0) - G[IDX(i1, j, N)], b = (j == i2 ? 1.0 : 0.0) - G[IDX(i2, j, N)];
+                r1[j] = -(s11 * a + s12 * b);
+                r2[j] = -(s21 * a + s22 * b);
+                c1[j] = G[IDX(j, i1, N)];
+                c2[j] = G[IDX(j, i2, N)];
+            }
+            for (int j = 0; j < N; j++)
+                for (int i = 0; i < N; i++) G[IDX(i, j, N)] += c1[i] * r1[j] + c2[i] * r2[j];
+            free(r1); free(r2); free(c1); free(c2);
+            c->confv[((size_t)(nt - 1) * c->nn_nf + nf) * N + q] = isp + 1;
+        }
+    }
 }
 void orc_get_conf(const orc_ctx *c, int32_t *conf) { memcpy(conf, c->conf, sizeof(int32_t) * (size_t)c->ndim * c->ltrot); }
 void orc_set_conf(orc_ctx *c, const int32_t *conf) { memcpy(c->conf, conf, sizeof(int32_t) * (size_t)c->ndim * c->ltrot); }
@@ -906,6 +1031,8 @@ static void bmat_left_forward(orc_ctx *c, int nt1, int nt2, zc *M)
 {
     for (int nt = nt2; nt <= nt1; nt++) {
         khalf_left(c, c->urt, M, 0);
+        /* bond field, families 1..nn_nf, nflag 2 then 1 (src/dqmc_ft_core.f90:98-104) */
+        for (int nf = 0; nf < c->nn_nf && c->lwrapv; nf++) { fv_left(c, nt, nf, 0, M); fv_left(c, nt, nf, 1, M); }
         v_rows(c, nt, c->bmat, 0, M);
         khalf_left(c, c->urt, M, 1);
     }
@@ -915,6 +1042,9 @@ static void bmat_left_forward_hc(orc_ctx *c, int nt1, int nt2, zc *M)
 {
     for (int nt = nt1; nt >= nt2; nt--) {
         khalf_left(c, c->urt, M, 0);
+        /* families nn_nf..1 (src/dqmc_ft_core.f90:150-156); the real bond factors are their own
+         * conjugates */
+        for (int nf = c->nn_nf - 1; nf >= 0 && c->lwrapv; nf--) { fv_left(c, nt, nf, 0, M); fv_left(c, nt, nf, 1, M); }
         v_rows(c, nt, c->bmat, 1, M);
         khalf_left(c, c->urt, M, 1);
     }
@@ -1727,6 +1857,14 @@ int orc_sweep_b0(orc_ctx *c, int lupdate, int lmeasure)
             v_cols(c, nt, c->bmat, c->G);
             v_rows(c, nt, c->bmat_inv, 0, c->G);
         }
+        /* bond field, families nn_nf..1 (src/dqmc_ft_core/dqmc_ft_sweep_b0.f90:78-89) */
+        for (int nf = c->nn_nf - 1; nf >= 0 && c->lwrapv; nf--) {
+            fv_right(c, nt, nf, 1, c->G);
+            fv_left(c, nt, nf, 1, c->G);
+            if (lupdate) fv_update(c, nt, nf);
+            fv_right(c, nt, nf, 2, c->G);
+            fv_left(c, nt, nf, 2, c->G);
+        }
         if (c->lwrapT) {
             int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
             for (int q = nfam - 1; q >= 0; q--) { k_right(c, q, c->urt, c->G); k_left(c, q, c->urtm1, c->G); }
@@ -1763,6 +1901,14 @@ int orc_sweep_0b(orc_ctx *c, int lupdate, int lmeasure)
         if (c->lwrapT) {
             int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
             for (int q = 0; q < nfam; q++) { k_left(c, q, c->urt, c->G); k_right(c, q, c->urtm1, c->G); }
+        }
+        /* bond field, families 1..nn_nf (src/dqmc_ft_core/dqmc_ft_sweep_0b.f90:95-106) */
+        for (int nf = 0; nf < c->nn_nf && c->lwrapv; nf++) {
+            fv_left(c, nt, nf, 0, c->G);
+            fv_right(c, nt, nf, 0, c->G);
+            if (lupdate) fv_update(c, nt, nf);
+            fv_left(c, nt, nf, 1, c->G);
+            fv_right(c, nt, nf, 1, c->G);
         }
         if (c->lwrapu) {
             v_rows(c, nt, c->bmat, 0, c->G);
@@ -1804,7 +1950,7 @@ void orc_get_g(const orc_ctx *c, double *g) { memcpy(g, c->G, sizeof(zc) * (size
  * znn_tau(lq, 2, 2, ntdm+1) (src/dqmc_measure_2d.f90:30, :61) */
 int orc_set_dyn(orc_ctx *c, int ntdm)
 {
-    if (ntdm < 0 || ntdm > c->ltrot) return -1;
+    if (ntdm < 0 || ntdm > c->ltrot || c->lwrapv) return -1;
     size_t nn = (size_t)c->ndim * c->ndim;
     free(c->gt0); free(c->g0t); free(c->g00); free(c->znn);
     c->gt0 = zalloc(nn); c->g0t = zalloc(nn); c->g00 = zalloc(nn);

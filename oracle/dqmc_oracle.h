@@ -29,6 +29,7 @@ typedef struct orc_params {
     int nwrap, nublock;
     int seed;       /* SFMT19937 seed, int32 */
     int lc;         /* third linear size (cubic) */
+    double rv;      /* nearest-neighbour V: the bond field of the finite-temperature t-V model (0: off) */
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;
@@ -76,6 +77,11 @@ int orc_set_dyn(orc_ctx *c, int ntdm);
  * theta in radians (model_para theta * pi).  Layout of orc_get_eqt: 20 lq + 8 complex, see the
  * source. */
 int orc_set_eqt(orc_ctx *c, double theta);
+/* the bond field conf(i, nf, nt) of the finite-temperature t-V model, [ltrot][nn_nf][nsites]
+ * (entries i < nn_lf are the bonds); acceptance sums (accepted, tried) of its updates */
+void orc_get_confv(const orc_ctx *c, int32_t *conf);
+void orc_set_confv(orc_ctx *c, const int32_t *conf);
+void orc_get_vacc(const orc_ctx *c, double *acc);
 void orc_get_eqt(const orc_ctx *c, double *out);
 int orc_sweep_0b_dyn(orc_ctx *c, int lmeasure);
 void orc_get_gtau(const orc_ctx *c, double *gt0, double *g0t, double *g00);

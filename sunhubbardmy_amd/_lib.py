@@ -23,7 +23,7 @@ class ShModel(ctypes.Structure):
         ("bmat", dp), ("bmat_inv", dp), ("phase_arg", dp), ("delta_bmat", dp), ("phase_ratio", dp),
         ("gaml", dp), ("phase_from_conf", ctypes.c_int), ("stag", dp),
         ("update", ctypes.c_int), ("nublock", ctypes.c_int), ("seed", ctypes.c_int32),
-        ("kin_weight", ctypes.c_double),
+        ("kin_weight", ctypes.c_double), ("vtab", dp), ("vdelta", dp),
     ]
 
 
@@ -31,7 +31,7 @@ EXPORTS = [
     "sh_create", "sh_destroy", "sh_last_error", "sh_init_conf", "sh_set_conf", "sh_get_conf",
     "sh_sweep_start_0b", "sh_sweep_b0", "sh_sweep_0b", "sh_push_stage", "sh_pop_stage",
     "sh_get_g", "sh_get_diag", "sh_reset_diag", "sh_get_obs", "sh_get_obs_all", "sh_reset_obs", "sh_set_trace",
-    "sh_get_trace", "sh_timers", "sh_synchronize", "sh_get_rng_state", "sh_set_rng_state", "sh_set_dyn", "sh_set_eqt", "sh_get_eqt",
+    "sh_get_trace", "sh_timers", "sh_synchronize", "sh_get_rng_state", "sh_set_rng_state", "sh_set_dyn", "sh_set_eqt", "sh_get_eqt", "sh_set_confv", "sh_get_confv", "sh_get_vacc",
     "sh_get_dyn_obs", "sh_reset_dyn_obs", "sh_get_gtau",
 ]
 
@@ -94,6 +94,9 @@ def lib():
     L.sh_set_eqt.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, ip, ctypes.c_int, ip,
                              ctypes.c_double]
     L.sh_get_eqt.argtypes = [P, ctypes.c_int, dp]
+    L.sh_set_confv.argtypes = [P, ctypes.c_int, ip]
+    L.sh_get_confv.argtypes = [P, ctypes.c_int, ip]
+    L.sh_get_vacc.argtypes = [P, ctypes.c_int, dp]
     L.sh_get_dyn_obs.argtypes = [P, ctypes.c_int, dp, dp]
     L.sh_reset_dyn_obs.argtypes = [P]
     L.sh_get_gtau.argtypes = [P, ctypes.c_int, dp, dp, dp]

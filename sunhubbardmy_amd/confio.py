@@ -95,6 +95,8 @@ def save_state(path, walkers):
     d["conf"] = np.stack([walkers.get_conf(c) for c in range(walkers.nchains)])
     if getattr(walkers, "has_uconf", False):
         d["uconf"] = np.stack([walkers.get_uconf(c) for c in range(walkers.nchains)])
+    if type(walkers).__name__ == "FTWalkers" and walkers.has_vconf:
+        d["confv"] = np.stack([walkers.get_confv(c) for c in range(walkers.nchains)])
     np.savez(path, **d)
 
 
@@ -110,4 +112,6 @@ def load_state(path, walkers):
                 walkers.set_conf(c, z["conf"][c])
             if "uconf" in z:
                 walkers.set_uconf(c, z["uconf"][c])
+            if "confv" in z:
+                walkers.set_confv(z["confv"][c], c)
             walkers.set_rng_state(c, z["rng"][c])

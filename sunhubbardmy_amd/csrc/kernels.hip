@@ -102,13 +102,34 @@ k_sfmt_draw(const DChain *chains, int nchains, int mode)
 {
     const DChain &c = chains[blockIdx.x];
     if (mode == 0) {
-        if (!c.lwrapu) return;
-        const int lc = c.lcomp;
-        int32_t *conf = c.conf;
-        sfmt_draw_block(c.rng, (size_t)c.n * c.ltrot, [=](size_t k, double u) { conf[k] = (int32_t)ceil(u * lc); });
-    } else {
+        // u0conf%input_conf, then v0conf%input_conf (src/dqmc_ft_ctrl.f90:25-35)
+        if (c.lwrapu) {
+            const int lc = c.lcomp;
+            int32_t *conf = c.conf;
+            sfmt_draw_block(c.rng, (size_t)c.n * c.ltrot, [=](size_t k, double u) { conf[k] = (int32_t)ceil(u * lc); });
+        }
+        if (c.lwrapv) {
+            int32_t *cv = c.confv;
+            sfmt_draw_block(c.rng, (size_t)c.n * c.nn_nf * c.ltrot, [=](size_t k, double u) { cv[k] = (int32_t)ceil(u * 2); });
+        }
+    } else if (!c.lwrapv) {
         double *un = c.unif;
         sfmt_draw_block(c.rng, (size_t)2 * c.n * c.ltrot, [=](size_t k, double u) { un[k] = u; });
+    } else {
+        // per slice the on-site draws (2 n, when the Hubbard field is on) and the bond draws
+        // (2 nn_nf nn_lf) in the order the pass consumes them: b0 (mode 1) on-site first, 0b
+        // (mode 2) bonds first (dqmc_ft_sweep_b0.f90:70-89, dqmc_ft_sweep_0b.f90:95-112)
+        double *un = c.unif, *uv = c.unifv;
+        const size_t nu = c.lwrapu ? (size_t)2 * c.n : 0, nv = (size_t)2 * c.nn_nf * c.nn_lf, per = nu + nv;
+        const bool ufirst = mode == 1;
+        sfmt_draw_block(c.rng, per * c.ltrot, [=](size_t k, double u) {
+            const size_t s = k / per, r = k % per;
+            if (ufirst) {
+                if (r < nu) un[s * nu + r]= u; else uv[s * nv + r - nu] = u;
+            } else {
+                if (r < nv) uv[s * nv + r] = u; else un[s * nu + r - nv] = u;
+            }
+        });
     }
 }
 

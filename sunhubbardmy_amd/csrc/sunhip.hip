@@ -7,6 +7,7 @@
 #include "learn_kernels.hip"
 #include "dyn_kernels.hip"
 #include "eqt_kernels.hip"
+#include "tv_kernels.hip"
 #include "../../include/sunhip.h"
 #include "../../include/sunhip_learn.h"
 
@@ -66,6 +67,7 @@ struct sh_ctx : ctx_base {
     bool legacy_update = false;   // test hook: the L2-resident delay update kernel
     int ntdm = -1;                // time-displaced measurement: bins 0 .. ntdm, -1 = off
     size_t eqt_lds = 0;           // LDS of k_eqt, 0 while no chain has correlation bins
+    int lwrapv = 0;               // the batch carries the t-V bond field
     // push/pop_stage backups
     std::vector<void *> stage;
     std::vector<cplx *> trace_bufs;
@@ -301,6 +303,31 @@ static void udv_step(sh_ctx *ctx, bool first, int pt, int dpt)
     gemm<0, 0>(ctx, M_W3, 0, M_W4, 0, M_V, pt);
 }
 
+// bond-field props of family nf on rows (side 0) or columns (side 1) of matrix sel
+static void vprop(sh_ctx *ctx, int sel, int nf, int mode, int side, int nt)
+{
+    const unsigned nb = (unsigned)(((size_t)ctx->nmax * ctx->nmax / 2 + 255) / 256);   // n nn_lf <= n^2 / 2
+    hipLaunchKernelGGL(k_vprop, dim3(nb, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dc, sel, 0, nf, mode, side, nt);
+}
+static void vupdate(sh_ctx *ctx, int nt, int nf, int slice, int p)
+{
+    hipLaunchKernelGGL(k_vupdate, dim3(ctx->nch), dim3(256), (size_t)4 * ctx->nmax * sizeof(cplx), ctx->st, ctx->dc,
+                       nt, nf, slice, p);
+}
+// one slice of Bmat_left_forward / _hc with the bond field: K-half, the families (nflag 2 then 1;
+// 1..nn_nf forward, nn_nf..1 for _hc: src/dqmc_ft_core.f90:98-104, :150-156), V, K-half
+static void bprod_slice_v(sh_ctx *ctx, int nt, int hc)
+{
+    const int nfam = ctx->hc[0].nn_nf;
+    left_pass(ctx, M_W1, 0, 0, 1, 0, nt, nt, 1);
+    for (int q = 0; q < nfam; q++) {
+        const int nf = hc ? nfam - 1 - q : q;
+        vprop(ctx, M_W1, nf, 0, 0, nt);
+        vprop(ctx, M_W1, nf, 1, 0, nt);
+    }
+    left_pass(ctx, M_W1, 0, 0, 2 | 4, hc ? 2 : 0, nt, nt, 1);
+}
+
 // dqmc_ft_stablize_0b_qr(n): point n of the right chain from point n-1
 static void stablize_0b(sh_ctx *ctx, int n)
 {
@@ -309,7 +336,10 @@ static void stablize_0b(sh_ctx *ctx, int n)
         copy_mat(ctx, M_W1, 0, M_U, n - 1);
         int ws1 = ctx->wrap_step[2 * (n - 1)], ws2 = ctx->wrap_step[2 * (n - 1) + 1];
         // Bmat_left_forward: slices ws1..ws2, each K-half, V, K-half (src/dqmc_ft_core.f90:70-119)
-        left_pass(ctx, M_W1, 0, 0, 1 | 2 | 4, 0, ws1, ws2, +1);
+        if (ctx->lwrapv)
+            for (int nt = ws1; nt <= ws2; nt++) bprod_slice_v(ctx, nt, 0);
+        else
+            left_pass(ctx, M_W1, 0, 0, 1 | 2 | 4, 0, ws1, ws2, +1);
     }
     udv_step(ctx, n == 1, n, n - 1);
 }
@@ -322,7 +352,10 @@ static void stablize_b0(sh_ctx *ctx, int n)
         copy_mat(ctx, M_W1, 0, M_U, n);
         int ws1 = ctx->wrap_step[2 * (n - 1)], ws2 = ctx->wrap_step[2 * (n - 1) + 1];
         // Bmat_left_forward_hc: slices ws2 down to ws1 with conj(bmat) (src/dqmc_ft_core.f90:121-171)
-        left_pass(ctx, M_W1, 0, 0, 1 | 2 | 4, 2, ws2, ws1, -1);
+        if (ctx->lwrapv)
+            for (int nt = ws2; nt >= ws1; nt--) bprod_slice_v(ctx, nt, 1);
+        else
+            left_pass(ctx, M_W1, 0, 0, 1 | 2 | 4, 2, ws2, ws1, -1);
     }
     udv_step(ctx, n == ctx->nst, n - 1, n);
 }
@@ -553,7 +586,12 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
             delete ctx;
             return SH_E_ARG;
         }
-        if (q.kmode == SH_K_CHECKERBOARD && (q.nn_nf <= 0 || q.nn_lf <= 0)) { delete ctx; return SH_E_ARG; }
+        if ((q.kmode == SH_K_CHECKERBOARD || q.vtab) && (q.nn_nf <= 0 || q.nn_lf <= 0)) { delete ctx; return SH_E_ARG; }
+        if ((q.vtab != nullptr) != (m[0].vtab != nullptr) || (q.vtab && !q.vdelta)) {
+            int rc = fail(ctx, SH_E_ARG, "all chains of a batch must have the bond field or none");
+            delete ctx;
+            return rc;
+        }
         for (int b = 0; b < q.nn_nf * q.nn_lf * 2; b++)
             if (q.bonds[b] < 0 || q.bonds[b] >= q.ndim) { delete ctx; return SH_E_ARG; }
         for (int i = 0; i <= q.ltrot; i++)
@@ -602,6 +640,22 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
         c.kmode = q.kmode; c.nublock = q.nublock; c.update = q.update; c.lwrapT = q.lwrapT; c.lwrapu = q.lwrapu;
         c.phase_from_conf = q.phase_from_conf; c.ltrot = q.ltrot; c.nst = q.nst;
         c.kin_weight = q.kin_weight != 0.0 ? q.kin_weight : 1.0;
+        if (q.vtab) {
+            c.lwrapv = 1;
+            for (int k = 0; k < 8; k++) c.vtb[k] = make_double2(q.vtab[2 * k], q.vtab[2 * k + 1]);
+            for (int k = 0; k < 4; k++) c.vdl[k] = make_double2(q.vdelta[2 * k], q.vdelta[2 * k + 1]);
+            c.confv = dalloc<int32_t>(ctx, (size_t)n * q.nn_nf * q.ltrot);
+            c.unifv = dalloc<double>(ctx, (size_t)2 * q.nn_nf * q.nn_lf * q.ltrot);
+            c.vacc = dalloc<double>(ctx, 2);
+            if (!c.confv || !c.unifv || !c.vacc) {
+                int rc = fail(ctx, SH_E_HIP, "device allocation failed");
+                sh_destroy(ctx);
+                return rc;
+            }
+            std::vector<int32_t> ones((size_t)n * q.nn_nf * q.ltrot, 1);
+            hipMemcpy(c.confv, ones.data(), ones.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+            hipMemset(c.vacc, 0, 2 * sizeof(double));
+        }
         c.G = dalloc<cplx>(ctx, nn); c.Gs = dalloc<cplx>(ctx, nn);
         c.W1 = dalloc<cplx>(ctx, nn); c.W2 = dalloc<cplx>(ctx, nn); c.W3 = dalloc<cplx>(ctx, nn);
         c.W4 = dalloc<cplx>(ctx, nn); c.Us = dalloc<cplx>(ctx, nn); c.Vs = dalloc<cplx>(ctx, nn);
@@ -649,6 +703,7 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
         hipMemcpy(c.sc, sc0, sizeof(sc0), hipMemcpyHostToDevice);
         seeds[ci] = q.seed;
     }
+    ctx->lwrapv = ctx->hc[0].lwrapv;
     ctx->dc = dupload<DChain>(ctx, ctx->hc.data(), nchains);
     ctx->d_seeds = dupload<int32_t>(ctx, seeds.data(), nchains);
     ctx->seeds = seeds;
@@ -705,12 +760,46 @@ int sh_get_conf(sh_ctx *ctx, int chain, int32_t *conf)
     return 0;
 }
 
+int sh_set_confv(sh_ctx *ctx, int chain, const int32_t *conf)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !conf) return SH_E_ARG;
+    const DChain &c = ctx->hc[chain];
+    if (!c.lwrapv) return fail(ctx, SH_E_STATE, "the chain has no bond field");
+    const size_t cnt = (size_t)c.n * c.nn_nf * c.ltrot;
+    for (size_t k = 0; k < cnt; k++)
+        if (conf[k] < 1 || conf[k] > 2) return fail(ctx, SH_E_ARG, "bond field values must be 1 or 2");
+    HIPCHK(hipMemcpy(c.confv, conf, cnt * sizeof(int32_t), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int sh_get_confv(sh_ctx *ctx, int chain, int32_t *conf)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !conf) return SH_E_ARG;
+    const DChain &c = ctx->hc[chain];
+    if (!c.lwrapv) return fail(ctx, SH_E_STATE, "the chain has no bond field");
+    HIPCHK(hipMemcpyAsync(conf, c.confv, (size_t)c.n * c.nn_nf * c.ltrot * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int sh_get_vacc(sh_ctx *ctx, int chain, double *acc)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !acc) return SH_E_ARG;
+    const DChain &c = ctx->hc[chain];
+    if (!c.lwrapv) return fail(ctx, SH_E_STATE, "the chain has no bond field");
+    HIPCHK(hipMemcpyAsync(acc, c.vacc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
 int sh_set_dyn(sh_ctx *ctx, int chain, int ntdm, int l1, int l2, int l3, const int32_t *sites)
 {
     if (!ctx || chain < 0 || chain >= ctx->nch || !sites || ntdm < 0 || ntdm > ctx->ltrot || l1 < 1 || l2 < 1 ||
         l3 < 1)
         return SH_E_ARG;
     if (ctx->ntdm >= 0 && ntdm != ctx->ntdm) return fail(ctx, SH_E_ARG, "all chains of a batch share ntdm");
+    if (ctx->lwrapv) return fail(ctx, SH_E_ARG, "the time-displaced pass is not built for the t-V bond field");
     DChain &c = ctx->hc[chain];
     if (l1 * l2 * l3 != c.lq) return fail(ctx, SH_E_ARG, "l1 * l2 * l3 must be the number of unit cells");
     for (int i = 0; i < c.n; i++)
@@ -885,8 +974,8 @@ int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure)
     if (!ctx) return SH_E_ARG;
     if (!ctx->started) return fail(ctx, SH_E_STATE, "sweep_b0 before sweep_start_0b");
     set_identity(ctx, ctx->nst);
-    const bool upd = lupdate && ctx->hc[0].lwrapu;
-    if (upd) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
+    const bool upd = lupdate && ctx->hc[0].lwrapu, updv = lupdate && ctx->lwrapv;
+    if (upd || updv) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
     if (upd) prop_pass_start(ctx);
     int slice = 0;
     for (int nt = ctx->ltrot; nt >= 1; nt--, slice++) {
@@ -899,11 +988,30 @@ int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure)
         }
         if (upd && ctx->prop_local) update_slice(ctx, nt, slice);
         if (upd) block_move(ctx, nt);
-        {
+        if (!ctx->lwrapv) {
             Tic t(ctx, T_WRAP);
             // V (right_forward bmat, left_backward bmat_inv) then the K-half nn_nf..1
             right_pass(ctx, 0, 8 | 4, 0, nt);
             left_pass(ctx, M_G, 0, 1, 8 | 4, 1, nt, nt, 1);
+        } else {
+            // V, the bond field by families nn_nf..1 (nflag 2, update, nflag 1:
+            // src/dqmc_ft_core/dqmc_ft_sweep_b0.f90:70-89), then the K-half nn_nf..1
+            Tic t(ctx, T_WRAP);
+            if (ctx->hc[0].lwrapu) {
+                right_pass(ctx, 0, 8, 0, nt);
+                left_pass(ctx, M_G, 0, 1, 8, 1, nt, nt, 1);
+            }
+            const int nfam = ctx->hc[0].nn_nf;
+            for (int p = 0; p < nfam; p++) {
+                const int nf = nfam - 1 - p;
+                vprop(ctx, M_G, nf, 1, 1, nt);
+                vprop(ctx, M_G, nf, 1, 0, nt);
+                if (updv) vupdate(ctx, nt, nf, slice, p);
+                vprop(ctx, M_G, nf, 2, 1, nt);
+                vprop(ctx, M_G, nf, 2, 0, nt);
+            }
+            right_pass(ctx, 0, 4, 0, nt);
+            left_pass(ctx, M_G, 0, 1, 4, 1, nt, nt, 1);
         }
         if ((ctx->iwrap_nt[nt - 1] > 0 || ctx->nwrap == 1) || nt == 1) {
             int n = ctx->iwrap_nt[nt - 1];
@@ -992,15 +1100,33 @@ int sh_sweep_0b(sh_ctx *ctx, int lupdate, int lmeasure, int lmeasure_dyn)
         return sweep_0b_dyn(ctx, lmeasure);
     }
     set_identity(ctx, 0);
-    const bool upd = lupdate && ctx->hc[0].lwrapu;
-    if (upd) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
+    const bool upd = lupdate && ctx->hc[0].lwrapu, updv = lupdate && ctx->lwrapv;
+    if (upd || updv) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 2);
     if (upd) prop_pass_start(ctx);
     for (int nt = 1; nt <= ctx->ltrot; nt++) {
-        {
+        if (!ctx->lwrapv) {
             Tic t(ctx, T_WRAP);
             // K-half B G B^{-1}: left_forward (urt), right_backward (urtm1); then V rows/cols
             left_pass(ctx, M_G, 0, 0, 1 | 2, 0, nt, nt, 1);
             right_pass(ctx, 1, 1 | 2, 1, nt);
+        } else {
+            // K-half, the bond field by families 1..nn_nf (nflag 2, update, nflag 1:
+            // src/dqmc_ft_core/dqmc_ft_sweep_0b.f90:95-106), then V
+            Tic t(ctx, T_WRAP);
+            left_pass(ctx, M_G, 0, 0, 1, 0, nt, nt, 1);
+            right_pass(ctx, 1, 1, 1, nt);
+            const int nfam = ctx->hc[0].nn_nf;
+            for (int nf = 0; nf < nfam; nf++) {
+                vprop(ctx, M_G, nf, 0, 0, nt);
+                vprop(ctx, M_G, nf, 0, 1, nt);
+                if (updv) vupdate(ctx, nt, nf, nt - 1, nf);
+                vprop(ctx, M_G, nf, 1, 0, nt);
+                vprop(ctx, M_G, nf, 1, 1, nt);
+            }
+            if (ctx->hc[0].lwrapu) {
+                left_pass(ctx, M_G, 0, 0, 2, 0, nt, nt, 1);
+                right_pass(ctx, 1, 2, 1, nt);
+            }
         }
         if (upd && ctx->prop_local) update_slice(ctx, nt, nt - 1);
         if (upd) block_move(ctx, nt);

@@ -109,6 +109,13 @@ struct DChain {
     const int32_t *eqc, *plq;
     int zplq, pad3;
     double theta;
+    // bond field of the t-V model (lwrapv): field [ltrot][nn_nf][n], the pass's bond draws
+    // (2 nn_nf nn_lf per slice, in the order the families are updated), acceptance sums;
+    // vtb = (bp, bm, bpi, bmi) of component 1 then 2, vdl = (dp, dm) of 1 then 2
+    int lwrapv, pad5;
+    int32_t *confv;
+    double *unifv, *vacc;
+    cplx vtb[8], vdl[4];
 };
 
 __device__ __forceinline__ cplx cmk(double r, double i) { return make_double2(r, i); }

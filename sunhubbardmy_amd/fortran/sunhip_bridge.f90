@@ -25,6 +25,8 @@ module sunhip_bridge
      integer(c_int) :: update, nublock
      integer(c_int32_t) :: seed
      real(c_double) :: kin_weight
+     ! the t-V bond field (c_null_ptr: none)
+     type(c_ptr) :: vtab, vdelta
   end type sh_model
 
   ! mirrors struct shp_model of include/sunhip_proj.h
@@ -69,6 +71,25 @@ module sunhip_bridge
        integer(c_int), value :: chain
        integer(c_int32_t), intent(out) :: conf(*)
      end function sh_get_conf
+     ! the t-V bond field conf(ndim, nn_nf, ltrot), values 1..2, and its acceptance sums
+     integer(c_int) function sh_set_confv(ctx, chain, conf) bind(C, name="sh_set_confv")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(in) :: conf(*)
+     end function sh_set_confv
+     integer(c_int) function sh_get_confv(ctx, chain, conf) bind(C, name="sh_get_confv")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(out) :: conf(*)
+     end function sh_get_confv
+     integer(c_int) function sh_get_vacc(ctx, chain, acc) bind(C, name="sh_get_vacc")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       real(c_double), intent(out) :: acc(2)
+     end function sh_get_vacc
      ! correlation bins of equaltime_measure: cells(4, lq) = A site, nnlist(., 1), snlist(., 1),
      ! current partner (0-based); plq(zplq, lq) plaquette cycles; bins(2, 20 lq + 8)
      integer(c_int) function sh_set_eqt(ctx, chain, l1, l2, l3, sites, cells, zplq, plq, theta) &

@@ -37,6 +37,7 @@ class Model:
     nublock: int = 0
     seed: int = 1909706392
     lc: int = 1                    # third linear size (cubic lattice)
+    rv: float = 0.0                # nearest-neighbour V: the bond field of the t-V model
 
 
 def chain_seed(c):
@@ -322,6 +323,20 @@ def kinetic_tables(m, lat):
     return np.asfortranarray(urt).reshape(-1, order="F"), np.asfortranarray(urtm1).reshape(-1, order="F")
 
 
+def vconf_tables(m):
+    """vconf%set_conf with lprojv = .false. (src/dqmc_config_v/dqmc_set_conf.f90:14-68): bmat_p, bmat_m
+    and their inverses per field value, and delta_bmat_p / _m (iflip = 1)."""
+    alpha = np.arccosh(np.exp(m.dtau * m.rv * 0.5))
+    eta = np.array([-1.0, 1.0])
+    vt = np.zeros((2, 4), complex)
+    vd = np.zeros((2, 2), complex)
+    for k in range(2):
+        vt[k] = np.exp([alpha * eta[k], -alpha * eta[k], -alpha * eta[k], alpha * eta[k]])
+        kp = 1 - k
+        vd[k] = [np.exp(alpha * (eta[kp] - eta[k])) - 1.0, np.exp(-alpha * (eta[kp] - eta[k])) - 1.0]
+    return {"alpha": alpha, "vt": vt, "vd": vd}
+
+
 def make_tables(m):
     lat = lattice_tables(m)
     ltrot = int(round(m.beta / m.dtau))
@@ -335,4 +350,4 @@ def make_tables(m):
             "nwrap": nw, "iwrap_nt": iw, "wrap_step": ws, "hs": hs, "urt": urt, "urtm1": urtm1,
             "nublock": nub, "kmode": KMODES[m.kmode], "update": UPDATES[m.update],
             "lwrapT": int(m.rt != 0.0), "lwrapu": int(m.rhub != 0.0),
-            "phase_from_conf": int(m.rhub > 0.0)}
+            "phase_from_conf": int(m.rhub > 0.0), "v": vconf_tables(m) if m.rv != 0.0 else None}

@@ -16,7 +16,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .model import KMODES, Model, default_nublock, hs_tables, kinetic_tables, lattice_tables, schedule
+from .model import KMODES, Model, default_nublock, hs_tables, kinetic_tables, lattice_tables, schedule, vconf_tables
 from .walkers import TIMER_GROUPS, SweepError
 
 
@@ -41,19 +41,6 @@ class ProjModel:
     seed: int = 1909706392
     rhub: float = 0.0          # on-site U (the uconf field when != 0)
     exact_hs: bool = True      # EXACTHS tables of the on-site field
-
-
-def vconf_tables(m):
-    """bmat_p, bmat_m and their inverses per field value, and delta_bmat_p / _m (iflip = 1)."""
-    alpha = np.arccosh(np.exp(m.dtau * m.rv * 0.5))
-    eta = np.array([-1.0, 1.0])
-    vt = np.zeros((2, 4), complex)
-    vd = np.zeros((2, 2), complex)
-    for k in range(2):
-        vt[k] = np.exp([alpha * eta[k], -alpha * eta[k], -alpha * eta[k], alpha * eta[k]])
-        kp = 1 - k
-        vd[k] = [np.exp(alpha * (eta[kp] - eta[k])) - 1.0, np.exp(-alpha * (eta[kp] - eta[k])) - 1.0]
-    return {"alpha": alpha, "vt": vt, "vd": vd}
 
 
 def make_proj_tables(m):

@@ -74,6 +74,9 @@ class FTWalkers:
         s.stag = keep(lat["stag"], np.float64)
         s.update, s.nublock, s.seed = t["update"], t["nublock"], int(np.int32(m.seed))
         s.kin_weight = lat["kin_weight"]
+        if t["v"] is not None:
+            s.vtab = keep(cz(t["v"]["vt"].reshape(-1)), np.float64)
+            s.vdelta = keep(cz(t["v"]["vd"].reshape(-1)), np.float64)
         return s
 
     # -- lifecycle ---------------------------------------------------------------------------
@@ -107,6 +110,28 @@ class FTWalkers:
         t = self.tables[chain]
         c = _c(conf, np.int32).reshape(t["ltrot"], t["ndim"])
         self._check(_lib.lib().sh_set_conf(self._ctx, chain, c.ctypes.data_as(_lib.ip)), "set_conf")
+
+    @property
+    def has_vconf(self):
+        return self.tables[0]["v"] is not None
+
+    def get_confv(self, chain=0):
+        """the t-V bond field [ltrot][nn_nf][ndim] (entries i < nn_lf are the bonds)"""
+        t = self.tables[chain]
+        c = np.zeros((t["ltrot"], t["lat"]["nn_nf"], t["ndim"]), np.int32)
+        self._check(_lib.lib().sh_get_confv(self._ctx, chain, c.ctypes.data_as(_lib.ip)), "get_confv")
+        return c
+
+    def set_confv(self, conf, chain=0):
+        t = self.tables[chain]
+        c = _c(conf, np.int32).reshape(t["ltrot"], t["lat"]["nn_nf"], t["ndim"])
+        self._check(_lib.lib().sh_set_confv(self._ctx, chain, c.ctypes.data_as(_lib.ip)), "set_confv")
+
+    def vacc(self, chain=0):
+        """(accepted, tried) bond-field flips"""
+        a = np.zeros(2)
+        self._check(_lib.lib().sh_get_vacc(self._ctx, chain, a.ctypes.data_as(_lib.dp)), "get_vacc")
+        return a
 
     def rng_state(self, chain=0):
         """The chain's generator state (SH_RNG_WORDS uint32: SFMT19937 words, read index, block
