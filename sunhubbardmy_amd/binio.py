@@ -7,11 +7,15 @@ pair_nn, pair_sn (lq each), then zn(1), zn(2), zj, zb, zqsq, ztsq, ztq, zint_v
 (src/dqmc_measure/equaltime_measure.f90:44-68; zqsq without its 1/9 and zint_v without rv, both
 applied when energy.bin is written).
 
-`write_bin` appends one bin to the files equaltime_output writes on the square and honeycomb
-lattices (src/dqmc_measure/equaltime_output.f90:1-180): energy.bin (16e16.8), n.bin, j.bin,
-b.bin, nnnn.bin, Crmax.bin and the *_r.bin / *_k.bin pairs of fourier_trans_eqt and
-fourier_trans_eqt_22, with the MPI reduction replaced by the sum over chains and rnorm =
-(number of chains) * nobs.
+`write_bin` appends one bin to the files equaltime_output writes, with the MPI reduction
+replaced by the sum over chains and rnorm = (number of chains) * nobs:
+* square and honeycomb (src/dqmc_measure/equaltime_output.f90:1-180): energy.bin (16e16.8),
+  n.bin, j.bin, b.bin, nnnn.bin, Crmax.bin and the *_r.bin / *_k.bin pairs of the eight
+  correlation functions;
+* chain (equaltime_output1d.f90): energy.bin (20e16.8 of energy_bin(1:10), zint_v in column 5),
+  n.bin, j.bin, b.bin and the same eight pairs with one k coordinate;
+* cube (equaltime_output3d.f90): energy.bin (8e16.8), n.bin (2e16.8), b.bin and the pairs of
+  cpcm, spsm, nn, bb with (3f16.8, 2e16.8) k lines.
 """
 import os
 
@@ -64,56 +68,96 @@ def _reals(zs):
     return out
 
 
-def k_list(l1, l2):
-    """latt%listk of setup_latt2d_sub (src/lattice.f90:463-470): m outer, n inner"""
-    return [(m, n) for m in range(-((l1 + 1) // 2) + 1, l1 // 2 + 1) for n in range(-((l2 + 1) // 2) + 1, l2 // 2 + 1)]
+def _krange(l):
+    return range(-((l + 1) // 2) + 1, l // 2 + 1)
 
 
-def _phase_table(l1, l2):
-    """zexpiqr(nk, imj) = exp(i k . a_imj), k . a = 2 pi (m x / l1 + n y / l2) for cell imj at
-    (x, y), imj = y + (x - 1) l2 (lattice.f90:481-489)"""
-    ks = np.array(k_list(l1, l2), float)
-    x = np.repeat(np.arange(1, l1 + 1), l2)
-    y = np.tile(np.arange(1, l2 + 1), l1)
-    return np.exp(2j * np.pi * (np.outer(ks[:, 0], x) / l1 + np.outer(ks[:, 1], y) / l2)), ks
+def k_list(l1, l2, l3=None):
+    """latt%listk: 2D m outer, n inner (src/lattice.f90:463-470); 3D l (z) outermost, then m, n
+    (:1116-1125); the chain's list is m alone (:105-109, l2 = l3 = None is not used for it)"""
+    if l3 is None:
+        return [(m, n) for m in _krange(l1) for n in _krange(l2)]
+    return [(m, n, l) for l in _krange(l3) for m in _krange(l1) for n in _krange(l2)]
 
 
-def _fourier(gr, l1, l2, tag, path):
+def _phase_table(dims):
+    """zexpiqr(nk, imj) = exp(i k . a_imj) with k . a = 2 pi sum_d k_d x_d / l_d for the cell imj at
+    x (imj = y + (x - 1) l2 in 2D, + (z - 1) l1 l2 in 3D, x on the chain: lattice.f90:481-489,
+    :1147-1154)"""
+    if len(dims) == 1:
+        ks = np.array([(m,) for m in _krange(dims[0])], float)
+        xs = np.arange(1, dims[0] + 1)[:, None]
+    elif len(dims) == 2:
+        ks = np.array(k_list(*dims), float)
+        xs = np.array([(x, y) for x in range(1, dims[0] + 1) for y in range(1, dims[1] + 1)], float)
+    else:
+        l1, l2, l3 = dims
+        ks = np.array(k_list(l1, l2, l3), float)
+        xs = np.array([(x, y, z) for z in range(1, l3 + 1) for x in range(1, l1 + 1) for y in range(1, l2 + 1)], float)
+    arg = sum(np.outer(ks[:, d], xs[:, d]) / dims[d] for d in range(len(dims)))
+    return np.exp(2j * np.pi * arg), ks
+
+
+def _fourier(gr, dims, tag, path):
     """fourier_trans_eqt (gr (lq,)) or fourier_trans_eqt_22 (gr (lq, nsub, nsub))"""
-    lq = l1 * l2
-    z, ks = _phase_table(l1, l2)
+    lq = int(np.prod(dims))
+    z, ks = _phase_table(dims)
     with open(os.path.join(path, tag + "_r.bin"), "a") as f:
         for imj in range(lq):
             _record(f, _reals(gr[imj].T if gr.ndim == 3 else gr[imj]))
     gk = np.tensordot(1.0 / z, gr, axes=(1, 0)) / lq
     with open(os.path.join(path, tag + "_k.bin"), "a") as f:
         for nk in range(lq):
-            _record(f, [ks[nk, 0] / l1, ks[nk, 1] / l2] + _reals(gk[nk].T if gr.ndim == 3 else gk[nk]))
+            kc = [ks[nk, d] / dims[d] for d in range(len(dims))]
+            vals = _reals(gk[nk].T if gr.ndim == 3 else gk[nk])
+            if len(dims) == 3:
+                f.write("".join("%16.8f" % x for x in kc) + "".join(fortran_e(x) for x in vals) + "\n")
+            else:
+                _record(f, kc + vals)
 
 
-def write_bin(path, lattice, l1, l2, nobs, obs, eqt, rv=0.0):
+def write_bin(path, lattice, l1, l2, nobs, obs, eqt, rv=0.0, l3=1):
     """Append one bin: obs = per chain the complex sums (sign, density, zkint, zint_u) of the
     library's equal-time measurement, eqt = per chain the flat correlation bins, nobs = measured
-    slices per chain in the bin."""
-    if lattice not in ("square", "honeycomb"):
-        raise ValueError("the *.bin writers cover the square and honeycomb lattices")
+    slices per chain in the bin; (l1, l2, l3) the cell grid (l2 ignored on the chain)."""
+    if lattice not in ("square", "honeycomb", "chain", "cubic"):
+        raise ValueError("unknown lattice %r" % lattice)
     nsub = 2 if lattice == "honeycomb" else 1
-    lq = l1 * l2
+    dims = {"chain": (l1,), "cubic": (l1, l2, l3)}.get(lattice, (l1, l2))
+    lq = int(np.prod(dims))
     rnorm = float(len(eqt) * nobs)
     tot = unpack_eqt(np.sum(np.asarray(eqt), axis=0), lq)
     e4 = np.sum(np.asarray(obs), axis=0)
-    energy = np.array([e4[0], e4[1], e4[2], e4[3], tot["zqsq"] / 9.0, tot["ztsq"], tot["ztq"],
-                       tot["zint_v"] * rv]) / rnorm
     os.makedirs(path, exist_ok=True)
+    if lattice == "cubic":
+        for tag in ("cpcm", "spsm", "nn"):
+            _fourier(tot[tag][:, 0, 0] / rnorm, dims, tag, path)
+        _fourier(tot["bb"] / rnorm, dims, "bb", path)
+        with open(os.path.join(path, "energy.bin"), "a") as f:
+            _record(f, _reals(np.array(e4[:4]) / rnorm))
+        for name, v in (("n.bin", tot["zn1"]), ("b.bin", tot["zb"])):
+            with open(os.path.join(path, name), "a") as f:
+                _record(f, _reals(np.array([v]) / rnorm))
+        return
     for tag in KEYS22:
-        _fourier(tot[tag][:, :nsub, :nsub] / rnorm, l1, l2, tag, path)
+        _fourier(tot[tag][:, :nsub, :nsub] / rnorm, dims, tag, path)
     for tag in KEYS1:
-        _fourier(tot[tag] / rnorm, l1, l2, tag, path)
+        _fourier(tot[tag] / rnorm, dims, tag, path)
+    if lattice == "chain":
+        energy = np.zeros(10, complex)
+        energy[:4] = e4[:4]
+        energy[4] = tot["zint_v"]
+    else:
+        energy = np.array([e4[0], e4[1], e4[2], e4[3], tot["zqsq"] / 9.0, tot["ztsq"], tot["ztq"],
+                           tot["zint_v"] * rv])
     with open(os.path.join(path, "energy.bin"), "a") as f:
-        _record(f, _reals(energy))
+        _record(f, _reals(energy / rnorm))
     with open(os.path.join(path, "n.bin"), "a") as f:
         _record(f, _reals(np.array([tot["zn1"], tot["zn2"]]) / rnorm))
-    for name, v in (("j.bin", tot["zj"]), ("b.bin", tot["zb"]), ("nnnn.bin", 0j), ("Crmax.bin", 0j)):
+    extra = (("j.bin", tot["zj"]), ("b.bin", tot["zb"]))
+    if lattice != "chain":
+        extra += (("nnnn.bin", 0j), ("Crmax.bin", 0j))
+    for name, v in extra:
         # znnnn and zCrmax are never accumulated in the reference (their calls are commented out)
         with open(os.path.join(path, name), "a") as f:
             _record(f, _reals(np.array([v]) / rnorm))

@@ -220,3 +220,31 @@ def test_write_bin_files(tmp_path):
     for name in ("n.bin", "j.bin", "b.bin", "nnnn.bin", "Crmax.bin", "cpcm_k.bin", "nn_r.bin", "jj_k.bin",
                  "bb_r.bin", "pair_onsite_k.bin", "pair_nn_r.bin", "pair_sn_k.bin"):
         assert (tmp_path / name).exists(), name
+
+
+def test_write_bin_chain_and_cube(tmp_path):
+    for lattice, kw, dims in (("chain", dict(la=8, lb=1), (8,)), ("cubic", dict(la=4, lb=4, lc=2), (4, 4, 2))):
+        o = oracle.OracleChain(lattice=lattice, beta=1.0, rhub=4.0, nwrap=5, seed=oracle.chain_seed(4), **kw)
+        o.init_conf()
+        o.set_eqt(0.0)
+        o.sweep_start()
+        o.sweep(lupdate=True, lmeasure=True)
+        ob = o.obs()
+        obs = [np.array([ob["sign"], ob["density"], ob["ekin"], ob["dblocc"]])]
+        d = tmp_path / lattice
+        binio.write_bin(str(d), lattice, kw["la"], kw["lb"], ob["nobs"], obs, [o.eqt()], l3=kw.get("lc", 1))
+        e = np.loadtxt(d / "energy.bin")
+        assert e.shape == ((20,) if lattice == "chain" else (8,))
+        assert abs(e[2] - ob["density"].real / ob["nobs"]) < 1e-7
+        k = np.loadtxt(d / "spsm_k.bin", ndmin=2)
+        lq = int(np.prod(dims))
+        assert k.shape == (lq, len(dims) + 2)
+        # the k = pi line of spsm_k is the staggered structure factor
+        row = k[np.all(k[:, :len(dims)] == 0.5, axis=1)][0]
+        assert abs(row[len(dims)] - ob["s_stag"].real / ob["nobs"]) < 1e-7
+        names = {"chain": ("n.bin", "j.bin", "b.bin", "jj_k.bin", "pair_sn_r.bin"),
+                 "cubic": ("n.bin", "b.bin", "bb_k.bin", "nn_r.bin")}[lattice]
+        for name in names:
+            assert (d / name).exists(), name
+        if lattice == "cubic":
+            assert not (d / "jj_k.bin").exists()
