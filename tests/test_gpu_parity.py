@@ -107,3 +107,32 @@ def test_sweep_parity_metric_shape_short():
     kw = dict(BASE)
     kw.update(la=16, lb=16, beta=1.0, rhub=6.0, nublock=32, nwrap=5)
     run_parity(kw, nsweeps=1)
+
+
+def test_uneven_batch_matches_single_chains():
+    """Chains of different N and lattices in one batch (one launch per kernel for all of them):
+    every chain's field and G equal its own oracle run with the same seed."""
+    base = dict(beta=1.0, dtau=0.1, rhub=4.0, nflr=2, kmode="checkerboard", update="delay", nwrap=5, nublock=4)
+    shapes = [dict(lattice="square", la=4, lb=4), dict(lattice="honeycomb", la=3, lb=3),
+              dict(lattice="square", la=6, lb=6), dict(lattice="chain", la=10, lb=1)]
+    models = [Model(seed=chain_seed(c), **base, **sh) for c, sh in enumerate(shapes)]
+    say("uneven batch", [m.lattice for m in models])
+    w = FTWalkers(models)
+    oracles = [oracle.OracleChain(seed=chain_seed(c), **base, **sh) for c, sh in enumerate(shapes)]
+    w.init_conf()
+    for o in oracles:
+        o.init_conf()
+    w.sweep_start_0b()
+    for o in oracles:
+        o.sweep_start()
+    for p in ("b0", "0b"):
+        getattr(w, "sweep_" + p)(True, True)
+        for o in oracles:
+            getattr(o, "sweep_" + p)(True, True)
+    for c, o in enumerate(oracles):
+        assert np.array_equal(w.get_conf(c), o.conf), c
+        assert relmax(w.G(c), o.G) < 1e-10, c
+        ow, oo = w.obs(c), o.obs()
+        for k in ("density", "ekin", "dblocc", "s_stag"):
+            assert abs(ow[k] - oo[k]) <= 1e-9 * max(1.0, abs(oo[k])), (c, k)
+    w.close()
