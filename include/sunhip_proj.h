@@ -59,6 +59,11 @@ typedef struct shp_model {
 /* allocate_core (src/dqmc_proj_core.f90:23-49) for a batch of chains sharing ndim, ne, ltrot,
  * nwrap, kmode and nustock */
 int shp_create(shp_ctx **ctx, int nchains, const shp_model *models, int device);
+/* the correlation bins of equaltime_measure at the measured slices, as sh_set_eqt / sh_get_eqt
+ * (the projective sweep calls the same equaltime_measure: src/dqmc_proj_core/dqmc_proj_sweep_0b.f90) */
+int shp_set_eqt(shp_ctx *ctx, int chain, int l1, int l2, int l3, const int32_t *sites, const int32_t *cells, int zplq,
+                const int32_t *plq, double theta);
+int shp_get_eqt(shp_ctx *ctx, int chain, double *bins);
 int shp_destroy(shp_ctx *ctx);                         /* deallocate_core */
 const char *shp_last_error(const shp_ctx *ctx);
 

@@ -54,6 +54,9 @@ def lib():
         L.orc_set_eqt.argtypes = [P, ctypes.c_double]
         L.orc_get_eqt.argtypes = [P, dp]
         L.orc_get_eqt.restype = None
+        L.orc_pset_eqt.argtypes = [P, ctypes.c_double]
+        L.orc_pget_eqt.argtypes = [P, dp]
+        L.orc_pget_eqt.restype = None
         L.orc_sweep_0b_dyn.argtypes = [P, ctypes.c_int]
         L.orc_get_gtau.argtypes = [P, dp, dp, dp]
         L.orc_get_gtau.restype = None
@@ -471,6 +474,14 @@ class OracleProjChain:
 
     def reset_diag(self):
         lib().orc_preset_diag(self._c)
+
+    def set_eqt(self, theta=0.0):
+        lib().orc_pset_eqt(self._c, float(theta))
+
+    def eqt(self):
+        o = np.zeros(2 * (20 * self.lq + 8))
+        lib().orc_pget_eqt(self._c, _dptr(o))
+        return o[0::2] + 1j * o[1::2]
 
     def obs(self):
         o = np.zeros(11)

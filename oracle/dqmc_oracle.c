@@ -3055,6 +3055,8 @@ void orc_preset_diag(orc_pctx *c)
     c->max_wrap_error = c->max_phasediff = c->nchecks = c->sumlog10err = c->accepted = c->tried = 0;
 }
 void orc_pget_obs(const orc_pctx *c, double *o) { orc_get_obs(c->b, o); }
+int orc_pset_eqt(orc_pctx *c, double theta) { return orc_set_eqt(c->b, theta); }
+void orc_pget_eqt(const orc_pctx *c, double *out) { orc_get_eqt(c->b, out); }
 void orc_preset_obs(orc_pctx *c) { orc_reset_obs(c->b); }
 void orc_pset_trace(orc_pctx *c, double *buf, int cap) { orc_set_trace(c->b, buf, cap); }
 int orc_ptrace_count(const orc_pctx *c) { return c->b->trace_n; }

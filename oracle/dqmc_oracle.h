@@ -147,6 +147,9 @@ void orc_pget_state(const orc_pctx *c, double *ul, double *ur, double *ulrinv);
 void orc_pget_diag(const orc_pctx *c, double *d);
 void orc_preset_diag(orc_pctx *c);
 void orc_pget_obs(const orc_pctx *c, double *o);
+/* the correlation bins of the measured slices (layout of orc_get_eqt) */
+int orc_pset_eqt(orc_pctx *c, double theta);
+void orc_pget_eqt(const orc_pctx *c, double *out);
 void orc_preset_obs(orc_pctx *c);
 void orc_pset_trace(orc_pctx *c, double *buf, int cap);
 int orc_ptrace_count(const orc_pctx *c);

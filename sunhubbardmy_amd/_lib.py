@@ -44,7 +44,7 @@ EXPORTS_PROJ = [
     "shp_draw_uniforms", "shp_set_proj", "shp_sweep_start_0b", "shp_sweep_b0", "shp_sweep_0b", "shp_get_g",
     "shp_get_state", "shp_get_diag", "shp_reset_diag", "shp_get_obs", "shp_reset_obs", "shp_set_trace",
     "shp_get_trace", "shp_timers", "shp_synchronize", "shp_set_uconf", "shp_get_uconf", "shp_get_rng_state",
-    "shp_set_rng_state",
+    "shp_set_rng_state", "shp_set_eqt", "shp_get_eqt",
 ]
 
 
@@ -128,6 +128,9 @@ def lib():
     L.shp_timers.argtypes = [P, ctypes.c_int, dp]
     L.shp_synchronize.argtypes = [P]
     L.shp_get_rng_state.argtypes = [P, ctypes.c_int, u32p]
+    L.shp_set_eqt.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, ip, ctypes.c_int, ip,
+                              ctypes.c_double]
+    L.shp_get_eqt.argtypes = [P, ctypes.c_int, dp]
     L.shp_set_rng_state.argtypes = [P, ctypes.c_int, u32p]
     for name in EXPORTS_PROJ:
         if name != "shp_last_error":

@@ -15,6 +15,8 @@ struct shp_ctx : ctx_base {
     std::vector<PChain> hc;
     PChain *dc = nullptr;
     DChain *dview = nullptr;      // measurement view of each chain (G, bonds, staggering, bins)
+    std::vector<DChain> hview;    // host copy of the views
+    size_t eqt_lds = 0;           // LDS of k_eqt, 0 while no chain has correlation bins
     int32_t *d_seeds = nullptr;
     std::vector<cplx *> trace_bufs;
 };
@@ -193,6 +195,7 @@ void measure(shp_ctx *ctx)
     const int nb = (ctx->n + 31) / 32;
     hipLaunchKernelGGL(k_measure_tiles, dim3(nb * (nb + 1) / 2, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dview);
     hipLaunchKernelGGL(k_measure, dim3(ctx->nch), dim3(256), 0, ctx->st, ctx->dview);
+    if (ctx->eqt_lds) hipLaunchKernelGGL(k_eqt, dim3(ctx->nch), dim3(256), ctx->eqt_lds, ctx->st, ctx->dview);
 }
 
 bool in_window(const shp_ctx *ctx, int nt)
@@ -467,6 +470,7 @@ int shp_create(shp_ctx **out, int nchains, const shp_model *m, int device)
     }
     ctx->dc = dupload<PChain>(ctx, ctx->hc.data(), nchains);
     ctx->dview = dupload<DChain>(ctx, views.data(), nchains);
+    ctx->hview = views;
     ctx->d_seeds = dupload<int32_t>(ctx, seeds.data(), nchains);
     if (!ctx->dc || !ctx->dview || !ctx->d_seeds) { shp_destroy(ctx); return SH_E_HIP; }
     hipLaunchKernelGGL(kp_seed, dim3((nchains + 63) / 64), dim3(64), 0, ctx->st, ctx->dc, nchains, ctx->d_seeds);
@@ -755,10 +759,35 @@ int shp_get_obs(shp_ctx *ctx, int chain, double *o)
     return 0;
 }
 
+int shp_set_eqt(shp_ctx *ctx, int chain, int l1, int l2, int l3, const int32_t *sites, const int32_t *cells,
+                int zplq, const int32_t *plq, double theta)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch) return SH_E_ARG;
+    DChain &v = ctx->hview[chain];
+    size_t lds = 0;
+    int rc = eqt_setup(ctx, v, ctx->hview, l1, l2, l3, sites, cells, zplq, plq, theta, lds);
+    if (rc) return rc;
+    ctx->eqt_lds = lds;
+    HIPCHK(hipMemcpy(ctx->dview + chain, &v, sizeof(DChain), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int shp_get_eqt(shp_ctx *ctx, int chain, double *bins)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !bins) return SH_E_ARG;
+    const DChain &v = ctx->hview[chain];
+    if (!v.eqb) return fail(ctx, SH_E_STATE, "correlation bins not set up (shp_set_eqt)");
+    HIPCHK(hipMemcpyAsync(bins, v.eqb, sizeof(cplx) * (20 * (size_t)v.lq + 8), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
 int shp_reset_obs(shp_ctx *ctx)
 {
     if (!ctx) return SH_E_ARG;
     for (auto &c : ctx->hc) HIPCHK(hipMemsetAsync(c.sc + SC_NOBS, 0, 11 * sizeof(double), ctx->st));
+    for (auto &v : ctx->hview)
+        if (v.eqb) HIPCHK(hipMemsetAsync(v.eqb, 0, sizeof(cplx) * (20 * (size_t)v.lq + 8), ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     return 0;
 }

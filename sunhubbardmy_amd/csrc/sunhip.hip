@@ -831,13 +831,13 @@ int sh_set_dyn(sh_ctx *ctx, int chain, int ntdm, int l1, int l2, int l3, const i
 // LDS bytes of k_eqt for a chain
 static size_t eqt_lds(const DChain &c) { return sizeof(cplx) * ((size_t)16 * c.lq + 2 * EQT_T * (EQT_T + 1) + c.n); }
 
-int sh_set_eqt(sh_ctx *ctx, int chain, int l1, int l2, int l3, const int32_t *sites, const int32_t *cells, int zplq,
-               const int32_t *plq, double theta)
+// the correlation bins of one chain (shared by the finite-temperature and projective batches):
+// validation, device tables, the LDS size of k_eqt over every chain of the batch that has bins
+static int eqt_setup(ctx_base *ctx, DChain &c, const std::vector<DChain> &all, int l1, int l2, int l3,
+                     const int32_t *sites, const int32_t *cells, int zplq, const int32_t *plq, double theta,
+                     size_t &lds_out)
 {
-    if (!ctx || chain < 0 || chain >= ctx->nch || !sites || !cells || l1 < 1 || l2 < 1 || l3 < 1 || zplq < 0 ||
-        (zplq > 0 && !plq))
-        return SH_E_ARG;
-    DChain &c = ctx->hc[chain];
+    if (!sites || !cells || l1 < 1 || l2 < 1 || l3 < 1 || zplq < 0 || (zplq > 0 && !plq)) return SH_E_ARG;
     if (l1 * l2 * l3 != c.lq) return fail(ctx, SH_E_ARG, "l1 * l2 * l3 must be the number of unit cells");
     if (eqt_lds(c) > 160 * 1024) return fail(ctx, SH_E_ARG, "the correlation bins need 16 lq + n + 544 <= 10240");
     for (int i = 0; i < c.n; i++)
@@ -863,11 +863,23 @@ int sh_set_eqt(sh_ctx *ctx, int chain, int l1, int l2, int l3, const int32_t *si
     c.zplq = zplq;
     c.theta = theta;
     size_t lds = 0;
-    for (const DChain &d : ctx->hc)
+    for (const DChain &d : all)
         if (d.eqb || &d == &c) lds = std::max(lds, eqt_lds(d));
     if (lds > 64 * 1024 &&
         hipFuncSetAttribute((const void *)k_eqt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return fail(ctx, SH_E_HIP, "k_eqt: LDS request refused");
+    lds_out = lds;
+    return 0;
+}
+
+int sh_set_eqt(sh_ctx *ctx, int chain, int l1, int l2, int l3, const int32_t *sites, const int32_t *cells, int zplq,
+               const int32_t *plq, double theta)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch) return SH_E_ARG;
+    DChain &c = ctx->hc[chain];
+    size_t lds = 0;
+    int rc = eqt_setup(ctx, c, ctx->hc, l1, l2, l3, sites, cells, zplq, plq, theta, lds);
+    if (rc) return rc;
     ctx->eqt_lds = lds;
     HIPCHK(hipMemcpy(ctx->dc + chain, &c, sizeof(DChain), hipMemcpyHostToDevice));
     return 0;

@@ -197,6 +197,24 @@ class ProjWalkers:
     def has_vconf(self):
         return self.tables[0].get("v") is not None
 
+    def set_eqt(self, theta=0.0):
+        """correlation bins of the equal-time measurement at the measured slices (see FTWalkers.set_eqt)"""
+        for c, t in enumerate(self.tables):
+            lat = t["lat"]
+            l1, l2, l3 = lat["dims"]
+            sites = np.ascontiguousarray(lat["cells"], np.int32)
+            cells = np.ascontiguousarray(lat["eqt_cells"], np.int32)
+            plq = np.ascontiguousarray(lat["plq"], np.int32)
+            self._check(_lib.lib().shp_set_eqt(self._ctx, c, l1, l2, l3, sites.ctypes.data_as(_lib.ip),
+                                               cells.ctypes.data_as(_lib.ip), plq.shape[1],
+                                               plq.ctypes.data_as(_lib.ip), float(theta)), "set_eqt")
+
+    def eqt(self, chain=0):
+        """the flat correlation bins of a chain, 20 lq + 8 complex (binio.unpack_eqt splits them)"""
+        b = np.zeros(20 * self.tables[chain]["lq"] + 8, np.complex128)
+        self._check(_lib.lib().shp_get_eqt(self._ctx, chain, b.ctypes.data_as(_lib.dp)), "get_eqt")
+        return b
+
     def rng_state(self, chain=0):
         """The chain's SFMT19937 state (SHP_RNG_WORDS uint32: words, read index)."""
         s = np.zeros(_lib.SHP_RNG_WORDS, np.uint32)

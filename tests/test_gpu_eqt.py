@@ -59,3 +59,39 @@ def test_eqt_parity(name):
     w.reset_obs()
     assert not np.any(w.eqt())
     w.close()
+
+
+@pytest.mark.parametrize("name", ["honeycomb2_tv_dense", "square4_hubbard_cb"])
+def test_eqt_parity_projective(name):
+    from sunhubbardmy_amd.proj import ProjModel, ProjWalkers
+    kw = {"honeycomb2_tv_dense": dict(lattice="honeycomb", la=2, lb=2, beta=1.0, rv=1.2, rndness=0.2, nflr=1,
+                                      kmode="dense", nwrap=5, nustock=8, mid_len=2),
+          "square4_hubbard_cb": dict(lattice="square", la=4, lb=4, beta=1.0, rv=0.0, rhub=4.0, nflr=2, rndness=0.2,
+                                     kmode="checkerboard", nwrap=4, nustock=6, mid_len=2)}[name]
+    seed = chain_seed(3)
+    say("case", name, "seed", seed)
+    w = ProjWalkers([ProjModel(seed=seed, **kw)])
+    o = oracle.OracleProjChain(seed=seed, **kw)
+    w.init_conf()
+    o.init_conf()
+    w.set_trial()
+    o.set_trial()
+    o.set_proj(w.projs[0])
+    w.set_eqt(THETA)
+    o.set_eqt(THETA)
+    w.sweep_start_0b()
+    o.sweep_start()
+    for _ in range(2):
+        w.sweep_b0(True, True)
+        o.sweep_b0(True, True)
+        w.sweep_0b(True, True)
+        o.sweep_0b(True, True)
+    assert np.array_equal(w.get_conf(), o.conf)
+    uw, uo = binio.unpack_eqt(w.eqt(), o.lq), binio.unpack_eqt(o.eqt(), o.lq)
+    assert o.obs()["nobs"] > 0
+    for k in binio.KEYS22 + binio.KEYS1 + binio.SCALARS:
+        a, b = np.asarray(uw[k]), np.asarray(uo[k])
+        err = np.abs(a - b).max() / max(np.abs(b).max(), 1.0)
+        say("  ", k, err)
+        assert err < 1e-9, k
+    w.close()
