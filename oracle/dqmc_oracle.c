@@ -1042,10 +1042,10 @@ static void bmat_left_forward_hc(orc_ctx *c, int nt1, int nt2, zc *M)
 {
     for (int nt = nt1; nt >= nt2; nt--) {
         khalf_left(c, c->urt, M, 0);
-        /* families nn_nf..1 (src/dqmc_ft_core.f90:150-156); the real bond factors are their own
-         * conjugates */
-        for (int nf = c->nn_nf - 1; nf >= 0 && c->lwrapv; nf--) { fv_left(c, nt, nf, 0, M); fv_left(c, nt, nf, 1, M); }
         v_rows(c, nt, c->bmat, 1, M);
+        /* then the bond field, families nn_nf..1 (src/dqmc_ft_core.f90:141-156: the on-site field
+         * comes first in the conjugated product); the real bond factors are their own conjugates */
+        for (int nf = c->nn_nf - 1; nf >= 0 && c->lwrapv; nf--) { fv_left(c, nt, nf, 0, M); fv_left(c, nt, nf, 1, M); }
         khalf_left(c, c->urt, M, 1);
     }
 }
@@ -1950,7 +1950,7 @@ void orc_get_g(const orc_ctx *c, double *g) { memcpy(g, c->G, sizeof(zc) * (size
  * znn_tau(lq, 2, 2, ntdm+1) (src/dqmc_measure_2d.f90:30, :61) */
 int orc_set_dyn(orc_ctx *c, int ntdm)
 {
-    if (ntdm < 0 || ntdm > c->ltrot || c->lwrapv) return -1;
+    if (ntdm < 0 || ntdm > c->ltrot) return -1;
     size_t nn = (size_t)c->ndim * c->ndim;
     free(c->gt0); free(c->g0t); free(c->g00); free(c->znn);
     c->gt0 = zalloc(nn); c->g0t = zalloc(nn); c->g00 = zalloc(nn);
@@ -2066,6 +2066,8 @@ static void bmat_right_backward_slice(orc_ctx *c, int nt, zc *M)
 {
     int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
     if (c->lwrapT) for (int q = 0; q < nfam; q++) k_right(c, q, c->urtm1, M);
+    /* bond field: right_backward nflag 2 then 1, families 1..nn_nf */
+    for (int nf = 0; nf < c->nn_nf && c->lwrapv; nf++) { fv_right(c, nt, nf, 0, M); fv_right(c, nt, nf, 1, M); }
     v_cols(c, nt, c->bmat_inv, M);
     if (c->lwrapT) for (int q = nfam - 1; q >= 0; q--) k_right(c, q, c->urtm1, M);
 }
@@ -2100,6 +2102,12 @@ int orc_sweep_0b_dyn(orc_ctx *c, int lmeasure)
         if (c->lwrapT) {
             int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
             for (int q = 0; q < nfam; q++) { k_left(c, q, c->urt, c->G); k_right(c, q, c->urtm1, c->G); }
+        }
+        for (int nf = 0; nf < c->nn_nf && c->lwrapv; nf++) {
+            fv_left(c, nt, nf, 0, c->G);
+            fv_right(c, nt, nf, 0, c->G);
+            fv_left(c, nt, nf, 1, c->G);
+            fv_right(c, nt, nf, 1, c->G);
         }
         if (c->lwrapu) {
             v_rows(c, nt, c->bmat, 0, c->G);

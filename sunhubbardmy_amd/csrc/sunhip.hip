@@ -314,18 +314,19 @@ static void vupdate(sh_ctx *ctx, int nt, int nf, int slice, int p)
     hipLaunchKernelGGL(k_vupdate, dim3(ctx->nch), dim3(256), (size_t)4 * ctx->nmax * sizeof(cplx), ctx->st, ctx->dc,
                        nt, nf, slice, p);
 }
-// one slice of Bmat_left_forward / _hc with the bond field: K-half, the families (nflag 2 then 1;
-// 1..nn_nf forward, nn_nf..1 for _hc: src/dqmc_ft_core.f90:98-104, :150-156), V, K-half
+// one slice of Bmat_left_forward with the bond field: K-half, the families 1..nn_nf (nflag 2
+// then 1), V, K-half (src/dqmc_ft_core.f90:80-119); of Bmat_left_forward_hc: K-half, V^H, the
+// families nn_nf..1, K-half (:128-171: the conjugated product takes the on-site field first)
 static void bprod_slice_v(sh_ctx *ctx, int nt, int hc)
 {
     const int nfam = ctx->hc[0].nn_nf;
-    left_pass(ctx, M_W1, 0, 0, 1, 0, nt, nt, 1);
+    left_pass(ctx, M_W1, 0, 0, hc ? 1 | 2 : 1, hc ? 2 : 0, nt, nt, 1);
     for (int q = 0; q < nfam; q++) {
         const int nf = hc ? nfam - 1 - q : q;
         vprop(ctx, M_W1, nf, 0, 0, nt);
         vprop(ctx, M_W1, nf, 1, 0, nt);
     }
-    left_pass(ctx, M_W1, 0, 0, 2 | 4, hc ? 2 : 0, nt, nt, 1);
+    left_pass(ctx, M_W1, 0, 0, hc ? 4 : 2 | 4, 0, nt, nt, 1);
 }
 
 // dqmc_ft_stablize_0b_qr(n): point n of the right chain from point n-1
@@ -799,7 +800,6 @@ int sh_set_dyn(sh_ctx *ctx, int chain, int ntdm, int l1, int l2, int l3, const i
         l3 < 1)
         return SH_E_ARG;
     if (ctx->ntdm >= 0 && ntdm != ctx->ntdm) return fail(ctx, SH_E_ARG, "all chains of a batch share ntdm");
-    if (ctx->lwrapv) return fail(ctx, SH_E_ARG, "the time-displaced pass is not built for the t-V bond field");
     DChain &c = ctx->hc[chain];
     if (l1 * l2 * l3 != c.lq) return fail(ctx, SH_E_ARG, "l1 * l2 * l3 must be the number of unit cells");
     for (int i = 0; i < c.n; i++)
@@ -1063,19 +1063,46 @@ static int sweep_0b_dyn(sh_ctx *ctx, int lmeasure)
     }
     dyn_measure(0);
     for (int nt = 1; nt <= ctx->ltrot; nt++) {
-        {
+        if (!ctx->lwrapv) {
             Tic t(ctx, T_WRAP);
             left_pass(ctx, M_G, 0, 0, 1 | 2, 0, nt, nt, 1);
             right_pass(ctx, 1, 1 | 2, 1, nt);
             left_pass(ctx, M_G, 0, 0, 4, 0, nt, nt, 1);
             right_pass(ctx, 1, 4, 1, nt);
+        } else {
+            // K-half, the bond field by families 1..nn_nf (nflag 2 then 1), V, K-half
+            Tic t(ctx, T_WRAP);
+            const int nfam = ctx->hc[0].nn_nf;
+            left_pass(ctx, M_G, 0, 0, 1, 0, nt, nt, 1);
+            right_pass(ctx, 1, 1, 1, nt);
+            for (int nf = 0; nf < nfam; nf++) {
+                vprop(ctx, M_G, nf, 0, 0, nt);
+                vprop(ctx, M_G, nf, 0, 1, nt);
+                vprop(ctx, M_G, nf, 1, 0, nt);
+                vprop(ctx, M_G, nf, 1, 1, nt);
+            }
+            left_pass(ctx, M_G, 0, 0, 2 | 4, 0, nt, nt, 1);
+            right_pass(ctx, 1, 2 | 4, 1, nt);
         }
         if (lmeasure) measure(ctx);
-        {
+        if (!ctx->lwrapv) {
             // G(t,0) <- B(t) G(t,0), G(0,t) <- G(0,t) B(t)^-1 (Bmat_left_forward / _right_backward)
             Tic t(ctx, T_WRAP);
             left_pass(ctx, M_GT0, 0, 0, 1 | 2 | 4, 0, nt, nt, 1);
             right_pass(ctx, 1, 1 | 2 | 4, 1, nt, M_G0T);
+        } else {
+            Tic t(ctx, T_WRAP);
+            const int nfam = ctx->hc[0].nn_nf;
+            left_pass(ctx, M_GT0, 0, 0, 1, 0, nt, nt, 1);
+            right_pass(ctx, 1, 1, 1, nt, M_G0T);
+            for (int nf = 0; nf < nfam; nf++) {
+                vprop(ctx, M_GT0, nf, 0, 0, nt);
+                vprop(ctx, M_GT0, nf, 1, 0, nt);
+                vprop(ctx, M_G0T, nf, 0, 1, nt);
+                vprop(ctx, M_G0T, nf, 1, 1, nt);
+            }
+            left_pass(ctx, M_GT0, 0, 0, 2 | 4, 0, nt, nt, 1);
+            right_pass(ctx, 1, 2 | 4, 1, nt, M_G0T);
         }
         if (ctx->iwrap_nt[nt] > 0) {
             int n = ctx->iwrap_nt[nt];

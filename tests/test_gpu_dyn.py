@@ -25,6 +25,10 @@ CASES = {
                           update="delay", nublock=4),
     "cubic4_dense": dict(lattice="cubic", la=4, lb=4, lc=4, beta=0.8, rhub=4.0, kmode="dense", nwrap=4,
                          update="fast"),
+    "square4_tv": dict(lattice="square", la=4, lb=4, beta=1.0, rhub=0.0, rv=1.5, nflr=1, kmode="checkerboard",
+                       nwrap=4, update="delay", nublock=4),
+    "honeycomb3_tuv_dense": dict(lattice="honeycomb", la=3, lb=3, beta=1.0, rhub=3.0, rv=1.0, nflr=2, kmode="dense",
+                                 nwrap=5, update="delay", nublock=4),
 }
 
 
@@ -53,6 +57,8 @@ def test_dyn_parity(name):
     w.sweep(lupdate=True, lmeasure=False)
     o.sweep(lupdate=True, lmeasure=False)
     assert np.array_equal(w.get_conf(), o.conf), "field differs after the updating sweep"
+    if kw.get("rv"):
+        assert np.array_equal(w.get_confv(), o.confv), "bond field differs after the updating sweep"
     w.sweep_b0(False, False)
     o.sweep_b0(lupdate=False, lmeasure=False)
     g_before = w.G()

@@ -65,6 +65,7 @@ def test_tv_parity(name):
         assert rel(tw[k], to[k]) < 1e-10, "G differs at stabilization %d: %g" % (k, rel(tw[k], to[k]))
     dw, do = w.diag(), o.diag()
     assert abs(dw["phase"] - do["phase"]) < 1e-10
+    assert dw["max_wrap_error"] < 1e-8 and do["max_wrap_error"] < 1e-8
     assert np.array_equal(w.vacc(), o.vacc()) and 0 < w.vacc()[0] < w.vacc()[1]
     ow, oo = w.obs(), o.obs()
     assert ow["nobs"] == oo["nobs"]

@@ -21,6 +21,8 @@ CASES = {
     "honeycomb2_u3": dict(lattice="honeycomb", la=2, lb=2, beta=1.2, rhub=3.0, kmode="checkerboard", nwrap=3),
     "chain8_u4": dict(lattice="chain", la=8, lb=1, beta=1.0, rhub=4.0, kmode="checkerboard", nwrap=4),
     "cubic4_u4": dict(lattice="cubic", la=4, lb=4, lc=4, beta=0.6, rhub=4.0, kmode="checkerboard", nwrap=3),
+    "square4_tv": dict(lattice="square", la=4, lb=4, beta=1.0, rhub=0.0, rv=1.5, nflr=1, kmode="checkerboard",
+                       nwrap=4),
 }
 
 
@@ -37,8 +39,25 @@ def _cells(t):
     return [((s // nsub) // m.lb + 1, (s // nsub) % m.lb + 1, 1, s % nsub) for s in range(n)], (m.la, m.lb, 1)
 
 
-def _slices(t, conf, bm):
-    """[B(1), ..., B(ltrot)] with B(t) = A2 exp(V(t)) A1 from the checkerboard bond matrices"""
+def _vfield(t, confv, nt):
+    """the t-V bond factors of slice nt, V_nn ... V_1 with V_f = prod over the bonds of u diag(bp, bm) u"""
+    lat, n, m = t["lat"], t["ndim"], t["model"]
+    al = np.arccosh(np.exp(m.dtau * m.rv * 0.5))
+    u = np.array([[1.0, 1.0], [1.0, -1.0]]) / np.sqrt(2.0)
+    V = np.eye(n)
+    for nf in range(lat["nn_nf"]):
+        for q in range(lat["nn_lf"]):
+            i1, i2 = lat["bonds"][nf, q]
+            e = (-1.0, 1.0)[confv[nt, nf, q] - 1]
+            E = np.eye(n)
+            E[np.ix_([i1, i2], [i1, i2])] = u @ np.diag([np.exp(al * e), np.exp(-al * e)]) @ u
+            V = E @ V
+    return V
+
+
+def _slices(t, conf, bm, confv=None):
+    """[B(1), ..., B(ltrot)] with B(t) = A2 exp(V(t)) [bond factors] A1 from the checkerboard bond
+    matrices"""
     lat, n = t["lat"], t["ndim"]
     fams = []
     for nf in range(lat["nn_nf"]):
@@ -56,7 +75,9 @@ def _slices(t, conf, bm):
     A2 = np.eye(n, dtype=complex)
     for F in fams[::-1]:
         A2 = F @ A2
-    return [A2 @ np.diag(bm[conf[nt] - 1]) @ A1 for nt in range(t["ltrot"])]
+    if confv is None:
+        return [A2 @ np.diag(bm[conf[nt] - 1]) @ A1 for nt in range(t["ltrot"])]
+    return [A2 @ np.diag(bm[conf[nt] - 1]) @ _vfield(t, confv, nt) @ A1 for nt in range(t["ltrot"])]
 
 
 def _run(name, ntdm=None):
@@ -81,7 +102,7 @@ def _rel(a, b):
 def test_dyn_pass_matches_dense_products(name):
     o, t = _run(name)
     n = t["ndim"]
-    Bs = _slices(t, o.conf, t["hs"]["bmat"])
+    Bs = _slices(t, o.conf, t["hs"]["bmat"], o.confv if CASES[name].get("rv") else None)
     Bb = np.eye(n)
     for B in Bs:
         Bb = B @ Bb
