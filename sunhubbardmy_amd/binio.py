@@ -161,3 +161,29 @@ def write_bin(path, lattice, l1, l2, nobs, obs, eqt, rv=0.0, l3=1):
         # znnnn and zCrmax are never accumulated in the reference (their calls are commented out)
         with open(os.path.join(path, name), "a") as f:
             _record(f, _reals(np.array([v]) / rnorm))
+
+
+def write_dyn_bin(path, lattice, l1, l2, nsweep, znn):
+    """Append one bin of dyn_output (src/dqmc_measure/dyn_output.f90: square and honeycomb):
+    znn = per chain the library's znn_tau bins [ntdm+1][no_j][no_i][lq], summed over nsweep
+    dynamic passes; normalised by 1 / (nsweep * chains) and written as nn_rtau.bin (per imj, per
+    t: 8e16.8 of gr(imj, 1:nsub, 1:nsub, t)) and nn_ktau.bin (per k: its 2f16.8 coordinates, then
+    per t the 8e16.8 Fourier components)."""
+    if lattice not in ("square", "honeycomb"):
+        raise ValueError("the dyn_output layout covers the square and honeycomb lattices")
+    nsub = 2 if lattice == "honeycomb" else 1
+    z = np.sum(np.asarray(znn), axis=0) / float(nsweep * len(znn))     # [t][no_j][no_i][imj]
+    gr = z.transpose(3, 2, 1, 0)[:, :nsub, :nsub, :]                   # [imj][no_i][no_j][t]
+    lq, nt = gr.shape[0], gr.shape[3]
+    ph, ks = _phase_table((l1, l2))
+    os.makedirs(path, exist_ok=True)
+    with open(os.path.join(path, "nn_rtau.bin"), "a") as f:
+        for imj in range(lq):
+            for t in range(nt):
+                _record(f, _reals(gr[imj, :, :, t].T))
+    gk = np.tensordot(1.0 / ph, gr, axes=(1, 0)) / lq
+    with open(os.path.join(path, "nn_ktau.bin"), "a") as f:
+        for nk in range(lq):
+            f.write("%16.8f%16.8f\n" % (ks[nk, 0] / l1, ks[nk, 1] / l2))
+            for t in range(nt):
+                _record(f, _reals(gk[nk, :, :, t].T))

@@ -248,3 +248,26 @@ def test_write_bin_chain_and_cube(tmp_path):
             assert (d / name).exists(), name
         if lattice == "cubic":
             assert not (d / "jj_k.bin").exists()
+
+
+def test_write_dyn_bin(tmp_path):
+    la = 4
+    o = oracle.OracleChain(lattice="square", la=la, lb=la, beta=0.6, rhub=4.0, nwrap=3, seed=oracle.chain_seed(4))
+    o.init_conf()
+    o.sweep_start()
+    o.sweep(lupdate=True, lmeasure=False)
+    o.sweep_b0(lupdate=False, lmeasure=False)
+    o.set_dyn(o.ltrot)
+    o.sweep_0b_dyn()
+    z = o.znn()
+    binio.write_dyn_bin(str(tmp_path), "square", la, la, 1, [z])
+    r = np.loadtxt(tmp_path / "nn_rtau.bin", ndmin=2)
+    assert r.shape == (la * la * (o.ltrot + 1), 2)
+    assert np.allclose(r[:o.ltrot + 1, 0], z[:, 0, 0, 0].real, rtol=1e-7, atol=1e-12)
+    lines = (tmp_path / "nn_ktau.bin").read_text().splitlines()
+    assert len(lines) == la * la * (o.ltrot + 2)
+    # k = 0 block: the cell average of every t
+    ks = [tuple(map(float, lines[i * (o.ltrot + 2)].split())) for i in range(la * la)]
+    i0 = ks.index((0.0, 0.0))
+    k0 = np.array([list(map(float, lines[i0 * (o.ltrot + 2) + 1 + t].split())) for t in range(o.ltrot + 1)])
+    assert np.allclose(k0[:, 0], z[:, 0, 0, :].real.mean(axis=1), rtol=1e-7, atol=1e-12)
