@@ -593,6 +593,13 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
             delete ctx;
             return rc;
         }
+        // the pass loops take the family count and the on-site switch of the batch, so chains
+        // with a bond field share nn_nf, and every chain shares lwrapu (U != 0)
+        if ((q.vtab && q.nn_nf != m[0].nn_nf) || q.lwrapu != m[0].lwrapu) {
+            int rc = fail(ctx, SH_E_ARG, "chains of a batch must share lwrapu, and nn_nf when they carry the bond field");
+            delete ctx;
+            return rc;
+        }
         for (int b = 0; b < q.nn_nf * q.nn_lf * 2; b++)
             if (q.bonds[b] < 0 || q.bonds[b] >= q.ndim) { delete ctx; return SH_E_ARG; }
         for (int i = 0; i <= q.ltrot; i++)

@@ -125,6 +125,25 @@ def test_create_rejects_bad_arguments_before_touching_the_device():
     assert _lib.lib().sh_last_error(None) == b"null context"
 
 
+def test_create_rejects_batches_the_pass_loops_cannot_share():
+    """The pass loops take the bond-family count and the on-site switch of the whole batch: a
+    batch whose bond-field chains differ in nn_nf (square 4 families, chain 2), or whose chains
+    differ in lwrapu (U = 0 next to U != 0), is refused with SH_E_ARG before any device call."""
+    tsq = make_tables(Model(lattice="square", la=4, lb=4, beta=1.0, rv=1.0))
+    tch = make_tables(Model(lattice="chain", la=16, lb=1, beta=1.0, rv=1.0))
+    assert tsq["v"] is not None and tch["v"] is not None
+    a, ka = _model_struct(tsq)
+    b, kb = _model_struct(tch)
+    assert _create([a, b])[0] == -1
+    u0 = make_tables(Model(la=4, lb=4, beta=1.0, rhub=0.0))
+    u4 = make_tables(Model(la=4, lb=4, beta=1.0, rhub=4.0))
+    c, kc = _model_struct(u0)
+    d, kd = _model_struct(u4)
+    assert c.lwrapu == 0 and d.lwrapu == 1
+    assert _create([c, d])[0] == -1
+    assert _create([d, c])[0] == -1
+
+
 def test_walkers_raise_on_mixed_batches():
     with pytest.raises(SweepError):
         from sunhubbardmy_amd import FTWalkers
