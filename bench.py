@@ -12,11 +12,21 @@ sweep (sunhubbardmy_amd/autocorr.py tau_int_batch), so that the relaxation every
 batch still shares after the warm-up is not read as autocorrelation.  The line also carries the
 estimate with per-chain means only (tau_int_per_chain_means).
 
-Learned vs plain: the last 40 warm-up sweeps (plain delay update) feed every chain's
-configuration after every pass to the effective-action network (40 sweeps x 2 passes x 256
-chains = 20480 configurations), which the library's FP32 kernels then train.  The same chains are timed first
-with the plain sweep, then with block proposals after every slice's local update; value is the
-learned form's tau_int-corrected samples/s, and the line carries both phases and their ratio.
+Sequence: W warm-up sweeps (at least 20) of the plain delay update from random fields, then 40
+more plain sweeps whose configurations after every pass train the effective-action network
+(40 sweeps x 2 passes x 256 chains = 20480 configurations, whatever W is; the library's FP32
+kernels train it).  The same chains are then timed for exactly K sweeps with the plain sweep,
+and K sweeps with block proposals after every slice's local update.  Each phase continues
+untimed to --tau-sweeps sweeps in all, so that tau_int comes from a series long enough to
+resolve it (the window closes before a quarter of the series); the timed sweeps are the first
+K of that series.  value is the learned form's tau_int-corrected samples/s; the line carries
+both phases, every tau with its Sokal error, window and a resolved flag, and learned_over_plain
+with its error.  A tau whose window did not close makes value null.
+
+CPU baseline: the oracle (plain C restatement of the same sweep) runs one chain per core on up
+to 16 cores of the host for one sweep each; the line states the core count and the CPU model,
+and a one-core sweep at U = 4 relates the oracle to the reference's single-core figure at that
+shape (0.366 sweeps/s, measured on a 2.7 GHz Xeon when the reference was surveyed).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--chains C] [--nublock k] [--proposal both]
 """
@@ -63,20 +73,58 @@ def algorithmic_work(group, n, nchains, ltrot, nwrap, acc, k):
     return "bytes", nchains * 2 * 16 * nn
 
 
-def cpu_baseline(kw, tau):
-    """The CPU oracle (plain C, one thread) on a bounded sample: one sweep of one chain."""
+def _oracle_sweep(args):
+    """One measured sweep of one oracle chain (run in a worker process); returns its seconds."""
+    kw, chain = args
     import oracle
-    o = oracle.OracleChain(seed=oracle.chain_seed(0), **kw)
+    o = oracle.OracleChain(seed=oracle.chain_seed(chain), **kw)
     o.init_conf()
     o.sweep_start()
     t0 = time.perf_counter()
     o.sweep(lupdate=True, lmeasure=True)
-    dt = time.perf_counter() - t0
-    sweeps_per_s = 1.0 / dt
-    return {"value": sweeps_per_s / (2.0 * tau), "unit": "samples/s", "cores": 1, "kind": "port",
-            "sample": "1 sweep (b0+0b, measured) of one chain of the same workload, %.1f s; "
-                      "sweeps/s %.4g divided by 2 tau_int of the GPU run" % (dt, sweeps_per_s),
-            "sweeps_per_s": sweeps_per_s}
+    return time.perf_counter() - t0
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(kw):
+    """The CPU oracle on a bounded sample: one measured sweep of one chain per core, all cores
+    at once (one process each, the reference's one chain per MPI rank), plus one sweep at U = 4
+    on one core for the calibration against the reference's surveyed single-core rate.  Runs
+    before this process touches the GPU; the tau_int division happens after the GPU run."""
+    import multiprocessing as mp
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    cores = max(1, min(16, avail))
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        per = pool.map(_oracle_sweep, [(kw, c) for c in range(cores)])
+    wall = time.perf_counter() - t0
+    agg = cores / max(per)                       # sweeps per second of the whole host
+    kw4 = dict(kw)
+    kw4["rhub"] = 4.0
+    t_u4 = _oracle_sweep((kw4, 0))
+    return {"value": None, "unit": "samples/s", "cores": cores, "kind": "port",
+            "cpu_model": _cpu_model(),
+            "sample": "one measured sweep (b0+0b) of one chain per core, %d chains on %d cores in parallel "
+                      "(slowest %.1f s, wall %.1f s incl. start-up); sweeps/s divided by 2 tau_int of the GPU's plain "
+                      "sweep" % (cores, cores, max(per), wall),
+            "sweeps_per_s": agg, "sweeps_per_s_one_core": 1.0 / (sum(per) / len(per)),
+            "calibration_u4": {"oracle_sweeps_per_s_one_core": 1.0 / t_u4,
+                               "reference_sweeps_per_s_one_core": 0.366,
+                               "note": "same shape at U = 4; the reference figure was measured on another CPU "
+                                       "(2.7 GHz Xeon, AVX-512, MKL) when the reference was surveyed"}}
 
 
 def copy_calibration():
@@ -132,9 +180,10 @@ def bench_pqmc(args):
     print(json.dumps(line), flush=True)
 
 
-def timed_phase(w, steps, timers, label):
-    """steps sweeps with measurement, bracketed by device synchronisation; per-sweep bins."""
-    from sunhubbardmy_amd.autocorr import tau_int, tau_int_batch
+def timed_phase(w, steps, tau_sweeps, timers, label):
+    """steps sweeps with measurement, bracketed by device synchronisation, then untimed sweeps up
+    to tau_sweeps in all; per-sweep bins of the whole series give tau_int."""
+    from sunhubbardmy_amd.autocorr import tau_estimate
     w.reset_obs()
     w.reset_diag()
     w.synchronize()
@@ -150,24 +199,35 @@ def timed_phase(w, steps, timers, label):
     w.synchronize()
     dt = time.perf_counter() - t0
     tm = w.timers(enable=False)
-    series = np.array(series)            # [steps, chains, 5]
-    taus, taus_chain = {}, {}
-    for name, col in (("dblocc", 3), ("ekin", 2), ("s_pipi", 4)):
-        tau, win, _ = tau_int_batch(series[:, :, col].T)
-        taus[name] = round(tau, 3)
-        taus_chain[name] = round(tau_int(series[:, :, col].T)[0], 3)
-    tau = max(taus.values())
-    sweeps_per_s = w.nchains * steps / dt
     nd = min(w.nchains, 8)
     acc = np.mean([w.diag(c)["accepted"] / max(w.diag(c)["tried"], 1) for c in range(nd)])
+    for s in range(steps, max(steps, tau_sweeps)):
+        w.sweep(lupdate=True, lmeasure=True)
+        z, nobs = w.obs_all(reset=True)
+        series.append(np.real(z) / nobs[:, None])
+        if s % 20 == 0:
+            print("# %s tau series sweep %d" % (label, s), file=sys.stderr, flush=True)
+    series = np.array(series)            # [sweeps, chains, 5]
+    taus, taus_chain = {}, {}
+    for name, col in (("dblocc", 3), ("ekin", 2), ("s_pipi", 4)):
+        r = tau_estimate(series[:, :, col].T, batch=True)
+        taus[name] = {"tau": round(r["tau"], 4), "sigma": round(r["sigma"], 4), "window": r["window"],
+                      "max_lag": r["max_lag"], "resolved": r["resolved"]}
+        rc = tau_estimate(series[:, :, col].T, batch=False)
+        taus_chain[name] = {"tau": round(rc["tau"], 4), "window": rc["window"], "resolved": rc["resolved"]}
+    slow = max(taus, key=lambda k: taus[k]["tau"])
+    tau, sig = taus[slow]["tau"], taus[slow]["sigma"]
+    resolved = all(v["resolved"] for v in taus.values())
+    sweeps_per_s = w.nchains * steps / dt
     means = series.mean(axis=(0, 1))
-    return {"dt": dt, "tm": tm, "tau": tau, "taus": taus, "taus_chain": taus_chain, "sweeps_per_s": sweeps_per_s,
-            "samples_per_s": sweeps_per_s / (2.0 * tau), "acc": float(acc), "means": means, "series": series}
+    return {"dt": dt, "tm": tm, "tau": tau, "tau_sigma": sig, "tau_obs": slow, "resolved": resolved, "taus": taus,
+            "taus_chain": taus_chain, "sweeps_per_s": sweeps_per_s, "samples_per_s": sweeps_per_s / (2.0 * tau),
+            "acc": float(acc), "means": means, "series": series, "series_len": int(series.shape[0])}
 
 
 def read_traffic(chains, n, nublock):
-    """HBM bytes per launch of the update kernel from the committed counter summary, when the
-    run is the workload it was measured on."""
+    """HBM bytes per launch of the update kernel, replayed from the committed counter summary
+    (two rocprofv3 --pmc runs of this workload), when the run is the workload it was measured on."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_ft_hubbard16x16_256chains.txt")
     if not (chains == 256 and n == 256 and nublock == 32) or not os.path.exists(path):
         return None
@@ -183,6 +243,10 @@ def main():
     ap.add_argument("--chains", type=int, default=256)
     ap.add_argument("--nublock", type=int, default=32)
     ap.add_argument("--nprop", type=int, default=16, help="inner Metropolis attempts per block proposal (<= 64)")
+    ap.add_argument("--tau-sweeps", type=int, default=100,
+                    help="length of each phase's per-sweep series for tau_int (the first --steps are timed)")
+    ap.add_argument("--train-sweeps", type=int, default=40,
+                    help="plain sweeps after the warm-up whose configurations train the network")
     ap.add_argument("--epochs", type=int, default=30, help="training epochs of the effective action")
     ap.add_argument("--proposal", choices=("both", "plain", "learned"), default="both",
                     help="time the plain delay sweep, the learned-proposal sweep, or both (value = learned)")
@@ -211,6 +275,7 @@ def main():
     from sunhubbardmy_amd.learned import EffectiveAction
 
     kw = model_kw(args)
+    cpu = None if args.no_cpu else cpu_baseline(kw)
     models = [Model(seed=chain_seed(c), **kw) for c in range(args.chains)]
     w = FTWalkers(models)
     t = w.tables[0]
@@ -220,16 +285,18 @@ def main():
     w.set_neighbours()
     w.init_conf()
     w.sweep_start_0b()
-    # warm-up with the plain sweep; its configurations after every pass train the effective action
+    # warm-up with the plain sweep, then train_sweeps more whose configurations after every pass
+    # train the effective action
     learn = args.proposal != "plain"
-    net = EffectiveAction(capacity=max(2 * args.warmup * args.chains, 1), seed=chain_seed(0)) if learn else None
-    first_sample = max(0, args.warmup - 40)
-    for s in range(args.warmup):
+    nwarm = max(args.warmup, 20)
+    net = EffectiveAction(capacity=max(2 * args.train_sweeps * args.chains, 1), seed=chain_seed(0)) if learn else None
+    for s in range(nwarm + args.train_sweeps):
+        collect = learn and s >= nwarm
         w.sweep_b0(True, False)
-        if learn and s >= first_sample:
+        if collect:
             net.add_samples(w)
         w.sweep_0b(True, False)
-        if learn and s >= first_sample:
+        if collect:
             net.add_samples(w)
         if s % 10 == 0:
             print("# warmup %d" % s, file=sys.stderr, flush=True)
@@ -243,11 +310,11 @@ def main():
               file=sys.stderr, flush=True)
     plain = learned = None
     if args.proposal in ("both", "plain"):
-        plain = timed_phase(w, args.steps, args.timers, "plain")
+        plain = timed_phase(w, args.steps, args.tau_sweeps, args.timers, "plain")
     if learn:
         w.set_proposal(net, nprop=args.nprop, local_updates=True)
         w.reset_proposal_stats()
-        learned = timed_phase(w, args.steps, args.timers, "learned")
+        learned = timed_phase(w, args.steps, args.tau_sweeps, args.timers, "learned")
         pst = [w.proposal_stats(c) for c in range(min(args.chains, 16))]
         learned["block"] = {
             "acceptance": sum(x["accepted"] for x in pst) / max(sum(x["moves"] for x in pst), 1),
@@ -261,14 +328,18 @@ def main():
           "checkerboard K, delay update k=%d" % t["nublock"])
 
     def summary(r):
-        return {"samples_per_s": r["samples_per_s"], "sweeps_per_s": r["sweeps_per_s"], "tau_int": r["taus"],
+        return {"samples_per_s": r["samples_per_s"] if r["resolved"] else None, "sweeps_per_s": r["sweeps_per_s"],
+                "tau_int": r["taus"], "tau_int_headline": {"observable": r["tau_obs"], "tau": r["tau"],
+                                                           "sigma": r["tau_sigma"], "resolved": r["resolved"],
+                                                           "series_sweeps": r["series_len"]},
                 "tau_int_per_chain_means": r["taus_chain"],
                 "acceptance": round(r["acc"], 4), "ms_per_step": 1000.0 * r["dt"] / args.steps,
                 "observables": {"dblocc_per_site": r["means"][3] / n, "ekin_per_site_flavour": r["means"][2] / n,
                                 "s_pipi": r["means"][4]}}
 
     line = {
-        "metric": METRIC, "value": head["samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+        "metric": METRIC, "value": head["samples_per_s"] if head["resolved"] else None, "unit": "samples/s",
+        "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1000.0 * head["dt"] / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": wl, "chains": args.chains, "nublock": t["nublock"],
@@ -282,7 +353,11 @@ def main():
         line["learned"]["block"] = learned["block"]
         line["training"] = train
     if plain is not None and learned is not None:
-        line["learned_over_plain"] = learned["samples_per_s"] / plain["samples_per_s"]
+        ratio = learned["samples_per_s"] / plain["samples_per_s"]
+        rel = np.hypot(plain["tau_sigma"] / plain["tau"], learned["tau_sigma"] / learned["tau"])
+        line["learned_over_plain"] = ratio
+        line["learned_over_plain_sigma"] = ratio * rel
+        line["learned_over_plain_sweeps"] = learned["sweeps_per_s"] / plain["sweeps_per_s"]
     if args.timers:
         tm = head["tm"]
         a = head["acc"]
@@ -292,19 +367,23 @@ def main():
         avg_s = ms / 1000.0 / max(cnt, 1)
         achieved = amount / avg_s
         traffic = read_traffic(args.chains, n, t["nublock"]) if best == "update" else None
+        traffic_src = "profiles/hbm_traffic_ft_hubbard16x16_256chains.txt (replayed, not measured in this run)" \
+            if traffic is not None else None
         if kind == "flops":
             roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
                     "frac": achieved / FP64_MFMA_PEAK, "traffic": traffic}
         else:
             roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK, "traffic": traffic}
+        roof["traffic_source"] = traffic_src
         roof["kernel_group"] = best
         roof["avg_launch_ms"] = ms / max(cnt, 1)
         line["roofline"] = roof
         line["kernel_ms"] = {g: round(v[0], 3) for g, v in tm.items()}
         line["kernel_launches"] = {g: v[1] for g, v in tm.items()}
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(kw, (plain or head)["tau"])
+    if cpu is not None:
+        cpu["value"] = cpu["sweeps_per_s"] / (2.0 * (plain or head)["tau"])
+        line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 
 

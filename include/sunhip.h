@@ -26,7 +26,10 @@ extern "C" {
 #define SH_E_NONFINITE (-4) /* non-finite Green's function detected */
 
 #define SH_K_CHECKERBOARD 0 /* BREAKUP_T: 2x2 bond exponentials, nn_nf families */
-#define SH_K_DENSE 1        /* dense exp(-dtau K/2) (also the FFT build's operator) */
+#define SH_K_DENSE 1        /* dense exp(-dtau K/2), N x N */
+#define SH_K_FFT 2          /* FFT build: exp(-dtau K/2) = F^H diag(exph0k) F over the lattice cells
+                               (src/dqmc_config_h0/dqmc_left_forward_prop.f90:95-134,
+                               dqmc_set_h0conf.f90:225-291); one site per cell */
 
 #define SH_U_DELAY 0        /* dqmc_update_delay.f90 (nublock = 1 is the fast update) */
 #define SH_U_SUBMATRIX 1    /* dqmc_update_submatrix.f90 */
@@ -47,7 +50,8 @@ typedef struct sh_model {
     int kmode;              /* SH_K_* */
     int nn_nf, nn_lf;       /* bond families, bonds per family */
     const int32_t *bonds;   /* [nn_nf*nn_lf*2] (A site, B site) per bond, family-major */
-    const double *urt;      /* checkerboard: [nn_nf*nn_lf] 2x2 complex col-major; dense: N*N complex */
+    const double *urt;      /* checkerboard: [nn_nf*nn_lf] 2x2 complex col-major; dense: N*N complex;
+                               FFT: [N] complex exph0k in the order of the wave-vector index */
     const double *urtm1;    /* inverse propagator, same layout */
     int lcomp, nflr;
     const double *bmat;        /* [lcomp] complex  exp(i alpha eta_l) */
@@ -69,6 +73,11 @@ typedef struct sh_model {
      * bonds are the (nn_nf, nn_lf) table above, also for the dense kinetic term */
     const double *vtab;
     const double *vdelta;
+    /* SH_K_FFT: the lattice directions in site order, site s = i0 + n0 (i1 + n1 i2) with
+     * fft_dims = {n0, n1, n2}, n0 n1 n2 = ndim, each 1..16, 18, 20 or 24 (the reference's
+     * DFTI dimensions dimm); the wave-vector index of exph0k is ordered the same way.  Ignored
+     * for the other kinetic forms. */
+    int fft_dims[3];
 } sh_model;
 
 typedef struct sh_ctx sh_ctx;

@@ -24,6 +24,7 @@ class ShModel(ctypes.Structure):
         ("gaml", dp), ("phase_from_conf", ctypes.c_int), ("stag", dp),
         ("update", ctypes.c_int), ("nublock", ctypes.c_int), ("seed", ctypes.c_int32),
         ("kin_weight", ctypes.c_double), ("vtab", dp), ("vdelta", dp),
+        ("fft_dims", ctypes.c_int * 3),
     ]
 
 

@@ -8,6 +8,7 @@
 #include "dyn_kernels.hip"
 #include "eqt_kernels.hip"
 #include "tv_kernels.hip"
+#include "fft_kernels.hip"
 #include "../../include/sunhip.h"
 #include "../../include/sunhip_learn.h"
 
@@ -59,6 +60,7 @@ struct ctx_base {
 
 struct sh_ctx : ctx_base {
     int nch = 0, nmax = 0, ltrot = 0, nwrap = 0, nst = 0, kmode = 0, update = 0;
+    int fft_fcb = 0, fft_ld = 0;   // DFT kinetic form: vectors per workgroup, largest padded length
     std::vector<int> iwrap_nt, wrap_step;
     std::vector<DChain> hc;
     DChain *dc = nullptr;
@@ -185,6 +187,7 @@ static void copy_mat(sh_ctx *ctx, int dsel, int dpt, int ssel, int spt)
 }
 
 static size_t lds_cols(sh_ctx *ctx) { return (size_t)ctx->nmax * CPB * sizeof(cplx); }
+static size_t lds_fft(sh_ctx *ctx) { return (size_t)ctx->fft_fcb * ctx->fft_ld * sizeof(cplx); }
 static size_t lds_rows(sh_ctx *ctx) { return (size_t)cb_ldr(ctx->nmax) * RPB * sizeof(cplx); }
 
 // left product (rows mixed) on matrix sel: program bits as in k_cb_cols
@@ -193,6 +196,12 @@ static void left_pass(sh_ctx *ctx, int sel, int pt, int ksel, int prog, int vsel
     if (ctx->kmode == SH_K_CHECKERBOARD) {
         hipLaunchKernelGGL(k_cb_cols, dim3((ctx->nmax + CPB - 1) / CPB, 1, ctx->nch), dim3(256), lds_cols(ctx),
                            ctx->st, ctx->dc, sel, pt, ksel, prog, vsel, nt0, nt1, step);
+        return;
+    }
+    if (ctx->kmode == SH_K_FFT) {
+        const int f = ctx->fft_fcb;
+        hipLaunchKernelGGL(k_fft_cols, dim3((ctx->nmax + f - 1) / f, 1, ctx->nch), dim3(256), lds_fft(ctx), ctx->st,
+                           ctx->dc, sel, pt, ksel, prog, vsel, nt0, nt1, step, f);
         return;
     }
     // dense kinetic term: matrices K (ksel 0 -> urt, 1 -> urtm1) as GEMMs through W2
@@ -212,6 +221,12 @@ static void right_pass(sh_ctx *ctx, int ksel, int prog, int vsel, int nt, int se
     if (ctx->kmode == SH_K_CHECKERBOARD) {
         hipLaunchKernelGGL(k_cb_rows, dim3((ctx->nmax + RPB - 1) / RPB, 1, ctx->nch), dim3(256), lds_rows(ctx),
                            ctx->st, ctx->dc, sel, 0, ksel, prog, vsel, nt);
+        return;
+    }
+    if (ctx->kmode == SH_K_FFT) {
+        const int f = ctx->fft_fcb;
+        hipLaunchKernelGGL(k_fft_rows, dim3((ctx->nmax + f - 1) / f, 1, ctx->nch), dim3(256), lds_fft(ctx), ctx->st,
+                           ctx->dc, sel, 0, ksel, prog, vsel, nt, f);
         return;
     }
     const int kmat = ksel ? M_KTM1 : M_KT;
@@ -588,6 +603,17 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
             return SH_E_ARG;
         }
         if ((q.kmode == SH_K_CHECKERBOARD || q.vtab) && (q.nn_nf <= 0 || q.nn_lf <= 0)) { delete ctx; return SH_E_ARG; }
+        if (q.kmode != SH_K_CHECKERBOARD && q.kmode != SH_K_DENSE && q.kmode != SH_K_FFT) { delete ctx; return SH_E_ARG; }
+        if (q.kmode == SH_K_FFT) {
+            auto ok = [](int l) { return (l >= 1 && l <= 16) || l == 18 || l == 20 || l == 24; };
+            const int *d = q.fft_dims;
+            if (!ok(d[0]) || !ok(d[1]) || !ok(d[2]) || d[0] * d[1] * d[2] != q.ndim) {
+                int rc = fail(ctx, SH_E_ARG, "FFT kinetic term: fft_dims must multiply to ndim, each 1..16, 18, 20 or 24");
+                delete ctx;
+                return rc;
+            }
+            ctx->fft_ld = std::max(ctx->fft_ld, fft_ld(q.ndim, d[0]));
+        }
         if ((q.vtab != nullptr) != (m[0].vtab != nullptr) || (q.vtab && !q.vdelta)) {
             int rc = fail(ctx, SH_E_ARG, "all chains of a batch must have the bond field or none");
             delete ctx;
@@ -608,6 +634,11 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
             if (q.wrap_step[i] < 1 || q.wrap_step[i] > q.ltrot) { delete ctx; return SH_E_ARG; }
         ctx->nmax = std::max(ctx->nmax, q.ndim);
     }
+    // DFT kinetic form: as many vectors per workgroup (<= 16) as fit in 96 KiB of LDS
+    if (ctx->kmode == SH_K_FFT) {
+        ctx->fft_fcb = 16;
+        while (ctx->fft_fcb > 1 && (size_t)ctx->fft_fcb * ctx->fft_ld * sizeof(cplx) > 96 * 1024) ctx->fft_fcb /= 2;
+    }
     // arguments are validated before the first HIP call
     if (hipSetDevice(device) != hipSuccess) { delete ctx; return SH_E_HIP; }
     ctx->iwrap_nt.assign(m[0].iwrap_nt, m[0].iwrap_nt + m[0].ltrot + 1);
@@ -617,6 +648,7 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
     {
         struct { const void *f; size_t bytes; } big[] = {
             {(const void *)k_cb_rows, lds_rows(ctx)}, {(const void *)k_cb_cols, lds_cols(ctx)},
+            {(const void *)k_fft_rows, lds_fft(ctx)}, {(const void *)k_fft_cols, lds_fft(ctx)},
             {(const void *)k_update_delay_reg<4>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 16 * sizeof(cplx)},
             {(const void *)k_update_delay_reg<8>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 32 * sizeof(cplx)},
             {(const void *)k_update_delay_reg<16>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 64 * sizeof(cplx)},
@@ -681,7 +713,8 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
         c.unif = dalloc<double>(ctx, (size_t)2 * n * q.ltrot);
         c.rng = dalloc<uint32_t>(ctx, 625);
         c.bonds = dupload<int32_t>(ctx, q.bonds, (size_t)2 * q.nn_nf * q.nn_lf);
-        size_t kl = (q.kmode == SH_K_CHECKERBOARD) ? (size_t)4 * q.nn_nf * q.nn_lf : nn;
+        size_t kl = (q.kmode == SH_K_CHECKERBOARD) ? (size_t)4 * q.nn_nf * q.nn_lf : q.kmode == SH_K_FFT ? (size_t)n : nn;
+        if (q.kmode == SH_K_FFT) { c.fd0 = q.fft_dims[0]; c.fd1 = q.fft_dims[1]; c.fd2 = q.fft_dims[2]; }
         c.kt = dupload<cplx>(ctx, (const cplx *)q.urt, kl);
         c.ktm1 = dupload<cplx>(ctx, (const cplx *)q.urtm1, kl);
         c.stag = dupload<double>(ctx, q.stag, n);

@@ -116,6 +116,9 @@ struct DChain {
     int32_t *confv;
     double *unifv, *vacc;
     cplx vtb[8], vdl[4];
+    // the DFT form of the kinetic term: lattice directions in site order (fd0 fastest); kt / ktm1
+    // then hold exph0k / exph0kinv [n]
+    int fd0, fd1, fd2, pad6;
 };
 
 __device__ __forceinline__ cplx cmk(double r, double i) { return make_double2(r, i); }

@@ -9,7 +9,7 @@ module sunhip_bridge
   use, intrinsic :: iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: SH_K_CHECKERBOARD = 0, SH_K_DENSE = 1
+  integer(c_int), parameter :: SH_K_CHECKERBOARD = 0, SH_K_DENSE = 1, SH_K_FFT = 2
   integer(c_int), parameter :: SH_U_DELAY = 0, SH_U_SUBMATRIX = 1
 
   ! mirrors struct sh_model of include/sunhip.h
@@ -27,6 +27,8 @@ module sunhip_bridge
      real(c_double) :: kin_weight
      ! the t-V bond field (c_null_ptr: none)
      type(c_ptr) :: vtab, vdelta
+     ! SH_K_FFT: lattice directions in site order (n0 fastest); urt / urtm1 = exph0k / exph0kinv
+     integer(c_int) :: fft_dims(3)
   end type sh_model
 
   ! mirrors struct shp_model of include/sunhip_proj.h

@@ -14,7 +14,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-KMODES = {"checkerboard": 0, "dense": 1, "fft": 1}
+KMODES = {"checkerboard": 0, "dense": 1, "fft": 2}
+# line lengths of the DFT form of the kinetic term (each lattice direction)
+FFT_LENGTHS = frozenset(list(range(1, 17)) + [18, 20, 24])
 UPDATES = {"fast": 0, "delay": 0, "submatrix": 1}
 
 
@@ -304,7 +306,45 @@ def h0_matrix(m, lat):
     return h
 
 
+def fft_dims(m):
+    """Lattice directions in site order, fastest first: the square's site (x-1) lb + y-1, the
+    cube's (z-1) la lb + (x-1) lb + y-1, the chain's x-1."""
+    if m.lattice == "square":
+        dims = (m.lb, m.la, 1)
+    elif m.lattice == "cubic":
+        dims = (m.lb, m.la, m.lc)
+    elif m.lattice == "chain":
+        dims = (m.la, 1, 1)
+    else:
+        raise ValueError("the FFT kinetic term needs one site per unit cell (square, chain or cubic lattice)")
+    if any(d not in FFT_LENGTHS for d in dims):
+        raise ValueError("FFT kinetic term: every lattice length must be one of %s" % sorted(FFT_LENGTHS))
+    return dims
+
+
+def fft_tables(m, lat):
+    """exph0k / exph0kinv = exp(-+dtau eps_k / 2) with eps_k the diagonal of F H F^H, F the unitary
+    DFT over the cells in site order (src/dqmc_config_h0/dqmc_set_h0conf.f90:225-291: a forward
+    transform of H's rows, a backward transform of its columns, the diagonal).  The hopping must
+    be translation invariant (F H F^H diagonal)."""
+    dims = fft_dims(m)
+    n = lat["nsites"]
+    h = h0_matrix(m, lat)
+    shp = dims[::-1]                      # C order (n2, n1, n0) of s = i0 + n0 (i1 + n1 i2)
+    hk = np.fft.fftn(h.reshape(shp + (n,)), axes=(0, 1, 2), norm="ortho").reshape(n, n)
+    hk = np.fft.ifftn(hk.reshape((n,) + shp), axes=(1, 2, 3), norm="ortho").reshape(n, n)
+    eps = np.diag(hk).copy()
+    off = np.abs(hk - np.diag(eps)).max()
+    if off > 1e-10 * max(1.0, np.abs(h).max()):
+        raise ValueError("FFT kinetic term: the hopping is not diagonal in the Fourier basis (%.3g)" % off)
+    eps = eps.real
+    return np.exp(-0.5 * m.dtau * eps).astype(complex), np.exp(0.5 * m.dtau * eps).astype(complex), dims
+
+
 def kinetic_tables(m, lat):
+    if KMODES[m.kmode] == 2:
+        e, einv, _ = fft_tables(m, lat)
+        return e, einv
     if KMODES[m.kmode] == 0:
         nb = lat["nn_nf"] * lat["nn_lf"]
         urt = np.zeros((nb, 2, 2), complex)
@@ -350,4 +390,5 @@ def make_tables(m):
             "nwrap": nw, "iwrap_nt": iw, "wrap_step": ws, "hs": hs, "urt": urt, "urtm1": urtm1,
             "nublock": nub, "kmode": KMODES[m.kmode], "update": UPDATES[m.update],
             "lwrapT": int(m.rt != 0.0), "lwrapu": int(m.rhub != 0.0),
-            "phase_from_conf": int(m.rhub > 0.0), "v": vconf_tables(m) if m.rv != 0.0 else None}
+            "phase_from_conf": int(m.rhub > 0.0), "v": vconf_tables(m) if m.rv != 0.0 else None,
+            "fft_dims": fft_dims(m) if KMODES[m.kmode] == 2 else (0, 0, 0)}

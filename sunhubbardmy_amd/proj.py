@@ -44,6 +44,8 @@ class ProjModel:
 
 
 def make_proj_tables(m):
+    if KMODES.get(m.kmode) not in (0, 1):
+        raise ValueError("the projective sweep applies the kinetic term as 'checkerboard' or 'dense'")
     lat = lattice_tables(m)
     n = lat["nsites"]
     ltrot = int(round(m.beta / m.dtau))

@@ -74,6 +74,7 @@ class FTWalkers:
         s.stag = keep(lat["stag"], np.float64)
         s.update, s.nublock, s.seed = t["update"], t["nublock"], int(np.int32(m.seed))
         s.kin_weight = lat["kin_weight"]
+        s.fft_dims[:] = [int(d) for d in t["fft_dims"]]
         if t["v"] is not None:
             s.vtab = keep(cz(t["v"]["vt"].reshape(-1)), np.float64)
             s.vdelta = keep(cz(t["v"]["vd"].reshape(-1)), np.float64)

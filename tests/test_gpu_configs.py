@@ -1,0 +1,152 @@
+"""The workload configurations on the GPU, and the FFT kinetic term, against the CPU oracle.
+
+Each configuration is run at its own lattice, field, kinetic form and update algorithm.  Where
+the oracle can follow in seconds the comparison is the parity bar of test_gpu_parity (HS field
+bit-identical after every pass, G within 1e-10 after every stabilization, observables within
+1e-9); at the full imaginary-time extent the sweep is checked by size-independent properties:
+the wrap error of the propagated against the recomputed G, a finite G, an acceptance strictly
+between 0 and 1, and Re(density) = 1/2 per site and flavour at half filling.
+
+C1  square 4x4, U = 4, beta = 4, fast update, checkerboard K (the CPU-runnable case)
+C2  honeycomb L = 12 (N = 288), U = 4, delay update k = 32, dense K, one walker
+C3  SU(4) square lattices L = 8, 12, 16 in one batch, U = 6, general 4-component HS field,
+    submatrix update, FFT K (32 + 24 + 16 chains at beta = 8)
+FFT the DFT kinetic term on non-square, chain and cubic lattices
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from sunhubbardmy_amd import FTWalkers, Model, chain_seed
+
+from test_gpu_parity import relmax, run_parity
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+C1 = dict(lattice="square", la=4, lb=4, beta=4.0, dtau=0.1, rt=1.0, mu=0.0, rhub=4.0, nflr=2, exact_hs=True,
+          kmode="checkerboard", update="fast", nwrap=10)
+C2 = dict(lattice="honeycomb", la=12, lb=12, beta=10.0, dtau=0.1, rt=1.0, mu=0.0, rhub=4.0, nflr=2, exact_hs=True,
+          kmode="dense", update="delay", nwrap=10, nublock=32)
+C3 = dict(lattice="square", beta=8.0, dtau=0.1, rt=1.0, mu=0.0, rhub=6.0, nflr=4, exact_hs=False, kmode="fft",
+          update="submatrix", nwrap=10, nublock=32)
+
+FFT_CASES = {
+    "square4x6_delay": dict(lattice="square", la=4, lb=6, beta=2.0, rhub=4.0, kmode="fft", update="delay", nublock=4,
+                            nwrap=10),
+    "square8_su4_general_submatrix": dict(lattice="square", la=8, lb=8, beta=1.0, rhub=6.0, nflr=4, exact_hs=False,
+                                          kmode="fft", update="submatrix", nublock=16, nwrap=5),
+    "chain12_fast_mu": dict(lattice="chain", la=12, lb=1, beta=2.0, rhub=4.0, mu=0.3, kmode="fft", update="fast",
+                            nwrap=10),
+    "cubic4x6x4_delay": dict(lattice="cubic", la=4, lb=6, lc=4, beta=0.8, rhub=4.0, kmode="fft", update="delay",
+                             nublock=8, nwrap=4),
+    "square12_delay_nwrap7": dict(lattice="square", la=12, lb=12, beta=1.4, rhub=4.0, kmode="fft", update="delay",
+                                  nublock=16, nwrap=7),
+}
+
+
+@pytest.mark.parametrize("name", list(FFT_CASES))
+def test_fft_kinetic_parity(name):
+    """The FFT kinetic kernels against the oracle's dense exp(-dtau K/2) (the same operator)."""
+    run_parity(dict(FFT_CASES[name]))
+
+
+def test_c1_full_config_parity():
+    """C1 at its stated beta = 4 (Ltrot = 40), two sweeps against the oracle."""
+    run_parity(dict(C1), nsweeps=2)
+
+
+def test_c2_parity_short_beta():
+    """C2's lattice, field, kinetic form and delay depth (N = 288 > 256: the delay update with the
+    delayed vectors in memory; QR and LU beyond the blocked size) at beta = 1."""
+    kw = dict(C2)
+    kw["beta"] = 1.0
+    run_parity(kw, nsweeps=1)
+
+
+def _properties(w, c, orbitals=1):
+    g = w.G(c)
+    assert np.all(np.isfinite(g))
+    d = w.diag(c)
+    assert d["max_wrap_error"] < 1e-8, d
+    assert 0 < d["accepted"] < d["tried"], d
+    ob = w.obs(c)
+    assert ob["nobs"] > 0
+    # particle-hole symmetry at half filling: Re(density) = 1/2 per site and flavour in every
+    # configuration (density is normalised per unit cell)
+    assert abs(ob["density"].real / ob["nobs"] - 0.5 * orbitals) < 1e-8, ob
+    return d
+
+
+def test_c2_full_beta_properties():
+    """C2 at beta = 10 (Ltrot = 100, nst = 10): one sweep of the single walker."""
+    w = FTWalkers([Model(seed=chain_seed(0), **C2)])
+    assert w.tables[0]["ndim"] == 288 and w.tables[0]["ltrot"] == 100
+    w.init_conf()
+    w.sweep_start_0b()
+    w.sweep(True, True)
+    d = _properties(w, 0, orbitals=2)
+    say("C2 beta=10:", d)
+    w.close()
+
+
+def test_c3_uneven_batch_parity_short_beta():
+    """One chain of each C3 lattice (L = 8, 12, 16) in one batch, every chain against its own
+    oracle run (HS field bit-identical, G 1e-10, observables 1e-9) at beta = 0.5."""
+    kw = dict(C3)
+    kw["beta"] = 0.5
+    ls = (8, 12, 16)
+    models = [Model(seed=chain_seed(c), la=l, lb=l, **kw) for c, l in enumerate(ls)]
+    say("C3 batch", [m.la for m in models])
+    w = FTWalkers(models)
+    ors = [oracle.OracleChain(seed=chain_seed(c), la=l, lb=l, **kw) for c, l in enumerate(ls)]
+    w.init_conf()
+    for o in ors:
+        o.init_conf()
+    w.sweep_start_0b()
+    for o in ors:
+        o.sweep_start()
+    for c, o in enumerate(ors):
+        assert relmax(w.G(c), o.G) < 1e-10, c
+    for p in ("b0", "0b"):
+        getattr(w, "sweep_" + p)(True, True)
+        for o in ors:
+            getattr(o, "sweep_" + p)(True, True)
+        say("  pass", p)
+        for c, o in enumerate(ors):
+            assert np.array_equal(w.get_conf(c), o.conf), (c, p)
+            assert relmax(w.G(c), o.G) < 1e-10, (c, p)
+    for c, o in enumerate(ors):
+        dw, do = w.diag(c), o.diag()
+        assert dw["accepted"] == do["accepted"] and dw["tried"] == do["tried"]
+        assert abs(dw["phase"] - do["phase"]) < 1e-10
+        ow, oo = w.obs(c), o.obs()
+        for k in ("sign", "density", "ekin", "dblocc", "s_stag"):
+            assert abs(ow[k] - oo[k]) <= 1e-9 * max(1.0, abs(oo[k])), (c, k)
+    w.close()
+
+
+def test_c3_full_batch_properties():
+    """The C3 batch as stated: 32 chains at L = 8, 24 at L = 12, 16 at L = 16, each with its own
+    seed, one launch per kernel, beta = 8; one sweep."""
+    models = []
+    for l, cnt in ((8, 32), (12, 24), (16, 16)):
+        for _ in range(cnt):
+            models.append(Model(seed=chain_seed(len(models)), la=l, lb=l, **C3))
+    w = FTWalkers(models)
+    w.init_conf()
+    w.sweep_start_0b()
+    w.sweep(True, True)
+    accs = []
+    for c in range(len(models)):
+        d = _properties(w, c)
+        accs.append(d["accepted"] / d["tried"])
+    say("C3 acceptance per lattice:", [float(np.mean(accs[a:b])) for a, b in ((0, 32), (32, 56), (56, 72))])
+    w.close()

@@ -124,10 +124,10 @@ def test_proj_batch_matches_single_chains():
 
 
 def test_proj_full_size_properties():
-    """The L = 18 honeycomb workload (N = 648, ne = 324, nustock 64) over a short projection:
+    """The L = 18 honeycomb workload (N = 648, ne = 324, nustock 64) at its projection length theta = 20:
     UL UR stays invertible, G is a projector of rank N - ne (G^2 = G, tr G = N - ne), the wrap
     error stays small and the acceptance is a proper fraction."""
-    m = ProjModel(lattice="honeycomb", la=18, lb=18, beta=1.0, rv=1.2, rndness=1e-4, kmode="dense", nwrap=10,
+    m = ProjModel(lattice="honeycomb", la=18, lb=18, beta=20.0, rv=1.2, rndness=1e-4, kmode="dense", nwrap=10,
                   nustock=0, seed=chain_seed(0))
     w = ProjWalkers(m)
     assert (w.ndim, w.ne) == (648, 324)

@@ -216,3 +216,60 @@ def test_proj_create_rejects_bad_arguments():
     assert _lib.lib().shp_destroy(None) == 0
     assert _lib.lib().shp_sweep_b0(None, 1, 1) == -1
     assert _lib.lib().shp_last_error(None) == b"null context"
+
+
+# ---- the DFT form of the kinetic term ---------------------------------------------------------
+@pytest.mark.parametrize("kw", [dict(lattice="square", la=4, lb=6), dict(lattice="square", la=12, lb=12, mu=0.3),
+                                dict(lattice="chain", la=10, lb=1), dict(lattice="cubic", la=4, lb=6, lc=4)])
+def test_fft_tables_reproduce_the_dense_exponential(kw):
+    """F^H diag(exph0k) F, applied as the kernels do (unnormalised transforms along each lattice
+    direction of the site order, the diagonal divided by N), equals expm(-dtau H / 2) built from
+    the bond list; the same for the inverse.  A transposed direction order or a sign error in the
+    exponent fails on the non-square shapes."""
+    import scipy.linalg
+    from sunhubbardmy_amd.model import h0_matrix
+    m = Model(kmode="fft", dtau=0.1, **kw)
+    t = make_tables(m)
+    n = t["ndim"]
+    dims = t["fft_dims"]
+    assert int(np.prod(dims)) == n
+    H = h0_matrix(m, t["lat"])
+    shp = tuple(dims[::-1])
+
+    def apply(e, X, side):
+        # left: columns transformed forward, scaled, transformed back; right: rows conjugate first
+        Y = X if side == "left" else X.T
+        A = Y.reshape(shp + (n,))
+        fwd, bwd = (np.fft.fftn, np.fft.ifftn) if side == "left" else (np.fft.ifftn, np.fft.fftn)
+        A = fwd(A, axes=(0, 1, 2), norm="ortho").reshape(n, n) * e[:, None]
+        A = bwd(A.reshape(shp + (n,)), axes=(0, 1, 2), norm="ortho").reshape(n, n)
+        return A if side == "left" else A.T
+    eye = np.eye(n, dtype=complex)
+    for e, s in ((t["urt"], -0.05), (t["urtm1"], 0.05)):
+        ref = scipy.linalg.expm(s * H)
+        assert np.abs(apply(e, eye, "left") - ref).max() < 1e-13
+        assert np.abs(apply(e, eye, "right") - ref).max() < 1e-13
+    assert np.abs(t["urt"] * t["urtm1"] - 1.0).max() < 1e-14
+
+
+def test_fft_tables_refuse_lattices_they_cannot_diagonalise():
+    with pytest.raises(ValueError):
+        make_tables(Model(lattice="honeycomb", la=3, lb=3, kmode="fft"))
+    with pytest.raises(ValueError):
+        make_tables(Model(lattice="square", la=17, lb=4, kmode="fft"))
+    from sunhubbardmy_amd.proj import ProjModel, make_proj_tables
+    with pytest.raises(ValueError):
+        make_proj_tables(ProjModel(kmode="fft"))
+
+
+def test_create_rejects_bad_fft_dimensions():
+    t = make_tables(Model(lattice="square", la=4, lb=6, beta=1.0, kmode="fft"))
+    s, k = _model_struct(t)
+    assert list(s.fft_dims) == [6, 4, 1]
+    s.fft_dims[:] = [6, 5, 1]                  # product != ndim
+    assert _create([s])[0] == -1
+    s.fft_dims[:] = [17, 1, 1]                 # a length the transform does not support
+    assert _create([s])[0] == -1
+    s2, k2 = _model_struct(t)
+    s2.kmode = 3                               # unknown kinetic form
+    assert _create([s2])[0] == -1
