@@ -71,6 +71,14 @@ int shl_train(shl_net *net, int epochs, int batch, double lr, double *loss);
 int shl_loss_grad(shl_net *net, const double *x, int n, double *loss, float *grad);
 /* E_eff / Ns = ymu + ysd g(x) for host features f[n][4] */
 int shl_predict(shl_net *net, const double *f, int n, double *e);
+/* shl_loss_grad without any allocation, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight training or serving loops).  It has no counterpart in the
+ * reference.  x[n][5] (double), loss[1] (double) and grad[SHL_NWEIGHTS] (float) are DEVICE
+ * pointers owned by the caller; the results are those of shl_loss_grad on the same samples.
+ * The partial sums live in a fixed 4.4 MB device array that the library holds per GPU until
+ * the process ends (set up by shl_create); n <= 16777216 (SH_E_ARG beyond).  Calls that overlap
+ * on one GPU share that array and race.  Returns after the results are written. */
+int shl_loss_grad_noalloc(shl_net *net, const double *x, int n, double *loss, float *grad);
 
 /* Neighbour lists of the lattice sites, int32 [sum over chains of ndim][z], 0-based, symmetric
  * (j in list(i) as often as i in list(j)); z the same for every chain.  Switches block

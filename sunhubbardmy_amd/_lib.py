@@ -53,7 +53,8 @@ EXPORTS_PROJ = [
 EXPORTS_LEARN = [
     "shl_create", "shl_destroy", "shl_last_error", "shl_get_weights", "shl_set_weights", "shl_set_norm",
     "shl_get_norm", "shl_add_samples", "shl_push_samples", "shl_num_samples", "shl_get_samples",
-    "shl_clear_samples", "shl_train", "shl_loss_grad", "shl_predict", "sh_set_neighbours", "sh_features",
+    "shl_clear_samples", "shl_train", "shl_loss_grad", "shl_predict", "shl_loss_grad_noalloc", "sh_set_neighbours",
+    "sh_features",
     "sh_set_proposal", "sh_get_proposal_stats", "sh_reset_proposal_stats",
 ]
 
@@ -153,6 +154,8 @@ def lib():
     L.shl_train.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
     L.shl_loss_grad.argtypes = [P, dp, ctypes.c_int, dp, fp]
     L.shl_predict.argtypes = [P, dp, ctypes.c_int, dp]
+    # device pointers (x, loss, grad) as integers
+    L.shl_loss_grad_noalloc.argtypes = [P, P, ctypes.c_int, P, P]
     L.sh_set_neighbours.argtypes = [P, ctypes.c_int, ip]
     L.sh_features.argtypes = [P, dp]
     L.sh_set_proposal.argtypes = [P, P, ctypes.c_int, ctypes.c_int]

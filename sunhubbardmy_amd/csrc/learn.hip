@@ -2,6 +2,7 @@
 // Included at the end of sunhip.hip; the kernels are in learn_kernels.hip.
 
 struct shl_net : ctx_base {
+    void *na_scratch = nullptr;   // this device's no-allocation scratch (shl_loss_grad_noalloc)
     int cap = 0, count = 0;
     float *w = nullptr, *am = nullptr, *av = nullptr, *gpart = nullptr, *gout = nullptr;
     double *samp = nullptr, *lpart = nullptr, *dloss = nullptr, *pred = nullptr;
@@ -12,6 +13,36 @@ struct shl_net : ctx_base {
     long adam_t = 0;
     uint64_t rng = 0;
 };
+
+// ---- scratch of the no-allocation loss/gradient entry point ---------------------------------
+// The partial sums of shl_loss_grad_noalloc: SHL_TRAIN_WG gradient vectors (float) and their
+// loss partials (double), 4.4 MB in one file-scope device array that every GPU the library
+// runs on holds until the process ends.  Host code never passes the array's name to a kernel:
+// its device address is taken once per device by hipGetSymbolAddress (in shl_create, with the
+// device current) and kept in noalloc_ptr[device].  Calls that overlap on one GPU share the
+// array and race.
+#define SHL_NA_GFLOATS ((size_t)SHL_TRAIN_WG * SHL_NWT)
+#define SHL_NA_DOUBLES ((SHL_NA_GFLOATS + 1) / 2 + SHL_TRAIN_WG)
+#define SHL_NA_MAX_SAMPLES (1 << 24)
+static __device__ __align__(256) double shl_noalloc_scratch[SHL_NA_DOUBLES];
+static void *noalloc_ptr[64];
+static std::mutex noalloc_mu;
+
+static int noalloc_setup(int device, void **out)
+{
+    if (device < 0 || device >= 64) return SH_E_ARG;
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (!noalloc_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(shl_noalloc_scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return SH_E_HIP;
+        }
+        noalloc_ptr[device] = p;
+    }
+    *out = noalloc_ptr[device];
+    return 0;
+}
 
 static size_t lds_block_move(int nmax, int z)
 {
@@ -208,6 +239,12 @@ int shl_create(shl_net **out, int capacity, int32_t seed, int device)
         shl_destroy(ctx);
         return SH_E_HIP;
     }
+    // the device address of this GPU's no-allocation scratch, with the device current
+    if (noalloc_setup(device, &ctx->na_scratch) != 0) {
+        int rc = fail(ctx, SH_E_HIP, "hipGetSymbolAddress of the no-allocation scratch failed");
+        shl_destroy(ctx);
+        return rc;
+    }
     ctx->w = dalloc<float>(ctx, SHL_NWT);
     ctx->am = dalloc<float>(ctx, SHL_NWT);
     ctx->av = dalloc<float>(ctx, SHL_NWT);
@@ -350,7 +387,7 @@ static int mlp_step(shl_net *ctx, const int32_t *idx, int count, float lr, float
         c2 = (float)(1.0 / (1.0 - pow(0.999, (double)ctx->adam_t)));
     }
     hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((SHL_NWT + 255) / 256), dim3(256), 0, ctx->st, ctx->gpart, nwg,
-                       ctx->lpart, ctx->w, ctx->am, ctx->av, lr, c1, c2, gout, ctx->dloss);
+                       ctx->lpart, ctx->w, ctx->am, ctx->av, lr, c1, c2, gout, ctx->dloss, 1.0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(ctx, SH_E_HIP, std::string("training launch: ") + hipGetErrorString(e));
     return 0;
@@ -429,6 +466,29 @@ int shl_loss_grad(shl_net *ctx, const double *x, int n, double *loss, float *gra
     hipFree(d);
     ctx->allocs.pop_back();
     return rc;
+}
+
+int shl_loss_grad_noalloc(shl_net *ctx, const double *x, int n, double *loss, float *grad)
+{
+    if (!ctx || !x || !loss || !grad || n < 1) return SH_E_ARG;
+    if (n > SHL_NA_MAX_SAMPLES)
+        return fail(ctx, SH_E_ARG, "shl_loss_grad_noalloc: at most 16777216 samples per call");
+    if (!ctx->norm_set) return fail(ctx, SH_E_STATE, "normalisation not set");
+    if (!ctx->na_scratch) return fail(ctx, SH_E_STATE, "no-allocation scratch not set up");
+    // the partitions and kernels of shl_loss_grad, with the partial sums in this device's
+    // scratch: gradient partials first, the loss partials after them (8-byte aligned)
+    float *gpart = (float *)ctx->na_scratch;
+    double *lpart = (double *)ctx->na_scratch + (SHL_NA_GFLOATS + 1) / 2;
+    const int nwg = std::min(SHL_TRAIN_WG, std::max(1, (n + 3) / 4));
+    const int per = (n + nwg - 1) / nwg;
+    hipLaunchKernelGGL(k_mlp_batch, dim3(nwg), dim3(256), lds_mlp_batch(), ctx->st, ctx->w, x, (const int32_t *)nullptr,
+                       n, per, normf_of(ctx->norm), 1.0f / (float)n, 0, gpart, lpart, (double *)nullptr);
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((SHL_NWT + 255) / 256), dim3(256), 0, ctx->st, gpart, nwg, lpart,
+                       ctx->w, ctx->am, ctx->av, 0.0f, 1.0f, 1.0f, grad, loss, (double)n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
+    if (e != hipSuccess) return fail(ctx, SH_E_HIP, std::string("shl_loss_grad_noalloc: ") + hipGetErrorString(e));
+    return 0;
 }
 
 int shl_predict(shl_net *ctx, const double *f, int n, double *e)

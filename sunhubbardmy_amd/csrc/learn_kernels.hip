@@ -594,16 +594,16 @@ __global__ __launch_bounds__(256) void k_mlp_batch(const float *wt, const double
 }
 
 // sum of the workgroup gradients in a fixed order, then one Adam step (lr > 0) or the plain
-// gradient into gout; lout = sum of the squared errors
+// gradient into gout; lout = (sum of the squared errors) / ldiv
 __global__ __launch_bounds__(256) void k_mlp_reduce_adam(const float *gpart, int nwg, const double *lpart, float *wt,
                                                          float *am, float *av, float lr, float c1, float c2,
-                                                         float *gout, double *lout)
+                                                         float *gout, double *lout, double ldiv)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         double s = 0.0;
         for (int b = 0; b < nwg; b++) s += lpart[b];
-        *lout = s;
+        *lout = s / ldiv;
     }
     if (i >= SHL_NWT) return;
     float g = 0.f;
