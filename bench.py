@@ -47,8 +47,35 @@ HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
 
 
 def model_kw(args):
+    # nublock 0: start from the reference's heuristic depth, then the autotuner picks it
     return dict(lattice="square", la=16, lb=16, beta=8.0, dtau=0.1, rt=1.0, mu=0.0, rhub=6.0, nu=0.0, nflr=2,
                 exact_hs=True, kmode="checkerboard", update="delay", nwrap=10, nublock=args.nublock)
+
+
+def graph_demo(sweeps=40):
+    """Host launches against captured passes on the CPU-runnable configuration (one chain of the
+    4x4 square lattice, U = 4, beta = 4, fast update, checkerboard K), where the sweep is
+    launch-bound: chain sweeps/s with every kernel launched from the host, then with every pass
+    replayed from its hipGraph (same results, bit for bit, tests/test_gpu_graphs_tune.py)."""
+    from sunhubbardmy_amd import FTWalkers, Model, chain_seed
+    kw = dict(lattice="square", la=4, lb=4, beta=4.0, dtau=0.1, rhub=4.0, kmode="checkerboard", update="fast",
+              nwrap=10)
+    out = {"workload": "SU(2) square 4x4, U=4, beta=4 (Ltrot=40), fast update, checkerboard K, 1 chain"}
+    for mode in ("direct", "graphs"):
+        w = FTWalkers([Model(seed=chain_seed(0), **kw)])
+        w.set_graphs(mode == "graphs")
+        w.init_conf()
+        w.sweep_start_0b()
+        w.sweep(True, True)
+        w.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(sweeps):
+            w.sweep(True, True)
+        w.synchronize()
+        out[mode + "_sweeps_per_s"] = sweeps / (time.perf_counter() - t0)
+        w.close()
+    out["speedup"] = out["graphs_sweeps_per_s"] / out["direct_sweeps_per_s"]
+    return out
 
 
 def algorithmic_work(group, n, nchains, ltrot, nwrap, acc, k):
@@ -180,6 +207,72 @@ def bench_pqmc(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_c3(args):
+    """The batched submatrix configuration: SU(4) Hubbard, general 4-component HS field, U = 6,
+    half filling, beta = 8, dtau = 0.1 (Ltrot = 80), nwrap = 10, FFT kinetic term, submatrix
+    update; 32 chains at L = 8 (N = 64), 24 at L = 12 (N = 144) and 16 at L = 16 (N = 256) in
+    one batch, one launch per kernel for all of them; the depth and the DFT tiles picked by the
+    autotuner.  A step is one sweep of every chain; value = chain sweeps/s of the batch."""
+    from sunhubbardmy_amd import FTWalkers, Model, chain_seed
+    steps = args.steps if args.steps != 60 else 5
+    warmup = args.warmup if args.warmup != 60 else 2
+    kw = dict(lattice="square", beta=8.0, dtau=0.1, rt=1.0, mu=0.0, rhub=6.0, nflr=4, exact_hs=False, kmode="fft",
+              update="submatrix", nwrap=10, nublock=args.nublock if args.nublock > 0 else 0)
+    models, sizes = [], []
+    for l, cnt in ((8, 32), (12, 24), (16, 16)):
+        for _ in range(cnt):
+            models.append(Model(seed=chain_seed(len(models)), la=l, lb=l, **kw))
+            sizes.append(l * l)
+    w = FTWalkers(models)
+    w.init_conf()
+    w.sweep_start_0b()
+    tuned = w.autotune() if args.nublock == 0 else None
+    for s in range(warmup):
+        w.sweep(True, False)
+    w.reset_diag()
+    w.synchronize()
+    w.timers(enable=args.timers)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        w.sweep(True, True)
+        print("# c3 step %d" % s, file=sys.stderr, flush=True)
+    w.synchronize()
+    dt = time.perf_counter() - t0
+    tm = w.timers(enable=False)
+    acc = [w.diag(c)["accepted"] / max(w.diag(c)["tried"], 1) for c in range(len(models))]
+    line = {"metric": "chain sweeps/s, SU(4) batched submatrix (C3)", "value": len(models) * steps / dt,
+            "unit": "sweeps/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": 1000.0 * dt / steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "SU(4) Hubbard, general 4-component HS, U=6, beta=8, dtau=0.1, FFT K, submatrix; "
+                                   "32 x L=8 + 24 x L=12 + 16 x L=16 square chains in one batch",
+                       "chains": len(models), "autotuned": tuned,
+                       "nublock": w.tables[0]["nublock"]},
+            "acceptance_by_lattice": {str(l): float(np.mean([a for a, n in zip(acc, sizes) if n == l * l]))
+                                      for l in (8, 12, 16)}}
+    if args.timers:
+        best = max(tm, key=lambda g: tm[g][0])
+        ms, cnt = tm[best]
+        a = float(np.mean(acc))
+        k = w.tables[0]["nublock"]
+        if best == "update":
+            amount = sum(8.0 * a * n * n * (n + k) for n in sizes)
+            kind = "flops"
+        else:
+            amount = sum(2 * 16.0 * n * n for n in sizes)
+            kind = "bytes"
+        avg_s = ms / 1000.0 / max(cnt, 1)
+        if kind == "flops":
+            line["roofline"] = {"bound": "mfma", "achieved": amount / avg_s / 1e12, "peak": FP64_MFMA_PEAK / 1e12,
+                                "unit": "TFLOP/s", "frac": amount / avg_s / FP64_MFMA_PEAK, "traffic": None}
+        else:
+            line["roofline"] = {"bound": "hbm", "achieved": amount / avg_s / 1e9, "peak": HBM_PEAK / 1e9,
+                                "unit": "GB/s", "frac": amount / avg_s / HBM_PEAK, "traffic": None}
+        line["roofline"].update({"kernel_group": best, "avg_launch_ms": ms / max(cnt, 1)})
+        line["kernel_ms"] = {g: round(v[0], 3) for g, v in tm.items()}
+        line["kernel_launches"] = {g: v[1] for g, v in tm.items()}
+    print(json.dumps(line), flush=True)
+
+
 def timed_phase(w, steps, tau_sweeps, timers, label):
     """steps sweeps with measurement, bracketed by device synchronisation, then untimed sweeps up
     to tau_sweeps in all; per-sweep bins of the whole series give tau_int."""
@@ -241,7 +334,8 @@ def main():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--chains", type=int, default=256)
-    ap.add_argument("--nublock", type=int, default=32)
+    ap.add_argument("--nublock", type=int, default=0,
+                    help="delay depth; 0: the autotuner picks it on the warmed-up chains")
     ap.add_argument("--nprop", type=int, default=16, help="inner Metropolis attempts per block proposal (<= 64)")
     ap.add_argument("--tau-sweeps", type=int, default=100,
                     help="length of each phase's per-sweep series for tau_int (the first --steps are timed)")
@@ -253,11 +347,14 @@ def main():
     ap.add_argument("--series-out", default=None,
                     help="write the per-sweep observable series of the timed phases to this .npz file")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph-demo", action="store_true",
+                    help="skip the direct-launch vs captured-pass comparison on the 4x4 configuration")
     ap.add_argument("--no-timers", dest="timers", action="store_false",
                     help="skip the per-kernel hipEvent timing of the timed steps")
-    ap.add_argument("--workload", choices=("ft", "pqmc"), default="ft",
+    ap.add_argument("--workload", choices=("ft", "pqmc", "c3"), default="ft",
                     help="ft: the headline finite-temperature workload; pqmc: the projective t-V workload "
-                         "(honeycomb L=18, N=648, 324 particles, V=1.2, theta=20, submatrix-LR)")
+                         "(honeycomb L=18, N=648, 324 particles, V=1.2, theta=20, submatrix-LR); c3: the SU(4) "
+                         "batched submatrix workload (32 x L=8 + 24 x L=12 + 16 x L=16, FFT K, beta=8)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 512 MiB device-to-device copies (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
@@ -270,6 +367,8 @@ def main():
         sys.exit(2)
     if args.workload == "pqmc":
         return bench_pqmc(args)
+    if args.workload == "c3":
+        return bench_c3(args)
 
     from sunhubbardmy_amd import FTWalkers, Model, chain_seed
     from sunhubbardmy_amd.learned import EffectiveAction
@@ -300,6 +399,10 @@ def main():
             net.add_samples(w)
         if s % 10 == 0:
             print("# warmup %d" % s, file=sys.stderr, flush=True)
+    tuned = None
+    if args.nublock == 0:
+        tuned = w.autotune()
+        print("# autotuned: %s" % tuned, file=sys.stderr, flush=True)
     train = None
     if learn:
         t0 = time.perf_counter()
@@ -343,6 +446,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": 1000.0 * head["dt"] / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": wl, "chains": args.chains, "nublock": t["nublock"],
+                   "nublock_from": "autotuner (sh_autotune)" if tuned is not None else "--nublock",
                    "proposal": "learned (block moves of %d attempts after each slice's local update)" % args.nprop
                    if learned is not None else "plain local delay update"},
     }
@@ -384,6 +488,8 @@ def main():
     if cpu is not None:
         cpu["value"] = cpu["sweeps_per_s"] / (2.0 * (plain or head)["tau"])
         line["cpu_baseline"] = cpu
+    if not args.no_graph_demo:
+        line["graph_capture"] = graph_demo()
     print(json.dumps(line), flush=True)
 
 

@@ -163,6 +163,7 @@ int sh_get_g(sh_ctx *ctx, int chain, double *g);
 /* diag[10]: max_wrap_error, max_phasediff, phase(re,im), logweightf(re,im),
  * accepted, tried (main_obs(3)), number of wrap checks, sum log10 wrap error */
 int sh_get_diag(sh_ctx *ctx, int chain, double *diag);
+/* clears the wrap-error and acceptance accumulators and the non-finite flag */
 int sh_reset_diag(sh_ctx *ctx);
 /* obs[11]: nobs, then complex bin sums of sign, density/lq (energy_bin(1:2)), zkint
  * (energy_bin(3)), zint_u (energy_bin(4)), S_stag (zspsm Fourier component at the
@@ -184,6 +185,23 @@ int sh_get_trace(sh_ctx *ctx, int chain, double *buf, int *count);
  * same groups.  enable = 0 stops recording (events cost a little launch overhead). */
 int sh_timers(sh_ctx *ctx, int enable, double *t);
 int sh_synchronize(sh_ctx *ctx);
+
+/* Graph-captured passes: with enable = 1 (and the timers off) each kind of sh_sweep_b0 /
+ * sh_sweep_0b call (pass, lupdate, lmeasure) is captured once into a hipGraph and replayed, one
+ * host launch per pass instead of one per kernel; results are those of the direct launches.
+ * Setters that change what a pass launches retire the captured graphs.  The reference has no
+ * counterpart (its passes are host loops, src/dqmc_ft_core/dqmc_ft_sweep_b0.f90:36-207). */
+int sh_set_graphs(sh_ctx *ctx, int enable);
+
+/* Autotuner of the delay depth / submatrix block and of the DFT kernels' vectors per workgroup,
+ * in place of the heuristic tables of src/model_para.f90:382-415: after sh_sweep_start_0b, the
+ * update of one slice is timed on the chains' own G at every candidate depth (delay 4, 8, 16, 32,
+ * and 64 where the delayed vectors do not fit in registers; submatrix 8 .. 64) and the fastest is
+ * kept for every chain; with the FFT kinetic form the DFT kernels are timed at 2 .. 16 vectors
+ * per workgroup.  The chains' state is left bit for bit as it was (the next updating pass draws
+ * its own uniforms).  tuned[0] = chosen nublock, tuned[1] = vectors per workgroup (0: not FFT).
+ * The delay depth changes rounding only, not the algorithm (dqmc_update_delay.f90:39-128). */
+int sh_autotune(sh_ctx *ctx, int *tuned);
 
 #ifdef __cplusplus
 }

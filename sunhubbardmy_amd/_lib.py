@@ -32,7 +32,7 @@ EXPORTS = [
     "sh_create", "sh_destroy", "sh_last_error", "sh_init_conf", "sh_set_conf", "sh_get_conf",
     "sh_sweep_start_0b", "sh_sweep_b0", "sh_sweep_0b", "sh_push_stage", "sh_pop_stage",
     "sh_get_g", "sh_get_diag", "sh_reset_diag", "sh_get_obs", "sh_get_obs_all", "sh_reset_obs", "sh_set_trace",
-    "sh_get_trace", "sh_timers", "sh_synchronize", "sh_get_rng_state", "sh_set_rng_state", "sh_set_dyn", "sh_set_eqt", "sh_get_eqt", "sh_set_confv", "sh_get_confv", "sh_get_vacc",
+    "sh_get_trace", "sh_timers", "sh_synchronize", "sh_set_graphs", "sh_autotune", "sh_get_rng_state", "sh_set_rng_state", "sh_set_dyn", "sh_set_eqt", "sh_get_eqt", "sh_set_confv", "sh_get_confv", "sh_get_vacc",
     "sh_get_dyn_obs", "sh_reset_dyn_obs", "sh_get_gtau",
 ]
 
@@ -89,6 +89,8 @@ def lib():
     L.sh_get_trace.argtypes = [P, ctypes.c_int, dp, ip]
     L.sh_timers.argtypes = [P, ctypes.c_int, dp]
     L.sh_synchronize.argtypes = [P]
+    L.sh_set_graphs.argtypes = [P, ctypes.c_int]
+    L.sh_autotune.argtypes = [P, ip]
     u32p = ctypes.POINTER(ctypes.c_uint32)
     L.sh_get_rng_state.argtypes = [P, ctypes.c_int, u32p]
     L.sh_set_rng_state.argtypes = [P, ctypes.c_int, u32p]

@@ -163,27 +163,34 @@ def write_bin(path, lattice, l1, l2, nobs, obs, eqt, rv=0.0, l3=1):
             _record(f, _reals(np.array([v]) / rnorm))
 
 
-def write_dyn_bin(path, lattice, l1, l2, nsweep, znn):
-    """Append one bin of dyn_output (src/dqmc_measure/dyn_output.f90: square and honeycomb):
-    znn = per chain the library's znn_tau bins [ntdm+1][no_j][no_i][lq], summed over nsweep
-    dynamic passes; normalised by 1 / (nsweep * chains) and written as nn_rtau.bin (per imj, per
-    t: 8e16.8 of gr(imj, 1:nsub, 1:nsub, t)) and nn_ktau.bin (per k: its 2f16.8 coordinates, then
-    per t the 8e16.8 Fourier components)."""
-    if lattice not in ("square", "honeycomb"):
-        raise ValueError("the dyn_output layout covers the square and honeycomb lattices")
+def write_dyn_bin(path, lattice, l1, l2, nsweep, znn, l3=1):
+    """Append one bin of dyn_output: znn = per chain the library's znn_tau bins
+    [ntdm+1][no_j][no_i][lq], summed over nsweep dynamic passes, normalised by
+    1 / (nsweep * chains) and written as nn_rtau.bin / nn_ktau.bin in the layout of the lattice's
+    dyn_output:
+      square, honeycomb (src/dqmc_measure/dyn_output.f90) and chain (dyn_output1d.f90): per imj,
+        per t = 0 .. ntdm the 8e16.8 record of gr(imj, 1:nsub, 1:nsub, t); per k its coordinates
+        (2f16.8 in 2D; the one coordinate k/l1 on the chain), then per t the 8e16.8 components;
+      cube (dyn_output3d.f90): per imj, per t = 1 .. ntdm (the reference skips t = 0) the 2e16.8
+        record of gr(imj, t); per k its 3f16.8 coordinates, then per t the 2e16.8 components.
+    (dyn_output3d also writes spsm_tau and gf_tau files; those bins are not measured here.)"""
+    if lattice not in ("square", "honeycomb", "chain", "cubic"):
+        raise ValueError("unknown lattice %r" % lattice)
     nsub = 2 if lattice == "honeycomb" else 1
+    dims = {"chain": (l1,), "cubic": (l1, l2, l3)}.get(lattice, (l1, l2))
     z = np.sum(np.asarray(znn), axis=0) / float(nsweep * len(znn))     # [t][no_j][no_i][imj]
     gr = z.transpose(3, 2, 1, 0)[:, :nsub, :nsub, :]                   # [imj][no_i][no_j][t]
     lq, nt = gr.shape[0], gr.shape[3]
-    ph, ks = _phase_table((l1, l2))
+    ph, ks = _phase_table(dims)
+    gk = np.tensordot(1.0 / ph, gr, axes=(1, 0)) / lq
     os.makedirs(path, exist_ok=True)
+    t0 = 1 if lattice == "cubic" else 0
     with open(os.path.join(path, "nn_rtau.bin"), "a") as f:
         for imj in range(lq):
-            for t in range(nt):
+            for t in range(t0, nt):
                 _record(f, _reals(gr[imj, :, :, t].T))
-    gk = np.tensordot(1.0 / ph, gr, axes=(1, 0)) / lq
     with open(os.path.join(path, "nn_ktau.bin"), "a") as f:
         for nk in range(lq):
-            f.write("%16.8f%16.8f\n" % (ks[nk, 0] / l1, ks[nk, 1] / l2))
-            for t in range(nt):
+            f.write("".join("%16.8f" % (ks[nk, d] / dims[d]) for d in range(len(dims))) + "\n")
+            for t in range(t0, nt):
                 _record(f, _reals(gk[nk, :, :, t].T))

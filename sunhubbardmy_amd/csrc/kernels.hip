@@ -791,16 +791,21 @@ k_update_submatrix(const DChain *chains, int nt, int slice_in_pass)
             bv[i] = G[(size_t)pvec[i] + (size_t)s * n];
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < ik; i += blockDim.x) {
-            cplx a = make_double2(0.0, 0.0);
-            for (int m = 0; m < ik; m++) a = cadd(a, cmul(cv[m], gi[m + i * K]));
-            v1[i] = a;
+        // v1 = G(s, p) Gamma^-1 (thread i: entry i) and v = v1 . G(p, s), summed by wave 0
+        // (ik <= K <= 64 entries)
+        if (threadIdx.x < 64) {
+            const int i = threadIdx.x;
+            cplx part = make_double2(0.0, 0.0);
+            if (i < ik) {
+                cplx a = make_double2(0.0, 0.0);
+                for (int m = 0; m < ik; m++) a = cadd(a, cmul(cv[m], gi[m + i * K]));
+                v1[i] = a;
+                part = cmul(a, bv[i]);
+            }
+            part.x = wave_sum(part.x);
+            part.y = wave_sum(part.y);
+            if (i == 0) red[0] = part;
         }
-        __syncthreads();
-        cplx part = make_double2(0.0, 0.0);
-        if (threadIdx.x == 0)
-            for (int i = 0; i < ik; i++) part = cadd(part, cmul(v1[i], bv[i]));
-        if (threadIdx.x == 0) red[0] = part;
         __syncthreads();
         cplx v = red[0];
         cplx gss = G[(size_t)s + (size_t)s * n];
@@ -843,15 +848,18 @@ k_update_submatrix(const DChain *chains, int nt, int slice_in_pass)
         }
         if (ik == K || s == n - 1) {
             if (ik > 0) {
-                // Gmat1 = G(:,p) Gamma^-1 ; gm2t(j, i) = G(p_i, j)
-                for (int r = threadIdx.x; r < n; r += blockDim.x) {
-                    for (int j = 0; j < ik; j++) {
-                        cplx a = make_double2(0.0, 0.0);
-                        for (int m = 0; m < ik; m++) a = cadd(a, cmul(gm1[(size_t)r + (size_t)m * n], gi[m + j * K]));
-                        gmat1[(size_t)r + (size_t)j * n] = a;
-                    }
-                    for (int i = 0; i < ik; i++) gm2t[(size_t)r + (size_t)i * n] = G[(size_t)pvec[i] + (size_t)r * n];
+                // Gmat1 = G(:,p) Gamma^-1 as an MFMA product (Gamma^-1 staged through W2, which
+                // the update does not use); gm2t(j, i) = G(p_i, j)
+                cplx *gig = c.W2;
+                for (int e = threadIdx.x; e < ik * ik; e += blockDim.x) {
+                    const int i = e % ik, j = e / ik;
+                    gig[i + (size_t)j * ik] = gi[i + j * K];   // ld ik: ik^2 <= n^2 entries of W2
                 }
+                for (int r = threadIdx.x; r < n; r += blockDim.x)
+                    for (int i = 0; i < ik; i++) gm2t[(size_t)r + (size_t)i * n] = G[(size_t)pvec[i] + (size_t)r * n];
+                __syncthreads();
+                for (int tm = 0; tm < n; tm += GT)
+                    zgemm_tile<0, 0>(gm1, n, gig, ik, gmat1, n, n, ik, ik, tm, 0, 1.0, 0.0, 0.0, 0.0);
                 __syncthreads();
                 wg_rank_k(G, n, gmat1, gm2t, ik, csr, csi);
             } else {

@@ -104,6 +104,7 @@ extern "C" {
 
 int sh_set_neighbours(sh_ctx *ctx, int z, const int32_t *nbr)
 {
+    if (ctx) ctx->graph_epoch++;
     if (!ctx || z < 1 || z > 16 || !nbr) return SH_E_ARG;
     size_t tot = 0;
     for (auto &c : ctx->hc) tot += (size_t)c.n * z;
@@ -165,6 +166,7 @@ int sh_features(sh_ctx *ctx, double *out)
 
 int sh_set_proposal(sh_ctx *ctx, const shl_net *net, int nprop, int local_updates)
 {
+    if (ctx) ctx->graph_epoch++;
     if (!ctx) return SH_E_ARG;
     if (!net) {
         ctx->prop_on = false;

@@ -84,6 +84,14 @@ struct sh_ctx : ctx_base {
     PropChain *d_prop = nullptr;
     float *pw = nullptr;
     PropNet pnet = {};
+    // hipGraph-captured passes (sh_set_graphs): one executable graph per pass kind and flags,
+    // captured again whenever a setter changes what a pass launches (graph_epoch)
+    bool use_graphs = false;
+    int graph_epoch = 1;
+    hipGraphExec_t gexec[8] = {};
+    int gepoch[8] = {};
+    // autotuner (sh_autotune): saved chain state per chain (G, HS field, scalars)
+    std::vector<void *> tune_save;
 };
 
 #define HIPCHK(x)                                                                  \
@@ -508,6 +516,8 @@ static void set_identity(sh_ctx *ctx, int pt)
     hipLaunchKernelGGL(k_set_identity, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, pt);
 }
 
+#define SH_KMAX 64   // largest delay depth / submatrix block
+
 // register-resident delay update: the template depth covering every chain's nublock, and the
 // thread count 2 * rows; 0 when the register budget does not allow it
 static int delay_reg_depth(sh_ctx *ctx, int *threads)
@@ -544,7 +554,7 @@ static void update_slice(sh_ctx *ctx, int nt, int slice_in_pass)
     } else {
         int K = 0;
         for (auto &c : ctx->hc) K = std::max(K, c.nublock);
-        size_t lds = (size_t)(K * K + 4 * K) * sizeof(cplx);
+        const size_t lds = (size_t)(K * K + 4 * K) * sizeof(cplx);
         hipLaunchKernelGGL(k_update_submatrix, dim3(ctx->nch), dim3(256), lds, ctx->st, ctx->dc, nt, slice_in_pass);
     }
 }
@@ -563,6 +573,41 @@ static int check_launch(sh_ctx *ctx)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(ctx, SH_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return 0;
+}
+
+// A pass: its launches issued directly, or (sh_set_graphs, timers off) captured once into an
+// executable hipGraph per pass kind and flags and replayed, so that a pass costs one host launch.
+// Kernel arguments are captured by value; the chain descriptors are read from device memory at
+// run time.  Every setter that changes which kernels a pass launches, their shapes or by-value
+// arguments (proposals, correlation bins, the time-displaced pass, the autotuned depth and tiles,
+// the delay-kernel switch) advances graph_epoch, which retires the captured graphs.
+template <class Body>
+static int run_pass(sh_ctx *ctx, int key, Body body)
+{
+    if (!ctx->use_graphs || ctx->timing) {
+        body();
+        return check_launch(ctx);
+    }
+    if (!ctx->gexec[key] || ctx->gepoch[key] != ctx->graph_epoch) {
+        if (ctx->gexec[key]) {
+            hipGraphExecDestroy(ctx->gexec[key]);
+            ctx->gexec[key] = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+        body();
+        hipError_t e = hipStreamEndCapture(ctx->st, &g);
+        if (e != hipSuccess || !g) return fail(ctx, SH_E_HIP, std::string("pass capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&ctx->gexec[key], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            ctx->gexec[key] = nullptr;
+            return fail(ctx, SH_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        ctx->gepoch[key] = ctx->graph_epoch;
+    }
+    HIPCHK(hipGraphLaunch(ctx->gexec[key], ctx->st));
+    return check_launch(ctx);
 }
 
 // learned block proposals (learn.hip): the per-pass refresh of the feature sums and one block
@@ -650,6 +695,7 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
         struct { const void *f; size_t bytes; } big[] = {
             {(const void *)k_cb_rows, lds_rows(ctx)}, {(const void *)k_cb_cols, lds_cols(ctx)},
             {(const void *)k_fft_rows, lds_fft(ctx)}, {(const void *)k_fft_cols, lds_fft(ctx)},
+            {(const void *)k_update_submatrix, (size_t)(SH_KMAX * SH_KMAX + 4 * SH_KMAX) * sizeof(cplx)},
             {(const void *)k_update_delay_reg<4>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 16 * sizeof(cplx)},
             {(const void *)k_update_delay_reg<8>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 32 * sizeof(cplx)},
             {(const void *)k_update_delay_reg<16>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 64 * sizeof(cplx)},
@@ -703,7 +749,8 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
         c.U = dalloc<cplx>(ctx, nn * (q.nst + 1)); c.V = dalloc<cplx>(ctx, nn * (q.nst + 1));
         c.D = dalloc<double>(ctx, (size_t)n * (q.nst + 1)); c.Ds = dalloc<double>(ctx, n);
         c.ldq = dalloc<cplx>(ctx, q.nst + 1);
-        int kb = std::max(q.nublock, 1);
+        // room for every depth the autotuner may pick
+        const int kb = std::max(q.nublock, SH_KMAX);
         c.avec = dalloc<cplx>(ctx, (size_t)n * kb); c.bvec = dalloc<cplx>(ctx, (size_t)n * kb);
         c.tau = dalloc<cplx>(ctx, n); c.jpvt = dalloc<int32_t>(ctx, n); c.piv = dalloc<int32_t>(ctx, n);
         c.lperm = dalloc<int32_t>(ctx, n);
@@ -767,6 +814,8 @@ int sh_destroy(sh_ctx *ctx)
     for (void *p : ctx->stage) hipFree(p);
     for (cplx *p : ctx->trace_bufs) if (p) hipFree(p);
     for (auto e : ctx->pool) hipEventDestroy(e);
+    for (auto g : ctx->gexec) if (g) hipGraphExecDestroy(g);
+    for (void *p : ctx->tune_save) hipFree(p);
     if (ctx->st) hipStreamDestroy(ctx->st);
     delete ctx;
     return 0;
@@ -840,6 +889,7 @@ int sh_set_dyn(sh_ctx *ctx, int chain, int ntdm, int l1, int l2, int l3, const i
     if (!ctx || chain < 0 || chain >= ctx->nch || !sites || ntdm < 0 || ntdm > ctx->ltrot || l1 < 1 || l2 < 1 ||
         l3 < 1)
         return SH_E_ARG;
+    ctx->graph_epoch++;
     if (ctx->ntdm >= 0 && ntdm != ctx->ntdm) return fail(ctx, SH_E_ARG, "all chains of a batch share ntdm");
     DChain &c = ctx->hc[chain];
     if (l1 * l2 * l3 != c.lq) return fail(ctx, SH_E_ARG, "l1 * l2 * l3 must be the number of unit cells");
@@ -917,6 +967,7 @@ int sh_set_eqt(sh_ctx *ctx, int chain, int l1, int l2, int l3, const int32_t *si
                const int32_t *plq, double theta)
 {
     if (!ctx || chain < 0 || chain >= ctx->nch) return SH_E_ARG;
+    ctx->graph_epoch++;
     DChain &c = ctx->hc[chain];
     size_t lds = 0;
     int rc = eqt_setup(ctx, c, ctx->hc, l1, l2, l3, sites, cells, zplq, plq, theta, lds);
@@ -1026,8 +1077,9 @@ int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure)
 {
     if (!ctx) return SH_E_ARG;
     if (!ctx->started) return fail(ctx, SH_E_STATE, "sweep_b0 before sweep_start_0b");
-    set_identity(ctx, ctx->nst);
     const bool upd = lupdate && ctx->hc[0].lwrapu, updv = lupdate && ctx->lwrapv;
+    auto body = [&]() {
+    set_identity(ctx, ctx->nst);
     if (upd || updv) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
     if (upd) prop_pass_start(ctx);
     int slice = 0;
@@ -1077,7 +1129,8 @@ int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure)
             wrap_replace(ctx, 1);
         }
     }
-    int rc = check_launch(ctx);
+    };
+    int rc = run_pass(ctx, 0 + 2 * (lupdate != 0) + (lmeasure != 0), body);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->st));
     int info = first_info(ctx);
@@ -1179,8 +1232,9 @@ int sh_sweep_0b(sh_ctx *ctx, int lupdate, int lmeasure, int lmeasure_dyn)
             if (!c.ltau) return fail(ctx, SH_E_STATE, "time-displaced measurement not set up on every chain (sh_set_dyn)");
         return sweep_0b_dyn(ctx, lmeasure);
     }
-    set_identity(ctx, 0);
     const bool upd = lupdate && ctx->hc[0].lwrapu, updv = lupdate && ctx->lwrapv;
+    auto body = [&]() {
+    set_identity(ctx, 0);
     if (upd || updv) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 2);
     if (upd) prop_pass_start(ctx);
     for (int nt = 1; nt <= ctx->ltrot; nt++) {
@@ -1227,7 +1281,8 @@ int sh_sweep_0b(sh_ctx *ctx, int lupdate, int lmeasure, int lmeasure_dyn)
             wrap_replace(ctx, 1);
         }
     }
-    int rc = check_launch(ctx);
+    };
+    int rc = run_pass(ctx, 4 + 2 * (lupdate != 0) + (lmeasure != 0), body);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->st));
     return 0;
@@ -1309,6 +1364,7 @@ int sh_reset_diag(sh_ctx *ctx)
     if (!ctx) return SH_E_ARG;
     for (auto &c : ctx->hc) {
         HIPCHK(hipMemsetAsync(c.sc + SC_MAXWRAP, 0, 6 * sizeof(double), ctx->st));
+        HIPCHK(hipMemsetAsync(c.sc + SC_NONFINITE, 0, sizeof(double), ctx->st));
     }
     HIPCHK(hipStreamSynchronize(ctx->st));
     return 0;
@@ -1357,6 +1413,7 @@ int sh_reset_obs(sh_ctx *ctx)
 int sh_set_trace(sh_ctx *ctx, int chain, int cap)
 {
     if (!ctx || chain < 0 || chain >= ctx->nch || cap < 0) return SH_E_ARG;
+    ctx->graph_epoch++;
     if ((int)ctx->trace_bufs.size() < ctx->nch) ctx->trace_bufs.resize(ctx->nch, nullptr);
     DChain &c = ctx->hc[chain];
     if (ctx->trace_bufs[chain]) { HIPCHK(hipFree(ctx->trace_bufs[chain])); ctx->trace_bufs[chain] = nullptr; }
@@ -1390,6 +1447,7 @@ int sh_get_trace(sh_ctx *ctx, int chain, double *buf, int *count)
 int sh_timers(sh_ctx *ctx, int enable, double *t)
 {
     if (!ctx) return SH_E_ARG;
+    ctx->graph_epoch++;
     timers_collect(ctx);
     if (t) {
         for (int k = 0; k < T_NCLS; k++) { t[k] = ctx->tms[k]; t[T_NCLS + k] = ctx->tcnt[k]; }
@@ -1404,6 +1462,151 @@ int sh_synchronize(sh_ctx *ctx)
     if (!ctx) return SH_E_ARG;
     HIPCHK(hipStreamSynchronize(ctx->st));
     return check_launch(ctx);
+}
+
+int sh_set_graphs(sh_ctx *ctx, int enable)
+{
+    if (!ctx) return SH_E_ARG;
+    ctx->use_graphs = enable != 0;
+    ctx->graph_epoch++;
+    return 0;
+}
+
+// ---- autotuner -------------------------------------------------------------------------------
+// The update of one slice (slice ltrot, the first of a b0 pass) is timed on the chains' current
+// G at every candidate depth, with the state every trial changes (G, the slice's HS field, the
+// scalar slots) restored before each trial and at the end, and the slice's uniforms replaced
+// by a fixed low-discrepancy sequence (the next updating pass draws its own).  Then the DFT
+// kinetic kernels at every vectors-per-workgroup that fits, on the scratch matrix W1.
+}  // extern "C"
+
+__global__ void k_tune_unif(const DChain *chains)
+{
+    const DChain &c = chains[blockIdx.x];
+    for (int i = threadIdx.x; i < c.n; i += blockDim.x) {
+        const double a = 0.5 + 0.6180339887498949 * i, b = 0.25 + 0.7548776662466927 * i;
+        c.unif[2 * i] = a - floor(a);
+        c.unif[2 * i + 1] = b - floor(b);
+    }
+}
+
+static int tune_state(sh_ctx *ctx, bool save)
+{
+    const int nt = ctx->ltrot;
+    for (int ci = 0; ci < ctx->nch; ci++) {
+        const DChain &c = ctx->hc[ci];
+        const size_t gb = (size_t)c.n * c.n * sizeof(cplx), cb = (size_t)c.n * sizeof(int32_t);
+        char *p = (char *)ctx->tune_save[ci];
+        int32_t *cf = c.conf + (size_t)(nt - 1) * c.n;
+        if (save) {
+            HIPCHK(hipMemcpyAsync(p, c.G, gb, hipMemcpyDeviceToDevice, ctx->st));
+            HIPCHK(hipMemcpyAsync(p + gb, cf, cb, hipMemcpyDeviceToDevice, ctx->st));
+            HIPCHK(hipMemcpyAsync(p + gb + cb, c.sc, SC_SIZE * sizeof(double), hipMemcpyDeviceToDevice, ctx->st));
+        } else {
+            HIPCHK(hipMemcpyAsync(c.G, p, gb, hipMemcpyDeviceToDevice, ctx->st));
+            HIPCHK(hipMemcpyAsync(cf, p + gb, cb, hipMemcpyDeviceToDevice, ctx->st));
+            HIPCHK(hipMemcpyAsync(c.sc, p + gb + cb, SC_SIZE * sizeof(double), hipMemcpyDeviceToDevice, ctx->st));
+        }
+    }
+    return 0;
+}
+
+static int set_nublock(sh_ctx *ctx, int k)
+{
+    for (auto &c : ctx->hc) c.nublock = k;
+    HIPCHK(hipMemcpy(ctx->dc, ctx->hc.data(), sizeof(DChain) * ctx->nch, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// best of `reps` timed runs of f (after one untimed run), in ms; before() runs outside the timing
+template <class Before, class F>
+static double tune_time(sh_ctx *ctx, int reps, Before before, F f)
+{
+    hipEvent_t a = ev_get(ctx), b = ev_get(ctx);
+    double best = 1e30;
+    for (int r = 0; r <= reps; r++) {
+        before();
+        hipEventRecord(a, ctx->st);
+        f();
+        hipEventRecord(b, ctx->st);
+        hipEventSynchronize(b);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, a, b);
+        if (r > 0) best = std::min(best, (double)ms);
+    }
+    ctx->pool_used -= 2;
+    return best;
+}
+
+extern "C" {
+
+int sh_autotune(sh_ctx *ctx, int *tuned)
+{
+    if (!ctx || !tuned) return SH_E_ARG;
+    if (!ctx->started) return fail(ctx, SH_E_STATE, "sh_autotune before sweep_start_0b");
+    if (ctx->timing) return fail(ctx, SH_E_STATE, "sh_autotune with the timers on");
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    const int k0 = ctx->hc[0].nublock;
+    int best_k = k0;
+    if (ctx->hc[0].lwrapu) {
+        if (ctx->tune_save.empty()) {
+            for (auto &c : ctx->hc) {
+                void *p = nullptr;
+                const size_t bytes = (size_t)c.n * c.n * sizeof(cplx) + (size_t)c.n * sizeof(int32_t) + SC_SIZE * sizeof(double);
+                HIPCHK(hipMalloc(&p, bytes));
+                ctx->tune_save.push_back(p);
+            }
+        }
+        int rc = tune_state(ctx, true);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_tune_unif, dim3(ctx->nch), dim3(256), 0, ctx->st, ctx->dc);
+        std::vector<int> cand;
+        if (ctx->update == SH_U_DELAY) {
+            if (k0 > 1) {
+                // register-resident depths where they fit, the memory-resident kernel above
+                for (int k : {4, 8, 16, 32}) cand.push_back(k);
+                int thr = 0;
+                for (auto &c : ctx->hc) c.nublock = 32;
+                const bool reg32 = delay_reg_depth(ctx, &thr) != 0;
+                for (auto &c : ctx->hc) c.nublock = k0;
+                if (!reg32) cand.push_back(64);
+            }
+        } else {
+            for (int k : {8, 16, 24, 32, 48, 64}) cand.push_back(k);
+        }
+        double best = 1e30;
+        for (int k : cand) {
+            if (k > ctx->nmax) continue;
+            if ((rc = set_nublock(ctx, k))) return rc;
+            const double ms = tune_time(ctx, 3, [&]() { tune_state(ctx, false); }, [&]() { update_slice(ctx, ctx->ltrot, 0); });
+            if (hipGetLastError() != hipSuccess) return fail(ctx, SH_E_HIP, "autotune launch failed");
+            if (ms < best) { best = ms; best_k = k; }
+        }
+        if ((rc = set_nublock(ctx, best_k))) return rc;
+        if ((rc = tune_state(ctx, false))) return rc;
+    }
+    int best_f = 0;
+    if (ctx->kmode == SH_K_FFT) {
+        double best = 1e30;
+        const int f0 = ctx->fft_fcb;
+        best_f = f0;
+        for (int f = 2; f <= 16; f *= 2) {
+            if ((size_t)f * ctx->fft_ld * sizeof(cplx) > 96 * 1024) continue;
+            ctx->fft_fcb = f;
+            const double ms = tune_time(ctx, 3, []() {}, [&]() {
+                left_pass(ctx, M_W1, 0, 0, 1 | 2 | 4, 0, ctx->ltrot, ctx->ltrot, 1);
+                right_pass(ctx, 1, 1 | 2 | 4, 1, ctx->ltrot, M_W1);
+            });
+            if (hipGetLastError() != hipSuccess) return fail(ctx, SH_E_HIP, "autotune launch failed");
+            if (ms < best) { best = ms; best_f = f; }
+        }
+        ctx->fft_fcb = best_f;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    ctx->graph_epoch++;
+    tuned[0] = best_k;
+    tuned[1] = best_f;
+    return 0;
 }
 
 }  // extern "C"
@@ -1514,7 +1717,7 @@ extern "C" int sh_dbg_op(sh_ctx *ctx, int op, const int *a)
         break;
     case 12: measure(ctx); break;
     case 13: wrap_replace(ctx, a[0]); break;
-    case 14: ctx->legacy_update = a[0] != 0; break;
+    case 14: ctx->legacy_update = a[0] != 0; ctx->graph_epoch++; break;
     default: return SH_E_ARG;
     }
     int rc = check_launch(ctx);

@@ -241,6 +241,19 @@ class FTWalkers:
     def synchronize(self):
         self._check(_lib.lib().sh_synchronize(self._ctx), "synchronize")
 
+    def set_graphs(self, enable=True):
+        """Replay every kind of pass from a captured hipGraph (one host launch per pass)."""
+        self._check(_lib.lib().sh_set_graphs(self._ctx, int(bool(enable))), "set_graphs")
+
+    def autotune(self):
+        """Pick the delay depth / submatrix block (and the DFT kernels' tile) by timing them on
+        the chains' current state, which is left unchanged; call after sweep_start_0b."""
+        out = np.zeros(2, np.int32)
+        self._check(_lib.lib().sh_autotune(self._ctx, out.ctypes.data_as(_lib.ip)), "autotune")
+        for t in self.tables:
+            t["nublock"] = int(out[0])
+        return {"nublock": int(out[0]), "fft_vectors_per_workgroup": int(out[1])}
+
     # -- state readback ------------------------------------------------------------------------
     def G(self, chain=0):
         n = self.tables[chain]["ndim"]

@@ -11,6 +11,8 @@ C1  square 4x4, U = 4, beta = 4, fast update, checkerboard K (the CPU-runnable c
 C2  honeycomb L = 12 (N = 288), U = 4, delay update k = 32, dense K, one walker
 C3  SU(4) square lattices L = 8, 12, 16 in one batch, U = 6, general 4-component HS field,
     submatrix update, FFT K (32 + 24 + 16 chains at beta = 8)
+C5  learned-proposal form: square 20x20 (N = 400), U = 6, beta = 12, 16 chains, block moves of
+    64 inner attempts judged by the exact determinant ratio
 FFT the DFT kinetic term on non-square, chain and cubic lattices
 """
 import sys
@@ -35,6 +37,8 @@ C1 = dict(lattice="square", la=4, lb=4, beta=4.0, dtau=0.1, rt=1.0, mu=0.0, rhub
           kmode="checkerboard", update="fast", nwrap=10)
 C2 = dict(lattice="honeycomb", la=12, lb=12, beta=10.0, dtau=0.1, rt=1.0, mu=0.0, rhub=4.0, nflr=2, exact_hs=True,
           kmode="dense", update="delay", nwrap=10, nublock=32)
+C5 = dict(lattice="square", la=20, lb=20, beta=12.0, dtau=0.1, rt=1.0, mu=0.0, rhub=6.0, nflr=2, exact_hs=True,
+          kmode="checkerboard", update="delay", nwrap=10, nublock=32)
 C3 = dict(lattice="square", beta=8.0, dtau=0.1, rt=1.0, mu=0.0, rhub=6.0, nflr=4, exact_hs=False, kmode="fft",
           update="submatrix", nwrap=10, nublock=32)
 
@@ -150,3 +154,81 @@ def test_c3_full_batch_properties():
         accs.append(d["accepted"] / d["tried"])
     say("C3 acceptance per lattice:", [float(np.mean(accs[a:b])) for a, b in ((0, 32), (32, 56), (56, 72))])
     w.close()
+
+
+def _random_net(seed=13):
+    from sunhubbardmy_amd.learned import NWEIGHTS, EffectiveAction
+    wts = (np.random.default_rng(seed).standard_normal(NWEIGHTS) * np.sqrt(2.0 / 132)).astype(np.float32)
+    norm = np.array([0.0, 0.0, 0.0, 0.0, 0.05, 0.05, 0.05, 0.05, 0.0, 0.01])
+    net = EffectiveAction(capacity=16, seed=1)
+    net.set_weights(wts)
+    net.set_norm(norm)
+    return net, wts, norm
+
+
+def test_c5_learned_parity_short_beta():
+    """C5's lattice and field with block proposals of 64 inner attempts after every slice's
+    local update, at beta = 0.5, against the oracle's restatement of the block move: HS field
+    bit-identical after both passes, G within 1e-10, identical proposal statistics."""
+    kw = dict(C5)
+    kw["beta"] = 0.5
+    kw["nwrap"] = 5
+    seed = chain_seed(3)
+    net, wts, norm = _random_net()
+    w = FTWalkers([Model(seed=seed, **kw)])
+    o = oracle.OracleChain(seed=seed, **kw)
+    w.set_neighbours()
+    w.set_proposal(net, nprop=64)
+    o.set_proposal(w.tables[0]["lat"]["nnlist"], 64, wts, norm)
+    w.init_conf()
+    o.init_conf()
+    w.sweep_start_0b()
+    o.sweep_start()
+    for p in ("b0", "0b"):
+        getattr(w, "sweep_" + p)(True, True)
+        getattr(o, "sweep_" + p)(True, True)
+        say("  C5 pass", p)
+        assert np.array_equal(w.get_conf(), o.conf), p
+        assert relmax(w.G(), o.G) < 1e-10, p
+    st = w.proposal_stats()
+    assert st == o.proposal_stats()
+    assert st["moves"] == 2 * w.tables[0]["ltrot"] and st["spins_proposed"] > 0
+    say("  C5 proposals:", st)
+    w.close()
+    net.close()
+
+
+def test_c5_full_shape_trained_properties():
+    """The C5 batch as stated (16 chains of 20x20, beta = 12, Ltrot = 120): plain sweeps feed the
+    network, the library's kernels train it, and one sweep with 64-attempt block proposals keeps
+    the properties of the sweep (wrap error, finite G, half filling) with moves made and judged."""
+    from sunhubbardmy_amd.learned import EffectiveAction
+    models = [Model(seed=chain_seed(c), **C5) for c in range(16)]
+    w = FTWalkers(models)
+    assert w.tables[0]["ndim"] == 400 and w.tables[0]["ltrot"] == 120
+    w.set_neighbours()
+    w.init_conf()
+    w.sweep_start_0b()
+    net = EffectiveAction(capacity=256, seed=5)
+    for s in range(3):
+        w.sweep_b0(True, False)
+        net.add_samples(w)
+        w.sweep_0b(True, False)
+        net.add_samples(w)
+    loss = net.train(epochs=20, batch=32, lr=1e-3)
+    assert np.all(np.isfinite(loss))
+    w.set_proposal(net, nprop=64)
+    w.reset_proposal_stats()
+    w.reset_diag()
+    w.reset_obs()
+    w.sweep(True, True)
+    moves = acc = 0
+    for c in range(16):
+        _properties(w, c)
+        st = w.proposal_stats(c)
+        moves += st["moves"]
+        acc += st["accepted"]
+        assert st["moves"] == 2 * 120
+    say("C5 full shape: block acceptance %.3f over %d moves, loss %.3g -> %.3g" % (acc / moves, moves, loss[0], loss[-1]))
+    w.close()
+    net.close()

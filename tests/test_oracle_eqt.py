@@ -271,3 +271,32 @@ def test_write_dyn_bin(tmp_path):
     i0 = ks.index((0.0, 0.0))
     k0 = np.array([list(map(float, lines[i0 * (o.ltrot + 2) + 1 + t].split())) for t in range(o.ltrot + 1)])
     assert np.allclose(k0[:, 0], z[:, 0, 0, :].real.mean(axis=1), rtol=1e-7, atol=1e-12)
+
+
+@pytest.mark.parametrize("lattice,dims", [("chain", (8, 1, 1)), ("cubic", (4, 4, 2))])
+def test_write_dyn_bin_chain_and_cube(tmp_path, lattice, dims):
+    """dyn_output1d (per t = 0 .. ntdm, one k coordinate) and dyn_output3d (t = 0 skipped, three k
+    coordinates): the records of the synthetic bins znn(t, imj) = (t + 1) + i imj and the k = 0
+    component = the cell average of every written t."""
+    l1, l2, l3 = dims
+    lq = l1 * l2 * l3
+    nt = 5
+    z = np.zeros((nt, 2, 2, lq), complex)
+    for t in range(nt):
+        z[t, 0, 0, :] = (t + 1) + 1j * np.arange(lq)
+    binio.write_dyn_bin(str(tmp_path), lattice, l1, l2, 1, [z], l3=l3)
+    t0 = 1 if lattice == "cubic" else 0
+    ntw = nt - t0
+    r = np.loadtxt(tmp_path / "nn_rtau.bin", ndmin=2)
+    assert r.shape == (lq * ntw, 2)
+    assert np.allclose(r[:ntw, 0], np.arange(t0, nt) + 1.0)
+    assert np.allclose(r[ntw:2 * ntw, 1], 1.0)                 # imj = 1: Im = 1 at every t
+    lines = (tmp_path / "nn_ktau.bin").read_text().splitlines()
+    assert len(lines) == lq * (ntw + 1)
+    ncoord = 1 if lattice == "chain" else 3
+    ks = [tuple(map(float, lines[i * (ntw + 1)].split())) for i in range(lq)]
+    assert all(len(k) == ncoord for k in ks)
+    i0 = ks.index((0.0,) * ncoord)
+    k0 = np.array([list(map(float, lines[i0 * (ntw + 1) + 1 + t].split())) for t in range(ntw)])
+    assert np.allclose(k0[:, 0], np.arange(t0, nt) + 1.0)
+    assert np.allclose(k0[:, 1], (lq - 1) / 2.0)
