@@ -12,7 +12,7 @@ sweep (sunhubbardmy_amd/autocorr.py tau_int_batch), so that the relaxation every
 batch still shares after the warm-up is not read as autocorrelation.  The line also carries the
 estimate with per-chain means only (tau_int_per_chain_means).
 
-Sequence: W warm-up sweeps (at least 20) of the plain delay update from random fields, then 40
+Sequence: W warm-up sweeps (at least --therm, 20) of the plain delay update from random fields, then 40
 more plain sweeps whose configurations after every pass train the effective-action network
 (40 sweeps x 2 passes x 256 chains = 20480 configurations, whatever W is; the library's FP32
 kernels train it).  The same chains are then timed for exactly K sweeps with the plain sweep,
@@ -339,6 +339,8 @@ def main():
     ap.add_argument("--nprop", type=int, default=16, help="inner Metropolis attempts per block proposal (<= 64)")
     ap.add_argument("--tau-sweeps", type=int, default=100,
                     help="length of each phase's per-sweep series for tau_int (the first --steps are timed)")
+    ap.add_argument("--therm", type=int, default=20,
+                    help="least number of plain warm-up sweeps from the random start, whatever --warmup is")
     ap.add_argument("--train-sweeps", type=int, default=40,
                     help="plain sweeps after the warm-up whose configurations train the network")
     ap.add_argument("--epochs", type=int, default=30, help="training epochs of the effective action")
@@ -387,7 +389,7 @@ def main():
     # warm-up with the plain sweep, then train_sweeps more whose configurations after every pass
     # train the effective action
     learn = args.proposal != "plain"
-    nwarm = max(args.warmup, 20)
+    nwarm = max(args.warmup, args.therm)
     net = EffectiveAction(capacity=max(2 * args.train_sweeps * args.chains, 1), seed=chain_seed(0)) if learn else None
     for s in range(nwarm + args.train_sweeps):
         collect = learn and s >= nwarm

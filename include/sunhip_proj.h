@@ -54,7 +54,22 @@ typedef struct shp_model {
     int uphase_from_conf;         /* 1 when set_phase multiplies the field's phases (rhub > 0) */
     double kin_weight;            /* factor of the zkint bin: 1 (0 is read as 1), nflr * rt for the
                                      chain (src/dqmc_measure/equaltime_measure1d.f90:76) */
+    /* the update algorithm of both fields (src/dqmc_config_u/dqmc_proj_update*.f90,
+     * src/dqmc_config_v/dqmc_proj_update*.f90; the reference selects it for both by its build):
+     * SHP_U_LR submatrix-LR in stocks of nustock rows (default 0);
+     * SHP_U_FAST one site (bond) at a time: rank-1 (rank-2) updates of UR and (UL UR)^-1;
+     * SHP_U_DELAY / SHP_U_SUBMATRIX_G updates of G = 1 - UR (UL UR)^-1 UL (the delay form with
+     * depth gblock 1..64, or G updated at every accepted site or bond), then UR's flipped rows
+     * and a fresh (UL UR)^-1.  The four take the same decisions from the same draws.  The
+     * G-based forms need one field alone (on-site or bond).  All chains of a batch share update
+     * and gblock. */
+    int update, gblock;
 } shp_model;
+
+#define SHP_U_LR 0
+#define SHP_U_FAST 1
+#define SHP_U_DELAY 2
+#define SHP_U_SUBMATRIX_G 3
 
 /* allocate_core (src/dqmc_proj_core.f90:23-49) for a batch of chains sharing ndim, ne, ltrot,
  * nwrap, kmode and nustock */

@@ -767,14 +767,16 @@ k_update_submatrix(const DChain *chains, int nt, int slice_in_pass)
 {
     extern __shared__ cplx sh[];
     const DChain &c = chains[blockIdx.x];
-    const int n = c.n, K = c.nublock;
-    cplx *gi = sh;                    // K*K Gamma^-1, column-major
-    cplx *cv = gi + K * K;            // K
+    const int n = c.n, K = c.nublock, KL = K + 1;
+    cplx *gi = sh;                    // Gamma^-1, column-major, leading dimension K + 1 (lanes reading
+                                      // one entry of K columns each fall on different banks)
+    cplx *cv = gi + KL * K;           // K
     cplx *bv = cv + K;                // K
     cplx *v1 = bv + K;                // K
     cplx *v4 = v1 + K;                // K
     __shared__ int pvec[256];
     __shared__ cplx red[256];
+    __shared__ cplx s_gss[2];         // G(s, s), by the parity of s
     cplx *G = c.G, *gm1 = c.avec, *gmat1 = c.bvec, *gm2t = c.W1;
     double *csr = c.rwork, *csi = c.rwork + n;
     int32_t *cf = c.conf + (size_t)(nt - 1) * n;
@@ -790,6 +792,7 @@ k_update_submatrix(const DChain *chains, int nt, int slice_in_pass)
             cv[i] = G[(size_t)s + (size_t)pvec[i] * n];
             bv[i] = G[(size_t)pvec[i] + (size_t)s * n];
         }
+        if (threadIdx.x == blockDim.x - 1) s_gss[s & 1] = G[(size_t)s + (size_t)s * n];
         __syncthreads();
         // v1 = G(s, p) Gamma^-1 (thread i: entry i) and v = v1 . G(p, s), summed by wave 0
         // (ik <= K <= 64 entries)
@@ -798,7 +801,7 @@ k_update_submatrix(const DChain *chains, int nt, int slice_in_pass)
             cplx part = make_double2(0.0, 0.0);
             if (i < ik) {
                 cplx a = make_double2(0.0, 0.0);
-                for (int m = 0; m < ik; m++) a = cadd(a, cmul(cv[m], gi[m + i * K]));
+                for (int m = 0; m < ik; m++) a = cadd(a, cmul(cv[m], gi[m + i * KL]));
                 v1[i] = a;
                 part = cmul(a, bv[i]);
             }
@@ -807,13 +810,14 @@ k_update_submatrix(const DChain *chains, int nt, int slice_in_pass)
             if (i == 0) red[0] = part;
         }
         __syncthreads();
+        // red[0] and s_gss[s & 1] are rewritten only after the next site's first barrier, which
+        // every thread reaches after these reads
         cplx v = red[0];
-        cplx gss = G[(size_t)s + (size_t)s * n];
+        cplx gss = s_gss[s & 1];
         cplx v2 = make_double2(v.x + gss.x - 1.0, v.y + gss.y);
         cplx ratio1 = csub(make_double2(1.0, 0.0), cmul(v2, d.db));
         cplx rt;
         bool acc = site_accept(c, ratio1, d, u[2 * s + 1], phase, &rt);
-        __syncthreads();
         if (acc) {
             accm += 1.0;
             phase = phase_after(phase, rt);
@@ -822,20 +826,20 @@ k_update_submatrix(const DChain *chains, int nt, int slice_in_pass)
             cplx v3 = cinv(den);
             for (int i = threadIdx.x; i < ik; i += blockDim.x) {
                 cplx a = make_double2(0.0, 0.0);
-                for (int m = 0; m < ik; m++) a = cadd(a, cmul(gi[i + m * K], bv[m]));
+                for (int m = 0; m < ik; m++) a = cadd(a, cmul(gi[i + m * KL], bv[m]));
                 v4[i] = a;
             }
             __syncthreads();
             for (int e = threadIdx.x; e < ik * ik; e += blockDim.x) {
                 int i = e % ik, m = e / ik;
-                gi[i + m * K] = cadd(gi[i + m * K], cmul(cmul(v4[i], v3), v1[m]));
+                gi[i + m * KL] = cadd(gi[i + m * KL], cmul(cmul(v4[i], v3), v1[m]));
             }
             for (int i = threadIdx.x; i < ik; i += blockDim.x) {
-                gi[ik + i * K] = cmul(v3, v1[i]);
-                gi[i + ik * K] = cmul(v4[i], v3);
+                gi[ik + i * KL] = cmul(v3, v1[i]);
+                gi[i + ik * KL] = cmul(v4[i], v3);
             }
             if (threadIdx.x == 0) {
-                gi[ik + ik * K] = v3;
+                gi[ik + ik * KL] = v3;
                 pvec[ik] = s;
                 cf[s] = d.isp;
                 cplx f = cdiv(make_double2(1.0, 0.0), cadd(make_double2(1.0, 0.0), d.db));
@@ -853,7 +857,7 @@ k_update_submatrix(const DChain *chains, int nt, int slice_in_pass)
                 cplx *gig = c.W2;
                 for (int e = threadIdx.x; e < ik * ik; e += blockDim.x) {
                     const int i = e % ik, j = e / ik;
-                    gig[i + (size_t)j * ik] = gi[i + j * K];   // ld ik: ik^2 <= n^2 entries of W2
+                    gig[i + (size_t)j * ik] = gi[i + j * KL];  // ld ik: ik^2 <= n^2 entries of W2
                 }
                 for (int r = threadIdx.x; r < n; r += blockDim.x)
                     for (int i = 0; i < ik; i++) gm2t[(size_t)r + (size_t)i * n] = G[(size_t)pvec[i] + (size_t)r * n];

@@ -8,6 +8,7 @@
 struct shp_ctx : ctx_base {
     int nch = 0, n = 0, ne = 0, ltrot = 0, nwrap = 0, nst = 0, kmode = 0, ns = 0, mid_len = 1;
     int nn_nf = 0, nn_lf = 0, wrap_check = 1, pb = PB;
+    int update = SHP_U_LR, gblock = 0;   // update algorithm of the on-site field and its depth
     bool lwrapu = false, lwrapv = true;
     size_t slice_draws = 0;       // uniforms per slice of a pass
     bool tracing = false, started = false;
@@ -116,12 +117,44 @@ void run_ops(shp_ctx *ctx, const POps &o)
     if (part.nops) launch_prop(ctx, part);
 }
 
-// ---- submatrix-LR update of family nf at slice nt; uoff = its first uniform in the pass ----
+void green(shp_ctx *ctx, int ul, int ur, int inv, int g);
+void inverse(shp_ctx *ctx, int ul, int ur, int inv, int slot);
+
+// the G-based forms for the bond field (src/dqmc_config_v/dqmc_proj_update_delay.f90,
+// dqmc_proj_update_submatrix_G.f90): G from the current propagators; the finite-temperature bond
+// update on it through the chain's view, the delay form or every accepted bond applied to G at
+// once (its draws: the pass buffer at 2 nn_lf nn_nf per slice, the bond field alone); UR's rows of
+// the flipped bonds; a fresh (UL UR)^-1
+void update_family_g(shp_ctx *ctx, int nt, int nf, size_t uoff0)
+{
+    const int nl = ctx->nn_lf;
+    green(ctx, P_UL, P_UR, P_ULRINV, P_G);
+    Tic t(ctx, T_UPDATE);
+    hipLaunchKernelGGL(kp_v_save, dim3((nl + 255) / 256, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dc, nt, nf);
+    const int slice = (int)(uoff0 / ctx->slice_draws);
+    const int p = (int)((uoff0 % ctx->slice_draws) / (2 * (size_t)nl));
+    if (ctx->update == SHP_U_DELAY)
+        hipLaunchKernelGGL(k_vupdate_delay, dim3(ctx->nch), dim3(256), (size_t)4 * 64 * sizeof(cplx), ctx->st,
+                           ctx->dview, nt, nf, slice, p);
+    else
+        hipLaunchKernelGGL(k_vupdate, dim3(ctx->nch), dim3(256), (size_t)4 * ctx->n * sizeof(cplx), ctx->st,
+                           ctx->dview, nt, nf, slice, p);
+    hipLaunchKernelGGL(kp_v_rescale, dim3((unsigned)(((size_t)nl * ctx->ne + 255) / 256), 1, ctx->nch), dim3(256), 0,
+                       ctx->st, ctx->dc, nt, nf);
+    inverse(ctx, P_UL, P_UR, P_ULRINV, -1);
+}
+
+// ---- submatrix-LR update of family nf at slice nt (the fast form: stocks of one bond), or a
+// G-based form; uoff = its first uniform in the pass ----
 void update_family(shp_ctx *ctx, int nt, int nf, size_t uoff0)
 {
+    if (ctx->update == SHP_U_DELAY || ctx->update == SHP_U_SUBMATRIX_G) {
+        update_family_g(ctx, nt, nf, uoff0);
+        return;
+    }
     Tic t(ctx, T_UPDATE);
     const int ns = ctx->ns, ne = ctx->ne, nl = ctx->nn_lf;
-    const int per = ns / 2;
+    const int per = ctx->update == SHP_U_FAST ? 1 : ns / 2;
     for (int ib0 = 0; ib0 < nl; ib0 += per) {
         const int nb = std::min(per, nl - ib0), nup = 2 * nb;
         const int cnt = std::max(ns * ne, ns * ns);
@@ -142,13 +175,45 @@ void update_family(shp_ctx *ctx, int nt, int nf, size_t uoff0)
     }
 }
 
-// ---- submatrix-LR update of the on-site field at slice nt, stocks of ns consecutive sites ----
+// the G-based forms (dqmc_proj_update_delay.f90, dqmc_proj_update_submatrix_G.f90): G from the
+// current UR, (UL UR)^-1, UL; the finite-temperature delay or submatrix kernel on it through the
+// chain's view (its uniforms are the pass buffer at 2 N per slice: the on-site field alone);
+// UR's flipped rows; a fresh (UL UR)^-1
+
+void update_u_g(shp_ctx *ctx, int nt, size_t uoff0)
+{
+    const int n = ctx->n;
+    green(ctx, P_UL, P_UR, P_ULRINV, P_G);
+    Tic t(ctx, T_UPDATE);
+    hipLaunchKernelGGL(kp_u_save, dim3((n + 255) / 256, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dc, nt);
+    const int slice = (int)(uoff0 / (2 * (size_t)n));
+    if (ctx->update == SHP_U_DELAY) {
+        const size_t lds = (size_t)(n + 2 * 64) * sizeof(cplx);
+        hipLaunchKernelGGL(k_update_delay, dim3(ctx->nch), dim3(256), lds, ctx->st, ctx->dview, nt, slice);
+    } else {
+        const int K = ctx->gblock;
+        const size_t lds = (size_t)(K * (K + 1) + 4 * K) * sizeof(cplx);
+        hipLaunchKernelGGL(k_update_submatrix, dim3(ctx->nch), dim3(256), lds, ctx->st, ctx->dview, nt, slice);
+    }
+    hipLaunchKernelGGL(kp_u_rescale, dim3((unsigned)(((size_t)n * ctx->ne + 255) / 256), 1, ctx->nch), dim3(256), 0,
+                       ctx->st, ctx->dc, nt);
+    inverse(ctx, P_UL, P_UR, P_ULRINV, -1);
+}
+
+// ---- update of the on-site field at slice nt: submatrix-LR in stocks of ns consecutive sites
+// (the fast form: stocks of one site), or a G-based form ----
 void update_u(shp_ctx *ctx, int nt, size_t uoff0)
 {
+    if (ctx->update == SHP_U_DELAY || ctx->update == SHP_U_SUBMATRIX_G) {
+        update_u_g(ctx, nt, uoff0);
+        return;
+    }
     Tic t(ctx, T_UPDATE);
     const int ns = ctx->ns, ne = ctx->ne, n = ctx->n;
-    for (int i0 = 0; i0 < n; i0 += ns) {
-        const int nup = std::min(ns, n - i0);
+    // the fast update is the same algebra one site at a time: stocks of one site
+    const int step = ctx->update == SHP_U_FAST ? 1 : ns;
+    for (int i0 = 0; i0 < n; i0 += step) {
+        const int nup = std::min(step, n - i0);
         const int cnt = std::max(ns * ne, ns * ns);
         hipLaunchKernelGGL(kp_stock_gather, dim3((cnt + 255) / 256, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dc,
                            -1, i0, nup);
@@ -325,6 +390,8 @@ int shp_create(shp_ctx **out, int nchains, const shp_model *m, int device)
     ctx->nn_lf = q0.nn_lf; ctx->wrap_check = q0.wrap_check ? 1 : 0;
     ctx->lwrapv = q0.vtab != nullptr;
     ctx->lwrapu = q0.ubmat != nullptr;
+    ctx->update = q0.update;
+    ctx->gblock = q0.gblock;
     ctx->slice_draws = (ctx->lwrapu ? (size_t)2 * q0.ndim : 0) + (ctx->lwrapv ? (size_t)2 * q0.nn_lf * q0.nn_nf : 0);
     for (int c = 0; c < nchains; c++) {
         const shp_model &q = m[c];
@@ -339,6 +406,14 @@ int shp_create(shp_ctx **out, int nchains, const shp_model *m, int device)
             (q.vtab != nullptr) != (m[0].vtab != nullptr) || (q.ubmat != nullptr) != (m[0].ubmat != nullptr)) {
             delete ctx;
             return SH_E_ARG;
+        }
+        if (q.update != q0.update || q.gblock != q0.gblock || q.update < SHP_U_LR ||
+            q.update > SHP_U_SUBMATRIX_G ||
+            (q.update >= SHP_U_DELAY && ((q.ubmat != nullptr) == (q.vtab != nullptr) || q.gblock < 1 || q.gblock > 64))) {
+            int rc = fail(ctx, SH_E_ARG, "update: one algorithm for the batch; the G-based forms need one field alone "
+                                         "(on-site or bond) and a depth 1..64");
+            delete ctx;
+            return rc;
         }
         if (q.ubmat && (q.ulcomp < 2 || q.ulcomp > 4 || !q.ubmat_inv || !q.udelta || !q.uratio || !q.ugaml ||
                         !q.uphase_arg || q.nn_nf * 2 + 3 > POPS_MAX)) {
@@ -467,6 +542,51 @@ int shp_create(shp_ctx **out, int nchains, const shp_model *m, int device)
         v.n = n; v.lq = q.lq; v.nn_nf = q.nn_nf; v.nn_lf = q.nn_lf; v.nflr = q.nflr; v.lcomp = 2;
         v.G = c.G; v.bonds = c.bonds; v.stag = c.stag; v.sc = c.sc; v.rwork = c.rwork;
         v.kin_weight = q.kin_weight != 0.0 ? q.kin_weight : 1.0;
+        if (ctx->update >= SHP_U_DELAY) {
+            // ... and the G-based updates of the on-site field the HS field, its draws and tables,
+            // the delayed vectors and the submatrix scratch
+            const int K = ctx->gblock;
+            c.uold = dalloc<int32_t>(ctx, n);
+            v.ltrot = q.ltrot; v.nublock = K;
+            v.update = ctx->update == SHP_U_DELAY ? SH_U_DELAY : SH_U_SUBMATRIX;
+            if (ctx->lwrapu) {
+                v.conf = c.uconf; v.unif = c.unif;
+                v.lcomp = q.ulcomp;
+                v.lwrapu = 1;
+                for (int l = 0; l < q.ulcomp; l++) v.gaml[l] = q.ugaml[l];
+                for (int k = 0; k < (q.ulcomp - 1) * q.ulcomp; k++) {
+                    v.delta_bmat[k] = c.ud[k];
+                    v.phase_ratio[k] = c.uratio[k];
+                }
+            } else {
+                // the bond field: its draws, tables and field; acceptance into the chain's
+                // (accepted, tried) slots
+                v.confv = c.conf; v.unifv = c.unif; v.lwrapv = 1;
+                for (int k = 0; k < 8; k++) v.vtb[k] = c.vt[k];
+                for (int k = 0; k < 4; k++) v.vdl[k] = c.vd[k];
+                v.vacc = c.sc + SC_ACC;
+            }
+            v.avec = dalloc<cplx>(ctx, (size_t)n * K);
+            v.bvec = dalloc<cplx>(ctx, (size_t)n * K);
+            v.W1 = dalloc<cplx>(ctx, (size_t)n * K);
+            v.W2 = dalloc<cplx>(ctx, (size_t)K * K);
+            if (!c.uold || !v.avec || !v.bvec || !v.W1 || !v.W2) {
+                int rc = fail(ctx, SH_E_HIP, "device allocation failed");
+                shp_destroy(ctx);
+                return rc;
+            }
+        }
+    }
+    if (ctx->update == SHP_U_SUBMATRIX_G) {
+        const size_t lds = (size_t)(ctx->gblock * (ctx->gblock + 1) + 4 * ctx->gblock) * sizeof(cplx);
+        if (lds > 64 * 1024 &&
+            hipFuncSetAttribute((const void *)k_update_submatrix, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+                hipSuccess) {
+            (void)hipGetLastError();
+            int rc = fail(ctx, SH_E_HIP, "dynamic LDS request above the device limit");
+            shp_destroy(ctx);
+            return rc;
+        }
     }
     ctx->dc = dupload<PChain>(ctx, ctx->hc.data(), nchains);
     ctx->dview = dupload<DChain>(ctx, views.data(), nchains);

@@ -35,6 +35,8 @@ struct PChain {
     int32_t *uconf;
     cplx ub[4], ubi[4], ud[12], uratio[12];   // bmat, bmat_inv, delta_bmat(iflip, is), phase_ratio
     double ug[4], upa[4];                      // gaml, arg of phase(is)
+    // the G-based updates of the on-site field: the slice's field before the update
+    int32_t *uold;
 };
 
 // uniforms of one slice in a pass: 2 per site of the on-site field, 2 per bond of every family
@@ -855,4 +857,61 @@ extern "C" __global__ void kp_seed(const PChain *chains, int nchains, const int3
     const int ci = blockIdx.x * blockDim.x + threadIdx.x;
     if (ci >= nchains) return;
     sfmt_seed_state(chains[ci].rng, seeds[ci]);
+}
+
+
+// ------------------------------------------------------------------------------------------
+// The G-based updates of the on-site field (src/dqmc_config_u/dqmc_proj_update_delay.f90,
+// dqmc_proj_update_submatrix_G.f90): the finite-temperature delay / submatrix kernels run on
+// G = 1 - UR (UL UR)^-1 UL through the chain's view; then the rows of UR whose site flipped are
+// multiplied by bmat(new) / bmat(old) = 1 + delta_bmat(iflip, old) (urrecord of the reference).
+// ------------------------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(256)
+kp_u_save(const PChain *chains, int nt)
+{
+    const PChain &c = chains[blockIdx.z];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < c.n) c.uold[i] = c.uconf[(size_t)(nt - 1) * c.n + i];
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+kp_u_rescale(const PChain *chains, int nt)
+{
+    const PChain &c = chains[blockIdx.z];
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)c.n * c.ne) return;
+    const int i = (int)(e % c.n), lc = c.ulcomp;
+    const int old = c.uold[i], nw = c.uconf[(size_t)(nt - 1) * c.n + i];
+    if (old == nw) return;
+    const int iflip = ((nw - old) % lc + lc) % lc;           // nw = mod(old + iflip - 1, lc) + 1
+    const cplx d = c.ud[(iflip - 1) + (lc - 1) * (old - 1)];
+    c.UR[e] = cmul(c.UR[e], make_double2(1.0 + d.x, d.y));
+}
+
+// the bond field of family nf at slice nt before a G-based update, and UR's rows i1, i2 of every
+// bond it flipped times 1 + delta_bmat_p, 1 + delta_bmat_m of the old value
+// (src/dqmc_config_v/dqmc_proj_update_delay.f90:124-125; UR is in the family's rotated basis)
+extern "C" __global__ void __launch_bounds__(256)
+kp_v_save(const PChain *chains, int nt, int nf)
+{
+    const PChain &c = chains[blockIdx.z];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < c.nn_lf) c.uold[q] = c.conf[((size_t)(nt - 1) * c.nn_nf + nf) * c.n + q];
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+kp_v_rescale(const PChain *chains, int nt, int nf)
+{
+    const PChain &c = chains[blockIdx.z];
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)c.nn_lf * c.ne) return;
+    const int q = (int)(e % c.nn_lf), j = (int)(e / c.nn_lf);
+    const int old = c.uold[q], nw = c.conf[((size_t)(nt - 1) * c.nn_nf + nf) * c.n + q];
+    if (old == nw) return;
+    const int ist = nf * c.nn_lf + q;
+    const int i1 = c.bonds[2 * ist], i2 = c.bonds[2 * ist + 1];
+    const cplx dp = c.vd[2 * (old - 1)], dm = c.vd[2 * (old - 1) + 1];
+    cplx *a = c.UR + (size_t)j * c.n;
+    a[i1] = cmul(a[i1], make_double2(1.0 + dp.x, dp.y));
+    a[i2] = cmul(a[i2], make_double2(1.0 + dm.x, dm.y));
 }

@@ -19,6 +19,8 @@
 #include <string>
 #include <vector>
 
+#define SH_KMAX 64   // largest delay depth / submatrix block
+
 namespace {
 
 // timer classes: one per kernel group, in the order reported by sh_timers
@@ -333,8 +335,18 @@ static void vprop(sh_ctx *ctx, int sel, int nf, int mode, int side, int nt)
     const unsigned nb = (unsigned)(((size_t)ctx->nmax * ctx->nmax / 2 + 255) / 256);   // n nn_lf <= n^2 / 2
     hipLaunchKernelGGL(k_vprop, dim3(nb, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dc, sel, 0, nf, mode, side, nt);
 }
+// the bond update follows the build's algorithm as the on-site one does: the delay form
+// (dqmc_config_v/dqmc_update_delay.f90) with the delay depth, else the fast form
+// (dqmc_update.f90; the submatrix build's bond update takes the same decisions by the fast form)
 static void vupdate(sh_ctx *ctx, int nt, int nf, int slice, int p)
 {
+    int kmax = 0;
+    for (auto &c : ctx->hc) kmax = std::max(kmax, c.nublock);
+    if (ctx->update == SH_U_DELAY && kmax >= 2 && !ctx->legacy_update) {
+        hipLaunchKernelGGL(k_vupdate_delay, dim3(ctx->nch), dim3(256), (size_t)4 * SH_KMAX * sizeof(cplx), ctx->st,
+                           ctx->dc, nt, nf, slice, p);
+        return;
+    }
     hipLaunchKernelGGL(k_vupdate, dim3(ctx->nch), dim3(256), (size_t)4 * ctx->nmax * sizeof(cplx), ctx->st, ctx->dc,
                        nt, nf, slice, p);
 }
@@ -516,8 +528,6 @@ static void set_identity(sh_ctx *ctx, int pt)
     hipLaunchKernelGGL(k_set_identity, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, pt);
 }
 
-#define SH_KMAX 64   // largest delay depth / submatrix block
-
 // register-resident delay update: the template depth covering every chain's nublock, and the
 // thread count 2 * rows; 0 when the register budget does not allow it
 static int delay_reg_depth(sh_ctx *ctx, int *threads)
@@ -554,7 +564,7 @@ static void update_slice(sh_ctx *ctx, int nt, int slice_in_pass)
     } else {
         int K = 0;
         for (auto &c : ctx->hc) K = std::max(K, c.nublock);
-        const size_t lds = (size_t)(K * K + 4 * K) * sizeof(cplx);
+        const size_t lds = (size_t)(K * (K + 1) + 4 * K) * sizeof(cplx);
         hipLaunchKernelGGL(k_update_submatrix, dim3(ctx->nch), dim3(256), lds, ctx->st, ctx->dc, nt, slice_in_pass);
     }
 }
@@ -695,7 +705,7 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
         struct { const void *f; size_t bytes; } big[] = {
             {(const void *)k_cb_rows, lds_rows(ctx)}, {(const void *)k_cb_cols, lds_cols(ctx)},
             {(const void *)k_fft_rows, lds_fft(ctx)}, {(const void *)k_fft_cols, lds_fft(ctx)},
-            {(const void *)k_update_submatrix, (size_t)(SH_KMAX * SH_KMAX + 4 * SH_KMAX) * sizeof(cplx)},
+            {(const void *)k_update_submatrix, (size_t)(SH_KMAX * (SH_KMAX + 1) + 4 * SH_KMAX) * sizeof(cplx)},
             {(const void *)k_update_delay_reg<4>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 16 * sizeof(cplx)},
             {(const void *)k_update_delay_reg<8>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 32 * sizeof(cplx)},
             {(const void *)k_update_delay_reg<16>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 64 * sizeof(cplx)},

@@ -11,6 +11,7 @@ module sunhip_bridge
 
   integer(c_int), parameter :: SH_K_CHECKERBOARD = 0, SH_K_DENSE = 1, SH_K_FFT = 2
   integer(c_int), parameter :: SH_U_DELAY = 0, SH_U_SUBMATRIX = 1
+  integer(c_int), parameter :: SHP_U_LR = 0, SHP_U_FAST = 1, SHP_U_DELAY = 2, SHP_U_SUBMATRIX_G = 3
 
   ! mirrors struct sh_model of include/sunhip.h
   type, bind(C) :: sh_model
@@ -44,6 +45,8 @@ module sunhip_bridge
      type(c_ptr) :: ubmat, ubmat_inv, udelta, uratio, ugaml, uphase_arg
      integer(c_int) :: uphase_from_conf
      real(c_double) :: kin_weight
+     ! the on-site update: SHP_U_LR, SHP_U_FAST, SHP_U_DELAY, SHP_U_SUBMATRIX_G, and its depth
+     integer(c_int) :: update, gblock
   end type shp_model
 
   interface

@@ -41,11 +41,20 @@ class ProjModel:
     seed: int = 1909706392
     rhub: float = 0.0          # on-site U (the uconf field when != 0)
     exact_hs: bool = True      # EXACTHS tables of the on-site field
+    update: str = "submatrix_lr"  # update of both fields: submatrix_lr, fast, delay, submatrix_g
+    gblock: int = 0            # depth of the G-based forms (0: the reference's nublock default)
+
+
+PUPDATES = {"submatrix_lr": 0, "fast": 1, "delay": 2, "submatrix_g": 3}
 
 
 def make_proj_tables(m):
     if KMODES.get(m.kmode) not in (0, 1):
         raise ValueError("the projective sweep applies the kinetic term as 'checkerboard' or 'dense'")
+    if m.update not in PUPDATES:
+        raise ValueError("update must be one of %s" % sorted(PUPDATES))
+    if PUPDATES[m.update] >= 2 and (m.rhub != 0.0) == (m.rv != 0.0):
+        raise ValueError("the G-based updates (delay, submatrix_g) need one field alone (rhub or rv)")
     lat = lattice_tables(m)
     n = lat["nsites"]
     ltrot = int(round(m.beta / m.dtau))
@@ -60,7 +69,8 @@ def make_proj_tables(m):
     return {"model": m, "lat": lat, "ndim": n, "ne": n // 2 + int(m.nu * n / 8.0), "lq": lat["lq"],
             "ltrot": ltrot, "nst": nst, "nwrap": nw, "iwrap_nt": iw, "wrap_step": ws, "urt": urt,
             "urtm1": urtm1, "v": vconf_tables(m) if m.rv != 0.0 else None, "u": u, "nustock": max(ns, 2),
-            "kmode": KMODES[m.kmode]}
+            "kmode": KMODES[m.kmode], "update": PUPDATES[m.update],
+            "gblock": min(64, m.gblock if m.gblock > 0 else default_nublock(n, "delay" if m.update == "delay" else "submatrix"))}
 
 
 def trial_hamiltonian(m, lat, u):
@@ -90,7 +100,7 @@ class ShpModel(ctypes.Structure):
         ("wrap_check", ctypes.c_int), ("seed", ctypes.c_int32),
         ("ulcomp", ctypes.c_int), ("ubmat", _lib.dp), ("ubmat_inv", _lib.dp), ("udelta", _lib.dp),
         ("uratio", _lib.dp), ("ugaml", _lib.dp), ("uphase_arg", _lib.dp), ("uphase_from_conf", ctypes.c_int),
-        ("kin_weight", ctypes.c_double),
+        ("kin_weight", ctypes.c_double), ("update", ctypes.c_int), ("gblock", ctypes.c_int),
     ]
 
 
@@ -150,6 +160,7 @@ class ProjWalkers:
         s.lq, s.nflr, s.nustock, s.mid_len = t["lq"], m.nflr, t["nustock"], m.mid_len
         s.wrap_check, s.seed = int(bool(m.wrap_check)), int(np.int32(m.seed))
         s.kin_weight = lat["kin_weight"]
+        s.update, s.gblock = t["update"], t["gblock"]
         return s
 
     def close(self):

@@ -1,7 +1,8 @@
-"""GPU parity of the finite-temperature t-V bond field (k_vprop, k_vupdate and the B products
-with the field) with the oracle, through the C ABI: the on-site and bond fields are identical
-after every pass, G agrees to 1e-10 at every stabilization, and the phase, the acceptance sums
-and the observable bins agree."""
+"""GPU parity of the finite-temperature t-V bond field (k_vprop, the B products with the field,
+and the bond update: the delay form k_vupdate_delay in delay builds, the fast form k_vupdate
+otherwise) with the oracle's fast form, through the C ABI: the two algorithms take the same
+decisions, so the on-site and bond fields are identical after every pass, G agrees to 1e-10 at
+every stabilization, and the phase, the acceptance sums and the observable bins agree."""
 import sys
 
 import numpy as np
@@ -21,6 +22,12 @@ CASES = {
                                  kmode="checkerboard", nwrap=5, update="delay", nublock=4),
     "square4_tuv_submatrix_dense": dict(lattice="square", la=4, lb=4, beta=0.8, rhub=2.0, rv=1.0, nflr=2,
                                         kmode="dense", nwrap=4, update="submatrix", nublock=4),
+    "square6_tv_delay16": dict(lattice="square", la=6, lb=6, beta=1.0, rhub=0.0, rv=1.5, nflr=1,
+                               kmode="checkerboard", nwrap=5, update="delay", nublock=16),
+    "honeycomb4_tuv_delay32_dense": dict(lattice="honeycomb", la=4, lb=4, beta=0.6, rhub=3.0, rv=1.2, nflr=2,
+                                         kmode="dense", nwrap=3, update="delay", nublock=32),
+    "chain16_tv_delay6": dict(lattice="chain", la=16, lb=1, beta=1.0, rhub=0.0, rv=2.0, nflr=1,
+                              kmode="checkerboard", nwrap=5, update="delay", nublock=6),
 }
 
 

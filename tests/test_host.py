@@ -218,6 +218,24 @@ def test_proj_create_rejects_bad_arguments():
     assert _lib.lib().shp_last_error(None) == b"null context"
 
 
+def test_proj_onsite_update_variants_are_checked():
+    """The G-based on-site updates need the on-site field without a bond field and a depth
+    1..64; an unknown algorithm is refused; all before any device call."""
+    from sunhubbardmy_amd.proj import PUPDATES
+    hub = make_proj_tables(ProjModel(lattice="square", la=4, lb=4, beta=1.0, rv=0.0, rhub=4.0, nflr=2,
+                                     update="delay", gblock=8))
+    assert hub["update"] == PUPDATES["delay"] and hub["gblock"] == 8
+    assert _proj_create([hub], gblock=0) == -1
+    assert _proj_create([hub], gblock=65) == -1
+    assert _proj_create([hub], update=4) == -1
+    tuv = make_proj_tables(ProjModel(lattice="square", la=4, lb=4, beta=1.0, rv=1.0, rhub=4.0, nflr=2))
+    assert _proj_create([tuv], update=2, gblock=8) == -1      # bond field present
+    with pytest.raises(ValueError):
+        make_proj_tables(ProjModel(lattice="square", la=4, lb=4, rv=1.0, rhub=4.0, update="submatrix_g"))
+    with pytest.raises(ValueError):
+        make_proj_tables(ProjModel(update="unknown"))
+
+
 # ---- the DFT form of the kinetic term ---------------------------------------------------------
 @pytest.mark.parametrize("kw", [dict(lattice="square", la=4, lb=6), dict(lattice="square", la=12, lb=12, mu=0.3),
                                 dict(lattice="chain", la=10, lb=1), dict(lattice="cubic", la=4, lb=6, lc=4)])
