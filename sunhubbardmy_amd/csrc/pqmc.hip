@@ -133,10 +133,14 @@ void update_family_g(shp_ctx *ctx, int nt, int nf, size_t uoff0)
     hipLaunchKernelGGL(kp_v_save, dim3((nl + 255) / 256, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dc, nt, nf);
     const int slice = (int)(uoff0 / ctx->slice_draws);
     const int p = (int)((uoff0 % ctx->slice_draws) / (2 * (size_t)nl));
-    if (ctx->update == SHP_U_DELAY)
+    if (ctx->update == SHP_U_DELAY) {
         hipLaunchKernelGGL(k_vupdate_delay, dim3(ctx->nch), dim3(256), (size_t)4 * 64 * sizeof(cplx), ctx->st,
                            ctx->dview, nt, nf, slice, p);
-    else
+    } else if (ctx->update == SHP_U_SUBMATRIX_G) {
+        const int K = std::max(2, ctx->gblock & ~1);
+        hipLaunchKernelGGL(k_vupdate_submatrix, dim3(ctx->nch), dim3(256), (size_t)(K * (K + 1) + 8 * K) * sizeof(cplx),
+                           ctx->st, ctx->dview, nt, nf, slice, p);
+    } else
         hipLaunchKernelGGL(k_vupdate, dim3(ctx->nch), dim3(256), (size_t)4 * ctx->n * sizeof(cplx), ctx->st,
                            ctx->dview, nt, nf, slice, p);
     hipLaunchKernelGGL(kp_v_rescale, dim3((unsigned)(((size_t)nl * ctx->ne + 255) / 256), 1, ctx->nch), dim3(256), 0,
@@ -566,10 +570,12 @@ int shp_create(shp_ctx **out, int nchains, const shp_model *m, int device)
                 for (int k = 0; k < 4; k++) v.vdl[k] = c.vd[k];
                 v.vacc = c.sc + SC_ACC;
             }
-            v.avec = dalloc<cplx>(ctx, (size_t)n * K);
-            v.bvec = dalloc<cplx>(ctx, (size_t)n * K);
-            v.W1 = dalloc<cplx>(ctx, (size_t)n * K);
-            v.W2 = dalloc<cplx>(ctx, (size_t)K * K);
+            // (the bond field's submatrix form borders by 2 x 2 blocks: at least two columns)
+            const int Ka = std::max(K, 2);
+            v.avec = dalloc<cplx>(ctx, (size_t)n * Ka);
+            v.bvec = dalloc<cplx>(ctx, (size_t)n * Ka);
+            v.W1 = dalloc<cplx>(ctx, (size_t)n * Ka);
+            v.W2 = dalloc<cplx>(ctx, (size_t)Ka * Ka);
             if (!c.uold || !v.avec || !v.bvec || !v.W1 || !v.W2) {
                 int rc = fail(ctx, SH_E_HIP, "device allocation failed");
                 shp_destroy(ctx);
@@ -578,10 +584,15 @@ int shp_create(shp_ctx **out, int nchains, const shp_model *m, int device)
         }
     }
     if (ctx->update == SHP_U_SUBMATRIX_G) {
+        const int Kb = std::max(2, ctx->gblock & ~1);
         const size_t lds = (size_t)(ctx->gblock * (ctx->gblock + 1) + 4 * ctx->gblock) * sizeof(cplx);
-        if (lds > 64 * 1024 &&
-            hipFuncSetAttribute((const void *)k_update_submatrix, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-                hipSuccess) {
+        const size_t ldsv = (size_t)(Kb * (Kb + 1) + 8 * Kb) * sizeof(cplx);
+        if ((lds > 64 * 1024 &&
+             hipFuncSetAttribute((const void *)k_update_submatrix, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+                 hipSuccess) ||
+            (ldsv > 64 * 1024 &&
+             hipFuncSetAttribute((const void *)k_vupdate_submatrix, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsv) !=
+                 hipSuccess)) {
             (void)hipGetLastError();
             int rc = fail(ctx, SH_E_HIP, "dynamic LDS request above the device limit");
             shp_destroy(ctx);

@@ -19,8 +19,6 @@
 #include <string>
 #include <vector>
 
-#define SH_KMAX 64   // largest delay depth / submatrix block
-
 namespace {
 
 // timer classes: one per kernel group, in the order reported by sh_timers
@@ -336,8 +334,8 @@ static void vprop(sh_ctx *ctx, int sel, int nf, int mode, int side, int nt)
     hipLaunchKernelGGL(k_vprop, dim3(nb, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dc, sel, 0, nf, mode, side, nt);
 }
 // the bond update follows the build's algorithm as the on-site one does: the delay form
-// (dqmc_config_v/dqmc_update_delay.f90) with the delay depth, else the fast form
-// (dqmc_update.f90; the submatrix build's bond update takes the same decisions by the fast form)
+// (dqmc_config_v/dqmc_update_delay.f90) or the submatrix form (dqmc_update_submatrix.f90) with
+// the build's depth, else the fast form (dqmc_update.f90)
 static void vupdate(sh_ctx *ctx, int nt, int nf, int slice, int p)
 {
     int kmax = 0;
@@ -345,6 +343,12 @@ static void vupdate(sh_ctx *ctx, int nt, int nf, int slice, int p)
     if (ctx->update == SH_U_DELAY && kmax >= 2 && !ctx->legacy_update) {
         hipLaunchKernelGGL(k_vupdate_delay, dim3(ctx->nch), dim3(256), (size_t)4 * SH_KMAX * sizeof(cplx), ctx->st,
                            ctx->dc, nt, nf, slice, p);
+        return;
+    }
+    if (ctx->update == SH_U_SUBMATRIX && !ctx->legacy_update) {
+        const int K = std::max(2, std::min(kmax, SH_KMAX) & ~1);
+        hipLaunchKernelGGL(k_vupdate_submatrix, dim3(ctx->nch), dim3(256), (size_t)(K * (K + 1) + 8 * K) * sizeof(cplx),
+                           ctx->st, ctx->dc, nt, nf, slice, p);
         return;
     }
     hipLaunchKernelGGL(k_vupdate, dim3(ctx->nch), dim3(256), (size_t)4 * ctx->nmax * sizeof(cplx), ctx->st, ctx->dc,
@@ -706,6 +710,7 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
             {(const void *)k_cb_rows, lds_rows(ctx)}, {(const void *)k_cb_cols, lds_cols(ctx)},
             {(const void *)k_fft_rows, lds_fft(ctx)}, {(const void *)k_fft_cols, lds_fft(ctx)},
             {(const void *)k_update_submatrix, (size_t)(SH_KMAX * (SH_KMAX + 1) + 4 * SH_KMAX) * sizeof(cplx)},
+            {(const void *)k_vupdate_submatrix, (size_t)(SH_KMAX * (SH_KMAX + 1) + 8 * SH_KMAX) * sizeof(cplx)},
             {(const void *)k_update_delay_reg<4>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 16 * sizeof(cplx)},
             {(const void *)k_update_delay_reg<8>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 32 * sizeof(cplx)},
             {(const void *)k_update_delay_reg<16>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 64 * sizeof(cplx)},

@@ -6,6 +6,8 @@
 typedef double2 cplx;                                   // (re, im), the complex(dp) layout
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
+#define SH_KMAX 64   // largest delay depth / submatrix block
+
 // Global-memory view of a device pointer.  The chain descriptors hold generic pointers, whose
 // accesses compile to FLAT instructions; the result of a FLAT load can only be awaited together
 // with every other outstanding memory and LDS access, so loads issued ahead of use stall the

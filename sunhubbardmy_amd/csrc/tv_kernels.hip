@@ -221,3 +221,187 @@ k_vupdate_delay(const DChain *chains, int nt, int nf, int slice, int p)
         c.vacc[1] += (double)nl;
     }
 }
+
+// The submatrix form of the same update (src/dqmc_config_v/dqmc_update_submatrix.f90:63-250):
+// G stays fixed within a block; the 2x2 block of the current G at a bond (i1, i2) is
+// G_bb + v with v = G(b, P) Gamma^-1 G(P, b) over the accepted sites P (pairs, in order).  An
+// accepted bond borders Gamma^-1 by a 2x2 block: v3 = v5^-1 with v5 = diag(1 + 1/dp, 1 + 1/dm)
+// - G_bb - v, the new rows v3 v1 (v1 = G(b, P) Gamma^-1), the new columns v4 v3 (v4 = Gamma^-1
+// G(P, b)), the old block + v4 v3 v1; the columns i1, i2 of the updated G take the factors
+// 1/(1 + dp), 1/(1 + dm).  Every K entries (K even) and after the family's last bond the block
+// is flushed: Gmat1 = G(:, P) Gamma^-1 as an MFMA product, G <- (G + Gmat1 G(P, :)) diag(scale).
+// (The reference's v3(2,1) at :153 is -v5(2,1) v5(2,2) / (v5(1,1) det v5), not the inverse's
+// -v5(2,1) / det v5; this kernel uses the inverse, with which the submatrix, delay and fast
+// forms take the same decisions.)  Draws, decisions, phase and field bookkeeping as k_vupdate.
+// One workgroup of 256 threads per chain; LDS: Gamma^-1 (leading dimension K + 1), G(b, P),
+// G(P, b), v1, v4 (2 K each).
+extern "C" __global__ void __launch_bounds__(256)
+k_vupdate_submatrix(const DChain *chains, int nt, int nf, int slice, int p)
+{
+    extern __shared__ cplx vsh[];
+    const DChain &c = chains[blockIdx.x];
+    if (!c.lwrapv) return;
+    const int n = c.n, nl = c.nn_lf, tid = threadIdx.x, lane = tid & 63;
+    const int K = max(2, min(c.nublock, SH_KMAX) & ~1), KL = K + 1;
+    cplx *gi = vsh;                     // Gamma^-1, column-major, leading dimension K + 1
+    cplx *cv = gi + (size_t)KL * K;     // G(i1, P), G(i2, P): 2 x K, entry (r, m) at r + 2 m
+    cplx *bv = cv + 2 * K;              // G(P, i1), G(P, i2): K x 2, entry (m, c) at m + K c
+    cplx *v1 = bv + 2 * K;              // G(b, P) Gamma^-1: 2 x K, (r, m) at r + 2 m
+    cplx *v4 = v1 + 2 * K;              // Gamma^-1 G(P, b): K x 2, (m, c) at m + K c
+    __shared__ int pvec[SH_KMAX];
+    __shared__ cplx s_gb[8], s_v[4];   // s_gb: G_bb by the parity of the bond
+    GAS cplx *G = gp(c.G);
+    cplx *gm1 = c.avec, *gmat1 = c.bvec, *gm2t = c.W1;
+    double *csr = c.rwork, *csi = c.rwork + n;
+    const double *u = c.unifv + (size_t)2 * nl * (c.nn_nf * slice + p);
+    int32_t *cf = c.confv + ((size_t)(nt - 1) * c.nn_nf + nf) * n;
+    cplx phase = make_double2(c.sc[SC_PHASE], c.sc[SC_PHASE + 1]);
+    for (int r = tid; r < n; r += blockDim.x) { csr[r] = 1.0; csi[r] = 0.0; }
+    __syncthreads();
+    double nacc = 0.0;
+    int ik = 0;
+    for (int q = 0; q < nl; q++) {
+        const int ist = nf * nl + q;
+        const int i1 = c.bonds[2 * ist], i2 = c.bonds[2 * ist + 1];
+        for (int m = tid; m < ik; m += blockDim.x) {
+            const size_t pm = (size_t)pvec[m];
+            cv[2 * m] = gld(G + pm * n + i1);
+            cv[2 * m + 1] = gld(G + pm * n + i2);
+            bv[m] = gld(G + (size_t)i1 * n + pm);
+            bv[m + K] = gld(G + (size_t)i2 * n + pm);
+        }
+        // G_bb goes to the buffer of this bond's parity: after a rejected bond, a thread may still
+        // read the previous bond's values while this one is written
+        cplx *s_g = s_gb + 4 * (q & 1);
+        if (tid == blockDim.x - 1) {
+            s_g[0] = gld(G + (size_t)i1 * n + i1);
+            s_g[1] = gld(G + (size_t)i2 * n + i1);
+            s_g[2] = gld(G + (size_t)i1 * n + i2);
+            s_g[3] = gld(G + (size_t)i2 * n + i2);
+        }
+        __syncthreads();
+        // v1 = G(b, P) Gamma^-1 (lane i: column i) and v = v1 G(P, b), summed by wave 0 (ik <= 64)
+        if (tid < 64) {
+            cplx t11 = make_double2(0.0, 0.0), t12 = t11, t21 = t11, t22 = t11;
+            if (lane < ik) {
+                cplx a1 = make_double2(0.0, 0.0), a2 = a1;
+                for (int m = 0; m < ik; m++) {
+                    const cplx g = gi[m + lane * KL];
+                    a1 = cadd(a1, cmul(cv[2 * m], g));
+                    a2 = cadd(a2, cmul(cv[2 * m + 1], g));
+                }
+                v1[2 * lane] = a1;
+                v1[2 * lane + 1] = a2;
+                t11 = cmul(a1, bv[lane]);
+                t12 = cmul(a1, bv[lane + K]);
+                t21 = cmul(a2, bv[lane]);
+                t22 = cmul(a2, bv[lane + K]);
+            }
+            t11.x = wave_sum(t11.x); t11.y = wave_sum(t11.y);
+            t12.x = wave_sum(t12.x); t12.y = wave_sum(t12.y);
+            t21.x = wave_sum(t21.x); t21.y = wave_sum(t21.y);
+            t22.x = wave_sum(t22.x); t22.y = wave_sum(t22.y);
+            if (lane == 0) {
+                s_v[0] = t11; s_v[1] = t12; s_v[2] = t21; s_v[3] = t22;
+            }
+        }
+        __syncthreads();
+        const int is = cf[q] - 1;
+        const int iflip = (int)ceil(u[2 * q]);
+        const int isp = (is + iflip) & 1;
+        const cplx dp = c.vdl[2 * is], dm = c.vdl[2 * is + 1];
+        // the 2x2 block of the current G
+        const cplx g11 = cadd(s_g[0], s_v[0]), g12 = cadd(s_g[1], s_v[1]);
+        const cplx g21 = cadd(s_g[2], s_v[2]), g22 = cadd(s_g[3], s_v[3]);
+        const cplx m11 = cadd(make_double2(1.0, 0.0), cmul(make_double2(1.0 - g11.x, -g11.y), dp));
+        const cplx m12 = cmul(make_double2(-g12.x, -g12.y), dm);
+        const cplx m21 = cmul(make_double2(-g21.x, -g21.y), dp);
+        const cplx m22 = cadd(make_double2(1.0, 0.0), cmul(make_double2(1.0 - g22.x, -g22.y), dm));
+        const cplx ratio = csub(cmul(m11, m22), cmul(m12, m21));
+        const double rr = cmul(ratio, phase).x / phase.x;
+        if (fabs(rr) > u[2 * q + 1]) {
+            nacc += 1.0;
+            phase = cmul(phase, cscale(ratio, 1.0 / hypot(ratio.x, ratio.y)));
+            // v3 = v5^-1
+            const cplx w11 = csub(cadd(make_double2(1.0, 0.0), cinv(dp)), g11);
+            const cplx w12 = make_double2(-g12.x, -g12.y), w21 = make_double2(-g21.x, -g21.y);
+            const cplx w22 = csub(cadd(make_double2(1.0, 0.0), cinv(dm)), g22);
+            const cplx idet = cinv(csub(cmul(w11, w22), cmul(w12, w21)));
+            const cplx v3_11 = cmul(w22, idet), v3_12 = cmul(make_double2(-w12.x, -w12.y), idet);
+            const cplx v3_21 = cmul(make_double2(-w21.x, -w21.y), idet), v3_22 = cmul(w11, idet);
+            for (int i = tid; i < ik; i += blockDim.x) {
+                cplx a1 = make_double2(0.0, 0.0), a2 = a1;
+                for (int m = 0; m < ik; m++) {
+                    const cplx g = gi[i + m * KL];
+                    a1 = cadd(a1, cmul(g, bv[m]));
+                    a2 = cadd(a2, cmul(g, bv[m + K]));
+                }
+                v4[i] = a1;
+                v4[i + K] = a2;
+            }
+            __syncthreads();
+            // old block += v4 (v3 v1); new rows v3 v1; new columns v4 v3; corner v3
+            for (int e = tid; e < ik * ik; e += blockDim.x) {
+                const int i = e % ik, m = e / ik;
+                const cplx n1 = cadd(cmul(v3_11, v1[2 * m]), cmul(v3_12, v1[2 * m + 1]));
+                const cplx n2 = cadd(cmul(v3_21, v1[2 * m]), cmul(v3_22, v1[2 * m + 1]));
+                gi[i + m * KL] = cadd(gi[i + m * KL], cadd(cmul(v4[i], n1), cmul(v4[i + K], n2)));
+            }
+            for (int i = tid; i < ik; i += blockDim.x) {
+                gi[ik + i * KL] = cadd(cmul(v3_11, v1[2 * i]), cmul(v3_12, v1[2 * i + 1]));
+                gi[ik + 1 + i * KL] = cadd(cmul(v3_21, v1[2 * i]), cmul(v3_22, v1[2 * i + 1]));
+                gi[i + ik * KL] = cadd(cmul(v4[i], v3_11), cmul(v4[i + K], v3_21));
+                gi[i + (ik + 1) * KL] = cadd(cmul(v4[i], v3_12), cmul(v4[i + K], v3_22));
+            }
+            if (tid == 0) {
+                gi[ik + ik * KL] = v3_11;
+                gi[ik + 1 + ik * KL] = v3_21;
+                gi[ik + (ik + 1) * KL] = v3_12;
+                gi[ik + 1 + (ik + 1) * KL] = v3_22;
+                pvec[ik] = i1;
+                pvec[ik + 1] = i2;
+                cf[q] = isp + 1;
+                const cplx f1 = cinv(cadd(make_double2(1.0, 0.0), dp)), f2 = cinv(cadd(make_double2(1.0, 0.0), dm));
+                const cplx s1 = cmul(make_double2(csr[i1], csi[i1]), f1), s2 = cmul(make_double2(csr[i2], csi[i2]), f2);
+                csr[i1] = s1.x; csi[i1] = s1.y;
+                csr[i2] = s2.x; csi[i2] = s2.y;
+            }
+            for (int r = tid; r < n; r += blockDim.x) {
+                gm1[(size_t)r + (size_t)ik * n] = gld(G + (size_t)i1 * n + r);
+                gm1[(size_t)r + (size_t)(ik + 1) * n] = gld(G + (size_t)i2 * n + r);
+            }
+            ik += 2;
+            __threadfence_block();
+            __syncthreads();
+        }
+        if (ik > 0 && (ik >= K || q == nl - 1)) {
+            // Gmat1 = G(:, P) Gamma^-1 (Gamma^-1 staged through W2 at leading dimension ik: the
+            // bonds of a family are disjoint, so ik <= n); gm2t(j, i) = G(p_i, j)
+            cplx *gig = c.W2;
+            for (int e = tid; e < ik * ik; e += blockDim.x) {
+                const int i = e % ik, j = e / ik;
+                gig[i + (size_t)j * ik] = gi[i + j * KL];
+            }
+            for (int r = tid; r < n; r += blockDim.x)
+                for (int i = 0; i < ik; i++) gm2t[(size_t)r + (size_t)i * n] = gld(G + (size_t)r * n + pvec[i]);
+            __threadfence_block();
+            __syncthreads();
+            for (int tm = 0; tm < n; tm += GT)
+                zgemm_tile<0, 0>(gm1, n, gig, ik, gmat1, n, n, ik, ik, tm, 0, 1.0, 0.0, 0.0, 0.0);
+            __threadfence_block();
+            __syncthreads();
+            wg_rank_k(c.G, n, gmat1, gm2t, ik, csr, csi);
+            __threadfence_block();
+            __syncthreads();
+            for (int r = tid; r < n; r += blockDim.x) { csr[r] = 1.0; csi[r] = 0.0; }
+            ik = 0;
+            __syncthreads();
+        }
+    }
+    if (tid == 0) {
+        c.sc[SC_PHASE] = phase.x;
+        c.sc[SC_PHASE + 1] = phase.y;
+        c.vacc[0] += nacc;
+        c.vacc[1] += (double)nl;
+    }
+}

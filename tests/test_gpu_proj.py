@@ -96,7 +96,7 @@ def run_parity(kw, nsweeps=2, chain=1, gpu_kw=None):
 
 
 @pytest.mark.parametrize("variant,gblock", [("fast", 0), ("delay", 4), ("delay", 32), ("submatrix_g", 8),
-                                             ("submatrix_g", 64)])
+                                             ("submatrix_g", 64), ("submatrix_g", 1), ("submatrix_g", 5)])
 @pytest.mark.parametrize("name", ["square4_hubbard_cb", "square4_su4_dense", "honeycomb2_dense",
                                   "square4_checkerboard_nwrap3"])
 def test_proj_update_variants(name, variant, gblock):

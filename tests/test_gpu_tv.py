@@ -1,6 +1,7 @@
 """GPU parity of the finite-temperature t-V bond field (k_vprop, the B products with the field,
-and the bond update: the delay form k_vupdate_delay in delay builds, the fast form k_vupdate
-otherwise) with the oracle's fast form, through the C ABI: the two algorithms take the same
+and the bond update: the delay form k_vupdate_delay in delay builds, the submatrix form
+k_vupdate_submatrix in submatrix builds, the fast form k_vupdate otherwise) with the oracle's fast
+form, through the C ABI: the two algorithms take the same
 decisions, so the on-site and bond fields are identical after every pass, G agrees to 1e-10 at
 every stabilization, and the phase, the acceptance sums and the observable bins agree."""
 import sys
@@ -28,6 +29,16 @@ CASES = {
                                          kmode="dense", nwrap=3, update="delay", nublock=32),
     "chain16_tv_delay6": dict(lattice="chain", la=16, lb=1, beta=1.0, rhub=0.0, rv=2.0, nflr=1,
                               kmode="checkerboard", nwrap=5, update="delay", nublock=6),
+    # the submatrix form k_vupdate_submatrix: 2x2 bordering of Gamma^-1, flush every K entries
+    # (K = nublock rounded down to even) and after each family's last bond
+    "square6_tv_submatrix16": dict(lattice="square", la=6, lb=6, beta=1.0, rhub=0.0, rv=1.5, nflr=1,
+                                   kmode="checkerboard", nwrap=5, update="submatrix", nublock=16),
+    "honeycomb4_tuv_submatrix32_dense": dict(lattice="honeycomb", la=4, lb=4, beta=0.6, rhub=3.0, rv=1.2, nflr=2,
+                                             kmode="dense", nwrap=3, update="submatrix", nublock=32),
+    "chain16_tv_submatrix5": dict(lattice="chain", la=16, lb=1, beta=1.0, rhub=0.0, rv=2.0, nflr=1,
+                                  kmode="checkerboard", nwrap=5, update="submatrix", nublock=5),
+    "square8_tv_submatrix64": dict(lattice="square", la=8, lb=8, beta=0.5, rhub=0.0, rv=1.5, nflr=1,
+                                   kmode="checkerboard", nwrap=5, update="submatrix", nublock=64),
 }
 
 
