@@ -78,6 +78,17 @@ typedef struct sh_model {
      * DFTI dimensions dimm); the wave-vector index of exph0k is ordered the same way.  Ignored
      * for the other kinetic forms. */
     int fft_dims[3];
+    /* the plaquette field (hconf with rhub_plq != 0 and lprojplqu = .false.; zplq = 0: none):
+     * zplq sites per plaquette (4 square, 6 honeycomb), lqp plaquettes in the reference's order,
+     * plq_cord [lqp][zplq] their 0-based sites (latt%plq_cord, src/lattice.f90:735-773, 898-958);
+     * per component is = 1..4 the zplq x zplq complex column-major factors bmat_plqu_blk1(:,:,is)
+     * (plq_b) and bmat_plqu_blk1_inv(:,:,is) (plq_bi), per (is, iflip) delta_bmat_plqu_blk1(:,:,
+     * iflip, is) at block (is-1)*3 + iflip-1 (plq_delta); phase_ratio(iflip, is) complex at
+     * (iflip-1) + 3 (is-1); plq_parg[4] the arg of hconf%phase(is); plq_gaml[4]
+     * (src/dqmc_config_plqu/dqmc_set_plqconf.f90:57-199, dqmc_config_plqu.f90:42-51) */
+    int zplq, lqp;
+    const int32_t *plq_cord;
+    const double *plq_b, *plq_bi, *plq_delta, *plq_pratio, *plq_parg, *plq_gaml;
 } sh_model;
 
 typedef struct sh_ctx sh_ctx;
@@ -136,6 +147,13 @@ int sh_get_eqt(sh_ctx *ctx, int chain, double *bins);
 int sh_set_confv(sh_ctx *ctx, int chain, const int32_t *conf);
 int sh_get_confv(sh_ctx *ctx, int chain, int32_t *conf);
 int sh_get_vacc(sh_ctx *ctx, int chain, double *acc);
+/* The plaquette field (hconf%conf_plqu(1:lqp, 1:ltrot), values 1..4) as int32 [ltrot][lqp], and
+ * its acceptance sums (accepted, tried): main_obs(2) of dqmc_update_plqu.f90:85.  The field is
+ * drawn first by sh_init_conf (hconf%input_plqconf before u0conf%input_conf, src/dqmc_ft_ctrl.f90:
+ * 19-35), ceiling(u*4), nt outer, plaquette inner (dqmc_input_plqconf.f90:55-62). */
+int sh_set_confp(sh_ctx *ctx, int chain, const int32_t *conf);
+int sh_get_confp(sh_ctx *ctx, int chain, int32_t *conf);
+int sh_get_pacc(sh_ctx *ctx, int chain, double *acc);
 
 /* The chain's random-number state, SH_RNG_WORDS uint32: the SFMT19937 state that spring_sfmt
  * keeps in module storage (lib/m_spring.f90:66-71: 624 words, then the read index 0..624) and,

@@ -42,6 +42,11 @@ def lib():
         L.orc_set_confv.argtypes = [P, ip]
         L.orc_get_vacc.argtypes = [P, dp]
         L.orc_get_vacc.restype = None
+        L.orc_get_confp.argtypes = [P, ip]
+        L.orc_set_confp.argtypes = [P, ip]
+        L.orc_get_pacc.argtypes = [P, dp]
+        L.orc_get_pacc.restype = None
+        L.orc_lqp.argtypes = [P]
         L.orc_sweep_start.argtypes = [P]
         L.orc_sweep_b0.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.orc_sweep_0b.argtypes = [P, ctypes.c_int, ctypes.c_int]
@@ -121,6 +126,7 @@ class OrcParams(ctypes.Structure):
         ("nflr", ctypes.c_int), ("exact_hs", ctypes.c_int), ("kmode", ctypes.c_int),
         ("update", ctypes.c_int), ("nwrap", ctypes.c_int), ("nublock", ctypes.c_int),
         ("seed", ctypes.c_int32), ("lc", ctypes.c_int), ("rv", ctypes.c_double),
+        ("rhub_plq", ctypes.c_double), ("alpha", ctypes.c_double), ("theta", ctypes.c_double),
     ]
 
 
@@ -154,13 +160,16 @@ class OracleChain:
 
     def __init__(self, lattice="square", la=4, lb=4, beta=4.0, dtau=0.1, rt=1.0, mu=0.0,
                  rhub=4.0, nu=0.0, nflr=2, exact_hs=True, kmode="checkerboard",
-                 update="delay", nwrap=10, nublock=8, seed=None, lc=1, rv=0.0):
+                 update="delay", nwrap=10, nublock=8, seed=None, lc=1, rv=0.0, rhub_plq=0.0, alpha=0.2,
+                 theta=0.0):
         L = lib()
         if seed is None:
             seed = L.orc_chain_seed(0)
+        if rhub_plq != 0.0 and lattice not in ("square", "honeycomb"):
+            raise ValueError("the plaquette field needs the square or the honeycomb lattice")
         self.params = OrcParams(LATTICES[lattice], la, lb, beta, dtau, rt, mu, rhub, nu, nflr,
                                 int(bool(exact_hs)), KMODES[kmode], UPDATES[update], nwrap,
-                                nublock, int(np.int32(seed)), lc, rv)
+                                nublock, int(np.int32(seed)), lc, rv, rhub_plq, alpha, theta)
         self._c = L.orc_create(ctypes.byref(self.params))
         s = np.zeros(8, np.int32)
         L.orc_sizes(self._c, _iptr(s))
@@ -202,6 +211,23 @@ class OracleChain:
     def vacc(self):
         a = np.zeros(2)
         lib().orc_get_vacc(self._c, _dptr(a))
+        return a
+
+    @property
+    def confp(self):
+        """plaquette field [ltrot][lqp], values 1..4"""
+        c = np.zeros((self.ltrot, lib().orc_lqp(self._c)), np.int32)
+        lib().orc_get_confp(self._c, _iptr(c))
+        return c
+
+    @confp.setter
+    def confp(self, value):
+        c = np.ascontiguousarray(value, dtype=np.int32).reshape(self.ltrot, lib().orc_lqp(self._c))
+        lib().orc_set_confp(self._c, _iptr(c))
+
+    def pacc(self):
+        a = np.zeros(2)
+        lib().orc_get_pacc(self._c, _dptr(a))
         return a
 
     def sweep_start(self):

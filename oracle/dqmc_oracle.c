@@ -473,6 +473,13 @@ struct orc_ctx {
     zc vbp[2], vbm[2], vbpi[2], vbmi[2], vdp[2], vdm[2];
     int32_t *confv;             /* [ltrot][nn_nf][ndim] */
     double vacc[2];
+    /* the plaquette field (hconf with lprojplqu = .false.) */
+    int lwrapp, zp, lqp;
+    int *pcord;                 /* [lqp][zp] */
+    zc pb[4][36], pbi[4][36], pdl[4][3][36], ppr[4][3], pph[4];
+    double pgaml[4];
+    int32_t *confp;             /* [ltrot][lqp] */
+    double pacc[2];
 };
 
 /* src/lattice.f90:792-961 (square), :611-776 (honeycomb), :240-316 (chain), :1278-1400 (cubic);
@@ -768,6 +775,7 @@ static void setup_kinetic(orc_ctx *c)
 }
 
 static void setup_vfield(orc_ctx *c);
+static void setup_pfield(orc_ctx *c);
 
 orc_ctx *orc_create(const orc_params *p)
 {
@@ -791,6 +799,12 @@ orc_ctx *orc_create(const orc_params *p)
         for (size_t k = 0; k < (size_t)N * c->nn_nf * c->ltrot; k++) c->confv[k] = 1;
     }
     for (size_t k = 0; k < (size_t)N * c->ltrot; k++) c->conf[k] = 1;
+    c->lwrapp = (p->rhub_plq != 0.0);
+    if (c->lwrapp) {
+        setup_pfield(c);
+        c->confp = (int32_t *)malloc(sizeof(int32_t) * (size_t)c->lqp * c->ltrot);
+        for (size_t k = 0; k < (size_t)c->lqp * c->ltrot; k++) c->confp[k] = 1;
+    }
     c->G = zalloc((size_t)N * N);
     c->Gtmp = zalloc((size_t)N * N);
     c->Ust = (zc **)calloc(c->nst + 1, sizeof(zc *));
@@ -815,7 +829,7 @@ void orc_destroy(orc_ctx *c)
     free(c->urt); free(c->urtm1);
     free(c->nnlf); free(c->nnlist); free(c->cx); free(c->cy); free(c->cz); free(c->sub);
     free(c->eqb); free(c->ca); free(c->cnn); free(c->csn); free(c->cjj); free(c->plq);
-    free(c->confv);
+    free(c->confv); free(c->confp); free(c->pcord);
     free(c->iwrap_nt); free(c->wrap_step);
     free(c->prop_nbr); free(c->prop_w);
     free(c->gt0); free(c->g0t); free(c->g00); free(c->znn);
@@ -831,6 +845,11 @@ void orc_sizes(const orc_ctx *c, int *out)
 /* dqmc_input_conf.f90:55-63 (single rank): conf(i,nt) = ceiling(u*lcomp), nt outer, i inner */
 void orc_init_conf(orc_ctx *c)
 {
+    /* hconf%input_plqconf first (src/dqmc_ft_ctrl.f90:19-23; dqmc_input_plqconf.f90:55-62):
+     * conf_plqu(i, nt) = ceiling(u * 4), nt outer, plaquette inner */
+    if (c->lwrapp)
+        for (size_t k = 0; k < (size_t)c->lqp * c->ltrot; k++)
+            c->confp[k] = (int32_t)ceil(sfmt_stream(&c->rng) * 4);
     if (c->lwrapu)
         for (int nt = 0; nt < c->ltrot; nt++)
             for (int i = 0; i < c->ndim; i++)
@@ -850,6 +869,16 @@ void orc_set_confv(orc_ctx *c, const int32_t *conf)
     if (c->confv) memcpy(c->confv, conf, sizeof(int32_t) * (size_t)c->ndim * c->nn_nf * c->ltrot);
 }
 void orc_get_vacc(const orc_ctx *c, double *acc) { acc[0] = c->vacc[0]; acc[1] = c->vacc[1]; }
+void orc_get_confp(const orc_ctx *c, int32_t *conf)
+{
+    if (c->confp) memcpy(conf, c->confp, sizeof(int32_t) * (size_t)c->lqp * c->ltrot);
+}
+void orc_set_confp(orc_ctx *c, const int32_t *conf)
+{
+    if (c->confp) memcpy(c->confp, conf, sizeof(int32_t) * (size_t)c->lqp * c->ltrot);
+}
+void orc_get_pacc(const orc_ctx *c, double *acc) { acc[0] = c->pacc[0]; acc[1] = c->pacc[1]; }
+int orc_lqp(const orc_ctx *c) { return c->lqp; }
 
 /* ---- the bond field of the finite-temperature t-V model ------------------------------- */
 /* vconf%set_conf with lprojv = .false. (src/dqmc_config_v/dqmc_set_conf.f90:14-77): alpha =
@@ -947,6 +976,189 @@ static void fv_update(orc_ctx *c, int nt, int nf)
         }
     }
 }
+/* ---- the plaquette field (hconf, lprojplqu = .false.) ---------------------------------- */
+static void expm_herm(int N, const zc *H, double s, zc *out);
+/* latt%plq_cord (src/lattice.f90:898-958 square, :735-773 honeycomb), 0-based: the square's
+ * plaquettes A (i1, i2 odd), B (i1 even, i2 odd), C (both even), D (i1 odd, i2 even), each class
+ * in (i1, i2) order, anticlockwise from the lower-left corner; the honeycomb's A (mod(i1+i2, 3) =
+ * 2), B (= 1), C (= 0), clockwise from orbital 2 of cell (i1, i2).  set_plqconf
+ * (src/dqmc_config_plqu/dqmc_set_plqconf.f90:57-199): V = the plaquette ring with 1/nn_nf on the
+ * diagonal and alpha e^{-+i theta}, a = sqrt(dtau rhub_plq), bmat(is) = exp(i a eta_is V) (the
+ * reference's U exp(i a eta eig) U^H from zheev is the same matrix function), its inverse,
+ * delta(iflip, is) = exp(i a (eta_isp - eta_is) V) - 1, phase(is) = exp(-i a eta_is f (nflr +
+ * nu)), phase_ratio(iflip, is) = exp(i a (eta_is - eta_isp) f (nflr + nu)), f = 1/2 on the
+ * square and 1 on the honeycomb; eta and gamma of dqmc_config_plqu.f90:42-51 */
+static void setup_pfield(orc_ctx *c)
+{
+    int l1 = c->p.la, l2 = c->p.lb, z = (c->p.lattice == 0) ? 4 : 6;
+    c->zp = z;
+    c->lqp = l1 * l2;
+    c->pcord = (int *)malloc(sizeof(int) * (size_t)c->lqp * z);
+#define PWR(v, l) ((v) > (l) ? (v) - (l) : ((v) < 1 ? (v) + (l) : (v)))
+#define PSQ(x, y) ((PWR(x, l1) - 1) * l2 + PWR(y, l2) - 1)
+#define PHC(x, y, o) (2 * ((PWR(x, l1) - 1) * l2 + PWR(y, l2) - 1) + (o) - 1)
+    int id = 0;
+    for (int cls = 0; cls < (z == 4 ? 4 : 3); cls++)
+        for (int i1 = 1; i1 <= l1; i1++)
+            for (int i2 = 1; i2 <= l2; i2++) {
+                int k;
+                if (z == 4) k = (i1 % 2 == 1 && i2 % 2 == 1) ? 0 : (i1 % 2 == 0 && i2 % 2 == 1) ? 1 : (i1 % 2 == 0) ? 2 : 3;
+                else k = ((i1 + i2) % 3 == 2) ? 0 : ((i1 + i2) % 3 == 1) ? 1 : 2;
+                if (k != cls) continue;
+                int *q = c->pcord + (size_t)id * z;
+                if (z == 4) {
+                    q[0] = PSQ(i1, i2); q[1] = PSQ(i1 + 1, i2); q[2] = PSQ(i1 + 1, i2 + 1); q[3] = PSQ(i1, i2 + 1);
+                } else {
+                    q[0] = PHC(i1, i2, 2); q[1] = PHC(i1, i2, 1); q[2] = PHC(i1, i2 - 1, 2);
+                    q[3] = PHC(i1 - 1, i2 - 1, 1); q[4] = PHC(i1 - 1, i2 - 1, 2); q[5] = PHC(i1 - 1, i2, 1);
+                }
+                id++;
+            }
+#undef PSQ
+#undef PHC
+#undef PWR
+    double th = c->p.theta * M_PI, f = (z == 4) ? 0.5 : 1.0;
+    zc z0 = 1.0 / c->nn_nf, z1 = c->p.alpha * cexp(CMPLX(0.0, -th)), z1h = c->p.alpha * cexp(CMPLX(0.0, th));
+    zc V[36] = {0};
+    /* the ring link i -> j = i+1 (mod z) of the reference's column-major initialisers
+     * (dqmc_set_plqconf.f90:42-57): V(j, i) = z1, V(i, j) = z1h for even i; V(j, i) = -z1h,
+     * V(i, j) = -z1 for odd i */
+    for (int i = 0; i < z; i++) V[IDX(i, i, z)] = z0;
+    for (int i = 0; i < z; i++) {
+        int j = (i + 1) % z;
+        V[IDX(j, i, z)] = (i % 2 == 0) ? z1 : -z1h;
+        V[IDX(i, j, z)] = (i % 2 == 0) ? z1h : -z1;
+    }
+    zc iV[36];
+    for (int k = 0; k < z * z; k++) iV[k] = I * V[k];
+    const double eta[4] = {-sqrt(2.0 * (3.0 + sqrt(6.0))), -sqrt(2.0 * (3.0 - sqrt(6.0))), sqrt(2.0 * (3.0 - sqrt(6.0))),
+                           sqrt(2.0 * (3.0 + sqrt(6.0)))};
+    const double gam[4] = {1.0 - sqrt(6.0) / 3.0, 1.0 + sqrt(6.0) / 3.0, 1.0 + sqrt(6.0) / 3.0, 1.0 - sqrt(6.0) / 3.0};
+    double a = sqrt(c->p.dtau * c->p.rhub_plq);
+    for (int is = 0; is < 4; is++) {
+        c->pgaml[is] = gam[is];
+        expm_herm(z, iV, a * eta[is], c->pb[is]);
+        expm_herm(z, iV, -a * eta[is], c->pbi[is]);
+        c->pph[is] = cexp(CMPLX(0.0, -a * eta[is] * f * (c->p.nflr + c->p.nu)));
+        for (int fl = 1; fl <= 3; fl++) {
+            int isp = (is + fl) % 4;
+            expm_herm(z, iV, a * (eta[isp] - eta[is]), c->pdl[is][fl - 1]);
+            for (int i = 0; i < z; i++) c->pdl[is][fl - 1][IDX(i, i, z)] -= 1.0;
+            c->ppr[is][fl - 1] = cexp(CMPLX(0.0, a * (eta[is] - eta[isp]) * f * (c->p.nflr + c->p.nu)));
+        }
+    }
+}
+static int fp_is(const orc_ctx *c, int nt, int p) { return c->confp[(size_t)(nt - 1) * c->lqp + p] - 1; }
+static zc zpow(zc z, int n);
+/* one plaquette: rows P of M <- T M(P, :) (left_forward with bmat, left_backward / _hc with
+ * bmat_inv: dqmc_left_forward_prop.f90:22-40) */
+static void pl_rows1(orc_ctx *c, int p, const zc *T, zc *M)
+{
+    int N = c->ndim, z = c->zp;
+    const int *P = c->pcord + (size_t)p * z;
+    zc x[6];
+    for (int j = 0; j < N; j++) {
+        for (int m = 0; m < z; m++) x[m] = M[IDX(P[m], j, N)];
+        for (int r = 0; r < z; r++) {
+            zc v = 0;
+            for (int m = 0; m < z; m++) v += T[IDX(r, m, z)] * x[m];
+            M[IDX(P[r], j, N)] = v;
+        }
+    }
+}
+/* columns P of M <- M(:, P) T (right_backward with bmat_inv, right_forward with bmat:
+ * dqmc_right_backward_prop.f90:22-40) */
+static void pl_cols1(orc_ctx *c, int p, const zc *T, zc *M)
+{
+    int N = c->ndim, z = c->zp;
+    const int *P = c->pcord + (size_t)p * z;
+    zc x[6];
+    for (int i = 0; i < N; i++) {
+        for (int m = 0; m < z; m++) x[m] = M[IDX(i, P[m], N)];
+        for (int r = 0; r < z; r++) {
+            zc v = 0;
+            for (int m = 0; m < z; m++) v += x[m] * T[IDX(m, r, z)];
+            M[IDX(i, P[r], N)] = v;
+        }
+    }
+}
+/* dqmc_update_plqu (src/dqmc_config_plqu/dqmc_update_plqu.f90:25-84): M = 1 + (1 - G)_PP
+ * delta(iflip, is), ratio1 = det M (here by LU; the reference's s_inv_det_qr_z by QR), ratiotot
+ * = ratio1^nflr gaml(isp) / gaml(is) phase_ratio(iflip, is); on acceptance G <- G - G(:, P)
+ * delta M^-1 (1 - G)(P, :) */
+static void pl_update(orc_ctx *c, int nt, int p)
+{
+    int N = c->ndim, z = c->zp;
+    const int *P = c->pcord + (size_t)p * z;
+    zc *G = c->G;
+    int is = fp_is(c, nt, p);
+    int iflip = (int)ceil(sfmt_stream(&c->rng) * 3.0);
+    int isp = (is + iflip) % 4;
+    const zc *dl = c->pdl[is][iflip - 1];
+    double Mr[72];
+    zc Mz[36];
+    for (int a = 0; a < z; a++)
+        for (int b = 0; b < z; b++) {
+            zc v = (a == b) ? 1.0 : 0.0;
+            for (int m = 0; m < z; m++) v += ((a == m ? 1.0 : 0.0) - G[IDX(P[a], P[m], N)]) * dl[IDX(m, b, z)];
+            Mz[IDX(a, b, z)] = v;
+            Mr[2 * IDX(a, b, z)] = creal(v);
+            Mr[2 * IDX(a, b, z) + 1] = cimag(v);
+        }
+    double ld[2];
+    int info = orc_inv_logdet(z, Mr, ld);   /* Mr <- M^-1 */
+    zc ratio1 = info ? 0.0 : cexp(CMPLX(ld[0], ld[1]));
+    zc ratiotot = zpow(ratio1, c->p.nflr) * (c->pgaml[isp] / c->pgaml[is]) * c->ppr[is][iflip - 1];
+    double rr = creal(ratiotot * c->phase) / creal(c->phase);
+    double random = sfmt_stream(&c->rng);
+    c->pacc[1] += 1.0;
+    (void)Mz;
+    if (fabs(rr) > random) {
+        c->pacc[0] += 1.0;
+        c->phase *= ratiotot / cabs(ratiotot);
+        zc S[36];
+        for (int a = 0; a < z; a++)
+            for (int b = 0; b < z; b++) {
+                zc v = 0;
+                for (int m = 0; m < z; m++) v += dl[IDX(a, m, z)] * CMPLX(Mr[2 * IDX(m, b, z)], Mr[2 * IDX(m, b, z) + 1]);
+                S[IDX(a, b, z)] = v;
+            }
+        zc *cp = zalloc((size_t)N * z), *sv = zalloc((size_t)N * z);
+        for (int m = 0; m < z; m++)
+            for (int i = 0; i < N; i++) cp[IDX(i, m, N)] = G[IDX(i, P[m], N)];
+        for (int j = 0; j < N; j++)
+            for (int a = 0; a < z; a++) {
+                zc v = 0;
+                for (int m = 0; m < z; m++) v += S[IDX(a, m, z)] * ((j == P[m] ? 1.0 : 0.0) - G[IDX(P[m], j, N)]);
+                sv[IDX(a, j, z)] = -v;
+            }
+        for (int j = 0; j < N; j++)
+            for (int i = 0; i < N; i++) {
+                zc v = 0;
+                for (int m = 0; m < z; m++) v += cp[IDX(i, m, N)] * sv[IDX(m, j, z)];
+                G[IDX(i, j, N)] += v;
+            }
+        free(cp); free(sv);
+        c->confp[(size_t)(nt - 1) * c->lqp + p] = isp + 1;
+    }
+}
+/* every plaquette of slice nt on rows (left: bmat 1..lq; left_hc: bmat_inv lq..1) or columns
+ * (right_backward: bmat_inv 1..lq) */
+static void pl_rows(orc_ctx *c, int nt, int inv, int rev, zc *M)
+{
+    for (int q = 0; q < c->lqp; q++) {
+        int p = rev ? c->lqp - 1 - q : q, is = fp_is(c, nt, p);
+        pl_rows1(c, p, inv ? c->pbi[is] : c->pb[is], M);
+    }
+}
+static void pl_cols(orc_ctx *c, int nt, int inv, int rev, zc *M)
+{
+    for (int q = 0; q < c->lqp; q++) {
+        int p = rev ? c->lqp - 1 - q : q, is = fp_is(c, nt, p);
+        pl_cols1(c, p, inv ? c->pbi[is] : c->pb[is], M);
+    }
+}
+
 void orc_get_conf(const orc_ctx *c, int32_t *conf) { memcpy(conf, c->conf, sizeof(int32_t) * (size_t)c->ndim * c->ltrot); }
 void orc_set_conf(orc_ctx *c, const int32_t *conf) { memcpy(c->conf, conf, sizeof(int32_t) * (size_t)c->ndim * c->ltrot); }
 
@@ -1031,6 +1243,8 @@ static void bmat_left_forward(orc_ctx *c, int nt1, int nt2, zc *M)
 {
     for (int nt = nt2; nt <= nt1; nt++) {
         khalf_left(c, c->urt, M, 0);
+        /* plaquettes 1..lq (src/dqmc_ft_core.f90:91-96) */
+        if (c->lwrapp) pl_rows(c, nt, 0, 0, M);
         /* bond field, families 1..nn_nf, nflag 2 then 1 (src/dqmc_ft_core.f90:98-104) */
         for (int nf = 0; nf < c->nn_nf && c->lwrapv; nf++) { fv_left(c, nt, nf, 0, M); fv_left(c, nt, nf, 1, M); }
         v_rows(c, nt, c->bmat, 0, M);
@@ -1046,6 +1260,8 @@ static void bmat_left_forward_hc(orc_ctx *c, int nt1, int nt2, zc *M)
         /* then the bond field, families nn_nf..1 (src/dqmc_ft_core.f90:141-156: the on-site field
          * comes first in the conjugated product); the real bond factors are their own conjugates */
         for (int nf = c->nn_nf - 1; nf >= 0 && c->lwrapv; nf--) { fv_left(c, nt, nf, 0, M); fv_left(c, nt, nf, 1, M); }
+        /* plaquettes lq..1 with left_forward_prop_hc = bmat_inv (src/dqmc_ft_core.f90:158-163) */
+        if (c->lwrapp) pl_rows(c, nt, 1, 1, M);
         khalf_left(c, c->urt, M, 1);
     }
 }
@@ -1535,6 +1751,8 @@ static int green_equaltimebb(orc_ctx *c, const zc *ur, const double *dr, const z
 static zc set_phase(orc_ctx *c, zc logw)
 {
     zc w = cexp(CMPLX(0.0, (double)c->p.nflr * cimag(logw)));
+    if (c->lwrapp)
+        for (size_t k = 0; k < (size_t)c->ltrot * c->lqp; k++) w *= c->pph[c->confp[k] - 1];
     if (c->lwrapu && c->p.rhub > 0.0)
         for (size_t k = 0; k < (size_t)c->ltrot * c->ndim; k++) w *= c->phase_tab[c->conf[k] - 1];
     return w;
@@ -1865,6 +2083,14 @@ int orc_sweep_b0(orc_ctx *c, int lupdate, int lmeasure)
             fv_right(c, nt, nf, 2, c->G);
             fv_left(c, nt, nf, 2, c->G);
         }
+        /* plaquettes lq..1: update, right_forward (bmat), left_backward (bmat_inv)
+         * (src/dqmc_ft_core/dqmc_ft_sweep_b0.f90:91-100) */
+        for (int p = c->lqp - 1; p >= 0 && c->lwrapp; p--) {
+            if (lupdate) pl_update(c, nt, p);
+            int is = fp_is(c, nt, p);
+            pl_cols1(c, p, c->pb[is], c->G);
+            pl_rows1(c, p, c->pbi[is], c->G);
+        }
         if (c->lwrapT) {
             int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
             for (int q = nfam - 1; q >= 0; q--) { k_right(c, q, c->urt, c->G); k_left(c, q, c->urtm1, c->G); }
@@ -1901,6 +2127,14 @@ int orc_sweep_0b(orc_ctx *c, int lupdate, int lmeasure)
         if (c->lwrapT) {
             int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
             for (int q = 0; q < nfam; q++) { k_left(c, q, c->urt, c->G); k_right(c, q, c->urtm1, c->G); }
+        }
+        /* plaquettes 1..lq: left_forward, right_backward, update
+         * (src/dqmc_ft_core/dqmc_ft_sweep_0b.f90:82-93) */
+        for (int p = 0; p < c->lqp && c->lwrapp; p++) {
+            int is = fp_is(c, nt, p);
+            pl_rows1(c, p, c->pb[is], c->G);
+            pl_cols1(c, p, c->pbi[is], c->G);
+            if (lupdate) pl_update(c, nt, p);
         }
         /* bond field, families 1..nn_nf (src/dqmc_ft_core/dqmc_ft_sweep_0b.f90:95-106) */
         for (int nf = 0; nf < c->nn_nf && c->lwrapv; nf++) {
@@ -2066,6 +2300,8 @@ static void bmat_right_backward_slice(orc_ctx *c, int nt, zc *M)
 {
     int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
     if (c->lwrapT) for (int q = 0; q < nfam; q++) k_right(c, q, c->urtm1, M);
+    /* plaquettes 1..lq (right_backward: bmat_inv, src/dqmc_ft_core.f90:257-262) */
+    if (c->lwrapp) pl_cols(c, nt, 1, 0, M);
     /* bond field: right_backward nflag 2 then 1, families 1..nn_nf */
     for (int nf = 0; nf < c->nn_nf && c->lwrapv; nf++) { fv_right(c, nt, nf, 0, M); fv_right(c, nt, nf, 1, M); }
     v_cols(c, nt, c->bmat_inv, M);
@@ -2102,6 +2338,10 @@ int orc_sweep_0b_dyn(orc_ctx *c, int lmeasure)
         if (c->lwrapT) {
             int nfam = (c->p.kmode == 1) ? 1 : c->nn_nf;
             for (int q = 0; q < nfam; q++) { k_left(c, q, c->urt, c->G); k_right(c, q, c->urtm1, c->G); }
+        }
+        if (c->lwrapp) {
+            pl_rows(c, nt, 0, 0, c->G);
+            pl_cols(c, nt, 1, 0, c->G);
         }
         for (int nf = 0; nf < c->nn_nf && c->lwrapv; nf++) {
             fv_left(c, nt, nf, 0, c->G);

@@ -30,6 +30,8 @@ typedef struct orc_params {
     int seed;       /* SFMT19937 seed, int32 */
     int lc;         /* third linear size (cubic) */
     double rv;      /* nearest-neighbour V: the bond field of the finite-temperature t-V model (0: off) */
+    double rhub_plq;        /* plaquette interaction: the plaquette field (0: off; square, honeycomb) */
+    double alpha, theta;    /* its assisting hopping: amplitude, phase in units of pi */
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;
@@ -82,6 +84,11 @@ int orc_set_eqt(orc_ctx *c, double theta);
 void orc_get_confv(const orc_ctx *c, int32_t *conf);
 void orc_set_confv(orc_ctx *c, const int32_t *conf);
 void orc_get_vacc(const orc_ctx *c, double *acc);
+/* the plaquette field [ltrot][lqp] (values 1..4), its (accepted, tried) counts and size lqp */
+void orc_get_confp(const orc_ctx *c, int32_t *conf);
+void orc_set_confp(orc_ctx *c, const int32_t *conf);
+void orc_get_pacc(const orc_ctx *c, double *acc);
+int orc_lqp(const orc_ctx *c);
 void orc_get_eqt(const orc_ctx *c, double *out);
 int orc_sweep_0b_dyn(orc_ctx *c, int lmeasure);
 void orc_get_gtau(const orc_ctx *c, double *gt0, double *g0t, double *g00);

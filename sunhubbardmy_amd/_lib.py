@@ -25,6 +25,8 @@ class ShModel(ctypes.Structure):
         ("update", ctypes.c_int), ("nublock", ctypes.c_int), ("seed", ctypes.c_int32),
         ("kin_weight", ctypes.c_double), ("vtab", dp), ("vdelta", dp),
         ("fft_dims", ctypes.c_int * 3),
+        ("zplq", ctypes.c_int), ("lqp", ctypes.c_int), ("plq_cord", ip),
+        ("plq_b", dp), ("plq_bi", dp), ("plq_delta", dp), ("plq_pratio", dp), ("plq_parg", dp), ("plq_gaml", dp),
     ]
 
 
@@ -33,6 +35,7 @@ EXPORTS = [
     "sh_sweep_start_0b", "sh_sweep_b0", "sh_sweep_0b", "sh_push_stage", "sh_pop_stage",
     "sh_get_g", "sh_get_diag", "sh_reset_diag", "sh_get_obs", "sh_get_obs_all", "sh_reset_obs", "sh_set_trace",
     "sh_get_trace", "sh_timers", "sh_synchronize", "sh_set_graphs", "sh_autotune", "sh_get_rng_state", "sh_set_rng_state", "sh_set_dyn", "sh_set_eqt", "sh_get_eqt", "sh_set_confv", "sh_get_confv", "sh_get_vacc",
+    "sh_set_confp", "sh_get_confp", "sh_get_pacc",
     "sh_get_dyn_obs", "sh_reset_dyn_obs", "sh_get_gtau",
 ]
 
@@ -101,6 +104,9 @@ def lib():
     L.sh_set_confv.argtypes = [P, ctypes.c_int, ip]
     L.sh_get_confv.argtypes = [P, ctypes.c_int, ip]
     L.sh_get_vacc.argtypes = [P, ctypes.c_int, dp]
+    L.sh_set_confp.argtypes = [P, ctypes.c_int, ip]
+    L.sh_get_confp.argtypes = [P, ctypes.c_int, ip]
+    L.sh_get_pacc.argtypes = [P, ctypes.c_int, dp]
     L.sh_get_dyn_obs.argtypes = [P, ctypes.c_int, dp, dp]
     L.sh_reset_dyn_obs.argtypes = [P]
     L.sh_get_gtau.argtypes = [P, ctypes.c_int, dp, dp, dp]

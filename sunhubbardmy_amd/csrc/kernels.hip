@@ -102,7 +102,11 @@ k_sfmt_draw(const DChain *chains, int nchains, int mode)
 {
     const DChain &c = chains[blockIdx.x];
     if (mode == 0) {
-        // u0conf%input_conf, then v0conf%input_conf (src/dqmc_ft_ctrl.f90:25-35)
+        // hconf%input_plqconf, u0conf%input_conf, then v0conf%input_conf (src/dqmc_ft_ctrl.f90:19-35)
+        if (c.lwrapp) {
+            int32_t *cp = c.confp;
+            sfmt_draw_block(c.rng, (size_t)c.lqp * c.ltrot, [=](size_t k, double u) { cp[k] = (int32_t)ceil(u * 4); });
+        }
         if (c.lwrapu) {
             const int lc = c.lcomp;
             int32_t *conf = c.conf;
@@ -112,22 +116,28 @@ k_sfmt_draw(const DChain *chains, int nchains, int mode)
             int32_t *cv = c.confv;
             sfmt_draw_block(c.rng, (size_t)c.n * c.nn_nf * c.ltrot, [=](size_t k, double u) { cv[k] = (int32_t)ceil(u * 2); });
         }
-    } else if (!c.lwrapv) {
+    } else if (!c.lwrapv && !c.lwrapp) {
         double *un = c.unif;
         sfmt_draw_block(c.rng, (size_t)2 * c.n * c.ltrot, [=](size_t k, double u) { un[k] = u; });
     } else {
-        // per slice the on-site draws (2 n, when the Hubbard field is on) and the bond draws
-        // (2 nn_nf nn_lf) in the order the pass consumes them: b0 (mode 1) on-site first, 0b
-        // (mode 2) bonds first (dqmc_ft_sweep_b0.f90:70-89, dqmc_ft_sweep_0b.f90:95-112)
-        double *un = c.unif, *uv = c.unifv;
-        const size_t nu = c.lwrapu ? (size_t)2 * c.n : 0, nv = (size_t)2 * c.nn_nf * c.nn_lf, per = nu + nv;
+        // per slice the on-site draws (2 n, when the Hubbard field is on), the bond draws
+        // (2 nn_nf nn_lf) and the plaquette draws (2 lqp) in the order the pass consumes them:
+        // b0 (mode 1) on-site, bonds, plaquettes; 0b (mode 2) plaquettes, bonds, on-site
+        // (dqmc_ft_sweep_b0.f90:70-100, dqmc_ft_sweep_0b.f90:82-112)
+        double *un = c.unif, *uv = c.unifv, *up = c.unifp;
+        const size_t nu = c.lwrapu ? (size_t)2 * c.n : 0, nv = c.lwrapv ? (size_t)2 * c.nn_nf * c.nn_lf : 0;
+        const size_t np = c.lwrapp ? (size_t)2 * c.lqp : 0, per = nu + nv + np;
         const bool ufirst = mode == 1;
         sfmt_draw_block(c.rng, per * c.ltrot, [=](size_t k, double u) {
             const size_t s = k / per, r = k % per;
             if (ufirst) {
-                if (r < nu) un[s * nu + r]= u; else uv[s * nv + r - nu] = u;
+                if (r < nu) un[s * nu + r] = u;
+                else if (r < nu + nv) uv[s * nv + r - nu] = u;
+                else up[s * np + r - nu - nv] = u;
             } else {
-                if (r < nv) uv[s * nv + r] = u; else un[s * nu + r - nv] = u;
+                if (r < np) up[s * np + r] = u;
+                else if (r < np + nv) uv[s * nv + r - np] = u;
+                else un[s * nu + r - np - nv] = u;
             }
         });
     }
@@ -1653,7 +1663,7 @@ extern "C" __global__ void __launch_bounds__(1024)
 k_wrap(const DChain *chains, int compare)
 {
     __shared__ double red[16];
-    __shared__ double cnt_sh[4][16];
+    __shared__ double cnt_sh[8][16];
     const DChain &c = chains[blockIdx.x];
     const int n = c.n;
     double *sc = c.sc;
@@ -1676,7 +1686,9 @@ k_wrap(const DChain *chains, int compare)
     mx = 0.0;
     for (int i = 0; i < nw; i++) mx = fmax(mx, red[i]);
     // set_phase: exp(i nflr Im logw) * prod_{i,nt} phase(conf) = exp(i (nflr Im logw + sum cnt_l arg_l))
-    double cnt[4] = {0, 0, 0, 0};
+    // counts 0-3: the on-site field's components; 4-7: the plaquette field's (hconf%phase of
+    // every plaquette and slice, set_phase.f90:9-15)
+    double cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (c.phase_from_conf && c.lwrapu) {
         size_t tot = (size_t)n * c.ltrot;
         for (size_t e = threadIdx.x; e < tot; e += blockDim.x) {
@@ -1684,7 +1696,14 @@ k_wrap(const DChain *chains, int compare)
             cnt[0] += (v == 1); cnt[1] += (v == 2); cnt[2] += (v == 3); cnt[3] += (v == 4);
         }
     }
-    for (int l = 0; l < 4; l++) {
+    if (c.lwrapp) {
+        size_t tot = (size_t)c.lqp * c.ltrot;
+        for (size_t e = threadIdx.x; e < tot; e += blockDim.x) {
+            int v = c.confp[e];
+            cnt[4] += (v == 1); cnt[5] += (v == 2); cnt[6] += (v == 3); cnt[7] += (v == 4);
+        }
+    }
+    for (int l = 0; l < 8; l++) {
         double v = cnt[l];
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
         if (lane == 0) cnt_sh[l][w] = v;
@@ -1697,6 +1716,12 @@ k_wrap(const DChain *chains, int compare)
             for (int i = 0; i < nw; i++) s += cnt_sh[l][i];
             ang += s * c.parg[l];
         }
+        if (c.lwrapp)
+            for (int l = 0; l < 4; l++) {
+                double s = 0.0;
+                for (int i = 0; i < nw; i++) s += cnt_sh[4 + l][i];
+                ang += s * c.pparg[l];
+            }
         cplx ph = make_double2(cos(ang), sin(ang));
         int info = (int)sc[SC_INFO];
         if (nf_flag) sc[SC_NONFINITE] = 1.0;

@@ -8,6 +8,7 @@
 #include "dyn_kernels.hip"
 #include "eqt_kernels.hip"
 #include "tv_kernels.hip"
+#include "plq_kernels.hip"
 #include "fft_kernels.hip"
 #include "../../include/sunhip.h"
 #include "../../include/sunhip_learn.h"
@@ -71,6 +72,7 @@ struct sh_ctx : ctx_base {
     int ntdm = -1;                // time-displaced measurement: bins 0 .. ntdm, -1 = off
     size_t eqt_lds = 0;           // LDS of k_eqt, 0 while no chain has correlation bins
     int lwrapv = 0;               // the batch carries the t-V bond field
+    int lwrapp = 0;               // the batch carries the plaquette field
     // push/pop_stage backups
     std::vector<void *> stage;
     std::vector<cplx *> trace_bufs;
@@ -198,6 +200,22 @@ static void copy_mat(sh_ctx *ctx, int dsel, int dpt, int ssel, int spt)
 static size_t lds_cols(sh_ctx *ctx) { return (size_t)ctx->nmax * CPB * sizeof(cplx); }
 static size_t lds_fft(sh_ctx *ctx) { return (size_t)ctx->fft_fcb * ctx->fft_ld * sizeof(cplx); }
 static size_t lds_rows(sh_ctx *ctx) { return (size_t)cb_ldr(ctx->nmax) * RPB * sizeof(cplx); }
+// k_plq_prop: PLQ_CB lines of n + 1, the four factors, the plaquette cycles and the slice's field;
+// k_plq_slice: one plaquette's G(:, P) and (1 - G)(P, :)
+static size_t lds_plq_prop(sh_ctx *ctx)
+{
+    size_t b = 0;
+    for (const auto &c : ctx->hc)
+        b = std::max(b, (size_t)PLQ_CB * (c.n + 1) * sizeof(cplx) + (size_t)4 * c.zp * c.zp * sizeof(cplx) +
+                            (size_t)c.lqp * (c.zp + 1) * sizeof(int));
+    return b;
+}
+static size_t lds_plq_slice(sh_ctx *ctx)
+{
+    size_t b = 0;
+    for (const auto &c : ctx->hc) b = std::max(b, (size_t)2 * c.zp * c.n * sizeof(cplx));
+    return b;
+}
 
 // left product (rows mixed) on matrix sel: program bits as in k_cb_cols
 static void left_pass(sh_ctx *ctx, int sel, int pt, int ksel, int prog, int vsel, int nt0, int nt1, int step)
@@ -354,18 +372,44 @@ static void vupdate(sh_ctx *ctx, int nt, int nf, int slice, int p)
     hipLaunchKernelGGL(k_vupdate, dim3(ctx->nch), dim3(256), (size_t)4 * ctx->nmax * sizeof(cplx), ctx->st, ctx->dc,
                        nt, nf, slice, p);
 }
-// one slice of Bmat_left_forward with the bond field: K-half, the families 1..nn_nf (nflag 2
-// then 1), V, K-half (src/dqmc_ft_core.f90:80-119); of Bmat_left_forward_hc: K-half, V^H, the
-// families nn_nf..1, K-half (:128-171: the conjugated product takes the on-site field first)
+// every plaquette of slice nt on a chain matrix: rows (side 0) or columns (side 1), the factors
+// exp(i a eta V) (inv 0) or their inverses (inv 1), plaquettes 1..lqp (rev 0) or lqp..1 (rev 1)
+static void pprop(sh_ctx *ctx, int sel, int pt, int side, int inv, int rev, int nt)
+{
+    hipLaunchKernelGGL(k_plq_prop, dim3((ctx->nmax + PLQ_CB - 1) / PLQ_CB, 1, ctx->nch), dim3(256),
+                       lds_plq_prop(ctx), ctx->st, ctx->dc, sel, pt, side, inv, rev, nt);
+}
+// the plaquettes of slice nt on G in pass order: 0b (dir 0) B_p G B_p^-1 for p = 1..lqp, b0
+// (dir 1) B_p^-1 G B_p for p = lqp..1 (dqmc_ft_sweep_0b.f90:82-93, dqmc_ft_sweep_b0.f90:91-100);
+// with updates one sequential kernel per chain, else all rows and all columns at once
+static void plq_slice(sh_ctx *ctx, int nt, int dir, int slice, bool upd)
+{
+    if (upd) {
+        hipLaunchKernelGGL(k_plq_slice, dim3(ctx->nch), dim3(256), lds_plq_slice(ctx), ctx->st, ctx->dc, nt, dir,
+                           slice, 1);
+    } else if (dir == 0) {
+        pprop(ctx, M_G, 0, 0, 0, 0, nt);
+        pprop(ctx, M_G, 0, 1, 1, 0, nt);
+    } else {
+        pprop(ctx, M_G, 0, 1, 0, 1, nt);
+        pprop(ctx, M_G, 0, 0, 1, 1, nt);
+    }
+}
+// one slice of Bmat_left_forward with the bond or the plaquette field: K-half, the plaquettes
+// 1..lq, the bond families 1..nn_nf (nflag 2 then 1), V, K-half (src/dqmc_ft_core.f90:80-119); of
+// Bmat_left_forward_hc: K-half, V^H, the families nn_nf..1, the plaquettes lq..1 with bmat^H =
+// bmat_inv, K-half (:128-171: the conjugated product takes the fields in reverse order)
 static void bprod_slice_v(sh_ctx *ctx, int nt, int hc)
 {
     const int nfam = ctx->hc[0].nn_nf;
     left_pass(ctx, M_W1, 0, 0, hc ? 1 | 2 : 1, hc ? 2 : 0, nt, nt, 1);
-    for (int q = 0; q < nfam; q++) {
+    if (ctx->lwrapp && !hc) pprop(ctx, M_W1, 0, 0, 0, 0, nt);
+    for (int q = 0; q < nfam && ctx->lwrapv; q++) {
         const int nf = hc ? nfam - 1 - q : q;
         vprop(ctx, M_W1, nf, 0, 0, nt);
         vprop(ctx, M_W1, nf, 1, 0, nt);
     }
+    if (ctx->lwrapp && hc) pprop(ctx, M_W1, 0, 0, 1, 1, nt);
     left_pass(ctx, M_W1, 0, 0, hc ? 4 : 2 | 4, 0, nt, nt, 1);
 }
 
@@ -377,7 +421,7 @@ static void stablize_0b(sh_ctx *ctx, int n)
         copy_mat(ctx, M_W1, 0, M_U, n - 1);
         int ws1 = ctx->wrap_step[2 * (n - 1)], ws2 = ctx->wrap_step[2 * (n - 1) + 1];
         // Bmat_left_forward: slices ws1..ws2, each K-half, V, K-half (src/dqmc_ft_core.f90:70-119)
-        if (ctx->lwrapv)
+        if (ctx->lwrapv || ctx->lwrapp)
             for (int nt = ws1; nt <= ws2; nt++) bprod_slice_v(ctx, nt, 0);
         else
             left_pass(ctx, M_W1, 0, 0, 1 | 2 | 4, 0, ws1, ws2, +1);
@@ -393,7 +437,7 @@ static void stablize_b0(sh_ctx *ctx, int n)
         copy_mat(ctx, M_W1, 0, M_U, n);
         int ws1 = ctx->wrap_step[2 * (n - 1)], ws2 = ctx->wrap_step[2 * (n - 1) + 1];
         // Bmat_left_forward_hc: slices ws2 down to ws1 with conj(bmat) (src/dqmc_ft_core.f90:121-171)
-        if (ctx->lwrapv)
+        if (ctx->lwrapv || ctx->lwrapp)
             for (int nt = ws2; nt >= ws1; nt--) bprod_slice_v(ctx, nt, 1);
         else
             left_pass(ctx, M_W1, 0, 0, 1 | 2 | 4, 2, ws2, ws1, -1);
@@ -686,6 +730,30 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
             delete ctx;
             return rc;
         }
+        if ((q.zplq != 0) != (m[0].zplq != 0)) {
+            int rc = fail(ctx, SH_E_ARG, "all chains of a batch must have the plaquette field or none");
+            delete ctx;
+            return rc;
+        }
+        if (q.zplq) {
+            // square (4) or honeycomb (6) plaquettes of distinct sites; G(:, P) and (1 - G)(P, :)
+            // of one plaquette stay in LDS
+            bool ok = (q.zplq == 4 || q.zplq == 6) && q.lqp >= 1 && q.plq_cord && q.plq_b && q.plq_bi &&
+                      q.plq_delta && q.plq_pratio && q.plq_parg && q.plq_gaml &&
+                      (size_t)2 * q.zplq * q.ndim * sizeof(cplx) <= 150 * 1024;
+            for (int p = 0; ok && p < q.lqp; p++)
+                for (int a = 0; a < q.zplq && ok; a++) {
+                    const int sa = q.plq_cord[p * q.zplq + a];
+                    ok = sa >= 0 && sa < q.ndim;
+                    for (int b = 0; b < a && ok; b++) ok = sa != q.plq_cord[p * q.zplq + b];
+                }
+            if (!ok) {
+                int rc = fail(ctx, SH_E_ARG, "plaquette field: zplq 4 or 6, plaquettes of distinct sites in 0..ndim-1, "
+                                             "every table given, 2 zplq ndim complex within 150 KiB");
+                delete ctx;
+                return rc;
+            }
+        }
         for (int b = 0; b < q.nn_nf * q.nn_lf * 2; b++)
             if (q.bonds[b] < 0 || q.bonds[b] >= q.ndim) { delete ctx; return SH_E_ARG; }
         for (int i = 0; i <= q.ltrot; i++)
@@ -758,6 +826,27 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
             hipMemcpy(c.confv, ones.data(), ones.size() * sizeof(int32_t), hipMemcpyHostToDevice);
             hipMemset(c.vacc, 0, 2 * sizeof(double));
         }
+        if (q.zplq) {
+            const int z = q.zplq;
+            c.lwrapp = 1; c.zp = z; c.lqp = q.lqp;
+            c.confp = dalloc<int32_t>(ctx, (size_t)q.lqp * q.ltrot);
+            c.unifp = dalloc<double>(ctx, (size_t)2 * q.lqp * q.ltrot);
+            c.pacc = dalloc<double>(ctx, 2);
+            c.plqc = dupload<int32_t>(ctx, q.plq_cord, (size_t)q.lqp * z);
+            c.pb = dupload<cplx>(ctx, (const cplx *)q.plq_b, (size_t)4 * z * z);
+            c.pbi = dupload<cplx>(ctx, (const cplx *)q.plq_bi, (size_t)4 * z * z);
+            c.pdl = dupload<cplx>(ctx, (const cplx *)q.plq_delta, (size_t)12 * z * z);
+            if (!c.confp || !c.unifp || !c.pacc || !c.plqc || !c.pb || !c.pbi || !c.pdl) {
+                int rc = fail(ctx, SH_E_HIP, "device allocation failed");
+                sh_destroy(ctx);
+                return rc;
+            }
+            for (int k = 0; k < 12; k++) c.ppr[k] = make_double2(q.plq_pratio[2 * k], q.plq_pratio[2 * k + 1]);
+            for (int k = 0; k < 4; k++) { c.pgaml[k] = q.plq_gaml[k]; c.pparg[k] = q.plq_parg[k]; }
+            std::vector<int32_t> ones((size_t)q.lqp * q.ltrot, 1);
+            hipMemcpy(c.confp, ones.data(), ones.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+            hipMemset(c.pacc, 0, 2 * sizeof(double));
+        }
         c.G = dalloc<cplx>(ctx, nn); c.Gs = dalloc<cplx>(ctx, nn);
         c.W1 = dalloc<cplx>(ctx, nn); c.W2 = dalloc<cplx>(ctx, nn); c.W3 = dalloc<cplx>(ctx, nn);
         c.W4 = dalloc<cplx>(ctx, nn); c.Us = dalloc<cplx>(ctx, nn); c.Vs = dalloc<cplx>(ctx, nn);
@@ -808,6 +897,21 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
         seeds[ci] = q.seed;
     }
     ctx->lwrapv = ctx->hc[0].lwrapv;
+    ctx->lwrapp = ctx->hc[0].lwrapp;
+    if (ctx->lwrapp) {
+        // the plaquette kernels' LDS: whole lines of a chain matrix plus the tables, and one
+        // plaquette's rows and columns of G
+        const size_t a = lds_plq_prop(ctx), b = lds_plq_slice(ctx);
+        if ((a > 64 * 1024 && hipFuncSetAttribute((const void *)k_plq_prop, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  (int)a) != hipSuccess) ||
+            (b > 64 * 1024 && hipFuncSetAttribute((const void *)k_plq_slice, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  (int)b) != hipSuccess)) {
+            (void)hipGetLastError();
+            int rc = fail(ctx, SH_E_HIP, "dynamic LDS request above the device limit");
+            sh_destroy(ctx);
+            return rc;
+        }
+    }
     ctx->dc = dupload<DChain>(ctx, ctx->hc.data(), nchains);
     ctx->d_seeds = dupload<int32_t>(ctx, seeds.data(), nchains);
     ctx->seeds = seeds;
@@ -895,6 +999,39 @@ int sh_get_vacc(sh_ctx *ctx, int chain, double *acc)
     const DChain &c = ctx->hc[chain];
     if (!c.lwrapv) return fail(ctx, SH_E_STATE, "the chain has no bond field");
     HIPCHK(hipMemcpyAsync(acc, c.vacc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int sh_set_confp(sh_ctx *ctx, int chain, const int32_t *conf)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !conf) return SH_E_ARG;
+    const DChain &c = ctx->hc[chain];
+    if (!c.lwrapp) return fail(ctx, SH_E_STATE, "the chain has no plaquette field");
+    const size_t cnt = (size_t)c.lqp * c.ltrot;
+    for (size_t k = 0; k < cnt; k++)
+        if (conf[k] < 1 || conf[k] > 4) return fail(ctx, SH_E_ARG, "plaquette field values are 1..4");
+    HIPCHK(hipMemcpyAsync(c.confp, conf, cnt * sizeof(int32_t), hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int sh_get_confp(sh_ctx *ctx, int chain, int32_t *conf)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !conf) return SH_E_ARG;
+    const DChain &c = ctx->hc[chain];
+    if (!c.lwrapp) return fail(ctx, SH_E_STATE, "the chain has no plaquette field");
+    HIPCHK(hipMemcpyAsync(conf, c.confp, (size_t)c.lqp * c.ltrot * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return 0;
+}
+
+int sh_get_pacc(sh_ctx *ctx, int chain, double *acc)
+{
+    if (!ctx || chain < 0 || chain >= ctx->nch || !acc) return SH_E_ARG;
+    const DChain &c = ctx->hc[chain];
+    if (!c.lwrapp) return fail(ctx, SH_E_STATE, "the chain has no plaquette field");
+    HIPCHK(hipMemcpyAsync(acc, c.pacc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     return 0;
 }
@@ -1092,10 +1229,11 @@ int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure)
 {
     if (!ctx) return SH_E_ARG;
     if (!ctx->started) return fail(ctx, SH_E_STATE, "sweep_b0 before sweep_start_0b");
-    const bool upd = lupdate && ctx->hc[0].lwrapu, updv = lupdate && ctx->lwrapv;
+    const bool upd = lupdate && ctx->hc[0].lwrapu, updv = lupdate && ctx->lwrapv, updp = lupdate && ctx->lwrapp;
+    const bool split = ctx->lwrapv || ctx->lwrapp;
     auto body = [&]() {
     set_identity(ctx, ctx->nst);
-    if (upd || updv) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
+    if (upd || updv || updp) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 1);
     if (upd) prop_pass_start(ctx);
     int slice = 0;
     for (int nt = ctx->ltrot; nt >= 1; nt--, slice++) {
@@ -1108,20 +1246,21 @@ int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure)
         }
         if (upd && ctx->prop_local) update_slice(ctx, nt, slice);
         if (upd) block_move(ctx, nt);
-        if (!ctx->lwrapv) {
+        if (!split) {
             Tic t(ctx, T_WRAP);
             // V (right_forward bmat, left_backward bmat_inv) then the K-half nn_nf..1
             right_pass(ctx, 0, 8 | 4, 0, nt);
             left_pass(ctx, M_G, 0, 1, 8 | 4, 1, nt, nt, 1);
         } else {
             // V, the bond field by families nn_nf..1 (nflag 2, update, nflag 1:
-            // src/dqmc_ft_core/dqmc_ft_sweep_b0.f90:70-89), then the K-half nn_nf..1
+            // src/dqmc_ft_core/dqmc_ft_sweep_b0.f90:70-89), the plaquettes lq..1 (update, then
+            // right_forward / left_backward: :91-100), then the K-half nn_nf..1
             Tic t(ctx, T_WRAP);
             if (ctx->hc[0].lwrapu) {
                 right_pass(ctx, 0, 8, 0, nt);
                 left_pass(ctx, M_G, 0, 1, 8, 1, nt, nt, 1);
             }
-            const int nfam = ctx->hc[0].nn_nf;
+            const int nfam = ctx->lwrapv ? ctx->hc[0].nn_nf : 0;
             for (int p = 0; p < nfam; p++) {
                 const int nf = nfam - 1 - p;
                 vprop(ctx, M_G, nf, 1, 1, nt);
@@ -1130,6 +1269,7 @@ int sh_sweep_b0(sh_ctx *ctx, int lupdate, int lmeasure)
                 vprop(ctx, M_G, nf, 2, 1, nt);
                 vprop(ctx, M_G, nf, 2, 0, nt);
             }
+            if (ctx->lwrapp) plq_slice(ctx, nt, 1, slice, updp);
             right_pass(ctx, 0, 4, 0, nt);
             left_pass(ctx, M_G, 0, 1, 4, 1, nt, nt, 1);
         }
@@ -1171,19 +1311,22 @@ static int sweep_0b_dyn(sh_ctx *ctx, int lmeasure)
         hipLaunchKernelGGL(k_dyn_elem, grid_elem(ctx), dim3(256), 0, ctx->st, ctx->dc, 0, (int)M_G, 0);
     }
     dyn_measure(0);
+    const bool split = ctx->lwrapv || ctx->lwrapp;
     for (int nt = 1; nt <= ctx->ltrot; nt++) {
-        if (!ctx->lwrapv) {
+        if (!split) {
             Tic t(ctx, T_WRAP);
             left_pass(ctx, M_G, 0, 0, 1 | 2, 0, nt, nt, 1);
             right_pass(ctx, 1, 1 | 2, 1, nt);
             left_pass(ctx, M_G, 0, 0, 4, 0, nt, nt, 1);
             right_pass(ctx, 1, 4, 1, nt);
         } else {
-            // K-half, the bond field by families 1..nn_nf (nflag 2 then 1), V, K-half
+            // K-half, the plaquettes 1..lq, the bond field by families 1..nn_nf (nflag 2 then 1),
+            // V, K-half
             Tic t(ctx, T_WRAP);
-            const int nfam = ctx->hc[0].nn_nf;
+            const int nfam = ctx->lwrapv ? ctx->hc[0].nn_nf : 0;
             left_pass(ctx, M_G, 0, 0, 1, 0, nt, nt, 1);
             right_pass(ctx, 1, 1, 1, nt);
+            if (ctx->lwrapp) plq_slice(ctx, nt, 0, nt - 1, false);
             for (int nf = 0; nf < nfam; nf++) {
                 vprop(ctx, M_G, nf, 0, 0, nt);
                 vprop(ctx, M_G, nf, 0, 1, nt);
@@ -1194,16 +1337,20 @@ static int sweep_0b_dyn(sh_ctx *ctx, int lmeasure)
             right_pass(ctx, 1, 2 | 4, 1, nt);
         }
         if (lmeasure) measure(ctx);
-        if (!ctx->lwrapv) {
+        if (!split) {
             // G(t,0) <- B(t) G(t,0), G(0,t) <- G(0,t) B(t)^-1 (Bmat_left_forward / _right_backward)
             Tic t(ctx, T_WRAP);
             left_pass(ctx, M_GT0, 0, 0, 1 | 2 | 4, 0, nt, nt, 1);
             right_pass(ctx, 1, 1 | 2 | 4, 1, nt, M_G0T);
         } else {
             Tic t(ctx, T_WRAP);
-            const int nfam = ctx->hc[0].nn_nf;
+            const int nfam = ctx->lwrapv ? ctx->hc[0].nn_nf : 0;
             left_pass(ctx, M_GT0, 0, 0, 1, 0, nt, nt, 1);
             right_pass(ctx, 1, 1, 1, nt, M_G0T);
+            if (ctx->lwrapp) {
+                pprop(ctx, M_GT0, 0, 0, 0, 0, nt);
+                pprop(ctx, M_G0T, 0, 1, 1, 0, nt);
+            }
             for (int nf = 0; nf < nfam; nf++) {
                 vprop(ctx, M_GT0, nf, 0, 0, nt);
                 vprop(ctx, M_GT0, nf, 1, 0, nt);
@@ -1247,24 +1394,27 @@ int sh_sweep_0b(sh_ctx *ctx, int lupdate, int lmeasure, int lmeasure_dyn)
             if (!c.ltau) return fail(ctx, SH_E_STATE, "time-displaced measurement not set up on every chain (sh_set_dyn)");
         return sweep_0b_dyn(ctx, lmeasure);
     }
-    const bool upd = lupdate && ctx->hc[0].lwrapu, updv = lupdate && ctx->lwrapv;
+    const bool upd = lupdate && ctx->hc[0].lwrapu, updv = lupdate && ctx->lwrapv, updp = lupdate && ctx->lwrapp;
+    const bool split = ctx->lwrapv || ctx->lwrapp;
     auto body = [&]() {
     set_identity(ctx, 0);
-    if (upd || updv) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 2);
+    if (upd || updv || updp) hipLaunchKernelGGL(k_sfmt_draw, dim3(ctx->nch), dim3(64), 0, ctx->st, ctx->dc, ctx->nch, 2);
     if (upd) prop_pass_start(ctx);
     for (int nt = 1; nt <= ctx->ltrot; nt++) {
-        if (!ctx->lwrapv) {
+        if (!split) {
             Tic t(ctx, T_WRAP);
             // K-half B G B^{-1}: left_forward (urt), right_backward (urtm1); then V rows/cols
             left_pass(ctx, M_G, 0, 0, 1 | 2, 0, nt, nt, 1);
             right_pass(ctx, 1, 1 | 2, 1, nt);
         } else {
-            // K-half, the bond field by families 1..nn_nf (nflag 2, update, nflag 1:
-            // src/dqmc_ft_core/dqmc_ft_sweep_0b.f90:95-106), then V
+            // K-half, the plaquettes 1..lq (left_forward, right_backward, update:
+            // src/dqmc_ft_core/dqmc_ft_sweep_0b.f90:82-93), the bond field by families 1..nn_nf
+            // (nflag 2, update, nflag 1: :95-106), then V
             Tic t(ctx, T_WRAP);
             left_pass(ctx, M_G, 0, 0, 1, 0, nt, nt, 1);
             right_pass(ctx, 1, 1, 1, nt);
-            const int nfam = ctx->hc[0].nn_nf;
+            if (ctx->lwrapp) plq_slice(ctx, nt, 0, nt - 1, updp);
+            const int nfam = ctx->lwrapv ? ctx->hc[0].nn_nf : 0;
             for (int nf = 0; nf < nfam; nf++) {
                 vprop(ctx, M_G, nf, 0, 0, nt);
                 vprop(ctx, M_G, nf, 0, 1, nt);

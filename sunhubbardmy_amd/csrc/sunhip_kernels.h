@@ -121,6 +121,18 @@ struct DChain {
     // the DFT form of the kinetic term: lattice directions in site order (fd0 fastest); kt / ktm1
     // then hold exph0k / exph0kinv [n]
     int fd0, fd1, fd2, pad6;
+    // plaquette field (lwrapp): field [ltrot][lqp] (1..4), the pass's plaquette draws (2 lqp per
+    // slice, in the order the plaquettes are updated), acceptance sums; plaquette cycles
+    // [lqp][zp]; per component the zp x zp column-major factors exp(i a eta V) (pb) and
+    // exp(-i a eta V) (pbi), per (is, iflip) exp(i a (eta_isp - eta_is) V) - 1 (pdl, block
+    // 3 is + iflip, 0-based); phase_ratio at iflip + 3 is, gaml, arg of the phase per component
+    int lwrapp, zp, lqp, pad7;
+    int32_t *confp;
+    double *unifp, *pacc;
+    const int32_t *plqc;
+    const cplx *pb, *pbi, *pdl;
+    cplx ppr[12];
+    double pgaml[4], pparg[4];
 };
 
 __device__ __forceinline__ cplx cmk(double r, double i) { return make_double2(r, i); }

@@ -57,6 +57,9 @@ program ft_driver
   m(1)%kin_weight = kw
   m(1)%vtab = c_null_ptr; m(1)%vdelta = c_null_ptr
   m(1)%fft_dims = fdims
+  m(1)%zplq = 0; m(1)%lqp = 0
+  m(1)%plq_cord = c_null_ptr; m(1)%plq_b = c_null_ptr; m(1)%plq_bi = c_null_ptr; m(1)%plq_delta = c_null_ptr
+  m(1)%plq_pratio = c_null_ptr; m(1)%plq_parg = c_null_ptr; m(1)%plq_gaml = c_null_ptr
 
   call sh_check(sh_create(ctx, 1_c_int, m, 0_c_int), 'allocate_core')
   call sh_check(sh_init_conf(ctx), 'input_conf')

@@ -30,6 +30,11 @@ module sunhip_bridge
      type(c_ptr) :: vtab, vdelta
      ! SH_K_FFT: lattice directions in site order (n0 fastest); urt / urtm1 = exph0k / exph0kinv
      integer(c_int) :: fft_dims(3)
+     ! the plaquette field (zplq = 0: none): sites per plaquette, plaquettes, plq_cord(zplq, lqp)
+     ! (0-based), the factor tables bmat / bmat_inv (zplq, zplq, 4), delta (zplq, zplq, 3, 4),
+     ! phase_ratio (3, 4) complex, the phase args (4) and gaml (4)
+     integer(c_int) :: zplq, lqp
+     type(c_ptr) :: plq_cord, plq_b, plq_bi, plq_delta, plq_pratio, plq_parg, plq_gaml
   end type sh_model
 
   ! mirrors struct shp_model of include/sunhip_proj.h
@@ -95,6 +100,25 @@ module sunhip_bridge
        integer(c_int), value :: chain
        real(c_double), intent(out) :: acc(2)
      end function sh_get_vacc
+     ! the plaquette field conf_plqu(lqp, ltrot) and its (accepted, tried) counts
+     integer(c_int) function sh_set_confp(ctx, chain, conf) bind(C, name="sh_set_confp")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(in) :: conf(*)
+     end function sh_set_confp
+     integer(c_int) function sh_get_confp(ctx, chain, conf) bind(C, name="sh_get_confp")
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       integer(c_int32_t), intent(out) :: conf(*)
+     end function sh_get_confp
+     integer(c_int) function sh_get_pacc(ctx, chain, acc) bind(C, name="sh_get_pacc")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: chain
+       real(c_double), intent(out) :: acc(2)
+     end function sh_get_pacc
      ! correlation bins of equaltime_measure: cells(4, lq) = A site, nnlist(., 1), snlist(., 1),
      ! current partner (0-based); plq(zplq, lq) plaquette cycles; bins(2, 20 lq + 8)
      integer(c_int) function sh_set_eqt(ctx, chain, l1, l2, l3, sites, cells, zplq, plq, theta) &

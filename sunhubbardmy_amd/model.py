@@ -40,6 +40,9 @@ class Model:
     seed: int = 1909706392
     lc: int = 1                    # third linear size (cubic lattice)
     rv: float = 0.0                # nearest-neighbour V: the bond field of the t-V model
+    rhub_plq: float = 0.0          # plaquette interaction: the plaquette field (square, honeycomb)
+    alpha: float = 0.2             # amplitude of the plaquette's assisting hopping
+    theta: float = 0.0             # its phase, in units of pi (src/model_para.f90:266)
 
 
 def chain_seed(c):
@@ -377,6 +380,80 @@ def vconf_tables(m):
     return {"alpha": alpha, "vt": vt, "vd": vd}
 
 
+# the 4-component field of the plaquette interaction: eta and gamma for -2, -1, 1, 2
+# (src/dqmc_config_plqu.f90:42-51)
+PLQ_ETA = np.array([-np.sqrt(2.0 * (3.0 + np.sqrt(6.0))), -np.sqrt(2.0 * (3.0 - np.sqrt(6.0))),
+                    np.sqrt(2.0 * (3.0 - np.sqrt(6.0))), np.sqrt(2.0 * (3.0 + np.sqrt(6.0)))])
+PLQ_GAML = np.array([1.0 - np.sqrt(6.0) / 3.0, 1.0 + np.sqrt(6.0) / 3.0, 1.0 + np.sqrt(6.0) / 3.0,
+                     1.0 - np.sqrt(6.0) / 3.0])
+
+
+def plaquette_order(lattice, l1, l2):
+    """latt%plq_cord, 0-based, in the reference's plaquette order: the square's plaquettes by class
+    A (x, y odd), B (x even, y odd), C (both even), D (x odd, y even), each class in (x, y) order,
+    sites anticlockwise from the lower-left corner (src/lattice.f90:898-958); the honeycomb's by
+    class A ((x + y) mod 3 = 2), B (= 1), C (= 0), sites clockwise from the B site of cell (x, y)
+    (src/lattice.f90:735-773)."""
+    def sq(x, y):
+        return (_npbc(x, l1) - 1) * l2 + _npbc(y, l2) - 1
+
+    def hc(x, y, o):
+        return 2 * ((_npbc(x, l1) - 1) * l2 + _npbc(y, l2) - 1) + o
+    classes = {}
+    for x in range(1, l1 + 1):
+        for y in range(1, l2 + 1):
+            if lattice == "square":
+                k = {(1, 1): 0, (0, 1): 1, (0, 0): 2, (1, 0): 3}[(x % 2, y % 2)]
+                cyc = (sq(x, y), sq(x + 1, y), sq(x + 1, y + 1), sq(x, y + 1))
+            elif lattice == "honeycomb":
+                k = {2: 0, 1: 1, 0: 2}[(x + y) % 3]
+                cyc = (hc(x, y, 1), hc(x, y, 0), hc(x, y - 1, 1), hc(x - 1, y - 1, 0), hc(x - 1, y - 1, 1),
+                       hc(x - 1, y, 0))
+            else:
+                raise ValueError("the plaquette field needs the square or the honeycomb lattice")
+            classes.setdefault(k, []).append(cyc)
+    return np.array([c for k in sorted(classes) for c in classes[k]], np.int32)
+
+
+def plq_tables(m, lat):
+    """hconf%set_plqconf with lprojplqu = .false. (src/dqmc_config_plqu/dqmc_set_plqconf.f90:57-199):
+    the plaquette matrix V (z x z, 1/nn_nf on the diagonal, alpha e^{-+i theta} on the ring with the
+    reference's signs), a = sqrt(dtau rhub_plq), bmat(is) = exp(i a eta_is V), bmat_inv(is) =
+    exp(-i a eta_is V), delta(iflip, is) = exp(i a (eta_isp - eta_is) V) - 1 with isp = mod(is +
+    iflip - 1, 4) + 1, phase(is) = exp(-i a eta_is f (nflr + nu)), phase_ratio(iflip, is) =
+    exp(i a (eta_is - eta_isp) f (nflr + nu)), f = 1/2 on the square, 1 on the honeycomb."""
+    if m.rhub_plq <= 0.0:
+        raise ValueError("the plaquette field takes rhub_plq > 0 (a = sqrt(dtau rhub_plq))")
+    cord = plaquette_order(m.lattice, m.la, m.lb)
+    z = cord.shape[1]
+    z0, th = 1.0 / lat["nn_nf"], m.theta * np.pi
+    z1, z1h = m.alpha * np.exp(-1j * th), m.alpha * np.exp(1j * th)
+    if z == 4:
+        cols = [[z0, z1, 0, -z1], [z1h, z0, -z1h, 0], [0, -z1, z0, z1], [-z1h, 0, z1h, z0]]
+        f = 0.5
+    else:
+        cols = [[z0, z1, 0, 0, 0, -z1], [z1h, z0, -z1h, 0, 0, 0], [0, -z1, z0, z1, 0, 0],
+                [0, 0, z1h, z0, -z1h, 0], [0, 0, 0, -z1, z0, z1], [-z1h, 0, 0, 0, z1h, z0]]
+        f = 1.0
+    v = np.array(cols, complex).T          # the reference fills V column by column
+    w, u = np.linalg.eigh(v)
+    a = np.sqrt(m.dtau * m.rhub_plq)
+
+    def ex(c):
+        return (u * np.exp(1j * c * w)) @ u.conj().T
+    b = np.array([ex(a * e) for e in PLQ_ETA])
+    bi = np.array([ex(-a * e) for e in PLQ_ETA])
+    delta = np.zeros((4, 3, z, z), complex)
+    pratio = np.zeros((4, 3), complex)
+    for is_ in range(4):
+        for fl in range(1, 4):
+            isp = (is_ + fl) % 4
+            delta[is_, fl - 1] = ex(a * (PLQ_ETA[isp] - PLQ_ETA[is_])) - np.eye(z)
+            pratio[is_, fl - 1] = np.exp(1j * a * (PLQ_ETA[is_] - PLQ_ETA[isp]) * f * (m.nflr + m.nu))
+    return {"z": z, "lqp": len(cord), "cord": cord, "v": v, "a": a, "b": b, "bi": bi, "delta": delta,
+            "pratio": pratio, "parg": -a * PLQ_ETA * f * (m.nflr + m.nu), "gaml": PLQ_GAML.copy()}
+
+
 def make_tables(m):
     lat = lattice_tables(m)
     ltrot = int(round(m.beta / m.dtau))
@@ -391,4 +468,5 @@ def make_tables(m):
             "nublock": nub, "kmode": KMODES[m.kmode], "update": UPDATES[m.update],
             "lwrapT": int(m.rt != 0.0), "lwrapu": int(m.rhub != 0.0),
             "phase_from_conf": int(m.rhub > 0.0), "v": vconf_tables(m) if m.rv != 0.0 else None,
+            "plq": plq_tables(m, lat) if m.rhub_plq != 0.0 else None,
             "fft_dims": fft_dims(m) if KMODES[m.kmode] == 2 else (0, 0, 0)}

@@ -78,6 +78,17 @@ class FTWalkers:
         if t["v"] is not None:
             s.vtab = keep(cz(t["v"]["vt"].reshape(-1)), np.float64)
             s.vdelta = keep(cz(t["v"]["vd"].reshape(-1)), np.float64)
+        if t["plq"] is not None:
+            pq = t["plq"]
+            s.zplq, s.lqp = pq["z"], pq["lqp"]
+            s.plq_cord = keep(pq["cord"].reshape(-1), np.int32)
+            # z x z blocks column-major: the transpose of each block in C order
+            s.plq_b = keep(cz(pq["b"].transpose(0, 2, 1).reshape(-1)), np.float64)
+            s.plq_bi = keep(cz(pq["bi"].transpose(0, 2, 1).reshape(-1)), np.float64)
+            s.plq_delta = keep(cz(pq["delta"].transpose(0, 1, 3, 2).reshape(-1)), np.float64)
+            s.plq_pratio = keep(cz(pq["pratio"].reshape(-1)), np.float64)
+            s.plq_parg = keep(pq["parg"], np.float64)
+            s.plq_gaml = keep(pq["gaml"], np.float64)
         return s
 
     # -- lifecycle ---------------------------------------------------------------------------
@@ -132,6 +143,24 @@ class FTWalkers:
         """(accepted, tried) bond-field flips"""
         a = np.zeros(2)
         self._check(_lib.lib().sh_get_vacc(self._ctx, chain, a.ctypes.data_as(_lib.dp)), "get_vacc")
+        return a
+
+    def get_confp(self, chain=0):
+        """the plaquette field [ltrot][lqp], values 1..4"""
+        t = self.tables[chain]
+        c = np.zeros((t["ltrot"], t["plq"]["lqp"]), np.int32)
+        self._check(_lib.lib().sh_get_confp(self._ctx, chain, c.ctypes.data_as(_lib.ip)), "get_confp")
+        return c
+
+    def set_confp(self, conf, chain=0):
+        t = self.tables[chain]
+        c = _c(conf, np.int32).reshape(t["ltrot"], t["plq"]["lqp"])
+        self._check(_lib.lib().sh_set_confp(self._ctx, chain, c.ctypes.data_as(_lib.ip)), "set_confp")
+
+    def pacc(self, chain=0):
+        """(accepted, tried) plaquette-field flips"""
+        a = np.zeros(2)
+        self._check(_lib.lib().sh_get_pacc(self._ctx, chain, a.ctypes.data_as(_lib.dp)), "get_pacc")
         return a
 
     def rng_state(self, chain=0):

@@ -29,6 +29,11 @@ CASES = {
                        nwrap=4, update="delay", nublock=4),
     "honeycomb3_tuv_dense": dict(lattice="honeycomb", la=3, lb=3, beta=1.0, rhub=3.0, rv=1.0, nflr=2, kmode="dense",
                                  nwrap=5, update="delay", nublock=4),
+    # the plaquette field (hconf): its factors in G(t,0), G(0,t) and the B products
+    "square4_plq_delay": dict(lattice="square", la=4, lb=4, beta=1.0, rhub=3.0, rhub_plq=1.5, alpha=0.3,
+                              kmode="checkerboard", nwrap=4, update="delay", nublock=4),
+    "honeycomb3_plq_dense": dict(lattice="honeycomb", la=3, lb=3, beta=0.8, rhub=0.0, rhub_plq=1.2, alpha=0.3,
+                                 theta=0.25, kmode="dense", nwrap=4, update="fast"),
 }
 
 
@@ -59,6 +64,8 @@ def test_dyn_parity(name):
     assert np.array_equal(w.get_conf(), o.conf), "field differs after the updating sweep"
     if kw.get("rv"):
         assert np.array_equal(w.get_confv(), o.confv), "bond field differs after the updating sweep"
+    if kw.get("rhub_plq"):
+        assert np.array_equal(w.get_confp(), o.confp), "plaquette field differs after the updating sweep"
     w.sweep_b0(False, False)
     o.sweep_b0(lupdate=False, lmeasure=False)
     g_before = w.G()
