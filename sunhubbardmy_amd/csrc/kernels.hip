@@ -548,14 +548,14 @@ __device__ static void wg_rank_k(cplx *G, int n, const cplx *A, const cplx *B, i
 // registers while the wave walks the 16-column tiles.  Every load has a clamped, in-bounds
 // address and no branch around it (rows and columns past n compute values that are never
 // stored; entries past k are zero in the A fragment), so nothing waits on a load before its
-// use: G tiles are loaded PF tiles ahead into PF rotating register sets (PF = 2: the flush alone
-// takes 210 VGPRs, 251 at PF = 3, and spills at 4), the B fragment one tile ahead (a second B set
-// would spill in the delay kernel).  At k = 32 the
+// use: G tiles are loaded two tiles ahead into one of two alternating register sets (the flush
+// alone takes 210 VGPRs so; three sets take 251, four spill), the B fragment one tile ahead (a
+// second B set would spill in the delay kernel).  At k = 32 the
 // update reads and writes G for 8 flops per byte, so it is bound by HBM, not by the MFMAs; G
 // streams through with nontemporal accesses so that the A and B fragments, re-read for every
 // tile, stay in L2.
 // ------------------------------------------------------------------------------------------
-template <int KS, int PF>
+template <int KS>
 __device__ __forceinline__ void rank_k_tile(GAS cplx *G, GAS cplx *B, int n, int k, int row, bool rok, int kq, int l16,
                                             int tj, const cplx (&y)[KS], cplx (&x)[KS], cplx (&g)[4])
 {
@@ -566,12 +566,12 @@ __device__ __forceinline__ void rank_k_tile(GAS cplx *G, GAS cplx *B, int n, int
         accr[r] = g[r].x;
         acci[r] = g[r].y;
     }
-    // this register set now takes tile tj + PF (clamped: past the last tile it reloads the last
+    // this register set now takes tile tj + 2 (clamped: past the last tile it reloads the last
     // columns, which are never used); the B fragment takes tile tj + 1
     const size_t rowc = (size_t)min(row, n - 1);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        const int col = min(j0 + 16 * PF + kq + 4 * r, n - 1);
+        const int col = min(j0 + 32 + kq + 4 * r, n - 1);
         g[r] = gld_nt(G + (rowc + (size_t)col * n));
     }
     const size_t xcol = (size_t)min(j0 + 16 + l16, n - 1);
@@ -591,7 +591,7 @@ __device__ __forceinline__ void rank_k_tile(GAS cplx *G, GAS cplx *B, int n, int
     }
 }
 
-template <int K, int PF = 2>
+template <int K>
 __device__ static void wg_rank_k_strips(cplx *G_, int n, const cplx *A_, const cplx *B_, int k)
 {
     constexpr int KS = K / 4;
@@ -604,7 +604,7 @@ __device__ static void wg_rank_k_strips(cplx *G_, int n, const cplx *A_, const c
         const bool rok = row < n;
         const size_t rowc = (size_t)min(row, n - 1);
         const size_t xcol = (size_t)min(l16, n - 1);
-        cplx y[KS], x[KS], g[PF][4];
+        cplx y[KS], x[KS], ga[4], gb[4];
 #pragma unroll
         for (int s = 0; s < KS; s++) {
             const int m = 4 * s + kq, mc = min(m, k - 1);
@@ -613,13 +613,13 @@ __device__ static void wg_rank_k_strips(cplx *G_, int n, const cplx *A_, const c
             x[s] = gld(B + (xcol + (size_t)mc * n));
         }
 #pragma unroll
-        for (int p = 0; p < PF; p++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) g[p][r] = gld_nt(G + (rowc + (size_t)min(16 * p + kq + 4 * r, n - 1) * n));
-        for (int tj = 0; tj < nt; tj += PF) {
-#pragma unroll
-            for (int p = 0; p < PF; p++)
-                if (tj + p < nt) rank_k_tile<KS, PF>(G, B, n, k, row, rok, kq, l16, tj + p, y, x, g[p]);
+        for (int r = 0; r < 4; r++) {
+            ga[r] = gld_nt(G + (rowc + (size_t)min(kq + 4 * r, n - 1) * n));
+            gb[r] = gld_nt(G + (rowc + (size_t)min(16 + kq + 4 * r, n - 1) * n));
+        }
+        for (int tj = 0; tj < nt; tj += 2) {
+            rank_k_tile<KS>(G, B, n, k, row, rok, kq, l16, tj, y, x, ga);
+            if (tj + 1 < nt) rank_k_tile<KS>(G, B, n, k, row, rok, kq, l16, tj + 1, y, x, gb);
         }
     }
 }
