@@ -4,7 +4,8 @@ conf_u.out (src/dqmc_config_u/dqmc_output_conf.f90:27-41): a first record `1` wr
 list-directed (gfortran: an integer right-justified in 12 columns), then one `(i4)` record per
 value, conf(i, nt) with nt outer and i inner, rank 0's field first and then every other rank's
 in rank order.  conf_v.out (src/dqmc_config_v/dqmc_output_conf.f90:27-44) is the same with
-conf(i, nf, nt), nf innermost.  Here chain c plays the role of MPI rank c.
+conf(i, nf, nt), nf innermost; conf_plqu.out (src/dqmc_config_plqu/dqmc_output_plqconf.f90:
+15-39) with conf_plqu(i, nt) over the plaquettes.  Here chain c plays the role of MPI rank c.
 
 Reading follows dqmc_input_conf (src/dqmc_config_u/dqmc_input_conf.f90:21-99): a missing file or
 a first value of 0 means "start from scratch" (fresh fields drawn from the chains' own streams,
@@ -51,6 +52,12 @@ def write_conf_v(path, confs):
     _write_records(path, 1, vals)
 
 
+def write_conf_plqu(path, confs):
+    """confs: per chain an int array [ltrot][lqp] (values 1..4)."""
+    vals = [v for c in confs for v in np.asarray(c, np.int64).reshape(-1)]
+    _write_records(path, 1, vals)
+
+
 def _read(path, nchains, count):
     """(lwarmup, list of flat per-chain arrays or None)"""
     if not os.path.exists(path):
@@ -89,6 +96,13 @@ def read_conf_v(path, nchains, nsites, nn_nf, ltrot):
     return lw, [np.ascontiguousarray(f.reshape(ltrot, nsites, nn_nf).transpose(0, 2, 1)) for f in flat]
 
 
+def read_conf_plqu(path, nchains, lqp, ltrot):
+    """dqmc_input_plqconf (src/dqmc_config_plqu/dqmc_input_plqconf.f90:21-99): (lwarmup,
+    [conf_plqu [ltrot][lqp]] per chain, or None to start from scratch)"""
+    lw, flat = _read(path, nchains, lqp * ltrot)
+    return lw, None if flat is None else [f.reshape(ltrot, lqp) for f in flat]
+
+
 def save_state(path, walkers):
     """Fields and generator states of every chain of FTWalkers or ProjWalkers to an .npz file."""
     d = {"kind": np.array(type(walkers).__name__), "rng": np.stack([walkers.rng_state(c) for c in range(walkers.nchains)])}
@@ -97,6 +111,8 @@ def save_state(path, walkers):
         d["uconf"] = np.stack([walkers.get_uconf(c) for c in range(walkers.nchains)])
     if type(walkers).__name__ == "FTWalkers" and walkers.has_vconf:
         d["confv"] = np.stack([walkers.get_confv(c) for c in range(walkers.nchains)])
+    if type(walkers).__name__ == "FTWalkers" and walkers.has_pconf:
+        d["confp"] = np.stack([walkers.get_confp(c) for c in range(walkers.nchains)])
     np.savez(path, **d)
 
 
@@ -114,4 +130,6 @@ def load_state(path, walkers):
                 walkers.set_uconf(c, z["uconf"][c])
             if "confv" in z:
                 walkers.set_confv(z["confv"][c], c)
+            if "confp" in z:
+                walkers.set_confp(z["confp"][c], c)
             walkers.set_rng_state(c, z["rng"][c])

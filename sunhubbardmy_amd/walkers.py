@@ -127,6 +127,10 @@ class FTWalkers:
     def has_vconf(self):
         return self.tables[0]["v"] is not None
 
+    @property
+    def has_pconf(self):
+        return self.tables[0]["plq"] is not None
+
     def get_confv(self, chain=0):
         """the t-V bond field [ltrot][nn_nf][ndim] (entries i < nn_lf are the bonds)"""
         t = self.tables[chain]
@@ -194,6 +198,23 @@ class FTWalkers:
         else:
             for c, conf in enumerate(confs):
                 self.set_conf(conf, c)
+        return lw
+
+    def output_plqconf(self, path="conf_plqu.out"):
+        """hconf%output_plqconf: every chain's plaquette field to `path` in the reference's format."""
+        from .confio import write_conf_plqu
+        write_conf_plqu(path, [self.get_confp(c) for c in range(self.nchains)])
+
+    def input_plqconf(self, path="conf_plqu.in"):
+        """hconf%input_plqconf: the plaquette fields from `path`; returns lwarmup.  A missing file
+        or a leading 0 keeps the fields sh_init_conf drew (the library draws every field at once,
+        the plaquette field first, as dqmc_ft_ctrl.f90:19-35 does)."""
+        from .confio import read_conf_plqu
+        t = self.tables[0]
+        lw, confs = read_conf_plqu(path, self.nchains, t["plq"]["lqp"], t["ltrot"])
+        if confs is not None:
+            for c, conf in enumerate(confs):
+                self.set_confp(conf, c)
         return lw
 
     # -- sweeps ------------------------------------------------------------------------------

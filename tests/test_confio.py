@@ -30,6 +30,19 @@ def test_conf_v_nf_innermost(tmp_path):
     assert lw is False and np.array_equal(back[0], conf)
 
 
+def test_conf_plqu_layout_and_short_file(tmp_path):
+    # conf_plqu(i, nt): plaquette inner, nt outer (dqmc_output_plqconf.f90:23-38), same records as
+    # conf_u; a file short for chain 1 copies chain 0 (dqmc_input_plqconf.f90:85-95)
+    c0 = np.array([[1, 4, 2], [3, 3, 1]])
+    p = tmp_path / "conf_plqu.out"
+    confio.write_conf_plqu(p, [c0])
+    assert p.read_text() == "           1\n" + "".join("%4d\n" % v for v in [1, 4, 2, 3, 3, 1])
+    lw, back = confio.read_conf_plqu(p, 3, 3, 2)
+    assert lw is False and len(back) == 3 and all(np.array_equal(b, c0) for b in back)
+    (tmp_path / "zero").write_text("0\n")
+    assert confio.read_conf_plqu(tmp_path / "zero", 2, 3, 2) == (True, None)
+
+
 def test_round_trip_many_chains(tmp_path):
     rng = np.random.default_rng(5)
     confs = [rng.integers(1, 5, size=(7, 9)) for _ in range(4)]
