@@ -102,3 +102,27 @@ def test_plq_field_io_and_batch_rules():
     plain = dict(kw, rhub_plq=0.0)
     with pytest.raises(SweepError):
         FTWalkers([Model(seed=1, **kw), Model(seed=2, **plain)])
+
+
+def test_plq_resume_bit_exact(tmp_path):
+    """save_state after one sweep, then one more sweep, equals a fresh batch restored from the
+    file: the plaquette field travels with the on-site field and the generator states"""
+    from sunhubbardmy_amd import confio
+    kw = dict(CASES["square4_hubbard_plq_delay"])
+    ms = [Model(seed=chain_seed(c), **kw) for c in range(2)]
+    a = FTWalkers(ms)
+    a.init_conf()
+    a.sweep_start_0b()
+    a.sweep(lupdate=True, lmeasure=False)
+    path = str(tmp_path / "state.npz")
+    confio.save_state(path, a)
+    a.sweep(lupdate=True, lmeasure=False)
+    b = FTWalkers(ms)
+    confio.load_state(path, b)
+    b.sweep_start_0b()
+    b.sweep(lupdate=True, lmeasure=False)
+    for c in range(2):
+        assert np.array_equal(a.get_confp(c), b.get_confp(c)) and np.array_equal(a.get_conf(c), b.get_conf(c))
+        assert rel(a.G(c), b.G(c)) < 1e-10
+    a.close()
+    b.close()
