@@ -770,123 +770,283 @@ k_update_delay(const DChain *chains, int nt, int slice_in_pass)
     }
 }
 
+// the factors of one site's draw that do not depend on G (the submatrix kernel evaluates them
+// for a staged block of sites at once, off the per-site critical path)
+struct SubSite {
+    cplx db, idb, f, pr;   // delta, 1 / delta, 1 / (1 + delta), phase_ratio(fl, is)
+    double gr, u2;         // gaml(isp) / gaml(is), the acceptance uniform
+    int isp;
+};
+
+// lane l's complex value as a wave-uniform (scalar) operand; l must be wave-uniform
+__device__ __forceinline__ cplx readlane_c(cplx v, int l)
+{
+    auto rd = [l](double x) {
+        const long long b = __double_as_longlong(x);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+        return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    };
+    return make_double2(rd(v.x), rd(v.y));
+}
+
 // submatrix update of one slice (src/dqmc_config_u/dqmc_update_submatrix.f90:55-162).
-// Gcuta is kept implicitly as G diag(cs); the flush writes (G + (G(:,p) Gamma^-1) G(p,:)) diag(cs).
-// Scratch: avec = G(:,p) columns, bvec = Gmat1, W1 = G(p,:)^T, rwork = cs.
+// Gcuta is kept implicitly as G diag(cs); the flush writes (G + (G(:,p) Gamma^-1) G(p,:)) diag(cs),
+// where cs differs from 1 only at the accepted sites p.
+// G does not change between flushes, so the columns and rows of G at the next dss sites are
+// staged in LDS (as in the delay kernel): G(s, p), G(p, s), G(s, s) and the column and row of G
+// an accepted site records come from LDS.  Per site, v1 = G(s, p) Gamma^-1 and
+// v4 = Gamma^-1 G(p, s) are formed before the decision, each by two waves over halves of p
+// (partial sums in LDS, double-buffered by site parity so that no barrier separates one site's
+// reads from the next site's writes); the decision needs v = v1 . G(p, s) only.
+// The site loop itself reads and writes no global memory (its draws are staged with the block
+// and the accepted sites' fields, columns and rows are taken at the flush).
+// Scratch: avec = G(:,p) columns, W1 = G(p,:)^T, bvec = Gmat1, W2 = Gamma^-1 (ld ik).
+// LDS: Gamma^-1 (K (K+1)), the K column scales, dss columns and dss rows of G.
 extern "C" __global__ void __launch_bounds__(256)
-k_update_submatrix(const DChain *chains, int nt, int slice_in_pass)
+k_update_submatrix(const DChain *chains, int nt, int slice_in_pass, int dss)
 {
     extern __shared__ cplx sh[];
     const DChain &c = chains[blockIdx.x];
     const int n = c.n, K = c.nublock, KL = K + 1;
+    dss = min(dss, 8);                // the staged draws hold 8 sites
     cplx *gi = sh;                    // Gamma^-1, column-major, leading dimension K + 1 (lanes reading
                                       // one entry of K columns each fall on different banks)
-    cplx *cv = gi + KL * K;           // K
-    cplx *bv = cv + K;                // K
-    cplx *v1 = bv + K;                // K
-    cplx *v4 = v1 + K;                // K
+    cplx *csk = gi + KL * K;          // K: cs of the accepted sites
+    cplx *gcol = csk + K;             // dss x n: G(:, s0 + j)
+    cplx *grow = gcol + (size_t)dss * n;  // dss x n: G(s0 + j, :)
     __shared__ int pvec[256];
-    __shared__ cplx red[256];
-    __shared__ cplx s_gss[2];         // G(s, s), by the parity of s
+    __shared__ cplx part[2][256];     // [parity][wave * 64 + i]: v1 halves (waves 0, 1), v4 halves (2, 3)
+    __shared__ cplx vsum[2][2];       // [parity][wave]: v halves
+    __shared__ HsTab tab;
+    __shared__ SubSite sfac[8];       // the draws of the staged sites
+    __shared__ int ispk[64];          // the new HS field of the accepted sites, written at the flush
     cplx *G = c.G, *gm1 = c.avec, *gmat1 = c.bvec, *gm2t = c.W1;
-    double *csr = c.rwork, *csi = c.rwork + n;
+    GAS cplx *Gg = gp(c.G);
     int32_t *cf = c.conf + (size_t)(nt - 1) * n;
-    const double *u = c.unif + (size_t)2 * n * slice_in_pass;
+    GAS const int32_t *cfg = gp(cf);
+    GAS const double *ug = gp(c.unif + (size_t)2 * n * slice_in_pass);
     cplx phase = make_double2(c.sc[SC_PHASE], c.sc[SC_PHASE + 1]);
-    for (int r = threadIdx.x; r < n; r += blockDim.x) { csr[r] = 1.0; csi[r] = 0.0; }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    hs_tab_load(tab, c);
     __syncthreads();
-    int ik = 0;
+    // every load issued so far has landed: without this the compiler cannot tell, inside the
+    // site loop, that the phase loaded above is no longer in flight, and waits for all
+    // outstanding memory operations (the accepted sites' stores) at each use
+    __builtin_amdgcn_s_waitcnt(0);
+    int ik = 0, s0 = -1;
     double accm = 0.0;
     for (int s = 0; s < n; s++) {
-        const SiteDraw d = site_draw(c, cf[s], u[2 * s]);
-        for (int i = threadIdx.x; i < ik; i += blockDim.x) {
-            cv[i] = G[(size_t)s + (size_t)pvec[i] * n];
-            bv[i] = G[(size_t)pvec[i] + (size_t)s * n];
-        }
-        if (threadIdx.x == blockDim.x - 1) s_gss[s & 1] = G[(size_t)s + (size_t)s * n];
-        __syncthreads();
-        // v1 = G(s, p) Gamma^-1 (thread i: entry i) and v = v1 . G(p, s), summed by wave 0
-        // (ik <= K <= 64 entries)
-        if (threadIdx.x < 64) {
-            const int i = threadIdx.x;
-            cplx part = make_double2(0.0, 0.0);
-            if (i < ik) {
-                cplx a = make_double2(0.0, 0.0);
-                for (int m = 0; m < ik; m++) a = cadd(a, cmul(cv[m], gi[m + i * KL]));
-                v1[i] = a;
-                part = cmul(a, bv[i]);
+        if (s0 < 0 || s >= s0 + dss) {
+            // stage G(:, s..s+nb) (one contiguous run) and G(s..s+nb, :) (nb-element runs);
+            // valid until the next flush changes G.  A lane may still read the previous block.
+            if (s0 >= 0) __syncthreads();
+            s0 = s;
+            const int nb = min(dss, n - s), tot = nb * n;
+            GAS cplx *Gc = Gg + (size_t)s * n;
+            // the sites' draws come with the block (thread t: site s + t), so that the site loop
+            // itself waits on no global load (a wait there would also wait for stores in flight)
+            int cfv = 1;
+            double u1v = 0.5, u2v = 0.0;
+            if (threadIdx.x < nb) {
+                cfv = cfg[s + threadIdx.x];
+                u1v = ug[2 * (s + threadIdx.x)];
+                u2v = ug[2 * (s + threadIdx.x) + 1];
             }
-            part.x = wave_sum(part.x);
-            part.y = wave_sum(part.y);
-            if (i == 0) red[0] = part;
+            // 8 column and 8 row elements per thread in flight (clamped addresses)
+            for (int base = threadIdx.x; base < tot; base += 8 * (int)blockDim.x) {
+                cplx a[8], b[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int e = min(base + u * (int)blockDim.x, tot - 1);
+                    a[u] = gld(Gc + e);
+                    b[u] = gld(Gg + ((size_t)(s + e % nb) + (size_t)(e / nb) * n));
+                }
+                if (base == (int)threadIdx.x && threadIdx.x < nb) {
+                    const SiteDraw dd = site_draw(tab, cfv, u1v);
+                    SubSite q;
+                    q.db = dd.db;
+                    q.idb = cinv(dd.db);
+                    q.f = cdiv(make_double2(1.0, 0.0), cadd(make_double2(1.0, 0.0), dd.db));
+                    q.pr = tab.phase_ratio[(dd.fl - 1) + (tab.lcomp - 1) * (dd.is - 1)];
+                    q.gr = tab.gaml[dd.isp - 1] / tab.gaml[dd.is - 1];
+                    q.u2 = u2v;
+                    q.isp = dd.isp;
+                    sfac[threadIdx.x] = q;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int e = base + u * (int)blockDim.x;
+                    if (e < tot) {
+                        gcol[e] = a[u];
+                        grow[(e % nb) * n + e / nb] = b[u];
+                    }
+                }
+            }
+            __syncthreads();
         }
+        const cplx *gcs = gcol + (size_t)(s - s0) * n, *grs = grow + (size_t)(s - s0) * n;
+        const int par = s & 1;
+        const SubSite q = sfac[s - s0];
+        {
+            // waves 0, 1: v1(i) = sum_m G(s, p_m) Gamma^-1(m, i); waves 2, 3: v4(i) =
+            // sum_m Gamma^-1(i, m) G(p_m, s); m over [0, h) (even waves) or [h, ik) (odd)
+            // lane j holds G(s, p_j) (v1 waves) or G(p_j, s), broadcast to the wave by readlane;
+            // lanes past ik compute on clamped indices and store nothing
+            const int i = lane, h = (ik + 1) >> 1;
+            const int mb = (w & 1) ? h : 0, me = (w & 1) ? ik : h;
+            const int pj = i < ik ? pvec[i] : s, ic = min(i, K - 1);
+            const cplx bvi = gcs[pj], own = w < 2 ? grs[pj] : bvi;
+            cplx a = make_double2(0.0, 0.0), pv = make_double2(0.0, 0.0);
+            // four LDS reads in flight per step; the sum keeps the order m = mb, mb + 1, ...
+            const int gs = w < 2 ? 1 : KL, go = w < 2 ? ic * KL : ic;   // Gamma^-1 entry (m, ic) or (ic, m)
+            int m = mb;
+            for (; m + 4 <= me; m += 4) {
+                const cplx g0 = gi[go + m * gs], g1 = gi[go + (m + 1) * gs], g2 = gi[go + (m + 2) * gs],
+                           g3 = gi[go + (m + 3) * gs];
+                if (w < 2) {
+                    a = cadd(a, cmul(readlane_c(own, m), g0));
+                    a = cadd(a, cmul(readlane_c(own, m + 1), g1));
+                    a = cadd(a, cmul(readlane_c(own, m + 2), g2));
+                    a = cadd(a, cmul(readlane_c(own, m + 3), g3));
+                } else {
+                    a = cadd(a, cmul(g0, readlane_c(own, m)));
+                    a = cadd(a, cmul(g1, readlane_c(own, m + 1)));
+                    a = cadd(a, cmul(g2, readlane_c(own, m + 2)));
+                    a = cadd(a, cmul(g3, readlane_c(own, m + 3)));
+                }
+            }
+            for (; m < me; m++) {
+                const cplx g = gi[go + m * gs];
+                a = w < 2 ? cadd(a, cmul(readlane_c(own, m), g)) : cadd(a, cmul(g, readlane_c(own, m)));
+            }
+            if (w < 2 && i < ik) pv = cmul(a, bvi);
+            if (i < ik) part[par][w * 64 + i] = a;
+            if (w < 2) {
+                pv.x = wave_sum_dpp(pv.x);
+                pv.y = wave_sum_dpp(pv.y);
+                if (i == 0) vsum[par][w] = pv;
+            }
+        }
+        // G(s, s) is in the staged column; read before the barrier, after which the next
+        // block may be staged
+        const cplx gss = gcs[s];
         __syncthreads();
-        // red[0] and s_gss[s & 1] are rewritten only after the next site's first barrier, which
-        // every thread reaches after these reads
-        cplx v = red[0];
-        cplx gss = s_gss[s & 1];
+        const cplx v = cadd(vsum[par][0], vsum[par][1]);
         cplx v2 = make_double2(v.x + gss.x - 1.0, v.y + gss.y);
-        cplx ratio1 = csub(make_double2(1.0, 0.0), cmul(v2, d.db));
-        cplx rt;
-        bool acc = site_accept(c, ratio1, d, u[2 * s + 1], phase, &rt);
+        cplx ratio1 = csub(make_double2(1.0, 0.0), cmul(v2, q.db));
+        // the decision of site_accept with its draw-only factors precomputed (the same
+        // operations in the same order); every thread evaluates it, and reading it from lane 0
+        // makes the branch scalar
+        cplx rt = make_double2(1.0, 0.0);
+        for (int q = 0; q < tab.nflr; q++) rt = cmul(rt, ratio1);
+        rt = cmul(cscale(rt, q.gr), q.pr);
+        const double ratio_re = cmul(rt, phase).x / phase.x;
+        const bool acc = __builtin_amdgcn_readfirstlane((int)(fabs(ratio_re) > q.u2)) != 0;
         if (acc) {
             accm += 1.0;
             phase = phase_after(phase, rt);
             cplx den = make_double2(-v.x + 1.0 - gss.x, -v.y - gss.y);
-            den = cadd(den, cinv(d.db));
+            den = cadd(den, q.idb);
             cplx v3 = cinv(den);
-            for (int i = threadIdx.x; i < ik; i += blockDim.x) {
-                cplx a = make_double2(0.0, 0.0);
-                for (int m = 0; m < ik; m++) a = cadd(a, cmul(gi[i + m * KL], bv[m]));
-                v4[i] = a;
+            // Gamma^-1 += (v4 v3) v1 and its new border: lane i of every wave holds v4(i) v3 and
+            // v1(i); wave w updates the columns m = w, w + 4, ... with v1(m) broadcast by readlane
+            const cplx *pp = part[par];
+            const int i = lane, ic = min(i, K - 1);
+            const cplx v1own = cadd(pp[ic], pp[64 + ic]);
+            const cplx v4v3 = cmul(cadd(pp[128 + ic], pp[192 + ic]), v3);
+#pragma unroll 4
+            for (int m = w; m < ik; m += 4) {
+                const cplx v1m = readlane_c(v1own, m);
+                if (i < ik) gi[i + m * KL] = cadd(gi[i + m * KL], cmul(v4v3, v1m));
             }
-            __syncthreads();
-            for (int e = threadIdx.x; e < ik * ik; e += blockDim.x) {
-                int i = e % ik, m = e / ik;
-                gi[i + m * KL] = cadd(gi[i + m * KL], cmul(cmul(v4[i], v3), v1[m]));
-            }
-            for (int i = threadIdx.x; i < ik; i += blockDim.x) {
-                gi[ik + i * KL] = cmul(v3, v1[i]);
-                gi[i + ik * KL] = cmul(v4[i], v3);
+            if (w == 0 && i < ik) {
+                gi[ik + i * KL] = cmul(v3, v1own);
+                gi[i + ik * KL] = v4v3;
             }
             if (threadIdx.x == 0) {
                 gi[ik + ik * KL] = v3;
                 pvec[ik] = s;
-                cf[s] = d.isp;
-                cplx f = cdiv(make_double2(1.0, 0.0), cadd(make_double2(1.0, 0.0), d.db));
-                cplx cs = cmul(make_double2(csr[s], csi[s]), f);
-                csr[s] = cs.x; csi[s] = cs.y;
+                ispk[ik] = q.isp;
+                // cs(s) is 1 until its site is accepted (reset at every flush)
+                csk[ik] = cmul(make_double2(1.0, 0.0), q.f);
             }
-            for (int r = threadIdx.x; r < n; r += blockDim.x) gm1[(size_t)r + (size_t)ik * n] = G[(size_t)r + (size_t)s * n];
             ik++;
             __syncthreads();
         }
         if (ik == K || s == n - 1) {
             if (ik > 0) {
                 // Gmat1 = G(:,p) Gamma^-1 as an MFMA product (Gamma^-1 staged through W2, which
-                // the update does not use); gm2t(j, i) = G(p_i, j)
+                // the update does not use)
                 cplx *gig = c.W2;
                 for (int e = threadIdx.x; e < ik * ik; e += blockDim.x) {
                     const int i = e % ik, j = e / ik;
-                    gig[i + (size_t)j * ik] = gi[i + j * KL];  // ld ik: ik^2 <= n^2 entries of W2
+                    gig[i + (size_t)j * ik] = gi[i + j * KL];  // ld ik: ik^2 entries
                 }
-                for (int r = threadIdx.x; r < n; r += blockDim.x)
-                    for (int i = 0; i < ik; i++) gm2t[(size_t)r + (size_t)i * n] = G[(size_t)pvec[i] + (size_t)r * n];
+                if (threadIdx.x < ik) cf[pvec[threadIdx.x]] = ispk[threadIdx.x];
+                // the accepted sites' columns G(:, p) and rows G(p, :) (the site loop itself
+                // touches no global memory), 8 + 8 elements per thread in flight
+                {
+                    const int tot = ik * n;
+                    for (int base = threadIdx.x; base < tot; base += 8 * (int)blockDim.x) {
+                        cplx a[8], b[8];
+#pragma unroll
+                        for (int u = 0; u < 8; u++) {
+                            const int e = min(base + u * (int)blockDim.x, tot - 1), i = e / n, r = e - i * n;
+                            a[u] = gld(Gg + ((size_t)r + (size_t)pvec[i] * n));
+                            b[u] = gld(Gg + ((size_t)pvec[i] + (size_t)r * n));
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; u++) {
+                            const int e = base + u * (int)blockDim.x;
+                            if (e < tot) {
+                                gm1[e] = a[u];      // gm1(r, i) = G(r, p_i), ld n
+                                gm2t[e] = b[u];     // gm2t(r, i) = G(p_i, r), ld n
+                            }
+                        }
+                    }
+                }
                 __syncthreads();
                 for (int tm = 0; tm < n; tm += GT)
                     zgemm_tile<0, 0>(gm1, n, gig, ik, gmat1, n, n, ik, ik, tm, 0, 1.0, 0.0, 0.0, 0.0);
                 __syncthreads();
-                wg_rank_k(G, n, gmat1, gm2t, ik, csr, csi);
-            } else {
-                for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
-                    int col = (int)(e / n);
-                    G[e] = cmul(G[e], make_double2(csr[col], csi[col]));
+                // G += Gmat1 G(p,:): one pass over G up to 48 sites (fragment depth the next
+                // multiple of 16), two passes of up to 32 beyond (a depth-64 fragment would not
+                // fit in registers)
+                if (ik <= 16) {
+                    wg_rank_k_strips<16>(G, n, gmat1, gm2t, ik);
+                } else if (ik <= 32) {
+                    wg_rank_k_strips<32>(G, n, gmat1, gm2t, ik);
+                } else if (ik <= 48) {
+                    wg_rank_k_strips<48>(G, n, gmat1, gm2t, ik);
+                } else {
+                    wg_rank_k_strips<32>(G, n, gmat1, gm2t, 32);
+                    __syncthreads();
+                    wg_rank_k_strips<32>(G, n, gmat1 + (size_t)32 * n, gm2t + (size_t)32 * n, ik - 32);
+                }
+                __syncthreads();
+                // the columns of the accepted sites times their cs, 8 elements per thread in flight
+                const int tot = ik * n;
+                for (int base = threadIdx.x; base < tot; base += 8 * (int)blockDim.x) {
+                    cplx v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int e = min(base + u * (int)blockDim.x, tot - 1), i = e / n;
+                        v[u] = gld(Gg + ((size_t)(e - i * n) + (size_t)pvec[i] * n));
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int e = base + u * (int)blockDim.x, i = e / n;
+                        if (e < tot) gst(Gg + ((size_t)(e - i * n) + (size_t)pvec[i] * n), cmul(v[u], csk[i]));
+                    }
                 }
             }
             __syncthreads();
-            for (int r = threadIdx.x; r < n; r += blockDim.x) { csr[r] = 1.0; csi[r] = 0.0; }
+            // the flush's last prefetches are never consumed; retiring them here keeps the
+            // compiler from waiting on them (and on every store in flight) inside the site loop
+            __builtin_amdgcn_s_waitcnt(0);
             ik = 0;
-            __syncthreads();
+            s0 = -1;                  // G changed: stage again
         }
     }
     if (threadIdx.x == 0) {

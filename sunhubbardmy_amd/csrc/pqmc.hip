@@ -195,9 +195,9 @@ void update_u_g(shp_ctx *ctx, int nt, size_t uoff0)
         const size_t lds = (size_t)(n + 2 * 64) * sizeof(cplx);
         hipLaunchKernelGGL(k_update_delay, dim3(ctx->nch), dim3(256), lds, ctx->st, ctx->dview, nt, slice);
     } else {
-        const int K = ctx->gblock;
-        const size_t lds = (size_t)(K * (K + 1) + 4 * K) * sizeof(cplx);
-        hipLaunchKernelGGL(k_update_submatrix, dim3(ctx->nch), dim3(256), lds, ctx->st, ctx->dview, nt, slice);
+        const int K = ctx->gblock, dss = sub_stage(n, K);
+        hipLaunchKernelGGL(k_update_submatrix, dim3(ctx->nch), dim3(256), lds_sub(n, K, dss), ctx->st, ctx->dview, nt,
+                           slice, dss);
     }
     hipLaunchKernelGGL(kp_u_rescale, dim3((unsigned)(((size_t)n * ctx->ne + 255) / 256), 1, ctx->nch), dim3(256), 0,
                        ctx->st, ctx->dc, nt);
@@ -585,7 +585,7 @@ int shp_create(shp_ctx **out, int nchains, const shp_model *m, int device)
     }
     if (ctx->update == SHP_U_SUBMATRIX_G) {
         const int Kb = std::max(2, ctx->gblock & ~1);
-        const size_t lds = (size_t)(ctx->gblock * (ctx->gblock + 1) + 4 * ctx->gblock) * sizeof(cplx);
+        const size_t lds = SUB_LDS_CAP;   // sub_stage's cap; the attribute is shared per function
         const size_t ldsv = (size_t)(Kb * (Kb + 1) + 8 * Kb) * sizeof(cplx);
         if ((lds > 64 * 1024 &&
              hipFuncSetAttribute((const void *)k_update_submatrix, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=

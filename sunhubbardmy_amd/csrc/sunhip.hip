@@ -578,6 +578,18 @@ static void set_identity(sh_ctx *ctx, int pt)
 
 // register-resident delay update: the template depth covering every chain's nublock, and the
 // thread count 2 * rows; 0 when the register budget does not allow it
+// the submatrix kernel's LDS: Gamma^-1, the K column scales and dss staged columns and rows of G;
+// dss = the largest staging depth <= 8 within SUB_LDS_CAP (>= 1 up to N = 1024 at K = 64)
+// the kernel also holds 44 KiB of static LDS (the MFMA tiles of the flush, the per-site partial
+// sums); static plus dynamic stays within the 160 KiB of a CU
+#define SUB_LDS_CAP ((size_t)112 * 1024)
+static size_t lds_sub(int n, int K, int dss) { return (size_t)(K * (K + 1) + K + 2 * (size_t)dss * n) * sizeof(cplx); }
+static int sub_stage(int n, int K)
+{
+    int dss = 8;
+    while (dss > 1 && lds_sub(n, K, dss) > SUB_LDS_CAP) dss--;
+    return dss;
+}
 static int delay_reg_depth(sh_ctx *ctx, int *threads)
 {
     int kmax = 0;
@@ -612,8 +624,9 @@ static void update_slice(sh_ctx *ctx, int nt, int slice_in_pass)
     } else {
         int K = 0;
         for (auto &c : ctx->hc) K = std::max(K, c.nublock);
-        const size_t lds = (size_t)(K * (K + 1) + 4 * K) * sizeof(cplx);
-        hipLaunchKernelGGL(k_update_submatrix, dim3(ctx->nch), dim3(256), lds, ctx->st, ctx->dc, nt, slice_in_pass);
+        const int dss = sub_stage(ctx->nmax, K);
+        hipLaunchKernelGGL(k_update_submatrix, dim3(ctx->nch), dim3(256), lds_sub(ctx->nmax, K, dss), ctx->st, ctx->dc, nt,
+                           slice_in_pass, dss);
     }
 }
 
@@ -740,7 +753,7 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
             // of one plaquette stay in LDS
             bool ok = (q.zplq == 4 || q.zplq == 6) && q.lqp >= 1 && q.plq_cord && q.plq_b && q.plq_bi &&
                       q.plq_delta && q.plq_pratio && q.plq_parg && q.plq_gaml &&
-                      (size_t)2 * q.zplq * q.ndim * sizeof(cplx) <= 150 * 1024;
+                      (size_t)2 * q.zplq * q.ndim * sizeof(cplx) <= 128 * 1024;
             for (int p = 0; ok && p < q.lqp; p++)
                 for (int a = 0; a < q.zplq && ok; a++) {
                     const int sa = q.plq_cord[p * q.zplq + a];
@@ -749,7 +762,7 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
                 }
             if (!ok) {
                 int rc = fail(ctx, SH_E_ARG, "plaquette field: zplq 4 or 6, plaquettes of distinct sites in 0..ndim-1, "
-                                             "every table given, 2 zplq ndim complex within 150 KiB");
+                                             "every table given, 2 zplq ndim complex within 128 KiB");
                 delete ctx;
                 return rc;
             }
@@ -777,7 +790,9 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
         struct { const void *f; size_t bytes; } big[] = {
             {(const void *)k_cb_rows, lds_rows(ctx)}, {(const void *)k_cb_cols, lds_cols(ctx)},
             {(const void *)k_fft_rows, lds_fft(ctx)}, {(const void *)k_fft_cols, lds_fft(ctx)},
-            {(const void *)k_update_submatrix, (size_t)(SH_KMAX * (SH_KMAX + 1) + 4 * SH_KMAX) * sizeof(cplx)},
+            // the cap sub_stage works to: the attribute is per function, shared with the
+            // projective library's launches of the same kernel
+            {(const void *)k_update_submatrix, SUB_LDS_CAP},
             {(const void *)k_vupdate_submatrix, (size_t)(SH_KMAX * (SH_KMAX + 1) + 8 * SH_KMAX) * sizeof(cplx)},
             {(const void *)k_update_delay_reg<4>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 16 * sizeof(cplx)},
             {(const void *)k_update_delay_reg<8>, (size_t)(1 + 2 * DSB) * ctx->nmax * sizeof(cplx) + ctx->nmax * 20 + 32 * sizeof(cplx)},
@@ -790,6 +805,13 @@ int sh_create(sh_ctx **out, int nchains, const sh_model *m, int device)
             {(const void *)k_solve_blk, blocked(ctx) ? lds_blk(ctx) : 0}};
         for (auto &b : big) {
             if (b.bytes <= 64 * 1024 || b.bytes > 150 * 1024) continue;
+            hipFuncAttributes fa;
+            if (hipFuncGetAttributes(&fa, b.f) != hipSuccess || fa.sharedSizeBytes + b.bytes > 160 * 1024) {
+                (void)hipGetLastError();
+                int rc = fail(ctx, SH_E_HIP, "static plus dynamic LDS of a kernel above the 160 KiB of a CU");
+                sh_destroy(ctx);
+                return rc;
+            }
             if (hipFuncSetAttribute(b.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.bytes) != hipSuccess) {
                 (void)hipGetLastError();
                 int rc = fail(ctx, SH_E_HIP, "dynamic LDS request above the device limit");

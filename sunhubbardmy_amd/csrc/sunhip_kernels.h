@@ -171,6 +171,38 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// the same sum without the LDS crossbar (needs all 64 lanes active): in-quad DPP exchanges
+// (lane ^ 1, lane ^ 2), the half-row and row mirrors, then the 16- and 32-lane swaps of gfx950.
+// Each step adds two partial sums of the same shape, so every lane ends with the same value.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false),
+                            __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double wave_sum_dpp(double v)
+{
+    v += dpp_d<0xB1>(v);     // quad_perm [1, 0, 3, 2]
+    v += dpp_d<0x4E>(v);     // quad_perm [2, 3, 0, 1]
+    v += dpp_d<0x141>(v);    // row_half_mirror
+    v += dpp_d<0x140>(v);    // row_mirror
+    const bool odd16 = (threadIdx.x & 16) != 0, odd32 = (threadIdx.x & 32) != 0;
+    {
+        const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+        const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        v += __hiloint2double((int)(odd16 ? b[0] : b[1]), (int)(odd16 ? a[0] : a[1]));
+    }
+    {
+        const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+        const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        v += __hiloint2double((int)(odd32 ? b[0] : b[1]), (int)(odd32 ? a[0] : a[1]));
+    }
+    return v;
+}
+
 __device__ __forceinline__ cplx *chain_mat(const DChain &c, int sel, int pt)
 {
     size_t nn = (size_t)c.n * c.n;
