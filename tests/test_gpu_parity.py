@@ -33,6 +33,10 @@ CASES = {
     "cubic4_delay_checkerboard": dict(lattice="cubic", la=4, lb=4, lc=4, beta=1.0, nublock=8),
     "cubic4_submatrix_dense": dict(lattice="cubic", la=4, lb=4, lc=4, beta=0.8, kmode="dense", update="submatrix",
                                    nublock=8),
+    # N = 400 at depth 64: the submatrix kernel stages 3 sites per block (LDS-bound depth) and
+    # its flush runs as two passes of 32 (acceptance ~0.8: about 5 full flushes per slice)
+    "square20_submatrix64_dense": dict(la=20, lb=20, beta=0.2, kmode="dense", update="submatrix", nublock=64,
+                                       nwrap=2),
 }
 
 
