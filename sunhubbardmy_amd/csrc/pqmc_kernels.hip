@@ -336,7 +336,10 @@ kp_stock_gather(const PChain *chains, int nf, int ib0, int nup)
 // the stock's bonds in order, one workgroup per chain: 2x2 ratio det(I + Vcal D) with
 // Vcal = PFP_kk - PFP_kik Gamma^-1 PFP_ikk (:146-175), Metropolis on Re(ratio phase)/Re(phase),
 // and on acceptance the bordering of Gamma^-1 by one 2x2 block (:176-222).  Felms and Gamma^-1
-// live in LDS (2 ns^2 complex).
+// live in LDS (2 ns^2 complex).  The stock's field, acceptance uniforms and bond sites are staged
+// in LDS and the field, xvec and Dvec written back at the end, so that the bond loop itself
+// touches no global memory (a global access there would put a memory round trip on every
+// bond's critical path).
 extern "C" __global__ void __launch_bounds__(256)
 kp_stock_bonds(const PChain *chains, int nt, int nf, int ib0, int nb, int uoff)
 {
@@ -350,18 +353,29 @@ kp_stock_bonds(const PChain *chains, int nt, int nf, int ib0, int nb, int uoff)
     __shared__ int s_acc, s_ik;
     __shared__ double s_accm;
     __shared__ cplx s_phase;
+    __shared__ int cfl[32], bsite[64], xv[64];   // field and sites of the stock's nb <= 32 bonds
+    __shared__ double ua[32];                    // their acceptance uniforms
+    __shared__ cplx vdl[4], dv[64];
+    int32_t *cf = c.conf + ((size_t)(nt - 1) * c.nn_nf + nf) * c.n + ib0;
+    const double *u = c.unif + uoff;
     for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
         F[e] = c.Felms[e];
         Gi[e] = make_double2(0.0, 0.0);
     }
+    if (threadIdx.x < nb) {
+        const int q = threadIdx.x, ist = nf * c.nn_lf + ib0 + q;
+        cfl[q] = cf[q];
+        ua[q] = u[2 * q + 1];
+        bsite[2 * q] = c.bonds[2 * ist];
+        bsite[2 * q + 1] = c.bonds[2 * ist + 1];
+    }
+    if (threadIdx.x < 4) vdl[threadIdx.x] = c.vd[threadIdx.x];
     if (threadIdx.x == 0) {
         s_ik = 0;
         s_accm = 0.0;
         s_phase = make_double2(c.sc[SC_PHASE], c.sc[SC_PHASE + 1]);
     }
     __syncthreads();
-    int32_t *cf = c.conf + ((size_t)(nt - 1) * c.nn_nf + nf) * c.n + ib0;
-    const double *u = c.unif + uoff;
     for (int q = 0; q < nb; q++) {
         const int ik = s_ik;
         const int s1 = 2 * q, s2 = s1 + 1;
@@ -372,14 +386,28 @@ kp_stock_bonds(const PChain *chains, int nt, int nf, int ib0, int nb, int uoff)
                 const int m = o % ik, col = o / ik;
                 const int sc_ = col ? s2 : s1;
                 double ar = 0.0, ai = 0.0;
-                if (o < 2 * ik)
-                    for (int p = h; p < ik; p += 2) {
+                if (o < 2 * ik) {
+                    // four terms' reads issued together, the sum in the order p = h, h + 2, ...
+                    int p = h;
+                    for (; p + 6 < ik; p += 8) {
+                        const int q0 = svec[p], q1 = svec[p + 2], q2 = svec[p + 4], q3 = svec[p + 6];
+                        const cplx g0 = Gi[m + p * ns], g1 = Gi[m + (p + 2) * ns], g2 = Gi[m + (p + 4) * ns],
+                                   g3 = Gi[m + (p + 6) * ns];
+                        const cplx f0 = F[q0 + sc_ * ns], f1 = F[q1 + sc_ * ns], f2 = F[q2 + sc_ * ns],
+                                   f3 = F[q3 + sc_ * ns];
+                        ar += g0.x * f0.x - g0.y * f0.y; ai += g0.x * f0.y + g0.y * f0.x;
+                        ar += g1.x * f1.x - g1.y * f1.y; ai += g1.x * f1.y + g1.y * f1.x;
+                        ar += g2.x * f2.x - g2.y * f2.y; ai += g2.x * f2.y + g2.y * f2.x;
+                        ar += g3.x * f3.x - g3.y * f3.y; ai += g3.x * f3.y + g3.y * f3.x;
+                    }
+                    for (; p < ik; p += 2) {
                         const cplx g = Gi[m + p * ns], f = F[svec[p] + sc_ * ns];
                         ar += g.x * f.x - g.y * f.y;
                         ai += g.x * f.y + g.y * f.x;
                     }
-                ar += __shfl_xor(ar, 1, 64);
-                ai += __shfl_xor(ai, 1, 64);
+                }
+                ar += dpp_d<0xB1>(ar);   // the partner lane (lane ^ 1)
+                ai += dpp_d<0xB1>(ai);
                 if (o < 2 * ik && h == 0) ikk[m][col] = make_double2(ar, ai);
             }
             __syncthreads();
@@ -395,8 +423,8 @@ kp_stock_bonds(const PChain *chains, int nt, int nf, int ib0, int nb, int uoff)
                         ar = f.x * x.x - f.y * x.y;
                         ai = f.x * x.y + f.y * x.x;
                     }
-                    ar = wave_sum(ar);
-                    ai = wave_sum(ai);
+                    ar = wave_sum_dpp(ar);
+                    ai = wave_sum_dpp(ai);
                     if (m == 0) vuk[r + 2 * col] = make_double2(ar, ai);
                 }
             }
@@ -405,9 +433,9 @@ kp_stock_bonds(const PChain *chains, int nt, int nf, int ib0, int nb, int uoff)
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            const int is = cf[q];
+            const int is = cfl[q];
             // iflip = ceiling(u * (lcomp - 1)) = 1 for the two-valued bond field
-            const cplx dp = c.vd[2 * (is - 1)], dm = c.vd[2 * (is - 1) + 1];
+            const cplx dp = vdl[2 * (is - 1)], dm = vdl[2 * (is - 1) + 1];
             cplx v11 = csub(F[s1 + s1 * ns], vuk[0]), v21 = csub(F[s2 + s1 * ns], vuk[1]);
             cplx v12 = csub(F[s1 + s2 * ns], vuk[2]), v22 = csub(F[s2 + s2 * ns], vuk[3]);
             v11 = cadd(make_double2(1.0, 0.0), cmul(v11, dp));
@@ -418,7 +446,7 @@ kp_stock_bonds(const PChain *chains, int nt, int nf, int ib0, int nb, int uoff)
             const cplx ph = s_phase;
             const double ratio_re = cmul(ratio1, ph).x / ph.x;
             const double ratio_abs = ratio_re < 0.0 ? -ratio_re : ratio_re;
-            const int acc = ratio_abs > u[2 * q + 1];
+            const int acc = ratio_abs > ua[q];
             s_acc = acc;
             if (acc) {
                 s_accm += 1.0;
@@ -428,11 +456,11 @@ kp_stock_bonds(const PChain *chains, int nt, int nf, int ib0, int nb, int uoff)
                 gkk[2] = cdiv(cmul(make_double2(-v12.x, -v12.y), dp), ratio1);
                 gkk[1] = cdiv(cmul(make_double2(-v21.x, -v21.y), dm), ratio1);
                 gkk[3] = cdiv(cmul(v11, dm), ratio1);
-                cf[q] = 3 - is;
-                c.xvec[ik] = c.bonds[2 * (nf * c.nn_lf + ib0 + q)];
-                c.xvec[ik + 1] = c.bonds[2 * (nf * c.nn_lf + ib0 + q) + 1];
-                c.Dvec[ik] = dp;
-                c.Dvec[ik + 1] = dm;
+                cfl[q] = 3 - is;
+                xv[ik] = bsite[2 * q];
+                xv[ik + 1] = bsite[2 * q + 1];
+                dv[ik] = dp;
+                dv[ik + 1] = dm;
             }
         }
         __syncthreads();
@@ -444,14 +472,26 @@ kp_stock_bonds(const PChain *chains, int nt, int nf, int ib0, int nb, int uoff)
                 const int r = o % 2, p = o / 2;
                 const int sr = r ? s2 : s1;
                 double ar = 0.0, ai = 0.0;
-                if (o < 2 * ik)
-                    for (int m = h; m < ik; m += 2) {
+                if (o < 2 * ik) {
+                    int m = h;
+                    for (; m + 6 < ik; m += 8) {
+                        const int q0 = svec[m], q1 = svec[m + 2], q2 = svec[m + 4], q3 = svec[m + 6];
+                        const cplx f0 = F[sr + q0 * ns], f1 = F[sr + q1 * ns], f2 = F[sr + q2 * ns], f3 = F[sr + q3 * ns];
+                        const cplx g0 = Gi[m + p * ns], g1 = Gi[(m + 2) + p * ns], g2 = Gi[(m + 4) + p * ns],
+                                   g3 = Gi[(m + 6) + p * ns];
+                        ar += f0.x * g0.x - f0.y * g0.y; ai += f0.x * g0.y + f0.y * g0.x;
+                        ar += f1.x * g1.x - f1.y * g1.y; ai += f1.x * g1.y + f1.y * g1.x;
+                        ar += f2.x * g2.x - f2.y * g2.y; ai += f2.x * g2.y + f2.y * g2.x;
+                        ar += f3.x * g3.x - f3.y * g3.y; ai += f3.x * g3.y + f3.y * g3.x;
+                    }
+                    for (; m < ik; m += 2) {
                         const cplx f = F[sr + svec[m] * ns], g = Gi[m + p * ns];
                         ar += f.x * g.x - f.y * g.y;
                         ai += f.x * g.y + f.y * g.x;
                     }
-                ar += __shfl_xor(ar, 1, 64);
-                ai += __shfl_xor(ai, 1, 64);
+                }
+                ar += dpp_d<0xB1>(ar);
+                ai += dpp_d<0xB1>(ai);
                 if (o < 2 * ik && h == 0) kik[r][p] = make_double2(ar, ai);
             }
             __syncthreads();
@@ -484,9 +524,15 @@ kp_stock_bonds(const PChain *chains, int nt, int nf, int ib0, int nb, int uoff)
         }
         __syncthreads();
     }
-    // Gamma^-1, the accepted rows of Rrows (Vmat) and columns of ulP (Umat) for the flush
+    // Gamma^-1, the accepted rows of Rrows (Vmat) and columns of ulP (Umat) for the flush; the
+    // field, the accepted sites and their deltas
     const int ik = s_ik;
     for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) c.Ginv[e] = Gi[e];
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) cf[q] = cfl[q];
+    for (int k = threadIdx.x; k < ik; k += blockDim.x) {
+        c.xvec[k] = xv[k];
+        c.Dvec[k] = dv[k];
+    }
     for (int e = threadIdx.x; e < ik * ne; e += blockDim.x) {
         const int m = e % ik, l = e / ik;
         c.Vmat[m + (size_t)l * ns] = c.Rrows[svec[m] + (size_t)l * ns];
@@ -506,7 +552,9 @@ kp_stock_bonds(const PChain *chains, int nt, int nf, int ib0, int nb, int uoff)
 // (src/dqmc_config_u/dqmc_proj_update_submatrix_LR.f90:83-237): per site the 1 x 1 ratio
 // 1 + (F_kk - F_kP Gamma^-1 F_Pk) Delta(iflip, is), ratiotot = ratio1^nflr gaml(isp)/gaml(is)
 // phase_ratio(iflip, is), the Metropolis test on Re(ratiotot phase)/Re(phase), and on
-// acceptance Gamma^-1 bordered by one row and column.  Felms and Gamma^-1 live in LDS.
+// acceptance Gamma^-1 bordered by one row and column.  Felms and Gamma^-1 live in LDS, and so do
+// the stock's field, draws and HS tables (the field, xvec and Dvec are written back at the end):
+// the site loop itself touches no global memory.
 extern "C" __global__ void __launch_bounds__(256)
 kp_stock_sites(const PChain *chains, int nt, int i0, int nup, size_t uoff)
 {
@@ -520,18 +568,32 @@ kp_stock_sites(const PChain *chains, int nt, int i0, int nup, size_t uoff)
     __shared__ int s_acc, s_ik;
     __shared__ double s_accm;
     __shared__ cplx s_phase;
+    __shared__ int cfl[64], xv[64];              // field of the stock's nup <= 64 sites, accepted sites
+    __shared__ double u1l[64], u2l[64], ugl[4];  // their two uniforms; gaml
+    __shared__ cplx udl[12], url[12], dv[64];    // delta_bmat, phase_ratio; accepted deltas
+    int32_t *cf = c.uconf + (size_t)(nt - 1) * c.n + i0;
+    const double *u = c.unif + uoff;
+    const int nflr = c.nflr;
     for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
         F[e] = c.Felms[e];
         Gi[e] = make_double2(0.0, 0.0);
     }
+    if (threadIdx.x < nup) {
+        cfl[threadIdx.x] = cf[threadIdx.x];
+        u1l[threadIdx.x] = u[2 * threadIdx.x];
+        u2l[threadIdx.x] = u[2 * threadIdx.x + 1];
+    }
+    if (threadIdx.x < 12) {
+        udl[threadIdx.x] = c.ud[threadIdx.x];
+        url[threadIdx.x] = c.uratio[threadIdx.x];
+    }
+    if (threadIdx.x < 4) ugl[threadIdx.x] = c.ug[threadIdx.x];
     if (threadIdx.x == 0) {
         s_ik = 0;
         s_accm = 0.0;
         s_phase = make_double2(c.sc[SC_PHASE], c.sc[SC_PHASE + 1]);
     }
     __syncthreads();
-    int32_t *cf = c.uconf + (size_t)(nt - 1) * c.n + i0;
-    const double *u = c.unif + uoff;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int q = 0; q < nup; q++) {
         const int ik = s_ik;
@@ -540,16 +602,29 @@ kp_stock_sites(const PChain *chains, int nt, int i0, int nup, size_t uoff)
             {
                 const int m = threadIdx.x >> 2, h = threadIdx.x & 3;
                 double ar = 0.0, ai = 0.0;
-                if (m < ik)
-                    for (int p = h; p < ik; p += 4) {
+                if (m < ik) {
+                    // four terms' reads issued together, the sum in the order p = h, h + 4, ...
+                    int p = h;
+                    for (; p + 12 < ik; p += 16) {
+                        const int q0 = svec[p], q1 = svec[p + 4], q2 = svec[p + 8], q3 = svec[p + 12];
+                        const cplx g0 = Gi[m + p * ns], g1 = Gi[m + (p + 4) * ns], g2 = Gi[m + (p + 8) * ns],
+                                   g3 = Gi[m + (p + 12) * ns];
+                        const cplx f0 = F[q0 + q * ns], f1 = F[q1 + q * ns], f2 = F[q2 + q * ns], f3 = F[q3 + q * ns];
+                        ar += g0.x * f0.x - g0.y * f0.y; ai += g0.x * f0.y + g0.y * f0.x;
+                        ar += g1.x * f1.x - g1.y * f1.y; ai += g1.x * f1.y + g1.y * f1.x;
+                        ar += g2.x * f2.x - g2.y * f2.y; ai += g2.x * f2.y + g2.y * f2.x;
+                        ar += g3.x * f3.x - g3.y * f3.y; ai += g3.x * f3.y + g3.y * f3.x;
+                    }
+                    for (; p < ik; p += 4) {
                         const cplx g = Gi[m + p * ns], f = F[svec[p] + q * ns];
                         ar += g.x * f.x - g.y * f.y;
                         ai += g.x * f.y + g.y * f.x;
                     }
-                ar += __shfl_xor(ar, 1, 64);
-                ai += __shfl_xor(ai, 1, 64);
-                ar += __shfl_xor(ar, 2, 64);
-                ai += __shfl_xor(ai, 2, 64);
+                }
+                ar += dpp_d<0xB1>(ar);   // lane ^ 1, then lane ^ 2 (in-quad DPP)
+                ai += dpp_d<0xB1>(ai);
+                ar += dpp_d<0x4E>(ar);
+                ai += dpp_d<0x4E>(ai);
                 if (m < ik && h == 0) ikk[m] = make_double2(ar, ai);
             }
             __syncthreads();
@@ -561,8 +636,8 @@ kp_stock_sites(const PChain *chains, int nt, int i0, int nup, size_t uoff)
                     ar = f.x * x.x - f.y * x.y;
                     ai = f.x * x.y + f.y * x.x;
                 }
-                ar = wave_sum(ar);
-                ai = wave_sum(ai);
+                ar = wave_sum_dpp(ar);
+                ai = wave_sum_dpp(ai);
                 if (lane == 0) s_vuk = make_double2(ar, ai);
             }
         } else if (threadIdx.x == 0) {
@@ -570,28 +645,28 @@ kp_stock_sites(const PChain *chains, int nt, int i0, int nup, size_t uoff)
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            const int is = cf[q];
-            const int iflip = (int)ceil(u[2 * q] * (lc - 1));
+            const int is = cfl[q];
+            const int iflip = (int)ceil(u1l[q] * (lc - 1));
             const int isp = (is + iflip - 1) % lc + 1;
             const int k = (iflip - 1) + (lc - 1) * (is - 1);
-            const cplx d = c.ud[k];
+            const cplx d = udl[k];
             const cplx ratio1 = cadd(make_double2(1.0, 0.0), cmul(csub(F[q + q * ns], s_vuk), d));
             cplx rt = ratio1;
-            for (int f = 1; f < c.nflr; f++) rt = cmul(rt, ratio1);
-            rt = cmul(cscale(rt, c.ug[isp - 1] / c.ug[is - 1]), c.uratio[k]);
+            for (int f = 1; f < nflr; f++) rt = cmul(rt, ratio1);
+            rt = cmul(cscale(rt, ugl[isp - 1] / ugl[is - 1]), url[k]);
             const cplx ph = s_phase;
             const double ratio_re = cmul(rt, ph).x / ph.x;
             const double ratio_abs = ratio_re < 0.0 ? -ratio_re : ratio_re;
-            const int acc = ratio_abs > u[2 * q + 1];
+            const int acc = ratio_abs > u2l[q];
             s_acc = acc;
             if (acc) {
                 s_accm += 1.0;
                 const double wgt = sqrt(cmul(rt, cconj(rt)).x);
                 s_phase = cscale(cmul(ph, rt), 1.0 / wgt);
                 s_gkk = cdiv(d, ratio1);
-                cf[q] = isp;
-                c.xvec[ik] = i0 + q;
-                c.Dvec[ik] = d;
+                cfl[q] = isp;
+                xv[ik] = i0 + q;
+                dv[ik] = d;
             }
         }
         __syncthreads();
@@ -602,16 +677,28 @@ kp_stock_sites(const PChain *chains, int nt, int i0, int nup, size_t uoff)
             {
                 const int p = threadIdx.x >> 2, h = threadIdx.x & 3;
                 double ar = 0.0, ai = 0.0;
-                if (p < ik)
-                    for (int m = h; m < ik; m += 4) {
+                if (p < ik) {
+                    int m = h;
+                    for (; m + 12 < ik; m += 16) {
+                        const int q0 = svec[m], q1 = svec[m + 4], q2 = svec[m + 8], q3 = svec[m + 12];
+                        const cplx f0 = F[q + q0 * ns], f1 = F[q + q1 * ns], f2 = F[q + q2 * ns], f3 = F[q + q3 * ns];
+                        const cplx g0 = Gi[m + p * ns], g1 = Gi[(m + 4) + p * ns], g2 = Gi[(m + 8) + p * ns],
+                                   g3 = Gi[(m + 12) + p * ns];
+                        ar += f0.x * g0.x - f0.y * g0.y; ai += f0.x * g0.y + f0.y * g0.x;
+                        ar += f1.x * g1.x - f1.y * g1.y; ai += f1.x * g1.y + f1.y * g1.x;
+                        ar += f2.x * g2.x - f2.y * g2.y; ai += f2.x * g2.y + f2.y * g2.x;
+                        ar += f3.x * g3.x - f3.y * g3.y; ai += f3.x * g3.y + f3.y * g3.x;
+                    }
+                    for (; m < ik; m += 4) {
                         const cplx f = F[q + svec[m] * ns], g = Gi[m + p * ns];
                         ar += f.x * g.x - f.y * g.y;
                         ai += f.x * g.y + f.y * g.x;
                     }
-                ar += __shfl_xor(ar, 1, 64);
-                ai += __shfl_xor(ai, 1, 64);
-                ar += __shfl_xor(ar, 2, 64);
-                ai += __shfl_xor(ai, 2, 64);
+                }
+                ar += dpp_d<0xB1>(ar);
+                ai += dpp_d<0xB1>(ai);
+                ar += dpp_d<0x4E>(ar);
+                ai += dpp_d<0x4E>(ai);
                 if (p < ik && h == 0) kik[p] = make_double2(ar, ai);
             }
             __syncthreads();
@@ -636,6 +723,11 @@ kp_stock_sites(const PChain *chains, int nt, int i0, int nup, size_t uoff)
     }
     const int ik = s_ik;
     for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) c.Ginv[e] = Gi[e];
+    for (int q = threadIdx.x; q < nup; q += blockDim.x) cf[q] = cfl[q];
+    for (int k = threadIdx.x; k < ik; k += blockDim.x) {
+        c.xvec[k] = xv[k];
+        c.Dvec[k] = dv[k];
+    }
     for (int e = threadIdx.x; e < ik * ne; e += blockDim.x) {
         const int m = e % ik, l = e / ik;
         c.Vmat[m + (size_t)l * ns] = c.Rrows[svec[m] + (size_t)l * ns];
