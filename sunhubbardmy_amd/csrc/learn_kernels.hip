@@ -449,7 +449,16 @@ __global__ __launch_bounds__(256) void k_block_move(const DChain *chains, const 
             p.pb[(size_t)col + (size_t)a * n] = cmul(dl[a], make_double2((F[a] == col ? 1.0 : 0.0) - g.x, -g.y));
         }
         __syncthreads();
-        wg_rank_k_strips<SHL_MB>(G, n, p.pa, p.pb, m);
+        // the fragment depth the next multiple of 16 above m: the strips run every k-column of
+        // their template depth, so a depth-64 update of a 13-site block would do 5x its MFMAs
+        if (m <= 16)
+            wg_rank_k_strips<16>(G, n, p.pa, p.pb, m);
+        else if (m <= 32)
+            wg_rank_k_strips<32>(G, n, p.pa, p.pb, m);
+        else if (m <= 48)
+            wg_rank_k_strips<48>(G, n, p.pa, p.pb, m);
+        else
+            wg_rank_k_strips<SHL_MB>(G, n, p.pa, p.pb, m);
         for (int a = t; a < m; a += 256) cf[F[a]] = 3 - isF[a];
     } else {
         for (int a = t; a < m; a += 256) R[n + F[a]] = 2 * isF[a] - 3;
