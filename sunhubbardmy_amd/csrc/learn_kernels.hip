@@ -421,32 +421,53 @@ __global__ __launch_bounds__(256) void k_block_move(const DChain *chains, const 
             }
         }
         __syncthreads();
-        // pa = -G_:F M^-1 (n x m), 16 columns per item
+        // pa = -G_:F M^-1 (n x m), 16 columns per item; the item's m entries of G are loaded
+        // eight at a time ahead of their use (clamped addresses), the sums in the order b = 0, 1, ...
         const int ngrp = (m + 15) / 16;
+        GAS cplx *Gg = gp(G);
         for (int it = t; it < n * ngrp; it += 256) {
             const int row = it % n, g0i = (it / n) * 16;
             double ar[16], ai[16];
 #pragma unroll
             for (int q = 0; q < 16; q++) { ar[q] = 0.0; ai[q] = 0.0; }
-            for (int b = 0; b < m; b++) {
-                const cplx gb = G[(size_t)row + (size_t)F[b] * n];
+            for (int b0 = 0; b0 < m; b0 += 8) {
+                cplx gbv[8];
 #pragma unroll
-                for (int q = 0; q < 16; q++) {
-                    const int col = min(g0i + q, m - 1);
-                    const cplx mi = Mm[b + cols[col] * SHL_MB];
-                    ar[q] += gb.x * mi.x - gb.y * mi.y;
-                    ai[q] += gb.x * mi.y + gb.y * mi.x;
+                for (int u = 0; u < 8; u++) gbv[u] = gld(Gg + ((size_t)row + (size_t)F[min(b0 + u, m - 1)] * n));
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int b = b0 + u;
+                    if (b < m) {
+                        const cplx gb = gbv[u];
+#pragma unroll
+                        for (int q = 0; q < 16; q++) {
+                            const int col = min(g0i + q, m - 1);
+                            const cplx mi = Mm[b + cols[col] * SHL_MB];
+                            ar[q] += gb.x * mi.x - gb.y * mi.y;
+                            ai[q] += gb.x * mi.y + gb.y * mi.x;
+                        }
+                    }
                 }
             }
 #pragma unroll
             for (int q = 0; q < 16; q++)
                 if (g0i + q < m) p.pa[(size_t)row + (size_t)(g0i + q) * n] = make_double2(-ar[q], -ai[q]);
         }
-        // pb(col, a) = Delta_a (delta(F_a, col) - G(F_a, col))
-        for (int e = t; e < n * m; e += 256) {
-            const int a = e / n, col = e % n;
-            const cplx g = G[(size_t)F[a] + (size_t)col * n];
-            p.pb[(size_t)col + (size_t)a * n] = cmul(dl[a], make_double2((F[a] == col ? 1.0 : 0.0) - g.x, -g.y));
+        // pb(col, a) = Delta_a (delta(F_a, col) - G(F_a, col)), eight rows' entries in flight per thread
+        for (int base = t; base < n * m; base += 8 * 256) {
+            cplx gv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int e = min(base + u * 256, n * m - 1), a = e / n, col = e - a * n;
+                gv[u] = gld(Gg + ((size_t)F[a] + (size_t)col * n));
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int e = base + u * 256, a = e / n, col = e - a * n;
+                if (e < n * m)
+                    p.pb[(size_t)col + (size_t)a * n] =
+                        cmul(dl[a], make_double2((F[a] == col ? 1.0 : 0.0) - gv[u].x, -gv[u].y));
+            }
         }
         __syncthreads();
         // the fragment depth the next multiple of 16 above m: the strips run every k-column of
