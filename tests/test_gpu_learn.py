@@ -163,6 +163,45 @@ def test_block_move_parity(name):
     net.close()
 
 
+@pytest.mark.parametrize("la,nprop,beta,min_block", [(8, 40, 1.0, 17), (12, 64, 1.0, 33), (20, 64, 0.5, 49)])
+def test_block_move_parity_large_blocks(la, nprop, beta, min_block):
+    """Accepted blocks of 17-64 sites, so that every depth of the rank-|F| update of G (16, 32,
+    48, 64) is taken: a network with all weights zero accepts every inner attempt, and at
+    U = 0.5 most blocks pass the exact test.  Field bit-identical with the oracle after every
+    pass, G within 1e-10, identical proposal statistics."""
+    kw = dict(lattice="square", la=la, lb=la, beta=beta, dtau=0.1, rhub=0.5, kmode="checkerboard", nwrap=5,
+              update="delay", nublock=8)
+    seed = chain_seed(2)
+    from sunhubbardmy_amd.learned import NWEIGHTS
+    wts = np.zeros(NWEIGHTS, np.float32)
+    norm = np.array([0.0, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 1.0, 0.0, 1.0])
+    say("large blocks", la, nprop, beta, "seed", seed)
+    w = FTWalkers(Model(seed=seed, **kw))
+    o = oracle.OracleChain(seed=seed, **kw)
+    net = EffectiveAction(capacity=16, seed=1)
+    net.set_weights(wts)
+    net.set_norm(norm)
+    w.set_neighbours()
+    w.set_proposal(net, nprop=nprop, local_updates=False)
+    o.set_proposal(w.tables[0]["lat"]["nnlist"], nprop, wts, norm, local_updates=False)
+    w.init_conf()
+    o.init_conf()
+    w.sweep_start_0b()
+    o.sweep_start()
+    for s in range(2):
+        for p in ("b0", "0b"):
+            getattr(w, "sweep_" + p)(True, True)
+            getattr(o, "sweep_" + p)(True, True)
+            assert np.array_equal(w.get_conf(), o.conf), (p, s)
+            assert relmax(w.G(), o.G) < 1e-10, (p, s)
+    sw, so = w.proposal_stats(), o.proposal_stats()
+    say("  stats", sw)
+    assert sw == so
+    assert sw["accepted"] > 0 and sw["spins_accepted"] / sw["accepted"] >= min_block
+    w.close()
+    net.close()
+
+
 KW2 = dict(lattice="square", la=2, lb=2, beta=1.0, dtau=0.5, rhub=6.0, kmode="checkerboard", nwrap=1,
            update="delay", nublock=2)
 
