@@ -703,7 +703,11 @@ __device__ __forceinline__ cplx phase_after(cplx phase, cplx rt)
 }
 
 // delay update of one slice (src/dqmc_config_u/dqmc_update_delay.f90:39-128); one workgroup per
-// chain; uoff = index of this slice's first uniform in the pass buffer
+// chain, the delayed vectors avec / bvec in global memory (any N and depth; the register kernel
+// below covers N <= 256).  The slice's HS field and uniforms and the HS tables are staged in LDS
+// (the field goes back at the end), so that the site loop reads global memory only for G's row
+// and column and the delayed vectors of an accepted site, four steps of the products in flight.
+// LDS: diag(G) (n), the accepted site's rows of avec / bvec (2 x 64), uniforms (2n), field (n).
 extern "C" __global__ void __launch_bounds__(256)
 k_update_delay(const DChain *chains, int nt, int slice_in_pass)
 {
@@ -711,37 +715,66 @@ k_update_delay(const DChain *chains, int nt, int slice_in_pass)
     const DChain &c = chains[blockIdx.x];
     const int n = c.n, K = c.nublock;
     cplx *diagg = sh;                 // n
-    cplx *rowa = sh + n;              // K: avec(s, 0:ik)
-    cplx *rowb = rowa + K;            // K: bvec(s, 0:ik)
+    cplx *rowa = sh + n;              // 64: avec(s, 0:ik)
+    cplx *rowb = rowa + 64;           // 64: bvec(s, 0:ik)
+    double *us = (double *)(rowb + 64);   // 2n
+    int *cfs = (int *)(us + 2 * n);   // n
+    __shared__ HsTab tab;
     cplx *G = c.G, *avec = c.avec, *bvec = c.bvec;
+    GAS cplx *Gg = gp(c.G), *ag = gp(c.avec), *bg = gp(c.bvec);
     int32_t *cf = c.conf + (size_t)(nt - 1) * n;
     const double *u = c.unif + (size_t)2 * n * slice_in_pass;
     cplx phase = make_double2(c.sc[SC_PHASE], c.sc[SC_PHASE + 1]);
-    for (int r = threadIdx.x; r < n; r += blockDim.x) diagg[r] = G[(size_t)r + (size_t)r * n];
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        diagg[r] = G[(size_t)r + (size_t)r * n];
+        cfs[r] = cf[r];
+    }
+    for (int q = threadIdx.x; q < 2 * n; q += blockDim.x) us[q] = u[q];
+    hs_tab_load(tab, c);
     __syncthreads();
+    // every load issued so far has landed (the phase above would otherwise keep the compiler
+    // waiting on all outstanding memory operations inside the site loop)
+    __builtin_amdgcn_s_waitcnt(0);
     int ik = 0;
     double accm = 0.0;
     for (int s = 0; s < n; s++) {
-        const SiteDraw d = site_draw(c, cf[s], u[2 * s]);
+        const SiteDraw d = site_draw(tab, cfs[s], us[2 * s]);
         cplx dg = diagg[s];
         cplx ratio1 = cadd(cmul(make_double2(1.0 - dg.x, -dg.y), d.db), make_double2(1.0, 0.0));
         cplx rt;
-        bool acc = site_accept(c, ratio1, d, u[2 * s + 1], phase, &rt);
+        const bool acc = __builtin_amdgcn_readfirstlane((int)site_accept(tab, ratio1, d, us[2 * s + 1], phase, &rt)) != 0;
         if (acc) {
             accm += 1.0;
             phase = phase_after(phase, rt);
             cplx sscl = cdiv(d.db, ratio1);
             for (int m = threadIdx.x; m < ik; m += blockDim.x) {
-                rowa[m] = avec[(size_t)s + (size_t)m * n];
-                rowb[m] = bvec[(size_t)s + (size_t)m * n];
+                rowa[m] = gld(ag + ((size_t)s + (size_t)m * n));
+                rowb[m] = gld(bg + ((size_t)s + (size_t)m * n));
             }
             __syncthreads();
+            // every thread has read this site's field above: thread 0 records the flip
+            if (threadIdx.x == 0) cfs[s] = d.isp;
             for (int r = threadIdx.x; r < n; r += blockDim.x) {
-                cplx a = G[(size_t)r + (size_t)s * n];
-                cplx b = G[(size_t)s + (size_t)r * n];
-                for (int m = 0; m < ik; m++) {
-                    a = cadd(a, cmul(rowb[m], avec[(size_t)r + (size_t)m * n]));
-                    b = cadd(b, cmul(rowa[m], bvec[(size_t)r + (size_t)m * n]));
+                cplx a = gld(Gg + ((size_t)r + (size_t)s * n));
+                cplx b = gld(Gg + ((size_t)s + (size_t)r * n));
+                // the sums in the order m = 0, 1, ..., four steps' loads issued together
+                int m = 0;
+                for (; m + 4 <= ik; m += 4) {
+                    cplx av[4], bv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        av[j] = gld(ag + ((size_t)r + (size_t)(m + j) * n));
+                        bv[j] = gld(bg + ((size_t)r + (size_t)(m + j) * n));
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        a = cadd(a, cmul(rowb[m + j], av[j]));
+                        b = cadd(b, cmul(rowa[m + j], bv[j]));
+                    }
+                }
+                for (; m < ik; m++) {
+                    a = cadd(a, cmul(rowb[m], gld(ag + ((size_t)r + (size_t)m * n))));
+                    b = cadd(b, cmul(rowa[m], gld(bg + ((size_t)r + (size_t)m * n))));
                 }
                 a = cmul(a, sscl);
                 if (r == s) b.x -= 1.0;
@@ -750,18 +783,32 @@ k_update_delay(const DChain *chains, int nt, int slice_in_pass)
                 bvec[(size_t)r + (size_t)ik * n] = b;
             }
             ik++;
-            if (threadIdx.x == 0) cf[s] = d.isp;
             __syncthreads();
         }
         if (ik == K || s == n - 1) {
-            if (ik > 0) wg_rank_k(G, n, avec, bvec, ik, nullptr, nullptr);
+            // G += avec bvec^T on the prefetching MFMA strips at the depth of the accepted count
+            if (ik > 0) {
+                if (ik <= 16) {
+                    wg_rank_k_strips<16>(G, n, avec, bvec, ik);
+                } else if (ik <= 32) {
+                    wg_rank_k_strips<32>(G, n, avec, bvec, ik);
+                } else if (ik <= 48) {
+                    wg_rank_k_strips<48>(G, n, avec, bvec, ik);
+                } else {
+                    wg_rank_k_strips<32>(G, n, avec, bvec, 32);
+                    __syncthreads();
+                    wg_rank_k_strips<32>(G, n, avec + (size_t)32 * n, bvec + (size_t)32 * n, ik - 32);
+                }
+            }
             __syncthreads();
             ik = 0;
             if (s < n - 1)
-                for (int r = threadIdx.x; r < n; r += blockDim.x) diagg[r] = G[(size_t)r + (size_t)r * n];
+                for (int r = threadIdx.x; r < n; r += blockDim.x) diagg[r] = gld(Gg + ((size_t)r + (size_t)r * n));
             __syncthreads();
+            __builtin_amdgcn_s_waitcnt(0);
         }
     }
+    for (int r = threadIdx.x; r < n; r += blockDim.x) cf[r] = cfs[r];
     if (threadIdx.x == 0) {
         c.sc[SC_PHASE] = phase.x;
         c.sc[SC_PHASE + 1] = phase.y;

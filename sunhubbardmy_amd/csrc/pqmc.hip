@@ -192,7 +192,7 @@ void update_u_g(shp_ctx *ctx, int nt, size_t uoff0)
     hipLaunchKernelGGL(kp_u_save, dim3((n + 255) / 256, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dc, nt);
     const int slice = (int)(uoff0 / (2 * (size_t)n));
     if (ctx->update == SHP_U_DELAY) {
-        const size_t lds = (size_t)(n + 2 * 64) * sizeof(cplx);
+        const size_t lds = (size_t)(n + 2 * 64) * sizeof(cplx) + (size_t)n * 20;
         hipLaunchKernelGGL(k_update_delay, dim3(ctx->nch), dim3(256), lds, ctx->st, ctx->dview, nt, slice);
     } else {
         const int K = ctx->gblock, dss = sub_stage(n, K);

@@ -619,7 +619,8 @@ static void update_slice(sh_ctx *ctx, int nt, int slice_in_pass)
         default: hipLaunchKernelGGL(k_update_delay_reg<32>, dim3(ctx->nch), dim3(nthr), lds, ctx->st, ctx->dc, nt, slice_in_pass); break;
         }
     } else if (ctx->update == SH_U_DELAY) {
-        size_t lds = (size_t)(ctx->nmax + 2 * 64) * sizeof(cplx);
+        // diag(G), two rows of the delayed vectors, the slice's uniforms and HS field
+        size_t lds = (size_t)(ctx->nmax + 2 * 64) * sizeof(cplx) + (size_t)ctx->nmax * 20;
         hipLaunchKernelGGL(k_update_delay, dim3(ctx->nch), dim3(256), lds, ctx->st, ctx->dc, nt, slice_in_pass);
     } else {
         int K = 0;
