@@ -856,7 +856,7 @@ k_update_submatrix(const DChain *chains, int nt, int slice_in_pass, int dss)
     extern __shared__ cplx sh[];
     const DChain &c = chains[blockIdx.x];
     const int n = c.n, K = c.nublock, KL = K + 1;
-    dss = min(dss, 8);                // the staged draws hold 8 sites
+    dss = max(1, min(dss, 8));        // the staged draws hold 8 sites
     cplx *gi = sh;                    // Gamma^-1, column-major, leading dimension K + 1 (lanes reading
                                       // one entry of K columns each fall on different banks)
     cplx *csk = gi + KL * K;          // K: cs of the accepted sites
