@@ -36,11 +36,7 @@ __device__ static double blk_sum(double v, double *red)
 __device__ static void blk_argmax(double v, int idx, double *rv, int *ri, int *outi)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int o = 32; o > 0; o >>= 1) {
-        double v2 = __shfl_xor(v, o, 64);
-        int i2 = __shfl_xor(idx, o, 64);
-        if (v2 > v || (v2 == v && i2 < idx)) { v = v2; idx = i2; }
-    }
+    wave_argmax_dpp(v, idx);
     __syncthreads();
     if (lane == 0) { rv[w] = v; ri[w] = idx; }
     __syncthreads();
@@ -401,12 +397,23 @@ __device__ __forceinline__ void lu_blk_body(cplx *A, int n, int lda, int32_t *pi
     __shared__ int ri[16];
     __shared__ int s_info;
     __shared__ int s_pv[PB];
+    GAS cplx *Ag = gp(A);
     if (threadIdx.x == 0) s_info = 0;
     for (int k0 = 0; k0 < n; k0 += PB) {
         const int m = n - k0, b = min(PB, m);
-        for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
-            int r = e % m, p = e / m;
-            P[p * n + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * lda];
+        // the panel, eight elements per thread in flight (clamped addresses)
+        for (int base = threadIdx.x; base < m * b; base += 8 * (int)blockDim.x) {
+            cplx v8[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int e = min(base + u * (int)blockDim.x, m * b - 1), r = e % m, p = e / m;
+                v8[u] = gld(Ag + ((size_t)(k0 + r) + (size_t)(k0 + p) * lda));
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int e = base + u * (int)blockDim.x;
+                if (e < m * b) P[(e / m) * n + e % m] = v8[u];
+            }
         }
         __syncthreads();
         for (int j = 0; j < b; j++) {

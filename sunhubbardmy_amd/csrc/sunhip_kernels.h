@@ -181,6 +181,31 @@ __device__ __forceinline__ double dpp_d(double v)
     return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false),
                             __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false));
 }
+// the lane's partner value at the same DPP / lane-swap steps as wave_sum_dpp, for one dword
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int swap16_i(int v)
+{
+    const auto a = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+    return (int)((threadIdx.x & 16) ? a[0] : a[1]);
+}
+__device__ __forceinline__ int swap32_i(int v)
+{
+    const auto a = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+    return (int)((threadIdx.x & 32) ? a[0] : a[1]);
+}
+__device__ __forceinline__ double swap16_d(double v)
+{
+    return __hiloint2double(swap16_i(__double2hiint(v)), swap16_i(__double2loint(v)));
+}
+__device__ __forceinline__ double swap32_d(double v)
+{
+    return __hiloint2double(swap32_i(__double2hiint(v)), swap32_i(__double2loint(v)));
+}
+
 __device__ __forceinline__ double wave_sum_dpp(double v)
 {
     v += dpp_d<0xB1>(v);     // quad_perm [1, 0, 3, 2]
@@ -201,6 +226,22 @@ __device__ __forceinline__ double wave_sum_dpp(double v)
         v += __hiloint2double((int)(odd32 ? b[0] : b[1]), (int)(odd32 ? a[0] : a[1]));
     }
     return v;
+}
+
+// (largest value, smallest index among equal values) over the 64 lanes (all active), by the
+// steps of wave_sum_dpp: the choice is associative and commutative, so every lane ends with the
+// same pair whatever the order of the steps
+__device__ __forceinline__ void wave_argmax_dpp(double &v, int &idx)
+{
+    auto take = [&](double v2, int i2) {
+        if (v2 > v || (v2 == v && i2 < idx)) { v = v2; idx = i2; }
+    };
+    take(dpp_d<0xB1>(v), dpp_i<0xB1>(idx));
+    take(dpp_d<0x4E>(v), dpp_i<0x4E>(idx));
+    take(dpp_d<0x141>(v), dpp_i<0x141>(idx));
+    take(dpp_d<0x140>(v), dpp_i<0x140>(idx));
+    take(swap16_d(v), swap16_i(idx));
+    take(swap32_d(v), swap32_i(idx));
 }
 
 __device__ __forceinline__ cplx *chain_mat(const DChain &c, int sel, int pt)
