@@ -47,6 +47,26 @@ __device__ static void blk_argmax(double v, int idx, double *rv, int *ri, int *o
     *outi = bi;
 }
 
+// the m x b panel A(k0 + r, k0 + p) into P (leading dimension ldp), eight elements per thread in
+// flight (clamped addresses)
+__device__ __forceinline__ void panel_stage(cplx *P, int ldp, const cplx *A_, int lda, int k0, int m, int b)
+{
+    GAS const cplx *A = gp(A_);
+    for (int base = threadIdx.x; base < m * b; base += 8 * (int)blockDim.x) {
+        cplx v8[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = min(base + u * (int)blockDim.x, m * b - 1), r = e % m, p = e / m;
+            v8[u] = gld(A + ((size_t)(k0 + r) + (size_t)(k0 + p) * lda));
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = base + u * (int)blockDim.x;
+            if (e < m * b) P[(e / m) * ldp + e % m] = v8[u];
+        }
+    }
+}
+
 // V[r][p] of a Householder panel stored LAPACK-style (unit diagonal implicit, zeros above)
 __device__ __forceinline__ cplx vpanel(const cplx *P, int ldp, int r, int p)
 {
@@ -288,10 +308,7 @@ __device__ __forceinline__ void qr_blk_body(cplx *A, int M, int Nc, int lda, cpl
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     for (int k0 = 0; k0 < Nc; k0 += pb) {
         const int m = M - k0, b = min(pb, Nc - k0);
-        for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
-            int r = e % m, p = e / m;
-            P[p * M + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * lda];
-        }
+        panel_stage(P, M, A, lda, k0, m, b);
         __syncthreads();
         for (int j = 0; j < b; j++) {
             double s = 0.0;
@@ -369,10 +386,7 @@ __device__ __forceinline__ void formq_blk_body(const cplx *A, int M, int Nc, int
     const int np = (Nc + pb - 1) / pb;
     for (int pi = np - 1; pi >= 0; pi--) {
         const int k0 = pi * pb, m = M - k0, b = min(pb, Nc - k0), nc = Nc - k0;
-        for (int e = threadIdx.x; e < m * b; e += blockDim.x) {
-            int r = e % m, p = e / m;
-            P[p * M + r] = A[(size_t)(k0 + r) + (size_t)(k0 + p) * lda];
-        }
+        panel_stage(P, M, A, lda, k0, m, b);
         for (int e = threadIdx.x; e < PB * PB; e += blockDim.x) T[e] = tblk[(size_t)pi * PB * PB + e];
         __syncthreads();
         cplx *Ct = Q + (size_t)k0 + (size_t)k0 * ldq;
@@ -401,20 +415,7 @@ __device__ __forceinline__ void lu_blk_body(cplx *A, int n, int lda, int32_t *pi
     if (threadIdx.x == 0) s_info = 0;
     for (int k0 = 0; k0 < n; k0 += PB) {
         const int m = n - k0, b = min(PB, m);
-        // the panel, eight elements per thread in flight (clamped addresses)
-        for (int base = threadIdx.x; base < m * b; base += 8 * (int)blockDim.x) {
-            cplx v8[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int e = min(base + u * (int)blockDim.x, m * b - 1), r = e % m, p = e / m;
-                v8[u] = gld(Ag + ((size_t)(k0 + r) + (size_t)(k0 + p) * lda));
-            }
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int e = base + u * (int)blockDim.x;
-                if (e < m * b) P[(e / m) * n + e % m] = v8[u];
-            }
-        }
+        panel_stage(P, n, A, lda, k0, m, b);
         __syncthreads();
         for (int j = 0; j < b; j++) {
             double bv = -1.0;
