@@ -73,7 +73,10 @@ static int prop_alloc(sh_ctx *ctx)
         p.pa = dalloc<cplx>(ctx, (size_t)c.n * SHL_MB);
         p.pb = dalloc<cplx>(ctx, (size_t)c.n * SHL_MB);
         p.stats = dalloc<double>(ctx, 8);
-        if (!p.psum || !p.rng || !p.pa || !p.pb || !p.stats) return fail(ctx, SH_E_HIP, "device allocation failed");
+        p.mapply = dalloc<int32_t>(ctx, 1);
+        if (!p.psum || !p.rng || !p.pa || !p.pb || !p.stats || !p.mapply)
+            return fail(ctx, SH_E_HIP, "device allocation failed");
+        HIPCHK(hipMemset(p.mapply, 0, sizeof(int32_t)));
         uint64_t s = smix_seed(ctx->seeds[ci]);
         HIPCHK(hipMemcpy(p.rng, &s, sizeof(s), hipMemcpyHostToDevice));
     }
@@ -98,6 +101,8 @@ static void block_move(sh_ctx *ctx, int nt)
     Tic t(ctx, T_PROPOSAL);
     hipLaunchKernelGGL(k_block_move, dim3(ctx->nch), dim3(256), lds_block_move(ctx->nmax, ctx->prop_z), ctx->st, ctx->dc,
                        ctx->d_prop, ctx->pnet, nt);
+    const int ntl = (ctx->nmax + GT - 1) / GT;
+    hipLaunchKernelGGL(k_block_apply, dim3(ntl * ntl, 1, ctx->nch), dim3(256), 0, ctx->st, ctx->dc, ctx->d_prop);
 }
 
 extern "C" {

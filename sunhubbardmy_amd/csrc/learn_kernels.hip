@@ -28,7 +28,8 @@ struct PropChain {
     const int32_t *nbr;   // n * z neighbour sites
     int32_t *psum;        // ltrot * 4 per-slice partial sums of the features
     uint64_t *rng;        // splitmix64 state
-    cplx *pa, *pb;        // n * 64: the two factors of the rank-|F| update of G
+    cplx *pa, *pb;        // n * 64: the two factors of the rank-|F| update of G (pb conjugated)
+    int32_t *mapply;      // |F| of the block move just accepted, 0 if none (k_block_apply)
     double *stats;        // 5 counters (sh_get_proposal_stats)
     int z, nb2;           // list length, sum of list lengths (twice the bond count)
 };
@@ -453,7 +454,8 @@ __global__ __launch_bounds__(256) void k_block_move(const DChain *chains, const 
             for (int q = 0; q < 16; q++)
                 if (g0i + q < m) p.pa[(size_t)row + (size_t)(g0i + q) * n] = make_double2(-ar[q], -ai[q]);
         }
-        // pb(col, a) = Delta_a (delta(F_a, col) - G(F_a, col)), eight rows' entries in flight per thread
+        // pb(col, a) = conj(Delta_a (delta(F_a, col) - G(F_a, col))), eight rows' entries in
+        // flight per thread; k_block_apply then adds pa pb^T to G over tiles of the whole chip
         for (int base = t; base < n * m; base += 8 * 256) {
             cplx gv[8];
 #pragma unroll
@@ -466,20 +468,9 @@ __global__ __launch_bounds__(256) void k_block_move(const DChain *chains, const 
                 const int e = base + u * 256, a = e / n, col = e - a * n;
                 if (e < n * m)
                     p.pb[(size_t)col + (size_t)a * n] =
-                        cmul(dl[a], make_double2((F[a] == col ? 1.0 : 0.0) - gv[u].x, -gv[u].y));
+                        cconj(cmul(dl[a], make_double2((F[a] == col ? 1.0 : 0.0) - gv[u].x, -gv[u].y)));
             }
         }
-        __syncthreads();
-        // the fragment depth the next multiple of 16 above m: the strips run every k-column of
-        // their template depth, so a depth-64 update of a 13-site block would do 5x its MFMAs
-        if (m <= 16)
-            wg_rank_k_strips<16>(G, n, p.pa, p.pb, m);
-        else if (m <= 32)
-            wg_rank_k_strips<32>(G, n, p.pa, p.pb, m);
-        else if (m <= 48)
-            wg_rank_k_strips<48>(G, n, p.pa, p.pb, m);
-        else
-            wg_rank_k_strips<SHL_MB>(G, n, p.pa, p.pb, m);
         for (int a = t; a < m; a += 256) cf[F[a]] = 3 - isF[a];
     } else {
         for (int a = t; a < m; a += 256) R[n + F[a]] = 2 * isF[a] - 3;
@@ -505,7 +496,22 @@ __global__ __launch_bounds__(256) void k_block_move(const DChain *chains, const 
         p.stats[2] += m;
         p.stats[3] += acc ? m : 0;
         p.stats[4] += ninner;
+        p.mapply[0] = acc ? m : 0;
     }
+}
+
+// The rank-|F| update G <- G + pa pb^T of the chains whose block move of this slice was
+// accepted, as 64 x 64 MFMA tiles over the whole chip (grid: the largest chain's tiles x
+// chains); every other chain's workgroups return at once.  pb is stored conjugated, so the
+// conjugate transpose of zgemm_tile gives pb^T.
+__global__ __launch_bounds__(256) void k_block_apply(const DChain *chains, const PropChain *props)
+{
+    const DChain &c = chains[blockIdx.z];
+    const PropChain &p = props[blockIdx.z];
+    const int m = p.mapply[0], n = c.n, ntl = (n + GT - 1) / GT;
+    if (m <= 0 || (int)blockIdx.x >= ntl * ntl) return;
+    const int tm = ((int)blockIdx.x % ntl) * GT, tn = ((int)blockIdx.x / ntl) * GT;
+    zgemm_tile<0, 1>(p.pa, n, p.pb, n, c.G, n, n, n, m, tm, tn, 1.0, 0.0, 1.0, 0.0);
 }
 
 // ------------------------------------------------------------------------------------------
