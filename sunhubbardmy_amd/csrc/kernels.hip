@@ -1223,11 +1223,7 @@ __device__ static void block_argmax(double v, int idx, double *rv, int *ri, doub
 {
     // first index of the largest value
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int o = 32; o > 0; o >>= 1) {
-        double v2 = __shfl_xor(v, o, 64);
-        int i2 = __shfl_xor(idx, o, 64);
-        if (v2 > v || (v2 == v && i2 < idx)) { v = v2; idx = i2; }
-    }
+    wave_argmax_dpp(v, idx);
     __syncthreads();
     if (lane == 0) { rv[w] = v; ri[w] = idx; }
     __syncthreads();
@@ -1264,8 +1260,8 @@ __device__ __forceinline__ void qr_apply_col(GAS cplx *a, const cplx (&x)[NQ], i
             si += vv.x * aa.y - vv.y * aa.x;
         }
     }
-    sr = wave_sum(sr);
-    si = wave_sum(si);
+    sr = wave_sum_dpp(sr);
+    si = wave_sum_dpp(si);
     const cplx f = cmul(tc, make_double2(sr, si));
     double nrm2 = 0.0;
     cplx ai_new = make_double2(0.0, 0.0);
@@ -1280,7 +1276,7 @@ __device__ __forceinline__ void qr_apply_col(GAS cplx *a, const cplx (&x)[NQ], i
         }
     }
     if (pivot) {
-        nrm2 = wave_sum(nrm2);
+        nrm2 = wave_sum_dpp(nrm2);
         if (lane == 0 && vn1[j] != 0.0) {
             double t = sqrt(cabs2(ai_new)) / vn1[j];
             t = 1.0 - t * t;
@@ -1326,7 +1322,7 @@ k_qr(const DChain *chains, int sel, int pivot, int out_pt)
 #pragma unroll
             for (int q = 0; q < NQ; q++)
                 if (lane + 64 * q < n) s += cabs2(x[q]);
-            s = wave_sum(s);
+            s = wave_sum_dpp(s);
             if (lane == 0) { vn1[j] = sqrt(s); vn2[j] = vn1[j]; }
         }
     }
@@ -1368,7 +1364,7 @@ k_qr(const DChain *chains, int sel, int pivot, int out_pt)
 #pragma unroll
             for (int q = 0; q < NQ; q++)
                 if (i + 1 + lane + 64 * q < n) s += cabs2(x[q]);
-            s = wave_sum(s);
+            s = wave_sum_dpp(s);
             double xnorm = sqrt(s);
             cplx tau = make_double2(0.0, 0.0);
             cplx scal = make_double2(0.0, 0.0);
@@ -1471,8 +1467,8 @@ k_formq(const DChain *chains, int sel, int out_pt)
                 sr += vv.x * qq.x + vv.y * qq.y;
                 si += vv.x * qq.y - vv.y * qq.x;
             }
-            sr = wave_sum(sr);
-            si = wave_sum(si);
+            sr = wave_sum_dpp(sr);
+            si = wave_sum_dpp(si);
             cplx f = cmul(tau, make_double2(sr, si));
             for (int r = i + lane; r < n; r += 64) q[r] = csub(q[r], cmul(v[r], f));
         }
@@ -1524,7 +1520,7 @@ k_prepivot(const DChain *chains, int dpt)
             B[(size_t)r + (size_t)j * n] = b;
             s += cabs2(b);
         }
-        s = wave_sum(s);
+        s = wave_sum_dpp(s);
         if (lane == 0) { vn[j] = sqrt(s); jp[j] = j + 1; }
     }
     __syncthreads();
