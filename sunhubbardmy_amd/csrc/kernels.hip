@@ -1501,6 +1501,8 @@ k_v_from_qrcp(const DChain *chains, int sel, int pt)
 // prepivot: Btmp = Bd diag(Dprev); columns sorted by decreasing norm (s_zmcpt); the sorted
 // matrix goes to W2, the order to jpvt.  One workgroup per chain; the selection sort (first
 // maximum wins, the displaced entry takes the maximum's slot) runs in wave 0 without barriers.
+// A wave scales two columns at a time and the copy moves eight entries per thread at a time,
+// every batch's loads issued before its first store (clamped addresses).
 extern "C" __global__ void __launch_bounds__(1024)
 k_prepivot(const DChain *chains, int dpt)
 {
@@ -1511,17 +1513,43 @@ k_prepivot(const DChain *chains, int dpt)
     volatile int *jp = (volatile int *)(psh + n);
     const double *Dp = c.D + (size_t)n * dpt;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    cplx *B = c.W1;
-    for (int j = w; j < n; j += nw) {
-        double s = 0.0;
-        double dj = Dp[j];
-        for (int r = lane; r < n; r += 64) {
-            cplx b = cscale(B[(size_t)r + (size_t)j * n], dj);
-            B[(size_t)r + (size_t)j * n] = b;
-            s += cabs2(b);
+    GAS cplx *B = gp(c.W1);
+    for (int j = w; j < n; j += 2 * nw) {
+        const int j2 = j + nw;
+        const bool two = j2 < n;
+        const double d1 = Dp[j], d2 = Dp[two ? j2 : j];
+        GAS cplx *B1 = B + (size_t)j * n, *B2 = B + (size_t)(two ? j2 : j) * n;
+        double s1 = 0.0, s2 = 0.0;
+        for (int r0 = lane; r0 < n; r0 += 4 * 64) {
+            cplx a1[4], a2[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int r = min(r0 + 64 * u, n - 1);
+                a1[u] = gld(B1 + r);
+                a2[u] = gld(B2 + r);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int r = r0 + 64 * u;
+                if (r < n) {
+                    const cplx b1 = cscale(a1[u], d1);
+                    gst(B1 + r, b1);
+                    s1 += cabs2(b1);
+                    if (two) {
+                        const cplx b2 = cscale(a2[u], d2);
+                        gst(B2 + r, b2);
+                        s2 += cabs2(b2);
+                    }
+                }
+            }
         }
-        s = wave_sum_dpp(s);
-        if (lane == 0) { vn[j] = sqrt(s); jp[j] = j + 1; }
+        s1 = wave_sum_dpp(s1);
+        s2 = wave_sum_dpp(s2);
+        if (lane == 0) {
+            vn[j] = sqrt(s1);
+            jp[j] = j + 1;
+            if (two) { vn[j2] = sqrt(s2); jp[j2] = j2 + 1; }
+        }
     }
     __syncthreads();
     if (w == 0) {
@@ -1532,11 +1560,7 @@ k_prepivot(const DChain *chains, int dpt)
                 double a = fabs(vn[j]);
                 if (a > bv) { bv = a; bi = j; }
             }
-            for (int o = 32; o > 0; o >>= 1) {
-                double v2 = __shfl_xor(bv, o, 64);
-                int i2 = __shfl_xor(bi, o, 64);
-                if (v2 > bv || (v2 == bv && i2 < bi)) { bv = v2; bi = i2; }
-            }
+            wave_argmax_dpp(bv, bi);
             if (lane == 0 && bi != k) {
                 int it = jp[bi]; jp[bi] = jp[k]; jp[k] = it;
                 vn[bi] = vn[k];
@@ -1546,9 +1570,20 @@ k_prepivot(const DChain *chains, int dpt)
     }
     __syncthreads();
     for (int j = threadIdx.x; j < n; j += blockDim.x) c.jpvt[j] = jp[j];
-    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
-        int r = (int)(e % n), k = (int)(e / n);
-        c.W2[e] = B[(size_t)r + (size_t)(jp[k] - 1) * n];
+    GAS cplx *W2 = gp(c.W2);
+    const int nn = n * n;
+    for (int base = threadIdx.x; base < nn; base += 8 * (int)blockDim.x) {
+        cplx v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = min(base + u * (int)blockDim.x, nn - 1), r = e % n, k = e / n;
+            v[u] = gld(B + ((size_t)r + (size_t)(jp[k] - 1) * n));
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = base + u * (int)blockDim.x;
+            if (e < nn) gst(W2 + e, v[u]);
+        }
     }
 }
 
@@ -1872,13 +1907,26 @@ k_wrap(const DChain *chains, int compare)
     double *sc = c.sc;
     double mx = 0.0;
     bool nonfinite = false;
-    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
-        cplx a = c.Gs[e];
-        if (!isfinite(a.x) || !isfinite(a.y)) nonfinite = true;
-        if (compare) {
-            cplx b = c.G[e];
-            double d = hypot(a.x - b.x, a.y - b.y);
-            if (d > mx) mx = d;
+    // eight entries of Gs (and G) per thread in flight at a time, clamped addresses; the maximum
+    // and the flag do not depend on the order
+    GAS cplx *Gsg = gp(c.Gs), *Gg = gp(c.G);
+    const int nn = n * n;
+    for (int base = threadIdx.x; base < nn; base += 8 * (int)blockDim.x) {
+        cplx a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = min(base + u * (int)blockDim.x, nn - 1);
+            a[u] = gld(Gsg + e);
+            if (compare) b[u] = gld(Gg + e);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (base + u * (int)blockDim.x >= nn) continue;
+            if (!isfinite(a[u].x) || !isfinite(a[u].y)) nonfinite = true;
+            if (compare) {
+                const double d = hypot(a[u].x - b[u].x, a[u].y - b[u].y);
+                if (d > mx) mx = d;
+            }
         }
     }
     // max reduction
@@ -1893,10 +1941,17 @@ k_wrap(const DChain *chains, int compare)
     // every plaquette and slice, set_phase.f90:9-15)
     double cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (c.phase_from_conf && c.lwrapu) {
-        size_t tot = (size_t)n * c.ltrot;
-        for (size_t e = threadIdx.x; e < tot; e += blockDim.x) {
-            int v = c.conf[e];
-            cnt[0] += (v == 1); cnt[1] += (v == 2); cnt[2] += (v == 3); cnt[3] += (v == 4);
+        const int tot = n * c.ltrot;
+        GAS const int32_t *cg = gp(c.conf);
+        for (int base = threadIdx.x; base < tot; base += 8 * (int)blockDim.x) {
+            int v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = cg[min(base + u * (int)blockDim.x, tot - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (base + u * (int)blockDim.x < tot) {
+                    cnt[0] += (v[u] == 1); cnt[1] += (v[u] == 2); cnt[2] += (v[u] == 3); cnt[3] += (v[u] == 4);
+                }
         }
     }
     if (c.lwrapp) {
@@ -1946,7 +2001,14 @@ k_wrap(const DChain *chains, int compare)
     }
     __syncthreads();
     if ((int)red[0] == 0) {
-        for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) c.G[e] = c.Gs[e];
+        for (int base = threadIdx.x; base < nn; base += 8 * (int)blockDim.x) {
+            cplx a[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) a[u] = gld(Gsg + min(base + u * (int)blockDim.x, nn - 1));
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (base + u * (int)blockDim.x < nn) gst(Gg + base + u * (int)blockDim.x, a[u]);
+        }
     }
     __syncthreads();
     int tn = (int)sc[SC_TRACE_N];
