@@ -1136,23 +1136,39 @@ k_measure_tiles(const DChain *chains)
     int bi = 0;
     while (tp >= nb - bi) { tp -= nb - bi; bi++; }
     const int bj = bi + tp;
-    const cplx *G = c.G;
-    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
-        int q = e % 32, p = e / 32;
-        int r1 = bi * 32 + q, c1 = bj * 32 + p;
-        ta[q][p] = (r1 < n && c1 < n) ? G[(size_t)r1 + (size_t)c1 * n] : make_double2(0.0, 0.0);
-        int r2 = bj * 32 + q, c2 = bi * 32 + p;
-        tb[q][p] = (r2 < n && c2 < n) ? G[(size_t)r2 + (size_t)c2 * n] : make_double2(0.0, 0.0);
+    // the eight G entries of a thread (four per tile, 256 threads) and its stag values are loaded
+    // in one batch through global-space pointers (clamped addresses), ahead of the LDS stores
+    GAS cplx *G = gp(c.G);
+    GAS const double *stg = gp(c.stag);
+    constexpr int MT_U = 1024 / 256;
+    cplx va[MT_U], vb[MT_U];
+    double si_[MT_U], sj_[MT_U];
+#pragma unroll
+    for (int u = 0; u < MT_U; u++) {
+        const int e = threadIdx.x + 256 * u, q = e % 32, p = e / 32;
+        const int r1 = bi * 32 + q, c1 = bj * 32 + p, r2 = bj * 32 + q, c2 = bi * 32 + p;
+        va[u] = gld(G + ((size_t)min(r1, n - 1) + (size_t)min(c1, n - 1) * n));
+        vb[u] = gld(G + ((size_t)min(r2, n - 1) + (size_t)min(c2, n - 1) * n));
+        si_[u] = stg[min(r1, n - 1)];
+        sj_[u] = stg[min(c1, n - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < MT_U; u++) {
+        const int e = threadIdx.x + 256 * u, q = e % 32, p = e / 32;
+        const int r1 = bi * 32 + q, c1 = bj * 32 + p, r2 = bj * 32 + q, c2 = bi * 32 + p;
+        ta[q][p] = (r1 < n && c1 < n) ? va[u] : make_double2(0.0, 0.0);
+        tb[q][p] = (r2 < n && c2 < n) ? vb[u] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     const double wgt = (bi == bj) ? 1.0 : 2.0;
     double ssr = 0.0, ssi = 0.0;
-    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
-        int q = e % 32, p = e / 32;
-        int i = bi * 32 + q, j = bj * 32 + p;
+#pragma unroll
+    for (int u = 0; u < MT_U; u++) {
+        const int e = threadIdx.x + 256 * u, q = e % 32, p = e / 32;
+        const int i = bi * 32 + q, j = bj * 32 + p;
         if (i < n && j < n) {
-            cplx pr = cmul(ta[q][p], tb[p][q]);   // G(i,j) G(j,i)
-            double sg = c.stag[i] * c.stag[j] * wgt;
+            const cplx pr = cmul(ta[q][p], tb[p][q]);   // G(i,j) G(j,i)
+            const double sg = si_[u] * sj_[u] * wgt;
             ssr += sg * pr.x;
             ssi += sg * pr.y;
         }
