@@ -1642,35 +1642,65 @@ k_green_m(const DChain *chains, int bb, int rsaved, int rpt, int lsaved, int lpt
     const int n = c.n;
     const int bx = blockIdx.x % ((n + 31) / 32), by = blockIdx.x / ((n + 31) / 32);
     if (by * 32 >= n) return;
-    const double *dr = dvec_of(c, rsaved, rpt);
+    // every thread's four entries (256 threads, 32 x 32 tile) are loaded in one batch through
+    // global-space pointers with clamped addresses, ahead of the stores
+    GAS const double *dr = gp(dvec_of(c, rsaved, rpt));
+    constexpr int GU = 1024 / 256;
     if (!bb) {
-        const double *dl = dvec_of(c, lsaved, lpt);
-        for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
-            int i = bx * 32 + (e % 32), j = by * 32 + (e / 32);
+        GAS const double *dl = gp(dvec_of(c, lsaved, lpt));
+        GAS cplx *W1 = gp(c.W1), *W2 = gp(c.W2), *W3 = gp(c.W3);
+        cplx x1[GU], x2[GU];
+        double di[GU], dj[GU];
+#pragma unroll
+        for (int u = 0; u < GU; u++) {
+            const int e = threadIdx.x + 256 * u;
+            const int i = min(bx * 32 + (e % 32), n - 1), j = min(by * 32 + (e / 32), n - 1);
+            const size_t o = (size_t)i + (size_t)j * n;
+            x1[u] = gld(W1 + o);
+            x2[u] = gld(W2 + o);
+            di[u] = dr[i];
+            dj[u] = dl[j];
+        }
+#pragma unroll
+        for (int u = 0; u < GU; u++) {
+            const int e = threadIdx.x + 256 * u;
+            const int i = bx * 32 + (e % 32), j = by * 32 + (e / 32);
             if (i < n && j < n) {
-                size_t o = (size_t)i + (size_t)j * n;
-                double a = 1.0 / (dmax_of(dr[i]) * dmax_of(dl[j]));
-                double b = dmin_of(dr[i]) * dmin_of(dl[j]);
-                c.W3[o] = cadd(cscale(c.W1[o], a), cscale(c.W2[o], b));
+                const double a = 1.0 / (dmax_of(di[u]) * dmax_of(dj[u]));
+                const double b = dmin_of(di[u]) * dmin_of(dj[u]);
+                gst(W3 + ((size_t)i + (size_t)j * n), cadd(cscale(x1[u], a), cscale(x2[u], b)));
             }
         }
     } else {
-        const cplx *ur = rsaved ? c.Us : c.U + (size_t)n * n * rpt;
-        const cplx *vr = rsaved ? c.Vs : c.V + (size_t)n * n * rpt;
+        GAS cplx *ur = gp(rsaved ? c.Us : c.U + (size_t)n * n * rpt);
+        GAS cplx *vr = gp(rsaved ? c.Vs : c.V + (size_t)n * n * rpt);
+        GAS cplx *W3 = gp(c.W3);
         // tile of UR^H: element (i, j) = conj(UR(j, i)); read UR block (by, bx) coalesced
-        for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
-            int q = e % 32, p = e / 32;
-            int r = by * 32 + q, cc = bx * 32 + p;
-            tile[q][p] = (r < n && cc < n) ? ur[(size_t)r + (size_t)cc * n] : make_double2(0.0, 0.0);
+        cplx xu[GU], xv[GU];
+        double di[GU];
+#pragma unroll
+        for (int u = 0; u < GU; u++) {
+            const int e = threadIdx.x + 256 * u, q = e % 32, p = e / 32;
+            const int r = min(by * 32 + q, n - 1), cc = min(bx * 32 + p, n - 1);
+            xu[u] = gld(ur + ((size_t)r + (size_t)cc * n));
+            const int i = min(bx * 32 + q, n - 1), j = min(by * 32 + p, n - 1);
+            xv[u] = gld(vr + ((size_t)i + (size_t)j * n));
+            di[u] = dr[i];
+        }
+#pragma unroll
+        for (int u = 0; u < GU; u++) {
+            const int e = threadIdx.x + 256 * u, q = e % 32, p = e / 32;
+            const int r = by * 32 + q, cc = bx * 32 + p;
+            tile[q][p] = (r < n && cc < n) ? xu[u] : make_double2(0.0, 0.0);
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
-            int q = e % 32, p = e / 32;
-            int i = bx * 32 + q, j = by * 32 + p;
+#pragma unroll
+        for (int u = 0; u < GU; u++) {
+            const int e = threadIdx.x + 256 * u, q = e % 32, p = e / 32;
+            const int i = bx * 32 + q, j = by * 32 + p;
             if (i < n && j < n) {
-                size_t o = (size_t)i + (size_t)j * n;
-                cplx uh = cconj(tile[p][q]);
-                c.W3[o] = cadd(cscale(uh, 1.0 / dmax_of(dr[i])), cscale(vr[o], dmin_of(dr[i])));
+                const cplx uh = cconj(tile[p][q]);
+                gst(W3 + ((size_t)i + (size_t)j * n), cadd(cscale(uh, 1.0 / dmax_of(di[u])), cscale(xv[u], dmin_of(di[u]))));
             }
         }
     }
@@ -1686,18 +1716,32 @@ k_green_rhs(const DChain *chains, int rsaved, int rpt)
     const int nb = (n + 31) / 32;
     const int bx = blockIdx.x % nb, by = blockIdx.x / nb;
     if (by >= nb) return;
-    const double *dr = dvec_of(c, rsaved, rpt);
-    const cplx *ur = rsaved ? c.Us : c.U + (size_t)n * n * rpt;
-    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
-        int q = e % 32, p = e / 32;
-        int r = by * 32 + q, cc = bx * 32 + p;
-        tile[q][p] = (r < n && cc < n) ? ur[(size_t)r + (size_t)cc * n] : make_double2(0.0, 0.0);
+    // the thread's four entries and scale factors in one batch (global-space pointers, clamped
+    // addresses), ahead of the LDS stores
+    GAS const double *dr = gp(dvec_of(c, rsaved, rpt));
+    GAS cplx *ur = gp(rsaved ? c.Us : c.U + (size_t)n * n * rpt);
+    GAS cplx *W4 = gp(c.W4);
+    constexpr int GU = 1024 / 256;
+    cplx xu[GU];
+    double di[GU];
+#pragma unroll
+    for (int u = 0; u < GU; u++) {
+        const int e = threadIdx.x + 256 * u, q = e % 32, p = e / 32;
+        xu[u] = gld(ur + ((size_t)min(by * 32 + q, n - 1) + (size_t)min(bx * 32 + p, n - 1) * n));
+        di[u] = dr[min(bx * 32 + q, n - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < GU; u++) {
+        const int e = threadIdx.x + 256 * u, q = e % 32, p = e / 32;
+        const int r = by * 32 + q, cc = bx * 32 + p;
+        tile[q][p] = (r < n && cc < n) ? xu[u] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
-        int q = e % 32, p = e / 32;
-        int i = bx * 32 + q, j = by * 32 + p;
-        if (i < n && j < n) c.W4[(size_t)i + (size_t)j * n] = cscale(cconj(tile[p][q]), 1.0 / dmax_of(dr[i]));
+#pragma unroll
+    for (int u = 0; u < GU; u++) {
+        const int e = threadIdx.x + 256 * u, q = e % 32, p = e / 32;
+        const int i = bx * 32 + q, j = by * 32 + p;
+        if (i < n && j < n) gst(W4 + ((size_t)i + (size_t)j * n), cscale(cconj(tile[p][q]), 1.0 / dmax_of(di[u])));
     }
 }
 
