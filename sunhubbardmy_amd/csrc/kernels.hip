@@ -1612,12 +1612,16 @@ k_after_qr(const DChain *chains, int pt, int vsel, int vpt)
     size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)n * n) return;
     int i = (int)(e % n), j = (int)(e / n);
-    const cplx *R = c.W2;
-    double di = R[(size_t)i + (size_t)i * n].x;
-    c.W3[e] = (i <= j) ? cscale(R[e], 1.0 / di) : make_double2(0.0, 0.0);
-    const cplx *Vp = chain_mat(c, vsel, vpt);
-    c.W4[e] = Vp[(size_t)(c.jpvt[i] - 1) + (size_t)j * n];
-    if (j == 0) c.D[(size_t)n * pt + i] = di;
+    // every load (the pivot index first, then the row of V it names) is issued before the
+    // first store, through global-space pointers
+    GAS cplx *R = gp(c.W2), *Vp = gp(chain_mat(c, vsel, vpt));
+    const int jp = gp(c.jpvt)[i];
+    const double di = gld(R + ((size_t)i + (size_t)i * n)).x;
+    const cplx r = gld(R + e);
+    const cplx vrow = gld(Vp + ((size_t)(jp - 1) + (size_t)j * n));
+    gst(gp(c.W3) + e, (i <= j) ? cscale(r, 1.0 / di) : make_double2(0.0, 0.0));
+    gst(gp(c.W4) + e, vrow);
+    if (j == 0) gp(c.D)[(size_t)n * pt + i] = di;
 }
 
 // ------------------------------------------------------------------------------------------
