@@ -453,16 +453,49 @@ __device__ __forceinline__ void lu_blk_body(cplx *A, int n, int lda, int32_t *pi
             int r = e % m, p = e / m;
             A[(size_t)(k0 + r) + (size_t)(k0 + p) * lda] = P[p * n + r];
         }
-        // the panel's interchanges on the columns left and right of it, in order
+        // the panel's interchanges on the columns left and right of it, in order.  When every
+        // swap pairs a panel row with a row below the panel that no other swap names (the usual
+        // case) the swaps touch distinct rows, so they run eight at a time with their loads
+        // issued before their stores; otherwise one after another.
+        bool dep = false;
+        if ((int)threadIdx.x < b) {
+            const int t = threadIdx.x, pt = s_pv[t];
+            if (pt != t) {
+                dep = pt < b;
+                for (int jj = 0; jj < t; jj++) dep = dep || s_pv[jj] == pt;
+            }
+        }
+        const bool indep = !__syncthreads_or(dep ? 1 : 0);
         for (int q = threadIdx.x; q < n; q += blockDim.x) {
             if (q >= k0 && q < k0 + b) continue;
             cplx *col = A + (size_t)q * lda;
-            for (int j = 0; j < b; j++) {
-                int p = s_pv[j];
-                if (p != j) {
-                    cplx t = col[k0 + p];
-                    col[k0 + p] = col[k0 + j];
-                    col[k0 + j] = t;
+            if (indep) {
+                GAS cplx *cg = gp(col) + k0;
+                for (int j0 = 0; j0 < b; j0 += 8) {
+                    cplx xa[8], xb[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int j = min(j0 + u, b - 1);
+                        xa[u] = gld(cg + j);
+                        xb[u] = gld(cg + s_pv[j]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int j = j0 + u;
+                        if (j < b && s_pv[j] != j) {
+                            gst(cg + j, xb[u]);
+                            gst(cg + s_pv[j], xa[u]);
+                        }
+                    }
+                }
+            } else {
+                for (int j = 0; j < b; j++) {
+                    int p = s_pv[j];
+                    if (p != j) {
+                        cplx t = col[k0 + p];
+                        col[k0 + p] = col[k0 + j];
+                        col[k0 + j] = t;
+                    }
                 }
             }
         }
