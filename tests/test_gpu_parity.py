@@ -1,6 +1,6 @@
 """GPU parity: the HIP sweep (through the C ABI) against the CPU oracle on identical seeds.
 
-Bar (BASELINE north_star): identical accept/reject sequences (the HS field after every pass is
+Bar: identical accept/reject sequences (the HS field after every pass is
 bit-identical), G within 1e-10 relative max-norm after every stabilization (nwrap), phase within
 1e-10, observable bin sums within 1e-9 relative.
 """

@@ -289,8 +289,9 @@ def test_update_algorithms_agree():
 
 
 def test_reference_probe_values_4x4():
-    # survey-time probe of the reference build at the C1 shape: acceptance
-    # 0.736, E_kin/N per flavour -0.656, <n_up n_dn> 0.1284 over 1200 sweeps; loose statistical match
+    # values measured with a build of the reference at this shape (4x4 square, U = 4, beta = 4, fast
+    # update): acceptance 0.736, E_kin/N per flavour -0.656, <n_up n_dn> 0.1284 over 1200 sweeps;
+    # loose statistical match
     o = oracle.OracleChain(la=4, lb=4, beta=4.0, rhub=4.0, update="fast", kmode="checkerboard", nwrap=10,
                            seed=oracle.chain_seed(0))
     o.init_conf()
